@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Training-step benchmark on one MI355X.
+
+A step is one iteration of the reference's training loop body
+(pipeline/mod.rs:232-237) over a batch of views: forward and backward of every
+view, gradients summed, one Adam step. The default workload is 1M Gaussians
+(SH degree 3) with 4 views of 1920x1080 per step, run as the device-resident,
+batched, hipGraph-replayed step. Inputs are synthetic and seeded (the scene
+distributions of the workload table in rgsplat_amd/synthetic.py); targets are
+renders of a second seeded scene.
+
+Prints one JSON line. Besides the throughput it reports the roofline of the
+dominant kernel (HIP-event timings of each stage on the context's stream,
+algorithmic work counted on the device), the CPU oracle timed on the host on a
+bounded sample of the same workload, the plain (one view per call, host
+buffers) GPU path, and the gradient relative error against the CPU oracle at
+the small parity size.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "train iters/sec (fwd+bwd+Adam) at 1M Gaussians 1080p; grad rel-err vs CPU"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+# Algorithmic work per unit (per stage):
+#   raster_fwd: 25 FP32 ops + 1 exp per evaluated (pixel, splat) pair
+#   raster_bwd: 60 FP32 ops + 1 exp + 1 div per evaluated pair
+#   loss_ssim:  1440 FLOP per pixel (30 blurs x 22 taps x 2 + elementwise)
+#   project:    236 B read per Gaussian per view + 52 B written per visible splat
+#   geometry_adam: 1652 B per Gaussian (59 params x 28 B + 8 B statistics)
+FWD_OPS_PER_PAIR = 26
+BWD_OPS_PER_PAIR = 62
+SSIM_FLOP_PER_PIXEL = 1440
+
+
+def sync():
+    """Device-wide barrier. Every train_step already ends with a stream
+    synchronisation; torch's is added when torch is loaded in the process."""
+    torch = sys.modules.get("torch")
+    if torch is not None and torch.cuda.is_available():
+        torch.cuda.synchronize()
+
+
+def with_alpha(rgb):
+    return np.concatenate([rgb, np.ones(rgb.shape[:2] + (1,), np.float32)], -1)
+
+
+def grad_rel_err(ctx) -> float:
+    """Max over parameter groups of ||g_gpu - g_cpu|| / ||g_cpu|| at the
+    10k-Gaussian 256x256 parity size."""
+    import oracle
+    from rgsplat_amd import synthetic as S
+    wl = S.WORKLOADS["c0_10k_256"]
+    g = wl.scene()
+    cam = wl.cameras()[0]
+    tgt = with_alpha(oracle.render_color(wl.target_scene(), cam))
+    _, gr = ctx.render_and_backward(g, cam, tgt)
+    _, ref = oracle.render_and_backward(g, cam, tgt)
+    worst = 0.0
+    for sl in [slice(0, 3), slice(3, 7), slice(7, 10), slice(10, 11), slice(11, 59)]:
+        d = np.linalg.norm(ref[:, sl])
+        worst = max(worst, float(np.linalg.norm(gr[:, sl] - ref[:, sl]) / d))
+    return worst
+
+
+def cpu_baseline(g, cams, targets, cfg, threads):
+    """The oracle (C++ restatement of the reference CPU path, OpenMP where the
+    reference uses rayon) on one view of the workload plus one Adam step."""
+    import oracle
+    oracle.set_threads(threads)
+    gg = np.ascontiguousarray(g.copy())
+    t0 = time.perf_counter()
+    _, grads = oracle.render_and_backward(gg, cams[0], targets[0])
+    t1 = time.perf_counter()
+    st = oracle.AdamState(len(gg))
+    oracle.adam_step(gg, grads, st, cfg)
+    t2 = time.perf_counter()
+    per_step = len(cams) * (t1 - t0) + (t2 - t1)
+    return {"value": 1.0 / per_step, "unit": "iters/s", "cores": threads, "kind": "port",
+            "sample": f"1 of the {len(cams)} views (render_and_backward) + 1 adam_step at the full workload size; "
+                      f"step time = {len(cams)} x {t1 - t0:.2f} s + {t2 - t1:.2f} s"}
+
+
+def plain_path(ctx, g, cams, targets, cfg):
+    """The reference-API path on the GPU: one render_and_backward call per
+    view (host buffers in and out) and one adam_step call."""
+    from rgsplat_amd.runtime import AdamState
+    gg = np.ascontiguousarray(g.copy())
+    st = AdamState(len(gg))
+    steps = 2
+    for it in range(steps + 1):
+        if it == 1:
+            t0 = time.perf_counter()
+        tot = np.zeros((len(gg), 60), np.float32)
+        for c, t in zip(cams, targets):
+            _, gr = ctx.render_and_backward(gg, c, t)
+            tot += gr
+        ctx.adam_step(gg, tot, st, cfg)
+    t1 = time.perf_counter()
+    return steps / (t1 - t0)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c3_1m_1080p_4views")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-path measurement")
+    ap.add_argument("--no-profile", action="store_true", help="skip the per-stage event timing")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the step shards by views "
+              "across GPUs as described in DESIGN.md, but no multi-GPU path is built", file=sys.stderr)
+        sys.exit(2)
+
+    from rgsplat_amd import runtime
+    from rgsplat_amd import synthetic as S
+
+    wl = S.WORKLOADS[args.workload]
+    print(f"[bench] workload {wl.name}: {wl.n} Gaussians, views {wl.sizes}", file=sys.stderr, flush=True)
+    g = wl.scene()
+    cams = wl.cameras()
+    ctx = runtime.Context(0)
+    tscene = wl.target_scene()
+    targets = [with_alpha(ctx.render_color(tscene, c)) for c in cams]
+    del tscene
+    cfg = S.benchmark_train_config()
+    ctx.scene_upload(g, capacity=len(g))
+    ctx.set_views(cams, targets)
+    ids = list(range(len(cams)))
+    use_graph = not args.no_graph
+    print("[bench] warmup", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        ctx.train_step(ids, cfg, use_graph=use_graph)
+    sync()
+    t0 = time.perf_counter()
+    losses = None
+    for _ in range(args.steps):
+        losses = ctx.train_step(ids, cfg, use_graph=use_graph)
+    sync()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1000.0 / args.steps
+    print(f"[bench] {ms:.3f} ms/step, losses {losses}", file=sys.stderr, flush=True)
+
+    out = {"metric": METRIC, "value": 1000.0 / ms, "unit": "iters/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene, targets rendered from a second seeded scene)",
+           "config": {"workload": wl.name, "gaussians": wl.n, "sh_degree": 3, "views_per_step": len(cams),
+                      "resolutions": sorted(set(f"{w}x{h}" for w, h in wl.sizes)),
+                      "execution": "device-resident step, views batched per launch, "
+                                   + ("hipGraph replay" if use_graph else "eager")}}
+
+    if not args.no_profile:
+        ctx.set_profiling(True)
+        for _ in range(3):
+            ctx.train_step(ids, cfg, use_graph=False)
+        stages, n_prof = ctx.stage_times()
+        ctx.set_profiling(False)
+        K, pf, pb = ctx.step_work()
+        npix = sum(w * h for w, h in wl.sizes)
+        ng = ctx.scene_size()
+        work = {
+            "raster_fwd": ("mfma", FWD_OPS_PER_PAIR * pf, "TFLOP/s"),
+            "raster_bwd": ("mfma", BWD_OPS_PER_PAIR * pb, "TFLOP/s"),
+            "loss_ssim": ("mfma", SSIM_FLOP_PER_PIXEL * npix, "TFLOP/s"),
+            "project": ("hbm", 236 * ng * len(cams) + 52 * ng * len(cams), "GB/s"),
+            "geometry_adam": ("hbm", 1652 * ng, "GB/s"),
+        }
+        dom = max(stages, key=lambda s: stages[s])
+        rl = {}
+        for s, (bound, amount, unit) in work.items():
+            sec = stages[s] / 1000.0
+            if bound == "hbm":
+                ach = amount / sec / 1e9 if sec > 0 else 0.0
+                peak = HBM_PEAK_GBS
+            else:
+                ach = amount / sec / 1e12 if sec > 0 else 0.0
+                peak = FP32_PEAK_TFLOPS
+            rl[s] = {"bound": bound, "achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak,
+                     "traffic": None, "ms": stages[s]}
+        if dom in rl:
+            out["roofline"] = dict(rl[dom], kernel=dom)
+        else:
+            out["roofline"] = {"kernel": dom, "bound": None, "achieved": None, "peak": None, "unit": None,
+                               "frac": None, "traffic": None, "ms": stages[dom]}
+        out["stages_ms"] = stages
+        out["stage_rooflines"] = rl
+        out["work"] = {"tile_pairs": K, "fwd_pair_evals": pf, "bwd_pair_evals": pb, "gaussians": ng,
+                       "profiled_steps": n_prof}
+        print(f"[bench] stages {json.dumps(stages)}", file=sys.stderr, flush=True)
+
+    if not args.no_plain:
+        print("[bench] plain path", file=sys.stderr, flush=True)
+        out["plain_gpu_path_iters_per_s"] = plain_path(ctx, g, cams, targets, cfg)
+    try:
+        out["grad_rel_err"] = grad_rel_err(ctx)
+    except Exception as e:  # the oracle may be unbuilt on a bare box
+        out["grad_rel_err"] = None
+        print(f"[bench] grad check skipped: {e}", file=sys.stderr)
+    if not args.no_cpu:
+        print("[bench] cpu baseline (oracle on the host)", file=sys.stderr, flush=True)
+        threads = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(g, cams, targets, cfg, threads)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

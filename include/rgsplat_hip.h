@@ -1,0 +1,186 @@
+/* rgsplat_hip.h -- C ABI of the MI355X (gfx950) Gaussian-splatting training
+ * step. Drop-in for the Rayon CPU path of the rgsplat crate: the record
+ * layouts are byte-compatible with its #[repr(C)] types, and each entry point
+ * below names the crate function it replaces.
+ *
+ * Conventions: every function returns 0 on success and a nonzero RGS_E_* code
+ * on failure (never aborts, never unwinds); rgs_last_error() describes the
+ * last failure on the calling thread. A context is single-threaded: it owns
+ * one HIP stream and its device buffers. Host pointers are borrowed for the
+ * duration of the call; results are complete when the call returns.
+ */
+#ifndef RGSPLAT_HIP_H
+#define RGSPLAT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  RGS_OK = 0,
+  RGS_E_ARG = 1,      /* invalid argument (null pointer, size mismatch) */
+  RGS_E_HIP = 2,      /* HIP runtime failure */
+  RGS_E_CAMERA = 3,   /* c2w not invertible (math/mod.rs:155-160 panics) */
+  RGS_E_NAN = 4,      /* NaN depth (backward.rs:191 panics in the sort) */
+  RGS_E_CAPACITY = 5  /* densify result exceeds the caller's capacity */
+};
+
+/* Gaussian3D (math/mod.rs:27-46), 59 x f32 = 236 bytes. */
+typedef struct rgs_gaussian {
+  float position[3];
+  float opacity_logit;
+  float rotation[4]; /* w, x, y, z */
+  float log_scale[3];
+  float sh_coeffs[48]; /* channel-major: sh[ch * 16 + k] */
+} rgs_gaussian;
+
+/* GaussianGrad (backward.rs:33-41), 60 x f32 = 240 bytes. Field order equals
+ * the optimizer's flat parameter order plus screen_grad_norm. */
+typedef struct rgs_grad {
+  float d_position[3];
+  float d_rotation[4];
+  float d_log_scale[3];
+  float d_opacity_logit;
+  float d_sh[48];
+  float screen_grad_norm;
+} rgs_grad;
+
+/* Camera (math/mod.rs:130-145); c2w row-major, OpenCV convention. */
+typedef struct rgs_camera {
+  uint32_t width, height;
+  float fx, fy, cx, cy;
+  float c2w[16];
+  float distortion[3]; /* ignored, as in the reference */
+} rgs_camera;
+
+/* TrainConfig (pipeline/optimizer.rs:23-40). */
+typedef struct rgs_train_cfg {
+  float lr_position, lr_rotation, lr_scale, lr_opacity, lr_sh_dc, lr_sh_rest;
+  float beta1, beta2, eps;
+  float densify_grad_threshold, prune_opacity_thresh;
+  uint32_t densify_from_iter, densify_until_iter, opacity_reset_iter;
+  uint64_t max_gaussians, num_iterations;
+} rgs_train_cfg;
+
+/* AdamState (pipeline/optimizer.rs:74-84) in host memory: m1/m2 are
+ * rows x 59, grad_accum/grad_denom have `rows` entries; buffers have room
+ * for `capacity` rows. rows only grows (AdamState::resize, :97-105). */
+typedef struct rgs_adam {
+  float* m1;
+  float* m2;
+  float* grad_accum;
+  uint32_t* grad_denom;
+  size_t rows;
+  size_t capacity;
+  uint64_t t;
+} rgs_adam;
+
+typedef struct rgs_ctx rgs_ctx;
+
+/* Context lifecycle (no reference counterpart: the crate uses the global rayon
+ * pool, pipeline/mod.rs:191-194). device < 0 selects the current device. */
+int rgs_create(int device, rgs_ctx** out);
+void rgs_destroy(rgs_ctx* ctx);
+const char* rgs_last_error(void);
+/* Use an external HIP stream (e.g. a framework's current stream); NULL
+ * restores the context's own stream. */
+int rgs_set_stream(rgs_ctx* ctx, void* hip_stream);
+
+/* render_and_backward (cpu/backward.rs:107-149). target_rgba is w*h*4 f32;
+ * grads_out has n entries, zero for culled Gaussians. */
+int rgs_render_and_backward(rgs_ctx* ctx, const rgs_gaussian* g, size_t n, const rgs_camera* cam,
+                            const float* target_rgba, const float bg[3], float lambda_dssim, float* loss_out,
+                            rgs_grad* grads_out);
+
+/* render_color (cpu/backward.rs:152-166): rgb_out is h*w*3. */
+int rgs_render_color(rgs_ctx* ctx, const rgs_gaussian* g, size_t n, const rgs_camera* cam, const float bg[3],
+                     float* rgb_out);
+
+/* rasterize (cpu/rasterizer.rs:72-192): rgba_out is h*w*4 (the Framebuffer
+ * data; pixels of tiles no splat touches stay 0). */
+int rgs_rasterize_rgba(rgs_ctx* ctx, const rgs_gaussian* g, size_t n, const rgs_camera* cam, const float bg[3],
+                       float* rgba_out);
+
+/* photometric_loss_grad (cpu/backward.rs:646-693): rgb is w*h*3, target
+ * w*h*4, dl_drgb_out w*h*3. */
+int rgs_loss_grad(rgs_ctx* ctx, const float* rgb, const float* target_rgba, uint32_t w, uint32_t h,
+                  float lambda_dssim, float* loss_out, float* dl_drgb_out);
+
+/* adam_step (pipeline/optimizer.rs:116-163): updates g and st in place. */
+int rgs_adam_step(rgs_ctx* ctx, rgs_gaussian* g, const rgs_grad* grads, size_t n, rgs_adam* st,
+                  const rgs_train_cfg* cfg);
+
+/* densify_and_prune (pipeline/optimizer.rs:206-272): g has room for
+ * `capacity` Gaussians, *n is updated. Fails with RGS_E_CAPACITY if the result
+ * does not fit (g and st are then unchanged). */
+int rgs_densify_and_prune(rgs_ctx* ctx, rgs_gaussian* g, size_t* n, size_t capacity, rgs_adam* st,
+                          const rgs_train_cfg* cfg, float scene_extent);
+
+/* reset_opacities (pipeline/optimizer.rs:314-319). */
+int rgs_reset_opacities(rgs_ctx* ctx, rgs_gaussian* g, size_t n);
+
+/* ---- Device-pointer forms for framework bindings (the torch.autograd
+ * Function over render_color / the raster backward). g_dev is n x 59 f32 in
+ * Gaussian3D order on the device. rgs_forward_dev keeps the binning state in
+ * the context for the next rgs_backward_dev. */
+int rgs_forward_dev(rgs_ctx* ctx, const float* g_dev, size_t n, const rgs_camera* cam, const float bg[3],
+                    float* rgb_dev);
+/* dL/drgb (h*w*3, device) -> grads_dev (n x 60 GaussianGrad, device). */
+int rgs_backward_dev(rgs_ctx* ctx, const float* dl_drgb_dev, float* grads_dev);
+
+/* ---- Device-resident training (the step body of pipeline/mod.rs:232-247,
+ * batched over views). Parameters, Adam moments and densify statistics stay
+ * in HBM between calls. */
+int rgs_scene_upload(rgs_ctx* ctx, const rgs_gaussian* g, size_t n, size_t capacity);
+int rgs_scene_size(rgs_ctx* ctx, size_t* n);
+int rgs_scene_download(rgs_ctx* ctx, rgs_gaussian* g, size_t capacity, size_t* n);
+/* Downloads the Adam state (m1/m2 rows x 59, accum, denom, t) into st. */
+int rgs_adam_download(rgs_ctx* ctx, rgs_adam* st);
+/* Registers the training views and their targets (w*h*4 f32 each). */
+int rgs_set_views(rgs_ctx* ctx, const rgs_camera* cams, const float* const* targets_rgba, int n_views);
+/* One training iteration over the views listed in view_ids: forward and
+ * backward of every listed view in single launches, gradients summed over the
+ * views, one Adam step. loss_out (may be NULL) receives one loss per listed
+ * view. use_graph != 0 replays the step as a captured hipGraph. */
+int rgs_train_step(rgs_ctx* ctx, const int* view_ids, int n_ids, const rgs_train_cfg* cfg, const float bg[3],
+                   float lambda_dssim, int use_graph, float* loss_out);
+/* densify_and_prune / reset_opacities on the device-resident scene. */
+int rgs_scene_densify(rgs_ctx* ctx, const rgs_train_cfg* cfg, float scene_extent);
+int rgs_scene_reset_opacities(rgs_ctx* ctx);
+/* Last render's (tile, splat) pair count, summed over its views, and the
+ * (pixel, splat) pairs its forward and backward passes evaluate (each output
+ * pointer may be NULL). */
+int rgs_step_stats(rgs_ctx* ctx, uint64_t* n_pairs, uint64_t* fwd_evals, uint64_t* bwd_evals);
+
+/* Raster tile-work decomposition chosen by the autotuner: the binning tile
+ * is fixed at 16x16 by the reference, so this selects the splat batch staged
+ * in LDS per round. 0 = automatic. */
+int rgs_set_raster_batch(rgs_ctx* ctx, int batch);
+
+/* ---- Stage timing (no reference counterpart). With profiling on,
+ * rgs_train_step runs its kernels eagerly and brackets each stage with HIP
+ * events on the context's stream. Stages, in order: 0 project, 1 depth sort,
+ * 2 pair-count scan, 3 pair generation, 4 tile sort, 5 tile ranges,
+ * 6 forward raster, 7 L1+D-SSIM, 8 backward raster, 9 geometry + Adam.
+ * rgs_stage_times returns the mean milliseconds per stage since profiling
+ * was enabled and the number of profiled steps. */
+#define RGS_N_STAGES 10
+int rgs_set_profiling(rgs_ctx* ctx, int enable);
+int rgs_stage_times(rgs_ctx* ctx, float* mean_ms, int n_stages, uint64_t* steps);
+
+/* ---- Diagnostics (no reference counterpart; used by the parity tests).
+ * rgs_eval_expf evaluates the device expf every kernel uses on n host
+ * values. rgs_debug_tiles returns the last render's tile lists as CSR of
+ * Gaussian indices in list order (offsets has total_tiles + 1 entries; gis
+ * room for k_max entries; *k_out receives the pair count). */
+int rgs_eval_expf(rgs_ctx* ctx, const float* x, float* out, size_t n);
+int rgs_debug_tiles(rgs_ctx* ctx, uint64_t* offsets, size_t n_offsets, uint32_t* gis, size_t k_max,
+                    uint64_t* k_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RGSPLAT_HIP_H */

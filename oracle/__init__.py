@@ -108,6 +108,7 @@ def _declare(L):
     L.orc_densify_and_prune.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t), sz, vp, vp, vp, vp,
                                         ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(TrainCfg), ctypes.c_float]
     L.orc_set_threads.argtypes = [i32]
+    L.orc_expf_batch.argtypes = [vp, vp, sz]
 
 
 def _p(a):
@@ -301,6 +302,13 @@ def densify_and_prune(g_buf, n, st: AdamState, cfg: dict, extent: float):
         raise ValueError("capacity exceeded")
     st.rows = rows.value
     return nn.value
+
+
+def expf_batch(x):
+    x = np.ascontiguousarray(x, np.float32).reshape(-1)
+    out = np.zeros_like(x)
+    lib().orc_expf_batch(_p(x), _p(out), len(x))
+    return out
 
 
 def f32vec(lst):

@@ -1106,6 +1106,12 @@ void orc_blur(const float* src, size_t w, size_t h, float* out) {
   blur<float>(src, w, h, k, 11, out);
 }
 
+// glibc expf over n values (the libm call behind Rust's f32::exp).
+void orc_expf_batch(const float* x, float* out, size_t n) {
+#pragma omp parallel for schedule(static)
+  for (long long i = 0; i < (long long)n; ++i) out[i] = expf(x[i]);
+}
+
 // ---- math unit-level exports (math/*.rs)
 float orc_sigmoid(float x) { return sigmoid(x); }
 float orc_logit(float p) { return logit(p); }

@@ -1,0 +1,499 @@
+// k_optim.hip -- geometry backprop, Adam, densify/prune, opacity reset and
+// parameter layout transposes.
+//
+// geometry_grad follows backward.rs:518-638 (with covariance.rs:42-84 and
+// quaternion.rs:10-70), recomputing p_cam, the 2-D covariance, the SH basis
+// and the raw colour from the parameters instead of storing a per-splat
+// cache. Adam follows optimizer.rs:116-195; densify/prune follows
+// optimizer.rs:206-311 as a scan-and-scatter compaction whose exclusive scan
+// of requested additions reproduces the order-dependent max_gaussians cap
+// exactly. All HBM-bound.
+#include "rgs_kernels.h"
+
+namespace rgs {
+
+// Gaussian3D offset of Adam flat index j (apply_update_to_gaussian).
+__device__ __forceinline__ int aos_offset(int j) {
+  if (j < 3) return j;
+  if (j < 10) return j + 1;
+  if (j == 10) return 3;
+  return j;
+}
+
+template <class Load>
+__device__ __forceinline__ void geometry_grad(const CamDev& c, Load prm, const float acc[9], float* out,
+                                              bool add) {
+  // prm(j) returns parameter j in Adam flat order.
+  const float px = prm(0), py = prm(1), pz = prm(2);
+  const float q[4] = {prm(3), prm(4), prm(5), prm(6)};
+  const float scale[3] = {rgs_expf(prm(7)), rgs_expf(prm(8)), rgs_expf(prm(9))};
+  const float op = sigmoid(prm(10));
+  float res[60];
+#define OUT(j, val) res[j] = (val)
+  OUT(10, acc[3] * op * (1.0f - op));
+
+  float pc[3];
+  to_camera(c, px, py, pz, pc);
+  const float tx = pc[0], ty = pc[1], tz = pc[2];
+  // SH basis and raw colour (projection side)
+  float u0 = c.cam_pos[0] - px, u1 = c.cam_pos[1] - py, u2 = c.cam_pos[2] - pz;
+  float ulen = __builtin_sqrtf(u0 * u0 + u1 * u1 + u2 * u2);
+  float d0, d1, d2;
+  if (ulen < 1e-8f) {
+    d0 = 0.0f; d1 = 0.0f; d2 = 1.0f;
+  } else {
+    d0 = u0 / ulen; d1 = u1 / ulen; d2 = u2 / ulen;
+  }
+  float basis[16];
+  sh_basis(d0, d1, d2, basis);
+  float masked[3];
+#pragma unroll
+  for (int ch = 0; ch < 3; ++ch) {
+    float a = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) a += basis[k] * prm(P_SH + ch * 16 + k);
+    float raw = 0.5f + a;
+    masked[ch] = (raw > 0.0f && raw < 1.0f) ? acc[ch] : 0.0f;
+  }
+#pragma unroll
+  for (int ch = 0; ch < 3; ++ch)
+#pragma unroll
+    for (int k = 0; k < 16; ++k) OUT(P_SH + ch * 16 + k, masked[ch] * basis[k]);
+
+  // 2-D covariance as projected (projection.rs:24-69)
+  M3 sigma = covariance_3d(q[0], q[1], q[2], q[3], scale[0], scale[1], scale[2]);
+  M3 T = ewa_T(c, tx, ty, tz);
+  M3 s2d = m3_mul(m3_mul(T, sigma), m3_t(T));
+  const float a2 = s2d.a[0][0] + 0.3f, b2 = s2d.a[0][1], c2 = s2d.a[1][1] + 0.3f;
+  const float ga = acc[4], gb = acc[5], gc = acc[6];
+  const float det = a2 * c2 - b2 * b2;
+  const float det2 = det * det;
+  const float dl_da = (-ga * c2 * c2 + gb * b2 * c2 - gc * b2 * b2) / det2;
+  const float dl_db = (2.0f * ga * b2 * c2 - gb * (det + 2.0f * b2 * b2) + 2.0f * gc * a2 * b2) / det2;
+  const float dl_dc = (-ga * b2 * b2 + gb * a2 * b2 - gc * a2 * a2) / det2;
+
+  M3 G = m3_zero();
+  G.a[0][0] = dl_da;
+  G.a[0][1] = 0.5f * dl_db;
+  G.a[1][0] = 0.5f * dl_db;
+  G.a[1][1] = dl_dc;
+  M3 dsig = m3_mul(m3_mul(m3_t(T), G), T);
+  {
+    // backprop_cov3d (covariance.rs:42-84)
+    float qn[4];
+    quat_normalize(q, qn);
+    M3 r = mat_from_quat(qn);
+    M3 m;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) m.a[i][j] = r.a[i][j] * scale[j];
+    M3 gs;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) gs.a[i][j] = dsig.a[i][j] + dsig.a[j][i];
+    M3 dl_dm = m3_mul(gs, m);
+    M3 dl_dr;
+    for (int j = 0; j < 3; ++j) {
+      float a = 0.0f;
+      for (int i = 0; i < 3; ++i) {
+        a += dl_dm.a[i][j] * r.a[i][j];
+        dl_dr.a[i][j] = dl_dm.a[i][j] * scale[j];
+      }
+      OUT(P_LS + j, a * scale[j]);
+    }
+    const float w = qn[0], x = qn[1], y = qn[2], z = qn[3];
+    const float jac[4][9] = {
+        {0.0f, -2.0f * z, 2.0f * y, 2.0f * z, 0.0f, -2.0f * x, -2.0f * y, 2.0f * x, 0.0f},
+        {0.0f, 2.0f * y, 2.0f * z, 2.0f * y, -4.0f * x, -2.0f * w, 2.0f * z, 2.0f * w, -4.0f * x},
+        {-4.0f * y, 2.0f * x, 2.0f * w, 2.0f * x, 0.0f, 2.0f * z, -2.0f * w, 2.0f * z, -4.0f * y},
+        {-4.0f * z, -2.0f * w, 2.0f * x, 2.0f * w, -4.0f * z, 2.0f * y, 2.0f * x, 2.0f * y, 0.0f}};
+    for (int k = 0; k < 4; ++k) {
+      float a = 0.0f;  // column-major fold of the component products
+      for (int cc = 0; cc < 3; ++cc)
+        for (int rr = 0; rr < 3; ++rr) a = a + jac[k][rr * 3 + cc] * dl_dr.a[rr][cc];
+      OUT(P_ROT + k, a);
+    }
+  }
+
+  // screen mean and the mean-dependence of J -> p_cam -> position
+  const float fx = c.fx, fy = c.fy;
+  const float asx = acc[7], asy = acc[8];
+  float dpc[3] = {asx * fx / tz, asy * fy / tz, asx * (-fx * tx / (tz * tz)) + asy * (-fy * ty / (tz * tz))};
+  M3 W;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) W.a[i][j] = c.view[i * 4 + j];
+  M3 sig_tt = m3_mul(sigma, m3_t(T));
+  M3 t_sig = m3_mul(T, sigma);
+  for (int k = 0; k < 3; ++k) {
+    M3 dj = m3_zero();
+    if (k == 0) {
+      dj.a[0][2] = -fx / (tz * tz);
+    } else if (k == 1) {
+      dj.a[1][2] = -fy / (tz * tz);
+    } else {
+      dj.a[0][0] = -fx / (tz * tz);
+      dj.a[0][2] = 2.0f * fx * tx / (tz * tz * tz);
+      dj.a[1][1] = -fy / (tz * tz);
+      dj.a[1][2] = 2.0f * fy * ty / (tz * tz * tz);
+    }
+    M3 dt = m3_mul(dj, W);
+    M3 p1 = m3_mul(dt, sig_tt);
+    M3 p2 = m3_mul(t_sig, m3_t(dt));
+    float d00 = p1.a[0][0] + p2.a[0][0], d01 = p1.a[0][1] + p2.a[0][1], d11 = p1.a[1][1] + p2.a[1][1];
+    float contrib = dl_da * d00 + dl_db * d01 + dl_dc * d11;
+    dpc[k] += contrib;
+  }
+  float dpos[3];
+  for (int i = 0; i < 3; ++i) {
+    float a = W.a[0][i] * dpc[0];
+    a = a + W.a[1][i] * dpc[1];
+    a = a + W.a[2][i] * dpc[2];
+    dpos[i] = a;
+  }
+  // view-dependent colour: dir = (cam - pos)/|cam - pos|
+  if (ulen > 1e-8f) {
+    float e0 = u0 / ulen, e1 = u1 / ulen, e2 = u2 / ulen;
+    float dk[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+      dk[k] = masked[0] * prm(P_SH + k) + masked[1] * prm(P_SH + 16 + k) + masked[2] * prm(P_SH + 32 + k);
+    float dd[3];
+    sh_basis_grad_contract(e0, e1, e2, dk, dd);
+    float dot = e0 * dd[0] + e1 * dd[1] + e2 * dd[2];
+    const float e[3] = {e0, e1, e2};
+    for (int cc = 0; cc < 3; ++cc) {
+      float proj = dd[cc] - e[cc] * dot;
+      dpos[cc] += -(proj / ulen);
+    }
+  }
+  OUT(0, dpos[0]);
+  OUT(1, dpos[1]);
+  OUT(2, dpos[2]);
+  OUT(59, __builtin_sqrtf(asx * asx + asy * asy));
+#undef OUT
+  if (add) {
+#pragma unroll
+    for (int j = 0; j < 60; ++j) out[j] += res[j];
+  } else {
+#pragma unroll
+    for (int j = 0; j < 60; ++j) out[j] = res[j];
+  }
+}
+
+struct SoaLoad {
+  const float* P;
+  size_t cap;
+  size_t i;
+  __device__ float operator()(int j) const { return P[(size_t)j * cap + i]; }
+};
+
+__global__ __launch_bounds__(256) void k_geometry_aos(const float* __restrict__ P, int cap, int n,
+                                                      const CamDev* __restrict__ cams, int view,
+                                                      const uint32_t* __restrict__ cnt,
+                                                      const float* __restrict__ accum, size_t acc_stride,
+                                                      float* __restrict__ grads) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  size_t vi = (size_t)view * cap + i;
+  float* o = grads + (size_t)i * 60;
+  if (cnt[vi] == 0) {
+    for (int j = 0; j < 60; ++j) o[j] = 0.0f;
+    return;
+  }
+  float acc[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) acc[k] = accum[(size_t)k * acc_stride + vi];
+  float g[60];
+  geometry_grad(cams[view], SoaLoad{P, (size_t)cap, (size_t)i}, acc, g, false);
+  for (int j = 0; j < 60; ++j) o[j] = g[j];
+}
+
+void launch_geometry_aos(const float* P, int cap, int n, const CamDev* cams, int view, const uint32_t* cnt,
+                         const float* accum, size_t acc_stride, float* grads, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_geometry_aos, dim3((n + 255) / 256), dim3(256), 0, st, P, cap, n, cams, view, cnt, accum,
+                     acc_stride, grads);
+}
+
+__device__ __forceinline__ float lr_of(int j, const AdamConsts& k) {
+  if (j < 3) return k.lr[0];
+  if (j < 7) return k.lr[1];
+  if (j < 10) return k.lr[2];
+  if (j == 10) return k.lr[3];
+  if (j < 14) return k.lr[4];
+  return k.lr_sh_rest;
+}
+
+// optimizer.rs:142-147
+__device__ __forceinline__ float adam_one(float p, float g, float& m, float& v, float lr, const AdamConsts& k) {
+  m = k.beta1 * m + k.one_minus_beta1 * g;
+  v = k.beta2 * v + k.one_minus_beta2 * g * g;
+  float mh = m / k.bc1;
+  float vh = v / k.bc2;
+  float upd = lr * mh / (__builtin_sqrtf(vh) + k.eps);
+  return p - upd;
+}
+
+__device__ __forceinline__ void renorm_quat(float& a, float& b, float& c, float& d) {
+  float nrm = __builtin_sqrtf(a * a + b * b + c * c + d * d);
+  if (nrm > 1e-8f) {
+    a /= nrm; b /= nrm; c /= nrm; d /= nrm;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_adam_aos(float* __restrict__ g, const float* __restrict__ grads, size_t n,
+                                                  float* __restrict__ m1, float* __restrict__ m2,
+                                                  float* __restrict__ accum, uint32_t* __restrict__ denom,
+                                                  AdamConsts k) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float* p = g + i * 59;
+  const float* gr = grads + i * 60;
+  float* a = m1 + i * 59;
+  float* b = m2 + i * 59;
+  for (int j = 0; j < 59; ++j) {
+    float mm = a[j], vv = b[j];
+    int o = aos_offset(j);
+    p[o] = adam_one(p[o], gr[j], mm, vv, lr_of(j, k), k);
+    a[j] = mm;
+    b[j] = vv;
+  }
+  accum[i] += gr[59];
+  denom[i] += 1u;
+  float q0 = p[4], q1 = p[5], q2 = p[6], q3 = p[7];
+  renorm_quat(q0, q1, q2, q3);
+  p[4] = q0; p[5] = q1; p[6] = q2; p[7] = q3;
+}
+
+void launch_adam_aos(float* g, const float* grads, size_t n, float* m1, float* m2, float* accum, uint32_t* denom,
+                     const AdamConsts& k, hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_adam_aos, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, grads, n, m1, m2, accum,
+                     denom, k);
+}
+
+__global__ __launch_bounds__(256) void k_train_adam(float* __restrict__ P, float* __restrict__ M1,
+                                                    float* __restrict__ M2, int cap, int n,
+                                                    const CamDev* __restrict__ cams, int n_views,
+                                                    const uint32_t* __restrict__ cnt, float* __restrict__ accum,
+                                                    size_t acc_stride, float* __restrict__ gaccum,
+                                                    uint32_t* __restrict__ gdenom,
+                                                    const AdamConsts* __restrict__ k_dev,
+                                                    const uint32_t* __restrict__ status) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (*status & (ST_PAIR_OVERFLOW | ST_NAN_DEPTH)) return;
+  const AdamConsts k = *k_dev;
+  float g[60];
+#pragma unroll
+  for (int j = 0; j < 60; ++j) g[j] = 0.0f;
+  SoaLoad L{P, (size_t)cap, (size_t)i};
+  for (int v = 0; v < n_views; ++v) {
+    size_t vi = (size_t)v * cap + i;
+    if (cnt[vi] == 0) continue;
+    float acc[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      acc[q] = accum[(size_t)q * acc_stride + vi];
+      accum[(size_t)q * acc_stride + vi] = 0.0f;
+    }
+    geometry_grad(cams[v], L, acc, g, true);
+  }
+  float rot[4];
+  for (int j = 0; j < 59; ++j) {
+    size_t o = (size_t)j * cap + i;
+    float mm = M1[o], vv = M2[o];
+    float p = adam_one(P[o], g[j], mm, vv, lr_of(j, k), k);
+    M1[o] = mm;
+    M2[o] = vv;
+    if (j >= P_ROT && j < P_ROT + 4)
+      rot[j - P_ROT] = p;
+    else
+      P[o] = p;
+  }
+  renorm_quat(rot[0], rot[1], rot[2], rot[3]);
+  for (int q = 0; q < 4; ++q) P[(size_t)(P_ROT + q) * cap + i] = rot[q];
+  gaccum[i] += g[59];
+  gdenom[i] += (uint32_t)n_views;
+}
+
+void launch_train_adam(float* P, float* M1, float* M2, int cap, int n, const CamDev* cams, int n_views,
+                       const uint32_t* cnt, float* accum, size_t acc_stride, float* gaccum, uint32_t* gdenom,
+                       const AdamConsts* k_dev, const uint32_t* status, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_train_adam, dim3((n + 255) / 256), dim3(256), 0, st, P, M1, M2, cap, n, cams, n_views, cnt,
+                     accum, acc_stride, gaccum, gdenom, k_dev, status);
+}
+
+// ln on the device: the f64 logarithm rounded once to f32 (correctly rounded
+// except in double-rounding corner cases).
+__device__ __forceinline__ float logf_dev(float x) { return (float)log((double)x); }
+__device__ __forceinline__ float logit_dev(float p) { return logf_dev(p / (1.0f - p)); }
+
+// optimizer.rs:314-319
+__global__ void k_reset_opacity(float* p, size_t n, size_t stride) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float* x = p + i * stride;
+  *x = logit_dev(fmin_(0.01f, sigmoid(*x)));
+}
+void launch_reset_opacity(float* p, size_t n, size_t stride, hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_reset_opacity, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, stride);
+}
+
+// ---- densify/prune (optimizer.rs:206-311)
+// want bits: 0-1 children requested (0, 1 clone, 2 split), bit 2 pruned.
+__global__ void k_densify_flags(const float* __restrict__ P, int cap, int n, const float* __restrict__ gaccum,
+                                const uint32_t* __restrict__ gdenom, float grad_thresh, float prune_thresh,
+                                float extent, uint32_t* __restrict__ want, uint32_t* __restrict__ cnt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float alpha = sigmoid(P[(size_t)P_OP * cap + i]);
+  float s0 = rgs_expf(P[(size_t)(P_LS + 0) * cap + i]);
+  float s1 = rgs_expf(P[(size_t)(P_LS + 1) * cap + i]);
+  float s2 = rgs_expf(P[(size_t)(P_LS + 2) * cap + i]);
+  float ms = fmax_(fmax_(s0, s1), s2);
+  uint32_t d = gdenom[i];
+  float avg = d > 0 ? gaccum[i] / (float)d : 0.0f;
+  uint32_t w = 0;
+  if (alpha < prune_thresh || ms > 0.1f * extent) {
+    w = 4u;
+  } else if (avg >= grad_thresh) {
+    w = (ms > 0.01f * extent) ? 2u : 1u;
+  }
+  want[i] = w;
+  cnt[i] = w & 3u;
+}
+
+// After the exclusive scan W of requested children: Gaussian i adds its
+// children iff n + W_i < max_gaussians (the reference's running check).
+__global__ void k_densify_decide(const uint32_t* __restrict__ want, const uint32_t* __restrict__ W, int n,
+                                 uint64_t max_gaussians, uint64_t* __restrict__ packed) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t w = want[i];
+  uint32_t c = w & 3u;
+  uint32_t actual = (c > 0 && (uint64_t)n + W[i] < max_gaussians) ? c : 0u;
+  uint32_t keep = (w & 4u) ? 0u : ((c == 2u && actual) ? 0u : 1u);
+  packed[i] = (uint64_t)keep | ((uint64_t)actual << 32);
+}
+
+__global__ void k_densify_scatter(const float* __restrict__ P, float* __restrict__ Q, int cap, int n,
+                                  const uint64_t* __restrict__ packed_in, const uint64_t* __restrict__ prefix,
+                                  const uint64_t* __restrict__ total, const float* __restrict__ gaccum,
+                                  const uint32_t* __restrict__ gdenom, uint32_t* __restrict__ n_out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t tot = *total;
+  uint32_t kept_total = (uint32_t)(tot & 0xffffffffu);
+  if (i == 0) *n_out = kept_total + (uint32_t)(tot >> 32);
+  if (i >= n) return;
+  uint64_t pk = packed_in[i];
+  uint64_t pre = prefix[i];
+  uint32_t keep = (uint32_t)(pk & 0xffffffffu), actual = (uint32_t)(pk >> 32);
+  uint32_t kpos = (uint32_t)(pre & 0xffffffffu), apos = kept_total + (uint32_t)(pre >> 32);
+  // a result larger than the capacity is reported through *n_out and
+  // nothing past the capacity is written
+  if (keep && kpos < (uint32_t)cap)
+    for (int j = 0; j < P_N; ++j) Q[(size_t)j * cap + kpos] = P[(size_t)j * cap + i];
+  if (!actual || apos + actual > (uint32_t)cap) return;
+  if (actual == 1) {
+    // clone_gaussian (:303-311)
+    uint32_t d = gdenom[i];
+    float avg = d > 0 ? gaccum[i] / (float)d : 0.0f;
+    float jitter = avg * 0.01f;
+    for (int j = 0; j < P_N; ++j) {
+      float x = P[(size_t)j * cap + i];
+      if (j == 0 || j == 1) x += jitter;
+      Q[(size_t)j * cap + apos] = x;
+    }
+    return;
+  }
+  // split_gaussian (:274-301)
+  float s0 = rgs_expf(P[(size_t)P_LS * cap + i]);
+  float qw = P[(size_t)(P_ROT + 0) * cap + i], qx = P[(size_t)(P_ROT + 1) * cap + i];
+  float qy = P[(size_t)(P_ROT + 2) * cap + i], qz = P[(size_t)(P_ROT + 3) * cap + i];
+  float a = qx * qx, b = qy * qy, cq = qz * qz, d = qw * qw;
+  a = a + cq;
+  b = b + d;
+  float nq = __builtin_sqrtf(a + b);
+  float x = qx / nq, y = qy / nq, z = qz / nq, w = qw / nq;
+  // UnitQuaternion * (s0, 0, 0): t = 2 (q_v x v); v' = t w + q_v x t + v
+  float v0 = s0, v1 = 0.0f, v2 = 0.0f;
+  float t0 = (y * v2 - z * v1) * 2.0f, t1 = (z * v0 - x * v2) * 2.0f, t2 = (x * v1 - y * v0) * 2.0f;
+  float c0 = y * t2 - z * t1, c1 = z * t0 - x * t2, c2 = x * t1 - y * t0;
+  float ax0 = t0 * w + c0 + v0, ax1 = t1 * w + c1 + v1, ax2 = t2 * w + c2 + v2;
+  float nrm = __builtin_sqrtf(ax0 * ax0 + ax1 * ax1 + ax2 * ax2);
+  float off[3] = {ax0 / nrm * s0, ax1 / nrm * s0, ax2 / nrm * s0};
+  float new_scale = logf_dev(fmax_(s0 / 1.41421356237309504880f, 1e-6f));
+  float alpha = sigmoid(P[(size_t)P_OP * cap + i]);
+  float op = alpha * 0.8f;
+  if (op < 1e-6f) op = 1e-6f;
+  if (op > 0.999f) op = 0.999f;
+  float new_logit = logit_dev(op);
+  for (int ci = 0; ci < 2; ++ci) {
+    size_t o = apos + ci;
+    for (int j = 0; j < P_N; ++j) {
+      float xv = P[(size_t)j * cap + i];
+      if (j < 3) xv = ci == 0 ? xv + off[j] : xv - off[j];
+      if (j == P_LS) xv = new_scale;
+      if (j == P_OP) xv = new_logit;
+      Q[(size_t)j * cap + o] = xv;
+    }
+  }
+}
+
+void launch_densify(const float* P, float* P_out, int cap, int n, const float* gaccum, const uint32_t* gdenom,
+                    float grad_thresh, float prune_thresh, uint64_t max_gaussians, float extent, DensifyScratch ds,
+                    uint32_t* n_out, hipStream_t st) {
+  if (n <= 0) {
+    launch_fill_u32(n_out, 1, 0u, st);
+    return;
+  }
+  dim3 g((n + 255) / 256), b(256);
+  hipLaunchKernelGGL(k_densify_flags, g, b, 0, st, P, cap, n, gaccum, gdenom, grad_thresh, prune_thresh, extent,
+                     ds.want, ds.cnt);
+  scan_u32(ds.cnt, (size_t)n, ds.W, ds.bsum32, ds.total32, st);
+  hipLaunchKernelGGL(k_densify_decide, g, b, 0, st, ds.want, ds.W, n, max_gaussians, ds.packed);
+  scan_u64(ds.packed, (size_t)n, ds.prefix, ds.bsum64, ds.total64, st);
+  hipLaunchKernelGGL(k_densify_scatter, g, b, 0, st, P, P_out, cap, n, ds.packed, ds.prefix, ds.total64, gaccum,
+                     gdenom, n_out);
+}
+
+__global__ void k_zero_rows(float* M, size_t cap, size_t r0, size_t r1) {
+  size_t len = r1 - r0;
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= len * P_N) return;
+  size_t j = idx / len, r = r0 + idx % len;
+  M[j * cap + r] = 0.0f;
+}
+void launch_zero_rows(float* M, size_t cap, size_t r0, size_t r1, hipStream_t st) {
+  if (r1 <= r0) return;
+  size_t tot = (r1 - r0) * P_N;
+  hipLaunchKernelGGL(k_zero_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, M, cap, r0, r1);
+}
+
+__global__ void k_aos_to_soa(const float* __restrict__ aos, size_t n, float* __restrict__ P, size_t cap) {
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * P_N) return;
+  size_t j = idx / n, i = idx % n;
+  P[j * cap + i] = aos[i * 59 + aos_offset((int)j)];
+}
+__global__ void k_soa_to_aos(const float* __restrict__ P, size_t cap, size_t n, float* __restrict__ aos) {
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * P_N) return;
+  size_t i = idx / P_N, o = idx % P_N;
+  // o is the Gaussian3D offset; find its flat index
+  int j = o < 3 ? (int)o : (o == 3 ? 10 : (o < 11 ? (int)o - 1 : (int)o));
+  aos[i * 59 + o] = P[(size_t)j * cap + i];
+}
+void launch_aos_to_soa(const float* aos, size_t n, float* P, size_t cap, hipStream_t st) {
+  if (!n) return;
+  size_t tot = n * P_N;
+  hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, aos, n, P, cap);
+}
+void launch_soa_to_aos(const float* P, size_t cap, size_t n, float* aos, hipStream_t st) {
+  if (!n) return;
+  size_t tot = n * P_N;
+  hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, P, cap, n, aos);
+}
+
+}  // namespace rgs

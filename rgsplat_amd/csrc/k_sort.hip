@@ -1,0 +1,415 @@
+// k_sort.hip -- depth sort, tile-key generation, tile sort and tile ranges.
+//
+// The reference sorts splats by depth (backward.rs:191) and pushes each into
+// the lists of the tiles its rect covers, in depth order (:193-205), so every
+// tile list is in global depth order. Here that is two stable sorts:
+//   1. (depth bits, v*cap + i) over all views at once -> depth order, ties by
+//      Gaussian index (stability over the index-ordered input);
+//   2. after writing the (global tile id, splat) pairs in that order, a
+//      stable sort on the tile id groups the pairs by tile without
+//      disturbing depth order inside a tile.
+// Both are the same LSD radix sort with 8-bit digits: per 4096-key tile a
+// histogram kernel, one scan block per digit, and a scatter kernel that ranks
+// keys stably with wave64 ballot matching, stages the sorted tile in LDS and
+// writes each digit's run contiguously. Counts come from device memory so the
+// whole sequence can be captured in a hipGraph.
+#include "rgs_kernels.h"
+
+namespace rgs {
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = RS_TILE / RS_THREADS;  // 16
+constexpr int RS_WAVES = RS_THREADS / 64;
+constexpr int RS_WAVE_ITEMS = RS_TILE / RS_WAVES;  // 1024
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Exclusive scan of one value per thread over a 256-thread block.
+__device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* s_tmp /*>= 4*/, uint32_t* total) {
+  uint32_t lane = lane_id();
+  uint32_t w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint32_t y = __shfl_up(x, d, 64);
+    if (lane >= (uint32_t)d) x += y;
+  }
+  if (lane == 63) s_tmp[w] = x;
+  __syncthreads();
+  uint32_t base = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    uint32_t t = s_tmp[k];
+    if ((uint32_t)k < w) base += t;
+    tot += t;
+  }
+  __syncthreads();
+  if (total) *total = tot;
+  return base + x - v;
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t* __restrict__ keys,
+                                                         const uint32_t* __restrict__ n_dev, int shift,
+                                                         uint32_t* __restrict__ hist, int nb_max) {
+  __shared__ uint32_t h[256];
+  uint32_t n = *n_dev;
+  int b = blockIdx.x;
+  uint32_t nb = (n + RS_TILE - 1) / RS_TILE;
+  if ((uint32_t)b >= nb) return;
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  size_t base = (size_t)b * RS_TILE;
+#pragma unroll 4
+  for (int k = 0; k < RS_ITEMS; ++k) {
+    size_t e = base + (size_t)k * RS_THREADS + threadIdx.x;
+    if (e < n) atomicAdd(&h[(keys[e] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(size_t)threadIdx.x * nb_max + b] = h[threadIdx.x];
+}
+
+// One block per digit: in-place exclusive scan of hist[d][0..nb), row total.
+__global__ __launch_bounds__(1024) void k_radix_rowscan(uint32_t* __restrict__ hist,
+                                                      const uint32_t* __restrict__ n_dev, int nb_max,
+                                                      uint32_t* __restrict__ row_total) {
+  __shared__ uint32_t s_w[16];
+  uint32_t n = *n_dev;
+  uint32_t nb = (n + RS_TILE - 1) / RS_TILE;
+  uint32_t* row = hist + (size_t)blockIdx.x * nb_max;
+  uint32_t per = (nb + 1023) / 1024;
+  uint32_t beg = threadIdx.x * per;
+  uint32_t end = min(beg + per, nb);
+  uint32_t sum = 0;
+  for (uint32_t j = beg; j < end; ++j) sum += row[j];
+  // block exclusive scan over 1024 threads
+  uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+  uint32_t x = sum;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint32_t y = __shfl_up(x, d, 64);
+    if (lane >= (uint32_t)d) x += y;
+  }
+  if (lane == 63) s_w[w] = x;
+  __syncthreads();
+  uint32_t base = 0, tot = 0;
+  for (int k = 0; k < 16; ++k) {
+    uint32_t t = s_w[k];
+    if ((uint32_t)k < w) base += t;
+    tot += t;
+  }
+  uint32_t run = base + x - sum;
+  for (uint32_t j = beg; j < end; ++j) {
+    uint32_t c = row[j];
+    row[j] = run;
+    run += c;
+  }
+  if (threadIdx.x == 0) row_total[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
+    const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ keys_out,
+    uint32_t* __restrict__ vals_out, const uint32_t* __restrict__ n_dev, int shift,
+    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ row_total, int nb_max) {
+  __shared__ uint32_t s_cnt[RS_WAVES][256];
+  __shared__ uint32_t s_blk_start[256];
+  __shared__ uint32_t s_glob[256];
+  __shared__ uint32_t s_keys[RS_TILE];
+  __shared__ uint32_t s_vals[RS_TILE];
+  __shared__ uint32_t s_tmp[8];
+
+  uint32_t n = *n_dev;
+  uint32_t nb = (n + RS_TILE - 1) / RS_TILE;
+  int b = blockIdx.x;
+  if ((uint32_t)b >= nb) return;
+  const uint32_t t = threadIdx.x;
+  const uint32_t lane = lane_id();
+  const uint32_t w = t >> 6;
+  size_t base = (size_t)b * RS_TILE;
+  uint32_t blk_n = (uint32_t)min((size_t)RS_TILE, (size_t)n - base);
+
+#pragma unroll
+  for (int k = 0; k < RS_WAVES; ++k) s_cnt[k][t] = 0;
+  // global base of digit t: exclusive scan of row totals + this block's prefix
+  uint32_t dig_off = block_excl_scan_256(row_total[t], s_tmp, nullptr);
+  s_glob[t] = dig_off + hist[(size_t)t * nb_max + b];
+  __syncthreads();
+
+  uint32_t key[RS_ITEMS], val[RS_ITEMS], rank[RS_ITEMS];
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  size_t wbase = base + (size_t)w * RS_WAVE_ITEMS;
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    size_t e = wbase + (size_t)r * 64 + lane;
+    bool valid = e < n;
+    key[r] = valid ? keys_in[e] : 0xffffffffu;
+    val[r] = valid ? vals_in[e] : 0u;
+    uint32_t d = (key[r] >> shift) & 255u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 8; ++bit) {
+      uint64_t bb = __ballot((d >> bit) & 1u);
+      peers &= ((d >> bit) & 1u) ? bb : ~bb;
+    }
+    uint32_t lrank = (uint32_t)__popcll(peers & lt_mask);
+    uint32_t before = 0;
+    if (valid) before = s_cnt[w][d];
+    __builtin_amdgcn_wave_barrier();
+    if (valid && lrank == 0) s_cnt[w][d] = before + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    rank[r] = before + lrank;
+  }
+  __syncthreads();
+  // per-digit: offsets of each wave's run inside the block, and block start
+  uint32_t c0 = s_cnt[0][t], c1 = s_cnt[1][t], c2 = s_cnt[2][t], c3 = s_cnt[3][t];
+  uint32_t tot = c0 + c1 + c2 + c3;
+  uint32_t start = block_excl_scan_256(tot, s_tmp, nullptr);
+  s_blk_start[t] = start;
+  __syncthreads();
+  s_cnt[0][t] = start;
+  s_cnt[1][t] = start + c0;
+  s_cnt[2][t] = start + c0 + c1;
+  s_cnt[3][t] = start + c0 + c1 + c2;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    size_t e = wbase + (size_t)r * 64 + lane;
+    if (e < n) {
+      uint32_t d = (key[r] >> shift) & 255u;
+      uint32_t p = s_cnt[w][d] + rank[r];
+      s_keys[p] = key[r];
+      s_vals[p] = val[r];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    uint32_t p = (uint32_t)r * RS_THREADS + t;
+    if (p < blk_n) {
+      uint32_t k = s_keys[p];
+      uint32_t d = (k >> shift) & 255u;
+      uint32_t o = s_glob[d] + (p - s_blk_start[d]);
+      keys_out[o] = k;
+      vals_out[o] = s_vals[p];
+    }
+  }
+}
+
+int radix_sort_pairs(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev, size_t n_max, int bits,
+                     RadixScratch rs, hipStream_t st) {
+  int cur = 0;
+  int nb = radix_nb(n_max);
+  if (nb == 0) return 0;
+  for (int shift = 0; shift < bits; shift += 8) {
+    hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(RS_THREADS), 0, st, keys[cur], n_dev, shift, rs.hist,
+                       rs.nb_max);
+    hipLaunchKernelGGL(k_radix_rowscan, dim3(256), dim3(1024), 0, st, rs.hist, n_dev, rs.nb_max, rs.row_total);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(RS_THREADS), 0, st, keys[cur], vals[cur], keys[cur ^ 1],
+                       vals[cur ^ 1], n_dev, shift, rs.hist, rs.row_total, rs.nb_max);
+    cur ^= 1;
+  }
+  return cur;
+}
+
+// ------------------------------------------------------------------- scans
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = SCAN_TILE / SCAN_THREADS;  // 8
+
+template <class T>
+__device__ __forceinline__ T block_excl_scan_generic(T v, T* s_tmp, T* total) {
+  uint32_t lane = lane_id();
+  uint32_t w = threadIdx.x >> 6;
+  T x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    T y = __shfl_up(x, d, 64);
+    if (lane >= (uint32_t)d) x += y;
+  }
+  if (lane == 63) s_tmp[w] = x;
+  __syncthreads();
+  T base = 0, tot = 0;
+  const int nw = blockDim.x >> 6;
+  for (int k = 0; k < nw; ++k) {
+    T t = s_tmp[k];
+    if ((uint32_t)k < w) base += t;
+    tot += t;
+  }
+  __syncthreads();
+  if (total) *total = tot;
+  return base + x - v;
+}
+
+struct GatherCnt {
+  const uint32_t* vals;
+  const uint32_t* cnt;
+  __device__ uint32_t operator()(size_t r) const { return cnt[vals[r]]; }
+};
+template <class T>
+struct Plain {
+  const T* in;
+  __device__ T operator()(size_t r) const { return in[r]; }
+};
+
+template <class T, class F>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(F in, size_t n, T* block_sums) {
+  __shared__ T s_tmp[8];
+  size_t base = (size_t)blockIdx.x * SCAN_TILE;
+  T sum = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    size_t e = base + (size_t)k * SCAN_THREADS + threadIdx.x;
+    if (e < n) sum += in(e);
+  }
+  T tot;
+  block_excl_scan_generic<T>(sum, s_tmp, &tot);
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
+}
+
+template <class T>
+__global__ __launch_bounds__(1024) void k_scan_top(T* block_sums, int nb, T* total) {
+  __shared__ T s_tmp[16];
+  int per = (nb + 1023) / 1024;
+  int beg = threadIdx.x * per;
+  int end = min(beg + per, nb);
+  T sum = 0;
+  for (int j = beg; j < end; ++j) sum += block_sums[j];
+  T tot;
+  T run = block_excl_scan_generic<T>(sum, s_tmp, &tot);
+  for (int j = beg; j < end; ++j) {
+    T c = block_sums[j];
+    block_sums[j] = run;
+    run += c;
+  }
+  if (threadIdx.x == 0) *total = tot;
+}
+
+template <class T, class F>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(F in, size_t n, const T* block_sums, T* out) {
+  __shared__ T s_tmp[8];
+  size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+  T v[SCAN_ITEMS];
+  T sum = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    size_t e = base + k;
+    v[k] = e < n ? in(e) : (T)0;
+    sum += v[k];
+  }
+  T run = block_excl_scan_generic<T>(sum, s_tmp, nullptr) + block_sums[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    size_t e = base + k;
+    if (e < n) out[e] = run;
+    run += v[k];
+  }
+}
+
+template <class T, class F>
+static void scan_impl(F in, size_t n, T* out, T* block_sums, T* total, hipStream_t st) {
+  int nb = scan_nb(n);
+  if (nb == 0) {
+    (void)hipMemsetAsync(total, 0, sizeof(T), st);
+    return;
+  }
+  hipLaunchKernelGGL((k_scan_reduce<T, F>), dim3(nb), dim3(SCAN_THREADS), 0, st, in, n, block_sums);
+  hipLaunchKernelGGL((k_scan_top<T>), dim3(1), dim3(1024), 0, st, block_sums, nb, total);
+  hipLaunchKernelGGL((k_scan_down<T, F>), dim3(nb), dim3(SCAN_THREADS), 0, st, in, n, block_sums, out);
+}
+
+void scan_gathered_counts(const uint32_t* vals, const uint32_t* cnt, size_t n, uint32_t* out,
+                          uint32_t* block_sums, uint32_t* total, hipStream_t st) {
+  scan_impl<uint32_t>(GatherCnt{vals, cnt}, n, out, block_sums, total, st);
+}
+void scan_u32(const uint32_t* in, size_t n, uint32_t* out, uint32_t* block_sums, uint32_t* total, hipStream_t st) {
+  scan_impl<uint32_t>(Plain<uint32_t>{in}, n, out, block_sums, total, st);
+}
+void scan_u64(const uint64_t* in, size_t n, uint64_t* out, uint64_t* block_sums, uint64_t* total, hipStream_t st) {
+  scan_impl<uint64_t>(Plain<uint64_t>{in}, n, out, block_sums, total, st);
+}
+
+// ------------------------------------------------------------ pairs/ranges
+__global__ __launch_bounds__(256) void k_gen_pairs(const uint32_t* __restrict__ sorted_vals, size_t n_sorted,
+                                                   const uint32_t* __restrict__ offsets,
+                                                   const uint32_t* __restrict__ total, const uint32_t* __restrict__ cnt,
+                                                   const uint2* __restrict__ rect, int cap,
+                                                   const CamDev* __restrict__ cams, uint32_t* __restrict__ pair_keys,
+                                                   uint32_t* __restrict__ pair_vals, size_t k_cap,
+                                                   uint32_t* __restrict__ k_out, uint32_t* __restrict__ status) {
+  size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r == 0) {
+    uint32_t tot = *total;
+    if ((size_t)tot > k_cap) {
+      atomicOr(status, ST_PAIR_OVERFLOW);
+      *k_out = (uint32_t)k_cap;
+    } else {
+      *k_out = tot;
+    }
+  }
+  if (r >= n_sorted) return;
+  uint32_t vi = sorted_vals[r];
+  uint32_t c = cnt[vi];
+  if (c == 0) return;
+  size_t off = offsets[r];
+  uint32_t v = vi / (uint32_t)cap;
+  const CamDev& cm = cams[v];
+  uint2 rc = rect[vi];
+  int tx0 = rc.x & 0xffff, ty0 = rc.x >> 16, tx1 = rc.y & 0xffff, ty1 = rc.y >> 16;
+  uint32_t tb = (uint32_t)cm.tile_base;
+  for (int ty = ty0; ty < ty1; ++ty)
+    for (int tx = tx0; tx < tx1; ++tx) {
+      if (off < k_cap) {
+        pair_keys[off] = tb + (uint32_t)(ty * cm.ntx + tx);
+        pair_vals[off] = vi;
+      }
+      ++off;
+    }
+}
+
+void launch_gen_pairs(const uint32_t* sorted_vals, size_t n_sorted, const uint32_t* offsets, const uint32_t* total,
+                      const SplatArrays S, int cap, const CamDev* cams, uint32_t* pair_keys, uint32_t* pair_vals,
+                      size_t k_cap, uint32_t* k_out, uint32_t* status, hipStream_t st) {
+  size_t nb = (n_sorted + 255) / 256;
+  if (nb == 0) nb = 1;
+  hipLaunchKernelGGL(k_gen_pairs, dim3((unsigned)nb), dim3(256), 0, st, sorted_vals, n_sorted, offsets, total, S.cnt,
+                     S.rect, cap, cams, pair_keys, pair_vals, k_cap, k_out, status);
+}
+
+__global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ k_dev,
+                                                size_t k_cap, uint32_t* __restrict__ rstart,
+                                                uint32_t* __restrict__ rend) {
+  size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t K = *k_dev;
+  if (p >= K) return;
+  uint32_t t = keys[p];
+  if (p == 0 || keys[p - 1] != t) rstart[t] = (uint32_t)p;
+  if (p + 1 == K || keys[p + 1] != t) rend[t] = (uint32_t)(p + 1);
+}
+
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_f32(float* p, size_t n, float v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+}
+void launch_fill_f32(float* p, size_t n, float v, hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+}
+
+void launch_ranges(const uint32_t* pair_keys, const uint32_t* k_dev, size_t k_cap, int total_tiles,
+                   uint32_t* rstart, uint32_t* rend, hipStream_t st) {
+  launch_fill_u32(rstart, (size_t)total_tiles, 0u, st);
+  launch_fill_u32(rend, (size_t)total_tiles, 0u, st);
+  size_t nb = (k_cap + 255) / 256;
+  if (nb == 0) return;
+  hipLaunchKernelGGL(k_ranges, dim3((unsigned)nb), dim3(256), 0, st, pair_keys, k_dev, k_cap, rstart, rend);
+}
+
+}  // namespace rgs

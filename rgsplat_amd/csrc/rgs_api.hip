@@ -1,0 +1,1107 @@
+// rgs_api.hip -- the C ABI (include/rgsplat_hip.h): contexts, device buffers,
+// the reference-shaped entry points and the device-resident batched training
+// step with hipGraph replay.
+//
+// The step is one stream of kernels whose sizes come from device memory:
+// project -> depth sort -> pair-count scan -> pair generation -> tile sort ->
+// tile ranges -> forward raster -> L1+D-SSIM -> backward raster -> fused
+// geometry backprop + Adam. The only host synchronisation is at the end of a
+// step (loss and status readback); the first step of a configuration runs
+// eagerly to size the pair buffers, later steps replay a captured graph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rgsplat_hip.h"
+#include "rgs_kernels.h"
+
+using namespace rgs;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                              \
+  do {                                                                                         \
+    hipError_t e_ = (x);                                                                       \
+    if (e_ != hipSuccess) return fail(RGS_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+#define CHK(x)            \
+  do {                    \
+    int r_ = (x);         \
+    if (r_) return r_;    \
+  } while (0)
+
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+struct rgs_ctx;
+namespace {
+uint64_t* ctx_gen(rgs_ctx* c);
+
+int ensure(rgs_ctx* c, DBuf& b, size_t bytes) {
+  if (bytes <= b.bytes && b.p) return 0;
+  ++*ctx_gen(c);  // a captured graph holding the old pointer is now stale
+  if (b.p) {
+    (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  size_t want = bytes ? bytes : 256;
+  hipError_t e = hipMalloc(&b.p, want);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    return fail(RGS_E_HIP, std::string("hipMalloc(") + std::to_string(want) + "): " + hipGetErrorString(e));
+  }
+  b.bytes = want;
+  return 0;
+}
+template <class T>
+T* P_(DBuf& b) {
+  return reinterpret_cast<T*>(b.p);
+}
+
+// Matrix4::try_inverse of the row-major c2w (math/mod.rs:155-160), evaluated
+// as the cofactor expansion on column-major storage; false when det == 0.
+bool view_from_c2w(const float* c2w, float view[16]) {
+  float m[16];
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) m[c * 4 + r] = c2w[r * 4 + c];
+  float inv[16];
+  inv[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] + m[9] * m[7] * m[14] +
+           m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
+  inv[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] + m[8] * m[6] * m[15] - m[8] * m[7] * m[14] -
+           m[12] * m[6] * m[11] + m[12] * m[7] * m[10];
+  inv[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] + m[8] * m[7] * m[13] +
+           m[12] * m[5] * m[11] - m[12] * m[7] * m[9];
+  inv[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] + m[8] * m[5] * m[14] - m[8] * m[6] * m[13] -
+            m[12] * m[5] * m[10] + m[12] * m[6] * m[9];
+  float det = m[0] * inv[0] + m[1] * inv[4] + m[2] * inv[8] + m[3] * inv[12];
+  if (det == 0.0f) return false;
+  inv[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] + m[9] * m[2] * m[15] - m[9] * m[3] * m[14] -
+           m[13] * m[2] * m[11] + m[13] * m[3] * m[10];
+  inv[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] + m[5] * m[3] * m[14] +
+           m[13] * m[2] * m[7] - m[13] * m[3] * m[6];
+  inv[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] - m[5] * m[3] * m[10] -
+           m[9] * m[2] * m[7] + m[9] * m[3] * m[6];
+  inv[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] + m[8] * m[3] * m[14] +
+           m[12] * m[2] * m[11] - m[12] * m[3] * m[10];
+  inv[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] - m[4] * m[3] * m[14] -
+           m[12] * m[2] * m[7] + m[12] * m[3] * m[6];
+  inv[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] + m[4] * m[3] * m[10] +
+           m[8] * m[2] * m[7] - m[8] * m[3] * m[6];
+  inv[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] - m[8] * m[3] * m[13] -
+           m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
+  inv[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] + m[4] * m[3] * m[13] +
+            m[12] * m[1] * m[7] - m[12] * m[3] * m[5];
+  inv[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] - m[4] * m[3] * m[9] -
+            m[8] * m[1] * m[7] + m[8] * m[3] * m[5];
+  inv[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] + m[8] * m[2] * m[13] +
+            m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
+  inv[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] - m[4] * m[2] * m[13] -
+            m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
+  inv[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] + m[4] * m[2] * m[9] +
+            m[8] * m[1] * m[6] - m[8] * m[2] * m[5];
+  float inv_det = 1.0f / det;
+  for (int i = 0; i < 16; ++i) inv[i] = inv[i] * inv_det;
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) view[r * 4 + c] = inv[c * 4 + r];
+  return true;
+}
+
+int make_camdev(const rgs_camera& cam, CamDev& out) {
+  float view[16];
+  if (!cam.width || !cam.height) return fail(RGS_E_ARG, "camera with zero width or height");
+  if (!view_from_c2w(cam.c2w, view)) return fail(RGS_E_CAMERA, "c2w must be invertible");
+  for (int i = 0; i < 12; ++i) out.view[i] = view[i];
+  out.fx = cam.fx;
+  out.fy = cam.fy;
+  out.cx = cam.cx;
+  out.cy = cam.cy;
+  out.cam_pos[0] = cam.c2w[3];
+  out.cam_pos[1] = cam.c2w[7];
+  out.cam_pos[2] = cam.c2w[11];
+  out.w = (int32_t)cam.width;
+  out.h = (int32_t)cam.height;
+  out.ntx = (out.w + TILE - 1) / TILE;
+  out.nty = (out.h + TILE - 1) / TILE;
+  out.tile_base = 0;
+  out.pix_base = 0;
+  out.tgt_base = 0;
+  return 0;
+}
+
+// A set of views rendered by one launch sequence.
+struct ViewSet {
+  std::vector<CamDev> cams;
+  int total_tiles = 0;
+  size_t total_pix = 0;
+  void layout() {
+    int tb = 0;
+    size_t pb = 0;
+    for (auto& c : cams) {
+      c.tile_base = tb;
+      c.pix_base = (int32_t)pb;
+      tb += c.ntx * c.nty;
+      pb += (size_t)c.w * c.h;
+    }
+    total_tiles = tb;
+    total_pix = pb;
+  }
+  int bits() const {
+    int b = 1;
+    while ((1ll << b) < (long long)total_tiles) ++b;
+    return b;
+  }
+};
+
+enum { SC_DEPTH_N = 0, SC_TOTAL = 1, SC_K = 2, SC_STATUS = 3, SC_NOUT = 4, SC_COUNT = 8 };
+
+}  // namespace
+
+struct rgs_ctx {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t st = nullptr;
+
+  // render workspace (shared by every entry point)
+  DBuf cams_d, xy_ab, c_op_rg, col_b, cnt, rect;
+  DBuf dk[2], dv[2], offsets, bsums, scalars;
+  DBuf pk[2], pv[2], hist, rowtot;
+  DBuf rstart, rend, rgb, final_t, last, dl, maps, partial, loss;
+  DBuf accum;
+  size_t k_cap = 0;
+  size_t depth_cap = 0;  // capacity (entries) of the depth-sort arrays
+  int hist_nb = 0;
+  const uint32_t* sorted_vals = nullptr;
+  const uint32_t* pair_vals = nullptr;
+  uint32_t* h_scalars = nullptr;  // pinned
+  float* h_loss = nullptr;        // pinned, 64 entries
+
+  // reference-API staging
+  DBuf api_aos, api_P, api_grads, api_target, api_m1, api_m2, api_acc, api_den, api_P2;
+  DBuf dens_want, dens_cnt, dens_W, dens_packed, dens_prefix, dens_b32, dens_b64, dens_t32, dens_t64;
+
+  // forward state kept for rgs_backward_dev
+  ViewSet fwd_views;
+  int fwd_n = 0;
+  float fwd_bg[3] = {0, 0, 0};
+  bool fwd_valid = false;
+
+  // device-resident scene
+  DBuf P, P2, M1, M2, gacc, gden;
+  size_t n = 0, cap = 0, m_rows = 0;
+  uint64_t t = 0;
+  std::vector<CamDev> reg_cams;
+  DBuf targets;
+  size_t target_pix = 0;
+  DBuf adam_d;
+  AdamConsts* h_adam = nullptr;  // pinned
+  CamDev* h_cams = nullptr;      // pinned, 64 entries
+  uint64_t last_pairs = 0;
+  size_t last_cap = 1;      // row capacity of the last render (vi = v * cap + i)
+  int last_tiles = 0;       // total tiles of the last render
+  int last_nviews = 0;      // views of the last render
+  float last_bg[3] = {0, 0, 0};
+
+  // captured step
+  hipGraphExec_t gexec = nullptr;
+  std::string gkey;
+  uint64_t gen = 0;        // bumped on every device (re)allocation
+  uint64_t graph_gen = 0;  // gen at capture time
+  int raster_batch = 0;
+  DBuf api_acc9;           // raster accumulators of the reference-API paths
+
+  // stage timing
+  bool profiling = false;
+  hipEvent_t ev_b[RGS_N_STAGES] = {}, ev_e[RGS_N_STAGES] = {};
+  double stage_ms[RGS_N_STAGES] = {};
+  uint64_t prof_steps = 0;
+};
+
+namespace {
+
+uint64_t* ctx_gen(rgs_ctx* c) { return &c->gen; }
+
+enum Stage { S_PROJECT, S_DSORT, S_SCAN, S_GEN, S_TSORT, S_RANGES, S_FWD, S_LOSS, S_BWD, S_ADAM };
+inline void stage_begin(rgs_ctx* c, int s) {
+  if (c->profiling) (void)hipEventRecord(c->ev_b[s], c->st);
+}
+inline void stage_end(rgs_ctx* c, int s) {
+  if (c->profiling) (void)hipEventRecord(c->ev_e[s], c->st);
+}
+
+void drop_graph(rgs_ctx* c) {
+  if (c->gexec) {
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  c->gkey.clear();
+}
+
+// Sizes every buffer the render sequence touches for n Gaussians (row
+// capacity cap) in the view set vs, pairs capacity k_cap.
+int ensure_render(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, size_t k_cap) {
+  size_t nv = vs.cams.size();
+  size_t vc = nv * cap;
+  size_t dn = nv * (size_t)n;
+  CHK(ensure(c, c->cams_d, nv * sizeof(CamDev)));
+  CHK(ensure(c, c->xy_ab, vc * sizeof(float4)));
+  CHK(ensure(c, c->c_op_rg, vc * sizeof(float4)));
+  CHK(ensure(c, c->col_b, vc * sizeof(float)));
+  CHK(ensure(c, c->cnt, vc * sizeof(uint32_t)));
+  CHK(ensure(c, c->rect, vc * sizeof(uint2)));
+  for (int i = 0; i < 2; ++i) {
+    CHK(ensure(c, c->dk[i], dn * 4));
+    CHK(ensure(c, c->dv[i], dn * 4));
+    CHK(ensure(c, c->pk[i], k_cap * 4));
+    CHK(ensure(c, c->pv[i], k_cap * 4));
+  }
+  c->depth_cap = dn;
+  CHK(ensure(c, c->offsets, dn * 4));
+  CHK(ensure(c, c->bsums, ((size_t)scan_nb(dn) + 1) * 8));
+  CHK(ensure(c, c->scalars, SC_COUNT * 4));
+  int nb = std::max(radix_nb(dn), radix_nb(k_cap));
+  if (nb < 1) nb = 1;
+  CHK(ensure(c, c->hist, (size_t)256 * nb * 4));
+  c->hist_nb = nb;
+  CHK(ensure(c, c->rowtot, 256 * 4));
+  CHK(ensure(c, c->rstart, (size_t)vs.total_tiles * 4));
+  CHK(ensure(c, c->rend, (size_t)vs.total_tiles * 4));
+  CHK(ensure(c, c->rgb, vs.total_pix * 4 * 4));
+  CHK(ensure(c, c->final_t, vs.total_pix * 4));
+  CHK(ensure(c, c->last, vs.total_pix * 4));
+  c->k_cap = k_cap;
+  return 0;
+}
+
+SplatArrays splats(rgs_ctx* c) {
+  return SplatArrays{P_<float4>(c->xy_ab), P_<float4>(c->c_op_rg), P_<float>(c->col_b), P_<uint32_t>(c->cnt),
+                     P_<uint2>(c->rect)};
+}
+
+// Projection, depth sort and the pair-count scan (sizes fixed by n and vs).
+void enqueue_project(rgs_ctx* c, const float* P, size_t cap, int n, const ViewSet& vs) {
+  hipStream_t st = c->st;
+  uint32_t* sc = P_<uint32_t>(c->scalars);
+  int nv = (int)vs.cams.size();
+  size_t dn = (size_t)nv * n;
+  launch_fill_u32(sc + SC_DEPTH_N, 1, (uint32_t)dn, st);
+  stage_begin(c, S_PROJECT);
+  launch_project(P, (int)cap, n, P_<CamDev>(c->cams_d), nv, splats(c), P_<uint32_t>(c->dk[0]),
+                 P_<uint32_t>(c->dv[0]), sc + SC_STATUS, st);
+  stage_end(c, S_PROJECT);
+  uint32_t* keys[2] = {P_<uint32_t>(c->dk[0]), P_<uint32_t>(c->dk[1])};
+  uint32_t* vals[2] = {P_<uint32_t>(c->dv[0]), P_<uint32_t>(c->dv[1])};
+  RadixScratch rs{P_<uint32_t>(c->hist), P_<uint32_t>(c->rowtot), c->hist_nb};
+  stage_begin(c, S_DSORT);
+  int cur = radix_sort_pairs(keys, vals, sc + SC_DEPTH_N, dn, 32, rs, st);
+  stage_end(c, S_DSORT);
+  c->sorted_vals = vals[cur];
+  stage_begin(c, S_SCAN);
+  scan_gathered_counts(vals[cur], P_<uint32_t>(c->cnt), dn, P_<uint32_t>(c->offsets), P_<uint32_t>(c->bsums),
+                       sc + SC_TOTAL, st);
+  stage_end(c, S_SCAN);
+}
+
+// Pair generation, tile sort, ranges and the forward raster.
+void enqueue_bin_and_raster(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, const float bg[3], int mode,
+                            float* rgb_out) {
+  hipStream_t st = c->st;
+  uint32_t* sc = P_<uint32_t>(c->scalars);
+  size_t dn = vs.cams.size() * (size_t)n;
+  stage_begin(c, S_GEN);
+  launch_gen_pairs(c->sorted_vals, dn, P_<uint32_t>(c->offsets), sc + SC_TOTAL, splats(c), (int)cap,
+                   P_<CamDev>(c->cams_d), P_<uint32_t>(c->pk[0]), P_<uint32_t>(c->pv[0]), c->k_cap, sc + SC_K,
+                   sc + SC_STATUS, st);
+  stage_end(c, S_GEN);
+  uint32_t* keys[2] = {P_<uint32_t>(c->pk[0]), P_<uint32_t>(c->pk[1])};
+  uint32_t* vals[2] = {P_<uint32_t>(c->pv[0]), P_<uint32_t>(c->pv[1])};
+  RadixScratch rs{P_<uint32_t>(c->hist), P_<uint32_t>(c->rowtot), c->hist_nb};
+  stage_begin(c, S_TSORT);
+  int cur = radix_sort_pairs(keys, vals, sc + SC_K, c->k_cap, vs.bits(), rs, st);
+  stage_end(c, S_TSORT);
+  stage_begin(c, S_RANGES);
+  launch_ranges(keys[cur], sc + SC_K, c->k_cap, vs.total_tiles, P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend),
+                st);
+  stage_end(c, S_RANGES);
+  c->pair_vals = vals[cur];
+  RasterIn in{P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend), c->pair_vals, splats(c), P_<CamDev>(c->cams_d),
+              (int)vs.cams.size(), vs.total_tiles, {bg[0], bg[1], bg[2]}};
+  stage_begin(c, S_FWD);
+  launch_raster_fwd(in, mode, rgb_out, P_<float>(c->final_t), P_<int32_t>(c->last), st);
+  stage_end(c, S_FWD);
+}
+
+RasterIn raster_in(rgs_ctx* c, const ViewSet& vs, const float bg[3]) {
+  return RasterIn{P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend), c->pair_vals, splats(c), P_<CamDev>(c->cams_d),
+                  (int)vs.cams.size(), vs.total_tiles, {bg[0], bg[1], bg[2]}};
+}
+
+// Eager render of n Gaussians (SoA P, row capacity cap) into vs: sizes the
+// pair buffers from the exact count.
+int render_eager(rgs_ctx* c, const float* P, size_t cap, int n, ViewSet& vs, const float bg[3], int mode,
+                 float* rgb_out) {
+  vs.layout();
+  CHK(ensure_render(c, cap, n, vs, std::max<size_t>(c->k_cap, 1)));
+  HIPCHK(hipMemcpyAsync(c->cams_d.p, vs.cams.data(), vs.cams.size() * sizeof(CamDev), hipMemcpyHostToDevice, c->st));
+  uint32_t* sc = P_<uint32_t>(c->scalars);
+  launch_fill_u32(sc, SC_COUNT, 0u, c->st);
+  enqueue_project(c, P, cap, n, vs);
+  HIPCHK(hipMemcpyAsync(c->h_scalars, sc, SC_COUNT * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (c->h_scalars[SC_STATUS] & ST_NAN_DEPTH) return fail(RGS_E_NAN, "NaN depth in projection");
+  size_t K = c->h_scalars[SC_TOTAL];
+  if (K > c->k_cap) {
+    size_t want = K + K / 4 + 1024;
+    CHK(ensure_render(c, cap, n, vs, want));
+  }
+  enqueue_bin_and_raster(c, cap, n, vs, bg, mode, rgb_out ? rgb_out : P_<float>(c->rgb));
+  c->last_pairs = K;
+  c->last_cap = cap;
+  c->last_tiles = vs.total_tiles;
+  c->last_nviews = (int)vs.cams.size();
+  return 0;
+}
+
+// Uploads n AoS Gaussians and converts them to the SoA scratch scene.
+int upload_api_scene(rgs_ctx* c, const void* g, size_t n, bool device_src) {
+  size_t cap = n ? n : 1;
+  CHK(ensure(c, c->api_aos, cap * 59 * 4));
+  CHK(ensure(c, c->api_P, cap * 59 * 4));
+  if (n) {
+    if (device_src) {
+      HIPCHK(hipMemcpyAsync(c->api_aos.p, g, n * 59 * 4, hipMemcpyDeviceToDevice, c->st));
+    } else {
+      HIPCHK(hipMemcpyAsync(c->api_aos.p, g, n * 59 * 4, hipMemcpyHostToDevice, c->st));
+    }
+    launch_aos_to_soa(P_<float>(c->api_aos), n, P_<float>(c->api_P), cap, c->st);
+  }
+  return 0;
+}
+
+int ensure_loss(rgs_ctx* c, const ViewSet& vs) {
+  CHK(ensure(c, c->dl, vs.total_pix * 3 * 4));
+  CHK(ensure(c, c->maps, vs.total_pix * 15 * 4));
+  CHK(ensure(c, c->partial, (size_t)vs.total_tiles * 6 * 4));
+  CHK(ensure(c, c->loss, 64 * 4));
+  return 0;
+}
+
+void enqueue_loss(rgs_ctx* c, const ViewSet& vs, const float* rgb, const float* target, float lambda) {
+  LossIn li{rgb, target, P_<CamDev>(c->cams_d), (int)vs.cams.size(), vs.total_tiles, vs.total_pix, lambda};
+  launch_loss(li, P_<float>(c->dl), P_<float>(c->maps), P_<float>(c->partial), P_<float>(c->loss), c->st);
+}
+
+AdamConsts adam_consts(const rgs_train_cfg& cfg, uint64_t t) {
+  AdamConsts k;
+  k.lr[0] = cfg.lr_position;
+  k.lr[1] = cfg.lr_rotation;
+  k.lr[2] = cfg.lr_scale;
+  k.lr[3] = cfg.lr_opacity;
+  k.lr[4] = cfg.lr_sh_dc;
+  k.lr_sh_rest = cfg.lr_sh_rest;
+  k.beta1 = cfg.beta1;
+  k.beta2 = cfg.beta2;
+  k.one_minus_beta1 = 1.0f - cfg.beta1;
+  k.one_minus_beta2 = 1.0f - cfg.beta2;
+  // f32::powi -> compiler-rt __powisf2 (repeated squaring)
+  auto powi = [](float a, int b) {
+    const int recip = b < 0;
+    float r = 1;
+    while (1) {
+      if (b & 1) r *= a;
+      b /= 2;
+      if (b == 0) break;
+      a *= a;
+    }
+    return recip ? 1 / r : r;
+  };
+  k.bc1 = 1.0f - powi(cfg.beta1, (int)t);
+  k.bc2 = 1.0f - powi(cfg.beta2, (int)t);
+  k.eps = cfg.eps;
+  return k;
+}
+
+}  // namespace
+
+// ============================================================== C ABI
+extern "C" {
+
+const char* rgs_last_error(void) { return g_err.c_str(); }
+
+int rgs_create(int device, rgs_ctx** out) {
+  if (!out) return fail(RGS_E_ARG, "null out");
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) return fail(RGS_E_HIP, "no HIP device available");
+  if (device >= ndev) return fail(RGS_E_ARG, "device id out of range");
+  if (device < 0) {
+    HIPCHK(hipGetDevice(&device));
+  }
+  HIPCHK(hipSetDevice(device));
+  auto* c = new rgs_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(RGS_E_HIP, "hipStreamCreate failed");
+  }
+  c->st = c->own;
+  if (hipHostMalloc((void**)&c->h_scalars, SC_COUNT * 4, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_loss, 64 * 4, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_adam, sizeof(AdamConsts), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_cams, 64 * sizeof(CamDev), hipHostMallocDefault) != hipSuccess) {
+    rgs_destroy(c);
+    return fail(RGS_E_HIP, "hipHostMalloc failed");
+  }
+  *out = c;
+  return 0;
+}
+
+void rgs_destroy(rgs_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  drop_graph(c);
+  DBuf* all[] = {&c->cams_d, &c->xy_ab, &c->c_op_rg, &c->col_b, &c->cnt, &c->rect, &c->dk[0], &c->dk[1],
+                 &c->dv[0], &c->dv[1], &c->offsets, &c->bsums, &c->scalars, &c->pk[0], &c->pk[1], &c->pv[0],
+                 &c->pv[1], &c->hist, &c->rowtot, &c->rstart, &c->rend, &c->rgb, &c->final_t, &c->last, &c->dl,
+                 &c->maps, &c->partial, &c->loss, &c->accum, &c->api_aos, &c->api_P, &c->api_grads,
+                 &c->api_target, &c->api_m1, &c->api_m2, &c->api_acc, &c->api_den, &c->api_P2, &c->dens_want,
+                 &c->dens_cnt, &c->dens_W, &c->dens_packed, &c->dens_prefix, &c->dens_b32, &c->dens_b64,
+                 &c->dens_t32, &c->dens_t64, &c->P, &c->P2, &c->M1, &c->M2, &c->gacc, &c->gden, &c->targets,
+                 &c->adam_d, &c->api_acc9};
+  for (DBuf* b : all)
+    if (b->p) (void)hipFree(b->p);
+  if (c->h_scalars) (void)hipHostFree(c->h_scalars);
+  if (c->h_loss) (void)hipHostFree(c->h_loss);
+  if (c->h_adam) (void)hipHostFree(c->h_adam);
+  if (c->h_cams) (void)hipHostFree(c->h_cams);
+  for (int s = 0; s < RGS_N_STAGES; ++s) {
+    if (c->ev_b[s]) (void)hipEventDestroy(c->ev_b[s]);
+    if (c->ev_e[s]) (void)hipEventDestroy(c->ev_e[s]);
+  }
+  if (c->own) (void)hipStreamDestroy(c->own);
+  delete c;
+}
+
+int rgs_set_stream(rgs_ctx* c, void* s) {
+  if (!c) return fail(RGS_E_ARG, "null ctx");
+  drop_graph(c);
+  c->st = s ? (hipStream_t)s : c->own;
+  return 0;
+}
+
+int rgs_set_raster_batch(rgs_ctx* c, int batch) {
+  if (!c) return fail(RGS_E_ARG, "null ctx");
+  if (batch != 0 && batch != 256) return fail(RGS_E_ARG, "raster batch must be 0 (auto) or 256");
+  c->raster_batch = batch;
+  return 0;
+}
+
+static int check_common(rgs_ctx* c, const void* g, size_t n, const rgs_camera* cam) {
+  if (!c) return fail(RGS_E_ARG, "null ctx");
+  if (!cam) return fail(RGS_E_ARG, "null camera");
+  if (n && !g) return fail(RGS_E_ARG, "null gaussians");
+  if (n >= (1u << 31)) return fail(RGS_E_ARG, "too many gaussians");
+  (void)hipSetDevice(c->device);
+  c->fwd_valid = false;  // the shared render workspace is about to change
+  return 0;
+}
+
+int rgs_render_color(rgs_ctx* c, const rgs_gaussian* g, size_t n, const rgs_camera* cam, const float bg[3],
+                     float* rgb_out) {
+  CHK(check_common(c, g, n, cam));
+  if (!rgb_out || !bg) return fail(RGS_E_ARG, "null output or bg");
+  ViewSet vs;
+  vs.cams.resize(1);
+  CHK(make_camdev(*cam, vs.cams[0]));
+  CHK(upload_api_scene(c, g, n, false));
+  CHK(render_eager(c, P_<float>(c->api_P), n ? n : 1, (int)n, vs, bg, 0, nullptr));
+  HIPCHK(hipMemcpyAsync(rgb_out, c->rgb.p, vs.total_pix * 3 * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int rgs_rasterize_rgba(rgs_ctx* c, const rgs_gaussian* g, size_t n, const rgs_camera* cam, const float bg[3],
+                       float* rgba_out) {
+  CHK(check_common(c, g, n, cam));
+  if (!rgba_out || !bg) return fail(RGS_E_ARG, "null output or bg");
+  ViewSet vs;
+  vs.cams.resize(1);
+  CHK(make_camdev(*cam, vs.cams[0]));
+  vs.layout();
+  CHK(ensure_render(c, n ? n : 1, (int)n, vs, std::max<size_t>(c->k_cap, 1)));
+  launch_fill_f32(P_<float>(c->rgb), vs.total_pix * 4, 0.0f, c->st);
+  CHK(upload_api_scene(c, g, n, false));
+  CHK(render_eager(c, P_<float>(c->api_P), n ? n : 1, (int)n, vs, bg, 1, nullptr));
+  HIPCHK(hipMemcpyAsync(rgba_out, c->rgb.p, vs.total_pix * 4 * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int rgs_render_and_backward(rgs_ctx* c, const rgs_gaussian* g, size_t n, const rgs_camera* cam,
+                            const float* target_rgba, const float bg[3], float lambda, float* loss_out,
+                            rgs_grad* grads_out) {
+  CHK(check_common(c, g, n, cam));
+  if (!target_rgba || !bg || !grads_out) return fail(RGS_E_ARG, "null target, bg or grads");
+  ViewSet vs;
+  vs.cams.resize(1);
+  CHK(make_camdev(*cam, vs.cams[0]));
+  size_t npix = (size_t)cam->width * cam->height;
+  CHK(ensure(c, c->api_target, npix * 16));
+  HIPCHK(hipMemcpyAsync(c->api_target.p, target_rgba, npix * 16, hipMemcpyHostToDevice, c->st));
+  CHK(upload_api_scene(c, g, n, false));
+  size_t cap = n ? n : 1;
+  CHK(render_eager(c, P_<float>(c->api_P), cap, (int)n, vs, bg, 0, nullptr));
+  CHK(ensure_loss(c, vs));
+  enqueue_loss(c, vs, P_<float>(c->rgb), P_<float>(c->api_target), lambda);
+  CHK(ensure(c, c->api_acc9, 9 * cap * 4));
+  launch_fill_f32(P_<float>(c->api_acc9), 9 * cap, 0.0f, c->st);
+  launch_raster_bwd(raster_in(c, vs, bg), P_<float>(c->final_t), P_<int32_t>(c->last), P_<float>(c->dl),
+                    P_<float>(c->api_acc9), cap, c->st);
+  CHK(ensure(c, c->api_grads, cap * 60 * 4));
+  launch_geometry_aos(P_<float>(c->api_P), (int)cap, (int)n, P_<CamDev>(c->cams_d), 0, P_<uint32_t>(c->cnt),
+                      P_<float>(c->api_acc9), cap, P_<float>(c->api_grads), c->st);
+  if (n) HIPCHK(hipMemcpyAsync(grads_out, c->api_grads.p, n * 60 * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(c->h_loss, c->loss.p, 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (loss_out) *loss_out = c->h_loss[0];
+  return 0;
+}
+
+int rgs_loss_grad(rgs_ctx* c, const float* rgb, const float* target_rgba, uint32_t w, uint32_t h, float lambda,
+                  float* loss_out, float* dl_out) {
+  if (!c || !rgb || !target_rgba || !dl_out) return fail(RGS_E_ARG, "null argument");
+  if (!w || !h) return fail(RGS_E_ARG, "empty image");
+  (void)hipSetDevice(c->device);
+  ViewSet vs;
+  vs.cams.resize(1);
+  CamDev& cd = vs.cams[0];
+  std::memset(&cd, 0, sizeof(cd));
+  cd.w = (int32_t)w;
+  cd.h = (int32_t)h;
+  cd.ntx = (cd.w + TILE - 1) / TILE;
+  cd.nty = (cd.h + TILE - 1) / TILE;
+  vs.layout();
+  size_t npix = (size_t)w * h;
+  CHK(ensure(c, c->cams_d, sizeof(CamDev)));
+  CHK(ensure(c, c->rgb, npix * 16));
+  CHK(ensure(c, c->api_target, npix * 16));
+  CHK(ensure_loss(c, vs));
+  HIPCHK(hipMemcpyAsync(c->cams_d.p, vs.cams.data(), sizeof(CamDev), hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->rgb.p, rgb, npix * 12, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->api_target.p, target_rgba, npix * 16, hipMemcpyHostToDevice, c->st));
+  enqueue_loss(c, vs, P_<float>(c->rgb), P_<float>(c->api_target), lambda);
+  HIPCHK(hipMemcpyAsync(dl_out, c->dl.p, npix * 12, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(c->h_loss, c->loss.p, 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (loss_out) *loss_out = c->h_loss[0];
+  return 0;
+}
+
+int rgs_adam_step(rgs_ctx* c, rgs_gaussian* g, const rgs_grad* grads, size_t n, rgs_adam* st_,
+                  const rgs_train_cfg* cfg) {
+  if (!c || !st_ || !cfg || (n && (!g || !grads))) return fail(RGS_E_ARG, "null argument");
+  if (n > st_->rows) return fail(RGS_E_ARG, "AdamState has fewer rows than Gaussians");
+  (void)hipSetDevice(c->device);
+  st_->t += 1;
+  if (!n) return 0;
+  AdamConsts k = adam_consts(*cfg, st_->t);
+  CHK(ensure(c, c->api_aos, n * 59 * 4));
+  CHK(ensure(c, c->api_grads, n * 60 * 4));
+  CHK(ensure(c, c->api_m1, n * 59 * 4));
+  CHK(ensure(c, c->api_m2, n * 59 * 4));
+  CHK(ensure(c, c->api_acc, n * 4));
+  CHK(ensure(c, c->api_den, n * 4));
+  HIPCHK(hipMemcpyAsync(c->api_aos.p, g, n * 59 * 4, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->api_grads.p, grads, n * 60 * 4, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->api_m1.p, st_->m1, n * 59 * 4, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->api_m2.p, st_->m2, n * 59 * 4, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->api_acc.p, st_->grad_accum, n * 4, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->api_den.p, st_->grad_denom, n * 4, hipMemcpyHostToDevice, c->st));
+  launch_adam_aos(P_<float>(c->api_aos), P_<float>(c->api_grads), n, P_<float>(c->api_m1), P_<float>(c->api_m2),
+                  P_<float>(c->api_acc), P_<uint32_t>(c->api_den), k, c->st);
+  HIPCHK(hipMemcpyAsync(g, c->api_aos.p, n * 59 * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(st_->m1, c->api_m1.p, n * 59 * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(st_->m2, c->api_m2.p, n * 59 * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(st_->grad_accum, c->api_acc.p, n * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(st_->grad_denom, c->api_den.p, n * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+static int ensure_densify_scratch(rgs_ctx* c, size_t n) {
+  size_t nb = (size_t)scan_nb(n) + 1;
+  CHK(ensure(c, c->dens_want, n * 4));
+  CHK(ensure(c, c->dens_cnt, n * 4));
+  CHK(ensure(c, c->dens_W, n * 4));
+  CHK(ensure(c, c->dens_packed, n * 8));
+  CHK(ensure(c, c->dens_prefix, n * 8));
+  CHK(ensure(c, c->dens_b32, nb * 4));
+  CHK(ensure(c, c->dens_b64, nb * 8));
+  CHK(ensure(c, c->dens_t32, 8));
+  CHK(ensure(c, c->dens_t64, 8));
+  CHK(ensure(c, c->scalars, SC_COUNT * 4));
+  return 0;
+}
+static DensifyScratch dscratch(rgs_ctx* c) {
+  return DensifyScratch{P_<uint32_t>(c->dens_want), P_<uint32_t>(c->dens_cnt), P_<uint32_t>(c->dens_W),
+                        P_<uint64_t>(c->dens_packed), P_<uint64_t>(c->dens_prefix), P_<uint32_t>(c->dens_b32),
+                        P_<uint64_t>(c->dens_b64), P_<uint32_t>(c->dens_t32), P_<uint64_t>(c->dens_t64)};
+}
+
+int rgs_densify_and_prune(rgs_ctx* c, rgs_gaussian* g, size_t* n_io, size_t capacity, rgs_adam* st_,
+                          const rgs_train_cfg* cfg, float extent) {
+  if (!c || !n_io || !st_ || !cfg || (*n_io && !g)) return fail(RGS_E_ARG, "null argument");
+  size_t n = *n_io;
+  if (n > capacity || n > st_->rows) return fail(RGS_E_ARG, "n exceeds capacity or AdamState rows");
+  (void)hipSetDevice(c->device);
+  size_t cap = capacity ? capacity : 1;
+  CHK(ensure(c, c->api_aos, cap * 59 * 4));
+  CHK(ensure(c, c->api_P, cap * 59 * 4));
+  CHK(ensure(c, c->api_P2, cap * 59 * 4));
+  CHK(ensure(c, c->api_acc, (n ? n : 1) * 4));
+  CHK(ensure(c, c->api_den, (n ? n : 1) * 4));
+  CHK(ensure_densify_scratch(c, n ? n : 1));
+  if (n) {
+    HIPCHK(hipMemcpyAsync(c->api_aos.p, g, n * 59 * 4, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->api_acc.p, st_->grad_accum, n * 4, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->api_den.p, st_->grad_denom, n * 4, hipMemcpyHostToDevice, c->st));
+    launch_aos_to_soa(P_<float>(c->api_aos), n, P_<float>(c->api_P), cap, c->st);
+  }
+  uint32_t* sc = P_<uint32_t>(c->scalars);
+  launch_densify(P_<float>(c->api_P), P_<float>(c->api_P2), (int)cap, (int)n, P_<float>(c->api_acc),
+                 P_<uint32_t>(c->api_den), cfg->densify_grad_threshold, cfg->prune_opacity_thresh,
+                 cfg->max_gaussians, extent, dscratch(c), sc + SC_NOUT, c->st);
+  HIPCHK(hipMemcpyAsync(c->h_scalars, sc, SC_COUNT * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  size_t nn = c->h_scalars[SC_NOUT];
+  if (nn > capacity) return fail(RGS_E_CAPACITY, "densify result exceeds capacity");
+  if (nn > st_->capacity) return fail(RGS_E_CAPACITY, "densify result exceeds AdamState capacity");
+  if (nn) {
+    launch_soa_to_aos(P_<float>(c->api_P2), cap, nn, P_<float>(c->api_aos), c->st);
+    HIPCHK(hipMemcpyAsync(g, c->api_aos.p, nn * 59 * 4, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+  }
+  // AdamState::resize (grow only) then the statistics reset
+  if (nn > st_->rows) {
+    std::memset(st_->m1 + st_->rows * 59, 0, (nn - st_->rows) * 59 * 4);
+    std::memset(st_->m2 + st_->rows * 59, 0, (nn - st_->rows) * 59 * 4);
+    st_->rows = nn;
+  }
+  std::memset(st_->grad_accum, 0, st_->rows * 4);
+  std::memset(st_->grad_denom, 0, st_->rows * 4);
+  *n_io = nn;
+  return 0;
+}
+
+int rgs_reset_opacities(rgs_ctx* c, rgs_gaussian* g, size_t n) {
+  if (!c || (n && !g)) return fail(RGS_E_ARG, "null argument");
+  if (!n) return 0;
+  (void)hipSetDevice(c->device);
+  CHK(ensure(c, c->api_aos, n * 59 * 4));
+  HIPCHK(hipMemcpyAsync(c->api_aos.p, g, n * 59 * 4, hipMemcpyHostToDevice, c->st));
+  launch_reset_opacity(P_<float>(c->api_aos) + 3, n, 59, c->st);
+  HIPCHK(hipMemcpyAsync(g, c->api_aos.p, n * 59 * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+// ---- device-pointer forms
+int rgs_forward_dev(rgs_ctx* c, const float* g_dev, size_t n, const rgs_camera* cam, const float bg[3],
+                    float* rgb_dev) {
+  CHK(check_common(c, g_dev, n, cam));
+  if (!rgb_dev || !bg) return fail(RGS_E_ARG, "null output or bg");
+  c->fwd_valid = false;
+  ViewSet vs;
+  vs.cams.resize(1);
+  CHK(make_camdev(*cam, vs.cams[0]));
+  CHK(upload_api_scene(c, g_dev, n, true));
+  CHK(render_eager(c, P_<float>(c->api_P), n ? n : 1, (int)n, vs, bg, 0, rgb_dev));
+  c->fwd_views = vs;
+  c->fwd_n = (int)n;
+  for (int i = 0; i < 3; ++i) c->fwd_bg[i] = bg[i];
+  c->fwd_valid = true;
+  return 0;
+}
+
+int rgs_backward_dev(rgs_ctx* c, const float* dl_dev, float* grads_dev) {
+  if (!c || !dl_dev || !grads_dev) return fail(RGS_E_ARG, "null argument");
+  if (!c->fwd_valid) return fail(RGS_E_ARG, "rgs_backward_dev without a preceding rgs_forward_dev");
+  (void)hipSetDevice(c->device);
+  size_t n = (size_t)c->fwd_n, cap = n ? n : 1;
+  CHK(ensure(c, c->api_acc9, 9 * cap * 4));
+  launch_fill_f32(P_<float>(c->api_acc9), 9 * cap, 0.0f, c->st);
+  launch_raster_bwd(raster_in(c, c->fwd_views, c->fwd_bg), P_<float>(c->final_t), P_<int32_t>(c->last), dl_dev,
+                    P_<float>(c->api_acc9), cap, c->st);
+  launch_geometry_aos(P_<float>(c->api_P), (int)cap, (int)n, P_<CamDev>(c->cams_d), 0, P_<uint32_t>(c->cnt),
+                      P_<float>(c->api_acc9), cap, grads_dev, c->st);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// ---- device-resident scene
+int rgs_scene_upload(rgs_ctx* c, const rgs_gaussian* g, size_t n, size_t capacity) {
+  if (!c || (n && !g)) return fail(RGS_E_ARG, "null argument");
+  if (capacity < n) capacity = n;
+  if (capacity == 0) capacity = 1;
+  if (capacity >= (1u << 31)) return fail(RGS_E_ARG, "capacity too large");
+  (void)hipSetDevice(c->device);
+  drop_graph(c);
+  CHK(ensure(c, c->P, capacity * 59 * 4));
+  CHK(ensure(c, c->P2, capacity * 59 * 4));
+  CHK(ensure(c, c->M1, capacity * 59 * 4));
+  CHK(ensure(c, c->M2, capacity * 59 * 4));
+  CHK(ensure(c, c->gacc, capacity * 4));
+  CHK(ensure(c, c->gden, capacity * 4));
+  CHK(ensure(c, c->api_aos, (n ? n : 1) * 59 * 4));
+  c->cap = capacity;
+  c->n = n;
+  c->m_rows = n;
+  c->t = 0;
+  launch_fill_f32(P_<float>(c->M1), capacity * 59, 0.0f, c->st);
+  launch_fill_f32(P_<float>(c->M2), capacity * 59, 0.0f, c->st);
+  launch_fill_f32(P_<float>(c->gacc), capacity, 0.0f, c->st);
+  launch_fill_u32(P_<uint32_t>(c->gden), capacity, 0u, c->st);
+  if (n) {
+    HIPCHK(hipMemcpyAsync(c->api_aos.p, g, n * 59 * 4, hipMemcpyHostToDevice, c->st));
+    launch_aos_to_soa(P_<float>(c->api_aos), n, P_<float>(c->P), capacity, c->st);
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int rgs_scene_size(rgs_ctx* c, size_t* n) {
+  if (!c || !n) return fail(RGS_E_ARG, "null argument");
+  *n = c->n;
+  return 0;
+}
+
+int rgs_scene_download(rgs_ctx* c, rgs_gaussian* g, size_t capacity, size_t* n) {
+  if (!c || !n || (c->n && !g)) return fail(RGS_E_ARG, "null argument");
+  if (capacity < c->n) return fail(RGS_E_CAPACITY, "output capacity smaller than the scene");
+  (void)hipSetDevice(c->device);
+  if (c->n) {
+    CHK(ensure(c, c->api_aos, c->n * 59 * 4));
+    launch_soa_to_aos(P_<float>(c->P), c->cap, c->n, P_<float>(c->api_aos), c->st);
+    HIPCHK(hipMemcpyAsync(g, c->api_aos.p, c->n * 59 * 4, hipMemcpyDeviceToHost, c->st));
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  *n = c->n;
+  return 0;
+}
+
+int rgs_adam_download(rgs_ctx* c, rgs_adam* st_) {
+  if (!c || !st_) return fail(RGS_E_ARG, "null argument");
+  if (st_->capacity < c->m_rows) return fail(RGS_E_CAPACITY, "AdamState capacity smaller than its rows");
+  (void)hipSetDevice(c->device);
+  size_t rows = c->m_rows;
+  if (rows) {
+    // SoA [59][cap] -> row-major rows x 59
+    std::vector<float> tmp(59 * c->cap);
+    HIPCHK(hipMemcpy(tmp.data(), c->M1.p, 59 * c->cap * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < rows; ++i)
+      for (int j = 0; j < 59; ++j) st_->m1[i * 59 + j] = tmp[(size_t)j * c->cap + i];
+    HIPCHK(hipMemcpy(tmp.data(), c->M2.p, 59 * c->cap * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < rows; ++i)
+      for (int j = 0; j < 59; ++j) st_->m2[i * 59 + j] = tmp[(size_t)j * c->cap + i];
+    HIPCHK(hipMemcpy(st_->grad_accum, c->gacc.p, rows * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(st_->grad_denom, c->gden.p, rows * 4, hipMemcpyDeviceToHost));
+  }
+  st_->rows = rows;
+  st_->t = c->t;
+  return 0;
+}
+
+int rgs_set_views(rgs_ctx* c, const rgs_camera* cams, const float* const* targets, int nv) {
+  if (!c || nv <= 0 || !cams || !targets) return fail(RGS_E_ARG, "null argument or no views");
+  (void)hipSetDevice(c->device);
+  drop_graph(c);
+  std::vector<CamDev> v(nv);
+  size_t tot = 0;
+  for (int i = 0; i < nv; ++i) {
+    if (!targets[i]) return fail(RGS_E_ARG, "null target");
+    CHK(make_camdev(cams[i], v[i]));
+    v[i].tgt_base = (uint32_t)tot;
+    tot += (size_t)cams[i].width * cams[i].height;
+  }
+  if (tot >= (1ull << 32)) return fail(RGS_E_ARG, "targets too large");
+  CHK(ensure(c, c->targets, tot * 16));
+  for (int i = 0; i < nv; ++i)
+    HIPCHK(hipMemcpyAsync(P_<float>(c->targets) + (size_t)v[i].tgt_base * 4, targets[i],
+                          (size_t)cams[i].width * cams[i].height * 16, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->reg_cams = v;
+  c->target_pix = tot;
+  return 0;
+}
+
+// Enqueues the step after the projection/depth-sort/scan prefix.
+static void enqueue_step_rest(rgs_ctx* c, const ViewSet& vs, const float bg[3], float lambda) {
+  int n = (int)c->n;
+  size_t cap = c->cap;
+  uint32_t* sc = P_<uint32_t>(c->scalars);
+  enqueue_bin_and_raster(c, cap, n, vs, bg, 0, P_<float>(c->rgb));
+  stage_begin(c, S_LOSS);
+  enqueue_loss(c, vs, P_<float>(c->rgb), P_<float>(c->targets), lambda);
+  stage_end(c, S_LOSS);
+  stage_begin(c, S_BWD);
+  launch_raster_bwd(raster_in(c, vs, bg), P_<float>(c->final_t), P_<int32_t>(c->last), P_<float>(c->dl),
+                    P_<float>(c->accum), vs.cams.size() * cap, c->st);
+  stage_end(c, S_BWD);
+  stage_begin(c, S_ADAM);
+  launch_train_adam(P_<float>(c->P), P_<float>(c->M1), P_<float>(c->M2), (int)cap, n, P_<CamDev>(c->cams_d),
+                    (int)vs.cams.size(), P_<uint32_t>(c->cnt), P_<float>(c->accum), vs.cams.size() * cap,
+                    P_<float>(c->gacc), P_<uint32_t>(c->gden), P_<AdamConsts>(c->adam_d), sc + SC_STATUS, c->st);
+  stage_end(c, S_ADAM);
+}
+
+// Enqueues one whole training step on the context's stream (no host sync).
+static void enqueue_step(rgs_ctx* c, const ViewSet& vs, const float bg[3], float lambda) {
+  launch_fill_u32(P_<uint32_t>(c->scalars), SC_COUNT, 0u, c->st);
+  enqueue_project(c, P_<float>(c->P), c->cap, (int)c->n, vs);
+  enqueue_step_rest(c, vs, bg, lambda);
+}
+
+int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* cfg, const float bg[3],
+                   float lambda, int use_graph, float* loss_out) {
+  if (!c || !ids || n_ids <= 0 || !cfg || !bg) return fail(RGS_E_ARG, "null argument");
+  c->fwd_valid = false;
+  if (n_ids > 64) return fail(RGS_E_ARG, "at most 64 views per step");
+  if (!c->P.p) return fail(RGS_E_ARG, "no scene uploaded");
+  (void)hipSetDevice(c->device);
+  ViewSet vs;
+  std::string key;
+  for (int i = 0; i < n_ids; ++i) {
+    if (ids[i] < 0 || ids[i] >= (int)c->reg_cams.size()) return fail(RGS_E_ARG, "view id out of range");
+    vs.cams.push_back(c->reg_cams[ids[i]]);
+    key += std::to_string(c->reg_cams[ids[i]].w) + "x" + std::to_string(c->reg_cams[ids[i]].h) + ",";
+  }
+  vs.layout();
+  size_t cap = c->cap;
+  int n = (int)c->n;
+  char buf[160];
+  std::snprintf(buf, sizeof(buf), "|n%d|bg%a,%a,%a|l%a", n, bg[0], bg[1], bg[2], lambda);
+  key += buf;
+
+  c->t += 1;
+  *c->h_adam = adam_consts(*cfg, c->t);
+  CHK(ensure(c, c->adam_d, sizeof(AdamConsts)));
+  size_t acc_n = 9 * vs.cams.size() * cap;
+  bool fresh_accum = c->accum.bytes < acc_n * 4;
+  CHK(ensure(c, c->accum, acc_n * 4));
+  if (fresh_accum) launch_fill_f32(P_<float>(c->accum), acc_n, 0.0f, c->st);
+  CHK(ensure_loss(c, vs));
+  std::memcpy(c->h_cams, vs.cams.data(), vs.cams.size() * sizeof(CamDev));
+
+  if (c->profiling) use_graph = 0;
+  for (int attempt = 0; attempt < 3; ++attempt) {
+    bool replay = use_graph && c->gexec && c->gkey == key && c->graph_gen == c->gen;
+    if (!replay) {
+      drop_graph(c);
+      // eager: exact pair count sizes the buffers, then the rest of the step
+      CHK(ensure_render(c, cap, n, vs, std::max<size_t>(c->k_cap, 1)));
+      HIPCHK(hipMemcpyAsync(c->cams_d.p, c->h_cams, vs.cams.size() * sizeof(CamDev), hipMemcpyHostToDevice, c->st));
+      HIPCHK(hipMemcpyAsync(c->adam_d.p, c->h_adam, sizeof(AdamConsts), hipMemcpyHostToDevice, c->st));
+      uint32_t* sc = P_<uint32_t>(c->scalars);
+      launch_fill_u32(sc, SC_COUNT, 0u, c->st);
+      enqueue_project(c, P_<float>(c->P), cap, n, vs);
+      HIPCHK(hipMemcpyAsync(c->h_scalars, sc, SC_COUNT * 4, hipMemcpyDeviceToHost, c->st));
+      HIPCHK(hipStreamSynchronize(c->st));
+      if (c->h_scalars[SC_STATUS] & ST_NAN_DEPTH) {
+        c->t -= 1;
+        return fail(RGS_E_NAN, "NaN depth in projection");
+      }
+      size_t K = c->h_scalars[SC_TOTAL];
+      size_t want = K + K / 4 + 4096;  // headroom for the following steps
+      if (K > c->k_cap) CHK(ensure_render(c, cap, n, vs, std::max(want, c->k_cap)));
+      if (use_graph) {
+        // capture the full step for replay (the projection is re-run inside)
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+        enqueue_step(c, vs, bg, lambda);
+        hipError_t e = hipStreamEndCapture(c->st, &g);
+        if (e != hipSuccess) return fail(RGS_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+          c->gexec = nullptr;
+          return fail(RGS_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+        }
+        c->gkey = key;
+        c->graph_gen = c->gen;
+        HIPCHK(hipGraphLaunch(c->gexec, c->st));
+      } else {
+        enqueue_step_rest(c, vs, bg, lambda);
+      }
+    } else {
+      HIPCHK(hipMemcpyAsync(c->cams_d.p, c->h_cams, vs.cams.size() * sizeof(CamDev), hipMemcpyHostToDevice, c->st));
+      HIPCHK(hipMemcpyAsync(c->adam_d.p, c->h_adam, sizeof(AdamConsts), hipMemcpyHostToDevice, c->st));
+      HIPCHK(hipGraphLaunch(c->gexec, c->st));
+    }
+    HIPCHK(hipMemcpyAsync(c->h_scalars, c->scalars.p, SC_COUNT * 4, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(c->h_loss, c->loss.p, vs.cams.size() * 4, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    uint32_t status = c->h_scalars[SC_STATUS];
+    if (status & ST_NAN_DEPTH) {
+      c->t -= 1;
+      return fail(RGS_E_NAN, "NaN depth in projection");
+    }
+    if (status & ST_PAIR_OVERFLOW) {
+      // the step was skipped on the device (parameters untouched): grow and redo
+      size_t K = c->h_scalars[SC_TOTAL];
+      drop_graph(c);
+      CHK(ensure_render(c, cap, n, vs, K + K / 2 + 4096));
+      launch_fill_f32(P_<float>(c->accum), acc_n, 0.0f, c->st);
+      continue;
+    }
+    c->last_pairs = c->h_scalars[SC_K];
+    c->last_cap = cap;
+    c->last_tiles = vs.total_tiles;
+    c->last_nviews = (int)vs.cams.size();
+    if (c->profiling) {
+      for (int s = 0; s < RGS_N_STAGES; ++s) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev_b[s], c->ev_e[s]));
+        c->stage_ms[s] += ms;
+      }
+      c->prof_steps += 1;
+    }
+    if (loss_out)
+      for (int i = 0; i < n_ids; ++i) loss_out[i] = c->h_loss[i];
+    return 0;
+  }
+  return fail(RGS_E_HIP, "pair buffer kept overflowing");
+}
+
+int rgs_scene_densify(rgs_ctx* c, const rgs_train_cfg* cfg, float extent) {
+  if (!c || !cfg) return fail(RGS_E_ARG, "null argument");
+  (void)hipSetDevice(c->device);
+  size_t n = c->n;
+  CHK(ensure_densify_scratch(c, n ? n : 1));
+  uint32_t* sc = P_<uint32_t>(c->scalars);
+  launch_densify(P_<float>(c->P), P_<float>(c->P2), (int)c->cap, (int)n, P_<float>(c->gacc), P_<uint32_t>(c->gden),
+                 cfg->densify_grad_threshold, cfg->prune_opacity_thresh, cfg->max_gaussians, extent, dscratch(c),
+                 sc + SC_NOUT, c->st);
+  HIPCHK(hipMemcpyAsync(c->h_scalars, sc, SC_COUNT * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  size_t nn = c->h_scalars[SC_NOUT];
+  if (nn > c->cap) return fail(RGS_E_CAPACITY, "densify result exceeds the scene capacity");
+  std::swap(c->P, c->P2);
+  if (nn > c->m_rows) {
+    launch_zero_rows(P_<float>(c->M1), c->cap, c->m_rows, nn, c->st);
+    launch_zero_rows(P_<float>(c->M2), c->cap, c->m_rows, nn, c->st);
+    c->m_rows = nn;
+  }
+  launch_fill_f32(P_<float>(c->gacc), c->m_rows, 0.0f, c->st);
+  launch_fill_u32(P_<uint32_t>(c->gden), c->m_rows, 0u, c->st);
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->n = nn;
+  drop_graph(c);
+  return 0;
+}
+
+int rgs_scene_reset_opacities(rgs_ctx* c) {
+  if (!c) return fail(RGS_E_ARG, "null ctx");
+  (void)hipSetDevice(c->device);
+  if (c->n) launch_reset_opacity(P_<float>(c->P) + (size_t)P_OP * c->cap, c->n, 1, c->st);
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int rgs_set_profiling(rgs_ctx* c, int enable) {
+  if (!c) return fail(RGS_E_ARG, "null ctx");
+  (void)hipSetDevice(c->device);
+  if (enable && !c->ev_b[0]) {
+    for (int s = 0; s < RGS_N_STAGES; ++s) {
+      HIPCHK(hipEventCreate(&c->ev_b[s]));
+      HIPCHK(hipEventCreate(&c->ev_e[s]));
+    }
+  }
+  c->profiling = enable != 0;
+  for (int s = 0; s < RGS_N_STAGES; ++s) c->stage_ms[s] = 0.0;
+  c->prof_steps = 0;
+  return 0;
+}
+
+int rgs_stage_times(rgs_ctx* c, float* mean_ms, int n_stages, uint64_t* steps) {
+  if (!c || !mean_ms) return fail(RGS_E_ARG, "null argument");
+  for (int s = 0; s < n_stages && s < RGS_N_STAGES; ++s)
+    mean_ms[s] = c->prof_steps ? (float)(c->stage_ms[s] / (double)c->prof_steps) : 0.0f;
+  if (steps) *steps = c->prof_steps;
+  return 0;
+}
+
+int rgs_step_stats(rgs_ctx* c, uint64_t* n_pairs, uint64_t* fwd_evals, uint64_t* bwd_evals) {
+  if (!c) return fail(RGS_E_ARG, "null ctx");
+  if (n_pairs) *n_pairs = c->last_pairs;
+  if (fwd_evals || bwd_evals) {
+    (void)hipSetDevice(c->device);
+    CHK(ensure(c, c->api_den, 16));
+    unsigned long long* d = P_<unsigned long long>(c->api_den);
+    launch_fill_u32(P_<uint32_t>(c->api_den), 4, 0u, c->st);
+    RasterIn in{P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend), c->pair_vals, splats(c), P_<CamDev>(c->cams_d),
+                c->last_nviews, c->last_tiles, {0.0f, 0.0f, 0.0f}};
+    launch_work_counts(in, P_<float>(c->final_t), P_<int32_t>(c->last), d, c->st);
+    unsigned long long h[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(h, d, 16, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (fwd_evals) *fwd_evals = h[0];
+    if (bwd_evals) *bwd_evals = h[1];
+  }
+  return 0;
+}
+
+int rgs_eval_expf(rgs_ctx* c, const float* x, float* out, size_t n) {
+  if (!c || (n && (!x || !out))) return fail(RGS_E_ARG, "null argument");
+  if (!n) return 0;
+  (void)hipSetDevice(c->device);
+  CHK(ensure(c, c->api_aos, n * 8));
+  HIPCHK(hipMemcpyAsync(c->api_aos.p, x, n * 4, hipMemcpyHostToDevice, c->st));
+  launch_eval_expf(P_<float>(c->api_aos), P_<float>(c->api_aos) + n, n, c->st);
+  HIPCHK(hipMemcpyAsync(out, P_<float>(c->api_aos) + n, n * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int rgs_debug_tiles(rgs_ctx* c, uint64_t* offsets, size_t n_offsets, uint32_t* gis, size_t k_max, uint64_t* k_out) {
+  if (!c || !offsets || !k_out) return fail(RGS_E_ARG, "null argument");
+  if (n_offsets < (size_t)c->last_tiles + 1) return fail(RGS_E_ARG, "offsets too small");
+  (void)hipSetDevice(c->device);
+  HIPCHK(hipStreamSynchronize(c->st));
+  size_t nt = (size_t)c->last_tiles;
+  std::vector<uint32_t> rs(nt), re(nt), pv(c->last_pairs);
+  if (nt) {
+    HIPCHK(hipMemcpy(rs.data(), c->rstart.p, nt * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(re.data(), c->rend.p, nt * 4, hipMemcpyDeviceToHost));
+  }
+  if (c->last_pairs) HIPCHK(hipMemcpy(pv.data(), c->pair_vals, c->last_pairs * 4, hipMemcpyDeviceToHost));
+  size_t k = 0;
+  for (size_t t = 0; t < nt; ++t) {
+    offsets[t] = k;
+    for (uint32_t p = rs[t]; p < re[t]; ++p) {
+      if (gis && k < k_max) gis[k] = (uint32_t)(pv[p] % c->last_cap);
+      ++k;
+    }
+  }
+  offsets[nt] = k;
+  *k_out = k;
+  if (k > k_max && gis) return fail(RGS_E_ARG, "gis too small");
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,163 @@
+// rgs_kernels.h -- launchers of the gfx950 kernels (internal to librgsplat_hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "rgs_math.h"
+
+namespace rgs {
+
+// Per (view, Gaussian) projection results at index vi = v * cap + i.
+struct SplatArrays {
+  float4* xy_ab;    // sx, sy, conic a, conic b
+  float4* c_op_rg;  // conic c, opacity, red, green
+  float* col_b;     // blue
+  uint32_t* cnt;    // tiles touched (0 when culled)
+  uint2* rect;      // (tx0 | ty0 << 16, tx1 | ty1 << 16)
+};
+
+// Status bits written by kernels into a device word.
+enum : uint32_t { ST_NAN_DEPTH = 1u, ST_PAIR_OVERFLOW = 2u };
+
+// ---- projection (k_project.hip)
+// Projects n Gaussians (SoA params P[59][cap]) into n_views views. Writes the
+// splat arrays and the depth-sort input (key = depth bits, 0xffffffff when
+// culled; value = v * cap + i) at dense index v * n + i.
+void launch_project(const float* P, int cap, int n, const CamDev* cams, int n_views, SplatArrays S,
+                    uint32_t* sort_keys, uint32_t* sort_vals, uint32_t* status, hipStream_t st);
+
+// Device expf over n values (diagnostic).
+void launch_eval_expf(const float* x, float* out, size_t n, hipStream_t st);
+
+// ---- sorting and binning (k_sort.hip)
+struct RadixScratch {
+  uint32_t* hist;       // 256 * nb_max
+  uint32_t* row_total;  // 256
+  int nb_max;
+};
+constexpr int RS_TILE = 4096;
+inline int radix_nb(size_t n_max) { return (int)((n_max + RS_TILE - 1) / RS_TILE); }
+// Stable LSD radix sort of (key, value) pairs over bits [0, bits); count read
+// from device memory (*n_dev <= n_max). Ping-pongs between the two buffer
+// pairs; returns the index (0/1) of the buffer pair holding the result.
+int radix_sort_pairs(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev, size_t n_max, int bits,
+                     RadixScratch rs, hipStream_t st);
+
+constexpr int SCAN_TILE = 2048;
+inline int scan_nb(size_t n) { return (int)((n + SCAN_TILE - 1) / SCAN_TILE); }
+// Exclusive scan of cnt[vals[r]] over r < n (pair offsets in depth order).
+// block_sums needs scan_nb(n) + 1 entries; total written to *total.
+void scan_gathered_counts(const uint32_t* vals, const uint32_t* cnt, size_t n, uint32_t* out,
+                          uint32_t* block_sums, uint32_t* total, hipStream_t st);
+// Exclusive scan of a u32 array (in may alias out).
+void scan_u32(const uint32_t* in, size_t n, uint32_t* out, uint32_t* block_sums, uint32_t* total, hipStream_t st);
+// Exclusive scan of a u64 array.
+void scan_u64(const uint64_t* in, size_t n, uint64_t* out, uint64_t* block_sums, uint64_t* total, hipStream_t st);
+
+// Writes the (tile, splat) pairs in depth order; K = min(total, k_cap)
+// written to *k_out, ST_PAIR_OVERFLOW raised when total > k_cap.
+void launch_gen_pairs(const uint32_t* sorted_vals, size_t n_sorted, const uint32_t* offsets, const uint32_t* total,
+                      const SplatArrays S, int cap, const CamDev* cams, uint32_t* pair_keys, uint32_t* pair_vals,
+                      size_t k_cap, uint32_t* k_out, uint32_t* status, hipStream_t st);
+// Tile ranges [start, end) over the sorted pair keys (tiles with no pairs get
+// start = end = 0).
+void launch_ranges(const uint32_t* pair_keys, const uint32_t* k_dev, size_t k_cap, int total_tiles,
+                   uint32_t* rstart, uint32_t* rend, hipStream_t st);
+
+// ---- rasterization (k_raster.hip)
+struct RasterIn {
+  const uint32_t* rstart;
+  const uint32_t* rend;
+  const uint32_t* pair_vals;
+  SplatArrays S;
+  const CamDev* cams;
+  int n_views;
+  int total_tiles;
+  float bg[3];
+};
+// mode 0: training forward (rgb = C + T bg; final T; last contributor);
+// mode 1: rasterizer.rs RGBA (rgba = bg + C, A = 1 - T; empty tiles left alone).
+void launch_raster_fwd(const RasterIn& in, int mode, float* rgb, float* final_t, int32_t* last, hipStream_t st);
+// Accumulates the 9 per-splat screen-space gradients (dL/dcolour[3],
+// dL/dopacity, dL/dconic[3], dL/dsx, dL/dsy) into accum[k * acc_stride + vi].
+void launch_raster_bwd(const RasterIn& in, const float* final_t, const int32_t* last, const float* dl_dpix,
+                       float* accum, size_t acc_stride, hipStream_t st);
+
+// out[0] += forward pair evaluations, out[1] += backward pair evaluations.
+void launch_work_counts(const RasterIn& in, const float* final_t, const int32_t* last, unsigned long long* out,
+                        hipStream_t st);
+
+// ---- loss (k_loss.hip)
+struct LossIn {
+  const float* rgb;     // total_pix * 3
+  const float* target;  // target store, RGBA; view v starts at pixel cams[v].tgt_base
+  const CamDev* cams;
+  int n_views;
+  int total_tiles;
+  size_t total_pix;
+  float lambda;
+};
+// ssim_maps: 5 * 3 * total_pix floats; partial: 2 * 3 * total_tiles floats;
+// loss_out: n_views floats.
+void launch_loss(const LossIn& in, float* dl, float* ssim_maps, float* partial, float* loss_out, hipStream_t st);
+
+// ---- optimizer and parameter kernels (k_optim.hip)
+struct AdamConsts {
+  float lr[5];  // position, rotation, scale, opacity, sh_dc
+  float lr_sh_rest;
+  float beta1, beta2, one_minus_beta1, one_minus_beta2, bc1, bc2, eps;
+};
+// AoS (Gaussian3D, 59 floats) <-> SoA (P[59][cap], Adam flat order).
+void launch_aos_to_soa(const float* aos, size_t n, float* P, size_t cap, hipStream_t st);
+void launch_soa_to_aos(const float* P, size_t cap, size_t n, float* aos, hipStream_t st);
+// GaussianGrad AoS (60 floats, zero when culled) for one view.
+void launch_geometry_aos(const float* P, int cap, int n, const CamDev* cams, int view, const uint32_t* cnt,
+                         const float* accum, size_t acc_stride, float* grads, hipStream_t st);
+// Adam on the reference layouts: AoS params/grads, row-major moments.
+void launch_adam_aos(float* g, const float* grads, size_t n, float* m1, float* m2, float* accum, uint32_t* denom,
+                     const AdamConsts& k, hipStream_t st);
+// Fused per-Gaussian geometry backprop over all views + Adam on SoA buffers.
+// Skips everything when *status has ST_PAIR_OVERFLOW. Clears accum.
+void launch_train_adam(float* P, float* M1, float* M2, int cap, int n, const CamDev* cams, int n_views,
+                       const uint32_t* cnt, float* accum, size_t acc_stride, float* gaccum, uint32_t* gdenom,
+                       const AdamConsts* k_dev, const uint32_t* status, hipStream_t st);
+// reset_opacities on SoA (row 10) or AoS (offset 3) params.
+void launch_reset_opacity(float* p, size_t n, size_t stride, hipStream_t st);
+// Densify/prune on SoA params: P -> P_out (compacted + children). Writes the
+// new count to *n_out. Scratch sized for n Gaussians.
+struct DensifyScratch {
+  uint32_t* want;        // n
+  uint32_t* cnt;         // n
+  uint32_t* W;           // n
+  uint64_t* packed;      // n
+  uint64_t* prefix;      // n
+  uint32_t* bsum32;      // scan_nb(n) + 1
+  uint64_t* bsum64;      // scan_nb(n) + 1
+  uint32_t* total32;     // 1
+  uint64_t* total64;     // 1
+};
+void launch_densify(const float* P, float* P_out, int cap, int n, const float* gaccum, const uint32_t* gdenom,
+                    float grad_thresh, float prune_thresh, uint64_t max_gaussians, float extent, DensifyScratch ds,
+                    uint32_t* n_out, hipStream_t st);
+void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t st);
+void launch_fill_f32(float* p, size_t n, float v, hipStream_t st);
+// Zero rows [r0, r1) of each of the 59 rows of a SoA moment buffer.
+void launch_zero_rows(float* M, size_t cap, size_t r0, size_t r1, hipStream_t st);
+
+// xcd-aware block remap: consecutive logical tiles share an XCD (and its L2).
+__device__ __forceinline__ int xcd_remap(int b, int nblocks) {
+  const int nx = 8;
+  int per = nblocks / nx;
+  int rem = nblocks - per * nx;
+  int x = b % nx;  // hardware dispatches block b to XCD b % 8
+  int k = b / nx;
+  // XCD x owns logical tiles [x*per + min(x, rem), ...) of length per + (x < rem)
+  int base = x * per + (x < rem ? x : rem);
+  int len = per + (x < rem ? 1 : 0);
+  if (k < len) return base + k;
+  return b;  // unreachable for a full grid
+}
+
+}  // namespace rgs

@@ -1,0 +1,349 @@
+"""ctypes binding of librgsplat_hip.so and a Python mirror of the crate API.
+
+The mirrored functions keep the reference's names, argument meaning and
+layouts (cpu/backward.rs, cpu/rasterizer.rs, pipeline/optimizer.rs):
+Gaussians are (n, 59) float32 Gaussian3D records, gradients (n, 60)
+GaussianGrad records, images row-major (h, w, 3) RGB / (h, w, 4) RGBA float32.
+Failures raise RgsError with the library's message instead of panicking.
+
+There is no CPU fallback: importing this module without the built library, or
+calling it on a machine without a HIP device, raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import synthetic
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librgsplat_hip.so")
+
+_lib = None
+
+
+class RgsError(RuntimeError):
+    pass
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("fx", ctypes.c_float),
+                ("fy", ctypes.c_float), ("cx", ctypes.c_float), ("cy", ctypes.c_float),
+                ("c2w", ctypes.c_float * 16), ("distortion", ctypes.c_float * 3)]
+
+
+class TrainCfg(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_float) for k in ("lr_position", "lr_rotation", "lr_scale", "lr_opacity", "lr_sh_dc",
+                                              "lr_sh_rest", "beta1", "beta2", "eps", "densify_grad_threshold",
+                                              "prune_opacity_thresh")] + \
+               [(k, ctypes.c_uint32) for k in ("densify_from_iter", "densify_until_iter", "opacity_reset_iter")] + \
+               [(k, ctypes.c_uint64) for k in ("max_gaussians", "num_iterations")]
+
+
+class AdamRec(ctypes.Structure):
+    _fields_ = [("m1", ctypes.c_void_p), ("m2", ctypes.c_void_p), ("grad_accum", ctypes.c_void_p),
+                ("grad_denom", ctypes.c_void_p), ("rows", ctypes.c_size_t), ("capacity", ctypes.c_size_t),
+                ("t", ctypes.c_uint64)]
+
+
+# exported symbol -> argtypes (restype int unless noted)
+_vp, _sz, _i, _f = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_float
+_CAM = ctypes.POINTER(Camera)
+_CFG = ctypes.POINTER(TrainCfg)
+_ADAM = ctypes.POINTER(AdamRec)
+SIGNATURES = {
+    "rgs_create": [_i, ctypes.POINTER(_vp)],
+    "rgs_destroy": [_vp],
+    "rgs_last_error": [],
+    "rgs_set_stream": [_vp, _vp],
+    "rgs_render_and_backward": [_vp, _vp, _sz, _CAM, _vp, _vp, _f, ctypes.POINTER(_f), _vp],
+    "rgs_render_color": [_vp, _vp, _sz, _CAM, _vp, _vp],
+    "rgs_rasterize_rgba": [_vp, _vp, _sz, _CAM, _vp, _vp],
+    "rgs_loss_grad": [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32, _f, ctypes.POINTER(_f), _vp],
+    "rgs_adam_step": [_vp, _vp, _vp, _sz, _ADAM, _CFG],
+    "rgs_densify_and_prune": [_vp, _vp, ctypes.POINTER(_sz), _sz, _ADAM, _CFG, _f],
+    "rgs_reset_opacities": [_vp, _vp, _sz],
+    "rgs_forward_dev": [_vp, _vp, _sz, _CAM, _vp, _vp],
+    "rgs_backward_dev": [_vp, _vp, _vp],
+    "rgs_scene_upload": [_vp, _vp, _sz, _sz],
+    "rgs_scene_size": [_vp, ctypes.POINTER(_sz)],
+    "rgs_scene_download": [_vp, _vp, _sz, ctypes.POINTER(_sz)],
+    "rgs_adam_download": [_vp, _ADAM],
+    "rgs_set_views": [_vp, _CAM, ctypes.POINTER(ctypes.c_void_p), _i],
+    "rgs_train_step": [_vp, ctypes.POINTER(_i), _i, _CFG, _vp, _f, _i, _vp],
+    "rgs_scene_densify": [_vp, _CFG, _f],
+    "rgs_scene_reset_opacities": [_vp],
+    "rgs_step_stats": [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                       ctypes.POINTER(ctypes.c_uint64)],
+    "rgs_set_raster_batch": [_vp, _i],
+    "rgs_eval_expf": [_vp, _vp, _vp, _sz],
+    "rgs_set_profiling": [_vp, _i],
+    "rgs_stage_times": [_vp, _vp, _i, ctypes.POINTER(ctypes.c_uint64)],
+    "rgs_debug_tiles": [_vp, _vp, _sz, _vp, _sz, ctypes.POINTER(ctypes.c_uint64)],
+}
+
+
+def lib():
+    """Load the built library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m rgsplat_amd.build` "
+                              "(there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        L.rgs_destroy.restype = None
+        L.rgs_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(rc: int):
+    if rc != 0:
+        msg = lib().rgs_last_error()
+        raise RgsError(f"rgsplat_hip error {rc}: {msg.decode() if msg else ''}")
+
+
+def camera(cam) -> Camera:
+    c = Camera()
+    c.width, c.height = int(cam.width), int(cam.height)
+    c.fx, c.fy, c.cx, c.cy = float(cam.fx), float(cam.fy), float(cam.cx), float(cam.cy)
+    for i, v in enumerate(np.asarray(cam.c2w, dtype=np.float32).reshape(16)):
+        c.c2w[i] = float(v)
+    return c
+
+
+def train_cfg(d: dict | None = None) -> TrainCfg:
+    d = d or synthetic.train_config()
+    c = TrainCfg()
+    for k, _ in TrainCfg._fields_:
+        setattr(c, k, d[k])
+    return c
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None and a.shape != shape:
+        a = a.reshape(shape)
+    return a
+
+
+class AdamState:
+    """AdamState (optimizer.rs:74-106): m1/m2 rows x 59, grad statistics, t.
+    Buffers are allocated with room for `capacity` rows."""
+
+    def __init__(self, n: int, capacity: int | None = None):
+        cap = max(capacity or n, n, 1)
+        self.m1 = np.zeros((cap, 59), np.float32)
+        self.m2 = np.zeros((cap, 59), np.float32)
+        self.grad_accum = np.zeros(cap, np.float32)
+        self.grad_denom = np.zeros(cap, np.uint32)
+        self.rows = n
+        self.t = 0
+
+    def rec(self) -> AdamRec:
+        r = AdamRec()
+        r.m1, r.m2 = _p(self.m1), _p(self.m2)
+        r.grad_accum, r.grad_denom = _p(self.grad_accum), _p(self.grad_denom)
+        r.rows, r.capacity, r.t = self.rows, len(self.m1), self.t
+        return r
+
+    def pull(self, r: AdamRec):
+        self.rows, self.t = r.rows, r.t
+
+
+class Context:
+    """One HIP device context (stream + device buffers)."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib().rgs_create(device, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rgs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------- reference API mirror
+    def render_and_backward(self, gaussians, cam, target, bg=(0.0, 0.0, 0.0), lambda_dssim=0.2):
+        """backward.rs:107-149 -> (loss, grads (n, 60))."""
+        g = _f32(gaussians)
+        n = len(g)
+        tg = _f32(target)
+        if tg.size != cam.width * cam.height * 4:
+            raise RgsError("target must be h*w*4")
+        grads = np.zeros((n, 60), np.float32)
+        loss = ctypes.c_float()
+        c = camera(cam)
+        _check(lib().rgs_render_and_backward(self.h, _p(g), n, ctypes.byref(c), _p(tg), _p(_f32(bg)),
+                                             float(lambda_dssim), ctypes.byref(loss), _p(grads)))
+        return float(loss.value), grads
+
+    def render_color(self, gaussians, cam, bg=(0.0, 0.0, 0.0)):
+        """backward.rs:152-166 -> (h, w, 3)."""
+        g = _f32(gaussians)
+        out = np.zeros((cam.height, cam.width, 3), np.float32)
+        c = camera(cam)
+        _check(lib().rgs_render_color(self.h, _p(g), len(g), ctypes.byref(c), _p(_f32(bg)), _p(out)))
+        return out
+
+    def rasterize(self, gaussians, cam, bg=(0.0, 0.0, 0.0)):
+        """rasterizer.rs:72-192 -> Framebuffer data (h, w, 4)."""
+        g = _f32(gaussians)
+        out = np.zeros((cam.height, cam.width, 4), np.float32)
+        c = camera(cam)
+        _check(lib().rgs_rasterize_rgba(self.h, _p(g), len(g), ctypes.byref(c), _p(_f32(bg)), _p(out)))
+        return out
+
+    def photometric_loss_grad(self, rendered, target_rgba, w, h, lambda_dssim=0.2):
+        """backward.rs:646-693 -> (loss, dL/drendered (w*h*3,))."""
+        r = _f32(rendered)
+        t = _f32(target_rgba)
+        if r.size != w * h * 3 or t.size != w * h * 4:
+            raise RgsError("rendered must be w*h*3 and target w*h*4")
+        dl = np.zeros(w * h * 3, np.float32)
+        loss = ctypes.c_float()
+        _check(lib().rgs_loss_grad(self.h, _p(r), _p(t), w, h, float(lambda_dssim), ctypes.byref(loss), _p(dl)))
+        return float(loss.value), dl
+
+    def adam_step(self, gaussians: np.ndarray, grads, state: AdamState, cfg: dict | None = None):
+        """optimizer.rs:116-163; updates `gaussians` (float32, C-contiguous) in place."""
+        assert gaussians.dtype == np.float32 and gaussians.flags.c_contiguous
+        gr = _f32(grads)
+        rec = state.rec()
+        c = train_cfg(cfg)
+        _check(lib().rgs_adam_step(self.h, _p(gaussians), _p(gr), len(gaussians), ctypes.byref(rec),
+                                   ctypes.byref(c)))
+        state.pull(rec)
+
+    def densify_and_prune(self, gbuf: np.ndarray, n: int, state: AdamState, cfg: dict | None, extent: float):
+        """optimizer.rs:206-272 on a (capacity, 59) buffer whose first n rows
+        are live; returns the new n."""
+        assert gbuf.dtype == np.float32 and gbuf.flags.c_contiguous
+        nn = ctypes.c_size_t(n)
+        rec = state.rec()
+        c = train_cfg(cfg)
+        _check(lib().rgs_densify_and_prune(self.h, _p(gbuf), ctypes.byref(nn), len(gbuf), ctypes.byref(rec),
+                                           ctypes.byref(c), float(extent)))
+        state.pull(rec)
+        return nn.value
+
+    def reset_opacities(self, gaussians: np.ndarray):
+        """optimizer.rs:314-319 (in place)."""
+        assert gaussians.dtype == np.float32 and gaussians.flags.c_contiguous
+        _check(lib().rgs_reset_opacities(self.h, _p(gaussians), len(gaussians)))
+
+    # --------------------------------------------- device-resident training
+    def scene_upload(self, gaussians, capacity: int | None = None):
+        g = _f32(gaussians)
+        _check(lib().rgs_scene_upload(self.h, _p(g), len(g), int(capacity or len(g))))
+
+    def scene_size(self) -> int:
+        n = ctypes.c_size_t()
+        _check(lib().rgs_scene_size(self.h, ctypes.byref(n)))
+        return n.value
+
+    def scene_download(self) -> np.ndarray:
+        n = self.scene_size()
+        out = np.zeros((max(n, 1), 59), np.float32)
+        nn = ctypes.c_size_t()
+        _check(lib().rgs_scene_download(self.h, _p(out), len(out), ctypes.byref(nn)))
+        return out[: nn.value]
+
+    def adam_download(self, capacity: int | None = None) -> AdamState:
+        n = self.scene_size()
+        st = AdamState(0, capacity=max(capacity or 0, n, 1) * 2)
+        rec = st.rec()
+        _check(lib().rgs_adam_download(self.h, ctypes.byref(rec)))
+        st.pull(rec)
+        return st
+
+    def set_views(self, cams, targets):
+        arr = (Camera * len(cams))(*[camera(c) for c in cams])
+        self._targets = [_f32(t) for t in targets]
+        ptrs = (ctypes.c_void_p * len(cams))(*[t.ctypes.data for t in self._targets])
+        _check(lib().rgs_set_views(self.h, arr, ptrs, len(cams)))
+
+    def train_step(self, view_ids, cfg: dict | None = None, bg=(0.0, 0.0, 0.0), lambda_dssim=0.2,
+                   use_graph=True):
+        ids = (ctypes.c_int * len(view_ids))(*[int(v) for v in view_ids])
+        loss = np.zeros(len(view_ids), np.float32)
+        c = train_cfg(cfg)
+        _check(lib().rgs_train_step(self.h, ids, len(view_ids), ctypes.byref(c), _p(_f32(bg)),
+                                    float(lambda_dssim), int(bool(use_graph)), _p(loss)))
+        return loss
+
+    def scene_densify(self, cfg: dict | None, extent: float):
+        c = train_cfg(cfg)
+        _check(lib().rgs_scene_densify(self.h, ctypes.byref(c), float(extent)))
+
+    def scene_reset_opacities(self):
+        _check(lib().rgs_scene_reset_opacities(self.h))
+
+    def step_pairs(self) -> int:
+        k = ctypes.c_uint64()
+        _check(lib().rgs_step_stats(self.h, ctypes.byref(k), None, None))
+        return k.value
+
+    def step_work(self) -> tuple[int, int, int]:
+        """(tile pairs, forward pair evaluations, backward pair evaluations)
+        of the last render."""
+        k, f, b = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().rgs_step_stats(self.h, ctypes.byref(k), ctypes.byref(f), ctypes.byref(b)))
+        return k.value, f.value, b.value
+
+    def set_raster_batch(self, batch: int):
+        _check(lib().rgs_set_raster_batch(self.h, int(batch)))
+
+    # ---------------------------------------------------------- stage timing
+    STAGES = ["project", "depth_sort", "pair_scan", "pair_gen", "tile_sort", "tile_ranges", "raster_fwd",
+              "loss_ssim", "raster_bwd", "geometry_adam"]
+
+    def set_profiling(self, enable: bool):
+        _check(lib().rgs_set_profiling(self.h, int(bool(enable))))
+
+    def stage_times(self) -> tuple[dict, int]:
+        ms = np.zeros(len(self.STAGES), np.float32)
+        n = ctypes.c_uint64()
+        _check(lib().rgs_stage_times(self.h, _p(ms), len(ms), ctypes.byref(n)))
+        return dict(zip(self.STAGES, ms.tolist())), n.value
+
+    # ------------------------------------------------------------ diagnostics
+    def eval_expf(self, x) -> np.ndarray:
+        x = _f32(x).reshape(-1)
+        out = np.zeros_like(x)
+        _check(lib().rgs_eval_expf(self.h, _p(x), _p(out), len(x)))
+        return out
+
+    def debug_tiles(self, n_tiles: int, k_max: int):
+        off = np.zeros(n_tiles + 1, np.uint64)
+        gis = np.zeros(max(k_max, 1), np.uint32)
+        k = ctypes.c_uint64()
+        _check(lib().rgs_debug_tiles(self.h, _p(off), len(off), _p(gis), len(gis), ctypes.byref(k)))
+        return off, gis[: k.value]
+
+
+def scene_extent(gaussians) -> float:
+    """estimate_scene_extent (pipeline/mod.rs:299-313): bbox diagonal of the
+    positions (f32 arithmetic)."""
+    g = np.asarray(gaussians, np.float32)
+    if len(g) == 0:
+        return 1.0
+    d = (g[:, 0:3].max(0) - g[:, 0:3].min(0)).astype(np.float32)
+    return float(np.sqrt(np.float32(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])))
