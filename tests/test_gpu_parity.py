@@ -287,7 +287,8 @@ def test_train_step_single_view(ctx, use_graph):
     g1 = c.scene_download()
     lr = np.array([cfg["lr_position"]] * 3 + [cfg["lr_opacity"]])
     dp = np.abs(g1[:, [0, 1, 2, 3]] - g[:, [0, 1, 2, 3]])
-    assert np.all(dp <= lr * 1.0001 + 1e-7)
+    # |update| <= lr, up to the f32 rounding of p - update (2 ulp of p)
+    assert np.all(dp <= lr * 1.0001 + 2 * np.spacing(np.abs(g[:, [0, 1, 2, 3]])))
     # further steps: graph replay and eager agree on the loss trajectory
     for _ in range(3):
         c.train_step([0], cfg, use_graph=use_graph)
@@ -439,10 +440,12 @@ def test_full_size_properties(ctx):
     K = ctx.step_pairs()
     off, gis = ctx.debug_tiles(120 * 68, K + 16)
     assert len(gis) == K and off[-1] == K
-    # every tile list is in depth order (ties by index) and duplicate-free
-    pc = np.asarray(cam.c2w, np.float64)
-    view = np.linalg.inv(pc)
-    depth = (g[:, 0:3].astype(np.float64) @ view[2, :3] + view[2, 3]).astype(np.float32)
+    # every tile list is in depth order (ties by index): depth is the f32
+    # p_cam.z in the reference's operation order ((v0 x + v1 y) + v2 z) + v3
+    view = oracle.view_matrix(cam)
+    v = view[2]
+    depth = ((v[0] * g[:, 0] + v[1] * g[:, 1]) + v[2] * g[:, 2]) + v[3] * np.float32(1.0)
+    assert depth.dtype == np.float32
     for t in range(0, 120 * 68, 97):
         L = gis[off[t]:off[t + 1]].astype(np.int64)
         if len(L) > 1:
