@@ -20,22 +20,12 @@ __device__ __forceinline__ int aos_offset(int j) {
   return j;
 }
 
+// Per view: SH basis at the view direction and the clamp-masked dL/dcolour
+// (backward.rs:533-544, sh.rs:39-61,122-139).
 template <class Load>
-__device__ __forceinline__ void geometry_grad(const CamDev& c, Load prm, const float acc[9], float* out,
-                                              bool add) {
-  // prm(j) returns parameter j in Adam flat order.
+__device__ __forceinline__ void view_sh(const CamDev& c, Load prm, const float acc[9], float basis[16],
+                                        float masked[3]) {
   const float px = prm(0), py = prm(1), pz = prm(2);
-  const float q[4] = {prm(3), prm(4), prm(5), prm(6)};
-  const float scale[3] = {rgs_expf(prm(7)), rgs_expf(prm(8)), rgs_expf(prm(9))};
-  const float op = sigmoid(prm(10));
-  float res[60];
-#define OUT(j, val) res[j] = (val)
-  OUT(10, acc[3] * op * (1.0f - op));
-
-  float pc[3];
-  to_camera(c, px, py, pz, pc);
-  const float tx = pc[0], ty = pc[1], tz = pc[2];
-  // SH basis and raw colour (projection side)
   float u0 = c.cam_pos[0] - px, u1 = c.cam_pos[1] - py, u2 = c.cam_pos[2] - pz;
   float ulen = __builtin_sqrtf(u0 * u0 + u1 * u1 + u2 * u2);
   float d0, d1, d2;
@@ -44,9 +34,7 @@ __device__ __forceinline__ void geometry_grad(const CamDev& c, Load prm, const f
   } else {
     d0 = u0 / ulen; d1 = u1 / ulen; d2 = u2 / ulen;
   }
-  float basis[16];
   sh_basis(d0, d1, d2, basis);
-  float masked[3];
 #pragma unroll
   for (int ch = 0; ch < 3; ++ch) {
     float a = 0.0f;
@@ -55,10 +43,30 @@ __device__ __forceinline__ void geometry_grad(const CamDev& c, Load prm, const f
     float raw = 0.5f + a;
     masked[ch] = (raw > 0.0f && raw < 1.0f) ? acc[ch] : 0.0f;
   }
-#pragma unroll
-  for (int ch = 0; ch < 3; ++ch)
-#pragma unroll
-    for (int k = 0; k < 16; ++k) OUT(P_SH + ch * 16 + k, masked[ch] * basis[k]);
+}
+
+// Gradients of everything but the SH coefficients: position (0-2), rotation
+// (3-6), log-scale (7-9), opacity logit (10) and the screen-gradient norm
+// (59) (backward.rs:529-531, 546-636). ADD accumulates over views.
+template <bool ADD, class Load>
+__device__ __forceinline__ void geometry_nonsh(const CamDev& c, Load prm, const float acc[9],
+                                               const float masked[3], float* out) {
+  const float px = prm(0), py = prm(1), pz = prm(2);
+  const float q[4] = {prm(3), prm(4), prm(5), prm(6)};
+  const float scale[3] = {rgs_expf(prm(7)), rgs_expf(prm(8)), rgs_expf(prm(9))};
+  const float op = sigmoid(prm(10));
+#define OUT(j, val)          \
+  do {                       \
+    if (ADD)                 \
+      out[j] += (val);       \
+    else                     \
+      out[j] = (val);        \
+  } while (0)
+  OUT(10, acc[3] * op * (1.0f - op));
+
+  float pc[3];
+  to_camera(c, px, py, pz, pc);
+  const float tx = pc[0], ty = pc[1], tz = pc[2];
 
   // 2-D covariance as projected (projection.rs:24-69)
   M3 sigma = covariance_3d(q[0], q[1], q[2], q[3], scale[0], scale[1], scale[2]);
@@ -149,6 +157,8 @@ __device__ __forceinline__ void geometry_grad(const CamDev& c, Load prm, const f
     dpos[i] = a;
   }
   // view-dependent colour: dir = (cam - pos)/|cam - pos|
+  float u0 = c.cam_pos[0] - px, u1 = c.cam_pos[1] - py, u2 = c.cam_pos[2] - pz;
+  float ulen = __builtin_sqrtf(u0 * u0 + u1 * u1 + u2 * u2);
   if (ulen > 1e-8f) {
     float e0 = u0 / ulen, e1 = u1 / ulen, e2 = u2 / ulen;
     float dk[16];
@@ -169,13 +179,6 @@ __device__ __forceinline__ void geometry_grad(const CamDev& c, Load prm, const f
   OUT(2, dpos[2]);
   OUT(59, __builtin_sqrtf(asx * asx + asy * asy));
 #undef OUT
-  if (add) {
-#pragma unroll
-    for (int j = 0; j < 60; ++j) out[j] += res[j];
-  } else {
-#pragma unroll
-    for (int j = 0; j < 60; ++j) out[j] = res[j];
-  }
 }
 
 struct SoaLoad {
@@ -201,9 +204,13 @@ __global__ __launch_bounds__(256) void k_geometry_aos(const float* __restrict__ 
   float acc[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) acc[k] = accum[(size_t)k * acc_stride + vi];
-  float g[60];
-  geometry_grad(cams[view], SoaLoad{P, (size_t)cap, (size_t)i}, acc, g, false);
-  for (int j = 0; j < 60; ++j) o[j] = g[j];
+  SoaLoad L{P, (size_t)cap, (size_t)i};
+  const CamDev& c = cams[view];
+  float basis[16], masked[3];
+  view_sh(c, L, acc, basis, masked);
+  for (int ch = 0; ch < 3; ++ch)
+    for (int k = 0; k < 16; ++k) o[P_SH + ch * 16 + k] = masked[ch] * basis[k];
+  geometry_nonsh<false>(c, L, acc, masked, o);
 }
 
 void launch_geometry_aos(const float* P, int cap, int n, const CamDev* cams, int view, const uint32_t* cnt,
@@ -270,7 +277,7 @@ void launch_adam_aos(float* g, const float* grads, size_t n, float* m1, float* m
                      denom, k);
 }
 
-__global__ __launch_bounds__(256) void k_train_adam(float* __restrict__ P, float* __restrict__ M1,
+__global__ __launch_bounds__(256, 2) void k_train_adam(float* __restrict__ P, float* __restrict__ M1,
                                                     float* __restrict__ M2, int cap, int n,
                                                     const CamDev* __restrict__ cams, int n_views,
                                                     const uint32_t* __restrict__ cnt, float* __restrict__ accum,
@@ -281,27 +288,70 @@ __global__ __launch_bounds__(256) void k_train_adam(float* __restrict__ P, float
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (*status & (ST_PAIR_OVERFLOW | ST_NAN_DEPTH)) return;
-  const AdamConsts k = *k_dev;
-  float g[60];
-#pragma unroll
-  for (int j = 0; j < 60; ++j) g[j] = 0.0f;
+  const AdamConsts k_ = *k_dev;
+  // Geometry of every view this Gaussian is visible in, summed; the SH part
+  // one colour channel at a time so that only 16 SH sums are live.
   SoaLoad L{P, (size_t)cap, (size_t)i};
+  float g[11];  // flat parameters 0-10
+#pragma unroll
+  for (int j = 0; j < 11; ++j) g[j] = 0.0f;
+  float sgn = 0.0f;
+  uint32_t vis = 0;
   for (int v = 0; v < n_views; ++v) {
     size_t vi = (size_t)v * cap + i;
     if (cnt[vi] == 0) continue;
+    vis |= 1u << v;
     float acc[9];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) {
-      acc[q] = accum[(size_t)q * acc_stride + vi];
-      accum[(size_t)q * acc_stride + vi] = 0.0f;
+    for (int q = 0; q < 9; ++q) acc[q] = accum[(size_t)q * acc_stride + vi];
+    float basis[16], masked[3];
+    view_sh(cams[v], L, acc, basis, masked);
+    float gg[60];
+    gg[59] = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 11; ++j) gg[j] = g[j];
+    geometry_nonsh<true>(cams[v], L, acc, masked, gg);
+#pragma unroll
+    for (int j = 0; j < 11; ++j) g[j] = gg[j];
+    sgn += gg[59];
+  }
+  for (int ch = 0; ch < 3; ++ch) {
+    float gs[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) gs[k] = 0.0f;
+    for (int v = 0; v < n_views; ++v) {
+      if (!(vis >> v & 1u)) continue;
+      size_t vi = (size_t)v * cap + i;
+      float acc[9];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) acc[q] = accum[(size_t)q * acc_stride + vi];
+      float basis[16], masked[3];
+      view_sh(cams[v], L, acc, basis, masked);
+#pragma unroll
+      for (int k = 0; k < 16; ++k) gs[k] += masked[ch] * basis[k];
     }
-    geometry_grad(cams[v], L, acc, g, true);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      size_t o = (size_t)(P_SH + ch * 16 + k) * cap + i;
+      float mm = M1[o], vv = M2[o];
+      P[o] = adam_one(P[o], gs[k], mm, vv, lr_of(P_SH + ch * 16 + k, k_), k_);
+      M1[o] = mm;
+      M2[o] = vv;
+    }
+  }
+  // the raster accumulators of this Gaussian are consumed: clear them
+  for (int v = 0; v < n_views; ++v) {
+    if (!(vis >> v & 1u)) continue;
+    size_t vi = (size_t)v * cap + i;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) accum[(size_t)q * acc_stride + vi] = 0.0f;
   }
   float rot[4];
-  for (int j = 0; j < 59; ++j) {
+#pragma unroll
+  for (int j = 0; j < 11; ++j) {
     size_t o = (size_t)j * cap + i;
     float mm = M1[o], vv = M2[o];
-    float p = adam_one(P[o], g[j], mm, vv, lr_of(j, k), k);
+    float p = adam_one(P[o], g[j], mm, vv, lr_of(j, k_), k_);
     M1[o] = mm;
     M2[o] = vv;
     if (j >= P_ROT && j < P_ROT + 4)
@@ -311,7 +361,7 @@ __global__ __launch_bounds__(256) void k_train_adam(float* __restrict__ P, float
   }
   renorm_quat(rot[0], rot[1], rot[2], rot[3]);
   for (int q = 0; q < 4; ++q) P[(size_t)(P_ROT + q) * cap + i] = rot[q];
-  gaccum[i] += g[59];
+  gaccum[i] += sgn;
   gdenom[i] += (uint32_t)n_views;
 }
 

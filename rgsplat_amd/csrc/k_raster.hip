@@ -24,6 +24,21 @@ __device__ __forceinline__ int view_of_tile(const CamDev* cams, int n_views, int
   return v;
 }
 
+// Bit w of the result: the splat's alpha-reach box meets the pixel centres
+// of wave w (rows 4w..4w+3 of the tile). NaN inputs give "meets".
+__device__ __forceinline__ uint32_t wave_mask(float sx, float sy, float2 reach, int tx, int ty) {
+  const float x_lo = (float)(tx * TILE) + 0.5f, x_hi = (float)(tx * TILE) + 15.5f;
+  const bool out_x = (sx + reach.x < x_lo) || (sx - reach.x > x_hi);
+  uint32_t m = 0;
+#pragma unroll
+  for (int w = 0; w < RT / 64; ++w) {
+    const float y_lo = (float)(ty * TILE + 4 * w) + 0.5f, y_hi = (float)(ty * TILE + 4 * w) + 3.5f;
+    const bool out = out_x || (sy + reach.y < y_lo) || (sy - reach.y > y_hi);
+    m |= out ? 0u : (1u << w);
+  }
+  return m;
+}
+
 template <int MODE>
 __global__ __launch_bounds__(RT) void k_raster_fwd(RasterIn in, float* __restrict__ out_rgb,
                                                    float* __restrict__ out_t, int32_t* __restrict__ out_last) {
@@ -31,7 +46,9 @@ __global__ __launch_bounds__(RT) void k_raster_fwd(RasterIn in, float* __restric
   __shared__ float4 s_a[RT];
   __shared__ float4 s_b[RT];
   __shared__ float s_c[RT];
+  __shared__ uint32_t s_m[RT];
   const int t = threadIdx.x;
+  const uint32_t wbit = 1u << (t >> 6);
   if (t < 32) s_tab[t] = kExp2Tab[t];
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
   const int v = view_of_tile(in.cams, in.n_views, tile);
@@ -52,14 +69,17 @@ __global__ __launch_bounds__(RT) void k_raster_fwd(RasterIn in, float* __restric
     uint32_t idx = b0 + t;
     if (idx < end) {
       uint32_t vi = in.pair_vals[idx];
-      s_a[t] = in.S.xy_ab[vi];
+      float4 A = in.S.xy_ab[vi];
+      s_a[t] = A;
       s_b[t] = in.S.c_op_rg[vi];
       s_c[t] = in.S.col_b[vi];
+      s_m[t] = wave_mask(A.x, A.y, in.S.reach[vi], tx, ty);
     }
     __syncthreads();
     const int nb = (int)min((uint32_t)RT, end - b0);
     if (!done) {
       for (int j = 0; j < nb; ++j) {
+        if (!(s_m[j] & wbit)) continue;  // wave-uniform
         const float4 A = s_a[j];
         const float4 B = s_b[j];
         float dx = fpx - A.x;
@@ -112,10 +132,30 @@ void launch_raster_fwd(const RasterIn& in, int mode, float* rgb, float* final_t,
     hipLaunchKernelGGL(k_raster_fwd<1>, dim3(in.total_tiles), dim3(RT), 0, st, in, rgb, final_t, last);
 }
 
-__device__ __forceinline__ float wave_sum(float x) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
-  return x;
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float read_lane_f(float v, int lane) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+// Sum over the 64 lanes (all lanes active): butterfly inside each row of 16
+// with DPP, then the four row sums through readlane. Wave-uniform result.
+__device__ __forceinline__ float wave_sum(float v) {
+  v += dpp_f<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f<0x141>(v);  // row_half_mirror
+  v += dpp_f<0x140>(v);  // row_mirror
+  return (read_lane_f(v, 0) + read_lane_f(v, 16)) + (read_lane_f(v, 32) + read_lane_f(v, 48));
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+  v = max(v, __shfl_xor(v, 1, 64));
+  v = max(v, __shfl_xor(v, 2, 64));
+  v = max(v, __shfl_xor(v, 4, 64));
+  v = max(v, __shfl_xor(v, 8, 64));
+  v = max(v, __shfl_xor(v, 16, 64));
+  v = max(v, __shfl_xor(v, 32, 64));
+  return v;
 }
 
 __global__ __launch_bounds__(RT) void k_raster_bwd(RasterIn in, const float* __restrict__ final_t,
@@ -127,9 +167,12 @@ __global__ __launch_bounds__(RT) void k_raster_bwd(RasterIn in, const float* __r
   __shared__ float4 s_b[RT];
   __shared__ float s_c[RT];
   __shared__ uint32_t s_vi[RT];
+  __shared__ uint32_t s_m[RT];
   __shared__ float s_g[9][RT];
   __shared__ int s_max;
   const int t = threadIdx.x;
+  const int lane = t & 63;
+  const uint32_t wbit = 1u << (t >> 6);
   if (t < 32) s_tab[t] = kExp2Tab[t];
   if (t == 0) s_max = -1;
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
@@ -152,8 +195,9 @@ __global__ __launch_bounds__(RT) void k_raster_bwd(RasterIn in, const float* __r
   }
   const float bg0 = in.bg[0], bg1 = in.bg[1], bg2 = in.bg[2];
   float bh0 = T * bg0, bh1 = T * bg1, bh2 = T * bg2;
+  const int wave_lc = wave_max_i(lc);
   __syncthreads();
-  if (lc >= 0) atomicMax(&s_max, lc);
+  if (lane == 0 && wave_lc >= 0) atomicMax(&s_max, wave_lc);
   __syncthreads();
   const int top = s_max;
   if (top < 0) return;
@@ -165,14 +209,19 @@ __global__ __launch_bounds__(RT) void k_raster_bwd(RasterIn in, const float* __r
     if (t < nb) {
       uint32_t vi = in.pair_vals[start + lo + t];
       s_vi[t] = vi;
-      s_a[t] = in.S.xy_ab[vi];
+      float4 A = in.S.xy_ab[vi];
+      s_a[t] = A;
       s_b[t] = in.S.c_op_rg[vi];
       s_c[t] = in.S.col_b[vi];
+      s_m[t] = wave_mask(A.x, A.y, in.S.reach[vi], tx, ty);
     }
 #pragma unroll
     for (int k = 0; k < 9; ++k) s_g[k][t] = 0.0f;
     __syncthreads();
-    for (int j = nb - 1; j >= 0; --j) {
+    // list entries above this wave's last contributor cannot contribute
+    const int jtop = min(nb - 1, wave_lc - lo);
+    for (int j = jtop; j >= 0; --j) {
+      if (!(s_m[j] & wbit)) continue;  // wave-uniform
       const int li = lo + j;
       float g[9];
 #pragma unroll
@@ -191,15 +240,14 @@ __global__ __launch_bounds__(RT) void k_raster_bwd(RasterIn in, const float* __r
           float alpha = fmin_(alpha_raw, MAX_ALPHA);
           if (alpha >= ALPHA_THRESHOLD) {
             contrib = true;
-            float one_m = 1.0f - alpha;
-            T = T / one_m;
+            // transmittance in front of this splat (only gradients use it)
+            float inv_1ma = __builtin_amdgcn_rcpf(1.0f - alpha);
+            T = T * inv_1ma;
             float weight = alpha * T;
             g[0] = dl0 * weight;
             g[1] = dl1 * weight;
             g[2] = dl2 * weight;
-            float inv_1ma = 1.0f / one_m;
-            float dl_dalpha = 0.0f;
-            dl_dalpha += dl0 * (T * B.z - bh0 * inv_1ma);
+            float dl_dalpha = dl0 * (T * B.z - bh0 * inv_1ma);
             dl_dalpha += dl1 * (T * B.w - bh1 * inv_1ma);
             dl_dalpha += dl2 * (T * cb - bh2 * inv_1ma);
             float dl_draw = alpha_raw > MAX_ALPHA ? 0.0f : dl_dalpha;
@@ -216,11 +264,18 @@ __global__ __launch_bounds__(RT) void k_raster_bwd(RasterIn in, const float* __r
           }
         }
       }
-      if (__any(contrib)) {
+      const uint64_t m = __ballot(contrib);
+      if (m == 0) continue;
+      if (__popcll(m) <= 2) {
+        if (contrib) {
+#pragma unroll
+          for (int k = 0; k < 9; ++k) atomicAdd(&s_g[k][j], g[k]);
+        }
+      } else {
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
           float s = wave_sum(g[k]);
-          if ((t & 63) == 0) atomicAdd(&s_g[k][j], s);
+          if (lane == 0) atomicAdd(&s_g[k][j], s);
         }
       }
     }

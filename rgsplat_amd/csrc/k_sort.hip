@@ -329,16 +329,32 @@ void scan_u64(const uint64_t* in, size_t n, uint64_t* out, uint64_t* block_sums,
 }
 
 // ------------------------------------------------------------ pairs/ranges
-__global__ __launch_bounds__(256) void k_gen_pairs(const uint32_t* __restrict__ sorted_vals, size_t n_sorted,
-                                                   const uint32_t* __restrict__ offsets,
-                                                   const uint32_t* __restrict__ total, const uint32_t* __restrict__ cnt,
-                                                   const uint2* __restrict__ rect, int cap,
-                                                   const CamDev* __restrict__ cams, uint32_t* __restrict__ pair_keys,
-                                                   uint32_t* __restrict__ pair_vals, size_t k_cap,
-                                                   uint32_t* __restrict__ k_out, uint32_t* __restrict__ status) {
-  size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r == 0) {
-    uint32_t tot = *total;
+// One block per 256 consecutive splats of the depth order. The block's pairs
+// [off[r0], off[r0 + 256]) are spread over its threads with stride 256 (a
+// binary search over the 257 staged offsets finds each pair's splat), so
+// writes are coalesced and a splat covering hundreds of tiles does not stall
+// one thread. Inside a splat, pairs follow the reference's push order: tile
+// rows, then columns (backward.rs:200-204).
+constexpr int GP_SPLATS = 256;
+
+__global__ __launch_bounds__(GP_SPLATS) void k_gen_pairs(const uint32_t* __restrict__ sorted_vals, size_t n_sorted,
+                                                         const uint32_t* __restrict__ offsets,
+                                                         const uint32_t* __restrict__ total,
+                                                         const uint32_t* __restrict__ cnt,
+                                                         const uint2* __restrict__ rect, int cap,
+                                                         const CamDev* __restrict__ cams,
+                                                         uint32_t* __restrict__ pair_keys,
+                                                         uint32_t* __restrict__ pair_vals, size_t k_cap,
+                                                         uint32_t* __restrict__ k_out, uint32_t* __restrict__ status) {
+  __shared__ uint32_t s_off[GP_SPLATS + 1];
+  __shared__ uint32_t s_vi[GP_SPLATS];
+  __shared__ uint32_t s_org[GP_SPLATS];  // tx0 | ty0 << 16
+  __shared__ uint32_t s_w[GP_SPLATS];    // rect width in tiles
+  __shared__ uint32_t s_tb[GP_SPLATS];   // tile base of the splat's view
+  __shared__ uint32_t s_ntx[GP_SPLATS];
+  const int t = threadIdx.x;
+  const uint32_t tot = *total;
+  if (blockIdx.x == 0 && t == 0) {
     if ((size_t)tot > k_cap) {
       atomicOr(status, ST_PAIR_OVERFLOW);
       *k_out = (uint32_t)k_cap;
@@ -346,33 +362,52 @@ __global__ __launch_bounds__(256) void k_gen_pairs(const uint32_t* __restrict__ 
       *k_out = tot;
     }
   }
-  if (r >= n_sorted) return;
-  uint32_t vi = sorted_vals[r];
-  uint32_t c = cnt[vi];
-  if (c == 0) return;
-  size_t off = offsets[r];
-  uint32_t v = vi / (uint32_t)cap;
-  const CamDev& cm = cams[v];
-  uint2 rc = rect[vi];
-  int tx0 = rc.x & 0xffff, ty0 = rc.x >> 16, tx1 = rc.y & 0xffff, ty1 = rc.y >> 16;
-  uint32_t tb = (uint32_t)cm.tile_base;
-  for (int ty = ty0; ty < ty1; ++ty)
-    for (int tx = tx0; tx < tx1; ++tx) {
-      if (off < k_cap) {
-        pair_keys[off] = tb + (uint32_t)(ty * cm.ntx + tx);
-        pair_vals[off] = vi;
-      }
-      ++off;
+  const size_t r0 = (size_t)blockIdx.x * GP_SPLATS;
+  const size_t r = r0 + t;
+  if (r < n_sorted) {
+    uint32_t vi = sorted_vals[r];
+    s_off[t] = offsets[r];
+    s_vi[t] = vi;
+    uint2 rc = rect[vi];
+    uint32_t c = cnt[vi];
+    s_org[t] = rc.x;
+    s_w[t] = c ? (rc.y & 0xffffu) - (rc.x & 0xffffu) : 1u;
+    const CamDev& cm = cams[vi / (uint32_t)cap];
+    s_tb[t] = (uint32_t)cm.tile_base;
+    s_ntx[t] = (uint32_t)cm.ntx;
+  } else {
+    s_off[t] = tot;
+  }
+  if (t == 0) s_off[GP_SPLATS] = (r0 + GP_SPLATS < n_sorted) ? offsets[r0 + GP_SPLATS] : tot;
+  __syncthreads();
+  const uint32_t p0 = s_off[0], p1 = s_off[GP_SPLATS];
+  for (uint32_t p = p0 + t; p < p1; p += GP_SPLATS) {
+    // largest j with s_off[j] <= p (skips splats with no tiles)
+    int lo = 0, hi = GP_SPLATS - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (s_off[mid] <= p) lo = mid;
+      else hi = mid - 1;
     }
+    const uint32_t local = p - s_off[lo];
+    const uint32_t w = s_w[lo];
+    const uint32_t org = s_org[lo];
+    const uint32_t ty = (org >> 16) + local / w;
+    const uint32_t tx = (org & 0xffffu) + local % w;
+    if ((size_t)p < k_cap) {
+      pair_keys[p] = s_tb[lo] + ty * s_ntx[lo] + tx;
+      pair_vals[p] = s_vi[lo];
+    }
+  }
 }
 
 void launch_gen_pairs(const uint32_t* sorted_vals, size_t n_sorted, const uint32_t* offsets, const uint32_t* total,
                       const SplatArrays S, int cap, const CamDev* cams, uint32_t* pair_keys, uint32_t* pair_vals,
                       size_t k_cap, uint32_t* k_out, uint32_t* status, hipStream_t st) {
-  size_t nb = (n_sorted + 255) / 256;
+  size_t nb = (n_sorted + GP_SPLATS - 1) / GP_SPLATS;
   if (nb == 0) nb = 1;
-  hipLaunchKernelGGL(k_gen_pairs, dim3((unsigned)nb), dim3(256), 0, st, sorted_vals, n_sorted, offsets, total, S.cnt,
-                     S.rect, cap, cams, pair_keys, pair_vals, k_cap, k_out, status);
+  hipLaunchKernelGGL(k_gen_pairs, dim3((unsigned)nb), dim3(GP_SPLATS), 0, st, sorted_vals, n_sorted, offsets, total,
+                     S.cnt, S.rect, cap, cams, pair_keys, pair_vals, k_cap, k_out, status);
 }
 
 __global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ k_dev,

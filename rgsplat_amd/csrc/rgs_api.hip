@@ -178,7 +178,7 @@ struct rgs_ctx {
   hipStream_t st = nullptr;
 
   // render workspace (shared by every entry point)
-  DBuf cams_d, xy_ab, c_op_rg, col_b, cnt, rect;
+  DBuf cams_d, xy_ab, c_op_rg, col_b, cnt, rect, reach;
   DBuf dk[2], dv[2], offsets, bsums, scalars;
   DBuf pk[2], pv[2], hist, rowtot;
   DBuf rstart, rend, rgb, final_t, last, dl, maps, partial, loss;
@@ -264,6 +264,7 @@ int ensure_render(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, size_t k_cap
   CHK(ensure(c, c->col_b, vc * sizeof(float)));
   CHK(ensure(c, c->cnt, vc * sizeof(uint32_t)));
   CHK(ensure(c, c->rect, vc * sizeof(uint2)));
+  CHK(ensure(c, c->reach, vc * sizeof(float2)));
   for (int i = 0; i < 2; ++i) {
     CHK(ensure(c, c->dk[i], dn * 4));
     CHK(ensure(c, c->dv[i], dn * 4));
@@ -290,7 +291,7 @@ int ensure_render(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, size_t k_cap
 
 SplatArrays splats(rgs_ctx* c) {
   return SplatArrays{P_<float4>(c->xy_ab), P_<float4>(c->c_op_rg), P_<float>(c->col_b), P_<uint32_t>(c->cnt),
-                     P_<uint2>(c->rect)};
+                     P_<uint2>(c->rect), P_<float2>(c->reach)};
 }
 
 // Projection, depth sort and the pair-count scan (sizes fixed by n and vs).
@@ -476,7 +477,7 @@ void rgs_destroy(rgs_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->st) (void)hipStreamSynchronize(c->st);
   drop_graph(c);
-  DBuf* all[] = {&c->cams_d, &c->xy_ab, &c->c_op_rg, &c->col_b, &c->cnt, &c->rect, &c->dk[0], &c->dk[1],
+  DBuf* all[] = {&c->cams_d, &c->reach, &c->xy_ab, &c->c_op_rg, &c->col_b, &c->cnt, &c->rect, &c->dk[0], &c->dk[1],
                  &c->dv[0], &c->dv[1], &c->offsets, &c->bsums, &c->scalars, &c->pk[0], &c->pk[1], &c->pv[0],
                  &c->pv[1], &c->hist, &c->rowtot, &c->rstart, &c->rend, &c->rgb, &c->final_t, &c->last, &c->dl,
                  &c->maps, &c->partial, &c->loss, &c->accum, &c->api_aos, &c->api_P, &c->api_grads,

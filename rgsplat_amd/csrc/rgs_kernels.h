@@ -16,6 +16,7 @@ struct SplatArrays {
   float* col_b;     // blue
   uint32_t* cnt;    // tiles touched (0 when culled)
   uint2* rect;      // (tx0 | ty0 << 16, tx1 | ty1 << 16)
+  float2* reach;    // half-widths of the box outside which alpha < 1/255
 };
 
 // Status bits written by kernels into a device word.

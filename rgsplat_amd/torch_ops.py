@@ -10,6 +10,10 @@ in the reference.
 The library's forward state is single-slot, so backward re-runs the forward
 of its own inputs first; that keeps interleaved forwards (as in gradcheck)
 correct.
+
+torch ships its own HIP runtime (libamdhip64.so.7). Import torch before the
+first use of rgsplat_amd.runtime in a process, so that the library binds to
+the runtime torch already loaded instead of a second copy.
 """
 from __future__ import annotations
 

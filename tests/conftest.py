@@ -11,6 +11,14 @@ if ROOT not in sys.path:
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) to run")
     config.addinivalue_line("markers", "slow: longer CPU test")
+    # torch bundles its own HIP runtime; a process that uses torch tensors
+    # with librgsplat_hip must load torch's runtime first so both share it
+    markexpr = config.getoption("-m") or ""
+    if "gpu" in markexpr and "not gpu" not in markexpr:
+        try:
+            import torch  # noqa: F401
+        except Exception:
+            pass
 
 
 @pytest.fixture(scope="session")
