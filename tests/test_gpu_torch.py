@@ -40,32 +40,39 @@ def test_autograd_render_matches_oracle():
 
 
 def test_autograd_finite_differences():
-    """fp32 central differences of a torch loss on top of the renderer."""
+    """backward.rs:907-982 through the torch binding: one Gaussian, 24x24,
+    L1 against a black target (locally smooth), fp32 central differences
+    with eps 1e-3; rel < 0.05 or abs < 3e-4 (the reference's bars)."""
+    import math
+
     import torch
     from rgsplat_amd import torch_ops
-    cam = S.identity_camera(32, 24)
-    g = np.zeros((2, 59), np.float32)
-    g[:, 4] = 1.0
-    g[0, 0:3] = [0.05, -0.02, 2.0]
-    g[1, 0:3] = [-0.1, 0.05, 2.5]
-    g[:, 3] = 0.5
-    g[:, 8:11] = [[-2.2, -2.0, -2.1], [-2.0, -2.3, -2.2]]
-    g[:, 11] = 0.4
-    g[:, 27] = 0.3
-    g[:, 43] = 0.2
-    w = torch.linspace(0.1, 1.0, 24 * 32 * 3, device="cuda").reshape(24, 32, 3)
+    cam = S.identity_camera(24, 24)
+    g = np.zeros((1, 59), np.float32)
+    g[0, 0:3] = [0.1, -0.05, 2.0]
+    g[0, 3] = math.log(0.6 / 0.4)
+    q = np.array([0.9, 0.1, -0.2, 0.15])
+    g[0, 4:8] = q / np.linalg.norm(q)
+    g[0, 8:11] = [-2.2, -2.0, -2.1]
+    for ch in range(3):
+        g[0, 11 + ch * 16] = 0.4
+        for k in range(1, 16):
+            g[0, 11 + ch * 16 + k] = math.sin((ch * 16 + k) * 0.05) * 0.04
 
     def loss(x):
-        return (torch_ops.render(x, cam) * w).sum()
+        return torch_ops.render(x, cam).abs().mean()
     gt = torch.tensor(g, device="cuda", requires_grad=True)
     loss(gt).backward()
-    an = gt.grad.cpu().numpy()
+    an = gt.grad.cpu().numpy()[0]
     eps = 1e-3
-    for i in range(2):
-        for k in [0, 1, 2, 3, 8, 9, 10, 11, 27]:
-            gp, gm = g.copy(), g.copy()
-            gp[i, k] += eps
-            gm[i, k] -= eps
-            with torch.no_grad():
-                num = (loss(torch.tensor(gp, device="cuda")) - loss(torch.tensor(gm, device="cuda"))).item() / (2 * eps)
-            assert abs(an[i, k] - num) <= 0.02 * max(abs(num), 1e-2), (i, k, an[i, k], num)
+
+    def num(k):
+        gp, gm = g.copy(), g.copy()
+        gp[0, k] += eps
+        gm[0, k] -= eps
+        with torch.no_grad():
+            return (loss(torch.tensor(gp, device="cuda")) - loss(torch.tensor(gm, device="cuda"))).item() / (2 * eps)
+    for k in [0, 1, 2, 3, 8, 9, 10, 11, 12, 27, 44]:
+        n = num(k)
+        rel = abs(an[k] - n) / max(abs(an[k]), abs(n), 1e-4)
+        assert rel < 0.05 or abs(an[k] - n) < 3e-4, (k, an[k], n)
