@@ -196,20 +196,30 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
   }
 }
 
-int radix_sort_pairs(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev, size_t n_max, int bits,
-                     RadixScratch rs, hipStream_t st) {
+static int radix_sort_impl(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev, size_t n_max, int bits,
+                           RadixScratch rs, hipStream_t st, bool first_hist_done) {
   int cur = 0;
   int nb = radix_nb(n_max);
   if (nb == 0) return 0;
   for (int shift = 0; shift < bits; shift += 8) {
-    hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(RS_THREADS), 0, st, keys[cur], n_dev, shift, rs.hist,
-                       rs.nb_max);
+    if (!(shift == 0 && first_hist_done))
+      hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(RS_THREADS), 0, st, keys[cur], n_dev, shift, rs.hist,
+                         rs.nb_max);
     hipLaunchKernelGGL(k_radix_rowscan, dim3(256), dim3(1024), 0, st, rs.hist, n_dev, rs.nb_max, rs.row_total);
     hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(RS_THREADS), 0, st, keys[cur], vals[cur], keys[cur ^ 1],
                        vals[cur ^ 1], n_dev, shift, rs.hist, rs.row_total, rs.nb_max);
     cur ^= 1;
   }
   return cur;
+}
+
+int radix_sort_pairs(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev, size_t n_max, int bits,
+                     RadixScratch rs, hipStream_t st) {
+  return radix_sort_impl(keys, vals, n_dev, n_max, bits, rs, st, false);
+}
+int radix_sort_pairs_prehist(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev, size_t n_max, int bits,
+                             RadixScratch rs, hipStream_t st) {
+  return radix_sort_impl(keys, vals, n_dev, n_max, bits, rs, st, true);
 }
 
 // ------------------------------------------------------------------- scans
@@ -329,96 +339,167 @@ void scan_u64(const uint64_t* in, size_t n, uint64_t* out, uint64_t* block_sums,
 }
 
 // ------------------------------------------------------------ pairs/ranges
-// One block per 256 consecutive splats of the depth order. The block's pairs
-// [off[r0], off[r0 + 256]) are spread over its threads with stride 256 (a
-// binary search over the 257 staged offsets finds each pair's splat), so
-// writes are coalesced and a splat covering hundreds of tiles does not stall
-// one thread. Inside a splat, pairs follow the reference's push order: tile
-// rows, then columns (backward.rs:200-204).
-constexpr int GP_SPLATS = 256;
+// Pair generation. Block b writes pairs [b*RS_TILE, (b+1)*RS_TILE) of the
+// depth-ordered pair sequence (a splat's pairs follow the reference's push
+// order: tile rows, then columns, backward.rs:200-204), so the writes are
+// coalesced, every block has the same amount of work, and the block's keys
+// are exactly the radix sort's tile b, whose low-digit histogram it counts in
+// LDS on the way. first_splat[b] (the splat holding pair b*RS_TILE) comes
+// from k_block_first.
+constexpr int GP_THREADS = 256;
 
-__global__ __launch_bounds__(GP_SPLATS) void k_gen_pairs(const uint32_t* __restrict__ sorted_vals, size_t n_sorted,
-                                                         const uint32_t* __restrict__ offsets,
-                                                         const uint32_t* __restrict__ total,
-                                                         const uint32_t* __restrict__ cnt,
-                                                         const uint2* __restrict__ rect, int cap,
-                                                         const CamDev* __restrict__ cams,
-                                                         uint32_t* __restrict__ pair_keys,
-                                                         uint32_t* __restrict__ pair_vals, size_t k_cap,
-                                                         uint32_t* __restrict__ k_out, uint32_t* __restrict__ status) {
-  __shared__ uint32_t s_off[GP_SPLATS + 1];
-  __shared__ uint32_t s_vi[GP_SPLATS];
-  __shared__ uint32_t s_org[GP_SPLATS];  // tx0 | ty0 << 16
-  __shared__ uint32_t s_w[GP_SPLATS];    // rect width in tiles
-  __shared__ uint32_t s_tb[GP_SPLATS];   // tile base of the splat's view
-  __shared__ uint32_t s_ntx[GP_SPLATS];
+__global__ __launch_bounds__(256) void k_block_first(const uint32_t* __restrict__ offsets, size_t n_sorted,
+                                                     const uint32_t* __restrict__ total, size_t k_cap,
+                                                     uint32_t* __restrict__ first_splat) {
+  size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_sorted) return;
+  const uint32_t tot = *total;
+  uint64_t lo = offsets[r];
+  uint64_t hi = r + 1 < n_sorted ? offsets[r + 1] : tot;
+  hi = hi < k_cap ? hi : k_cap;
+  // blocks whose first pair lies in [lo, hi)
+  for (uint64_t b = (lo + RS_TILE - 1) / RS_TILE; b * RS_TILE < hi; ++b) first_splat[b] = (uint32_t)r;
+}
+
+__global__ __launch_bounds__(GP_THREADS) void k_gen_pairs(const uint32_t* __restrict__ sorted_vals, size_t n_sorted,
+                                                          const uint32_t* __restrict__ offsets,
+                                                          const uint32_t* __restrict__ total,
+                                                          const uint32_t* __restrict__ cnt,
+                                                          const uint2* __restrict__ rect, int cap,
+                                                          const CamDev* __restrict__ cams,
+                                                          uint32_t* __restrict__ pair_keys,
+                                                          uint32_t* __restrict__ pair_vals, size_t k_cap,
+                                                          uint32_t* __restrict__ k_out, uint32_t* __restrict__ status,
+                                                          const uint32_t* __restrict__ first_splat,
+                                                          uint32_t* __restrict__ hist, int nb_max) {
+  __shared__ uint32_t s_off[GP_THREADS + 1];
+  __shared__ uint32_t s_vi[GP_THREADS];
+  __shared__ uint32_t s_org[GP_THREADS];  // tx0 | ty0 << 16
+  __shared__ uint32_t s_w[GP_THREADS];    // rect width in tiles
+  __shared__ uint32_t s_tb[GP_THREADS];   // tile base of the splat's view
+  __shared__ uint32_t s_ntx[GP_THREADS];
+  __shared__ uint32_t s_h[256];
   const int t = threadIdx.x;
   const uint32_t tot = *total;
+  const size_t K = (size_t)tot < k_cap ? (size_t)tot : k_cap;
   if (blockIdx.x == 0 && t == 0) {
-    if ((size_t)tot > k_cap) {
-      atomicOr(status, ST_PAIR_OVERFLOW);
-      *k_out = (uint32_t)k_cap;
+    if ((size_t)tot > k_cap) atomicOr(status, ST_PAIR_OVERFLOW);
+    *k_out = (uint32_t)K;
+  }
+  const size_t b = blockIdx.x;
+  const size_t p_lo = b * RS_TILE;
+  if (p_lo >= K) return;
+  const size_t p_hi = p_lo + RS_TILE < K ? p_lo + RS_TILE : K;
+  s_h[t] = 0;
+  size_t rc = first_splat[b];
+  for (;;) {
+    __syncthreads();
+    const size_t r = rc + t;
+    if (r < n_sorted) {
+      uint32_t vi = sorted_vals[r];
+      s_off[t] = offsets[r];
+      s_vi[t] = vi;
+      uint2 rr = rect[vi];
+      s_org[t] = rr.x;
+      s_w[t] = cnt[vi] ? (rr.y & 0xffffu) - (rr.x & 0xffffu) : 1u;
+      const CamDev& cm = cams[vi / (uint32_t)cap];
+      s_tb[t] = (uint32_t)cm.tile_base;
+      s_ntx[t] = (uint32_t)cm.ntx;
     } else {
-      *k_out = tot;
+      s_off[t] = tot;
     }
-  }
-  const size_t r0 = (size_t)blockIdx.x * GP_SPLATS;
-  const size_t r = r0 + t;
-  if (r < n_sorted) {
-    uint32_t vi = sorted_vals[r];
-    s_off[t] = offsets[r];
-    s_vi[t] = vi;
-    uint2 rc = rect[vi];
-    uint32_t c = cnt[vi];
-    s_org[t] = rc.x;
-    s_w[t] = c ? (rc.y & 0xffffu) - (rc.x & 0xffffu) : 1u;
-    const CamDev& cm = cams[vi / (uint32_t)cap];
-    s_tb[t] = (uint32_t)cm.tile_base;
-    s_ntx[t] = (uint32_t)cm.ntx;
-  } else {
-    s_off[t] = tot;
-  }
-  if (t == 0) s_off[GP_SPLATS] = (r0 + GP_SPLATS < n_sorted) ? offsets[r0 + GP_SPLATS] : tot;
-  __syncthreads();
-  const uint32_t p0 = s_off[0], p1 = s_off[GP_SPLATS];
-  for (uint32_t p = p0 + t; p < p1; p += GP_SPLATS) {
-    // largest j with s_off[j] <= p (skips splats with no tiles)
-    int lo = 0, hi = GP_SPLATS - 1;
-    while (lo < hi) {
-      int mid = (lo + hi + 1) >> 1;
-      if (s_off[mid] <= p) lo = mid;
-      else hi = mid - 1;
-    }
-    const uint32_t local = p - s_off[lo];
-    const uint32_t w = s_w[lo];
-    const uint32_t org = s_org[lo];
-    const uint32_t ty = (org >> 16) + local / w;
-    const uint32_t tx = (org & 0xffffu) + local % w;
-    if ((size_t)p < k_cap) {
-      pair_keys[p] = s_tb[lo] + ty * s_ntx[lo] + tx;
+    if (t == 0) s_off[GP_THREADS] = (rc + GP_THREADS < n_sorted) ? offsets[rc + GP_THREADS] : tot;
+    __syncthreads();
+    const size_t c_lo = (size_t)s_off[0] > p_lo ? (size_t)s_off[0] : p_lo;
+    const size_t c_end = (size_t)s_off[GP_THREADS];
+    const size_t c_hi = c_end < p_hi ? c_end : p_hi;
+    for (size_t p = c_lo + t; p < c_hi; p += GP_THREADS) {
+      // largest j with s_off[j] <= p (skips splats with no tiles)
+      int lo = 0, hi = GP_THREADS - 1;
+      while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if ((size_t)s_off[mid] <= p) lo = mid;
+        else hi = mid - 1;
+      }
+      const uint32_t local = (uint32_t)(p - s_off[lo]);
+      const uint32_t w = s_w[lo];
+      const uint32_t org = s_org[lo];
+      const uint32_t ty = (org >> 16) + local / w;
+      const uint32_t tx = (org & 0xffffu) + local % w;
+      const uint32_t key = s_tb[lo] + ty * s_ntx[lo] + tx;
+      pair_keys[p] = key;
       pair_vals[p] = s_vi[lo];
+      atomicAdd(&s_h[key & 255u], 1u);
     }
+    if (c_end >= p_hi) break;  // uniform: every thread read the same s_off
+    rc += GP_THREADS;
   }
+  __syncthreads();
+  hist[(size_t)t * nb_max + b] = s_h[t];
 }
 
 void launch_gen_pairs(const uint32_t* sorted_vals, size_t n_sorted, const uint32_t* offsets, const uint32_t* total,
                       const SplatArrays S, int cap, const CamDev* cams, uint32_t* pair_keys, uint32_t* pair_vals,
-                      size_t k_cap, uint32_t* k_out, uint32_t* status, hipStream_t st) {
-  size_t nb = (n_sorted + GP_SPLATS - 1) / GP_SPLATS;
+                      size_t k_cap, uint32_t* k_out, uint32_t* status, uint32_t* first_splat, RadixScratch rs,
+                      hipStream_t st) {
+  if (n_sorted)
+    hipLaunchKernelGGL(k_block_first, dim3((unsigned)((n_sorted + 255) / 256)), dim3(256), 0, st, offsets, n_sorted,
+                       total, k_cap, first_splat);
+  size_t nb = (size_t)radix_nb(k_cap);
   if (nb == 0) nb = 1;
-  hipLaunchKernelGGL(k_gen_pairs, dim3((unsigned)nb), dim3(GP_SPLATS), 0, st, sorted_vals, n_sorted, offsets, total,
-                     S.cnt, S.rect, cap, cams, pair_keys, pair_vals, k_cap, k_out, status);
+  hipLaunchKernelGGL(k_gen_pairs, dim3((unsigned)nb), dim3(GP_THREADS), 0, st, sorted_vals, n_sorted, offsets, total,
+                     S.cnt, S.rect, cap, cams, pair_keys, pair_vals, k_cap, k_out, status, first_splat, rs.hist,
+                     rs.nb_max);
 }
 
-__global__ __launch_bounds__(256) void k_ranges(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ k_dev,
-                                                size_t k_cap, uint32_t* __restrict__ rstart,
-                                                uint32_t* __restrict__ rend) {
-  size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  size_t K = *k_dev;
-  if (p >= K) return;
-  uint32_t t = keys[p];
-  if (p == 0 || keys[p - 1] != t) rstart[t] = (uint32_t)p;
-  if (p + 1 == K || keys[p + 1] != t) rend[t] = (uint32_t)(p + 1);
+// ---- tile ranges from the tile rects
+__global__ __launch_bounds__(256) void k_tile_diff(const uint32_t* __restrict__ cnt, const uint2* __restrict__ rect,
+                                                   int cap, int n, const CamDev* __restrict__ cams,
+                                                   int32_t* __restrict__ diff) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int v = blockIdx.y;
+  if (i >= n) return;
+  size_t vi = (size_t)v * cap + i;
+  if (cnt[vi] == 0) return;
+  const CamDev& c = cams[v];
+  uint2 rc = rect[vi];
+  int tx0 = rc.x & 0xffff, ty0 = rc.x >> 16, tx1 = rc.y & 0xffff, ty1 = rc.y >> 16;
+  int32_t* d = diff + c.diff_base;
+  const int W = c.ntx + 1;
+  atomicAdd(&d[ty0 * W + tx0], 1);
+  atomicAdd(&d[ty0 * W + tx1], -1);
+  atomicAdd(&d[ty1 * W + tx0], -1);
+  atomicAdd(&d[ty1 * W + tx1], 1);
+}
+
+// One block per view: 2-D inclusive prefix of the difference grid -> counts.
+__global__ __launch_bounds__(256) void k_tile_prefix(const CamDev* __restrict__ cams, int32_t* __restrict__ diff,
+                                                     uint32_t* __restrict__ counts) {
+  const CamDev& c = cams[blockIdx.x];
+  int32_t* d = diff + c.diff_base;
+  const int W = c.ntx + 1, H = c.nty + 1;
+  for (int y = threadIdx.x; y < H; y += blockDim.x) {
+    int32_t run = 0;
+    for (int x = 0; x < W; ++x) {
+      run += d[y * W + x];
+      d[y * W + x] = run;
+    }
+  }
+  __syncthreads();
+  for (int x = threadIdx.x; x < W; x += blockDim.x) {
+    int32_t run = 0;
+    for (int y = 0; y < H; ++y) {
+      run += d[y * W + x];
+      if (y < c.nty && x < c.ntx) counts[c.tile_base + y * c.ntx + x] = (uint32_t)run;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_range_end(const uint32_t* __restrict__ start,
+                                                   const uint32_t* __restrict__ counts, int n,
+                                                   uint32_t* __restrict__ end) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) end[t] = start[t] + counts[t];
 }
 
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
@@ -438,13 +519,19 @@ void launch_fill_f32(float* p, size_t n, float v, hipStream_t st) {
   hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
 }
 
-void launch_ranges(const uint32_t* pair_keys, const uint32_t* k_dev, size_t k_cap, int total_tiles,
-                   uint32_t* rstart, uint32_t* rend, hipStream_t st) {
-  launch_fill_u32(rstart, (size_t)total_tiles, 0u, st);
-  launch_fill_u32(rend, (size_t)total_tiles, 0u, st);
-  size_t nb = (k_cap + 255) / 256;
-  if (nb == 0) return;
-  hipLaunchKernelGGL(k_ranges, dim3((unsigned)nb), dim3(256), 0, st, pair_keys, k_dev, k_cap, rstart, rend);
+void launch_tile_ranges(const SplatArrays S, int cap, int n, const CamDev* cams, const CamDev* cams_host,
+                        int n_views, int total_tiles, int32_t* diff, int diff_total, uint32_t* counts,
+                        uint32_t* bsums, uint32_t* tot, uint32_t* rstart, uint32_t* rend, hipStream_t st) {
+  (void)cams_host;
+  if (total_tiles <= 0) return;
+  launch_fill_u32(reinterpret_cast<uint32_t*>(diff), (size_t)diff_total, 0u, st);
+  if (n > 0)
+    hipLaunchKernelGGL(k_tile_diff, dim3((n + 255) / 256, n_views), dim3(256), 0, st, S.cnt, S.rect, cap, n, cams,
+                       diff);
+  hipLaunchKernelGGL(k_tile_prefix, dim3(n_views), dim3(256), 0, st, cams, diff, counts);
+  scan_u32(counts, (size_t)total_tiles, rstart, bsums, tot, st);
+  hipLaunchKernelGGL(k_range_end, dim3((total_tiles + 255) / 256), dim3(256), 0, st, rstart, counts, total_tiles,
+                     rend);
 }
 
 }  // namespace rgs

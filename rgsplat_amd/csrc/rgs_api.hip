@@ -141,6 +141,7 @@ int make_camdev(const rgs_camera& cam, CamDev& out) {
   out.tile_base = 0;
   out.pix_base = 0;
   out.tgt_base = 0;
+  out.diff_base = 0;
   return 0;
 }
 
@@ -149,16 +150,20 @@ struct ViewSet {
   std::vector<CamDev> cams;
   int total_tiles = 0;
   size_t total_pix = 0;
+  int diff_total = 0;
   void layout() {
-    int tb = 0;
+    int tb = 0, db = 0;
     size_t pb = 0;
     for (auto& c : cams) {
       c.tile_base = tb;
       c.pix_base = (int32_t)pb;
+      c.diff_base = db;
       tb += c.ntx * c.nty;
+      db += (c.ntx + 1) * (c.nty + 1);
       pb += (size_t)c.w * c.h;
     }
     total_tiles = tb;
+    diff_total = db;
     total_pix = pb;
   }
   int bits() const {
@@ -168,7 +173,7 @@ struct ViewSet {
   }
 };
 
-enum { SC_DEPTH_N = 0, SC_TOTAL = 1, SC_K = 2, SC_STATUS = 3, SC_NOUT = 4, SC_COUNT = 8 };
+enum { SC_DEPTH_N = 0, SC_TOTAL = 1, SC_K = 2, SC_STATUS = 3, SC_NOUT = 4, SC_TILETOT = 5, SC_COUNT = 8 };
 
 }  // namespace
 
@@ -182,6 +187,7 @@ struct rgs_ctx {
   DBuf dk[2], dv[2], offsets, bsums, scalars;
   DBuf pk[2], pv[2], hist, rowtot;
   DBuf rstart, rend, rgb, final_t, last, dl, maps, partial, loss;
+  DBuf first_splat, tr_diff, tr_counts, tr_bsums;
   DBuf accum;
   size_t k_cap = 0;
   size_t depth_cap = 0;  // capacity (entries) of the depth-sort arrays
@@ -282,6 +288,10 @@ int ensure_render(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, size_t k_cap
   CHK(ensure(c, c->rowtot, 256 * 4));
   CHK(ensure(c, c->rstart, (size_t)vs.total_tiles * 4));
   CHK(ensure(c, c->rend, (size_t)vs.total_tiles * 4));
+  CHK(ensure(c, c->first_splat, ((size_t)radix_nb(k_cap) + 1) * 4));
+  CHK(ensure(c, c->tr_diff, (size_t)vs.diff_total * 4));
+  CHK(ensure(c, c->tr_counts, (size_t)vs.total_tiles * 4));
+  CHK(ensure(c, c->tr_bsums, ((size_t)scan_nb(vs.total_tiles) + 1) * 4));
   CHK(ensure(c, c->rgb, vs.total_pix * 4 * 4));
   CHK(ensure(c, c->final_t, vs.total_pix * 4));
   CHK(ensure(c, c->last, vs.total_pix * 4));
@@ -324,21 +334,22 @@ void enqueue_bin_and_raster(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, co
   hipStream_t st = c->st;
   uint32_t* sc = P_<uint32_t>(c->scalars);
   size_t dn = vs.cams.size() * (size_t)n;
+  RadixScratch rs{P_<uint32_t>(c->hist), P_<uint32_t>(c->rowtot), c->hist_nb};
+  stage_begin(c, S_RANGES);
+  launch_tile_ranges(splats(c), (int)cap, n, P_<CamDev>(c->cams_d), vs.cams.data(), (int)vs.cams.size(),
+                     vs.total_tiles, P_<int32_t>(c->tr_diff), vs.diff_total, P_<uint32_t>(c->tr_counts),
+                     P_<uint32_t>(c->tr_bsums), sc + SC_TILETOT, P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend), st);
+  stage_end(c, S_RANGES);
   stage_begin(c, S_GEN);
   launch_gen_pairs(c->sorted_vals, dn, P_<uint32_t>(c->offsets), sc + SC_TOTAL, splats(c), (int)cap,
                    P_<CamDev>(c->cams_d), P_<uint32_t>(c->pk[0]), P_<uint32_t>(c->pv[0]), c->k_cap, sc + SC_K,
-                   sc + SC_STATUS, st);
+                   sc + SC_STATUS, P_<uint32_t>(c->first_splat), rs, st);
   stage_end(c, S_GEN);
   uint32_t* keys[2] = {P_<uint32_t>(c->pk[0]), P_<uint32_t>(c->pk[1])};
   uint32_t* vals[2] = {P_<uint32_t>(c->pv[0]), P_<uint32_t>(c->pv[1])};
-  RadixScratch rs{P_<uint32_t>(c->hist), P_<uint32_t>(c->rowtot), c->hist_nb};
   stage_begin(c, S_TSORT);
-  int cur = radix_sort_pairs(keys, vals, sc + SC_K, c->k_cap, vs.bits(), rs, st);
+  int cur = radix_sort_pairs_prehist(keys, vals, sc + SC_K, c->k_cap, vs.bits(), rs, st);
   stage_end(c, S_TSORT);
-  stage_begin(c, S_RANGES);
-  launch_ranges(keys[cur], sc + SC_K, c->k_cap, vs.total_tiles, P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend),
-                st);
-  stage_end(c, S_RANGES);
   c->pair_vals = vals[cur];
   RasterIn in{P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend), c->pair_vals, splats(c), P_<CamDev>(c->cams_d),
               (int)vs.cams.size(), vs.total_tiles, {bg[0], bg[1], bg[2]}};
@@ -483,7 +494,7 @@ void rgs_destroy(rgs_ctx* c) {
                  &c->maps, &c->partial, &c->loss, &c->accum, &c->api_aos, &c->api_P, &c->api_grads,
                  &c->api_target, &c->api_m1, &c->api_m2, &c->api_acc, &c->api_den, &c->api_P2, &c->dens_want,
                  &c->dens_cnt, &c->dens_W, &c->dens_packed, &c->dens_prefix, &c->dens_b32, &c->dens_b64,
-                 &c->dens_t32, &c->dens_t64, &c->P, &c->P2, &c->M1, &c->M2, &c->gacc, &c->gden, &c->targets,
+                 &c->dens_t32, &c->dens_t64, &c->first_splat, &c->tr_diff, &c->tr_counts, &c->tr_bsums, &c->P, &c->P2, &c->M1, &c->M2, &c->gacc, &c->gden, &c->targets,
                  &c->adam_d, &c->api_acc9};
   for (DBuf* b : all)
     if (b->p) (void)hipFree(b->p);

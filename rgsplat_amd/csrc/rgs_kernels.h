@@ -57,15 +57,28 @@ void scan_u32(const uint32_t* in, size_t n, uint32_t* out, uint32_t* block_sums,
 // Exclusive scan of a u64 array.
 void scan_u64(const uint64_t* in, size_t n, uint64_t* out, uint64_t* block_sums, uint64_t* total, hipStream_t st);
 
-// Writes the (tile, splat) pairs in depth order; K = min(total, k_cap)
-// written to *k_out, ST_PAIR_OVERFLOW raised when total > k_cap.
+// Tile ranges from the tile rects, before any pair exists: per-view 2-D
+// difference grids (4 atomics per visible splat), their 2-D prefix sums give
+// each tile's pair count, and an exclusive scan over global tile ids gives
+// [start, end) -- the positions the tile sort will produce. diff needs
+// sum_v (nty_v+1)(ntx_v+1) ints; counts/bsums as for scan_u32.
+void launch_tile_ranges(const SplatArrays S, int cap, int n, const CamDev* cams, const CamDev* cams_host,
+                        int n_views, int total_tiles, int32_t* diff, int diff_total, uint32_t* counts,
+                        uint32_t* bsums, uint32_t* tot, uint32_t* rstart, uint32_t* rend, hipStream_t st);
+
+// Writes the (tile, splat) pairs in depth order, one radix tile (RS_TILE
+// pairs) per block, and the first radix pass's per-tile digit histogram
+// (hist[d * nb_max + b] for the low 8 key bits). K = min(total, k_cap) goes to
+// *k_out; ST_PAIR_OVERFLOW is raised when total > k_cap. first_splat needs
+// radix_nb(k_cap) entries.
 void launch_gen_pairs(const uint32_t* sorted_vals, size_t n_sorted, const uint32_t* offsets, const uint32_t* total,
                       const SplatArrays S, int cap, const CamDev* cams, uint32_t* pair_keys, uint32_t* pair_vals,
-                      size_t k_cap, uint32_t* k_out, uint32_t* status, hipStream_t st);
-// Tile ranges [start, end) over the sorted pair keys (tiles with no pairs get
-// start = end = 0).
-void launch_ranges(const uint32_t* pair_keys, const uint32_t* k_dev, size_t k_cap, int total_tiles,
-                   uint32_t* rstart, uint32_t* rend, hipStream_t st);
+                      size_t k_cap, uint32_t* k_out, uint32_t* status, uint32_t* first_splat, RadixScratch rs,
+                      hipStream_t st);
+// Radix sort whose first pass's histogram was already written (by
+// launch_gen_pairs).
+int radix_sort_pairs_prehist(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev, size_t n_max, int bits,
+                             RadixScratch rs, hipStream_t st);
 
 // ---- rasterization (k_raster.hip)
 struct RasterIn {
