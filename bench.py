@@ -44,6 +44,23 @@ BWD_OPS_PER_PAIR = 62
 SSIM_FLOP_PER_PIXEL = 1440
 
 
+def measured_traffic(workload: str) -> dict:
+    """HBM bytes per launch of each stage's main kernel, from the committed
+    rocprofv3 PMC summary of this workload (FETCH_SIZE and WRITE_SIZE passes,
+    each scaled by the counter's reading for a streaming kernel of known
+    size). Empty when no summary exists for the workload."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.md")
+    if not os.path.exists(path):
+        return {}
+    text = open(path).read()
+    start = text.find("```json")
+    if start < 0:
+        return {}
+    body = text[start + 7:text.index("```", start + 7)]
+    table = json.loads(body)
+    return {stage: rec["traffic_bytes"] for stage, rec in table.get("stages", {}).items()}
+
+
 def sync():
     """Device-wide barrier. Every train_step already ends with a stream
     synchronisation; torch's is added when torch is loaded in the process."""
@@ -143,9 +160,17 @@ def main():
     ctx.set_views(cams, targets)
     ids = list(range(len(cams)))
     use_graph = not args.no_graph
+    # The steps carry HIP events around every stage (in graph mode they are
+    # event nodes of the captured step), so the per-kernel durations below
+    # come from the timed region itself.
+    profile = not args.no_profile
+    if profile:
+        ctx.set_profiling(True)
     print("[bench] warmup", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         ctx.train_step(ids, cfg, use_graph=use_graph)
+    if profile:
+        ctx.set_profiling(True)  # restart the stage counters for the timed steps
     sync()
     t0 = time.perf_counter()
     losses = None
@@ -164,12 +189,17 @@ def main():
                       "execution": "device-resident step, views batched per launch, "
                                    + ("hipGraph replay" if use_graph else "eager")}}
 
-    if not args.no_profile:
-        ctx.set_profiling(True)
-        for _ in range(3):
-            ctx.train_step(ids, cfg, use_graph=False)
-        stages, n_prof = ctx.stage_times()
+    if profile:
+        stages, n_prof, n_graph = ctx.stage_times()
         ctx.set_profiling(False)
+        # the same steps without the stage events, for the events' overhead
+        ctx.train_step(ids, cfg, use_graph=use_graph)
+        sync()
+        u0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.train_step(ids, cfg, use_graph=use_graph)
+        sync()
+        out["uninstrumented_iters_per_s"] = args.steps / (time.perf_counter() - u0)
         K, pf, pb = ctx.step_work()
         npix = sum(w * h for w, h in wl.sizes)
         ng = ctx.scene_size()
@@ -180,6 +210,7 @@ def main():
             "project": ("hbm", 236 * ng * len(cams) + 52 * ng * len(cams), "GB/s"),
             "geometry_adam": ("hbm", 1652 * ng, "GB/s"),
         }
+        traffic = measured_traffic(wl.name)
         dom = max(stages, key=lambda s: stages[s])
         rl = {}
         for s, (bound, amount, unit) in work.items():
@@ -191,16 +222,18 @@ def main():
                 ach = amount / sec / 1e12 if sec > 0 else 0.0
                 peak = FP32_PEAK_TFLOPS
             rl[s] = {"bound": bound, "achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak,
-                     "traffic": None, "ms": stages[s]}
+                     "traffic": traffic.get(s), "ms": stages[s]}
         if dom in rl:
             out["roofline"] = dict(rl[dom], kernel=dom)
+            if traffic.get(dom) is not None:
+                out["roofline"]["traffic_source"] = f"profiles/hbm_traffic_{wl.name}.md (rocprofv3 FETCH_SIZE/WRITE_SIZE)"
         else:
             out["roofline"] = {"kernel": dom, "bound": None, "achieved": None, "peak": None, "unit": None,
-                               "frac": None, "traffic": None, "ms": stages[dom]}
+                               "frac": None, "traffic": traffic.get(dom), "ms": stages[dom]}
         out["stages_ms"] = stages
         out["stage_rooflines"] = rl
         out["work"] = {"tile_pairs": K, "fwd_pair_evals": pf, "bwd_pair_evals": pb, "gaussians": ng,
-                       "profiled_steps": n_prof}
+                       "timed_steps_with_events": n_prof, "of_which_graph_replays": n_graph}
         print(f"[bench] stages {json.dumps(stages)}", file=sys.stderr, flush=True)
 
     if not args.no_plain:

@@ -85,9 +85,12 @@ typedef struct rgs_ctx rgs_ctx;
 int rgs_create(int device, rgs_ctx** out);
 void rgs_destroy(rgs_ctx* ctx);
 const char* rgs_last_error(void);
-/* Use an external HIP stream (e.g. a framework's current stream); NULL
- * restores the context's own stream. */
+/* Run all later work on an external HIP stream, e.g. a framework's current
+ * stream; NULL is the HIP null stream (a framework's default stream, whose
+ * handle is 0). rgs_train_step does not capture graphs on the null stream.
+ * rgs_use_own_stream returns to the context's own (non-blocking) stream. */
 int rgs_set_stream(rgs_ctx* ctx, void* hip_stream);
+int rgs_use_own_stream(rgs_ctx* ctx);
 
 /* render_and_backward (cpu/backward.rs:107-149). target_rgba is w*h*4 f32;
  * grads_out has n entries, zero for culled Gaussians. */
@@ -161,15 +164,18 @@ int rgs_step_stats(rgs_ctx* ctx, uint64_t* n_pairs, uint64_t* fwd_evals, uint64_
 int rgs_set_raster_batch(rgs_ctx* ctx, int batch);
 
 /* ---- Stage timing (no reference counterpart). With profiling on,
- * rgs_train_step runs its kernels eagerly and brackets each stage with HIP
- * events on the context's stream. Stages, in order: 0 project, 1 depth sort,
- * 2 pair-count scan, 3 pair generation, 4 tile sort, 5 tile ranges,
- * 6 forward raster, 7 L1+D-SSIM, 8 backward raster, 9 geometry + Adam.
+ * rgs_train_step brackets each stage with HIP events on the context's stream;
+ * in graph mode the events are nodes of the captured graph, so replays are
+ * timed as they run. Stages, in order: 0 project, 1 depth sort,
+ * 2 pair-count scan, 3 pair generation, 4 tile sort (its last pass also
+ * emits the tile ranges), 5 forward raster, 6 L1+D-SSIM, 7 backward raster,
+ * 8 geometry + Adam.
  * rgs_stage_times returns the mean milliseconds per stage since profiling
- * was enabled and the number of profiled steps. */
-#define RGS_N_STAGES 10
+ * was enabled, the number of profiled steps and how many of them were graph
+ * replays. */
+#define RGS_N_STAGES 9
 int rgs_set_profiling(rgs_ctx* ctx, int enable);
-int rgs_stage_times(rgs_ctx* ctx, float* mean_ms, int n_stages, uint64_t* steps);
+int rgs_stage_times(rgs_ctx* ctx, float* mean_ms, int n_stages, uint64_t* steps, uint64_t* graph_steps);
 
 /* ---- Diagnostics (no reference counterpart; used by the parity tests).
  * rgs_eval_expf evaluates the device expf every kernel uses on n host

@@ -191,7 +191,7 @@ struct SoaLoad {
 __global__ __launch_bounds__(256) void k_geometry_aos(const float* __restrict__ P, int cap, int n,
                                                       const CamDev* __restrict__ cams, int view,
                                                       const uint32_t* __restrict__ cnt,
-                                                      const float* __restrict__ accum, size_t acc_stride,
+                                                      const float* __restrict__ accum,
                                                       float* __restrict__ grads) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(256) void k_geometry_aos(const float* __restrict__ 
   }
   float acc[9];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) acc[k] = accum[(size_t)k * acc_stride + vi];
+  for (int k = 0; k < 9; ++k) acc[k] = accum[vi * 9 + k];
   SoaLoad L{P, (size_t)cap, (size_t)i};
   const CamDev& c = cams[view];
   float basis[16], masked[3];
@@ -214,10 +214,10 @@ __global__ __launch_bounds__(256) void k_geometry_aos(const float* __restrict__ 
 }
 
 void launch_geometry_aos(const float* P, int cap, int n, const CamDev* cams, int view, const uint32_t* cnt,
-                         const float* accum, size_t acc_stride, float* grads, hipStream_t st) {
+                         const float* accum, float* grads, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_geometry_aos, dim3((n + 255) / 256), dim3(256), 0, st, P, cap, n, cams, view, cnt, accum,
-                     acc_stride, grads);
+                     grads);
 }
 
 __device__ __forceinline__ float lr_of(int j, const AdamConsts& k) {
@@ -281,7 +281,7 @@ __global__ __launch_bounds__(256, 2) void k_train_adam(float* __restrict__ P, fl
                                                     float* __restrict__ M2, int cap, int n,
                                                     const CamDev* __restrict__ cams, int n_views,
                                                     const uint32_t* __restrict__ cnt, float* __restrict__ accum,
-                                                    size_t acc_stride, float* __restrict__ gaccum,
+                                                    float* __restrict__ gaccum,
                                                     uint32_t* __restrict__ gdenom,
                                                     const AdamConsts* __restrict__ k_dev,
                                                     const uint32_t* __restrict__ status) {
@@ -303,7 +303,7 @@ __global__ __launch_bounds__(256, 2) void k_train_adam(float* __restrict__ P, fl
     vis |= 1u << v;
     float acc[9];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) acc[q] = accum[(size_t)q * acc_stride + vi];
+    for (int q = 0; q < 9; ++q) acc[q] = accum[vi * 9 + q];
     float basis[16], masked[3];
     view_sh(cams[v], L, acc, basis, masked);
     float gg[60];
@@ -324,7 +324,7 @@ __global__ __launch_bounds__(256, 2) void k_train_adam(float* __restrict__ P, fl
       size_t vi = (size_t)v * cap + i;
       float acc[9];
 #pragma unroll
-      for (int q = 0; q < 3; ++q) acc[q] = accum[(size_t)q * acc_stride + vi];
+      for (int q = 0; q < 3; ++q) acc[q] = accum[vi * 9 + q];
       float basis[16], masked[3];
       view_sh(cams[v], L, acc, basis, masked);
 #pragma unroll
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(256, 2) void k_train_adam(float* __restrict__ P, fl
     if (!(vis >> v & 1u)) continue;
     size_t vi = (size_t)v * cap + i;
 #pragma unroll
-    for (int q = 0; q < 9; ++q) accum[(size_t)q * acc_stride + vi] = 0.0f;
+    for (int q = 0; q < 9; ++q) accum[vi * 9 + q] = 0.0f;
   }
   float rot[4];
 #pragma unroll
@@ -366,11 +366,11 @@ __global__ __launch_bounds__(256, 2) void k_train_adam(float* __restrict__ P, fl
 }
 
 void launch_train_adam(float* P, float* M1, float* M2, int cap, int n, const CamDev* cams, int n_views,
-                       const uint32_t* cnt, float* accum, size_t acc_stride, float* gaccum, uint32_t* gdenom,
+                       const uint32_t* cnt, float* accum, float* gaccum, uint32_t* gdenom,
                        const AdamConsts* k_dev, const uint32_t* status, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_train_adam, dim3((n + 255) / 256), dim3(256), 0, st, P, M1, M2, cap, n, cams, n_views, cnt,
-                     accum, acc_stride, gaccum, gdenom, k_dev, status);
+                     accum, gaccum, gdenom, k_dev, status);
 }
 
 // ln on the device: the f64 logarithm rounded once to f32 (correctly rounded

@@ -50,14 +50,15 @@ __global__ __launch_bounds__(RT) void k_raster_fwd(RasterIn in, float* __restric
   const int t = threadIdx.x;
   const uint32_t wbit = 1u << (t >> 6);
   if (t < 32) s_tab[t] = kExp2Tab[t];
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile = in.tile_order[xcd_remap(blockIdx.x, gridDim.x)];
   const int v = view_of_tile(in.cams, in.n_views, tile);
   const CamDev& c = in.cams[v];
   const int lt = tile - c.tile_base;
   const int tx = lt % c.ntx, ty = lt / c.ntx;
   const int px = tx * TILE + (t & 15), py = ty * TILE + (t >> 4);
   const bool inside = px < c.w && py < c.h;
-  const uint32_t start = in.rstart[tile], end = in.rend[tile];
+  uint32_t start = in.rstart[tile], end = in.rend[tile];
+  if (end < start) start = end;  // no pair reached this tile
   const float bg0 = in.bg[0], bg1 = in.bg[1], bg2 = in.bg[2];
   float T = 1.0f;
   float C0 = MODE == 1 ? bg0 : 0.0f, C1 = MODE == 1 ? bg1 : 0.0f, C2 = MODE == 1 ? bg2 : 0.0f;
@@ -160,8 +161,7 @@ __device__ __forceinline__ int wave_max_i(int v) {
 
 __global__ __launch_bounds__(RT) void k_raster_bwd(RasterIn in, const float* __restrict__ final_t,
                                                    const int32_t* __restrict__ last_contrib,
-                                                   const float* __restrict__ dl_dpix, float* __restrict__ accum,
-                                                   size_t acc_stride) {
+                                                   const float* __restrict__ dl_dpix, float* __restrict__ accum) {
   __shared__ uint64_t s_tab[32];
   __shared__ float4 s_a[RT];
   __shared__ float4 s_b[RT];
@@ -175,14 +175,15 @@ __global__ __launch_bounds__(RT) void k_raster_bwd(RasterIn in, const float* __r
   const uint32_t wbit = 1u << (t >> 6);
   if (t < 32) s_tab[t] = kExp2Tab[t];
   if (t == 0) s_max = -1;
-  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile = in.tile_order[xcd_remap(blockIdx.x, gridDim.x)];
   const int v = view_of_tile(in.cams, in.n_views, tile);
   const CamDev& c = in.cams[v];
   const int lt = tile - c.tile_base;
   const int tx = lt % c.ntx, ty = lt / c.ntx;
   const int px = tx * TILE + (t & 15), py = ty * TILE + (t >> 4);
   const bool inside = px < c.w && py < c.h;
-  const uint32_t start = in.rstart[tile], end = in.rend[tile];
+  uint32_t start = in.rstart[tile], end = in.rend[tile];
+  if (end < start) start = end;  // no pair reached this tile
   int lc = -1;
   float T = 1.0f, dl0 = 0.0f, dl1 = 0.0f, dl2 = 0.0f;
   if (inside && start != end) {
@@ -285,17 +286,16 @@ __global__ __launch_bounds__(RT) void k_raster_bwd(RasterIn in, const float* __r
 #pragma unroll
       for (int k = 0; k < 9; ++k) {
         float s = s_g[k][t];
-        if (s != 0.0f) atomicAdd(&accum[(size_t)k * acc_stride + vi], s);
+        if (s != 0.0f) atomicAdd(&accum[(size_t)vi * 9 + k], s);
       }
     }
   }
 }
 
 void launch_raster_bwd(const RasterIn& in, const float* final_t, const int32_t* last, const float* dl_dpix,
-                       float* accum, size_t acc_stride, hipStream_t st) {
+                       float* accum, hipStream_t st) {
   if (in.total_tiles <= 0) return;
-  hipLaunchKernelGGL(k_raster_bwd, dim3(in.total_tiles), dim3(RT), 0, st, in, final_t, last, dl_dpix, accum,
-                     acc_stride);
+  hipLaunchKernelGGL(k_raster_bwd, dim3(in.total_tiles), dim3(RT), 0, st, in, final_t, last, dl_dpix, accum);
 }
 
 // Algorithmic work of the last render: pairs the forward evaluates per pixel
@@ -314,7 +314,8 @@ __global__ __launch_bounds__(RT) void k_work_counts(RasterIn in, const float* __
   unsigned long long f = 0, b = 0;
   if (px < c.w && py < c.h) {
     const size_t pix = (size_t)c.pix_base + (size_t)py * c.w + px;
-    const uint32_t len = in.rend[tile] - in.rstart[tile];
+    const uint32_t s0 = in.rstart[tile], e0 = in.rend[tile];
+    const uint32_t len = e0 > s0 ? e0 - s0 : 0u;
     const int32_t lc = last[pix];
     f = final_t[pix] < TRANSMITTANCE_STOP ? (unsigned long long)(lc + 1) : (unsigned long long)len;
     b = (unsigned long long)(lc + 1);

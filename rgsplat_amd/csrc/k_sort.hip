@@ -53,9 +53,9 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t* __res
                                                          uint32_t* __restrict__ hist, int nb_max) {
   __shared__ uint32_t h[256];
   uint32_t n = *n_dev;
-  int b = blockIdx.x;
+  const uint32_t b = xcd_chunked(blockIdx.x);
   uint32_t nb = (n + RS_TILE - 1) / RS_TILE;
-  if ((uint32_t)b >= nb) return;
+  if (b >= nb) return;
   h[threadIdx.x] = 0;
   __syncthreads();
   size_t base = (size_t)b * RS_TILE;
@@ -69,63 +69,98 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t* __res
 }
 
 // One block per digit: in-place exclusive scan of hist[d][0..nb), row total.
-__global__ __launch_bounds__(1024) void k_radix_rowscan(uint32_t* __restrict__ hist,
-                                                      const uint32_t* __restrict__ n_dev, int nb_max,
-                                                      uint32_t* __restrict__ row_total) {
-  __shared__ uint32_t s_w[16];
-  uint32_t n = *n_dev;
-  uint32_t nb = (n + RS_TILE - 1) / RS_TILE;
+// Each thread owns 8 consecutive entries per 8192-entry chunk (two 16-byte
+// loads; rows are 16-byte aligned as nb_max is a multiple of 256).
+constexpr int RSCAN_THREADS = 1024;
+constexpr int RSCAN_ITEMS = 8;
+__global__ __launch_bounds__(RSCAN_THREADS) void k_radix_rowscan(uint32_t* __restrict__ hist,
+                                                               const uint32_t* __restrict__ n_dev, int nb_max,
+                                                               uint32_t* __restrict__ row_total) {
+  __shared__ uint32_t s_w[RSCAN_THREADS / 64];
+  const uint32_t n = *n_dev;
+  const uint32_t nb = (n + RS_TILE - 1) / RS_TILE;
   uint32_t* row = hist + (size_t)blockIdx.x * nb_max;
-  uint32_t per = (nb + 1023) / 1024;
-  uint32_t beg = threadIdx.x * per;
-  uint32_t end = min(beg + per, nb);
-  uint32_t sum = 0;
-  for (uint32_t j = beg; j < end; ++j) sum += row[j];
-  // block exclusive scan over 1024 threads
-  uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-  uint32_t x = sum;
+  const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+  uint32_t carry = 0;
+  for (uint32_t j0 = 0; j0 < nb; j0 += RSCAN_THREADS * RSCAN_ITEMS) {
+    const uint32_t j = j0 + threadIdx.x * RSCAN_ITEMS;
+    uint32_t v[RSCAN_ITEMS];
+    const bool full = j + RSCAN_ITEMS <= nb;
+    if (full) {
+      uint4 a = *reinterpret_cast<const uint4*>(row + j);
+      uint4 c = *reinterpret_cast<const uint4*>(row + j + 4);
+      v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+      v[4] = c.x; v[5] = c.y; v[6] = c.z; v[7] = c.w;
+    } else {
 #pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    uint32_t y = __shfl_up(x, d, 64);
-    if (lane >= (uint32_t)d) x += y;
+      for (int k = 0; k < RSCAN_ITEMS; ++k) v[k] = j + k < nb ? row[j + k] : 0u;
+    }
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < RSCAN_ITEMS; ++k) sum += v[k];
+    uint32_t x = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      uint32_t y = __shfl_up(x, d, 64);
+      if (lane >= (uint32_t)d) x += y;
+    }
+    if (lane == 63) s_w[w] = x;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < RSCAN_THREADS / 64; ++k) {
+      uint32_t t = s_w[k];
+      if ((uint32_t)k < w) base += t;
+      tot += t;
+    }
+    __syncthreads();  // s_w is rewritten by the next chunk
+    uint32_t run = carry + base + x - sum;
+#pragma unroll
+    for (int k = 0; k < RSCAN_ITEMS; ++k) {
+      uint32_t c = v[k];
+      v[k] = run;
+      run += c;
+    }
+    if (full) {
+      *reinterpret_cast<uint4*>(row + j) = make_uint4(v[0], v[1], v[2], v[3]);
+      *reinterpret_cast<uint4*>(row + j + 4) = make_uint4(v[4], v[5], v[6], v[7]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < RSCAN_ITEMS; ++k)
+        if (j + k < nb) row[j + k] = v[k];
+    }
+    carry += tot;
   }
-  if (lane == 63) s_w[w] = x;
-  __syncthreads();
-  uint32_t base = 0, tot = 0;
-  for (int k = 0; k < 16; ++k) {
-    uint32_t t = s_w[k];
-    if ((uint32_t)k < w) base += t;
-    tot += t;
-  }
-  uint32_t run = base + x - sum;
-  for (uint32_t j = beg; j < end; ++j) {
-    uint32_t c = row[j];
-    row[j] = run;
-    run += c;
-  }
-  if (threadIdx.x == 0) row_total[blockIdx.x] = tot;
+  if (threadIdx.x == 0) row_total[blockIdx.x] = carry;
 }
 
+// Stable scatter of one radix tile. Keys are ranked with wave64 ballot
+// matching, the tile is staged in LDS in digit order and each digit's run is
+// written contiguously (keys first, then the values through the same LDS).
+// LAST: the final pass of the tile sort; keys are not written, and the first
+// and last position of every key segment inside the tile go to the ranges
+// (atomicMin / atomicMax), which yields each key's global [start, end).
+template <bool LAST>
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ keys_out,
     uint32_t* __restrict__ vals_out, const uint32_t* __restrict__ n_dev, int shift,
-    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ row_total, int nb_max) {
+    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ row_total, int nb_max, TileRanges tr) {
   __shared__ uint32_t s_cnt[RS_WAVES][256];
   __shared__ uint32_t s_blk_start[256];
   __shared__ uint32_t s_glob[256];
-  __shared__ uint32_t s_keys[RS_TILE];
-  __shared__ uint32_t s_vals[RS_TILE];
+  __shared__ uint32_t s_stage[RS_TILE];
+  __shared__ uint8_t s_dig[RS_TILE];  // digit of each staged slot, for the value pass
   __shared__ uint32_t s_tmp[8];
 
-  uint32_t n = *n_dev;
-  uint32_t nb = (n + RS_TILE - 1) / RS_TILE;
-  int b = blockIdx.x;
-  if ((uint32_t)b >= nb) return;
+  const uint32_t n = *n_dev;
+  const uint32_t nb = (n + RS_TILE - 1) / RS_TILE;
+  const uint32_t b = xcd_chunked(blockIdx.x);
+  if (b >= nb) return;
   const uint32_t t = threadIdx.x;
   const uint32_t lane = lane_id();
   const uint32_t w = t >> 6;
-  size_t base = (size_t)b * RS_TILE;
-  uint32_t blk_n = (uint32_t)min((size_t)RS_TILE, (size_t)n - base);
+  const size_t base = (size_t)b * RS_TILE;
+  const uint32_t blk_n = (uint32_t)min((size_t)RS_TILE, (size_t)n - base);
 
 #pragma unroll
   for (int k = 0; k < RS_WAVES; ++k) s_cnt[k][t] = 0;
@@ -134,15 +169,21 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
   s_glob[t] = dig_off + hist[(size_t)t * nb_max + b];
   __syncthreads();
 
-  uint32_t key[RS_ITEMS], val[RS_ITEMS], rank[RS_ITEMS];
+  uint32_t key[RS_ITEMS], val[RS_ITEMS], pos[RS_ITEMS];
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  size_t wbase = base + (size_t)w * RS_WAVE_ITEMS;
+  // items of this wave: wave-local index r * 64 + lane < w_n
+  const uint32_t w_lo = w * RS_WAVE_ITEMS;
+  const uint32_t w_n = blk_n > w_lo ? min(blk_n - w_lo, (uint32_t)RS_WAVE_ITEMS) : 0u;
+  const uint32_t* kin = keys_in + base + w_lo + lane;
+  const uint32_t* vin = vals_in + base + w_lo + lane;
 #pragma unroll
   for (int r = 0; r < RS_ITEMS; ++r) {
-    size_t e = wbase + (size_t)r * 64 + lane;
-    bool valid = e < n;
-    key[r] = valid ? keys_in[e] : 0xffffffffu;
-    val[r] = valid ? vals_in[e] : 0u;
+    bool valid = (uint32_t)r * 64 + lane < w_n;
+    key[r] = valid ? kin[r * 64] : 0xffffffffu;
+  }
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    bool valid = (uint32_t)r * 64 + lane < w_n;
     uint32_t d = (key[r] >> shift) & 255u;
     uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -158,7 +199,14 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    rank[r] = before + lrank;
+    pos[r] = before + lrank;
+  }
+  // values are loaded only now (in flight across the block scan) to keep
+  // the register footprint of the ranking loop small
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    bool valid = (uint32_t)r * 64 + lane < w_n;
+    val[r] = valid ? vin[r * 64] : 0u;
   }
   __syncthreads();
   // per-digit: offsets of each wave's run inside the block, and block start
@@ -174,12 +222,13 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < RS_ITEMS; ++r) {
-    size_t e = wbase + (size_t)r * 64 + lane;
-    if (e < n) {
+    // recompute the digit here instead of keeping 16 LDS addresses live
+    // across the scan (an empty asm makes the key opaque to CSE)
+    asm volatile("" : "+v"(key[r]));
+    if ((uint32_t)r * 64 + lane < w_n) {
       uint32_t d = (key[r] >> shift) & 255u;
-      uint32_t p = s_cnt[w][d] + rank[r];
-      s_keys[p] = key[r];
-      s_vals[p] = val[r];
+      pos[r] += s_cnt[w][d];
+      s_stage[pos[r]] = key[r];
     }
   }
   __syncthreads();
@@ -187,27 +236,59 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
   for (int r = 0; r < RS_ITEMS; ++r) {
     uint32_t p = (uint32_t)r * RS_THREADS + t;
     if (p < blk_n) {
-      uint32_t k = s_keys[p];
+      uint32_t k = s_stage[p];
       uint32_t d = (k >> shift) & 255u;
+      s_dig[p] = (uint8_t)d;
       uint32_t o = s_glob[d] + (p - s_blk_start[d]);
-      keys_out[o] = k;
-      vals_out[o] = s_vals[p];
+      if (!LAST) {
+        keys_out[o] = k;
+      } else if (k < tr.n_keys) {
+        // within a digit run the keys are sorted (lower digits were sorted
+        // by the earlier passes), so equal keys form one segment here
+        if (p == 0 || s_stage[p - 1] != k) atomicMin(&tr.rstart[k], o);
+        if (p + 1 == blk_n || s_stage[p + 1] != k) atomicMax(&tr.rend[k], o + 1);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r)
+    if ((uint32_t)r * 64 + lane < w_n) s_stage[pos[r]] = val[r];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    uint32_t p = (uint32_t)r * RS_THREADS + t;
+    if (p < blk_n) {
+      uint32_t d = s_dig[p];
+      vals_out[s_glob[d] + (p - s_blk_start[d])] = s_stage[p];
     }
   }
 }
 
 static int radix_sort_impl(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev, size_t n_max, int bits,
-                           RadixScratch rs, hipStream_t st, bool first_hist_done) {
+                           RadixScratch rs, hipStream_t st, bool first_hist_done, const TileRanges* tr) {
   int cur = 0;
   int nb = radix_nb(n_max);
   if (nb == 0) return 0;
+  const unsigned grid = xcd_grid((size_t)nb);
+  if (tr) {
+    launch_fill_u32(tr->rstart, tr->n_keys, 0xffffffffu, st);
+    launch_fill_u32(tr->rend, tr->n_keys, 0u, st);
+  }
   for (int shift = 0; shift < bits; shift += 8) {
+    const bool last = shift + 8 >= bits;
     if (!(shift == 0 && first_hist_done))
-      hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(RS_THREADS), 0, st, keys[cur], n_dev, shift, rs.hist,
+      hipLaunchKernelGGL(k_radix_hist, dim3(grid), dim3(RS_THREADS), 0, st, keys[cur], n_dev, shift, rs.hist,
                          rs.nb_max);
-    hipLaunchKernelGGL(k_radix_rowscan, dim3(256), dim3(1024), 0, st, rs.hist, n_dev, rs.nb_max, rs.row_total);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(RS_THREADS), 0, st, keys[cur], vals[cur], keys[cur ^ 1],
-                       vals[cur ^ 1], n_dev, shift, rs.hist, rs.row_total, rs.nb_max);
+    hipLaunchKernelGGL(k_radix_rowscan, dim3(256), dim3(RSCAN_THREADS), 0, st, rs.hist, n_dev, rs.nb_max,
+                       rs.row_total);
+    if (last && tr)
+      hipLaunchKernelGGL(k_radix_scatter<true>, dim3(grid), dim3(RS_THREADS), 0, st, keys[cur], vals[cur],
+                         keys[cur ^ 1], vals[cur ^ 1], n_dev, shift, rs.hist, rs.row_total, rs.nb_max, *tr);
+    else
+      hipLaunchKernelGGL(k_radix_scatter<false>, dim3(grid), dim3(RS_THREADS), 0, st, keys[cur], vals[cur],
+                         keys[cur ^ 1], vals[cur ^ 1], n_dev, shift, rs.hist, rs.row_total, rs.nb_max,
+                         TileRanges{nullptr, nullptr, 0u});
     cur ^= 1;
   }
   return cur;
@@ -215,11 +296,11 @@ static int radix_sort_impl(uint32_t* keys[2], uint32_t* vals[2], const uint32_t*
 
 int radix_sort_pairs(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev, size_t n_max, int bits,
                      RadixScratch rs, hipStream_t st) {
-  return radix_sort_impl(keys, vals, n_dev, n_max, bits, rs, st, false);
+  return radix_sort_impl(keys, vals, n_dev, n_max, bits, rs, st, false, nullptr);
 }
-int radix_sort_pairs_prehist(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev, size_t n_max, int bits,
-                             RadixScratch rs, hipStream_t st) {
-  return radix_sort_impl(keys, vals, n_dev, n_max, bits, rs, st, true);
+int radix_sort_tiles(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev, size_t n_max, int bits,
+                     RadixScratch rs, TileRanges tr, hipStream_t st) {
+  return radix_sort_impl(keys, vals, n_dev, n_max, bits, rs, st, true, &tr);
 }
 
 // ------------------------------------------------------------------- scans
@@ -386,7 +467,7 @@ __global__ __launch_bounds__(GP_THREADS) void k_gen_pairs(const uint32_t* __rest
     if ((size_t)tot > k_cap) atomicOr(status, ST_PAIR_OVERFLOW);
     *k_out = (uint32_t)K;
   }
-  const size_t b = blockIdx.x;
+  const size_t b = xcd_chunked(blockIdx.x);
   const size_t p_lo = b * RS_TILE;
   if (p_lo >= K) return;
   const size_t p_hi = p_lo + RS_TILE < K ? p_lo + RS_TILE : K;
@@ -447,59 +528,9 @@ void launch_gen_pairs(const uint32_t* sorted_vals, size_t n_sorted, const uint32
                        total, k_cap, first_splat);
   size_t nb = (size_t)radix_nb(k_cap);
   if (nb == 0) nb = 1;
-  hipLaunchKernelGGL(k_gen_pairs, dim3((unsigned)nb), dim3(GP_THREADS), 0, st, sorted_vals, n_sorted, offsets, total,
+  hipLaunchKernelGGL(k_gen_pairs, dim3(xcd_grid(nb)), dim3(GP_THREADS), 0, st, sorted_vals, n_sorted, offsets, total,
                      S.cnt, S.rect, cap, cams, pair_keys, pair_vals, k_cap, k_out, status, first_splat, rs.hist,
                      rs.nb_max);
-}
-
-// ---- tile ranges from the tile rects
-__global__ __launch_bounds__(256) void k_tile_diff(const uint32_t* __restrict__ cnt, const uint2* __restrict__ rect,
-                                                   int cap, int n, const CamDev* __restrict__ cams,
-                                                   int32_t* __restrict__ diff) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int v = blockIdx.y;
-  if (i >= n) return;
-  size_t vi = (size_t)v * cap + i;
-  if (cnt[vi] == 0) return;
-  const CamDev& c = cams[v];
-  uint2 rc = rect[vi];
-  int tx0 = rc.x & 0xffff, ty0 = rc.x >> 16, tx1 = rc.y & 0xffff, ty1 = rc.y >> 16;
-  int32_t* d = diff + c.diff_base;
-  const int W = c.ntx + 1;
-  atomicAdd(&d[ty0 * W + tx0], 1);
-  atomicAdd(&d[ty0 * W + tx1], -1);
-  atomicAdd(&d[ty1 * W + tx0], -1);
-  atomicAdd(&d[ty1 * W + tx1], 1);
-}
-
-// One block per view: 2-D inclusive prefix of the difference grid -> counts.
-__global__ __launch_bounds__(256) void k_tile_prefix(const CamDev* __restrict__ cams, int32_t* __restrict__ diff,
-                                                     uint32_t* __restrict__ counts) {
-  const CamDev& c = cams[blockIdx.x];
-  int32_t* d = diff + c.diff_base;
-  const int W = c.ntx + 1, H = c.nty + 1;
-  for (int y = threadIdx.x; y < H; y += blockDim.x) {
-    int32_t run = 0;
-    for (int x = 0; x < W; ++x) {
-      run += d[y * W + x];
-      d[y * W + x] = run;
-    }
-  }
-  __syncthreads();
-  for (int x = threadIdx.x; x < W; x += blockDim.x) {
-    int32_t run = 0;
-    for (int y = 0; y < H; ++y) {
-      run += d[y * W + x];
-      if (y < c.nty && x < c.ntx) counts[c.tile_base + y * c.ntx + x] = (uint32_t)run;
-    }
-  }
-}
-
-__global__ __launch_bounds__(256) void k_range_end(const uint32_t* __restrict__ start,
-                                                   const uint32_t* __restrict__ counts, int n,
-                                                   uint32_t* __restrict__ end) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) end[t] = start[t] + counts[t];
 }
 
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
@@ -517,21 +548,6 @@ void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t st) {
 void launch_fill_f32(float* p, size_t n, float v, hipStream_t st) {
   if (!n) return;
   hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
-}
-
-void launch_tile_ranges(const SplatArrays S, int cap, int n, const CamDev* cams, const CamDev* cams_host,
-                        int n_views, int total_tiles, int32_t* diff, int diff_total, uint32_t* counts,
-                        uint32_t* bsums, uint32_t* tot, uint32_t* rstart, uint32_t* rend, hipStream_t st) {
-  (void)cams_host;
-  if (total_tiles <= 0) return;
-  launch_fill_u32(reinterpret_cast<uint32_t*>(diff), (size_t)diff_total, 0u, st);
-  if (n > 0)
-    hipLaunchKernelGGL(k_tile_diff, dim3((n + 255) / 256, n_views), dim3(256), 0, st, S.cnt, S.rect, cap, n, cams,
-                       diff);
-  hipLaunchKernelGGL(k_tile_prefix, dim3(n_views), dim3(256), 0, st, cams, diff, counts);
-  scan_u32(counts, (size_t)total_tiles, rstart, bsums, tot, st);
-  hipLaunchKernelGGL(k_range_end, dim3((total_tiles + 255) / 256), dim3(256), 0, st, rstart, counts, total_tiles,
-                     rend);
 }
 
 }  // namespace rgs

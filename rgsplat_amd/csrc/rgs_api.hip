@@ -141,7 +141,6 @@ int make_camdev(const rgs_camera& cam, CamDev& out) {
   out.tile_base = 0;
   out.pix_base = 0;
   out.tgt_base = 0;
-  out.diff_base = 0;
   return 0;
 }
 
@@ -150,20 +149,16 @@ struct ViewSet {
   std::vector<CamDev> cams;
   int total_tiles = 0;
   size_t total_pix = 0;
-  int diff_total = 0;
   void layout() {
-    int tb = 0, db = 0;
+    int tb = 0;
     size_t pb = 0;
     for (auto& c : cams) {
       c.tile_base = tb;
       c.pix_base = (int32_t)pb;
-      c.diff_base = db;
       tb += c.ntx * c.nty;
-      db += (c.ntx + 1) * (c.nty + 1);
       pb += (size_t)c.w * c.h;
     }
     total_tiles = tb;
-    diff_total = db;
     total_pix = pb;
   }
   int bits() const {
@@ -173,7 +168,7 @@ struct ViewSet {
   }
 };
 
-enum { SC_DEPTH_N = 0, SC_TOTAL = 1, SC_K = 2, SC_STATUS = 3, SC_NOUT = 4, SC_TILETOT = 5, SC_COUNT = 8 };
+enum { SC_DEPTH_N = 0, SC_TOTAL = 1, SC_K = 2, SC_STATUS = 3, SC_NOUT = 4, SC_COUNT = 8 };
 
 }  // namespace
 
@@ -187,7 +182,7 @@ struct rgs_ctx {
   DBuf dk[2], dv[2], offsets, bsums, scalars;
   DBuf pk[2], pv[2], hist, rowtot;
   DBuf rstart, rend, rgb, final_t, last, dl, maps, partial, loss;
-  DBuf first_splat, tr_diff, tr_counts, tr_bsums;
+  DBuf first_splat;
   DBuf accum;
   size_t k_cap = 0;
   size_t depth_cap = 0;  // capacity (entries) of the depth-sort arrays
@@ -230,24 +225,38 @@ struct rgs_ctx {
   uint64_t graph_gen = 0;  // gen at capture time
   int raster_batch = 0;
   DBuf api_acc9;           // raster accumulators of the reference-API paths
+  DBuf tile_order;         // raster schedule: slot -> global tile id
+  std::vector<int32_t> h_order;
+  std::string order_key;
+  int super_tile = 8;      // tiles per side of a schedule super-tile
 
   // stage timing
   bool profiling = false;
   hipEvent_t ev_b[RGS_N_STAGES] = {}, ev_e[RGS_N_STAGES] = {};
   double stage_ms[RGS_N_STAGES] = {};
   uint64_t prof_steps = 0;
+  uint64_t prof_graph_steps = 0;  // profiled steps that were graph launches
+  bool prof_graph_failed = false; // capture with event nodes was refused
 };
 
 namespace {
 
 uint64_t* ctx_gen(rgs_ctx* c) { return &c->gen; }
 
-enum Stage { S_PROJECT, S_DSORT, S_SCAN, S_GEN, S_TSORT, S_RANGES, S_FWD, S_LOSS, S_BWD, S_ADAM };
+enum Stage { S_PROJECT, S_DSORT, S_SCAN, S_GEN, S_TSORT, S_FWD, S_LOSS, S_BWD, S_ADAM };
+// Stage events. Under stream capture they become event-record nodes of the
+// graph (hipEventRecordExternal), so replays are timed stage by stage too.
+inline void record(rgs_ctx* c, hipEvent_t e) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  (void)hipStreamIsCapturing(c->st, &cs);
+  if (cs == hipStreamCaptureStatusActive) (void)hipEventRecordWithFlags(e, c->st, hipEventRecordExternal);
+  else (void)hipEventRecord(e, c->st);
+}
 inline void stage_begin(rgs_ctx* c, int s) {
-  if (c->profiling) (void)hipEventRecord(c->ev_b[s], c->st);
+  if (c->profiling) record(c, c->ev_b[s]);
 }
 inline void stage_end(rgs_ctx* c, int s) {
-  if (c->profiling) (void)hipEventRecord(c->ev_e[s], c->st);
+  if (c->profiling) record(c, c->ev_e[s]);
 }
 
 void drop_graph(rgs_ctx* c) {
@@ -256,6 +265,31 @@ void drop_graph(rgs_ctx* c) {
     c->gexec = nullptr;
   }
   c->gkey.clear();
+}
+
+// Raster schedule for the tiles of vs: views in order, each view walked in
+// super-tiles of super_tile x super_tile tiles (row-major inside and across),
+// so that the blocks one XCD holds in flight cover a compact patch of the
+// image and share the splats (and accumulator lines) of that patch in L2.
+int ensure_tile_order(rgs_ctx* c, const ViewSet& vs) {
+  std::string key = std::to_string(c->super_tile) + ":";
+  for (const auto& cd : vs.cams) key += std::to_string(cd.ntx) + "x" + std::to_string(cd.nty) + ",";
+  size_t bytes = (size_t)std::max(vs.total_tiles, 1) * 4;
+  if (key == c->order_key && c->tile_order.bytes >= bytes) return 0;
+  c->h_order.assign(std::max(vs.total_tiles, 1), 0);
+  const int s = std::max(c->super_tile, 1);
+  size_t k = 0;
+  for (const auto& cd : vs.cams)
+    for (int sy = 0; sy < cd.nty; sy += s)
+      for (int sx = 0; sx < cd.ntx; sx += s)
+        for (int ty = sy; ty < std::min(sy + s, (int)cd.nty); ++ty)
+          for (int tx = sx; tx < std::min(sx + s, (int)cd.ntx); ++tx) c->h_order[k++] = cd.tile_base + ty * cd.ntx + tx;
+  CHK(ensure(c, c->tile_order, bytes));
+  HIPCHK(hipStreamSynchronize(c->st));  // a running raster may still read the old schedule
+  HIPCHK(hipMemcpy(c->tile_order.p, c->h_order.data(), bytes, hipMemcpyHostToDevice));
+  ++c->gen;  // graphs captured with the old schedule are stale
+  c->order_key = key;
+  return 0;
 }
 
 // Sizes every buffer the render sequence touches for n Gaussians (row
@@ -281,20 +315,18 @@ int ensure_render(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, size_t k_cap
   CHK(ensure(c, c->offsets, dn * 4));
   CHK(ensure(c, c->bsums, ((size_t)scan_nb(dn) + 1) * 8));
   CHK(ensure(c, c->scalars, SC_COUNT * 4));
-  int nb = std::max(radix_nb(dn), radix_nb(k_cap));
-  if (nb < 1) nb = 1;
+  int nb = std::max(radix_nb_max(dn), radix_nb_max(k_cap));
+  if (nb < 256) nb = 256;
   CHK(ensure(c, c->hist, (size_t)256 * nb * 4));
   c->hist_nb = nb;
   CHK(ensure(c, c->rowtot, 256 * 4));
   CHK(ensure(c, c->rstart, (size_t)vs.total_tiles * 4));
   CHK(ensure(c, c->rend, (size_t)vs.total_tiles * 4));
   CHK(ensure(c, c->first_splat, ((size_t)radix_nb(k_cap) + 1) * 4));
-  CHK(ensure(c, c->tr_diff, (size_t)vs.diff_total * 4));
-  CHK(ensure(c, c->tr_counts, (size_t)vs.total_tiles * 4));
-  CHK(ensure(c, c->tr_bsums, ((size_t)scan_nb(vs.total_tiles) + 1) * 4));
   CHK(ensure(c, c->rgb, vs.total_pix * 4 * 4));
   CHK(ensure(c, c->final_t, vs.total_pix * 4));
   CHK(ensure(c, c->last, vs.total_pix * 4));
+  CHK(ensure_tile_order(c, vs));
   c->k_cap = k_cap;
   return 0;
 }
@@ -335,11 +367,6 @@ void enqueue_bin_and_raster(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, co
   uint32_t* sc = P_<uint32_t>(c->scalars);
   size_t dn = vs.cams.size() * (size_t)n;
   RadixScratch rs{P_<uint32_t>(c->hist), P_<uint32_t>(c->rowtot), c->hist_nb};
-  stage_begin(c, S_RANGES);
-  launch_tile_ranges(splats(c), (int)cap, n, P_<CamDev>(c->cams_d), vs.cams.data(), (int)vs.cams.size(),
-                     vs.total_tiles, P_<int32_t>(c->tr_diff), vs.diff_total, P_<uint32_t>(c->tr_counts),
-                     P_<uint32_t>(c->tr_bsums), sc + SC_TILETOT, P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend), st);
-  stage_end(c, S_RANGES);
   stage_begin(c, S_GEN);
   launch_gen_pairs(c->sorted_vals, dn, P_<uint32_t>(c->offsets), sc + SC_TOTAL, splats(c), (int)cap,
                    P_<CamDev>(c->cams_d), P_<uint32_t>(c->pk[0]), P_<uint32_t>(c->pv[0]), c->k_cap, sc + SC_K,
@@ -348,10 +375,12 @@ void enqueue_bin_and_raster(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, co
   uint32_t* keys[2] = {P_<uint32_t>(c->pk[0]), P_<uint32_t>(c->pk[1])};
   uint32_t* vals[2] = {P_<uint32_t>(c->pv[0]), P_<uint32_t>(c->pv[1])};
   stage_begin(c, S_TSORT);
-  int cur = radix_sort_pairs_prehist(keys, vals, sc + SC_K, c->k_cap, vs.bits(), rs, st);
+  TileRanges tr{P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend), (uint32_t)vs.total_tiles};
+  int cur = radix_sort_tiles(keys, vals, sc + SC_K, c->k_cap, vs.bits(), rs, tr, st);
   stage_end(c, S_TSORT);
   c->pair_vals = vals[cur];
-  RasterIn in{P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend), c->pair_vals, splats(c), P_<CamDev>(c->cams_d),
+  RasterIn in{P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend), c->pair_vals, P_<int32_t>(c->tile_order), splats(c),
+              P_<CamDev>(c->cams_d),
               (int)vs.cams.size(), vs.total_tiles, {bg[0], bg[1], bg[2]}};
   stage_begin(c, S_FWD);
   launch_raster_fwd(in, mode, rgb_out, P_<float>(c->final_t), P_<int32_t>(c->last), st);
@@ -359,7 +388,8 @@ void enqueue_bin_and_raster(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, co
 }
 
 RasterIn raster_in(rgs_ctx* c, const ViewSet& vs, const float bg[3]) {
-  return RasterIn{P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend), c->pair_vals, splats(c), P_<CamDev>(c->cams_d),
+  return RasterIn{P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend), c->pair_vals, P_<int32_t>(c->tile_order), splats(c),
+              P_<CamDev>(c->cams_d),
                   (int)vs.cams.size(), vs.total_tiles, {bg[0], bg[1], bg[2]}};
 }
 
@@ -494,8 +524,8 @@ void rgs_destroy(rgs_ctx* c) {
                  &c->maps, &c->partial, &c->loss, &c->accum, &c->api_aos, &c->api_P, &c->api_grads,
                  &c->api_target, &c->api_m1, &c->api_m2, &c->api_acc, &c->api_den, &c->api_P2, &c->dens_want,
                  &c->dens_cnt, &c->dens_W, &c->dens_packed, &c->dens_prefix, &c->dens_b32, &c->dens_b64,
-                 &c->dens_t32, &c->dens_t64, &c->first_splat, &c->tr_diff, &c->tr_counts, &c->tr_bsums, &c->P, &c->P2, &c->M1, &c->M2, &c->gacc, &c->gden, &c->targets,
-                 &c->adam_d, &c->api_acc9};
+                 &c->dens_t32, &c->dens_t64, &c->first_splat, &c->P, &c->P2, &c->M1, &c->M2, &c->gacc, &c->gden, &c->targets,
+                 &c->adam_d, &c->api_acc9, &c->tile_order};
   for (DBuf* b : all)
     if (b->p) (void)hipFree(b->p);
   if (c->h_scalars) (void)hipHostFree(c->h_scalars);
@@ -513,7 +543,14 @@ void rgs_destroy(rgs_ctx* c) {
 int rgs_set_stream(rgs_ctx* c, void* s) {
   if (!c) return fail(RGS_E_ARG, "null ctx");
   drop_graph(c);
-  c->st = s ? (hipStream_t)s : c->own;
+  c->st = (hipStream_t)s;
+  return 0;
+}
+
+int rgs_use_own_stream(rgs_ctx* c) {
+  if (!c) return fail(RGS_E_ARG, "null ctx");
+  drop_graph(c);
+  c->st = c->own;
   return 0;
 }
 
@@ -584,10 +621,10 @@ int rgs_render_and_backward(rgs_ctx* c, const rgs_gaussian* g, size_t n, const r
   CHK(ensure(c, c->api_acc9, 9 * cap * 4));
   launch_fill_f32(P_<float>(c->api_acc9), 9 * cap, 0.0f, c->st);
   launch_raster_bwd(raster_in(c, vs, bg), P_<float>(c->final_t), P_<int32_t>(c->last), P_<float>(c->dl),
-                    P_<float>(c->api_acc9), cap, c->st);
+                    P_<float>(c->api_acc9), c->st);
   CHK(ensure(c, c->api_grads, cap * 60 * 4));
   launch_geometry_aos(P_<float>(c->api_P), (int)cap, (int)n, P_<CamDev>(c->cams_d), 0, P_<uint32_t>(c->cnt),
-                      P_<float>(c->api_acc9), cap, P_<float>(c->api_grads), c->st);
+                      P_<float>(c->api_acc9), P_<float>(c->api_grads), c->st);
   if (n) HIPCHK(hipMemcpyAsync(grads_out, c->api_grads.p, n * 60 * 4, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipMemcpyAsync(c->h_loss, c->loss.p, 4, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
@@ -759,9 +796,9 @@ int rgs_backward_dev(rgs_ctx* c, const float* dl_dev, float* grads_dev) {
   CHK(ensure(c, c->api_acc9, 9 * cap * 4));
   launch_fill_f32(P_<float>(c->api_acc9), 9 * cap, 0.0f, c->st);
   launch_raster_bwd(raster_in(c, c->fwd_views, c->fwd_bg), P_<float>(c->final_t), P_<int32_t>(c->last), dl_dev,
-                    P_<float>(c->api_acc9), cap, c->st);
+                    P_<float>(c->api_acc9), c->st);
   launch_geometry_aos(P_<float>(c->api_P), (int)cap, (int)n, P_<CamDev>(c->cams_d), 0, P_<uint32_t>(c->cnt),
-                      P_<float>(c->api_acc9), cap, grads_dev, c->st);
+                      P_<float>(c->api_acc9), grads_dev, c->st);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -873,11 +910,11 @@ static void enqueue_step_rest(rgs_ctx* c, const ViewSet& vs, const float bg[3], 
   stage_end(c, S_LOSS);
   stage_begin(c, S_BWD);
   launch_raster_bwd(raster_in(c, vs, bg), P_<float>(c->final_t), P_<int32_t>(c->last), P_<float>(c->dl),
-                    P_<float>(c->accum), vs.cams.size() * cap, c->st);
+                    P_<float>(c->accum), c->st);
   stage_end(c, S_BWD);
   stage_begin(c, S_ADAM);
   launch_train_adam(P_<float>(c->P), P_<float>(c->M1), P_<float>(c->M2), (int)cap, n, P_<CamDev>(c->cams_d),
-                    (int)vs.cams.size(), P_<uint32_t>(c->cnt), P_<float>(c->accum), vs.cams.size() * cap,
+                    (int)vs.cams.size(), P_<uint32_t>(c->cnt), P_<float>(c->accum),
                     P_<float>(c->gacc), P_<uint32_t>(c->gden), P_<AdamConsts>(c->adam_d), sc + SC_STATUS, c->st);
   stage_end(c, S_ADAM);
 }
@@ -920,9 +957,12 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
   CHK(ensure_loss(c, vs));
   std::memcpy(c->h_cams, vs.cams.data(), vs.cams.size() * sizeof(CamDev));
 
-  if (c->profiling) use_graph = 0;
+  if (c->profiling && c->prof_graph_failed) use_graph = 0;
+  if (!c->st) use_graph = 0;  // no capture on the null stream
+  if (c->profiling) key += "|prof";
   for (int attempt = 0; attempt < 3; ++attempt) {
     bool replay = use_graph && c->gexec && c->gkey == key && c->graph_gen == c->gen;
+    bool launched_graph = replay;
     if (!replay) {
       drop_graph(c);
       // eager: exact pair count sizes the buffers, then the rest of the step
@@ -947,16 +987,25 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
         HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
         enqueue_step(c, vs, bg, lambda);
         hipError_t e = hipStreamEndCapture(c->st, &g);
-        if (e != hipSuccess) return fail(RGS_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-        e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) {
-          c->gexec = nullptr;
-          return fail(RGS_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+        if (e == hipSuccess) {
+          e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(g);
+          if (e != hipSuccess) c->gexec = nullptr;
         }
-        c->gkey = key;
-        c->graph_gen = c->gen;
-        HIPCHK(hipGraphLaunch(c->gexec, c->st));
+        if (e != hipSuccess && c->profiling) {
+          // the runtime refused event nodes: profile this context eagerly
+          (void)hipGetLastError();
+          c->prof_graph_failed = true;
+          use_graph = 0;
+          enqueue_step_rest(c, vs, bg, lambda);
+        } else if (e != hipSuccess) {
+          return fail(RGS_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        } else {
+          c->gkey = key;
+          c->graph_gen = c->gen;
+          HIPCHK(hipGraphLaunch(c->gexec, c->st));
+          launched_graph = true;
+        }
       } else {
         enqueue_step_rest(c, vs, bg, lambda);
       }
@@ -992,6 +1041,7 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
         c->stage_ms[s] += ms;
       }
       c->prof_steps += 1;
+      if (launched_graph) c->prof_graph_steps += 1;
     }
     if (loss_out)
       for (int i = 0; i < n_ids; ++i) loss_out[i] = c->h_loss[i];
@@ -1047,14 +1097,16 @@ int rgs_set_profiling(rgs_ctx* c, int enable) {
   c->profiling = enable != 0;
   for (int s = 0; s < RGS_N_STAGES; ++s) c->stage_ms[s] = 0.0;
   c->prof_steps = 0;
+  c->prof_graph_steps = 0;
   return 0;
 }
 
-int rgs_stage_times(rgs_ctx* c, float* mean_ms, int n_stages, uint64_t* steps) {
+int rgs_stage_times(rgs_ctx* c, float* mean_ms, int n_stages, uint64_t* steps, uint64_t* graph_steps) {
   if (!c || !mean_ms) return fail(RGS_E_ARG, "null argument");
   for (int s = 0; s < n_stages && s < RGS_N_STAGES; ++s)
     mean_ms[s] = c->prof_steps ? (float)(c->stage_ms[s] / (double)c->prof_steps) : 0.0f;
   if (steps) *steps = c->prof_steps;
+  if (graph_steps) *graph_steps = c->prof_graph_steps;
   return 0;
 }
 
@@ -1066,7 +1118,8 @@ int rgs_step_stats(rgs_ctx* c, uint64_t* n_pairs, uint64_t* fwd_evals, uint64_t*
     CHK(ensure(c, c->api_den, 16));
     unsigned long long* d = P_<unsigned long long>(c->api_den);
     launch_fill_u32(P_<uint32_t>(c->api_den), 4, 0u, c->st);
-    RasterIn in{P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend), c->pair_vals, splats(c), P_<CamDev>(c->cams_d),
+    RasterIn in{P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend), c->pair_vals, P_<int32_t>(c->tile_order), splats(c),
+              P_<CamDev>(c->cams_d),
                 c->last_nviews, c->last_tiles, {0.0f, 0.0f, 0.0f}};
     launch_work_counts(in, P_<float>(c->final_t), P_<int32_t>(c->last), d, c->st);
     unsigned long long h[2] = {0, 0};

@@ -34,12 +34,38 @@ void launch_eval_expf(const float* x, float* out, size_t n, hipStream_t st);
 
 // ---- sorting and binning (k_sort.hip)
 struct RadixScratch {
-  uint32_t* hist;       // 256 * nb_max
+  uint32_t* hist;       // 256 * nb_max, hist[d * nb_max + b]
   uint32_t* row_total;  // 256
-  int nb_max;
+  int nb_max;           // >= radix_nb(n_max), a multiple of 256
 };
 constexpr int RS_TILE = 4096;
 inline int radix_nb(size_t n_max) { return (int)((n_max + RS_TILE - 1) / RS_TILE); }
+inline int radix_nb_max(size_t n_max) { return (radix_nb(n_max) + 255) / 256 * 256; }
+
+// Logical block ids for per-radix-tile kernels. The hardware deals blocks to
+// the 8 XCDs round-robin; this mapping gives each XCD runs of XG consecutive
+// logical blocks, so the XG histogram columns that share a cache line
+// (hist[d * nb_max + b .. b + XG)) and the neighbouring digit runs a scatter
+// writes are merged in one XCD's L2 instead of being written back in
+// fragments by all eight. The grid must be xcd_grid(n) blocks (a multiple of
+// 8 * XG); kernels exit for logical ids past their work.
+constexpr uint32_t XG = 32;
+__device__ __forceinline__ uint32_t xcd_chunked(uint32_t b) {
+  const uint32_t x = b & 7u, k = b >> 3;
+  return (((k / XG) << 3) + x) * XG + (k % XG);
+}
+inline unsigned xcd_grid(size_t nb) { return (unsigned)((nb + 8 * XG - 1) / (8 * XG) * (8 * XG)); }
+
+// Tile ranges produced by the last pass of the tile sort: for every key k
+// present, rstart[k] / rend[k] = first / one-past-last position of k in the
+// sorted order. Keys absent from the input keep the init values written by
+// the sort (rstart 0xffffffff, rend 0), i.e. an empty range end < start.
+struct TileRanges {
+  uint32_t* rstart;
+  uint32_t* rend;
+  uint32_t n_keys;
+};
+
 // Stable LSD radix sort of (key, value) pairs over bits [0, bits); count read
 // from device memory (*n_dev <= n_max). Ping-pongs between the two buffer
 // pairs; returns the index (0/1) of the buffer pair holding the result.
@@ -57,15 +83,6 @@ void scan_u32(const uint32_t* in, size_t n, uint32_t* out, uint32_t* block_sums,
 // Exclusive scan of a u64 array.
 void scan_u64(const uint64_t* in, size_t n, uint64_t* out, uint64_t* block_sums, uint64_t* total, hipStream_t st);
 
-// Tile ranges from the tile rects, before any pair exists: per-view 2-D
-// difference grids (4 atomics per visible splat), their 2-D prefix sums give
-// each tile's pair count, and an exclusive scan over global tile ids gives
-// [start, end) -- the positions the tile sort will produce. diff needs
-// sum_v (nty_v+1)(ntx_v+1) ints; counts/bsums as for scan_u32.
-void launch_tile_ranges(const SplatArrays S, int cap, int n, const CamDev* cams, const CamDev* cams_host,
-                        int n_views, int total_tiles, int32_t* diff, int diff_total, uint32_t* counts,
-                        uint32_t* bsums, uint32_t* tot, uint32_t* rstart, uint32_t* rend, hipStream_t st);
-
 // Writes the (tile, splat) pairs in depth order, one radix tile (RS_TILE
 // pairs) per block, and the first radix pass's per-tile digit histogram
 // (hist[d * nb_max + b] for the low 8 key bits). K = min(total, k_cap) goes to
@@ -75,16 +92,18 @@ void launch_gen_pairs(const uint32_t* sorted_vals, size_t n_sorted, const uint32
                       const SplatArrays S, int cap, const CamDev* cams, uint32_t* pair_keys, uint32_t* pair_vals,
                       size_t k_cap, uint32_t* k_out, uint32_t* status, uint32_t* first_splat, RadixScratch rs,
                       hipStream_t st);
-// Radix sort whose first pass's histogram was already written (by
-// launch_gen_pairs).
-int radix_sort_pairs_prehist(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev, size_t n_max, int bits,
-                             RadixScratch rs, hipStream_t st);
+// Tile sort of the pairs from launch_gen_pairs (whose first-pass histogram is
+// already written). Only the values of the result are written (the keys are
+// implied by the ranges); the ranges come out of the last pass.
+int radix_sort_tiles(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev, size_t n_max, int bits,
+                     RadixScratch rs, TileRanges tr, hipStream_t st);
 
 // ---- rasterization (k_raster.hip)
 struct RasterIn {
   const uint32_t* rstart;
   const uint32_t* rend;
   const uint32_t* pair_vals;
+  const int32_t* tile_order;  // schedule slot -> global tile id
   SplatArrays S;
   const CamDev* cams;
   int n_views;
@@ -95,9 +114,10 @@ struct RasterIn {
 // mode 1: rasterizer.rs RGBA (rgba = bg + C, A = 1 - T; empty tiles left alone).
 void launch_raster_fwd(const RasterIn& in, int mode, float* rgb, float* final_t, int32_t* last, hipStream_t st);
 // Accumulates the 9 per-splat screen-space gradients (dL/dcolour[3],
-// dL/dopacity, dL/dconic[3], dL/dsx, dL/dsy) into accum[k * acc_stride + vi].
+// dL/dopacity, dL/dconic[3], dL/dsx, dL/dsy) into accum[vi * 9 + k] (one
+// splat's nine values share cache lines, so a tile's atomics touch few lines).
 void launch_raster_bwd(const RasterIn& in, const float* final_t, const int32_t* last, const float* dl_dpix,
-                       float* accum, size_t acc_stride, hipStream_t st);
+                       float* accum, hipStream_t st);
 
 // out[0] += forward pair evaluations, out[1] += backward pair evaluations.
 void launch_work_counts(const RasterIn& in, const float* final_t, const int32_t* last, unsigned long long* out,
@@ -128,14 +148,14 @@ void launch_aos_to_soa(const float* aos, size_t n, float* P, size_t cap, hipStre
 void launch_soa_to_aos(const float* P, size_t cap, size_t n, float* aos, hipStream_t st);
 // GaussianGrad AoS (60 floats, zero when culled) for one view.
 void launch_geometry_aos(const float* P, int cap, int n, const CamDev* cams, int view, const uint32_t* cnt,
-                         const float* accum, size_t acc_stride, float* grads, hipStream_t st);
+                         const float* accum, float* grads, hipStream_t st);
 // Adam on the reference layouts: AoS params/grads, row-major moments.
 void launch_adam_aos(float* g, const float* grads, size_t n, float* m1, float* m2, float* accum, uint32_t* denom,
                      const AdamConsts& k, hipStream_t st);
 // Fused per-Gaussian geometry backprop over all views + Adam on SoA buffers.
 // Skips everything when *status has ST_PAIR_OVERFLOW. Clears accum.
 void launch_train_adam(float* P, float* M1, float* M2, int cap, int n, const CamDev* cams, int n_views,
-                       const uint32_t* cnt, float* accum, size_t acc_stride, float* gaccum, uint32_t* gdenom,
+                       const uint32_t* cnt, float* accum, float* gaccum, uint32_t* gdenom,
                        const AdamConsts* k_dev, const uint32_t* status, hipStream_t st);
 // reset_opacities on SoA (row 10) or AoS (offset 3) params.
 void launch_reset_opacity(float* p, size_t n, size_t stride, hipStream_t st);

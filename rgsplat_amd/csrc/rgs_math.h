@@ -274,7 +274,6 @@ struct CamDev {
   int32_t tile_base;  // first global tile id of this view
   int32_t pix_base;   // first pixel of this view in the batched image arrays
   uint32_t tgt_base;  // first pixel of this view's target in the target store
-  int32_t diff_base;  // first entry of this view's (nty+1) x (ntx+1) tile-count grid
 };
 
 // p_cam = view * [p, 1] (gemv order, math/mod.rs:225-226)

@@ -58,6 +58,7 @@ SIGNATURES = {
     "rgs_destroy": [_vp],
     "rgs_last_error": [],
     "rgs_set_stream": [_vp, _vp],
+    "rgs_use_own_stream": [_vp],
     "rgs_render_and_backward": [_vp, _vp, _sz, _CAM, _vp, _vp, _f, ctypes.POINTER(_f), _vp],
     "rgs_render_color": [_vp, _vp, _sz, _CAM, _vp, _vp],
     "rgs_rasterize_rgba": [_vp, _vp, _sz, _CAM, _vp, _vp],
@@ -80,7 +81,7 @@ SIGNATURES = {
     "rgs_set_raster_batch": [_vp, _i],
     "rgs_eval_expf": [_vp, _vp, _vp, _sz],
     "rgs_set_profiling": [_vp, _i],
-    "rgs_stage_times": [_vp, _vp, _i, ctypes.POINTER(ctypes.c_uint64)],
+    "rgs_stage_times": [_vp, _vp, _i, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)],
     "rgs_debug_tiles": [_vp, _vp, _sz, _vp, _sz, ctypes.POINTER(ctypes.c_uint64)],
 }
 
@@ -312,17 +313,19 @@ class Context:
         _check(lib().rgs_set_raster_batch(self.h, int(batch)))
 
     # ---------------------------------------------------------- stage timing
-    STAGES = ["project", "depth_sort", "pair_scan", "pair_gen", "tile_sort", "tile_ranges", "raster_fwd",
-              "loss_ssim", "raster_bwd", "geometry_adam"]
+    STAGES = ["project", "depth_sort", "pair_scan", "pair_gen", "tile_sort", "raster_fwd", "loss_ssim",
+              "raster_bwd", "geometry_adam"]
 
     def set_profiling(self, enable: bool):
         _check(lib().rgs_set_profiling(self.h, int(bool(enable))))
 
-    def stage_times(self) -> tuple[dict, int]:
+    def stage_times(self) -> tuple[dict, int, int]:
+        """(mean ms per stage, profiled steps, of which graph replays)."""
         ms = np.zeros(len(self.STAGES), np.float32)
         n = ctypes.c_uint64()
-        _check(lib().rgs_stage_times(self.h, _p(ms), len(ms), ctypes.byref(n)))
-        return dict(zip(self.STAGES, ms.tolist())), n.value
+        ng = ctypes.c_uint64()
+        _check(lib().rgs_stage_times(self.h, _p(ms), len(ms), ctypes.byref(n), ctypes.byref(ng)))
+        return dict(zip(self.STAGES, ms.tolist())), n.value, ng.value
 
     # ------------------------------------------------------------ diagnostics
     def eval_expf(self, x) -> np.ndarray:

@@ -76,3 +76,29 @@ def test_autograd_finite_differences():
         n = num(k)
         rel = abs(an[k] - n) / max(abs(an[k]), abs(n), 1e-4)
         assert rel < 0.05 or abs(an[k] - n) < 3e-4, (k, an[k], n)
+
+
+def test_outputs_ordered_with_torch_streams():
+    """The renderer runs on torch's current stream, including the default
+    stream (handle 0): a torch reduction launched right after the render sees
+    the finished image, on the default stream and on a side stream."""
+    import math
+
+    import torch
+    from rgsplat_amd import torch_ops
+    cam = S.identity_camera(64, 48)
+    wl = S.scaled(S.WORKLOADS["c0_10k_256"], n=400, sizes=[(64, 48)])
+    g = wl.scene()
+    refs, xs = [], []
+    for k in range(6):
+        x = g.copy()
+        x[:, 2] += 0.05 * k
+        xs.append(x)
+        refs.append(float(np.abs(oracle.render_color(x, cam)).astype(np.float64).mean()))
+    side = torch.cuda.Stream()
+    for stream in (torch.cuda.default_stream(), side):
+        with torch.cuda.stream(stream):
+            got = [torch_ops.render(torch.tensor(x, device="cuda"), cam).abs().mean() for x in xs]
+            vals = [v.item() for v in got]
+        for v, r in zip(vals, refs):
+            assert math.isclose(v, r, rel_tol=1e-5), (v, r)
