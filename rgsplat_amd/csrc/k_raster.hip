@@ -1,17 +1,20 @@
 // k_raster.hip -- front-to-back alpha blending and its analytic backward.
 //
-// One 256-thread workgroup per 16x16 tile (the reference's binning tile,
-// backward.rs:21), one thread per pixel. Tile lists are consumed in batches of
-// 256 splats staged through LDS; the workgroup stops as soon as every pixel
-// has saturated (T < 1e-4). The exp2 table of the exact expf lives in LDS.
+// One wavefront per 16x16 tile (the reference's binning tile,
+// backward.rs:21): each lane owns four pixels of one column, one in each
+// 4-row pixel group. Tile lists are consumed in batches of 64 splats staged
+// through LDS (one splat per lane); a splat whose alpha-reach box misses a
+// pixel group is skipped for that group with a wave-uniform branch. The wave
+// stops as soon as every pixel has saturated (T < 1e-4). The exp2 table of
+// the exact expf lives in LDS.
 //
 // Forward follows backward.rs:296-385 (mode 0) and rasterizer.rs:72-192
 // (mode 1, RGBA). Backward follows backward.rs:391-512: each pixel walks its
 // list from last_contrib down to 0, recovering the transmittance in front of
-// each splat by division (no per-fragment storage); the per-splat gradient of
-// the 64 pixels of a wave is reduced across lanes, summed over the four waves
-// in LDS, and added to the per-splat accumulators with one FP32 atomic per
-// value per tile.
+// each splat by division (no per-fragment storage); a lane first sums its
+// four pixels' gradients for the splat, the wave then reduces the nine sums
+// across lanes, and each tile adds them to the per-splat accumulators with
+// one FP32 atomic per value.
 #include "rgs_kernels.h"
 
 namespace rgs {
@@ -39,98 +42,119 @@ __device__ __forceinline__ uint32_t wave_mask(float sx, float sy, float2 reach, 
   return m;
 }
 
+// Forward: one wave per tile, four pixels per lane (pixel group q = tile rows
+// 4q..4q+3, lane l at column l & 15, row 4q + (l >> 4)); batches of 64 splats
+// staged in LDS; the wave stops when all 256 pixels have saturated.
+constexpr int FT = 64;
 template <int MODE>
-__global__ __launch_bounds__(RT) void k_raster_fwd(RasterIn in, float* __restrict__ out_rgb,
+__global__ __launch_bounds__(FT) void k_raster_fwd(RasterIn in, float* __restrict__ out_rgb,
                                                    float* __restrict__ out_t, int32_t* __restrict__ out_last) {
   __shared__ uint64_t s_tab[32];
-  __shared__ float4 s_a[RT];
-  __shared__ float4 s_b[RT];
-  __shared__ float s_c[RT];
-  __shared__ uint32_t s_m[RT];
-  const int t = threadIdx.x;
-  const uint32_t wbit = 1u << (t >> 6);
-  if (t < 32) s_tab[t] = kExp2Tab[t];
+  __shared__ float4 s_a[FT];
+  __shared__ float4 s_b[FT];
+  __shared__ float s_c[FT];
+  __shared__ uint32_t s_m[FT];
+  const int lane = threadIdx.x;
+  if (lane < 32) s_tab[lane] = kExp2Tab[lane];
   const int tile = in.tile_order[xcd_remap(blockIdx.x, gridDim.x)];
   const int v = view_of_tile(in.cams, in.n_views, tile);
   const CamDev& c = in.cams[v];
   const int lt = tile - c.tile_base;
   const int tx = lt % c.ntx, ty = lt / c.ntx;
-  const int px = tx * TILE + (t & 15), py = ty * TILE + (t >> 4);
-  const bool inside = px < c.w && py < c.h;
+  const int px = tx * TILE + (lane & 15);
+  const float fpx = (float)px + 0.5f;
   uint32_t start = in.rstart[tile], end = in.rend[tile];
   if (end < start) start = end;  // no pair reached this tile
   const float bg0 = in.bg[0], bg1 = in.bg[1], bg2 = in.bg[2];
-  float T = 1.0f;
-  float C0 = MODE == 1 ? bg0 : 0.0f, C1 = MODE == 1 ? bg1 : 0.0f, C2 = MODE == 1 ? bg2 : 0.0f;
-  int32_t last = -1;
-  bool done = !inside;
-  const float fpx = (float)px + 0.5f, fpy = (float)py + 0.5f;
-  for (uint32_t b0 = start; b0 < end; b0 += RT) {
-    if (__syncthreads_count(done) == RT) break;
-    uint32_t idx = b0 + t;
+  float T[4], C0[4], C1[4], C2[4], fpy[4];
+  int32_t last[4];
+  bool done[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int py = ty * TILE + 4 * q + (lane >> 4);
+    fpy[q] = (float)py + 0.5f;
+    T[q] = 1.0f;
+    C0[q] = MODE == 1 ? bg0 : 0.0f;
+    C1[q] = MODE == 1 ? bg1 : 0.0f;
+    C2[q] = MODE == 1 ? bg2 : 0.0f;
+    last[q] = -1;
+    done[q] = !(px < c.w && py < c.h);
+  }
+  for (uint32_t b0 = start; b0 < end; b0 += FT) {
+    if (__ballot(!(done[0] && done[1] && done[2] && done[3])) == 0) break;
+    __syncthreads();  // previous batch fully read (one wave: s_barrier only)
+    const uint32_t idx = b0 + lane;
     if (idx < end) {
       uint32_t vi = in.pair_vals[idx];
       float4 A = in.S.xy_ab[vi];
-      s_a[t] = A;
-      s_b[t] = in.S.c_op_rg[vi];
-      s_c[t] = in.S.col_b[vi];
-      s_m[t] = wave_mask(A.x, A.y, in.S.reach[vi], tx, ty);
+      s_a[lane] = A;
+      s_b[lane] = in.S.c_op_rg[vi];
+      s_c[lane] = in.S.col_b[vi];
+      s_m[lane] = wave_mask(A.x, A.y, in.S.reach[vi], tx, ty);
     }
     __syncthreads();
-    const int nb = (int)min((uint32_t)RT, end - b0);
-    if (!done) {
-      for (int j = 0; j < nb; ++j) {
-        if (!(s_m[j] & wbit)) continue;  // wave-uniform
-        const float4 A = s_a[j];
-        const float4 B = s_b[j];
+    const int nb = (int)min((uint32_t)FT, end - b0);
+    for (int j = 0; j < nb; ++j) {
+      const uint32_t m = s_m[j];  // wave-uniform
+      if (!m) continue;
+      const float4 A = s_a[j];
+      const float4 B = s_b[j];
+      const float cb = s_c[j];
+      const int32_t li = (int32_t)(b0 - start) + j;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (!(m >> q & 1u)) continue;  // wave-uniform
+        if (done[q]) continue;
         float dx = fpx - A.x;
-        float dy = fpy - A.y;
+        float dy = fpy[q] - A.y;
         float power = -0.5f * (A.z * dx * dx + 2.0f * A.w * dx * dy + B.x * dy * dy);
         if (power > 0.0f) continue;
         float alpha = fmin_(B.y * expf_tab(power, s_tab), MAX_ALPHA);
         if (alpha < ALPHA_THRESHOLD) continue;
-        float weight = alpha * T;
-        C0 += B.z * weight;
-        C1 += B.w * weight;
-        C2 += s_c[j] * weight;
-        T *= 1.0f - alpha;
-        last = (int32_t)(b0 - start) + j;
-        if (T < TRANSMITTANCE_STOP) {
-          done = true;
-          break;
-        }
+        float weight = alpha * T[q];
+        C0[q] += B.z * weight;
+        C1[q] += B.w * weight;
+        C2[q] += cb * weight;
+        T[q] *= 1.0f - alpha;
+        last[q] = li;
+        if (T[q] < TRANSMITTANCE_STOP) done[q] = true;
       }
+      if (__ballot(!(done[0] && done[1] && done[2] && done[3])) == 0) break;
     }
   }
-  if (!inside) return;
-  const size_t pix = (size_t)c.pix_base + (size_t)py * c.w + px;
-  if (MODE == 0) {
-    if (start == end) {
-      out_rgb[pix * 3 + 0] = 0.0f;
-      out_rgb[pix * 3 + 1] = 0.0f;
-      out_rgb[pix * 3 + 2] = 0.0f;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int py = ty * TILE + 4 * q + (lane >> 4);
+    if (!(px < c.w && py < c.h)) continue;
+    const size_t pix = (size_t)c.pix_base + (size_t)py * c.w + px;
+    if (MODE == 0) {
+      if (start == end) {
+        out_rgb[pix * 3 + 0] = 0.0f;
+        out_rgb[pix * 3 + 1] = 0.0f;
+        out_rgb[pix * 3 + 2] = 0.0f;
+      } else {
+        out_rgb[pix * 3 + 0] = C0[q] + T[q] * bg0;
+        out_rgb[pix * 3 + 1] = C1[q] + T[q] * bg1;
+        out_rgb[pix * 3 + 2] = C2[q] + T[q] * bg2;
+      }
+      out_t[pix] = T[q];
+      out_last[pix] = last[q];
     } else {
-      out_rgb[pix * 3 + 0] = C0 + T * bg0;
-      out_rgb[pix * 3 + 1] = C1 + T * bg1;
-      out_rgb[pix * 3 + 2] = C2 + T * bg2;
+      if (start == end) continue;
+      out_rgb[pix * 4 + 0] = C0[q];
+      out_rgb[pix * 4 + 1] = C1[q];
+      out_rgb[pix * 4 + 2] = C2[q];
+      out_rgb[pix * 4 + 3] = 1.0f - T[q];
     }
-    out_t[pix] = T;
-    out_last[pix] = last;
-  } else {
-    if (start == end) return;
-    out_rgb[pix * 4 + 0] = C0;
-    out_rgb[pix * 4 + 1] = C1;
-    out_rgb[pix * 4 + 2] = C2;
-    out_rgb[pix * 4 + 3] = 1.0f - T;
   }
 }
 
 void launch_raster_fwd(const RasterIn& in, int mode, float* rgb, float* final_t, int32_t* last, hipStream_t st) {
   if (in.total_tiles <= 0) return;
   if (mode == 0)
-    hipLaunchKernelGGL(k_raster_fwd<0>, dim3(in.total_tiles), dim3(RT), 0, st, in, rgb, final_t, last);
+    hipLaunchKernelGGL(k_raster_fwd<0>, dim3(in.total_tiles), dim3(FT), 0, st, in, rgb, final_t, last);
   else
-    hipLaunchKernelGGL(k_raster_fwd<1>, dim3(in.total_tiles), dim3(RT), 0, st, in, rgb, final_t, last);
+    hipLaunchKernelGGL(k_raster_fwd<1>, dim3(in.total_tiles), dim3(FT), 0, st, in, rgb, final_t, last);
 }
 
 template <int CTRL>
@@ -149,6 +173,41 @@ __device__ __forceinline__ float wave_sum(float v) {
   v += dpp_f<0x140>(v);  // row_mirror
   return (read_lane_f(v, 0) + read_lane_f(v, 16)) + (read_lane_f(v, 32) + read_lane_f(v, 48));
 }
+// ds_swizzle in bit-mask mode: lane i reads lane i ^ X inside groups of 32.
+template <int X>
+__device__ __forceinline__ float swz_xor(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_ds_swizzle(__builtin_bit_cast(int, v), 0x1F | (X << 10)));
+}
+// Sums eight per-lane values over the wave as a butterfly that halves the
+// number of live values at each exchange (4 + 2 + 1 exchanges instead of
+// 8 x 6). On return lane l < 8 holds the wave sum of g[rev3(l)], where
+// rev3 reverses the three low bits of l.
+__device__ __forceinline__ float wave_sum8_scatter(const float g[8], int lane) {
+  const bool b0 = lane & 1, b1 = lane & 2, b2 = lane & 4;
+  float r[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    float snd = b0 ? g[i] : g[4 + i];
+    float kp = b0 ? g[4 + i] : g[i];
+    r[i] = kp + dpp_f<0xB1>(snd);  // partner lane ^ 1
+  }
+  float s[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    float snd = b1 ? r[i] : r[2 + i];
+    float kp = b1 ? r[2 + i] : r[i];
+    s[i] = kp + dpp_f<0x4E>(snd);  // partner lane ^ 2
+  }
+  float snd = b2 ? s[0] : s[1];
+  float kp = b2 ? s[1] : s[0];
+  float t = kp + swz_xor<4>(snd);
+  t += dpp_f<0x128>(t);  // row_ror:8 = lane ^ 8 inside a row of 16
+  t += swz_xor<16>(t);
+  t += __shfl_xor(t, 32, 64);
+  return t;
+}
+__device__ __forceinline__ int rev3(int l) { return ((l & 1) << 2) | (l & 2) | ((l >> 2) & 1); }
+
 __device__ __forceinline__ int wave_max_i(int v) {
   v = max(v, __shfl_xor(v, 1, 64));
   v = max(v, __shfl_xor(v, 2, 64));
@@ -159,134 +218,142 @@ __device__ __forceinline__ int wave_max_i(int v) {
   return v;
 }
 
-__global__ __launch_bounds__(RT) void k_raster_bwd(RasterIn in, const float* __restrict__ final_t,
+// Backward: one wave per tile, four pixels per lane (pixel group q = tile
+// rows 4q..4q+3, lane l at column l & 15, row 4q + (l >> 4)). A lane sums its
+// four pixels' gradients for a splat in registers before the single cross-lane
+// reduction, so the reduction is paid once per 256 pixels.
+constexpr int BT = 64;
+__global__ __launch_bounds__(BT) void k_raster_bwd(RasterIn in, const float* __restrict__ final_t,
                                                    const int32_t* __restrict__ last_contrib,
                                                    const float* __restrict__ dl_dpix, float* __restrict__ accum) {
   __shared__ uint64_t s_tab[32];
-  __shared__ float4 s_a[RT];
-  __shared__ float4 s_b[RT];
-  __shared__ float s_c[RT];
-  __shared__ uint32_t s_vi[RT];
-  __shared__ uint32_t s_m[RT];
-  __shared__ float s_g[9][RT];
-  __shared__ int s_max;
-  const int t = threadIdx.x;
-  const int lane = t & 63;
-  const uint32_t wbit = 1u << (t >> 6);
-  if (t < 32) s_tab[t] = kExp2Tab[t];
-  if (t == 0) s_max = -1;
+  __shared__ float4 s_a[BT];
+  __shared__ float4 s_b[BT];
+  __shared__ float s_c[BT];
+  __shared__ uint32_t s_vi[BT];
+  __shared__ uint32_t s_m[BT];
+  __shared__ float s_g[9][BT];
+  const int lane = threadIdx.x;
+  if (lane < 32) s_tab[lane] = kExp2Tab[lane];
   const int tile = in.tile_order[xcd_remap(blockIdx.x, gridDim.x)];
   const int v = view_of_tile(in.cams, in.n_views, tile);
   const CamDev& c = in.cams[v];
   const int lt = tile - c.tile_base;
   const int tx = lt % c.ntx, ty = lt / c.ntx;
-  const int px = tx * TILE + (t & 15), py = ty * TILE + (t >> 4);
-  const bool inside = px < c.w && py < c.h;
   uint32_t start = in.rstart[tile], end = in.rend[tile];
   if (end < start) start = end;  // no pair reached this tile
-  int lc = -1;
-  float T = 1.0f, dl0 = 0.0f, dl1 = 0.0f, dl2 = 0.0f;
-  if (inside && start != end) {
-    const size_t pix = (size_t)c.pix_base + (size_t)py * c.w + px;
-    lc = last_contrib[pix];
-    T = final_t[pix];
-    dl0 = dl_dpix[pix * 3 + 0];
-    dl1 = dl_dpix[pix * 3 + 1];
-    dl2 = dl_dpix[pix * 3 + 2];
-  }
+  const int px = tx * TILE + (lane & 15);
+  const float fpx = (float)px + 0.5f;
   const float bg0 = in.bg[0], bg1 = in.bg[1], bg2 = in.bg[2];
-  float bh0 = T * bg0, bh1 = T * bg1, bh2 = T * bg2;
-  const int wave_lc = wave_max_i(lc);
-  __syncthreads();
-  if (lane == 0 && wave_lc >= 0) atomicMax(&s_max, wave_lc);
-  __syncthreads();
-  const int top = s_max;
+  int lc[4];
+  float T[4], d0[4], d1[4], d2[4], h0[4], h1[4], h2[4], fpy[4];
+  int lc_max = -1;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int py = ty * TILE + 4 * q + (lane >> 4);
+    fpy[q] = (float)py + 0.5f;
+    lc[q] = -1;
+    T[q] = 1.0f;
+    d0[q] = d1[q] = d2[q] = 0.0f;
+    if (px < c.w && py < c.h && start != end) {
+      const size_t pix = (size_t)c.pix_base + (size_t)py * c.w + px;
+      lc[q] = last_contrib[pix];
+      T[q] = final_t[pix];
+      d0[q] = dl_dpix[pix * 3 + 0];
+      d1[q] = dl_dpix[pix * 3 + 1];
+      d2[q] = dl_dpix[pix * 3 + 2];
+    }
+    h0[q] = T[q] * bg0;
+    h1[q] = T[q] * bg1;
+    h2[q] = T[q] * bg2;
+    lc_max = max(lc_max, lc[q]);
+  }
+  const int top = wave_max_i(lc_max);
   if (top < 0) return;
-  const float fpx = (float)px + 0.5f, fpy = (float)py + 0.5f;
-  for (int hi = top; hi >= 0; hi -= RT) {
-    const int lo = max(0, hi - (RT - 1));
+  for (int hi = top; hi >= 0; hi -= BT) {
+    const int lo = max(0, hi - (BT - 1));
     const int nb = hi - lo + 1;
-    __syncthreads();
-    if (t < nb) {
-      uint32_t vi = in.pair_vals[start + lo + t];
-      s_vi[t] = vi;
+    __syncthreads();  // previous batch fully read (one wave: s_barrier only)
+    if (lane < nb) {
+      uint32_t vi = in.pair_vals[start + lo + lane];
+      s_vi[lane] = vi;
       float4 A = in.S.xy_ab[vi];
-      s_a[t] = A;
-      s_b[t] = in.S.c_op_rg[vi];
-      s_c[t] = in.S.col_b[vi];
-      s_m[t] = wave_mask(A.x, A.y, in.S.reach[vi], tx, ty);
+      s_a[lane] = A;
+      s_b[lane] = in.S.c_op_rg[vi];
+      s_c[lane] = in.S.col_b[vi];
+      s_m[lane] = wave_mask(A.x, A.y, in.S.reach[vi], tx, ty);
     }
 #pragma unroll
-    for (int k = 0; k < 9; ++k) s_g[k][t] = 0.0f;
+    for (int k = 0; k < 9; ++k) s_g[k][lane] = 0.0f;
     __syncthreads();
-    // list entries above this wave's last contributor cannot contribute
-    const int jtop = min(nb - 1, wave_lc - lo);
-    for (int j = jtop; j >= 0; --j) {
-      if (!(s_m[j] & wbit)) continue;  // wave-uniform
+    for (int j = nb - 1; j >= 0; --j) {
+      const uint32_t m = s_m[j];  // wave-uniform
+      if (!m) continue;
       const int li = lo + j;
+      const float4 A = s_a[j];
+      const float4 B = s_b[j];
+      const float cb = s_c[j];
       float g[9];
 #pragma unroll
       for (int k = 0; k < 9; ++k) g[k] = 0.0f;
       bool contrib = false;
-      if (li <= lc) {
-        const float4 A = s_a[j];
-        const float4 B = s_b[j];
-        const float cb = s_c[j];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (!(m >> q & 1u)) continue;  // wave-uniform
+        if (li > lc[q]) continue;
         float dx = fpx - A.x;
-        float dy = fpy - A.y;
+        float dy = fpy[q] - A.y;
         float power = -0.5f * (A.z * dx * dx + 2.0f * A.w * dx * dy + B.x * dy * dy);
-        if (!(power > 0.0f)) {  // NaN proceeds, as in the reference
-          float g_exp = expf_tab(power, s_tab);
-          float alpha_raw = B.y * g_exp;
-          float alpha = fmin_(alpha_raw, MAX_ALPHA);
-          if (alpha >= ALPHA_THRESHOLD) {
-            contrib = true;
-            // transmittance in front of this splat (only gradients use it)
-            float inv_1ma = __builtin_amdgcn_rcpf(1.0f - alpha);
-            T = T * inv_1ma;
-            float weight = alpha * T;
-            g[0] = dl0 * weight;
-            g[1] = dl1 * weight;
-            g[2] = dl2 * weight;
-            float dl_dalpha = dl0 * (T * B.z - bh0 * inv_1ma);
-            dl_dalpha += dl1 * (T * B.w - bh1 * inv_1ma);
-            dl_dalpha += dl2 * (T * cb - bh2 * inv_1ma);
-            float dl_draw = alpha_raw > MAX_ALPHA ? 0.0f : dl_dalpha;
-            g[3] = dl_draw * g_exp;
-            float dl_dpower = dl_draw * B.y * g_exp;
-            g[4] = dl_dpower * (-0.5f * dx * dx);
-            g[5] = dl_dpower * (-dx * dy);
-            g[6] = dl_dpower * (-0.5f * dy * dy);
-            g[7] = dl_dpower * (A.z * dx + A.w * dy);
-            g[8] = dl_dpower * (A.w * dx + B.x * dy);
-            bh0 += weight * B.z;
-            bh1 += weight * B.w;
-            bh2 += weight * cb;
-          }
-        }
+        if (power > 0.0f) continue;  // NaN proceeds, as in the reference
+        float g_exp = expf_tab(power, s_tab);
+        float alpha_raw = B.y * g_exp;
+        float alpha = fmin_(alpha_raw, MAX_ALPHA);
+        if (!(alpha >= ALPHA_THRESHOLD)) continue;
+        contrib = true;
+        // transmittance in front of this splat (only gradients use it)
+        float inv_1ma = __builtin_amdgcn_rcpf(1.0f - alpha);
+        float Tq = T[q] * inv_1ma;
+        T[q] = Tq;
+        float weight = alpha * Tq;
+        g[0] += d0[q] * weight;
+        g[1] += d1[q] * weight;
+        g[2] += d2[q] * weight;
+        float dl_dalpha = d0[q] * (Tq * B.z - h0[q] * inv_1ma);
+        dl_dalpha += d1[q] * (Tq * B.w - h1[q] * inv_1ma);
+        dl_dalpha += d2[q] * (Tq * cb - h2[q] * inv_1ma);
+        float dl_draw = alpha_raw > MAX_ALPHA ? 0.0f : dl_dalpha;
+        g[3] += dl_draw * g_exp;
+        float dl_dpower = dl_draw * B.y * g_exp;
+        g[4] += dl_dpower * (-0.5f * dx * dx);
+        g[5] += dl_dpower * (-dx * dy);
+        g[6] += dl_dpower * (-0.5f * dy * dy);
+        g[7] += dl_dpower * (A.z * dx + A.w * dy);
+        g[8] += dl_dpower * (A.w * dx + B.x * dy);
+        h0[q] += weight * B.z;
+        h1[q] += weight * B.w;
+        h2[q] += weight * cb;
       }
-      const uint64_t m = __ballot(contrib);
-      if (m == 0) continue;
-      if (__popcll(m) <= 2) {
+      const uint64_t bm = __ballot(contrib);
+      if (bm == 0) continue;
+      if (__popcll(bm) <= 2) {
         if (contrib) {
 #pragma unroll
           for (int k = 0; k < 9; ++k) atomicAdd(&s_g[k][j], g[k]);
         }
       } else {
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-          float s = wave_sum(g[k]);
-          if (lane == 0) atomicAdd(&s_g[k][j], s);
-        }
+        const float t8 = wave_sum8_scatter(g, lane);
+        const float s8 = wave_sum(g[8]);
+        if (lane < 8) s_g[rev3(lane)][j] += t8;
+        if (lane == 8) s_g[8][j] += s8;
       }
     }
     __syncthreads();
-    if (t < nb) {
-      const uint32_t vi = s_vi[t];
+    if (lane < nb) {
+      const uint32_t vi = s_vi[lane];
 #pragma unroll
       for (int k = 0; k < 9; ++k) {
-        float s = s_g[k][t];
-        if (s != 0.0f) atomicAdd(&accum[(size_t)vi * 9 + k], s);
+        float sum = s_g[k][lane];
+        if (sum != 0.0f) atomicAdd(&accum[(size_t)vi * 9 + k], sum);
       }
     }
   }
@@ -295,7 +362,7 @@ __global__ __launch_bounds__(RT) void k_raster_bwd(RasterIn in, const float* __r
 void launch_raster_bwd(const RasterIn& in, const float* final_t, const int32_t* last, const float* dl_dpix,
                        float* accum, hipStream_t st) {
   if (in.total_tiles <= 0) return;
-  hipLaunchKernelGGL(k_raster_bwd, dim3(in.total_tiles), dim3(RT), 0, st, in, final_t, last, dl_dpix, accum);
+  hipLaunchKernelGGL(k_raster_bwd, dim3(in.total_tiles), dim3(BT), 0, st, in, final_t, last, dl_dpix, accum);
 }
 
 // Algorithmic work of the last render: pairs the forward evaluates per pixel
