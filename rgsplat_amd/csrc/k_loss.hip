@@ -37,12 +37,12 @@ __device__ __forceinline__ float block_sum_256(float x, float* s_tmp) {
 __global__ __launch_bounds__(LT) void k_ssim_fwd(LossIn in, Kern11 K, float* __restrict__ dl,
                                                  float* __restrict__ maps, float* __restrict__ partial,
                                                  size_t total_pix) {
-  __shared__ float s_x[HN];
-  __shared__ float s_y[HN];
+  __shared__ float s_xa[3][HN];
+  __shared__ float s_ya[3][HN];
   __shared__ float s_h[5][HH];
   __shared__ float s_tmp[4];
   const int t = threadIdx.x;
-  const int tile = blockIdx.x;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);  // halo-sharing neighbours on one XCD
   int v = 0;
   while (v + 1 < in.n_views && in.cams[v + 1].tile_base <= tile) ++v;
   const CamDev& c = in.cams[v];
@@ -57,16 +57,23 @@ __global__ __launch_bounds__(LT) void k_ssim_fwd(LossIn in, Kern11 K, float* __r
   const float lam = in.lambda;
   const float C1 = 0.01f * 0.01f;
   const float C2 = 0.03f * 0.03f;
+  // all three channels of the haloed tile, one pass over the images
+  for (int idx = t; idx < HN; idx += LT) {
+    int gy = y0 + idx / HP, gx = x0 + idx % HP;
+    bool ok = gx >= 0 && gy >= 0 && gx < c.w && gy < c.h;
+    size_t gp = (size_t)c.pix_base + (size_t)gy * c.w + gx;
+    size_t tp = (size_t)c.tgt_base + (size_t)gy * c.w + gx;
+    float4 yv = ok ? *reinterpret_cast<const float4*>(in.target + tp * 4) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) s_xa[ch][idx] = ok ? in.rgb[gp * 3 + ch] : 0.0f;
+    s_ya[0][idx] = yv.x;
+    s_ya[1][idx] = yv.y;
+    s_ya[2][idx] = yv.z;
+  }
+  __syncthreads();
   for (int ch = 0; ch < 3; ++ch) {
-    for (int idx = t; idx < HN; idx += LT) {
-      int gy = y0 + idx / HP, gx = x0 + idx % HP;
-      bool ok = gx >= 0 && gy >= 0 && gx < c.w && gy < c.h;
-      size_t gp = (size_t)c.pix_base + (size_t)gy * c.w + gx;
-      size_t tp = (size_t)c.tgt_base + (size_t)gy * c.w + gx;
-      s_x[idx] = ok ? in.rgb[gp * 3 + ch] : 0.0f;
-      s_y[idx] = ok ? in.target[tp * 4 + ch] : 0.0f;
-    }
-    __syncthreads();
+    const float* s_x = s_xa[ch];
+    const float* s_y = s_ya[ch];
     for (int idx = t; idx < HH; idx += LT) {
       int r = idx / TILE, cc = idx % TILE;
       float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, a4 = 0.0f;
@@ -139,7 +146,7 @@ __global__ __launch_bounds__(LT) void k_ssim_bwd(LossIn in, Kern11 K, float* __r
   __shared__ float s_m[5][HN];
   __shared__ float s_h[5][HH];
   const int t = threadIdx.x;
-  const int tile = blockIdx.x;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);  // halo-sharing neighbours on one XCD
   int v = 0;
   while (v + 1 < in.n_views && in.cams[v + 1].tile_base <= tile) ++v;
   const CamDev& c = in.cams[v];
