@@ -305,8 +305,16 @@ __global__ __launch_bounds__(BT) void k_raster_bwd(RasterIn in, const float* __r
         float dy = fpy[q] - A.y;
         float power = -0.5f * (A.z * dx * dx + 2.0f * A.w * dx * dy + B.x * dy * dy);
         if (power > 0.0f) continue;  // NaN proceeds, as in the reference
-        float g_exp = expf_tab(power, s_tab);
+        // Hardware exp2 (relative error < 1e-6 where alpha can reach either
+        // threshold: power in [-5.6, 0]); within 1e-5 of a threshold the
+        // exact expf decides, so contribution and clamping match the forward.
+        float g_exp = __builtin_amdgcn_exp2f(power * 1.44269504088896341f);
         float alpha_raw = B.y * g_exp;
+        if (__builtin_fabsf(alpha_raw - ALPHA_THRESHOLD) <= 1e-5f * ALPHA_THRESHOLD ||
+            __builtin_fabsf(alpha_raw - MAX_ALPHA) <= 1e-5f * MAX_ALPHA) {
+          g_exp = expf_tab(power, s_tab);
+          alpha_raw = B.y * g_exp;
+        }
         float alpha = fmin_(alpha_raw, MAX_ALPHA);
         if (!(alpha >= ALPHA_THRESHOLD)) continue;
         contrib = true;

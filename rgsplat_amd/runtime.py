@@ -19,7 +19,7 @@ import numpy as np
 from . import synthetic
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "librgsplat_hip.so")
+LIB_PATH = os.environ.get("RGS_LIB", os.path.join(HERE, "librgsplat_hip.so"))
 
 _lib = None
 
