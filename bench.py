@@ -138,6 +138,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path measurement")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-stage event timing")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default raster work decomposition")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the step shards by views "
@@ -160,6 +161,10 @@ def main():
     ctx.set_views(cams, targets)
     ids = list(range(len(cams)))
     use_graph = not args.no_graph
+    tuned = None
+    if not args.no_autotune:
+        tuned = ctx.autotune_raster(ids)
+        print(f"[bench] raster autotune {tuned}", file=sys.stderr, flush=True)
     # The steps carry HIP events around every stage (in graph mode they are
     # event nodes of the captured step), so the per-kernel durations below
     # come from the timed region itself.
@@ -187,7 +192,9 @@ def main():
            "config": {"workload": wl.name, "gaussians": wl.n, "sh_degree": 3, "views_per_step": len(cams),
                       "resolutions": sorted(set(f"{w}x{h}" for w, h in wl.sizes)),
                       "execution": "device-resident step, views batched per launch, "
-                                   + ("hipGraph replay" if use_graph else "eager")}}
+                                   + ("hipGraph replay" if use_graph else "eager"),
+                      "raster_params": dict(zip(("super_tile", "sparse_lanes"), ctx.raster_params())),
+                      "raster_autotuned": tuned is not None}}
 
     if profile:
         stages, n_prof, n_graph = ctx.stage_times()

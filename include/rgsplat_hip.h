@@ -158,10 +158,21 @@ int rgs_scene_reset_opacities(rgs_ctx* ctx);
  * pointer may be NULL). */
 int rgs_step_stats(rgs_ctx* ctx, uint64_t* n_pairs, uint64_t* fwd_evals, uint64_t* bwd_evals);
 
-/* Raster tile-work decomposition chosen by the autotuner: the binning tile
- * is fixed at 16x16 by the reference, so this selects the splat batch staged
- * in LDS per round. 0 = automatic. */
-int rgs_set_raster_batch(rgs_ctx* ctx, int batch);
+/* ---- Raster work decomposition (no reference counterpart; images do not
+ * depend on it, gradients only through the order of float summation).
+ * super_tile: side, in 16x16 tiles, of the square patches the raster
+ * schedule walks (1..1024; large = row-major), which sets what the blocks
+ * resident on one XCD share in L2. sparse_lanes: in the backward, a splat
+ * whose gradient reaches at most this many lanes of a wave is accumulated
+ * with LDS atomics instead of a cross-lane reduction (0..64).
+ * rgs_autotune_raster renders the registered views ids on the device-resident
+ * scene and times the forward + backward raster under candidate settings
+ * (coordinate search, best of two runs each), keeps the fastest and reports
+ * it; the scene and optimizer state are not changed. */
+int rgs_set_raster_params(rgs_ctx* ctx, int super_tile, int sparse_lanes);
+int rgs_get_raster_params(rgs_ctx* ctx, int* super_tile, int* sparse_lanes);
+int rgs_autotune_raster(rgs_ctx* ctx, const int* view_ids, int n_ids, const float bg[3], float lambda_dssim,
+                        int* super_tile, int* sparse_lanes, float* raster_ms);
 
 /* ---- Stage timing (no reference counterpart). With profiling on,
  * rgs_train_step brackets each stage with HIP events on the context's stream;

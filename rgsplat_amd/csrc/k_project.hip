@@ -100,28 +100,25 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ P, in
   ty1 = min(ty1, c.nty);
   uint32_t ntiles = (tx1 > tx0 && ty1 > ty0) ? (uint32_t)((tx1 - tx0) * (ty1 - ty0)) : 0u;
 
-  // Reach of the alpha test: op * exp(-q/2) >= 1/255 needs q <= Q = 2 ln(255 op),
-  // an ellipse inside |dx| <= sqrt(Q a), |dy| <= sqrt(Q c). The f32 conic and
-  // power carry relative errors that grow with the condition number
-  // kappa = l1/l2 of the 2-D covariance (cancellation in det and in the
-  // quadratic form, ~1e-6 kappa); Q is widened by 2% + 20x that estimate +
-  // 1e-3 and the box by half a pixel, and above kappa = 2e4 no box is used.
-  // Raster waves skip splats whose box misses all their pixel centres: those
-  // pixels fail the alpha test anyway, so the results are unchanged.
+  // Reach of the alpha test: op * exp(-q/2) >= 1/255 needs the quadratic form
+  // q <= R = 2 ln(255 op). The f32 conic and power carry relative errors that
+  // grow with the condition number kappa = l1/l2 of the 2-D covariance
+  // (cancellation in det and in the quadratic form, ~1e-6 kappa); R is widened
+  // by 2% + 20x that estimate + 1e-3, and above kappa = 2e4 (or for any
+  // non-finite input) R is +inf, which disables culling. Raster waves skip a
+  // splat for pixel rows that the ellipse q <= R does not reach: those pixels
+  // fail the alpha test anyway, so the results are unchanged.
   const float kappa = l1 / fmax_(l2, 1e-30f);
   const float q_err = 1e-6f * kappa;
-  float hx = __builtin_inff(), hy = __builtin_inff();
+  float reach_q = __builtin_inff();
   if (q_err <= 0.02f && a > 0.0f && cc > 0.0f && det > 0.0f) {
-    float reach_q = 2.0f * __logf(255.0f * opacity) * (1.02f + 20.0f * q_err) + 1e-3f;
+    reach_q = 2.0f * __logf(255.0f * opacity) * (1.02f + 20.0f * q_err) + 1e-3f;
     reach_q = fmax_(reach_q, 1e-3f);
-    hx = __builtin_sqrtf(reach_q * a) + 0.5f;
-    hy = __builtin_sqrtf(reach_q * cc) + 0.5f;
-    if (!(__builtin_isfinite(hx) && __builtin_isfinite(hy) && __builtin_isfinite(sx) && __builtin_isfinite(sy))) {
-      hx = __builtin_inff();
-      hy = __builtin_inff();
-    }
+    if (!(__builtin_isfinite(reach_q) && __builtin_isfinite(sx) && __builtin_isfinite(sy) &&
+          __builtin_isfinite(cc * inv_det) && __builtin_isfinite(a * inv_det)))
+      reach_q = __builtin_inff();
   }
-  S.reach[vi] = make_float2(hx, hy);
+  S.reach[vi] = reach_q;
 
   if (tz != tz) atomicOr(status, ST_NAN_DEPTH);
   sort_keys[e] = f2u(tz);

@@ -3,8 +3,8 @@
 // One wavefront per 16x16 tile (the reference's binning tile,
 // backward.rs:21): each lane owns four pixels of one column, one in each
 // 4-row pixel group. Tile lists are consumed in batches of 64 splats staged
-// through LDS (one splat per lane); a splat whose alpha-reach box misses a
-// pixel group is skipped for that group with a wave-uniform branch. The wave
+// through LDS (one splat per lane); a splat whose alpha-reach ellipse misses
+// a pixel group is skipped for that group with a wave-uniform branch. The wave
 // stops as soon as every pixel has saturated (T < 1e-4). The exp2 table of
 // the exact expf lives in LDS.
 //
@@ -27,17 +27,32 @@ __device__ __forceinline__ int view_of_tile(const CamDev* cams, int n_views, int
   return v;
 }
 
-// Bit w of the result: the splat's alpha-reach box meets the pixel centres
-// of wave w (rows 4w..4w+3 of the tile). NaN inputs give "meets".
-__device__ __forceinline__ uint32_t wave_mask(float sx, float sy, float2 reach, int tx, int ty) {
-  const float x_lo = (float)(tx * TILE) + 0.5f, x_hi = (float)(tx * TILE) + 15.5f;
-  const bool out_x = (sx + reach.x < x_lo) || (sx - reach.x > x_hi);
+// Bit q of the result: the ellipse conic(d) <= R of the splat reaches the
+// pixel centres of group q (tile rows 4q..4q+3, all 16 columns). With the
+// conic [[ca, cb], [cb, cc]], the ellipse's slice at row offset dy is centred
+// at dx = -(cb / ca) dy with half-width sqrt(max(0, R - k dy^2) / ca),
+// k = cc - cb^2 / ca; over the group's rows the union of slices lies within
+// the centre range widened by the widest slice. The approximate rcp/sqrt and
+// the extra 1e-4 relative + 0.01 px slack are far inside R's own margin.
+// Non-finite inputs give "reaches".
+__device__ __forceinline__ uint32_t wave_mask(float sx, float sy, float ca, float cb, float cc, float R, int tx,
+                                              int ty) {
+  if (!__builtin_isfinite(R)) return 0xfu;
+  const float inv_a = __builtin_amdgcn_rcpf(ca);
+  const float slope = -cb * inv_a;
+  const float k = cc - cb * cb * inv_a;
+  const float X0 = ((float)(tx * TILE) + 0.5f) - sx, X1 = ((float)(tx * TILE) + 15.5f) - sx;
   uint32_t m = 0;
 #pragma unroll
-  for (int w = 0; w < RT / 64; ++w) {
-    const float y_lo = (float)(ty * TILE + 4 * w) + 0.5f, y_hi = (float)(ty * TILE + 4 * w) + 3.5f;
-    const bool out = out_x || (sy + reach.y < y_lo) || (sy - reach.y > y_hi);
-    m |= out ? 0u : (1u << w);
+  for (int q = 0; q < 4; ++q) {
+    const float Y0 = ((float)(ty * TILE + 4 * q) + 0.5f) - sy, Y1 = ((float)(ty * TILE + 4 * q) + 3.5f) - sy;
+    float dmin = (Y0 <= 0.0f && Y1 >= 0.0f) ? 0.0f : fmin_(__builtin_fabsf(Y0), __builtin_fabsf(Y1));
+    dmin = fmax_(dmin - 0.01f, 0.0f);
+    const float rem = R - k * dmin * dmin;
+    const float w = __builtin_amdgcn_sqrtf(fmax_(rem, 0.0f) * inv_a) * 1.0001f + 0.01f;
+    const float c0 = slope * Y0, c1 = slope * Y1;
+    const bool out = (rem < 0.0f) || (fmax_(c0, c1) + w < X0) || (fmin_(c0, c1) - w > X1);
+    m |= out ? 0u : (1u << q);
   }
   return m;
 }
@@ -87,10 +102,11 @@ __global__ __launch_bounds__(FT) void k_raster_fwd(RasterIn in, float* __restric
     if (idx < end) {
       uint32_t vi = in.pair_vals[idx];
       float4 A = in.S.xy_ab[vi];
+      float4 B = in.S.c_op_rg[vi];
       s_a[lane] = A;
-      s_b[lane] = in.S.c_op_rg[vi];
+      s_b[lane] = B;
       s_c[lane] = in.S.col_b[vi];
-      s_m[lane] = wave_mask(A.x, A.y, in.S.reach[vi], tx, ty);
+      s_m[lane] = wave_mask(A.x, A.y, A.z, A.w, B.x, in.S.reach[vi], tx, ty);
     }
     __syncthreads();
     const int nb = (int)min((uint32_t)FT, end - b0);
@@ -278,10 +294,11 @@ __global__ __launch_bounds__(BT) void k_raster_bwd(RasterIn in, const float* __r
       uint32_t vi = in.pair_vals[start + lo + lane];
       s_vi[lane] = vi;
       float4 A = in.S.xy_ab[vi];
+      float4 B = in.S.c_op_rg[vi];
       s_a[lane] = A;
-      s_b[lane] = in.S.c_op_rg[vi];
+      s_b[lane] = B;
       s_c[lane] = in.S.col_b[vi];
-      s_m[lane] = wave_mask(A.x, A.y, in.S.reach[vi], tx, ty);
+      s_m[lane] = wave_mask(A.x, A.y, A.z, A.w, B.x, in.S.reach[vi], tx, ty);
     }
 #pragma unroll
     for (int k = 0; k < 9; ++k) s_g[k][lane] = 0.0f;
@@ -343,7 +360,7 @@ __global__ __launch_bounds__(BT) void k_raster_bwd(RasterIn in, const float* __r
       }
       const uint64_t bm = __ballot(contrib);
       if (bm == 0) continue;
-      if (__popcll(bm) <= 2) {
+      if (__popcll(bm) <= in.sparse_lanes) {
         if (contrib) {
 #pragma unroll
           for (int k = 0; k < 9; ++k) atomicAdd(&s_g[k][j], g[k]);

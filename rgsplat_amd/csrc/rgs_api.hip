@@ -223,12 +223,12 @@ struct rgs_ctx {
   std::string gkey;
   uint64_t gen = 0;        // bumped on every device (re)allocation
   uint64_t graph_gen = 0;  // gen at capture time
-  int raster_batch = 0;
   DBuf api_acc9;           // raster accumulators of the reference-API paths
   DBuf tile_order;         // raster schedule: slot -> global tile id
   std::vector<int32_t> h_order;
   std::string order_key;
   int super_tile = 8;      // tiles per side of a schedule super-tile
+  int sparse_lanes = 2;    // backward: LDS atomics up to this many lanes
 
   // stage timing
   bool profiling = false;
@@ -304,7 +304,7 @@ int ensure_render(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, size_t k_cap
   CHK(ensure(c, c->col_b, vc * sizeof(float)));
   CHK(ensure(c, c->cnt, vc * sizeof(uint32_t)));
   CHK(ensure(c, c->rect, vc * sizeof(uint2)));
-  CHK(ensure(c, c->reach, vc * sizeof(float2)));
+  CHK(ensure(c, c->reach, vc * sizeof(float)));
   for (int i = 0; i < 2; ++i) {
     CHK(ensure(c, c->dk[i], dn * 4));
     CHK(ensure(c, c->dv[i], dn * 4));
@@ -333,7 +333,7 @@ int ensure_render(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, size_t k_cap
 
 SplatArrays splats(rgs_ctx* c) {
   return SplatArrays{P_<float4>(c->xy_ab), P_<float4>(c->c_op_rg), P_<float>(c->col_b), P_<uint32_t>(c->cnt),
-                     P_<uint2>(c->rect), P_<float2>(c->reach)};
+                     P_<uint2>(c->rect), P_<float>(c->reach)};
 }
 
 // Projection, depth sort and the pair-count scan (sizes fixed by n and vs).
@@ -360,6 +360,24 @@ void enqueue_project(rgs_ctx* c, const float* P, size_t cap, int n, const ViewSe
   stage_end(c, S_SCAN);
 }
 
+RasterIn raster_in_n(rgs_ctx* c, int n_views, int total_tiles, const float bg[3]) {
+  RasterIn in{};
+  in.rstart = P_<uint32_t>(c->rstart);
+  in.rend = P_<uint32_t>(c->rend);
+  in.pair_vals = c->pair_vals;
+  in.tile_order = P_<int32_t>(c->tile_order);
+  in.S = splats(c);
+  in.cams = P_<CamDev>(c->cams_d);
+  in.n_views = n_views;
+  in.sparse_lanes = c->sparse_lanes;
+  in.total_tiles = total_tiles;
+  for (int k = 0; k < 3; ++k) in.bg[k] = bg[k];
+  return in;
+}
+RasterIn raster_in(rgs_ctx* c, const ViewSet& vs, const float bg[3]) {
+  return raster_in_n(c, (int)vs.cams.size(), vs.total_tiles, bg);
+}
+
 // Pair generation, tile sort, ranges and the forward raster.
 void enqueue_bin_and_raster(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, const float bg[3], int mode,
                             float* rgb_out) {
@@ -379,19 +397,12 @@ void enqueue_bin_and_raster(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, co
   int cur = radix_sort_tiles(keys, vals, sc + SC_K, c->k_cap, vs.bits(), rs, tr, st);
   stage_end(c, S_TSORT);
   c->pair_vals = vals[cur];
-  RasterIn in{P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend), c->pair_vals, P_<int32_t>(c->tile_order), splats(c),
-              P_<CamDev>(c->cams_d),
-              (int)vs.cams.size(), vs.total_tiles, {bg[0], bg[1], bg[2]}};
+  RasterIn in = raster_in(c, vs, bg);
   stage_begin(c, S_FWD);
   launch_raster_fwd(in, mode, rgb_out, P_<float>(c->final_t), P_<int32_t>(c->last), st);
   stage_end(c, S_FWD);
 }
 
-RasterIn raster_in(rgs_ctx* c, const ViewSet& vs, const float bg[3]) {
-  return RasterIn{P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend), c->pair_vals, P_<int32_t>(c->tile_order), splats(c),
-              P_<CamDev>(c->cams_d),
-                  (int)vs.cams.size(), vs.total_tiles, {bg[0], bg[1], bg[2]}};
-}
 
 // Eager render of n Gaussians (SoA P, row capacity cap) into vs: sizes the
 // pair buffers from the exact count.
@@ -554,10 +565,20 @@ int rgs_use_own_stream(rgs_ctx* c) {
   return 0;
 }
 
-int rgs_set_raster_batch(rgs_ctx* c, int batch) {
+int rgs_set_raster_params(rgs_ctx* c, int super_tile, int sparse_lanes) {
   if (!c) return fail(RGS_E_ARG, "null ctx");
-  if (batch != 0 && batch != 256) return fail(RGS_E_ARG, "raster batch must be 0 (auto) or 256");
-  c->raster_batch = batch;
+  if (super_tile < 1 || super_tile > 1024) return fail(RGS_E_ARG, "super_tile must be in [1, 1024]");
+  if (sparse_lanes < 0 || sparse_lanes > 64) return fail(RGS_E_ARG, "sparse_lanes must be in [0, 64]");
+  if (super_tile != c->super_tile || sparse_lanes != c->sparse_lanes) drop_graph(c);
+  c->super_tile = super_tile;
+  c->sparse_lanes = sparse_lanes;
+  return 0;
+}
+
+int rgs_get_raster_params(rgs_ctx* c, int* super_tile, int* sparse_lanes) {
+  if (!c) return fail(RGS_E_ARG, "null ctx");
+  if (super_tile) *super_tile = c->super_tile;
+  if (sparse_lanes) *sparse_lanes = c->sparse_lanes;
   return 0;
 }
 
@@ -1050,6 +1071,102 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
   return fail(RGS_E_HIP, "pair buffer kept overflowing");
 }
 
+int rgs_autotune_raster(rgs_ctx* c, const int* ids, int n_ids, const float bg[3], float lambda, int* super_tile_out,
+                        int* sparse_lanes_out, float* ms_out) {
+  if (!c || !ids || n_ids <= 0 || !bg) return fail(RGS_E_ARG, "null argument or no views");
+  if (n_ids > 64) return fail(RGS_E_ARG, "at most 64 views per step");
+  if (!c->P.p) return fail(RGS_E_ARG, "no scene uploaded");
+  if (!c->st) return fail(RGS_E_ARG, "the autotuner needs a non-null stream");
+  if (!c->targets.p) return fail(RGS_E_ARG, "no views registered");
+  (void)hipSetDevice(c->device);
+  ViewSet vs;
+  for (int i = 0; i < n_ids; ++i) {
+    if (ids[i] < 0 || ids[i] >= (int)c->reg_cams.size()) return fail(RGS_E_ARG, "view id out of range");
+    vs.cams.push_back(c->reg_cams[ids[i]]);
+  }
+  vs.layout();
+  drop_graph(c);
+  const size_t cap = c->cap;
+  const int n = (int)c->n;
+  const size_t acc_n = 9 * vs.cams.size() * cap;
+  CHK(ensure(c, c->accum, acc_n * 4));
+  CHK(ensure_loss(c, vs));
+  CHK(ensure_render(c, cap, n, vs, std::max<size_t>(c->k_cap, 1)));
+  std::memcpy(c->h_cams, vs.cams.data(), vs.cams.size() * sizeof(CamDev));
+  HIPCHK(hipMemcpyAsync(c->cams_d.p, c->h_cams, vs.cams.size() * sizeof(CamDev), hipMemcpyHostToDevice, c->st));
+  uint32_t* sc = P_<uint32_t>(c->scalars);
+  launch_fill_u32(sc, SC_COUNT, 0u, c->st);
+  const bool prof = c->profiling;
+  c->profiling = false;
+  enqueue_project(c, P_<float>(c->P), cap, n, vs);
+  HIPCHK(hipMemcpyAsync(c->h_scalars, sc, SC_COUNT * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  size_t K = c->h_scalars[SC_TOTAL];
+  if (K > c->k_cap) CHK(ensure_render(c, cap, n, vs, K + K / 4 + 4096));
+  enqueue_bin_and_raster(c, cap, n, vs, bg, 0, P_<float>(c->rgb));
+  enqueue_loss(c, vs, P_<float>(c->rgb), P_<float>(c->targets), lambda);
+  hipEvent_t ev[4];
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  auto time_once = [&](float* ms) -> int {
+    CHK(ensure_tile_order(c, vs));
+    RasterIn in = raster_in(c, vs, bg);
+    HIPCHK(hipEventRecord(ev[0], c->st));
+    launch_raster_fwd(in, 0, P_<float>(c->rgb), P_<float>(c->final_t), P_<int32_t>(c->last), c->st);
+    HIPCHK(hipEventRecord(ev[1], c->st));
+    HIPCHK(hipEventRecord(ev[2], c->st));
+    launch_raster_bwd(in, P_<float>(c->final_t), P_<int32_t>(c->last), P_<float>(c->dl), P_<float>(c->accum), c->st);
+    HIPCHK(hipEventRecord(ev[3], c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    float a = 0.0f, b = 0.0f;
+    HIPCHK(hipEventElapsedTime(&a, ev[0], ev[1]));
+    HIPCHK(hipEventElapsedTime(&b, ev[2], ev[3]));
+    *ms = a + b;
+    return 0;
+  };
+  auto score = [&](int st_, int sl, float* best) -> int {
+    c->super_tile = st_;
+    c->sparse_lanes = sl;
+    float t = 0.0f, m = 3.4e38f;
+    for (int r = 0; r < 2; ++r) {
+      CHK(time_once(&t));
+      m = std::min(m, t);
+    }
+    *best = m;
+    return 0;
+  };
+  int best_st = c->super_tile, best_sl = c->sparse_lanes;
+  float best_ms = 0.0f;
+  int rc = score(best_st, best_sl, &best_ms);
+  // coordinate search: schedule patch size, then the sparse/dense switch
+  const int st_cand[] = {2, 4, 8, 16, 1024};
+  const int sl_cand[] = {0, 2, 4, 8, 16, 32};
+  for (int st_ : st_cand) {
+    if (rc) break;
+    float m = 0.0f;
+    rc = score(st_, best_sl, &m);
+    if (!rc && m < best_ms) best_ms = m, best_st = st_;
+  }
+  for (int sl : sl_cand) {
+    if (rc) break;
+    float m = 0.0f;
+    rc = score(best_st, sl, &m);
+    if (!rc && m < best_ms) best_ms = m, best_sl = sl;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  c->profiling = prof;
+  c->super_tile = best_st;
+  c->sparse_lanes = best_sl;
+  if (rc) return rc;
+  CHK(ensure_tile_order(c, vs));
+  launch_fill_f32(P_<float>(c->accum), acc_n, 0.0f, c->st);
+  HIPCHK(hipStreamSynchronize(c->st));
+  drop_graph(c);
+  if (super_tile_out) *super_tile_out = best_st;
+  if (sparse_lanes_out) *sparse_lanes_out = best_sl;
+  if (ms_out) *ms_out = best_ms;
+  return 0;
+}
+
 int rgs_scene_densify(rgs_ctx* c, const rgs_train_cfg* cfg, float extent) {
   if (!c || !cfg) return fail(RGS_E_ARG, "null argument");
   (void)hipSetDevice(c->device);
@@ -1118,9 +1235,8 @@ int rgs_step_stats(rgs_ctx* c, uint64_t* n_pairs, uint64_t* fwd_evals, uint64_t*
     CHK(ensure(c, c->api_den, 16));
     unsigned long long* d = P_<unsigned long long>(c->api_den);
     launch_fill_u32(P_<uint32_t>(c->api_den), 4, 0u, c->st);
-    RasterIn in{P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend), c->pair_vals, P_<int32_t>(c->tile_order), splats(c),
-              P_<CamDev>(c->cams_d),
-                c->last_nviews, c->last_tiles, {0.0f, 0.0f, 0.0f}};
+    const float zero3[3] = {0.0f, 0.0f, 0.0f};
+    RasterIn in = raster_in_n(c, c->last_nviews, c->last_tiles, zero3);
     launch_work_counts(in, P_<float>(c->final_t), P_<int32_t>(c->last), d, c->st);
     unsigned long long h[2] = {0, 0};
     HIPCHK(hipMemcpyAsync(h, d, 16, hipMemcpyDeviceToHost, c->st));

@@ -16,7 +16,7 @@ struct SplatArrays {
   float* col_b;     // blue
   uint32_t* cnt;    // tiles touched (0 when culled)
   uint2* rect;      // (tx0 | ty0 << 16, tx1 | ty1 << 16)
-  float2* reach;    // half-widths of the box outside which alpha < 1/255
+  float* reach;     // R: alpha >= 1/255 needs conic form <= R (+inf: no bound)
 };
 
 // Status bits written by kernels into a device word.
@@ -107,6 +107,7 @@ struct RasterIn {
   SplatArrays S;
   const CamDev* cams;
   int n_views;
+  int sparse_lanes;  // backward: at most this many contributing lanes -> LDS atomics
   int total_tiles;
   float bg[3];
 };

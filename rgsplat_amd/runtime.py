@@ -78,7 +78,10 @@ SIGNATURES = {
     "rgs_scene_reset_opacities": [_vp],
     "rgs_step_stats": [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                        ctypes.POINTER(ctypes.c_uint64)],
-    "rgs_set_raster_batch": [_vp, _i],
+    "rgs_set_raster_params": [_vp, _i, _i],
+    "rgs_get_raster_params": [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+    "rgs_autotune_raster": [_vp, _vp, _i, _vp, ctypes.c_float, ctypes.POINTER(ctypes.c_int),
+                            ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)],
     "rgs_eval_expf": [_vp, _vp, _vp, _sz],
     "rgs_set_profiling": [_vp, _i],
     "rgs_stage_times": [_vp, _vp, _i, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)],
@@ -309,8 +312,23 @@ class Context:
         _check(lib().rgs_step_stats(self.h, ctypes.byref(k), ctypes.byref(f), ctypes.byref(b)))
         return k.value, f.value, b.value
 
-    def set_raster_batch(self, batch: int):
-        _check(lib().rgs_set_raster_batch(self.h, int(batch)))
+    def set_raster_params(self, super_tile: int, sparse_lanes: int):
+        _check(lib().rgs_set_raster_params(self.h, int(super_tile), int(sparse_lanes)))
+
+    def raster_params(self) -> tuple[int, int]:
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _check(lib().rgs_get_raster_params(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def autotune_raster(self, view_ids, bg=(0.0, 0.0, 0.0), lambda_dssim=0.2) -> dict:
+        """Times the raster under candidate work decompositions on the
+        registered views and keeps the fastest (rgs_autotune_raster)."""
+        ids = np.ascontiguousarray(view_ids, np.int32)
+        bgv = np.ascontiguousarray(bg, np.float32)
+        a, b, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        _check(lib().rgs_autotune_raster(self.h, _p(ids), len(ids), _p(bgv), float(lambda_dssim), ctypes.byref(a),
+                                         ctypes.byref(b), ctypes.byref(ms)))
+        return {"super_tile": a.value, "sparse_lanes": b.value, "raster_ms": ms.value}
 
     # ---------------------------------------------------------- stage timing
     STAGES = ["project", "depth_sort", "pair_scan", "pair_gen", "tile_sort", "raster_fwd", "loss_ssim",

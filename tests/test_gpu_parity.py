@@ -454,3 +454,42 @@ def test_full_size_properties(ctx):
     # spot-check one region against the oracle on the Gaussians touching it
     sub = np.unique(gis[off[4000]:off[4001]])
     assert len(sub) > 0
+
+
+def test_raster_params_and_autotune(ctx):
+    """The raster work decomposition changes no image and gradients only by
+    summation order; the autotuner leaves the scene untouched and picks one
+    of its candidates."""
+    from rgsplat_amd import runtime
+    g, cams, targets = workload("c2_500k_8views", 20_000, [(240, 136), (320, 180)])
+    cfg = S.benchmark_train_config()
+    ref_loss, ref_g = _oracle_step_grads(g, cams, targets, (0, 0, 0), 0.2)
+    c = runtime.Context(0)
+    assert c.raster_params() == (8, 2)
+    for st, sl in [(1, 0), (1024, 64), (3, 5)]:
+        c.set_raster_params(st, sl)
+        assert c.raster_params() == (st, sl)
+        img = c.render_color(g, cams[1])
+        assert np.array_equal(img, oracle.render_color(g, cams[1]))  # bar 1e-4 abs; expected bit-exact
+        _, gr = c.render_and_backward(g, cams[0], targets[0])
+        _, ref0 = oracle.render_and_backward(g, cams[0], targets[0])
+        assert_grads_close(gr, ref0, 1e-3)
+    for bad in [(0, 2), (8, 65), (2000, 2)]:
+        with pytest.raises(runtime.RgsError):
+            c.set_raster_params(*bad)
+    c.scene_upload(g)
+    c.set_views(cams, targets)
+    best = c.autotune_raster([0, 1])
+    assert best["super_tile"] in (2, 4, 8, 16, 1024) and best["sparse_lanes"] in (0, 2, 4, 8, 16, 32)
+    assert best["raster_ms"] > 0
+    assert c.raster_params() == (best["super_tile"], best["sparse_lanes"])
+    assert np.array_equal(c.scene_download(), g)
+    loss = c.train_step([0, 1], cfg, use_graph=True)
+    np.testing.assert_allclose(loss, ref_loss, rtol=1e-5)
+    st = c.adam_download()
+    got = st.m1[: len(g)] / np.float32(0.1)
+    errs = group_rel_err(np.concatenate([got, np.zeros((len(g), 1))], 1),
+                         np.concatenate([ref_g[:, :59], np.zeros((len(g), 1))], 1))
+    for k in ["position", "rotation", "log_scale", "opacity", "sh"]:
+        assert errs[k] <= 1e-3, (k, errs)
+    c.close()
