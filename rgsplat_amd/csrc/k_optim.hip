@@ -23,8 +23,7 @@ __device__ __forceinline__ int aos_offset(int j) {
 // Per view: SH basis at the view direction and the clamp-masked dL/dcolour
 // (backward.rs:533-544, sh.rs:39-61,122-139).
 template <class Load>
-__device__ __forceinline__ void view_sh(const CamDev& c, Load prm, const float acc[9], float basis[16],
-                                        float masked[3]) {
+__device__ __forceinline__ void view_basis(const CamDev& c, Load prm, float basis[16]) {
   const float px = prm(0), py = prm(1), pz = prm(2);
   float u0 = c.cam_pos[0] - px, u1 = c.cam_pos[1] - py, u2 = c.cam_pos[2] - pz;
   float ulen = __builtin_sqrtf(u0 * u0 + u1 * u1 + u2 * u2);
@@ -35,14 +34,37 @@ __device__ __forceinline__ void view_sh(const CamDev& c, Load prm, const float a
     d0 = u0 / ulen; d1 = u1 / ulen; d2 = u2 / ulen;
   }
   sh_basis(d0, d1, d2, basis);
+}
+// bits: bit ch set when channel ch's colour is inside the clamp (0, 1).
+// Also dk[k] = sum_ch masked[ch] * sh[ch][k], the colour gradient contracted
+// with the coefficients, which the position gradient through the view
+// direction needs. One channel's 16 coefficients are live at a time.
+template <class Load>
+__device__ __forceinline__ void view_sh(const CamDev& c, Load prm, const float acc[9], float basis[16],
+                                        float masked[3], float dk[16], uint32_t* bits = nullptr) {
+  view_basis(c, prm, basis);
+  uint32_t b = 0;
 #pragma unroll
   for (int ch = 0; ch < 3; ++ch) {
+    float cf[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) cf[k] = prm(P_SH + ch * 16 + k);
     float a = 0.0f;
 #pragma unroll
-    for (int k = 0; k < 16; ++k) a += basis[k] * prm(P_SH + ch * 16 + k);
+    for (int k = 0; k < 16; ++k) a += basis[k] * cf[k];
     float raw = 0.5f + a;
-    masked[ch] = (raw > 0.0f && raw < 1.0f) ? acc[ch] : 0.0f;
+    const bool in = raw > 0.0f && raw < 1.0f;
+    masked[ch] = in ? acc[ch] : 0.0f;
+    b |= (in ? 1u : 0u) << ch;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      if (ch == 0)
+        dk[k] = masked[0] * cf[k];
+      else
+        dk[k] = dk[k] + masked[ch] * cf[k];
+    }
   }
+  if (bits) *bits = b;
 }
 
 // Gradients of everything but the SH coefficients: position (0-2), rotation
@@ -50,7 +72,7 @@ __device__ __forceinline__ void view_sh(const CamDev& c, Load prm, const float a
 // (59) (backward.rs:529-531, 546-636). ADD accumulates over views.
 template <bool ADD, class Load>
 __device__ __forceinline__ void geometry_nonsh(const CamDev& c, Load prm, const float acc[9],
-                                               const float masked[3], float* out) {
+                                               const float dk[16], float* out) {
   const float px = prm(0), py = prm(1), pz = prm(2);
   const float q[4] = {prm(3), prm(4), prm(5), prm(6)};
   const float scale[3] = {rgs_expf(prm(7)), rgs_expf(prm(8)), rgs_expf(prm(9))};
@@ -161,10 +183,6 @@ __device__ __forceinline__ void geometry_nonsh(const CamDev& c, Load prm, const 
   float ulen = __builtin_sqrtf(u0 * u0 + u1 * u1 + u2 * u2);
   if (ulen > 1e-8f) {
     float e0 = u0 / ulen, e1 = u1 / ulen, e2 = u2 / ulen;
-    float dk[16];
-#pragma unroll
-    for (int k = 0; k < 16; ++k)
-      dk[k] = masked[0] * prm(P_SH + k) + masked[1] * prm(P_SH + 16 + k) + masked[2] * prm(P_SH + 32 + k);
     float dd[3];
     sh_basis_grad_contract(e0, e1, e2, dk, dd);
     float dot = e0 * dd[0] + e1 * dd[1] + e2 * dd[2];
@@ -206,11 +224,11 @@ __global__ __launch_bounds__(256) void k_geometry_aos(const float* __restrict__ 
   for (int k = 0; k < 9; ++k) acc[k] = accum[vi * 9 + k];
   SoaLoad L{P, (size_t)cap, (size_t)i};
   const CamDev& c = cams[view];
-  float basis[16], masked[3];
-  view_sh(c, L, acc, basis, masked);
+  float basis[16], masked[3], dk[16];
+  view_sh(c, L, acc, basis, masked, dk);
   for (int ch = 0; ch < 3; ++ch)
     for (int k = 0; k < 16; ++k) o[P_SH + ch * 16 + k] = masked[ch] * basis[k];
-  geometry_nonsh<false>(c, L, acc, masked, o);
+  geometry_nonsh<false>(c, L, acc, dk, o);
 }
 
 void launch_geometry_aos(const float* P, int cap, int n, const CamDev* cams, int view, const uint32_t* cnt,
@@ -277,11 +295,76 @@ void launch_adam_aos(float* g, const float* grads, size_t n, float* m1, float* m
                      denom, k);
 }
 
-__global__ __launch_bounds__(256, 2) void k_train_adam(float* __restrict__ P, float* __restrict__ M1,
+// Training step tail, two kernels over Gaussians (SoA parameters):
+//   k_train_geometry: per view the Gaussian is visible in, the non-SH
+//     geometry backprop, summed over views, and the colour-clamp bits of each
+//     (view, channel); written to a scratch of TG_ROWS rows (TG_G + 11 grads,
+//     TG_SGN screen-gradient norm, TG_BITS 3 x 64-bit clamp masks);
+//   k_train_adam: per colour channel the SH gradient sum over views (basis
+//     recomputed from the position, the clamp bit from the scratch), then
+//     Adam on all 59 rows, quaternion renormalisation, the densification
+//     statistics, and clearing of the raster accumulators.
+// The first reads the SH coefficients before the second updates them, and
+// the second reads the positions before it updates them, as in the
+// reference's order (gradients of the whole step first, then Adam).
+constexpr int TG_G = 0, TG_SGN = 11, TG_BITS = 12;
+static_assert(TRAIN_SCRATCH_ROWS == 18, "scratch rows");
+
+__global__ __launch_bounds__(256) void k_train_geometry(const float* __restrict__ P, int cap, int n,
+                                                        const CamDev* __restrict__ cams, int n_views,
+                                                        const uint32_t* __restrict__ cnt,
+                                                        const float* __restrict__ accum, float* __restrict__ scr,
+                                                        const uint32_t* __restrict__ status) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (*status & (ST_PAIR_OVERFLOW | ST_NAN_DEPTH)) return;
+  SoaLoad L{P, (size_t)cap, (size_t)i};
+  float g[11];  // flat parameters 0-10
+#pragma unroll
+  for (int j = 0; j < 11; ++j) g[j] = 0.0f;
+  float sgn = 0.0f;
+  uint64_t clamp_in[3] = {0, 0, 0};  // per channel: views whose colour is inside (0, 1)
+  for (int v = 0; v < n_views; ++v) {
+    size_t vi = (size_t)v * cap + i;
+    if (cnt[vi] == 0) continue;
+    // reload the parameters in every view instead of keeping all 59 live
+    // across the loop (an empty asm makes the base pointer loop-variant)
+    const float* Pv = P;
+    asm volatile("" : "+s"(Pv));
+    SoaLoad Lv{Pv, (size_t)cap, (size_t)i};
+    float acc[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) acc[q] = accum[vi * 9 + q];
+    float basis[16], masked[3], dk[16];
+    uint32_t bits = 0;
+    view_sh(cams[v], Lv, acc, basis, masked, dk, &bits);
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) clamp_in[ch] |= (uint64_t)(bits >> ch & 1u) << v;
+    float gg[60];
+    gg[59] = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 11; ++j) gg[j] = g[j];
+    geometry_nonsh<true>(cams[v], Lv, acc, dk, gg);
+#pragma unroll
+    for (int j = 0; j < 11; ++j) g[j] = gg[j];
+    sgn += gg[59];
+  }
+#pragma unroll
+  for (int j = 0; j < 11; ++j) scr[(size_t)(TG_G + j) * cap + i] = g[j];
+  scr[(size_t)TG_SGN * cap + i] = sgn;
+  uint32_t* bits_out = reinterpret_cast<uint32_t*>(scr);
+#pragma unroll
+  for (int ch = 0; ch < 3; ++ch) {
+    bits_out[(size_t)(TG_BITS + 2 * ch) * cap + i] = (uint32_t)clamp_in[ch];
+    bits_out[(size_t)(TG_BITS + 2 * ch + 1) * cap + i] = (uint32_t)(clamp_in[ch] >> 32);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_train_adam(float* __restrict__ P, float* __restrict__ M1,
                                                     float* __restrict__ M2, int cap, int n,
                                                     const CamDev* __restrict__ cams, int n_views,
                                                     const uint32_t* __restrict__ cnt, float* __restrict__ accum,
-                                                    float* __restrict__ gaccum,
+                                                    const float* __restrict__ scr, float* __restrict__ gaccum,
                                                     uint32_t* __restrict__ gdenom,
                                                     const AdamConsts* __restrict__ k_dev,
                                                     const uint32_t* __restrict__ status) {
@@ -289,46 +372,25 @@ __global__ __launch_bounds__(256, 2) void k_train_adam(float* __restrict__ P, fl
   if (i >= n) return;
   if (*status & (ST_PAIR_OVERFLOW | ST_NAN_DEPTH)) return;
   const AdamConsts k_ = *k_dev;
-  // Geometry of every view this Gaussian is visible in, summed; the SH part
-  // one colour channel at a time so that only 16 SH sums are live.
   SoaLoad L{P, (size_t)cap, (size_t)i};
-  float g[11];  // flat parameters 0-10
-#pragma unroll
-  for (int j = 0; j < 11; ++j) g[j] = 0.0f;
-  float sgn = 0.0f;
-  uint32_t vis = 0;
-  for (int v = 0; v < n_views; ++v) {
-    size_t vi = (size_t)v * cap + i;
-    if (cnt[vi] == 0) continue;
-    vis |= 1u << v;
-    float acc[9];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) acc[q] = accum[vi * 9 + q];
-    float basis[16], masked[3];
-    view_sh(cams[v], L, acc, basis, masked);
-    float gg[60];
-    gg[59] = 0.0f;
-#pragma unroll
-    for (int j = 0; j < 11; ++j) gg[j] = g[j];
-    geometry_nonsh<true>(cams[v], L, acc, masked, gg);
-#pragma unroll
-    for (int j = 0; j < 11; ++j) g[j] = gg[j];
-    sgn += gg[59];
-  }
+  uint64_t vis = 0;
+  for (int v = 0; v < n_views; ++v)
+    if (cnt[(size_t)v * cap + i] != 0) vis |= 1ull << v;
+  const uint32_t* bits_in = reinterpret_cast<const uint32_t*>(scr);
   for (int ch = 0; ch < 3; ++ch) {
+    const uint64_t clamp_in = (uint64_t)bits_in[(size_t)(TG_BITS + 2 * ch) * cap + i] |
+                              (uint64_t)bits_in[(size_t)(TG_BITS + 2 * ch + 1) * cap + i] << 32;
     float gs[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) gs[k] = 0.0f;
     for (int v = 0; v < n_views; ++v) {
-      if (!(vis >> v & 1u)) continue;
+      if (!(vis >> v & 1ull)) continue;
       size_t vi = (size_t)v * cap + i;
-      float acc[9];
+      const float m = (clamp_in >> v & 1ull) ? accum[vi * 9 + ch] : 0.0f;
+      float basis[16];
+      view_basis(cams[v], L, basis);
 #pragma unroll
-      for (int q = 0; q < 3; ++q) acc[q] = accum[vi * 9 + q];
-      float basis[16], masked[3];
-      view_sh(cams[v], L, acc, basis, masked);
-#pragma unroll
-      for (int k = 0; k < 16; ++k) gs[k] += masked[ch] * basis[k];
+      for (int k = 0; k < 16; ++k) gs[k] += m * basis[k];
     }
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
@@ -341,7 +403,7 @@ __global__ __launch_bounds__(256, 2) void k_train_adam(float* __restrict__ P, fl
   }
   // the raster accumulators of this Gaussian are consumed: clear them
   for (int v = 0; v < n_views; ++v) {
-    if (!(vis >> v & 1u)) continue;
+    if (!(vis >> v & 1ull)) continue;
     size_t vi = (size_t)v * cap + i;
 #pragma unroll
     for (int q = 0; q < 9; ++q) accum[vi * 9 + q] = 0.0f;
@@ -351,7 +413,7 @@ __global__ __launch_bounds__(256, 2) void k_train_adam(float* __restrict__ P, fl
   for (int j = 0; j < 11; ++j) {
     size_t o = (size_t)j * cap + i;
     float mm = M1[o], vv = M2[o];
-    float p = adam_one(P[o], g[j], mm, vv, lr_of(j, k_), k_);
+    float p = adam_one(P[o], scr[(size_t)(TG_G + j) * cap + i], mm, vv, lr_of(j, k_), k_);
     M1[o] = mm;
     M2[o] = vv;
     if (j >= P_ROT && j < P_ROT + 4)
@@ -361,16 +423,18 @@ __global__ __launch_bounds__(256, 2) void k_train_adam(float* __restrict__ P, fl
   }
   renorm_quat(rot[0], rot[1], rot[2], rot[3]);
   for (int q = 0; q < 4; ++q) P[(size_t)(P_ROT + q) * cap + i] = rot[q];
-  gaccum[i] += sgn;
+  gaccum[i] += scr[(size_t)TG_SGN * cap + i];
   gdenom[i] += (uint32_t)n_views;
 }
 
 void launch_train_adam(float* P, float* M1, float* M2, int cap, int n, const CamDev* cams, int n_views,
-                       const uint32_t* cnt, float* accum, float* gaccum, uint32_t* gdenom,
+                       const uint32_t* cnt, float* accum, float* scratch, float* gaccum, uint32_t* gdenom,
                        const AdamConsts* k_dev, const uint32_t* status, hipStream_t st) {
   if (n <= 0) return;
+  hipLaunchKernelGGL(k_train_geometry, dim3((n + 255) / 256), dim3(256), 0, st, P, cap, n, cams, n_views, cnt, accum,
+                     scratch, status);
   hipLaunchKernelGGL(k_train_adam, dim3((n + 255) / 256), dim3(256), 0, st, P, M1, M2, cap, n, cams, n_views, cnt,
-                     accum, gaccum, gdenom, k_dev, status);
+                     accum, scratch, gaccum, gdenom, k_dev, status);
 }
 
 // ln on the device: the f64 logarithm rounded once to f32 (correctly rounded

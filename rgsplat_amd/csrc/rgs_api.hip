@@ -225,6 +225,7 @@ struct rgs_ctx {
   uint64_t graph_gen = 0;  // gen at capture time
   DBuf api_acc9;           // raster accumulators of the reference-API paths
   DBuf tile_order;         // raster schedule: slot -> global tile id
+  DBuf train_scr;          // per-Gaussian scratch between geometry and Adam
   std::vector<int32_t> h_order;
   std::string order_key;
   int super_tile = 8;      // tiles per side of a schedule super-tile
@@ -536,7 +537,7 @@ void rgs_destroy(rgs_ctx* c) {
                  &c->api_target, &c->api_m1, &c->api_m2, &c->api_acc, &c->api_den, &c->api_P2, &c->dens_want,
                  &c->dens_cnt, &c->dens_W, &c->dens_packed, &c->dens_prefix, &c->dens_b32, &c->dens_b64,
                  &c->dens_t32, &c->dens_t64, &c->first_splat, &c->P, &c->P2, &c->M1, &c->M2, &c->gacc, &c->gden, &c->targets,
-                 &c->adam_d, &c->api_acc9, &c->tile_order};
+                 &c->adam_d, &c->api_acc9, &c->tile_order, &c->train_scr};
   for (DBuf* b : all)
     if (b->p) (void)hipFree(b->p);
   if (c->h_scalars) (void)hipHostFree(c->h_scalars);
@@ -936,7 +937,7 @@ static void enqueue_step_rest(rgs_ctx* c, const ViewSet& vs, const float bg[3], 
   stage_begin(c, S_ADAM);
   launch_train_adam(P_<float>(c->P), P_<float>(c->M1), P_<float>(c->M2), (int)cap, n, P_<CamDev>(c->cams_d),
                     (int)vs.cams.size(), P_<uint32_t>(c->cnt), P_<float>(c->accum),
-                    P_<float>(c->gacc), P_<uint32_t>(c->gden), P_<AdamConsts>(c->adam_d), sc + SC_STATUS, c->st);
+                    P_<float>(c->train_scr), P_<float>(c->gacc), P_<uint32_t>(c->gden), P_<AdamConsts>(c->adam_d), sc + SC_STATUS, c->st);
   stage_end(c, S_ADAM);
 }
 
@@ -974,6 +975,7 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
   size_t acc_n = 9 * vs.cams.size() * cap;
   bool fresh_accum = c->accum.bytes < acc_n * 4;
   CHK(ensure(c, c->accum, acc_n * 4));
+  CHK(ensure(c, c->train_scr, (size_t)TRAIN_SCRATCH_ROWS * cap * 4));
   if (fresh_accum) launch_fill_f32(P_<float>(c->accum), acc_n, 0.0f, c->st);
   CHK(ensure_loss(c, vs));
   std::memcpy(c->h_cams, vs.cams.data(), vs.cams.size() * sizeof(CamDev));

@@ -153,10 +153,12 @@ void launch_geometry_aos(const float* P, int cap, int n, const CamDev* cams, int
 // Adam on the reference layouts: AoS params/grads, row-major moments.
 void launch_adam_aos(float* g, const float* grads, size_t n, float* m1, float* m2, float* accum, uint32_t* denom,
                      const AdamConsts& k, hipStream_t st);
-// Fused per-Gaussian geometry backprop over all views + Adam on SoA buffers.
-// Skips everything when *status has ST_PAIR_OVERFLOW. Clears accum.
+// Per-Gaussian geometry backprop over all views + Adam on SoA buffers (two
+// kernels; scratch holds TRAIN_SCRATCH_ROWS x cap floats). Skips everything
+// when *status has ST_PAIR_OVERFLOW or ST_NAN_DEPTH. Clears accum.
+constexpr int TRAIN_SCRATCH_ROWS = 18;
 void launch_train_adam(float* P, float* M1, float* M2, int cap, int n, const CamDev* cams, int n_views,
-                       const uint32_t* cnt, float* accum, float* gaccum, uint32_t* gdenom,
+                       const uint32_t* cnt, float* accum, float* scratch, float* gaccum, uint32_t* gdenom,
                        const AdamConsts* k_dev, const uint32_t* status, hipStream_t st);
 // reset_opacities on SoA (row 10) or AoS (offset 3) params.
 void launch_reset_opacity(float* p, size_t n, size_t stride, hipStream_t st);
