@@ -335,6 +335,11 @@ __global__ __launch_bounds__(BT) void k_raster_bwd(RasterIn in, const float* __r
         float alpha = fmin_(alpha_raw, MAX_ALPHA);
         if (!(alpha >= ALPHA_THRESHOLD)) continue;
         contrib = true;
+        {
+        // Gradient arithmetic only (the decisions above are uncontracted,
+        // as in the forward): fused multiply-adds are within the gradient
+        // tolerance.
+#pragma clang fp contract(fast)
         // transmittance in front of this splat (only gradients use it)
         float inv_1ma = __builtin_amdgcn_rcpf(1.0f - alpha);
         float Tq = T[q] * inv_1ma;
@@ -357,6 +362,7 @@ __global__ __launch_bounds__(BT) void k_raster_bwd(RasterIn in, const float* __r
         h0[q] += weight * B.z;
         h1[q] += weight * B.w;
         h2[q] += weight * cb;
+        }
       }
       const uint64_t bm = __ballot(contrib);
       if (bm == 0) continue;
