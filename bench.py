@@ -37,7 +37,8 @@ FP32_PEAK_TFLOPS = 157.3
 #   raster_fwd: 25 FP32 ops + 1 exp per evaluated (pixel, splat) pair
 #   raster_bwd: 60 FP32 ops + 1 exp + 1 div per evaluated pair
 #   loss_ssim:  1440 FLOP per pixel (30 blurs x 22 taps x 2 + elementwise)
-#   project:    236 B read per Gaussian per view + 52 B written per visible splat
+#   project:    236 B read per Gaussian + 60 B written per Gaussian and view
+#               (splat record 52 B, depth key and value 8 B)
 #   geometry_adam: 1652 B per Gaussian (59 params x 28 B + 8 B statistics)
 FWD_OPS_PER_PAIR = 26
 BWD_OPS_PER_PAIR = 62
@@ -214,7 +215,7 @@ def main():
             "raster_fwd": ("mfma", FWD_OPS_PER_PAIR * pf, "TFLOP/s"),
             "raster_bwd": ("mfma", BWD_OPS_PER_PAIR * pb, "TFLOP/s"),
             "loss_ssim": ("mfma", SSIM_FLOP_PER_PIXEL * npix, "TFLOP/s"),
-            "project": ("hbm", 236 * ng * len(cams) + 52 * ng * len(cams), "GB/s"),
+            "project": ("hbm", 236 * ng + 60 * ng * len(cams), "GB/s"),
             "geometry_adam": ("hbm", 1652 * ng, "GB/s"),
         }
         traffic = measured_traffic(wl.name)
