@@ -1,28 +1,51 @@
 // k_loss.hip -- L = (1 - lambda) L1 + lambda (1 - SSIM) and dL/dpixel.
 //
 // Follows backward.rs:646-807. SSIM uses the 11-tap separable Gaussian window
-// (sigma 1.5) with zero padding; its gradient is the adjoint blur of five
-// coefficient maps (backward.rs:744-754). Two passes over 16x16 output tiles
-// with a 5-pixel halo staged in LDS:
-//   k_ssim_fwd: blurs x, y, x^2, y^2, xy (horizontal into LDS, then vertical),
-//               writes the five coefficient maps, the L1 gradient and per-tile
-//               partial sums of |x - y| and SSIM;
-//   k_ssim_bwd: blurs the five maps and adds -lambda dSSIM/dx / M.
-// Taps are summed in the reference's order (horizontal then vertical,
-// k = 0..10) and out-of-image taps contribute +0, so the per-pixel values
-// equal the reference's. The loss scalar sums the partials in double.
+// (sigma 1.5) with zero padding. Its gradient with respect to x is the
+// adjoint blur of five per-pixel coefficient maps (backward.rs:744-754),
+//   g = B(P) + y B(Q) - B(Q mu_y) + 2 x B(R) - 2 B(R mu_x),
+// which by linearity of the blur B is computed from three maps:
+//   g = B(P - Q mu_y - 2 R mu_x) + y B(Q) + 2 x B(R).
+//
+// Two passes over 32x32 output tiles with a 5-pixel halo staged in LDS, one
+// colour channel at a time:
+//   k_ssim_fwd: blurs x, y, x^2, y^2, xy (horizontal into LDS, then
+//               vertical), writes the three coefficient maps, the L1
+//               gradient and per-tile partial sums of |x - y| and SSIM;
+//   k_ssim_bwd: blurs the three maps and adds -lambda dSSIM/dx / M.
+// Each thread produces four consecutive outputs of a row (horizontal pass)
+// or a column (vertical pass) from one 14-value window held in registers, so
+// every staged value is read once per window and the products x^2, y^2, xy
+// are formed once per input. Taps are summed in order k = 0..10 (horizontal,
+// then vertical) with fused multiply-adds; the per-pixel values agree with
+// the reference's to float rounding. The loss scalar sums the partials in
+// double.
 #include "rgs_kernels.h"
 
 namespace rgs {
 
 constexpr int LT = 256;
-constexpr int HP = TILE + 10;      // 26: tile plus halo
-constexpr int HN = HP * HP;        // 676 staged values per map
-constexpr int HH = HP * TILE;      // 416 horizontal outputs per map
+constexpr int LH = LTILE + 10;  // 42: tile plus halo
+constexpr int LN = LH * LH;     // staged values per map
+constexpr int RUN = 4;          // outputs per thread per window
+constexpr int WIN = RUN + 10;   // window length
 
 struct Kern11 {
   float k[11];
 };
+
+struct LossTile {
+  int v, tx, ty;  // view and loss-tile coordinates
+};
+
+__device__ __forceinline__ LossTile loss_tile(const LossIn& in, int t) {
+  int v = 0;
+  while (v + 1 < in.n_views && in.cams[v + 1].ltile_base <= t) ++v;
+  const CamDev& c = in.cams[v];
+  const int nlx = (c.w + LTILE - 1) / LTILE;
+  const int lt = t - c.ltile_base;
+  return LossTile{v, lt % nlx, lt / nlx};
+}
 
 __device__ __forceinline__ float block_sum_256(float x, float* s_tmp) {
 #pragma unroll
@@ -37,97 +60,110 @@ __device__ __forceinline__ float block_sum_256(float x, float* s_tmp) {
 __global__ __launch_bounds__(LT) void k_ssim_fwd(LossIn in, Kern11 K, float* __restrict__ dl,
                                                  float* __restrict__ maps, float* __restrict__ partial,
                                                  size_t total_pix) {
-  __shared__ float s_xa[3][HN];
-  __shared__ float s_ya[3][HN];
-  __shared__ float s_h[5][HH];
+#pragma clang fp contract(fast)
+  __shared__ float s_x[LN];
+  __shared__ float s_y[LN];
+  __shared__ float s_h[5][LH][LTILE];
   __shared__ float s_tmp[4];
   const int t = threadIdx.x;
   const int tile = xcd_remap(blockIdx.x, gridDim.x);  // halo-sharing neighbours on one XCD
-  int v = 0;
-  while (v + 1 < in.n_views && in.cams[v + 1].tile_base <= tile) ++v;
-  const CamDev& c = in.cams[v];
-  const int lt = tile - c.tile_base;
-  const int tx = lt % c.ntx, ty = lt / c.ntx;
-  const int x0 = tx * TILE - 5, y0 = ty * TILE - 5;
-  const int ox = t & 15, oy = t >> 4;
-  const int px = tx * TILE + ox, py = ty * TILE + oy;
-  const bool inside = px < c.w && py < c.h;
-  const size_t pix = (size_t)c.pix_base + (size_t)py * c.w + px;
+  const LossTile T = loss_tile(in, tile);
+  const CamDev& c = in.cams[T.v];
+  const int x0 = T.tx * LTILE - 5, y0 = T.ty * LTILE - 5;
   const float m_total = (float)((size_t)c.w * c.h * 3);
   const float lam = in.lambda;
   const float C1 = 0.01f * 0.01f;
   const float C2 = 0.03f * 0.03f;
-  // all three channels of the haloed tile, one pass over the images
-  for (int idx = t; idx < HN; idx += LT) {
-    int gy = y0 + idx / HP, gx = x0 + idx % HP;
-    bool ok = gx >= 0 && gy >= 0 && gx < c.w && gy < c.h;
-    size_t gp = (size_t)c.pix_base + (size_t)gy * c.w + gx;
-    size_t tp = (size_t)c.tgt_base + (size_t)gy * c.w + gx;
-    float4 yv = ok ? *reinterpret_cast<const float4*>(in.target + tp * 4) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) s_xa[ch][idx] = ok ? in.rgb[gp * 3 + ch] : 0.0f;
-    s_ya[0][idx] = yv.x;
-    s_ya[1][idx] = yv.y;
-    s_ya[2][idx] = yv.z;
-  }
-  __syncthreads();
+  // vertical pass / output mapping: column oc, rows 4 * oq .. 4 * oq + 3
+  const int oc = t & (LTILE - 1), oq = t >> 5;
   for (int ch = 0; ch < 3; ++ch) {
-    const float* s_x = s_xa[ch];
-    const float* s_y = s_ya[ch];
-    for (int idx = t; idx < HH; idx += LT) {
-      int r = idx / TILE, cc = idx % TILE;
-      float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, a4 = 0.0f;
-#pragma unroll
-      for (int ki = 0; ki < 11; ++ki) {
-        float xv = s_x[r * HP + cc + ki];
-        float yv = s_y[r * HP + cc + ki];
-        float kk = K.k[ki];
-        a0 += kk * xv;
-        a1 += kk * yv;
-        a2 += kk * (xv * xv);
-        a3 += kk * (yv * yv);
-        a4 += kk * (xv * yv);
-      }
-      s_h[0][idx] = a0;
-      s_h[1][idx] = a1;
-      s_h[2][idx] = a2;
-      s_h[3][idx] = a3;
-      s_h[4][idx] = a4;
+    for (int idx = t; idx < LN; idx += LT) {
+      int gy = y0 + idx / LH, gx = x0 + idx % LH;
+      bool ok = gx >= 0 && gy >= 0 && gx < c.w && gy < c.h;
+      size_t gp = (size_t)c.pix_base + (size_t)gy * c.w + gx;
+      size_t tp = (size_t)c.tgt_base + (size_t)gy * c.w + gx;
+      s_x[idx] = ok ? in.rgb[gp * 3 + ch] : 0.0f;
+      s_y[idx] = ok ? in.target[tp * 4 + ch] : 0.0f;
     }
     __syncthreads();
-    float mu[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    // horizontal: LH rows x (LTILE / RUN) runs
+    for (int task = t; task < LH * (LTILE / RUN); task += LT) {
+      const int r = task / (LTILE / RUN), c0 = (task % (LTILE / RUN)) * RUN;
+      float a[5][RUN];
 #pragma unroll
-    for (int ki = 0; ki < 11; ++ki) {
-      float kk = K.k[ki];
+      for (int m = 0; m < 5; ++m)
 #pragma unroll
-      for (int m = 0; m < 5; ++m) mu[m] += kk * s_h[m][(oy + ki) * TILE + ox];
+        for (int o = 0; o < RUN; ++o) a[m][o] = 0.0f;
+#pragma unroll
+      for (int w = 0; w < WIN; ++w) {
+        const float xv = s_x[r * LH + c0 + w], yv = s_y[r * LH + c0 + w];
+        const float xx = xv * xv, yy = yv * yv, xy = xv * yv;
+#pragma unroll
+        for (int o = 0; o < RUN; ++o) {
+          const int ki = w - o;
+          if (ki < 0 || ki > 10) continue;
+          const float kk = K.k[ki];
+          a[0][o] += kk * xv;
+          a[1][o] += kk * yv;
+          a[2][o] += kk * xx;
+          a[3][o] += kk * yy;
+          a[4][o] += kk * xy;
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < 5; ++m)
+#pragma unroll
+        for (int o = 0; o < RUN; ++o) s_h[m][r][c0 + o] = a[m][o];
+    }
+    __syncthreads();
+    float mu[5][RUN];
+#pragma unroll
+    for (int m = 0; m < 5; ++m)
+#pragma unroll
+      for (int o = 0; o < RUN; ++o) mu[m][o] = 0.0f;
+#pragma unroll
+    for (int w = 0; w < WIN; ++w) {
+      float hv[5];
+#pragma unroll
+      for (int m = 0; m < 5; ++m) hv[m] = s_h[m][oq * RUN + w][oc];
+#pragma unroll
+      for (int o = 0; o < RUN; ++o) {
+        const int ki = w - o;
+        if (ki < 0 || ki > 10) continue;
+        const float kk = K.k[ki];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) mu[m][o] += kk * hv[m];
+      }
     }
     float l1 = 0.0f, ss = 0.0f;
-    if (inside) {
-      float mx = mu[0], my = mu[1];
-      float sx2 = mu[2] - mx * mx;
-      float sy2 = mu[3] - my * my;
-      float sxy = mu[4] - mx * my;
+    const int px = T.tx * LTILE + oc;
+#pragma unroll
+    for (int o = 0; o < RUN; ++o) {
+      const int py = T.ty * LTILE + oq * RUN + o;
+      if (px >= c.w || py >= c.h) continue;
+      const size_t pix = (size_t)c.pix_base + (size_t)py * c.w + px;
+      float mx = mu[0][o], my = mu[1][o];
+      float sx2 = mu[2][o] - mx * mx;
+      float sy2 = mu[3][o] - my * my;
+      float sxy = mu[4][o] - mx * my;
       float a1 = 2.0f * mx * my + C1;
       float a2 = 2.0f * sxy + C2;
       float b1 = mx * mx + my * my + C1;
       float b2 = sx2 + sy2 + C2;
       float num = a1 * a2;
       float den = b1 * b2;
-      ss = num / den;
+      ss += num / den;
       float inv_den = 1.0f / den;
       float pc = inv_den * (2.0f * my * a2) - (num / (den * den)) * (2.0f * mx * b2);
       float qc = inv_den * (2.0f * a1);
       float rc = -(num / (den * den)) * b1;
-      maps[(size_t)(0 * 3 + ch) * total_pix + pix] = pc;
+      maps[(size_t)(0 * 3 + ch) * total_pix + pix] = pc - qc * my - 2.0f * (rc * mx);
       maps[(size_t)(1 * 3 + ch) * total_pix + pix] = qc;
       maps[(size_t)(2 * 3 + ch) * total_pix + pix] = rc;
-      maps[(size_t)(3 * 3 + ch) * total_pix + pix] = qc * my;
-      maps[(size_t)(4 * 3 + ch) * total_pix + pix] = rc * mx;
-      float xv = s_x[(oy + 5) * HP + ox + 5];
-      float yv = s_y[(oy + 5) * HP + ox + 5];
+      const float xv = s_x[(oq * RUN + o + 5) * LH + oc + 5];
+      const float yv = s_y[(oq * RUN + o + 5) * LH + oc + 5];
       float d = xv - yv;
-      l1 = __builtin_fabsf(d);
+      l1 += __builtin_fabsf(d);
       // Rust f32::signum: +1 for +0.0
       float sg = d != d ? d : __builtin_copysignf(1.0f, d);
       dl[pix * 3 + ch] = (1.0f - lam) * sg / m_total;
@@ -143,55 +179,81 @@ __global__ __launch_bounds__(LT) void k_ssim_fwd(LossIn in, Kern11 K, float* __r
 
 __global__ __launch_bounds__(LT) void k_ssim_bwd(LossIn in, Kern11 K, float* __restrict__ dl,
                                                  const float* __restrict__ maps, size_t total_pix) {
-  __shared__ float s_m[5][HN];
-  __shared__ float s_h[5][HH];
+#pragma clang fp contract(fast)
+  __shared__ float s_m[SSIM_MAPS][LN];
+  __shared__ float s_h[SSIM_MAPS][LH][LTILE];
   const int t = threadIdx.x;
   const int tile = xcd_remap(blockIdx.x, gridDim.x);  // halo-sharing neighbours on one XCD
-  int v = 0;
-  while (v + 1 < in.n_views && in.cams[v + 1].tile_base <= tile) ++v;
-  const CamDev& c = in.cams[v];
-  const int lt = tile - c.tile_base;
-  const int tx = lt % c.ntx, ty = lt / c.ntx;
-  const int x0 = tx * TILE - 5, y0 = ty * TILE - 5;
-  const int ox = t & 15, oy = t >> 4;
-  const int px = tx * TILE + ox, py = ty * TILE + oy;
-  const bool inside = px < c.w && py < c.h;
-  const size_t pix = (size_t)c.pix_base + (size_t)py * c.w + px;
+  const LossTile T = loss_tile(in, tile);
+  const CamDev& c = in.cams[T.v];
+  const int x0 = T.tx * LTILE - 5, y0 = T.ty * LTILE - 5;
   const float m_total = (float)((size_t)c.w * c.h * 3);
   const float lam = in.lambda;
+  const int oc = t & (LTILE - 1), oq = t >> 5;
   for (int ch = 0; ch < 3; ++ch) {
-    for (int idx = t; idx < HN; idx += LT) {
-      int gy = y0 + idx / HP, gx = x0 + idx % HP;
+    for (int idx = t; idx < LN; idx += LT) {
+      int gy = y0 + idx / LH, gx = x0 + idx % LH;
       bool ok = gx >= 0 && gy >= 0 && gx < c.w && gy < c.h;
       size_t gp = (size_t)c.pix_base + (size_t)gy * c.w + gx;
 #pragma unroll
-      for (int m = 0; m < 5; ++m) s_m[m][idx] = ok ? maps[(size_t)(m * 3 + ch) * total_pix + gp] : 0.0f;
+      for (int m = 0; m < SSIM_MAPS; ++m) s_m[m][idx] = ok ? maps[(size_t)(m * 3 + ch) * total_pix + gp] : 0.0f;
     }
     __syncthreads();
-    for (int idx = t; idx < HH; idx += LT) {
-      int r = idx / TILE, cc = idx % TILE;
-      float a[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    for (int task = t; task < LH * (LTILE / RUN); task += LT) {
+      const int r = task / (LTILE / RUN), c0 = (task % (LTILE / RUN)) * RUN;
+      float a[SSIM_MAPS][RUN];
 #pragma unroll
-      for (int ki = 0; ki < 11; ++ki) {
-        float kk = K.k[ki];
+      for (int m = 0; m < SSIM_MAPS; ++m)
 #pragma unroll
-        for (int m = 0; m < 5; ++m) a[m] += kk * s_m[m][r * HP + cc + ki];
+        for (int o = 0; o < RUN; ++o) a[m][o] = 0.0f;
+#pragma unroll
+      for (int w = 0; w < WIN; ++w) {
+        float v[SSIM_MAPS];
+#pragma unroll
+        for (int m = 0; m < SSIM_MAPS; ++m) v[m] = s_m[m][r * LH + c0 + w];
+#pragma unroll
+        for (int o = 0; o < RUN; ++o) {
+          const int ki = w - o;
+          if (ki < 0 || ki > 10) continue;
+          const float kk = K.k[ki];
+#pragma unroll
+          for (int m = 0; m < SSIM_MAPS; ++m) a[m][o] += kk * v[m];
+        }
       }
 #pragma unroll
-      for (int m = 0; m < 5; ++m) s_h[m][idx] = a[m];
+      for (int m = 0; m < SSIM_MAPS; ++m)
+#pragma unroll
+        for (int o = 0; o < RUN; ++o) s_h[m][r][c0 + o] = a[m][o];
     }
     __syncthreads();
-    float b[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    float b[SSIM_MAPS][RUN];
 #pragma unroll
-    for (int ki = 0; ki < 11; ++ki) {
-      float kk = K.k[ki];
+    for (int m = 0; m < SSIM_MAPS; ++m)
 #pragma unroll
-      for (int m = 0; m < 5; ++m) b[m] += kk * s_h[m][(oy + ki) * TILE + ox];
+      for (int o = 0; o < RUN; ++o) b[m][o] = 0.0f;
+#pragma unroll
+    for (int w = 0; w < WIN; ++w) {
+      float hv[SSIM_MAPS];
+#pragma unroll
+      for (int m = 0; m < SSIM_MAPS; ++m) hv[m] = s_h[m][oq * RUN + w][oc];
+#pragma unroll
+      for (int o = 0; o < RUN; ++o) {
+        const int ki = w - o;
+        if (ki < 0 || ki > 10) continue;
+        const float kk = K.k[ki];
+#pragma unroll
+        for (int m = 0; m < SSIM_MAPS; ++m) b[m][o] += kk * hv[m];
+      }
     }
-    if (inside) {
-      float xv = in.rgb[pix * 3 + ch];
-      float yv = in.target[((size_t)c.tgt_base + (size_t)py * c.w + px) * 4 + ch];
-      float grad = b[0] + yv * b[1] - b[3] + 2.0f * xv * b[2] - 2.0f * b[4];
+    const int px = T.tx * LTILE + oc;
+#pragma unroll
+    for (int o = 0; o < RUN; ++o) {
+      const int py = T.ty * LTILE + oq * RUN + o;
+      if (px >= c.w || py >= c.h) continue;
+      const size_t pix = (size_t)c.pix_base + (size_t)py * c.w + px;
+      const float xv = in.rgb[pix * 3 + ch];
+      const float yv = in.target[((size_t)c.tgt_base + (size_t)py * c.w + px) * 4 + ch];
+      const float grad = b[0][o] + yv * b[1][o] + 2.0f * xv * b[2][o];
       dl[pix * 3 + ch] += -lam * grad / m_total;
     }
     __syncthreads();
@@ -205,10 +267,10 @@ __global__ __launch_bounds__(256) void k_loss_reduce(LossIn in, const float* __r
   __shared__ double s_ss[256];
   const int v = blockIdx.x;
   const CamDev& c = in.cams[v];
-  const int nt = c.ntx * c.nty;
+  const int nt = ((c.w + LTILE - 1) / LTILE) * ((c.h + LTILE - 1) / LTILE);
   double l1 = 0.0, ss = 0.0;
   for (int i = threadIdx.x; i < nt * 3; i += 256) {
-    size_t tile = (size_t)c.tile_base + i / 3;
+    size_t tile = (size_t)c.ltile_base + i / 3;
     int ch = i % 3;
     l1 += (double)partial[(tile * 3 + ch) * 2 + 0];
     ss += (double)partial[(tile * 3 + ch) * 2 + 1];
@@ -247,11 +309,11 @@ static Kern11 make_kernel() {
 }
 
 void launch_loss(const LossIn& in, float* dl, float* ssim_maps, float* partial, float* loss_out, hipStream_t st) {
-  if (in.total_tiles <= 0) return;
+  if (in.total_ltiles <= 0) return;
   static const Kern11 K = make_kernel();
-  hipLaunchKernelGGL(k_ssim_fwd, dim3(in.total_tiles), dim3(LT), 0, st, in, K, dl, ssim_maps, partial,
+  hipLaunchKernelGGL(k_ssim_fwd, dim3(in.total_ltiles), dim3(LT), 0, st, in, K, dl, ssim_maps, partial,
                      in.total_pix);
-  hipLaunchKernelGGL(k_ssim_bwd, dim3(in.total_tiles), dim3(LT), 0, st, in, K, dl, ssim_maps, in.total_pix);
+  hipLaunchKernelGGL(k_ssim_bwd, dim3(in.total_ltiles), dim3(LT), 0, st, in, K, dl, ssim_maps, in.total_pix);
   hipLaunchKernelGGL(k_loss_reduce, dim3(in.n_views), dim3(256), 0, st, in, partial, loss_out);
 }
 

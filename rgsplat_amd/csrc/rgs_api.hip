@@ -141,6 +141,7 @@ int make_camdev(const rgs_camera& cam, CamDev& out) {
   out.tile_base = 0;
   out.pix_base = 0;
   out.tgt_base = 0;
+  out.ltile_base = 0;
   return 0;
 }
 
@@ -148,17 +149,21 @@ int make_camdev(const rgs_camera& cam, CamDev& out) {
 struct ViewSet {
   std::vector<CamDev> cams;
   int total_tiles = 0;
+  int total_ltiles = 0;
   size_t total_pix = 0;
   void layout() {
-    int tb = 0;
+    int tb = 0, lb = 0;
     size_t pb = 0;
     for (auto& c : cams) {
       c.tile_base = tb;
+      c.ltile_base = lb;
       c.pix_base = (int32_t)pb;
       tb += c.ntx * c.nty;
+      lb += ((c.w + LTILE - 1) / LTILE) * ((c.h + LTILE - 1) / LTILE);
       pb += (size_t)c.w * c.h;
     }
     total_tiles = tb;
+    total_ltiles = lb;
     total_pix = pb;
   }
   int bits() const {
@@ -449,14 +454,14 @@ int upload_api_scene(rgs_ctx* c, const void* g, size_t n, bool device_src) {
 
 int ensure_loss(rgs_ctx* c, const ViewSet& vs) {
   CHK(ensure(c, c->dl, vs.total_pix * 3 * 4));
-  CHK(ensure(c, c->maps, vs.total_pix * 15 * 4));
-  CHK(ensure(c, c->partial, (size_t)vs.total_tiles * 6 * 4));
+  CHK(ensure(c, c->maps, vs.total_pix * SSIM_MAPS * 3 * 4));
+  CHK(ensure(c, c->partial, (size_t)vs.total_ltiles * 6 * 4));
   CHK(ensure(c, c->loss, 64 * 4));
   return 0;
 }
 
 void enqueue_loss(rgs_ctx* c, const ViewSet& vs, const float* rgb, const float* target, float lambda) {
-  LossIn li{rgb, target, P_<CamDev>(c->cams_d), (int)vs.cams.size(), vs.total_tiles, vs.total_pix, lambda};
+  LossIn li{rgb, target, P_<CamDev>(c->cams_d), (int)vs.cams.size(), vs.total_ltiles, vs.total_pix, lambda};
   launch_loss(li, P_<float>(c->dl), P_<float>(c->maps), P_<float>(c->partial), P_<float>(c->loss), c->st);
 }
 

@@ -125,17 +125,19 @@ void launch_work_counts(const RasterIn& in, const float* final_t, const int32_t*
                         hipStream_t st);
 
 // ---- loss (k_loss.hip)
+constexpr int LTILE = 32;  // loss tile side
 struct LossIn {
   const float* rgb;     // total_pix * 3
   const float* target;  // target store, RGBA; view v starts at pixel cams[v].tgt_base
   const CamDev* cams;
   int n_views;
-  int total_tiles;
+  int total_ltiles;     // 32x32 loss tiles over all views
   size_t total_pix;
   float lambda;
 };
-// ssim_maps: 5 * 3 * total_pix floats; partial: 2 * 3 * total_tiles floats;
-// loss_out: n_views floats.
+// ssim_maps: SSIM_MAPS * 3 * total_pix floats; partial: 2 * 3 * total_ltiles
+// floats; loss_out: n_views floats.
+constexpr int SSIM_MAPS = 3;
 void launch_loss(const LossIn& in, float* dl, float* ssim_maps, float* partial, float* loss_out, hipStream_t st);
 
 // ---- optimizer and parameter kernels (k_optim.hip)

@@ -274,6 +274,7 @@ struct CamDev {
   int32_t tile_base;  // first global tile id of this view
   int32_t pix_base;   // first pixel of this view in the batched image arrays
   uint32_t tgt_base;  // first pixel of this view's target in the target store
+  int32_t ltile_base; // first 32x32 loss tile of this view
 };
 
 // p_cam = view * [p, 1] (gemv order, math/mod.rs:225-226)

@@ -2,11 +2,12 @@
 same seeded inputs.
 
 Bars (written at each assert): tile binning and integer outputs bit-exact;
-images within 1e-4 absolute (the kernels keep the reference's rounding, so
+images within 1e-4 absolute (the forward keeps the reference's rounding, so
 they are expected bit-exact and that is asserted too); gradients within 1e-3
 relative per parameter group (||g_gpu - g_cpu|| / ||g_cpu||), since the GPU
 sums per-pixel contributions in a different order (wave reduction + FP32
-atomics) than the reference's per-tile serial merge.
+atomics) than the reference's per-tile serial merge and uses fused
+multiply-adds in gradient arithmetic.
 """
 import math
 
@@ -126,8 +127,15 @@ def test_loss_grad_parity(ctx):
     rgb[3, 4, 1] = tg[3, 4, 1]  # a zero residual: Rust signum(+0.0) = +1
     loss, dl = ctx.photometric_loss_grad(rgb, tg, w, h, 0.2)
     loss_ref, dl_ref = oracle.loss_grad(rgb, tg, w, h, 0.2)
-    # per-pixel gradient keeps the reference's operation order: bit-exact
-    assert np.array_equal(dl, dl_ref)
+    # per-pixel gradient: blur taps in the reference's order with fused
+    # multiply-adds and the five SSIM coefficient maps folded into three, so
+    # equal to float rounding (bar: gradients within 1e-3 relative)
+    d = (dl - dl_ref).astype(np.float64)
+    assert np.linalg.norm(d) <= 1e-5 * np.linalg.norm(dl_ref)
+    assert np.abs(d).max() <= 1e-4 * np.abs(dl_ref).max()
+    # the L1 term alone is exact: sign at the zero residual is +1
+    zi = (3 * w + 4) * 3 + 1
+    assert np.sign(dl.reshape(-1)[zi]) == np.sign(dl_ref.reshape(-1)[zi])
     # the loss scalar is a sum over w*h*3 terms: the reference sums serially
     # in f32, the GPU per tile and then in f64
     assert abs(loss - loss_ref) <= 1e-5 * abs(loss_ref)
