@@ -34,9 +34,15 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+# Per-file flags. The raster's LDS atomics target one wave-uniform address
+# from a few lanes at a time; the compiler's atomic optimizer would turn each
+# into a lane-serial reduction loop.
+EXTRA = {"k_raster.hip": ["-mllvm", "-amdgpu-atomic-optimizer-strategy=None"]}
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, src.replace(".hip", ".o"))
-    cmd = [hipcc(), *FLAGS, "-I", CSRC, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [hipcc(), *FLAGS, *EXTRA.get(src, []), "-I", CSRC, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")

@@ -68,7 +68,6 @@ __global__ __launch_bounds__(FT) void k_raster_fwd(RasterIn in, float* __restric
   __shared__ float4 s_a[FT];
   __shared__ float4 s_b[FT];
   __shared__ float s_c[FT];
-  __shared__ uint32_t s_m[FT];
   const int lane = threadIdx.x;
   if (lane < 32) s_tab[lane] = kExp2Tab[lane];
   const int tile = in.tile_order[xcd_remap(blockIdx.x, gridDim.x)];
@@ -99,6 +98,7 @@ __global__ __launch_bounds__(FT) void k_raster_fwd(RasterIn in, float* __restric
     if (__ballot(!(done[0] && done[1] && done[2] && done[3])) == 0) break;
     __syncthreads();  // previous batch fully read (one wave: s_barrier only)
     const uint32_t idx = b0 + lane;
+    uint32_t my_m = 0;  // group mask of this lane's staged splat
     if (idx < end) {
       uint32_t vi = in.pair_vals[idx];
       float4 A = in.S.xy_ab[vi];
@@ -106,16 +106,29 @@ __global__ __launch_bounds__(FT) void k_raster_fwd(RasterIn in, float* __restric
       s_a[lane] = A;
       s_b[lane] = B;
       s_c[lane] = in.S.col_b[vi];
-      s_m[lane] = wave_mask(A.x, A.y, A.z, A.w, B.x, in.S.reach[vi], tx, ty);
+      my_m = wave_mask(A.x, A.y, A.z, A.w, B.x, in.S.reach[vi], tx, ty);
     }
     __syncthreads();
-    const int nb = (int)min((uint32_t)FT, end - b0);
-    for (int j = 0; j < nb; ++j) {
-      const uint32_t m = s_m[j];  // wave-uniform
-      if (!m) continue;
-      const float4 A = s_a[j];
-      const float4 B = s_b[j];
-      const float cb = s_c[j];
+    // splats that reach some pixel group, visited front to back as set bits
+    // of a scalar mask; the next one's LDS data is fetched while the current
+    // one is processed
+    uint64_t live = __ballot(my_m != 0);
+    int jn = live ? __builtin_ctzll(live) : 0;
+    float4 An = s_a[jn], Bn = s_b[jn];
+    float cbn = s_c[jn];
+    while (live) {
+      const int j = jn;
+      live &= live - 1;
+      const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)my_m, j);
+      const float4 A = An;
+      const float4 B = Bn;
+      const float cb = cbn;
+      if (live) {
+        jn = __builtin_ctzll(live);
+        An = s_a[jn];
+        Bn = s_b[jn];
+        cbn = s_c[jn];
+      }
       const int32_t li = (int32_t)(b0 - start) + j;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -239,7 +252,7 @@ __device__ __forceinline__ int wave_max_i(int v) {
 // four pixels' gradients for a splat in registers before the single cross-lane
 // reduction, so the reduction is paid once per 256 pixels.
 constexpr int BT = 64;
-__global__ __launch_bounds__(BT) void k_raster_bwd(RasterIn in, const float* __restrict__ final_t,
+__global__ __launch_bounds__(BT, 5) void k_raster_bwd(RasterIn in, const float* __restrict__ final_t,
                                                    const int32_t* __restrict__ last_contrib,
                                                    const float* __restrict__ dl_dpix, float* __restrict__ accum) {
   __shared__ uint64_t s_tab[32];
@@ -247,7 +260,6 @@ __global__ __launch_bounds__(BT) void k_raster_bwd(RasterIn in, const float* __r
   __shared__ float4 s_b[BT];
   __shared__ float s_c[BT];
   __shared__ uint32_t s_vi[BT];
-  __shared__ uint32_t s_m[BT];
   __shared__ float s_g[9][BT];
   const int lane = threadIdx.x;
   if (lane < 32) s_tab[lane] = kExp2Tab[lane];
@@ -262,7 +274,10 @@ __global__ __launch_bounds__(BT) void k_raster_bwd(RasterIn in, const float* __r
   const float fpx = (float)px + 0.5f;
   const float bg0 = in.bg[0], bg1 = in.bg[1], bg2 = in.bg[2];
   int lc[4];
-  float T[4], d0[4], d1[4], d2[4], h0[4], h1[4], h2[4], fpy[4];
+  // H: sum over channels of dL/dpixel_c times the colour that the splats
+  // behind the current one (and the background) add, i.e. d . h with the
+  // reference's per-channel accumulators h (backward.rs:470-480)
+  float T[4], d0[4], d1[4], d2[4], H[4], fpy[4];
   int lc_max = -1;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -279,17 +294,25 @@ __global__ __launch_bounds__(BT) void k_raster_bwd(RasterIn in, const float* __r
       d1[q] = dl_dpix[pix * 3 + 1];
       d2[q] = dl_dpix[pix * 3 + 2];
     }
-    h0[q] = T[q] * bg0;
-    h1[q] = T[q] * bg1;
-    h2[q] = T[q] * bg2;
+    H[q] = T[q] * ((d0[q] * bg0 + d1[q] * bg1) + d2[q] * bg2);
     lc_max = max(lc_max, lc[q]);
   }
-  const int top = wave_max_i(lc_max);
+  // wave-uniform (scalar) so that the splat loops are scalar-controlled
+  const int top = __builtin_amdgcn_readfirstlane(wave_max_i(lc_max));
   if (top < 0) return;
+  // the per-pixel inputs are needed by the first contributing splat; wait for
+  // them here rather than inside the loop, where the wait would also cover
+  // the previous batch's accumulator atomics
+#pragma unroll
+  for (int q = 0; q < 4; ++q) asm volatile("" ::"v"(d0[q]), "v"(d1[q]), "v"(d2[q]));
+  float g[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) g[k] = 0.0f;
   for (int hi = top; hi >= 0; hi -= BT) {
     const int lo = max(0, hi - (BT - 1));
     const int nb = hi - lo + 1;
     __syncthreads();  // previous batch fully read (one wave: s_barrier only)
+    uint32_t my_m = 0;  // group mask of this lane's staged splat
     if (lane < nb) {
       uint32_t vi = in.pair_vals[start + lo + lane];
       s_vi[lane] = vi;
@@ -298,21 +321,32 @@ __global__ __launch_bounds__(BT) void k_raster_bwd(RasterIn in, const float* __r
       s_a[lane] = A;
       s_b[lane] = B;
       s_c[lane] = in.S.col_b[vi];
-      s_m[lane] = wave_mask(A.x, A.y, A.z, A.w, B.x, in.S.reach[vi], tx, ty);
+      my_m = wave_mask(A.x, A.y, A.z, A.w, B.x, in.S.reach[vi], tx, ty);
     }
 #pragma unroll
     for (int k = 0; k < 9; ++k) s_g[k][lane] = 0.0f;
     __syncthreads();
-    for (int j = nb - 1; j >= 0; --j) {
-      const uint32_t m = s_m[j];  // wave-uniform
-      if (!m) continue;
+    // splats that reach some pixel group, visited back to front as set bits
+    // of a scalar mask; the next one's LDS data is fetched while the current
+    // one is processed
+    uint64_t live = __ballot(my_m != 0);
+    int jn = live ? 63 - __builtin_clzll(live) : 0;
+    float4 An = s_a[jn], Bn = s_b[jn];
+    float cbn = s_c[jn];
+    while (live) {
+      const int j = jn;
+      live &= ~(1ull << j);
+      const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)my_m, j);
       const int li = lo + j;
-      const float4 A = s_a[j];
-      const float4 B = s_b[j];
-      const float cb = s_c[j];
-      float g[9];
-#pragma unroll
-      for (int k = 0; k < 9; ++k) g[k] = 0.0f;
+      const float4 A = An;
+      const float4 B = Bn;
+      const float cb = cbn;
+      if (live) {
+        jn = 63 - __builtin_clzll(live);
+        An = s_a[jn];
+        Bn = s_b[jn];
+        cbn = s_c[jn];
+      }
       bool contrib = false;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -348,20 +382,20 @@ __global__ __launch_bounds__(BT) void k_raster_bwd(RasterIn in, const float* __r
         g[0] += d0[q] * weight;
         g[1] += d1[q] * weight;
         g[2] += d2[q] * weight;
-        float dl_dalpha = d0[q] * (Tq * B.z - h0[q] * inv_1ma);
-        dl_dalpha += d1[q] * (Tq * B.w - h1[q] * inv_1ma);
-        dl_dalpha += d2[q] * (Tq * cb - h2[q] * inv_1ma);
+        const float dcol = d0[q] * B.z + d1[q] * B.w + d2[q] * cb;
+        float dl_dalpha = Tq * dcol - H[q] * inv_1ma;
         float dl_draw = alpha_raw > MAX_ALPHA ? 0.0f : dl_dalpha;
-        g[3] += dl_draw * g_exp;
-        float dl_dpower = dl_draw * B.y * g_exp;
-        g[4] += dl_dpower * (-0.5f * dx * dx);
-        g[5] += dl_dpower * (-dx * dy);
-        g[6] += dl_dpower * (-0.5f * dy * dy);
-        g[7] += dl_dpower * (A.z * dx + A.w * dy);
-        g[8] += dl_dpower * (A.w * dx + B.x * dy);
-        h0[q] += weight * B.z;
-        h1[q] += weight * B.w;
-        h2[q] += weight * cb;
+        float e = dl_draw * g_exp;
+        g[3] += e;
+        // dL/dpower moments; the conic factors are applied at the flush
+        float dl_dpower = e * B.y;
+        float tx_ = dl_dpower * dx, ty_ = dl_dpower * dy;
+        g[4] += tx_;
+        g[5] += ty_;
+        g[6] += tx_ * dx;
+        g[7] += tx_ * dy;
+        g[8] += ty_ * dy;
+        H[q] += weight * dcol;
         }
       }
       const uint64_t bm = __ballot(contrib);
@@ -377,15 +411,33 @@ __global__ __launch_bounds__(BT) void k_raster_bwd(RasterIn in, const float* __r
         if (lane < 8) s_g[rev3(lane)][j] += t8;
         if (lane == 8) s_g[8][j] += s8;
       }
+      // the accumulators live across splats and restart here, which spares
+      // re-initialising them for every splat
+#pragma unroll
+      for (int k = 0; k < 9; ++k) g[k] = 0.0f;
     }
     __syncthreads();
     if (lane < nb) {
+      // s_g = (dL/dcolour[3], dL/dopacity, S1..S5) with S the dL/dpower
+      // moments sum(dlp dx), sum(dlp dy), sum(dlp dx^2), sum(dlp dx dy),
+      // sum(dlp dy^2); dL/dconic and dL/dmean follow from the conic.
       const uint32_t vi = s_vi[lane];
+      const float4 A = s_a[lane];
+      const float4 B = s_b[lane];
+      const float s1 = s_g[4][lane], s2 = s_g[5][lane], s3 = s_g[6][lane], s4 = s_g[7][lane], s5 = s_g[8][lane];
+      float out[9];
+      out[0] = s_g[0][lane];
+      out[1] = s_g[1][lane];
+      out[2] = s_g[2][lane];
+      out[3] = s_g[3][lane];
+      out[4] = -0.5f * s3;
+      out[5] = -s4;
+      out[6] = -0.5f * s5;
+      out[7] = A.z * s1 + A.w * s2;
+      out[8] = A.w * s1 + B.x * s2;
 #pragma unroll
-      for (int k = 0; k < 9; ++k) {
-        float sum = s_g[k][lane];
-        if (sum != 0.0f) atomicAdd(&accum[(size_t)vi * 9 + k], sum);
-      }
+      for (int k = 0; k < 9; ++k)
+        if (out[k] != 0.0f) atomicAdd(&accum[(size_t)vi * 9 + k], out[k]);
     }
   }
 }
