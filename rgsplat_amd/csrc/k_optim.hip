@@ -209,7 +209,7 @@ struct SoaLoad {
 __global__ __launch_bounds__(256) void k_geometry_aos(const float* __restrict__ P, int cap, int n,
                                                       const CamDev* __restrict__ cams, int view,
                                                       const uint32_t* __restrict__ cnt,
-                                                      const float* __restrict__ accum,
+                                                      const float* __restrict__ accum, size_t acc_stride,
                                                       float* __restrict__ grads) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(256) void k_geometry_aos(const float* __restrict__ 
   }
   float acc[9];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) acc[k] = accum[vi * 9 + k];
+  for (int k = 0; k < 9; ++k) acc[k] = accum[(size_t)k * acc_stride + vi];
   SoaLoad L{P, (size_t)cap, (size_t)i};
   const CamDev& c = cams[view];
   float basis[16], masked[3], dk[16];
@@ -232,10 +232,10 @@ __global__ __launch_bounds__(256) void k_geometry_aos(const float* __restrict__ 
 }
 
 void launch_geometry_aos(const float* P, int cap, int n, const CamDev* cams, int view, const uint32_t* cnt,
-                         const float* accum, float* grads, hipStream_t st) {
+                         const float* accum, size_t acc_stride, float* grads, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_geometry_aos, dim3((n + 255) / 256), dim3(256), 0, st, P, cap, n, cams, view, cnt, accum,
-                     grads);
+                     acc_stride, grads);
 }
 
 __device__ __forceinline__ float lr_of(int j, const AdamConsts& k) {
@@ -313,8 +313,8 @@ static_assert(TRAIN_SCRATCH_ROWS == 18, "scratch rows");
 __global__ __launch_bounds__(256) void k_train_geometry(const float* __restrict__ P, int cap, int n,
                                                         const CamDev* __restrict__ cams, int n_views,
                                                         const uint32_t* __restrict__ cnt,
-                                                        const float* __restrict__ accum, float* __restrict__ scr,
-                                                        const uint32_t* __restrict__ status) {
+                                                        const float* __restrict__ accum, size_t acc_stride,
+                                                        float* __restrict__ scr, const uint32_t* __restrict__ status) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (*status & (ST_PAIR_OVERFLOW | ST_NAN_DEPTH)) return;
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(256) void k_train_geometry(const float* __restrict_
     SoaLoad Lv{Pv, (size_t)cap, (size_t)i};
     float acc[9];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) acc[q] = accum[vi * 9 + q];
+    for (int q = 0; q < 9; ++q) acc[q] = accum[(size_t)q * acc_stride + vi];
     float basis[16], masked[3], dk[16];
     uint32_t bits = 0;
     view_sh(cams[v], Lv, acc, basis, masked, dk, &bits);
@@ -364,7 +364,8 @@ __global__ __launch_bounds__(256) void k_train_adam(float* __restrict__ P, float
                                                     float* __restrict__ M2, int cap, int n,
                                                     const CamDev* __restrict__ cams, int n_views,
                                                     const uint32_t* __restrict__ cnt, float* __restrict__ accum,
-                                                    const float* __restrict__ scr, float* __restrict__ gaccum,
+                                                    size_t acc_stride, const float* __restrict__ scr,
+                                                    float* __restrict__ gaccum,
                                                     uint32_t* __restrict__ gdenom,
                                                     const AdamConsts* __restrict__ k_dev,
                                                     const uint32_t* __restrict__ status) {
@@ -386,7 +387,7 @@ __global__ __launch_bounds__(256) void k_train_adam(float* __restrict__ P, float
     for (int v = 0; v < n_views; ++v) {
       if (!(vis >> v & 1ull)) continue;
       size_t vi = (size_t)v * cap + i;
-      const float m = (clamp_in >> v & 1ull) ? accum[vi * 9 + ch] : 0.0f;
+      const float m = (clamp_in >> v & 1ull) ? accum[(size_t)ch * acc_stride + vi] : 0.0f;
       float basis[16];
       view_basis(cams[v], L, basis);
 #pragma unroll
@@ -406,7 +407,7 @@ __global__ __launch_bounds__(256) void k_train_adam(float* __restrict__ P, float
     if (!(vis >> v & 1ull)) continue;
     size_t vi = (size_t)v * cap + i;
 #pragma unroll
-    for (int q = 0; q < 9; ++q) accum[vi * 9 + q] = 0.0f;
+    for (int q = 0; q < 9; ++q) accum[(size_t)q * acc_stride + vi] = 0.0f;
   }
   float rot[4];
 #pragma unroll
@@ -428,13 +429,13 @@ __global__ __launch_bounds__(256) void k_train_adam(float* __restrict__ P, float
 }
 
 void launch_train_adam(float* P, float* M1, float* M2, int cap, int n, const CamDev* cams, int n_views,
-                       const uint32_t* cnt, float* accum, float* scratch, float* gaccum, uint32_t* gdenom,
-                       const AdamConsts* k_dev, const uint32_t* status, hipStream_t st) {
+                       const uint32_t* cnt, float* accum, size_t acc_stride, float* scratch, float* gaccum,
+                       uint32_t* gdenom, const AdamConsts* k_dev, const uint32_t* status, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_train_geometry, dim3((n + 255) / 256), dim3(256), 0, st, P, cap, n, cams, n_views, cnt, accum,
-                     scratch, status);
+                     acc_stride, scratch, status);
   hipLaunchKernelGGL(k_train_adam, dim3((n + 255) / 256), dim3(256), 0, st, P, M1, M2, cap, n, cams, n_views, cnt,
-                     accum, scratch, gaccum, gdenom, k_dev, status);
+                     accum, acc_stride, scratch, gaccum, gdenom, k_dev, status);
 }
 
 // ln on the device: the f64 logarithm rounded once to f32 (correctly rounded

@@ -437,7 +437,7 @@ __global__ __launch_bounds__(BT, 5) void k_raster_bwd(RasterIn in, const float* 
       out[8] = A.w * s1 + B.x * s2;
 #pragma unroll
       for (int k = 0; k < 9; ++k)
-        if (out[k] != 0.0f) atomicAdd(&accum[(size_t)vi * 9 + k], out[k]);
+        if (out[k] != 0.0f) atomicAdd(&accum[(size_t)k * in.acc_stride + vi], out[k]);
     }
   }
 }

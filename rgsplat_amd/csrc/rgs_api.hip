@@ -376,6 +376,7 @@ RasterIn raster_in_n(rgs_ctx* c, int n_views, int total_tiles, const float bg[3]
   in.cams = P_<CamDev>(c->cams_d);
   in.n_views = n_views;
   in.sparse_lanes = c->sparse_lanes;
+  in.acc_stride = 0;  // set by the backward's callers
   in.total_tiles = total_tiles;
   for (int k = 0; k < 3; ++k) in.bg[k] = bg[k];
   return in;
@@ -647,11 +648,15 @@ int rgs_render_and_backward(rgs_ctx* c, const rgs_gaussian* g, size_t n, const r
   enqueue_loss(c, vs, P_<float>(c->rgb), P_<float>(c->api_target), lambda);
   CHK(ensure(c, c->api_acc9, 9 * cap * 4));
   launch_fill_f32(P_<float>(c->api_acc9), 9 * cap, 0.0f, c->st);
-  launch_raster_bwd(raster_in(c, vs, bg), P_<float>(c->final_t), P_<int32_t>(c->last), P_<float>(c->dl),
-                    P_<float>(c->api_acc9), c->st);
+  {
+    RasterIn in = raster_in(c, vs, bg);
+    in.acc_stride = cap;
+    launch_raster_bwd(in, P_<float>(c->final_t), P_<int32_t>(c->last), P_<float>(c->dl), P_<float>(c->api_acc9),
+                      c->st);
+  }
   CHK(ensure(c, c->api_grads, cap * 60 * 4));
   launch_geometry_aos(P_<float>(c->api_P), (int)cap, (int)n, P_<CamDev>(c->cams_d), 0, P_<uint32_t>(c->cnt),
-                      P_<float>(c->api_acc9), P_<float>(c->api_grads), c->st);
+                      P_<float>(c->api_acc9), cap, P_<float>(c->api_grads), c->st);
   if (n) HIPCHK(hipMemcpyAsync(grads_out, c->api_grads.p, n * 60 * 4, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipMemcpyAsync(c->h_loss, c->loss.p, 4, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
@@ -822,10 +827,11 @@ int rgs_backward_dev(rgs_ctx* c, const float* dl_dev, float* grads_dev) {
   size_t n = (size_t)c->fwd_n, cap = n ? n : 1;
   CHK(ensure(c, c->api_acc9, 9 * cap * 4));
   launch_fill_f32(P_<float>(c->api_acc9), 9 * cap, 0.0f, c->st);
-  launch_raster_bwd(raster_in(c, c->fwd_views, c->fwd_bg), P_<float>(c->final_t), P_<int32_t>(c->last), dl_dev,
-                    P_<float>(c->api_acc9), c->st);
+  RasterIn in = raster_in(c, c->fwd_views, c->fwd_bg);
+  in.acc_stride = cap;
+  launch_raster_bwd(in, P_<float>(c->final_t), P_<int32_t>(c->last), dl_dev, P_<float>(c->api_acc9), c->st);
   launch_geometry_aos(P_<float>(c->api_P), (int)cap, (int)n, P_<CamDev>(c->cams_d), 0, P_<uint32_t>(c->cnt),
-                      P_<float>(c->api_acc9), grads_dev, c->st);
+                      P_<float>(c->api_acc9), cap, grads_dev, c->st);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -936,12 +942,13 @@ static void enqueue_step_rest(rgs_ctx* c, const ViewSet& vs, const float bg[3], 
   enqueue_loss(c, vs, P_<float>(c->rgb), P_<float>(c->targets), lambda);
   stage_end(c, S_LOSS);
   stage_begin(c, S_BWD);
-  launch_raster_bwd(raster_in(c, vs, bg), P_<float>(c->final_t), P_<int32_t>(c->last), P_<float>(c->dl),
-                    P_<float>(c->accum), c->st);
+  RasterIn in = raster_in(c, vs, bg);
+  in.acc_stride = vs.cams.size() * cap;
+  launch_raster_bwd(in, P_<float>(c->final_t), P_<int32_t>(c->last), P_<float>(c->dl), P_<float>(c->accum), c->st);
   stage_end(c, S_BWD);
   stage_begin(c, S_ADAM);
   launch_train_adam(P_<float>(c->P), P_<float>(c->M1), P_<float>(c->M2), (int)cap, n, P_<CamDev>(c->cams_d),
-                    (int)vs.cams.size(), P_<uint32_t>(c->cnt), P_<float>(c->accum),
+                    (int)vs.cams.size(), P_<uint32_t>(c->cnt), P_<float>(c->accum), vs.cams.size() * cap,
                     P_<float>(c->train_scr), P_<float>(c->gacc), P_<uint32_t>(c->gden), P_<AdamConsts>(c->adam_d), sc + SC_STATUS, c->st);
   stage_end(c, S_ADAM);
 }
@@ -1117,6 +1124,7 @@ int rgs_autotune_raster(rgs_ctx* c, const int* ids, int n_ids, const float bg[3]
   auto time_once = [&](float* ms) -> int {
     CHK(ensure_tile_order(c, vs));
     RasterIn in = raster_in(c, vs, bg);
+    in.acc_stride = vs.cams.size() * cap;
     HIPCHK(hipEventRecord(ev[0], c->st));
     launch_raster_fwd(in, 0, P_<float>(c->rgb), P_<float>(c->final_t), P_<int32_t>(c->last), c->st);
     HIPCHK(hipEventRecord(ev[1], c->st));

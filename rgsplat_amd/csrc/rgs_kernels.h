@@ -108,6 +108,7 @@ struct RasterIn {
   const CamDev* cams;
   int n_views;
   int sparse_lanes;  // backward: at most this many contributing lanes -> LDS atomics
+  size_t acc_stride;  // backward accumulators: value k of entry vi at accum[k * acc_stride + vi]
   int total_tiles;
   float bg[3];
 };
@@ -115,8 +116,9 @@ struct RasterIn {
 // mode 1: rasterizer.rs RGBA (rgba = bg + C, A = 1 - T; empty tiles left alone).
 void launch_raster_fwd(const RasterIn& in, int mode, float* rgb, float* final_t, int32_t* last, hipStream_t st);
 // Accumulates the 9 per-splat screen-space gradients (dL/dcolour[3],
-// dL/dopacity, dL/dconic[3], dL/dsx, dL/dsy) into accum[vi * 9 + k] (one
-// splat's nine values share cache lines, so a tile's atomics touch few lines).
+// dL/dopacity, dL/dconic[3], dL/dsx, dL/dsy) into accum[k * acc_stride + vi]:
+// one row per value, so a splat's nine atomics land on nine different cache
+// lines (L2 channels) instead of queueing on one.
 void launch_raster_bwd(const RasterIn& in, const float* final_t, const int32_t* last, const float* dl_dpix,
                        float* accum, hipStream_t st);
 
@@ -151,7 +153,7 @@ void launch_aos_to_soa(const float* aos, size_t n, float* P, size_t cap, hipStre
 void launch_soa_to_aos(const float* P, size_t cap, size_t n, float* aos, hipStream_t st);
 // GaussianGrad AoS (60 floats, zero when culled) for one view.
 void launch_geometry_aos(const float* P, int cap, int n, const CamDev* cams, int view, const uint32_t* cnt,
-                         const float* accum, float* grads, hipStream_t st);
+                         const float* accum, size_t acc_stride, float* grads, hipStream_t st);
 // Adam on the reference layouts: AoS params/grads, row-major moments.
 void launch_adam_aos(float* g, const float* grads, size_t n, float* m1, float* m2, float* accum, uint32_t* denom,
                      const AdamConsts& k, hipStream_t st);
@@ -160,8 +162,8 @@ void launch_adam_aos(float* g, const float* grads, size_t n, float* m1, float* m
 // when *status has ST_PAIR_OVERFLOW or ST_NAN_DEPTH. Clears accum.
 constexpr int TRAIN_SCRATCH_ROWS = 18;
 void launch_train_adam(float* P, float* M1, float* M2, int cap, int n, const CamDev* cams, int n_views,
-                       const uint32_t* cnt, float* accum, float* scratch, float* gaccum, uint32_t* gdenom,
-                       const AdamConsts* k_dev, const uint32_t* status, hipStream_t st);
+                       const uint32_t* cnt, float* accum, size_t acc_stride, float* scratch, float* gaccum,
+                       uint32_t* gdenom, const AdamConsts* k_dev, const uint32_t* status, hipStream_t st);
 // reset_opacities on SoA (row 10) or AoS (offset 3) params.
 void launch_reset_opacity(float* p, size_t n, size_t stride, hipStream_t st);
 // Densify/prune on SoA params: P -> P_out (compacted + children). Writes the
