@@ -343,7 +343,8 @@ SplatArrays splats(rgs_ctx* c) {
 }
 
 // Projection, depth sort and the pair-count scan (sizes fixed by n and vs).
-void enqueue_project(rgs_ctx* c, const float* P, size_t cap, int n, const ViewSet& vs) {
+// tight: training-step binning (only the tiles a splat's alpha can reach).
+void enqueue_project(rgs_ctx* c, const float* P, size_t cap, int n, const ViewSet& vs, bool tight) {
   hipStream_t st = c->st;
   uint32_t* sc = P_<uint32_t>(c->scalars);
   int nv = (int)vs.cams.size();
@@ -351,7 +352,7 @@ void enqueue_project(rgs_ctx* c, const float* P, size_t cap, int n, const ViewSe
   launch_fill_u32(sc + SC_DEPTH_N, 1, (uint32_t)dn, st);
   stage_begin(c, S_PROJECT);
   launch_project(P, (int)cap, n, P_<CamDev>(c->cams_d), nv, splats(c), P_<uint32_t>(c->dk[0]),
-                 P_<uint32_t>(c->dv[0]), sc + SC_STATUS, st);
+                 P_<uint32_t>(c->dv[0]), sc + SC_STATUS, tight ? 1 : 0, st);
   stage_end(c, S_PROJECT);
   uint32_t* keys[2] = {P_<uint32_t>(c->dk[0]), P_<uint32_t>(c->dk[1])};
   uint32_t* vals[2] = {P_<uint32_t>(c->dv[0]), P_<uint32_t>(c->dv[1])};
@@ -420,7 +421,7 @@ int render_eager(rgs_ctx* c, const float* P, size_t cap, int n, ViewSet& vs, con
   HIPCHK(hipMemcpyAsync(c->cams_d.p, vs.cams.data(), vs.cams.size() * sizeof(CamDev), hipMemcpyHostToDevice, c->st));
   uint32_t* sc = P_<uint32_t>(c->scalars);
   launch_fill_u32(sc, SC_COUNT, 0u, c->st);
-  enqueue_project(c, P, cap, n, vs);
+  enqueue_project(c, P, cap, n, vs, false);
   HIPCHK(hipMemcpyAsync(c->h_scalars, sc, SC_COUNT * 4, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   if (c->h_scalars[SC_STATUS] & ST_NAN_DEPTH) return fail(RGS_E_NAN, "NaN depth in projection");
@@ -956,7 +957,7 @@ static void enqueue_step_rest(rgs_ctx* c, const ViewSet& vs, const float bg[3], 
 // Enqueues one whole training step on the context's stream (no host sync).
 static void enqueue_step(rgs_ctx* c, const ViewSet& vs, const float bg[3], float lambda) {
   launch_fill_u32(P_<uint32_t>(c->scalars), SC_COUNT, 0u, c->st);
-  enqueue_project(c, P_<float>(c->P), c->cap, (int)c->n, vs);
+  enqueue_project(c, P_<float>(c->P), c->cap, (int)c->n, vs, true);
   enqueue_step_rest(c, vs, bg, lambda);
 }
 
@@ -1006,7 +1007,7 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
       HIPCHK(hipMemcpyAsync(c->adam_d.p, c->h_adam, sizeof(AdamConsts), hipMemcpyHostToDevice, c->st));
       uint32_t* sc = P_<uint32_t>(c->scalars);
       launch_fill_u32(sc, SC_COUNT, 0u, c->st);
-      enqueue_project(c, P_<float>(c->P), cap, n, vs);
+      enqueue_project(c, P_<float>(c->P), cap, n, vs, true);
       HIPCHK(hipMemcpyAsync(c->h_scalars, sc, SC_COUNT * 4, hipMemcpyDeviceToHost, c->st));
       HIPCHK(hipStreamSynchronize(c->st));
       if (c->h_scalars[SC_STATUS] & ST_NAN_DEPTH) {
@@ -1112,7 +1113,7 @@ int rgs_autotune_raster(rgs_ctx* c, const int* ids, int n_ids, const float bg[3]
   launch_fill_u32(sc, SC_COUNT, 0u, c->st);
   const bool prof = c->profiling;
   c->profiling = false;
-  enqueue_project(c, P_<float>(c->P), cap, n, vs);
+  enqueue_project(c, P_<float>(c->P), cap, n, vs, true);
   HIPCHK(hipMemcpyAsync(c->h_scalars, sc, SC_COUNT * 4, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   size_t K = c->h_scalars[SC_TOTAL];

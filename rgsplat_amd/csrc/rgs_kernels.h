@@ -25,9 +25,12 @@ enum : uint32_t { ST_NAN_DEPTH = 1u, ST_PAIR_OVERFLOW = 2u };
 // ---- projection (k_project.hip)
 // Projects n Gaussians (SoA params P[59][cap]) into n_views views. Writes the
 // splat arrays and the depth-sort input (key = depth bits, 0xffffffff when
-// culled; value = v * cap + i) at dense index v * n + i.
+// culled; value = v * cap + i) at dense index v * n + i. tight != 0: each
+// splat's tile rect is the reference's 3-sigma rect cut down to the tiles its
+// alpha-reach ellipse (alpha >= 1/255) can reach; the dropped (tile, splat)
+// pairs change no image or gradient, only the list indices.
 void launch_project(const float* P, int cap, int n, const CamDev* cams, int n_views, SplatArrays S,
-                    uint32_t* sort_keys, uint32_t* sort_vals, uint32_t* status, hipStream_t st);
+                    uint32_t* sort_keys, uint32_t* sort_vals, uint32_t* status, int tight, hipStream_t st);
 
 // Device expf over n values (diagnostic).
 void launch_eval_expf(const float* x, float* out, size_t n, hipStream_t st);

@@ -1,9 +1,10 @@
-"""The raster's per-group culling test (k_raster.hip wave_mask, with the
-reach bound R from k_project.hip) restated in numpy f32 and checked against
-the alpha test itself on the oracle's splats: no (pixel, splat) pair whose
-alpha reaches 1/255 may lie in a pixel group the test culls. Skipping such a
-pair would change the image, so this is the culling's correctness condition
-(the GPU image parity tests cover the device code itself)."""
+"""The culling of the training step, restated in numpy f32 and checked
+against the alpha test itself on the oracle's splats: the raster's per-group
+test (k_raster.hip wave_mask, with the reach bound R from k_project.hip) and
+the tight tile rects of the training step's binning (k_project.hip, tight).
+No (pixel, splat) pair whose alpha reaches 1/255 may be culled by either:
+skipping one would change the image, so this is the culling's correctness
+condition (the GPU parity tests cover the device code itself)."""
 import numpy as np
 import pytest
 
@@ -27,6 +28,28 @@ def reach_bound(sp):
     R = np.maximum(R, f(1e-3))
     R[q_err > f(0.02)] = np.inf
     return R
+
+
+def tight_rect(sp, R):
+    """k_project.hip with tight != 0: the reference's 3-sigma tile rect cut to
+    the tiles whose pixel centres meet the box |dx| <= sqrt(R a) + 0.5,
+    |dy| <= sqrt(R c) + 0.5 of the alpha-reach ellipse."""
+    rect = sp["rect"].astype(np.int64).copy()
+    cov = sp["cov2d"]
+    fin = np.isfinite(R)
+    with np.errstate(invalid="ignore"):
+        hx = np.sqrt(R * cov[:, 0]) + f(0.5)
+        hy = np.sqrt(R * cov[:, 2]) + f(0.5)
+        fx0 = np.ceil((sp["sx"] - hx - f(15.5)) * f(1 / 16))
+        fx1 = np.floor((sp["sx"] + hx - f(0.5)) * f(1 / 16)) + f(1)
+        fy0 = np.ceil((sp["sy"] - hy - f(15.5)) * f(1 / 16))
+        fy1 = np.floor((sp["sy"] + hy - f(0.5)) * f(1 / 16)) + f(1)
+    out = rect.copy()
+    out[fin, 0] = np.maximum(rect[fin, 0], np.maximum(fx0[fin], rect[fin, 0]).astype(np.int64))
+    out[fin, 1] = np.maximum(rect[fin, 1], np.maximum(fy0[fin], rect[fin, 1]).astype(np.int64))
+    out[fin, 2] = np.minimum(rect[fin, 2], np.minimum(fx1[fin], rect[fin, 2]).astype(np.int64))
+    out[fin, 3] = np.minimum(rect[fin, 3], np.minimum(fy1[fin], rect[fin, 3]).astype(np.int64))
+    return out
 
 
 def group_masks(sx, sy, ca, cb, cc, R, tx, ty):
@@ -60,6 +83,7 @@ def test_culling_keeps_every_contributing_pair(name, n, size):
     sp = r.splats()
     off, gis = r.tiles()
     R = reach_bound(sp)
+    tight = tight_rect(sp, R)
     conic, op = sp["conic"], sp["opacity"]
     row = {int(gi): k for k, gi in enumerate(sp["gi"])}
     ntx = (cam.width + 15) // 16
@@ -80,6 +104,10 @@ def test_culling_keeps_every_contributing_pair(name, n, size):
             power = f(-0.5) * (A * dx * dx + f(2) * B * dx * dy + C * dy * dy)
             alpha = np.minimum(op[k] * np.exp(power).astype(f), f(0.99))
             hit = inside & (power <= 0) & (alpha >= f(1 / 255))
+            if hit.any():
+                # the training step's binning keeps this (tile, splat) pair
+                x0, y0, x1, y1 = tight[k]
+                assert x0 <= tx < x1 and y0 <= ty < y1, (t, int(lst[j]))
             for q in range(4):
                 kept += int(m[q][j])
                 if hit[4 * q:4 * q + 4].any():
@@ -87,3 +115,6 @@ def test_culling_keeps_every_contributing_pair(name, n, size):
                     assert m[q][j], (t, int(lst[j]), q)
     # the test culls something, and keeps little beyond the groups that pass
     assert passing <= kept < 1.25 * passing
+    # the tight rects drop a good share of the reference's pairs
+    area = lambda r: np.maximum(r[:, 2] - r[:, 0], 0) * np.maximum(r[:, 3] - r[:, 1], 0)
+    assert area(tight).sum() < 0.95 * area(sp["rect"].astype(np.int64)).sum()
