@@ -47,6 +47,39 @@ __device__ __forceinline__ LossTile loss_tile(const LossIn& in, int t) {
   return LossTile{v, lt % nlx, lt / nlx};
 }
 
+// Per-view constants, read once per block.
+struct LossView {
+  int w, h;
+  size_t pb, tb;  // first pixel of the view in the image arrays / target store
+  float m_total, lam;
+};
+
+__device__ __forceinline__ LossView loss_view(const LossIn& in, const LossTile& T) {
+  const CamDev& c = in.cams[T.v];
+  LossView V;
+  V.w = c.w;
+  V.h = c.h;
+  V.pb = (size_t)c.pix_base;
+  V.tb = (size_t)c.tgt_base;
+  V.m_total = (float)((size_t)c.w * c.h * 3);
+  V.lam = in.lambda;
+  return V;
+}
+
+constexpr int LPT = (LN + LT - 1) / LT;  // staged values per thread
+
+// Pixel offset within the view of each staged value this thread loads (the
+// tile plus a 5-pixel halo), -1 outside the image (zero padding).
+__device__ __forceinline__ void stage_offsets(const LossView& V, const LossTile& T, int off[LPT]) {
+  const int x0 = T.tx * LTILE - 5, y0 = T.ty * LTILE - 5;
+#pragma unroll
+  for (int i = 0; i < LPT; ++i) {
+    const int idx = (int)threadIdx.x + i * LT;
+    const int gy = y0 + idx / LH, gx = x0 + idx % LH;
+    off[i] = (idx < LN && gx >= 0 && gy >= 0 && gx < V.w && gy < V.h) ? gy * V.w + gx : -1;
+  }
+}
+
 __device__ __forceinline__ float block_sum_256(float x, float* s_tmp) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
@@ -57,8 +90,7 @@ __device__ __forceinline__ float block_sum_256(float x, float* s_tmp) {
   return r;
 }
 
-__global__ __launch_bounds__(LT) void k_ssim_fwd(LossIn in, Kern11 K, float* __restrict__ dl,
-                                                 float* __restrict__ maps, float* __restrict__ partial,
+__global__ __launch_bounds__(LT) void k_ssim_fwd(LossIn in, Kern11 K, float* __restrict__ maps, float* __restrict__ partial,
                                                  size_t total_pix) {
 #pragma clang fp contract(fast)
   __shared__ float s_x[LN];
@@ -68,24 +100,34 @@ __global__ __launch_bounds__(LT) void k_ssim_fwd(LossIn in, Kern11 K, float* __r
   const int t = threadIdx.x;
   const int tile = xcd_remap(blockIdx.x, gridDim.x);  // halo-sharing neighbours on one XCD
   const LossTile T = loss_tile(in, tile);
-  const CamDev& c = in.cams[T.v];
-  const int x0 = T.tx * LTILE - 5, y0 = T.ty * LTILE - 5;
-  const float m_total = (float)((size_t)c.w * c.h * 3);
-  const float lam = in.lambda;
+  const LossView V = loss_view(in, T);
   const float C1 = 0.01f * 0.01f;
   const float C2 = 0.03f * 0.03f;
   // vertical pass / output mapping: column oc, rows 4 * oq .. 4 * oq + 3
   const int oc = t & (LTILE - 1), oq = t >> 5;
+  int off[LPT];
+  stage_offsets(V, T, off);
+  // channel ch + 1 is fetched into registers while channel ch is blurred
+  float vx[LPT], vy[LPT];
+  auto fetch = [&](int ch) {
+#pragma unroll
+    for (int i = 0; i < LPT; ++i) {
+      vx[i] = off[i] >= 0 ? in.rgb[(V.pb + (size_t)off[i]) * 3 + ch] : 0.0f;
+      vy[i] = off[i] >= 0 ? in.target[(V.tb + (size_t)off[i]) * 4 + ch] : 0.0f;
+    }
+  };
+  fetch(0);
   for (int ch = 0; ch < 3; ++ch) {
-    for (int idx = t; idx < LN; idx += LT) {
-      int gy = y0 + idx / LH, gx = x0 + idx % LH;
-      bool ok = gx >= 0 && gy >= 0 && gx < c.w && gy < c.h;
-      size_t gp = (size_t)c.pix_base + (size_t)gy * c.w + gx;
-      size_t tp = (size_t)c.tgt_base + (size_t)gy * c.w + gx;
-      s_x[idx] = ok ? in.rgb[gp * 3 + ch] : 0.0f;
-      s_y[idx] = ok ? in.target[tp * 4 + ch] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < LPT; ++i) {
+      const int idx = t + i * LT;
+      if (idx < LN) {
+        s_x[idx] = vx[i];
+        s_y[idx] = vy[i];
+      }
     }
     __syncthreads();
+    if (ch < 2) fetch(ch + 1);
     // horizontal: LH rows x (LTILE / RUN) runs
     for (int task = t; task < LH * (LTILE / RUN); task += LT) {
       const int r = task / (LTILE / RUN), c0 = (task % (LTILE / RUN)) * RUN;
@@ -140,8 +182,8 @@ __global__ __launch_bounds__(LT) void k_ssim_fwd(LossIn in, Kern11 K, float* __r
 #pragma unroll
     for (int o = 0; o < RUN; ++o) {
       const int py = T.ty * LTILE + oq * RUN + o;
-      if (px >= c.w || py >= c.h) continue;
-      const size_t pix = (size_t)c.pix_base + (size_t)py * c.w + px;
+      if (px >= V.w || py >= V.h) continue;
+      const size_t pix = V.pb + (size_t)py * V.w + px;
       float mx = mu[0][o], my = mu[1][o];
       float sx2 = mu[2][o] - mx * mx;
       float sy2 = mu[3][o] - my * my;
@@ -162,11 +204,7 @@ __global__ __launch_bounds__(LT) void k_ssim_fwd(LossIn in, Kern11 K, float* __r
       maps[(size_t)(2 * 3 + ch) * total_pix + pix] = rc;
       const float xv = s_x[(oq * RUN + o + 5) * LH + oc + 5];
       const float yv = s_y[(oq * RUN + o + 5) * LH + oc + 5];
-      float d = xv - yv;
-      l1 += __builtin_fabsf(d);
-      // Rust f32::signum: +1 for +0.0
-      float sg = d != d ? d : __builtin_copysignf(1.0f, d);
-      dl[pix * 3 + ch] = (1.0f - lam) * sg / m_total;
+      l1 += __builtin_fabsf(xv - yv);
     }
     float bl1 = block_sum_256(l1, s_tmp);
     float bss = block_sum_256(ss, s_tmp);
@@ -185,20 +223,39 @@ __global__ __launch_bounds__(LT) void k_ssim_bwd(LossIn in, Kern11 K, float* __r
   const int t = threadIdx.x;
   const int tile = xcd_remap(blockIdx.x, gridDim.x);  // halo-sharing neighbours on one XCD
   const LossTile T = loss_tile(in, tile);
-  const CamDev& c = in.cams[T.v];
-  const int x0 = T.tx * LTILE - 5, y0 = T.ty * LTILE - 5;
-  const float m_total = (float)((size_t)c.w * c.h * 3);
-  const float lam = in.lambda;
+  const LossView V = loss_view(in, T);
   const int oc = t & (LTILE - 1), oq = t >> 5;
-  for (int ch = 0; ch < 3; ++ch) {
-    for (int idx = t; idx < LN; idx += LT) {
-      int gy = y0 + idx / LH, gx = x0 + idx % LH;
-      bool ok = gx >= 0 && gy >= 0 && gx < c.w && gy < c.h;
-      size_t gp = (size_t)c.pix_base + (size_t)gy * c.w + gx;
+  int off[LPT];
+  stage_offsets(V, T, off);
+  // pixels this thread writes: column px, rows py0 .. py0 + RUN - 1
+  const int px = T.tx * LTILE + oc, py0 = T.ty * LTILE + oq * RUN;
+  float vm[SSIM_MAPS][LPT];
+  auto fetch = [&](int ch) {
 #pragma unroll
-      for (int m = 0; m < SSIM_MAPS; ++m) s_m[m][idx] = ok ? maps[(size_t)(m * 3 + ch) * total_pix + gp] : 0.0f;
+    for (int m = 0; m < SSIM_MAPS; ++m)
+#pragma unroll
+      for (int i = 0; i < LPT; ++i)
+        vm[m][i] = off[i] >= 0 ? maps[(size_t)(m * 3 + ch) * total_pix + V.pb + (size_t)off[i]] : 0.0f;
+  };
+  fetch(0);
+  for (int ch = 0; ch < 3; ++ch) {
+#pragma unroll
+    for (int i = 0; i < LPT; ++i) {
+      const int idx = t + i * LT;
+      if (idx < LN)
+#pragma unroll
+        for (int m = 0; m < SSIM_MAPS; ++m) s_m[m][idx] = vm[m][i];
     }
     __syncthreads();
+    if (ch < 2) fetch(ch + 1);
+    float ox[RUN], oy[RUN];
+#pragma unroll
+    for (int o = 0; o < RUN; ++o) {
+      const bool in_img = px < V.w && py0 + o < V.h;
+      const size_t q = (size_t)(py0 + o) * V.w + px;
+      ox[o] = in_img ? in.rgb[(V.pb + q) * 3 + ch] : 0.0f;
+      oy[o] = in_img ? in.target[(V.tb + q) * 4 + ch] : 0.0f;
+    }
     for (int task = t; task < LH * (LTILE / RUN); task += LT) {
       const int r = task / (LTILE / RUN), c0 = (task % (LTILE / RUN)) * RUN;
       float a[SSIM_MAPS][RUN];
@@ -245,16 +302,16 @@ __global__ __launch_bounds__(LT) void k_ssim_bwd(LossIn in, Kern11 K, float* __r
         for (int m = 0; m < SSIM_MAPS; ++m) b[m][o] += kk * hv[m];
       }
     }
-    const int px = T.tx * LTILE + oc;
 #pragma unroll
     for (int o = 0; o < RUN; ++o) {
-      const int py = T.ty * LTILE + oq * RUN + o;
-      if (px >= c.w || py >= c.h) continue;
-      const size_t pix = (size_t)c.pix_base + (size_t)py * c.w + px;
-      const float xv = in.rgb[pix * 3 + ch];
-      const float yv = in.target[((size_t)c.tgt_base + (size_t)py * c.w + px) * 4 + ch];
+      if (px >= V.w || py0 + o >= V.h) continue;
+      const size_t pix = V.pb + (size_t)(py0 + o) * V.w + px;
+      const float xv = ox[o], yv = oy[o];
+      const float d = xv - yv;
+      // Rust f32::signum: +1 for +0.0
+      const float sg = d != d ? d : __builtin_copysignf(1.0f, d);
       const float grad = b[0][o] + yv * b[1][o] + 2.0f * xv * b[2][o];
-      dl[pix * 3 + ch] += -lam * grad / m_total;
+      dl[pix * 3 + ch] = (1.0f - V.lam) * sg / V.m_total + -V.lam * grad / V.m_total;
     }
     __syncthreads();
   }
@@ -311,7 +368,7 @@ static Kern11 make_kernel() {
 void launch_loss(const LossIn& in, float* dl, float* ssim_maps, float* partial, float* loss_out, hipStream_t st) {
   if (in.total_ltiles <= 0) return;
   static const Kern11 K = make_kernel();
-  hipLaunchKernelGGL(k_ssim_fwd, dim3(in.total_ltiles), dim3(LT), 0, st, in, K, dl, ssim_maps, partial,
+  hipLaunchKernelGGL(k_ssim_fwd, dim3(in.total_ltiles), dim3(LT), 0, st, in, K, ssim_maps, partial,
                      in.total_pix);
   hipLaunchKernelGGL(k_ssim_bwd, dim3(in.total_ltiles), dim3(LT), 0, st, in, K, dl, ssim_maps, in.total_pix);
   hipLaunchKernelGGL(k_loss_reduce, dim3(in.n_views), dim3(256), 0, st, in, partial, loss_out);
