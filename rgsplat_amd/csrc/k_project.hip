@@ -15,7 +15,8 @@ namespace rgs {
 __global__ __launch_bounds__(256) void k_project(const float* __restrict__ P, int cap, int n,
                                                  const CamDev* __restrict__ cams, int n_views, SplatArrays S,
                                                  uint32_t* __restrict__ sort_keys, uint32_t* __restrict__ sort_vals,
-                                                 uint32_t* __restrict__ status, int tight) {
+                                                 uint32_t* __restrict__ status, int tight,
+                                                 uint8_t* __restrict__ cover) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   // view-independent part: read once, shared by all views
@@ -150,13 +151,20 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ P, in
       const float fx1 = __builtin_floorf((sx + hx - 0.5f) * (1.0f / TILE)) + 1.0f;
       const float fy0 = __builtin_ceilf((sy - hy - 15.5f) * (1.0f / TILE));
       const float fy1 = __builtin_floorf((sy + hy - 0.5f) * (1.0f / TILE)) + 1.0f;
+      const int32_t rx0 = tx0, ry0 = ty0, rx1 = tx1, ry1 = ty1;  // reference rect
       // clamp in float before converting (the reference rect bounds the result)
       tx0 = max(tx0, (int32_t)fmax_(fx0, (float)tx0));
       ty0 = max(ty0, (int32_t)fmax_(fy0, (float)ty0));
       tx1 = min(tx1, (int32_t)fmin_(fx1, (float)tx1));
       ty1 = min(ty1, (int32_t)fmin_(fy1, (float)ty1));
+      const uint32_t ref_tiles = ntiles;
       ntiles = (tx1 > tx0 && ty1 > ty0) ? (uint32_t)((tx1 - tx0) * (ty1 - ty0)) : 0u;
       if (ntiles == 0) tx1 = tx0, ty1 = ty0;
+      if (cover && ntiles != ref_tiles) {
+        for (int32_t ty = ry0; ty < ry1; ++ty)
+          for (int32_t tx = rx0; tx < rx1; ++tx)
+            if (!(ty >= ty0 && ty < ty1 && tx >= tx0 && tx < tx1)) cover[c.tile_base + ty * c.ntx + tx] = 1;
+      }
     }
     if (tz != tz) atomicOr(status, ST_NAN_DEPTH);
     sort_keys[e] = f2u(tz);
@@ -169,10 +177,11 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ P, in
 }
 
 void launch_project(const float* P, int cap, int n, const CamDev* cams, int n_views, SplatArrays S,
-                    uint32_t* sort_keys, uint32_t* sort_vals, uint32_t* status, int tight, hipStream_t st) {
+                    uint32_t* sort_keys, uint32_t* sort_vals, uint32_t* status, int tight, uint8_t* cover,
+                    hipStream_t st) {
   if (n <= 0 || n_views <= 0) return;
   hipLaunchKernelGGL(k_project, dim3((n + 255) / 256), dim3(256), 0, st, P, cap, n, cams, n_views, S, sort_keys,
-                     sort_vals, status, tight);
+                     sort_vals, status, tight, cover);
 }
 
 __global__ void k_eval_expf(const float* __restrict__ x, float* __restrict__ out, size_t n) {

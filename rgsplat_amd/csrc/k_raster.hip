@@ -157,7 +157,7 @@ __global__ __launch_bounds__(FT) void k_raster_fwd(RasterIn in, float* __restric
     if (!(px < c.w && py < c.h)) continue;
     const size_t pix = (size_t)c.pix_base + (size_t)py * c.w + px;
     if (MODE == 0) {
-      if (start == end) {
+      if (start == end && !(in.cover && in.cover[tile])) {
         out_rgb[pix * 3 + 0] = 0.0f;
         out_rgb[pix * 3 + 1] = 0.0f;
         out_rgb[pix * 3 + 2] = 0.0f;

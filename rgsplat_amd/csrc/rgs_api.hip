@@ -231,6 +231,7 @@ struct rgs_ctx {
   DBuf api_acc9;           // raster accumulators of the reference-API paths
   DBuf tile_order;         // raster schedule: slot -> global tile id
   DBuf train_scr;          // per-Gaussian scratch between geometry and Adam
+  DBuf cover;              // per tile: composited although its tight list is empty
   std::vector<int32_t> h_order;
   std::string order_key;
   int super_tile = 8;      // tiles per side of a schedule super-tile
@@ -344,15 +345,18 @@ SplatArrays splats(rgs_ctx* c) {
 
 // Projection, depth sort and the pair-count scan (sizes fixed by n and vs).
 // tight: training-step binning (only the tiles a splat's alpha can reach).
-void enqueue_project(rgs_ctx* c, const float* P, size_t cap, int n, const ViewSet& vs, bool tight) {
+// cover: null, or the per-tile flags of launch_project (cleared here).
+void enqueue_project(rgs_ctx* c, const float* P, size_t cap, int n, const ViewSet& vs, bool tight,
+                     uint8_t* cover = nullptr) {
   hipStream_t st = c->st;
   uint32_t* sc = P_<uint32_t>(c->scalars);
   int nv = (int)vs.cams.size();
   size_t dn = (size_t)nv * n;
   launch_fill_u32(sc + SC_DEPTH_N, 1, (uint32_t)dn, st);
+  if (cover) launch_fill_u32(reinterpret_cast<uint32_t*>(cover), ((size_t)vs.total_tiles + 3) / 4, 0u, st);
   stage_begin(c, S_PROJECT);
   launch_project(P, (int)cap, n, P_<CamDev>(c->cams_d), nv, splats(c), P_<uint32_t>(c->dk[0]),
-                 P_<uint32_t>(c->dv[0]), sc + SC_STATUS, tight ? 1 : 0, st);
+                 P_<uint32_t>(c->dv[0]), sc + SC_STATUS, tight ? 1 : 0, cover, st);
   stage_end(c, S_PROJECT);
   uint32_t* keys[2] = {P_<uint32_t>(c->dk[0]), P_<uint32_t>(c->dk[1])};
   uint32_t* vals[2] = {P_<uint32_t>(c->dv[0]), P_<uint32_t>(c->dv[1])};
@@ -388,7 +392,7 @@ RasterIn raster_in(rgs_ctx* c, const ViewSet& vs, const float bg[3]) {
 
 // Pair generation, tile sort, ranges and the forward raster.
 void enqueue_bin_and_raster(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, const float bg[3], int mode,
-                            float* rgb_out) {
+                            float* rgb_out, const uint8_t* cover = nullptr) {
   hipStream_t st = c->st;
   uint32_t* sc = P_<uint32_t>(c->scalars);
   size_t dn = vs.cams.size() * (size_t)n;
@@ -406,6 +410,7 @@ void enqueue_bin_and_raster(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, co
   stage_end(c, S_TSORT);
   c->pair_vals = vals[cur];
   RasterIn in = raster_in(c, vs, bg);
+  in.cover = cover;
   stage_begin(c, S_FWD);
   launch_raster_fwd(in, mode, rgb_out, P_<float>(c->final_t), P_<int32_t>(c->last), st);
   stage_end(c, S_FWD);
@@ -544,7 +549,7 @@ void rgs_destroy(rgs_ctx* c) {
                  &c->api_target, &c->api_m1, &c->api_m2, &c->api_acc, &c->api_den, &c->api_P2, &c->dens_want,
                  &c->dens_cnt, &c->dens_W, &c->dens_packed, &c->dens_prefix, &c->dens_b32, &c->dens_b64,
                  &c->dens_t32, &c->dens_t64, &c->first_splat, &c->P, &c->P2, &c->M1, &c->M2, &c->gacc, &c->gden, &c->targets,
-                 &c->adam_d, &c->api_acc9, &c->tile_order, &c->train_scr};
+                 &c->adam_d, &c->api_acc9, &c->tile_order, &c->train_scr, &c->cover};
   for (DBuf* b : all)
     if (b->p) (void)hipFree(b->p);
   if (c->h_scalars) (void)hipHostFree(c->h_scalars);
@@ -934,11 +939,17 @@ int rgs_set_views(rgs_ctx* c, const rgs_camera* cams, const float* const* target
 }
 
 // Enqueues the step after the projection/depth-sort/scan prefix.
+// Tiles the training step's tight binning may leave empty although the
+// reference composites them: only a non-zero background tells them apart.
+static uint8_t* step_cover(rgs_ctx* c, const float bg[3]) {
+  return (bg[0] != 0.0f || bg[1] != 0.0f || bg[2] != 0.0f) ? P_<uint8_t>(c->cover) : nullptr;
+}
+
 static void enqueue_step_rest(rgs_ctx* c, const ViewSet& vs, const float bg[3], float lambda) {
   int n = (int)c->n;
   size_t cap = c->cap;
   uint32_t* sc = P_<uint32_t>(c->scalars);
-  enqueue_bin_and_raster(c, cap, n, vs, bg, 0, P_<float>(c->rgb));
+  enqueue_bin_and_raster(c, cap, n, vs, bg, 0, P_<float>(c->rgb), step_cover(c, bg));
   stage_begin(c, S_LOSS);
   enqueue_loss(c, vs, P_<float>(c->rgb), P_<float>(c->targets), lambda);
   stage_end(c, S_LOSS);
@@ -957,7 +968,7 @@ static void enqueue_step_rest(rgs_ctx* c, const ViewSet& vs, const float bg[3], 
 // Enqueues one whole training step on the context's stream (no host sync).
 static void enqueue_step(rgs_ctx* c, const ViewSet& vs, const float bg[3], float lambda) {
   launch_fill_u32(P_<uint32_t>(c->scalars), SC_COUNT, 0u, c->st);
-  enqueue_project(c, P_<float>(c->P), c->cap, (int)c->n, vs, true);
+  enqueue_project(c, P_<float>(c->P), c->cap, (int)c->n, vs, true, step_cover(c, bg));
   enqueue_step_rest(c, vs, bg, lambda);
 }
 
@@ -989,6 +1000,7 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
   bool fresh_accum = c->accum.bytes < acc_n * 4;
   CHK(ensure(c, c->accum, acc_n * 4));
   CHK(ensure(c, c->train_scr, (size_t)TRAIN_SCRATCH_ROWS * cap * 4));
+  CHK(ensure(c, c->cover, ((size_t)vs.total_tiles + 3) / 4 * 4));
   if (fresh_accum) launch_fill_f32(P_<float>(c->accum), acc_n, 0.0f, c->st);
   CHK(ensure_loss(c, vs));
   std::memcpy(c->h_cams, vs.cams.data(), vs.cams.size() * sizeof(CamDev));
@@ -1007,7 +1019,7 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
       HIPCHK(hipMemcpyAsync(c->adam_d.p, c->h_adam, sizeof(AdamConsts), hipMemcpyHostToDevice, c->st));
       uint32_t* sc = P_<uint32_t>(c->scalars);
       launch_fill_u32(sc, SC_COUNT, 0u, c->st);
-      enqueue_project(c, P_<float>(c->P), cap, n, vs, true);
+      enqueue_project(c, P_<float>(c->P), cap, n, vs, true, step_cover(c, bg));
       HIPCHK(hipMemcpyAsync(c->h_scalars, sc, SC_COUNT * 4, hipMemcpyDeviceToHost, c->st));
       HIPCHK(hipStreamSynchronize(c->st));
       if (c->h_scalars[SC_STATUS] & ST_NAN_DEPTH) {

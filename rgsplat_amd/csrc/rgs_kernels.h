@@ -28,9 +28,14 @@ enum : uint32_t { ST_NAN_DEPTH = 1u, ST_PAIR_OVERFLOW = 2u };
 // culled; value = v * cap + i) at dense index v * n + i. tight != 0: each
 // splat's tile rect is the reference's 3-sigma rect cut down to the tiles its
 // alpha-reach ellipse (alpha >= 1/255) can reach; the dropped (tile, splat)
-// pairs change no image or gradient, only the list indices.
+// pairs change no pixel's blend, only the list indices. A tile can lose all
+// its pairs, though, and the reference composites a tile (rgb = C + T bg)
+// iff its own list is non-empty: with cover non-null, every tile of a
+// splat's reference rect that the cut rect drops gets cover[tile] = 1, so
+// that "reference list non-empty" = (own list non-empty) || cover[tile].
 void launch_project(const float* P, int cap, int n, const CamDev* cams, int n_views, SplatArrays S,
-                    uint32_t* sort_keys, uint32_t* sort_vals, uint32_t* status, int tight, hipStream_t st);
+                    uint32_t* sort_keys, uint32_t* sort_vals, uint32_t* status, int tight, uint8_t* cover,
+                    hipStream_t st);
 
 // Device expf over n values (diagnostic).
 void launch_eval_expf(const float* x, float* out, size_t n, hipStream_t st);
@@ -114,6 +119,7 @@ struct RasterIn {
   size_t acc_stride;  // backward accumulators: value k of entry vi at accum[k * acc_stride + vi]
   int total_tiles;
   float bg[3];
+  const uint8_t* cover;  // null, or tiles composited although their list is empty (launch_project)
 };
 // mode 0: training forward (rgb = C + T bg; final T; last contributor);
 // mode 1: rasterizer.rs RGBA (rgba = bg + C, A = 1 - T; empty tiles left alone).

@@ -304,6 +304,36 @@ def test_train_step_single_view(ctx, use_graph):
     c.close()
 
 
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_train_step_background_with_emptied_tiles(ctx, use_graph):
+    """A background colour on a scene whose tight binning empties tiles that
+    the reference composites (rgb = bg there, 0 for tiles it never bins):
+    loss and gradients of the step against the oracle."""
+    from rgsplat_amd import runtime
+    from tests.test_reach_cull import faint_sparse_scene
+    g, cam = faint_sparse_scene()
+    tg = [with_alpha(oracle.render_color(S.make_scene(len(g), 32), cam))]
+    bg = (0.3, 0.1, 0.7)
+    cfg = S.benchmark_train_config()
+    c = runtime.Context(0)
+    c.scene_upload(g)
+    c.set_views([cam], tg)
+    loss = c.train_step([0], cfg, bg=bg, use_graph=use_graph)
+    ref_loss, ref_g = _oracle_step_grads(g, [cam], tg, bg, 0.2)
+    # The loss is a sum of 36,864 similar terms (flat background regions),
+    # which the reference adds serially in f32 (4.7e-5 relative off here);
+    # the device sums per tile and then in f64. Against the f64 loss of the
+    # oracle's image the device loss is exact to float rounding.
+    rgb_ref = oracle.Render(g, cam, None, bg=bg, backward=False).image()[0]
+    loss64, _ = oracle.loss_grad(rgb_ref, tg[0], cam.width, cam.height, 0.2, f64=True)
+    assert abs(loss[0] - loss64) <= 1e-6 * loss64
+    assert abs(loss[0] - ref_loss[0]) <= 1e-4 * ref_loss[0]
+    got = c.adam_download().m1[: len(g)] / np.float32(0.1)
+    assert_grads_close(np.concatenate([got, np.zeros((len(g), 1))], 1)[:, :60],
+                       np.concatenate([ref_g[:, :59], np.zeros((len(g), 1))], 1), 1e-3)
+    c.close()
+
+
 def test_graph_replay_matches_eager(ctx):
     from rgsplat_amd import runtime
     g, cams, targets = workload("c0_10k_256", 5000, [(128, 96), (160, 128)])

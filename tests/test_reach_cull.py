@@ -118,3 +118,36 @@ def test_culling_keeps_every_contributing_pair(name, n, size):
     # the tight rects drop a good share of the reference's pairs
     area = lambda r: np.maximum(r[:, 2] - r[:, 0], 0) * np.maximum(r[:, 3] - r[:, 1], 0)
     assert area(tight).sum() < 0.95 * area(sp["rect"].astype(np.int64)).sum()
+
+
+def faint_sparse_scene():
+    """A few faint splats (alpha reach well inside 3 sigma) in a small view:
+    many tiles of their reference rects see no pixel reach alpha 1/255."""
+    n = 60
+    g = S.make_scene(n, 31, log_scale_mean=-2.5)
+    g[:, 3] = S.Stream(5).normal((n,), mean=-3.0, std=0.5)
+    cam = S.make_cameras(3005, [(128, 96)], focal=150.0)[0]
+    return g, cam
+
+
+def test_tight_binning_can_empty_a_tile():
+    """Precondition of the GPU test with a background colour: some tile has a
+    non-empty reference list while every splat's tight rect drops it (the
+    reference composites it to the background, rgb = bg)."""
+    g, cam = faint_sparse_scene()
+    r = oracle.Render(g, cam, None, backward=False)
+    sp = r.splats()
+    off, gis = r.tiles()
+    tight = tight_rect(sp, reach_bound(sp))
+    row = {int(gi): k for k, gi in enumerate(sp["gi"])}
+    ntx = (cam.width + 15) // 16
+    emptied = 0
+    for t in range(len(off) - 1):
+        lst = gis[off[t]:off[t + 1]]
+        if len(lst) == 0:
+            continue
+        ty, tx = divmod(t, ntx)
+        ks = [row[int(x)] for x in lst]
+        if not any(tight[k][0] <= tx < tight[k][2] and tight[k][1] <= ty < tight[k][3] for k in ks):
+            emptied += 1
+    assert emptied > 0
