@@ -158,6 +158,20 @@ int rgs_scene_reset_opacities(rgs_ctx* ctx);
  * pointer may be NULL). */
 int rgs_step_stats(rgs_ctx* ctx, uint64_t* n_pairs, uint64_t* fwd_evals, uint64_t* bwd_evals);
 
+/* ---- Depth-cut binning of rgs_train_step (no reference counterpart; images,
+ * losses and gradients do not depend on it). A tile's list only matters up to
+ * the splat at which its last pixel saturates (T < 1e-4), so the step bins
+ * and sorts the pairs of the splats in front of a cut depth first, and only
+ * the tiles those lists leave unsaturated get the pairs behind the cut.
+ * mode 0 (default): the cut is the previous step's saturation depth + 5%
+ * (none on the first step); 1: no cut (whole lists); 2: cut at the fixed
+ * camera-space depth `depth` (> 0). rgs_cut_stats reports the last step's
+ * front and back pair counts, how many tiles needed their back lists, and
+ * the cut depth the next step will use (+inf: none); pointers may be NULL. */
+int rgs_set_depth_cut(rgs_ctx* ctx, int mode, float depth);
+int rgs_cut_stats(rgs_ctx* ctx, uint64_t* front_pairs, uint64_t* back_pairs, uint64_t* flagged_tiles,
+                  float* next_cut);
+
 /* ---- Raster work decomposition (no reference counterpart; images do not
  * depend on it, gradients only through the order of float summation).
  * super_tile: side, in 16x16 tiles, of the square patches the raster
@@ -180,11 +194,13 @@ int rgs_autotune_raster(rgs_ctx* ctx, const int* view_ids, int n_ids, const floa
  * timed as they run. Stages, in order: 0 project, 1 depth sort,
  * 2 pair-count scan, 3 pair generation, 4 tile sort (its last pass also
  * emits the tile ranges), 5 forward raster, 6 L1+D-SSIM, 7 backward raster,
- * 8 geometry + Adam.
+ * 8 geometry + Adam, 9 back lists of the depth cut (count, scan, generation,
+ * sort, resumed forward). Under the depth cut, stages 3-5 cover the front
+ * pairs (and 3 the cut itself).
  * rgs_stage_times returns the mean milliseconds per stage since profiling
  * was enabled, the number of profiled steps and how many of them were graph
  * replays. */
-#define RGS_N_STAGES 9
+#define RGS_N_STAGES 10
 int rgs_set_profiling(rgs_ctx* ctx, int enable);
 int rgs_stage_times(rgs_ctx* ctx, float* mean_ms, int n_stages, uint64_t* steps, uint64_t* graph_steps);
 

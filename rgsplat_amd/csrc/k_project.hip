@@ -168,6 +168,7 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ P, in
     }
     if (tz != tz) atomicOr(status, ST_NAN_DEPTH);
     sort_keys[e] = f2u(tz);
+    S.depth[vi] = tz;
     S.xy_ab[vi] = make_float4(sx, sy, cc * inv_det, -b * inv_det);
     S.c_op_rg[vi] = make_float4(a * inv_det, opacity, col[0], col[1]);
     S.col_b[vi] = col[2];

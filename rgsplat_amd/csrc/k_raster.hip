@@ -59,7 +59,10 @@ __device__ __forceinline__ uint32_t wave_mask(float sx, float sy, float ca, floa
 
 // Forward: one wave per tile, four pixels per lane (pixel group q = tile rows
 // 4q..4q+3, lane l at column l & 15, row 4q + (l >> 4)); batches of 64 splats
-// staged in LDS; the wave stops when all 256 pixels have saturated.
+// staged in LDS; the wave stops when all 256 pixels have saturated. MODE 2
+// resumes a tile flagged by MODE 0 on its back list: the state (C, T, last)
+// comes from the output buffers and list indices continue after the front
+// list.
 constexpr int FT = 64;
 template <int MODE>
 __global__ __launch_bounds__(FT) void k_raster_fwd(RasterIn in, float* __restrict__ out_rgb,
@@ -69,8 +72,9 @@ __global__ __launch_bounds__(FT) void k_raster_fwd(RasterIn in, float* __restric
   __shared__ float4 s_b[FT];
   __shared__ float s_c[FT];
   const int lane = threadIdx.x;
-  if (lane < 32) s_tab[lane] = kExp2Tab[lane];
   const int tile = in.tile_order[xcd_remap(blockIdx.x, gridDim.x)];
+  if (MODE == 2 && !in.flag[tile]) return;
+  if (lane < 32) s_tab[lane] = kExp2Tab[lane];
   const int v = view_of_tile(in.cams, in.n_views, tile);
   const CamDev& c = in.cams[v];
   const int lt = tile - c.tile_base;
@@ -79,6 +83,19 @@ __global__ __launch_bounds__(FT) void k_raster_fwd(RasterIn in, float* __restric
   const float fpx = (float)px + 0.5f;
   uint32_t start = in.rstart[tile], end = in.rend[tile];
   if (end < start) start = end;  // no pair reached this tile
+  // the list walked here: the tile's list, or for MODE 2 its back list,
+  // whose entries follow the front list's (li0) in the last indices
+  const uint32_t* vals = in.pair_vals;
+  int32_t li0 = 0;
+  uint32_t total_len = end - start;
+  if (MODE == 2) {
+    li0 = (int32_t)(end - start);
+    start = in.bstart[tile];
+    end = in.bend[tile];
+    if (end < start) start = end;
+    vals = in.back_vals;
+    total_len += end - start;
+  }
   const float bg0 = in.bg[0], bg1 = in.bg[1], bg2 = in.bg[2];
   float T[4], C0[4], C1[4], C2[4], fpy[4];
   int32_t last[4];
@@ -87,20 +104,30 @@ __global__ __launch_bounds__(FT) void k_raster_fwd(RasterIn in, float* __restric
   for (int q = 0; q < 4; ++q) {
     const int py = ty * TILE + 4 * q + (lane >> 4);
     fpy[q] = (float)py + 0.5f;
+    const bool inside = px < c.w && py < c.h;
     T[q] = 1.0f;
     C0[q] = MODE == 1 ? bg0 : 0.0f;
     C1[q] = MODE == 1 ? bg1 : 0.0f;
     C2[q] = MODE == 1 ? bg2 : 0.0f;
     last[q] = -1;
-    done[q] = !(px < c.w && py < c.h);
+    if (MODE == 2 && inside) {
+      const size_t pix = (size_t)c.pix_base + (size_t)py * c.w + px;
+      T[q] = out_t[pix];
+      C0[q] = out_rgb[pix * 3 + 0];
+      C1[q] = out_rgb[pix * 3 + 1];
+      C2[q] = out_rgb[pix * 3 + 2];
+      last[q] = out_last[pix];
+    }
+    done[q] = !inside || T[q] < TRANSMITTANCE_STOP;
   }
+  int32_t fin = -1;  // list index at which the last pixel saturated
   for (uint32_t b0 = start; b0 < end; b0 += FT) {
     if (__ballot(!(done[0] && done[1] && done[2] && done[3])) == 0) break;
     __syncthreads();  // previous batch fully read (one wave: s_barrier only)
     const uint32_t idx = b0 + lane;
     uint32_t my_m = 0;  // group mask of this lane's staged splat
     if (idx < end) {
-      uint32_t vi = in.pair_vals[idx];
+      uint32_t vi = vals[idx];
       float4 A = in.S.xy_ab[vi];
       float4 B = in.S.c_op_rg[vi];
       s_a[lane] = A;
@@ -129,7 +156,7 @@ __global__ __launch_bounds__(FT) void k_raster_fwd(RasterIn in, float* __restric
         Bn = s_b[jn];
         cbn = s_c[jn];
       }
-      const int32_t li = (int32_t)(b0 - start) + j;
+      const int32_t li = li0 + (int32_t)(b0 - start) + j;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         if (!(m >> q & 1u)) continue;  // wave-uniform
@@ -148,16 +175,39 @@ __global__ __launch_bounds__(FT) void k_raster_fwd(RasterIn in, float* __restric
         last[q] = li;
         if (T[q] < TRANSMITTANCE_STOP) done[q] = true;
       }
-      if (__ballot(!(done[0] && done[1] && done[2] && done[3])) == 0) break;
+      if (__ballot(!(done[0] && done[1] && done[2] && done[3])) == 0) {
+        fin = li;
+        break;
+      }
     }
   }
+  const bool all_done = __ballot(!(done[0] && done[1] && done[2] && done[3])) == 0;
+  // MODE 0 under a depth cut: a list that ended with pixels left is a front
+  // list; the tile keeps its raw state and waits for its back list
+  const bool partial = MODE == 0 && in.cut_on && *in.cut_on && !all_done;
+  if (MODE != 1 && in.zneed && fin >= 0) {
+    // the depth of the splat that saturated the tile, for the next step's cut
+    const uint32_t vi = vals[start + (uint32_t)(fin - li0)];
+    if (lane == 0) atomicMax(in.zneed, f2u(in.S.depth[vi]));
+  }
+  if (partial && lane == 0) {
+    in.flag[tile] = 1;
+    atomicAdd(in.n_flagged, 1u);
+  }
+  // a tile without any pair keeps rgb 0 (the reference composites only the
+  // tiles its binning reached; cover marks those the tight rects emptied)
+  const bool empty = total_len == 0 && !(in.cover && in.cover[tile]);
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int py = ty * TILE + 4 * q + (lane >> 4);
     if (!(px < c.w && py < c.h)) continue;
     const size_t pix = (size_t)c.pix_base + (size_t)py * c.w + px;
-    if (MODE == 0) {
-      if (start == end && !(in.cover && in.cover[tile])) {
+    if (MODE != 1) {
+      if (partial) {
+        out_rgb[pix * 3 + 0] = C0[q];
+        out_rgb[pix * 3 + 1] = C1[q];
+        out_rgb[pix * 3 + 2] = C2[q];
+      } else if (empty) {
         out_rgb[pix * 3 + 0] = 0.0f;
         out_rgb[pix * 3 + 1] = 0.0f;
         out_rgb[pix * 3 + 2] = 0.0f;
@@ -182,8 +232,10 @@ void launch_raster_fwd(const RasterIn& in, int mode, float* rgb, float* final_t,
   if (in.total_tiles <= 0) return;
   if (mode == 0)
     hipLaunchKernelGGL(k_raster_fwd<0>, dim3(in.total_tiles), dim3(FT), 0, st, in, rgb, final_t, last);
-  else
+  else if (mode == 1)
     hipLaunchKernelGGL(k_raster_fwd<1>, dim3(in.total_tiles), dim3(FT), 0, st, in, rgb, final_t, last);
+  else
+    hipLaunchKernelGGL(k_raster_fwd<2>, dim3(in.total_tiles), dim3(FT), 0, st, in, rgb, final_t, last);
 }
 
 template <int CTRL>
@@ -270,6 +322,14 @@ __global__ __launch_bounds__(BT, 5) void k_raster_bwd(RasterIn in, const float* 
   const int tx = lt % c.ntx, ty = lt / c.ntx;
   uint32_t start = in.rstart[tile], end = in.rend[tile];
   if (end < start) start = end;  // no pair reached this tile
+  // list index p < nf: pair_vals[start + p]; else back_vals[bs + p - nf]
+  const uint32_t nf = end - start;
+  uint32_t bs = 0, nbk = 0;
+  if (in.bstart) {
+    bs = in.bstart[tile];
+    const uint32_t be = in.bend[tile];
+    nbk = be > bs ? be - bs : 0u;
+  }
   const int px = tx * TILE + (lane & 15);
   const float fpx = (float)px + 0.5f;
   const float bg0 = in.bg[0], bg1 = in.bg[1], bg2 = in.bg[2];
@@ -286,7 +346,7 @@ __global__ __launch_bounds__(BT, 5) void k_raster_bwd(RasterIn in, const float* 
     lc[q] = -1;
     T[q] = 1.0f;
     d0[q] = d1[q] = d2[q] = 0.0f;
-    if (px < c.w && py < c.h && start != end) {
+    if (px < c.w && py < c.h && nf + nbk != 0) {
       const size_t pix = (size_t)c.pix_base + (size_t)py * c.w + px;
       lc[q] = last_contrib[pix];
       T[q] = final_t[pix];
@@ -314,7 +374,8 @@ __global__ __launch_bounds__(BT, 5) void k_raster_bwd(RasterIn in, const float* 
     __syncthreads();  // previous batch fully read (one wave: s_barrier only)
     uint32_t my_m = 0;  // group mask of this lane's staged splat
     if (lane < nb) {
-      uint32_t vi = in.pair_vals[start + lo + lane];
+      const uint32_t p = (uint32_t)(lo + lane);
+      uint32_t vi = p < nf ? in.pair_vals[start + p] : in.back_vals[bs + (p - nf)];
       s_vi[lane] = vi;
       float4 A = in.S.xy_ab[vi];
       float4 B = in.S.c_op_rg[vi];
@@ -465,7 +526,11 @@ __global__ __launch_bounds__(RT) void k_work_counts(RasterIn in, const float* __
   if (px < c.w && py < c.h) {
     const size_t pix = (size_t)c.pix_base + (size_t)py * c.w + px;
     const uint32_t s0 = in.rstart[tile], e0 = in.rend[tile];
-    const uint32_t len = e0 > s0 ? e0 - s0 : 0u;
+    uint32_t len = e0 > s0 ? e0 - s0 : 0u;
+    if (in.bstart) {
+      const uint32_t s1 = in.bstart[tile], e1 = in.bend[tile];
+      len += e1 > s1 ? e1 - s1 : 0u;
+    }
     const int32_t lc = last[pix];
     f = final_t[pix] < TRANSMITTANCE_STOP ? (unsigned long long)(lc + 1) : (unsigned long long)len;
     b = (unsigned long long)(lc + 1);

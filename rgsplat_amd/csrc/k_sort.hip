@@ -302,6 +302,10 @@ int radix_sort_tiles(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev
                      RadixScratch rs, TileRanges tr, hipStream_t st) {
   return radix_sort_impl(keys, vals, n_dev, n_max, bits, rs, st, true, &tr);
 }
+int radix_sort_tiles_full(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev, size_t n_max, int bits,
+                          RadixScratch rs, TileRanges tr, hipStream_t st) {
+  return radix_sort_impl(keys, vals, n_dev, n_max, bits, rs, st, false, &tr);
+}
 
 // ------------------------------------------------------------------- scans
 constexpr int SCAN_THREADS = 256;
@@ -532,6 +536,126 @@ void launch_gen_pairs(const uint32_t* sorted_vals, size_t n_sorted, const uint32
                      S.cnt, S.rect, cap, cams, pair_keys, pair_vals, k_cap, k_out, status, first_splat, rs.hist,
                      rs.nb_max);
 }
+
+// ------------------------------------------------------- depth-cut binning
+// One block: S = lower bound of the cut key in the sorted keys by a 256-ary
+// search (three rounds for 4M keys), then the front pair count.
+__global__ __launch_bounds__(256) void k_cut_begin(const uint32_t* __restrict__ keys, size_t n,
+                                                   const uint32_t* __restrict__ offsets,
+                                                   const uint32_t* __restrict__ total, const CutState* __restrict__ cs,
+                                                   uint32_t* __restrict__ out) {
+  const uint32_t key = cs->key;
+  size_t lo = 0, hi = n;  // keys[lo - 1] < key <= keys[hi]: S in [lo, hi]
+  while (lo < hi) {
+    const size_t step = (hi - lo + 255) / 256;
+    const size_t p = lo + (size_t)threadIdx.x * step;
+    const int c = __syncthreads_count(p < hi && keys[p] < key);
+    if (c == 0) break;  // keys[lo] >= key: S = lo
+    const size_t nlo = lo + (size_t)(c - 1) * step + 1;
+    const size_t nhi = lo + (size_t)c * step;
+    lo = nlo;
+    hi = nhi < hi ? nhi : hi;
+  }
+  if (threadIdx.x == 0) {
+    const uint32_t tot = *total;
+    const uint32_t kf = lo < n ? offsets[lo] : tot;
+    out[0] = (uint32_t)lo;
+    out[1] = kf;
+    out[2] = kf < tot ? 1u : 0u;
+  }
+}
+
+void launch_cut_begin(const uint32_t* sorted_keys, size_t n, const uint32_t* offsets, const uint32_t* total,
+                      const CutState* cs, uint32_t* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_cut_begin, dim3(1), dim3(256), 0, st, sorted_keys, n, offsets, total, cs, out);
+}
+
+__global__ __launch_bounds__(256) void k_back_count(const uint32_t* __restrict__ sorted_vals, size_t n,
+                                                    const uint32_t* __restrict__ cut_out,
+                                                    const uint32_t* __restrict__ n_flagged,
+                                                    const uint32_t* __restrict__ cnt, const uint2* __restrict__ rect,
+                                                    int cap, const CamDev* __restrict__ cams,
+                                                    const uint8_t* __restrict__ flag, uint32_t* __restrict__ count) {
+  const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  uint32_t k = 0;
+  if (*n_flagged != 0 && cut_out[2] != 0 && r >= cut_out[0]) {
+    const uint32_t vi = sorted_vals[r];
+    if (cnt[vi] != 0) {
+      const uint2 rr = rect[vi];
+      const CamDev& cm = cams[vi / (uint32_t)cap];
+      const uint32_t x0 = rr.x & 0xffffu, y0 = rr.x >> 16, x1 = rr.y & 0xffffu, y1 = rr.y >> 16;
+      for (uint32_t ty = y0; ty < y1; ++ty) {
+        const uint8_t* row = flag + cm.tile_base + (size_t)ty * cm.ntx;
+        for (uint32_t tx = x0; tx < x1; ++tx) k += row[tx];
+      }
+    }
+  }
+  count[r] = k;
+}
+
+void launch_back_count(const uint32_t* sorted_vals, size_t n, const uint32_t* cut_out, const uint32_t* n_flagged,
+                       const SplatArrays S, int cap, const CamDev* cams, const uint8_t* flag, uint32_t* count,
+                       hipStream_t st) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_back_count, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, sorted_vals, n, cut_out,
+                     n_flagged, S.cnt, S.rect, cap, cams, flag, count);
+}
+
+// One thread per sorted entry with back pairs: its flagged tiles in row-major
+// order at its offset.
+__global__ __launch_bounds__(256) void k_back_gen(const uint32_t* __restrict__ sorted_vals, size_t n,
+                                                  const uint32_t* __restrict__ offsets,
+                                                  const uint32_t* __restrict__ total,
+                                                  const uint32_t* __restrict__ cut_out,
+                                                  const uint32_t* __restrict__ n_flagged,
+                                                  const uint2* __restrict__ rect, int cap,
+                                                  const CamDev* __restrict__ cams, const uint8_t* __restrict__ flag,
+                                                  uint32_t* __restrict__ pair_keys, uint32_t* __restrict__ pair_vals,
+                                                  size_t k_cap, uint32_t* __restrict__ k_out,
+                                                  uint32_t* __restrict__ status) {
+  const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t tot = *total;
+  if (r == 0) {
+    *k_out = (size_t)tot < k_cap ? tot : (uint32_t)k_cap;
+    if ((size_t)tot > k_cap) atomicOr(status, ST_PAIR_OVERFLOW);
+  }
+  if (r >= n || *n_flagged == 0 || cut_out[2] == 0 || r < cut_out[0]) return;
+  size_t o = offsets[r];
+  const size_t o_end = r + 1 < n ? offsets[r + 1] : tot;
+  if (o == o_end) return;
+  const uint32_t vi = sorted_vals[r];
+  const uint2 rr = rect[vi];
+  const CamDev& cm = cams[vi / (uint32_t)cap];
+  const uint32_t x0 = rr.x & 0xffffu, y0 = rr.x >> 16, x1 = rr.y & 0xffffu, y1 = rr.y >> 16;
+  for (uint32_t ty = y0; ty < y1; ++ty) {
+    const uint32_t rowbase = (uint32_t)cm.tile_base + ty * (uint32_t)cm.ntx;
+    for (uint32_t tx = x0; tx < x1; ++tx) {
+      if (!flag[rowbase + tx]) continue;
+      if (o < k_cap) {
+        pair_keys[o] = rowbase + tx;
+        pair_vals[o] = vi;
+      }
+      ++o;
+    }
+  }
+}
+
+void launch_back_gen(const uint32_t* sorted_vals, size_t n, const uint32_t* offsets, const uint32_t* total,
+                     const uint32_t* cut_out, const uint32_t* n_flagged, const SplatArrays S, int cap,
+                     const CamDev* cams, const uint8_t* flag, uint32_t* pair_keys, uint32_t* pair_vals, size_t k_cap,
+                     uint32_t* k_out, uint32_t* status, hipStream_t st) {
+  hipLaunchKernelGGL(k_back_gen, dim3((unsigned)(((n ? n : 1) + 255) / 256)), dim3(256), 0, st,
+                     sorted_vals, n, offsets, total, cut_out, n_flagged, S.rect, cap, cams, flag, pair_keys, pair_vals,
+                     k_cap, k_out, status);
+}
+
+__global__ void k_cut_end(CutState* cs) {
+  if (cs->mode == 0) cs->key = cs->zneed ? f2u(__uint_as_float(cs->zneed) * 1.05f) : 0xffffffffu;
+  cs->zneed = 0;
+}
+
+void launch_cut_end(CutState* cs, hipStream_t st) { hipLaunchKernelGGL(k_cut_end, dim3(1), dim3(1), 0, st, cs); }
 
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

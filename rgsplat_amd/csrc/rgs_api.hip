@@ -173,7 +173,12 @@ struct ViewSet {
   }
 };
 
-enum { SC_DEPTH_N = 0, SC_TOTAL = 1, SC_K = 2, SC_STATUS = 3, SC_NOUT = 4, SC_COUNT = 8 };
+// Device scalars of a render or step (cleared at its start). SC_CUT: three
+// words from launch_cut_begin (cut position, front pairs, cut active).
+enum {
+  SC_DEPTH_N = 0, SC_TOTAL = 1, SC_K = 2, SC_STATUS = 3, SC_NOUT = 4,
+  SC_CUT = 5, SC_BTOTAL = 8, SC_KB = 9, SC_NFLAG = 10, SC_COUNT = 16
+};
 
 }  // namespace
 
@@ -183,7 +188,7 @@ struct rgs_ctx {
   hipStream_t st = nullptr;
 
   // render workspace (shared by every entry point)
-  DBuf cams_d, xy_ab, c_op_rg, col_b, cnt, rect, reach;
+  DBuf cams_d, xy_ab, c_op_rg, col_b, cnt, rect, reach, depth;
   DBuf dk[2], dv[2], offsets, bsums, scalars;
   DBuf pk[2], pv[2], hist, rowtot;
   DBuf rstart, rend, rgb, final_t, last, dl, maps, partial, loss;
@@ -193,7 +198,15 @@ struct rgs_ctx {
   size_t depth_cap = 0;  // capacity (entries) of the depth-sort arrays
   int hist_nb = 0;
   const uint32_t* sorted_vals = nullptr;
+  const uint32_t* sorted_keys = nullptr;
   const uint32_t* pair_vals = nullptr;
+  // depth-cut binning of the training step
+  DBuf flag, rbstart, rbend, pv2, cntb, cutst;
+  const uint32_t* back_vals = nullptr;
+  int cut_mode = 0;              // CutState::mode
+  float cut_depth = 0.0f;        // mode 2: fixed cut depth
+  uint64_t last_front_pairs = 0, last_back_pairs = 0, last_flagged = 0;
+  bool last_cut = false;         // the last render was a training step (two-part lists)
   uint32_t* h_scalars = nullptr;  // pinned
   float* h_loss = nullptr;        // pinned, 64 entries
 
@@ -250,7 +263,7 @@ namespace {
 
 uint64_t* ctx_gen(rgs_ctx* c) { return &c->gen; }
 
-enum Stage { S_PROJECT, S_DSORT, S_SCAN, S_GEN, S_TSORT, S_FWD, S_LOSS, S_BWD, S_ADAM };
+enum Stage { S_PROJECT, S_DSORT, S_SCAN, S_GEN, S_TSORT, S_FWD, S_LOSS, S_BWD, S_ADAM, S_BACK };
 // Stage events. Under stream capture they become event-record nodes of the
 // graph (hipEventRecordExternal), so replays are timed stage by stage too.
 inline void record(rgs_ctx* c, hipEvent_t e) {
@@ -312,6 +325,7 @@ int ensure_render(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, size_t k_cap
   CHK(ensure(c, c->cnt, vc * sizeof(uint32_t)));
   CHK(ensure(c, c->rect, vc * sizeof(uint2)));
   CHK(ensure(c, c->reach, vc * sizeof(float)));
+  CHK(ensure(c, c->depth, vc * sizeof(float)));
   for (int i = 0; i < 2; ++i) {
     CHK(ensure(c, c->dk[i], dn * 4));
     CHK(ensure(c, c->dv[i], dn * 4));
@@ -329,6 +343,11 @@ int ensure_render(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, size_t k_cap
   CHK(ensure(c, c->rowtot, 256 * 4));
   CHK(ensure(c, c->rstart, (size_t)vs.total_tiles * 4));
   CHK(ensure(c, c->rend, (size_t)vs.total_tiles * 4));
+  CHK(ensure(c, c->rbstart, (size_t)vs.total_tiles * 4));
+  CHK(ensure(c, c->rbend, (size_t)vs.total_tiles * 4));
+  CHK(ensure(c, c->flag, ((size_t)vs.total_tiles + 3) / 4 * 4));
+  CHK(ensure(c, c->pv2, k_cap * 4));
+  CHK(ensure(c, c->cntb, dn * 4));
   CHK(ensure(c, c->first_splat, ((size_t)radix_nb(k_cap) + 1) * 4));
   CHK(ensure(c, c->rgb, vs.total_pix * 4 * 4));
   CHK(ensure(c, c->final_t, vs.total_pix * 4));
@@ -340,7 +359,7 @@ int ensure_render(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, size_t k_cap
 
 SplatArrays splats(rgs_ctx* c) {
   return SplatArrays{P_<float4>(c->xy_ab), P_<float4>(c->c_op_rg), P_<float>(c->col_b), P_<uint32_t>(c->cnt),
-                     P_<uint2>(c->rect), P_<float>(c->reach)};
+                     P_<uint2>(c->rect), P_<float>(c->reach), P_<float>(c->depth)};
 }
 
 // Projection, depth sort and the pair-count scan (sizes fixed by n and vs).
@@ -365,6 +384,7 @@ void enqueue_project(rgs_ctx* c, const float* P, size_t cap, int n, const ViewSe
   int cur = radix_sort_pairs(keys, vals, sc + SC_DEPTH_N, dn, 32, rs, st);
   stage_end(c, S_DSORT);
   c->sorted_vals = vals[cur];
+  c->sorted_keys = keys[cur];
   stage_begin(c, S_SCAN);
   scan_gathered_counts(vals[cur], P_<uint32_t>(c->cnt), dn, P_<uint32_t>(c->offsets), P_<uint32_t>(c->bsums),
                        sc + SC_TOTAL, st);
@@ -417,6 +437,73 @@ void enqueue_bin_and_raster(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, co
 }
 
 
+// (Re)writes the persistent cut state from the host settings.
+int write_cut_state(rgs_ctx* c) {
+  CutState cs{0xffffffffu, 0u, (uint32_t)c->cut_mode, 0u};
+  if (c->cut_mode == 2) cs.key = f2u(c->cut_depth);
+  HIPCHK(hipMemcpyAsync(c->cutst.p, &cs, sizeof(cs), hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+// The training step's binning and forward raster with the depth cut: front
+// pairs, their sort and forward (flagging the tiles the front lists leave
+// unsaturated), then the back pairs of the flagged tiles, their sort and the
+// resumed forward, and the next step's cut. Every launch runs whatever the
+// counts turn out to be (no host sync): with nothing flagged the back stage
+// reduces to a count pass, a scan and empty launches.
+void enqueue_bin_and_raster_cut(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, const float bg[3], float* rgb_out,
+                                const uint8_t* cover) {
+  hipStream_t st = c->st;
+  uint32_t* sc = P_<uint32_t>(c->scalars);
+  size_t dn = vs.cams.size() * (size_t)n;
+  RadixScratch rs{P_<uint32_t>(c->hist), P_<uint32_t>(c->rowtot), c->hist_nb};
+  CutState* cs = P_<CutState>(c->cutst);
+  stage_begin(c, S_GEN);
+  launch_cut_begin(c->sorted_keys, dn, P_<uint32_t>(c->offsets), sc + SC_TOTAL, cs, sc + SC_CUT, st);
+  launch_gen_pairs(c->sorted_vals, dn, P_<uint32_t>(c->offsets), sc + SC_CUT + 1, splats(c), (int)cap,
+                   P_<CamDev>(c->cams_d), P_<uint32_t>(c->pk[0]), P_<uint32_t>(c->pv[0]), c->k_cap, sc + SC_K,
+                   sc + SC_STATUS, P_<uint32_t>(c->first_splat), rs, st);
+  stage_end(c, S_GEN);
+  uint32_t* keys[2] = {P_<uint32_t>(c->pk[0]), P_<uint32_t>(c->pk[1])};
+  uint32_t* vals[2] = {P_<uint32_t>(c->pv[0]), P_<uint32_t>(c->pv[1])};
+  stage_begin(c, S_TSORT);
+  TileRanges tr{P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend), (uint32_t)vs.total_tiles};
+  int cur = radix_sort_tiles(keys, vals, sc + SC_K, c->k_cap, vs.bits(), rs, tr, st);
+  stage_end(c, S_TSORT);
+  c->pair_vals = vals[cur];
+  RasterIn in = raster_in(c, vs, bg);
+  in.cover = cover;
+  in.cut_on = sc + SC_CUT + 2;
+  in.flag = P_<uint8_t>(c->flag);
+  in.n_flagged = sc + SC_NFLAG;
+  in.zneed = &cs->zneed;
+  launch_fill_u32(P_<uint32_t>(c->flag), ((size_t)vs.total_tiles + 3) / 4, 0u, st);
+  stage_begin(c, S_FWD);
+  launch_raster_fwd(in, 0, rgb_out, P_<float>(c->final_t), P_<int32_t>(c->last), st);
+  stage_end(c, S_FWD);
+  // back lists of the flagged tiles: keys ping-pong in the front sort's key
+  // buffers, values in its spare value buffer and pv2
+  stage_begin(c, S_BACK);
+  uint32_t* cntb = P_<uint32_t>(c->cntb);
+  launch_back_count(c->sorted_vals, dn, sc + SC_CUT, sc + SC_NFLAG, splats(c), (int)cap, P_<CamDev>(c->cams_d),
+                    P_<uint8_t>(c->flag), cntb, st);
+  scan_u32(cntb, dn, cntb, P_<uint32_t>(c->bsums), sc + SC_BTOTAL, st);
+  uint32_t* bvals[2] = {vals[cur ^ 1], P_<uint32_t>(c->pv2)};
+  launch_back_gen(c->sorted_vals, dn, cntb, sc + SC_BTOTAL, sc + SC_CUT, sc + SC_NFLAG, splats(c), (int)cap,
+                  P_<CamDev>(c->cams_d), P_<uint8_t>(c->flag), keys[0], bvals[0], c->k_cap, sc + SC_KB,
+                  sc + SC_STATUS, st);
+  TileRanges trb{P_<uint32_t>(c->rbstart), P_<uint32_t>(c->rbend), (uint32_t)vs.total_tiles};
+  int curb = radix_sort_tiles_full(keys, bvals, sc + SC_KB, c->k_cap, vs.bits(), rs, trb, st);
+  c->back_vals = bvals[curb];
+  in.bstart = P_<uint32_t>(c->rbstart);
+  in.bend = P_<uint32_t>(c->rbend);
+  in.back_vals = c->back_vals;
+  launch_raster_fwd(in, 2, rgb_out, P_<float>(c->final_t), P_<int32_t>(c->last), st);
+  launch_cut_end(cs, st);
+  stage_end(c, S_BACK);
+}
+
 // Eager render of n Gaussians (SoA P, row capacity cap) into vs: sizes the
 // pair buffers from the exact count.
 int render_eager(rgs_ctx* c, const float* P, size_t cap, int n, ViewSet& vs, const float bg[3], int mode,
@@ -437,6 +524,7 @@ int render_eager(rgs_ctx* c, const float* P, size_t cap, int n, ViewSet& vs, con
   }
   enqueue_bin_and_raster(c, cap, n, vs, bg, mode, rgb_out ? rgb_out : P_<float>(c->rgb));
   c->last_pairs = K;
+  c->last_cut = false;
   c->last_cap = cap;
   c->last_tiles = vs.total_tiles;
   c->last_nviews = (int)vs.cams.size();
@@ -549,7 +637,8 @@ void rgs_destroy(rgs_ctx* c) {
                  &c->api_target, &c->api_m1, &c->api_m2, &c->api_acc, &c->api_den, &c->api_P2, &c->dens_want,
                  &c->dens_cnt, &c->dens_W, &c->dens_packed, &c->dens_prefix, &c->dens_b32, &c->dens_b64,
                  &c->dens_t32, &c->dens_t64, &c->first_splat, &c->P, &c->P2, &c->M1, &c->M2, &c->gacc, &c->gden, &c->targets,
-                 &c->adam_d, &c->api_acc9, &c->tile_order, &c->train_scr, &c->cover};
+                 &c->adam_d, &c->api_acc9, &c->tile_order, &c->train_scr, &c->cover, &c->depth,
+                 &c->flag, &c->rbstart, &c->rbend, &c->pv2, &c->cntb, &c->cutst};
   for (DBuf* b : all)
     if (b->p) (void)hipFree(b->p);
   if (c->h_scalars) (void)hipHostFree(c->h_scalars);
@@ -949,13 +1038,16 @@ static void enqueue_step_rest(rgs_ctx* c, const ViewSet& vs, const float bg[3], 
   int n = (int)c->n;
   size_t cap = c->cap;
   uint32_t* sc = P_<uint32_t>(c->scalars);
-  enqueue_bin_and_raster(c, cap, n, vs, bg, 0, P_<float>(c->rgb), step_cover(c, bg));
+  enqueue_bin_and_raster_cut(c, cap, n, vs, bg, P_<float>(c->rgb), step_cover(c, bg));
   stage_begin(c, S_LOSS);
   enqueue_loss(c, vs, P_<float>(c->rgb), P_<float>(c->targets), lambda);
   stage_end(c, S_LOSS);
   stage_begin(c, S_BWD);
   RasterIn in = raster_in(c, vs, bg);
   in.acc_stride = vs.cams.size() * cap;
+  in.bstart = P_<uint32_t>(c->rbstart);
+  in.bend = P_<uint32_t>(c->rbend);
+  in.back_vals = c->back_vals;
   launch_raster_bwd(in, P_<float>(c->final_t), P_<int32_t>(c->last), P_<float>(c->dl), P_<float>(c->accum), c->st);
   stage_end(c, S_BWD);
   stage_begin(c, S_ADAM);
@@ -1001,6 +1093,10 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
   CHK(ensure(c, c->accum, acc_n * 4));
   CHK(ensure(c, c->train_scr, (size_t)TRAIN_SCRATCH_ROWS * cap * 4));
   CHK(ensure(c, c->cover, ((size_t)vs.total_tiles + 3) / 4 * 4));
+  if (!c->cutst.p) {
+    CHK(ensure(c, c->cutst, sizeof(CutState)));
+    CHK(write_cut_state(c));
+  }
   if (fresh_accum) launch_fill_f32(P_<float>(c->accum), acc_n, 0.0f, c->st);
   CHK(ensure_loss(c, vs));
   std::memcpy(c->h_cams, vs.cams.data(), vs.cams.size() * sizeof(CamDev));
@@ -1078,7 +1174,11 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
       launch_fill_f32(P_<float>(c->accum), acc_n, 0.0f, c->st);
       continue;
     }
-    c->last_pairs = c->h_scalars[SC_K];
+    c->last_front_pairs = c->h_scalars[SC_K];
+    c->last_back_pairs = c->h_scalars[SC_KB];
+    c->last_flagged = c->h_scalars[SC_NFLAG];
+    c->last_pairs = c->last_front_pairs + c->last_back_pairs;
+    c->last_cut = true;
     c->last_cap = cap;
     c->last_tiles = vs.total_tiles;
     c->last_nviews = (int)vs.cams.size();
@@ -1265,12 +1365,49 @@ int rgs_step_stats(rgs_ctx* c, uint64_t* n_pairs, uint64_t* fwd_evals, uint64_t*
     launch_fill_u32(P_<uint32_t>(c->api_den), 4, 0u, c->st);
     const float zero3[3] = {0.0f, 0.0f, 0.0f};
     RasterIn in = raster_in_n(c, c->last_nviews, c->last_tiles, zero3);
+    if (c->last_cut) {
+      in.bstart = P_<uint32_t>(c->rbstart);
+      in.bend = P_<uint32_t>(c->rbend);
+    }
     launch_work_counts(in, P_<float>(c->final_t), P_<int32_t>(c->last), d, c->st);
     unsigned long long h[2] = {0, 0};
     HIPCHK(hipMemcpyAsync(h, d, 16, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     if (fwd_evals) *fwd_evals = h[0];
     if (bwd_evals) *bwd_evals = h[1];
+  }
+  return 0;
+}
+
+int rgs_set_depth_cut(rgs_ctx* c, int mode, float depth) {
+  if (!c) return fail(RGS_E_ARG, "null ctx");
+  if (mode < 0 || mode > 2) return fail(RGS_E_ARG, "depth cut mode must be 0, 1 or 2");
+  if (mode == 2 && !(depth > 0.0f)) return fail(RGS_E_ARG, "fixed depth cut needs a positive depth");
+  (void)hipSetDevice(c->device);
+  c->cut_mode = mode;
+  c->cut_depth = depth;
+  if (c->cutst.p) CHK(write_cut_state(c));
+  return 0;
+}
+
+int rgs_cut_stats(rgs_ctx* c, uint64_t* front_pairs, uint64_t* back_pairs, uint64_t* flagged_tiles,
+                  float* next_cut) {
+  if (!c) return fail(RGS_E_ARG, "null ctx");
+  if (front_pairs) *front_pairs = c->last_front_pairs;
+  if (back_pairs) *back_pairs = c->last_back_pairs;
+  if (flagged_tiles) *flagged_tiles = c->last_flagged;
+  if (next_cut) {
+    *next_cut = __builtin_inff();
+    if (c->cutst.p) {
+      (void)hipSetDevice(c->device);
+      CutState cs{};
+      HIPCHK(hipMemcpyAsync(&cs, c->cutst.p, sizeof(cs), hipMemcpyDeviceToHost, c->st));
+      HIPCHK(hipStreamSynchronize(c->st));
+      if (cs.key != 0xffffffffu) {
+        uint32_t k = cs.key;
+        std::memcpy(next_cut, &k, 4);
+      }
+    }
   }
   return 0;
 }

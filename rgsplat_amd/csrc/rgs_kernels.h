@@ -17,6 +17,7 @@ struct SplatArrays {
   uint32_t* cnt;    // tiles touched (0 when culled)
   uint2* rect;      // (tx0 | ty0 << 16, tx1 | ty1 << 16)
   float* reach;     // R: alpha >= 1/255 needs conic form <= R (+inf: no bound)
+  float* depth;     // camera-space depth (visible splats)
 };
 
 // Status bits written by kernels into a device word.
@@ -106,6 +107,44 @@ void launch_gen_pairs(const uint32_t* sorted_vals, size_t n_sorted, const uint32
 int radix_sort_tiles(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev, size_t n_max, int bits,
                      RadixScratch rs, TileRanges tr, hipStream_t st);
 
+// Depth-cut binning of the training step. The pairs of the splats in front
+// of a cut depth are generated, sorted and rasterised first; a tile whose
+// front list ends before all its pixels saturate is flagged, gets the pairs
+// of the splats behind the cut (in depth order, after its front list) and
+// the raster resumes it. A tile's front and back lists concatenated are its
+// whole list, so images, last-contributor indices and gradients are those of
+// whole lists; the cut only decides how many pairs are sorted. The cut of a
+// step is the depth at which the previous step's tiles saturated (the
+// deepest such splat, plus 5%).
+struct CutState {       // persistent device words
+  uint32_t key;         // f2u(depth) of the next cut; 0xffffffff: no cut
+  uint32_t zneed;       // max f2u(depth) of a splat at which a tile saturated
+  uint32_t mode;        // 0 adaptive, 1 no cut, 2 fixed key
+  uint32_t pad;
+};
+// out[0] = S, the number of sorted depth keys below the cut; out[1] = the
+// front pair count (offsets[S], *total when S = n); out[2] = 1 when a
+// front list may be partial (front count < *total).
+void launch_cut_begin(const uint32_t* sorted_keys, size_t n, const uint32_t* offsets, const uint32_t* total,
+                      const CutState* cs, uint32_t* out, hipStream_t st);
+// Back-pair count of every sorted entry r: for r >= S the tiles of the
+// splat's rect whose flag is set, else 0 (all 0 when no tile is flagged).
+void launch_back_count(const uint32_t* sorted_vals, size_t n, const uint32_t* cut_out, const uint32_t* n_flagged,
+                       const SplatArrays S, int cap, const CamDev* cams, const uint8_t* flag, uint32_t* count,
+                       hipStream_t st);
+// Writes the back pairs at the exclusive offsets of those counts (total in
+// *total); K = min(total, k_cap) goes to *k_out, ST_PAIR_OVERFLOW when
+// total > k_cap.
+void launch_back_gen(const uint32_t* sorted_vals, size_t n, const uint32_t* offsets, const uint32_t* total,
+                     const uint32_t* cut_out, const uint32_t* n_flagged, const SplatArrays S, int cap,
+                     const CamDev* cams, const uint8_t* flag, uint32_t* pair_keys, uint32_t* pair_vals, size_t k_cap,
+                     uint32_t* k_out, uint32_t* status, hipStream_t st);
+// Next cut from this step's saturation depths (mode 0); clears zneed.
+void launch_cut_end(CutState* cs, hipStream_t st);
+// Tile sort whose first-pass histogram is not yet computed.
+int radix_sort_tiles_full(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev, size_t n_max, int bits,
+                          RadixScratch rs, TileRanges tr, hipStream_t st);
+
 // ---- rasterization (k_raster.hip)
 struct RasterIn {
   const uint32_t* rstart;
@@ -120,9 +159,20 @@ struct RasterIn {
   int total_tiles;
   float bg[3];
   const uint8_t* cover;  // null, or tiles composited although their list is empty (launch_project)
+  // depth-cut binning (null when every list is whole)
+  const uint32_t* cut_on;     // device word: 1 when a front list may be partial
+  uint8_t* flag;              // per tile: front list ended before all pixels saturated
+  uint32_t* n_flagged;        // number of flagged tiles
+  uint32_t* zneed;            // CutState::zneed
+  const uint32_t* bstart;     // back lists: ranges and values
+  const uint32_t* bend;
+  const uint32_t* back_vals;
 };
-// mode 0: training forward (rgb = C + T bg; final T; last contributor);
-// mode 1: rasterizer.rs RGBA (rgba = bg + C, A = 1 - T; empty tiles left alone).
+// mode 0: training forward (rgb = C + T bg; final T; last contributor); with
+//   cut_on set, a tile whose (front) list ends before all its pixels
+//   saturate is flagged and keeps its raw state (rgb = C, T, last);
+// mode 1: rasterizer.rs RGBA (rgba = bg + C, A = 1 - T; empty tiles left alone);
+// mode 2: resumes the flagged tiles of mode 0 on their back lists.
 void launch_raster_fwd(const RasterIn& in, int mode, float* rgb, float* final_t, int32_t* last, hipStream_t st);
 // Accumulates the 9 per-splat screen-space gradients (dL/dcolour[3],
 // dL/dopacity, dL/dconic[3], dL/dsx, dL/dsy) into accum[k * acc_stride + vi]:

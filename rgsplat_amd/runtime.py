@@ -78,6 +78,9 @@ SIGNATURES = {
     "rgs_scene_reset_opacities": [_vp],
     "rgs_step_stats": [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                        ctypes.POINTER(ctypes.c_uint64)],
+    "rgs_set_depth_cut": [_vp, _i, _f],
+    "rgs_cut_stats": [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                      ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_float)],
     "rgs_set_raster_params": [_vp, _i, _i],
     "rgs_get_raster_params": [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
     "rgs_autotune_raster": [_vp, _vp, _i, _vp, ctypes.c_float, ctypes.POINTER(ctypes.c_int),
@@ -312,6 +315,19 @@ class Context:
         _check(lib().rgs_step_stats(self.h, ctypes.byref(k), ctypes.byref(f), ctypes.byref(b)))
         return k.value, f.value, b.value
 
+    def set_depth_cut(self, mode: int, depth: float = 0.0):
+        """Depth-cut binning of train_step: 0 adaptive (default), 1 off
+        (whole lists), 2 fixed camera-space depth."""
+        _check(lib().rgs_set_depth_cut(self.h, int(mode), float(depth)))
+
+    def cut_stats(self) -> dict:
+        """Last step's front / back pair counts, tiles that needed their back
+        lists, and the next step's cut depth (inf: none)."""
+        a, b, t = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        d = ctypes.c_float()
+        _check(lib().rgs_cut_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(t), ctypes.byref(d)))
+        return {"front_pairs": a.value, "back_pairs": b.value, "flagged_tiles": t.value, "next_cut": d.value}
+
     def set_raster_params(self, super_tile: int, sparse_lanes: int):
         _check(lib().rgs_set_raster_params(self.h, int(super_tile), int(sparse_lanes)))
 
@@ -332,7 +348,7 @@ class Context:
 
     # ---------------------------------------------------------- stage timing
     STAGES = ["project", "depth_sort", "pair_scan", "pair_gen", "tile_sort", "raster_fwd", "loss_ssim",
-              "raster_bwd", "geometry_adam"]
+              "raster_bwd", "geometry_adam", "cut_back"]
 
     def set_profiling(self, enable: bool):
         _check(lib().rgs_set_profiling(self.h, int(bool(enable))))
