@@ -57,21 +57,43 @@ __device__ __forceinline__ uint32_t wave_mask(float sx, float sy, float ca, floa
   return m;
 }
 
+// alpha = min(op exp(power), 0.99) from the hardware exp2 (relative error
+// < 1e-6 where alpha can reach either threshold: power in [-5.6, 0]); within
+// 1e-5 of a threshold the exact expf decides, so the alpha tests and the
+// clamp agree with the reference's. The training forward and the backward
+// both use it, with the same operations, so they agree with each other.
+__device__ __forceinline__ float alpha_fast(float power, float op, const uint64_t* s_tab, float& g_exp,
+                                            float& alpha_raw) {
+  g_exp = __builtin_amdgcn_exp2f(power * 1.44269504088896341f);
+  alpha_raw = op * g_exp;
+  if (__builtin_fabsf(alpha_raw - ALPHA_THRESHOLD) <= 1e-5f * ALPHA_THRESHOLD ||
+      __builtin_fabsf(alpha_raw - MAX_ALPHA) <= 1e-5f * MAX_ALPHA) {
+    g_exp = expf_tab(power, s_tab);
+    alpha_raw = op * g_exp;
+  }
+  return fmin_(alpha_raw, MAX_ALPHA);
+}
+
 // Forward: one wave per tile, four pixels per lane (pixel group q = tile rows
 // 4q..4q+3, lane l at column l & 15, row 4q + (l >> 4)); batches of 64 splats
-// staged in LDS; the wave stops when all 256 pixels have saturated. MODE 2
+// staged in LDS; the wave stops when all 256 pixels have saturated. Modes 0
+// and 1 evaluate alpha with the exact expf (images bit-identical to the
+// reference's); the training step's modes 3 (as 0) and 2 use alpha_fast
+// (images equal to float rounding, alpha decisions exact). MODE 2
 // resumes a tile flagged by MODE 0 on its back list: the state (C, T, last)
 // comes from the output buffers and list indices continue after the front
 // list.
 constexpr int FT = 64;
 template <int MODE>
-__global__ __launch_bounds__(FT) void k_raster_fwd(RasterIn in, float* __restrict__ out_rgb,
+__global__ __launch_bounds__(FT) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_raster_fwd(RasterIn in, float* __restrict__ out_rgb,
                                                    float* __restrict__ out_t, int32_t* __restrict__ out_last) {
   __shared__ uint64_t s_tab[32];
   __shared__ float4 s_a[FT];
   __shared__ float4 s_b[FT];
   __shared__ float s_c[FT];
   const int lane = threadIdx.x;
+  constexpr bool FAST = MODE >= 2;
+  constexpr bool RGBA = MODE == 1;
   const int tile = in.tile_order[xcd_remap(blockIdx.x, gridDim.x)];
   if (MODE == 2 && !in.flag[tile]) return;
   if (lane < 32) s_tab[lane] = kExp2Tab[lane];
@@ -106,9 +128,9 @@ __global__ __launch_bounds__(FT) void k_raster_fwd(RasterIn in, float* __restric
     fpy[q] = (float)py + 0.5f;
     const bool inside = px < c.w && py < c.h;
     T[q] = 1.0f;
-    C0[q] = MODE == 1 ? bg0 : 0.0f;
-    C1[q] = MODE == 1 ? bg1 : 0.0f;
-    C2[q] = MODE == 1 ? bg2 : 0.0f;
+    C0[q] = RGBA ? bg0 : 0.0f;
+    C1[q] = RGBA ? bg1 : 0.0f;
+    C2[q] = RGBA ? bg2 : 0.0f;
     last[q] = -1;
     if (MODE == 2 && inside) {
       const size_t pix = (size_t)c.pix_base + (size_t)py * c.w + px;
@@ -165,7 +187,13 @@ __global__ __launch_bounds__(FT) void k_raster_fwd(RasterIn in, float* __restric
         float dy = fpy[q] - A.y;
         float power = -0.5f * (A.z * dx * dx + 2.0f * A.w * dx * dy + B.x * dy * dy);
         if (power > 0.0f) continue;
-        float alpha = fmin_(B.y * expf_tab(power, s_tab), MAX_ALPHA);
+        float alpha;
+        if (FAST) {
+          float g_exp, alpha_raw;
+          alpha = alpha_fast(power, B.y, s_tab, g_exp, alpha_raw);
+        } else {
+          alpha = fmin_(B.y * expf_tab(power, s_tab), MAX_ALPHA);
+        }
         if (alpha < ALPHA_THRESHOLD) continue;
         float weight = alpha * T[q];
         C0[q] += B.z * weight;
@@ -184,8 +212,8 @@ __global__ __launch_bounds__(FT) void k_raster_fwd(RasterIn in, float* __restric
   const bool all_done = __ballot(!(done[0] && done[1] && done[2] && done[3])) == 0;
   // MODE 0 under a depth cut: a list that ended with pixels left is a front
   // list; the tile keeps its raw state and waits for its back list
-  const bool partial = MODE == 0 && in.cut_on && *in.cut_on && !all_done;
-  if (MODE != 1 && in.zneed && fin >= 0) {
+  const bool partial = (MODE == 0 || MODE == 3) && in.cut_on && *in.cut_on && !all_done;
+  if (!RGBA && in.zneed && fin >= 0) {
     // the depth of the splat that saturated the tile, for the next step's cut
     const uint32_t vi = vals[start + (uint32_t)(fin - li0)];
     if (lane == 0) atomicMax(in.zneed, f2u(in.S.depth[vi]));
@@ -202,7 +230,7 @@ __global__ __launch_bounds__(FT) void k_raster_fwd(RasterIn in, float* __restric
     const int py = ty * TILE + 4 * q + (lane >> 4);
     if (!(px < c.w && py < c.h)) continue;
     const size_t pix = (size_t)c.pix_base + (size_t)py * c.w + px;
-    if (MODE != 1) {
+    if (!RGBA) {
       if (partial) {
         out_rgb[pix * 3 + 0] = C0[q];
         out_rgb[pix * 3 + 1] = C1[q];
@@ -234,8 +262,10 @@ void launch_raster_fwd(const RasterIn& in, int mode, float* rgb, float* final_t,
     hipLaunchKernelGGL(k_raster_fwd<0>, dim3(in.total_tiles), dim3(FT), 0, st, in, rgb, final_t, last);
   else if (mode == 1)
     hipLaunchKernelGGL(k_raster_fwd<1>, dim3(in.total_tiles), dim3(FT), 0, st, in, rgb, final_t, last);
-  else
+  else if (mode == 2)
     hipLaunchKernelGGL(k_raster_fwd<2>, dim3(in.total_tiles), dim3(FT), 0, st, in, rgb, final_t, last);
+  else
+    hipLaunchKernelGGL(k_raster_fwd<3>, dim3(in.total_tiles), dim3(FT), 0, st, in, rgb, final_t, last);
 }
 
 template <int CTRL>
@@ -417,17 +447,8 @@ __global__ __launch_bounds__(BT, 5) void k_raster_bwd(RasterIn in, const float* 
         float dy = fpy[q] - A.y;
         float power = -0.5f * (A.z * dx * dx + 2.0f * A.w * dx * dy + B.x * dy * dy);
         if (power > 0.0f) continue;  // NaN proceeds, as in the reference
-        // Hardware exp2 (relative error < 1e-6 where alpha can reach either
-        // threshold: power in [-5.6, 0]); within 1e-5 of a threshold the
-        // exact expf decides, so contribution and clamping match the forward.
-        float g_exp = __builtin_amdgcn_exp2f(power * 1.44269504088896341f);
-        float alpha_raw = B.y * g_exp;
-        if (__builtin_fabsf(alpha_raw - ALPHA_THRESHOLD) <= 1e-5f * ALPHA_THRESHOLD ||
-            __builtin_fabsf(alpha_raw - MAX_ALPHA) <= 1e-5f * MAX_ALPHA) {
-          g_exp = expf_tab(power, s_tab);
-          alpha_raw = B.y * g_exp;
-        }
-        float alpha = fmin_(alpha_raw, MAX_ALPHA);
+        float g_exp, alpha_raw;
+        const float alpha = alpha_fast(power, B.y, s_tab, g_exp, alpha_raw);
         if (!(alpha >= ALPHA_THRESHOLD)) continue;
         contrib = true;
         {

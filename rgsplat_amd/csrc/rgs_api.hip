@@ -480,7 +480,7 @@ void enqueue_bin_and_raster_cut(rgs_ctx* c, size_t cap, int n, const ViewSet& vs
   in.zneed = &cs->zneed;
   launch_fill_u32(P_<uint32_t>(c->flag), ((size_t)vs.total_tiles + 3) / 4, 0u, st);
   stage_begin(c, S_FWD);
-  launch_raster_fwd(in, 0, rgb_out, P_<float>(c->final_t), P_<int32_t>(c->last), st);
+  launch_raster_fwd(in, 3, rgb_out, P_<float>(c->final_t), P_<int32_t>(c->last), st);
   stage_end(c, S_FWD);
   // back lists of the flagged tiles: keys ping-pong in the front sort's key
   // buffers, values in its spare value buffer and pv2
@@ -1239,7 +1239,7 @@ int rgs_autotune_raster(rgs_ctx* c, const int* ids, int n_ids, const float bg[3]
     RasterIn in = raster_in(c, vs, bg);
     in.acc_stride = vs.cams.size() * cap;
     HIPCHK(hipEventRecord(ev[0], c->st));
-    launch_raster_fwd(in, 0, P_<float>(c->rgb), P_<float>(c->final_t), P_<int32_t>(c->last), c->st);
+    launch_raster_fwd(in, 3, P_<float>(c->rgb), P_<float>(c->final_t), P_<int32_t>(c->last), c->st);
     HIPCHK(hipEventRecord(ev[1], c->st));
     HIPCHK(hipEventRecord(ev[2], c->st));
     launch_raster_bwd(in, P_<float>(c->final_t), P_<int32_t>(c->last), P_<float>(c->dl), P_<float>(c->accum), c->st);

@@ -172,7 +172,9 @@ struct RasterIn {
 //   cut_on set, a tile whose (front) list ends before all its pixels
 //   saturate is flagged and keeps its raw state (rgb = C, T, last);
 // mode 1: rasterizer.rs RGBA (rgba = bg + C, A = 1 - T; empty tiles left alone);
-// mode 2: resumes the flagged tiles of mode 0 on their back lists.
+// mode 2: resumes the flagged tiles of mode 3 on their back lists;
+// mode 3: mode 0 of the training step, alpha from the hardware exp2 with the
+//   exact expf near the alpha thresholds (modes 0 and 1 use the exact expf).
 void launch_raster_fwd(const RasterIn& in, int mode, float* rgb, float* final_t, int32_t* last, hipStream_t st);
 // Accumulates the 9 per-splat screen-space gradients (dL/dcolour[3],
 // dL/dopacity, dL/dconic[3], dL/dsx, dL/dsy) into accum[k * acc_stride + vi]:
