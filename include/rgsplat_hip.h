@@ -171,6 +171,9 @@ int rgs_step_stats(rgs_ctx* ctx, uint64_t* n_pairs, uint64_t* fwd_evals, uint64_
 int rgs_set_depth_cut(rgs_ctx* ctx, int mode, float depth);
 int rgs_cut_stats(rgs_ctx* ctx, uint64_t* front_pairs, uint64_t* back_pairs, uint64_t* flagged_tiles,
                   float* next_cut);
+/* The images the last rgs_train_step rendered (its views in order, h x w x 3
+ * each; n_floats >= 3 x total pixels). */
+int rgs_step_image(rgs_ctx* ctx, float* rgb, size_t n_floats);
 
 /* ---- Raster work decomposition (no reference counterpart; images do not
  * depend on it, gradients only through the order of float summation).
@@ -184,6 +187,12 @@ int rgs_cut_stats(rgs_ctx* ctx, uint64_t* front_pairs, uint64_t* back_pairs, uin
  * (coordinate search, best of two runs each), keeps the fastest and reports
  * it; the scene and optimizer state are not changed. */
 int rgs_set_raster_params(rgs_ctx* ctx, int super_tile, int sparse_lanes);
+/* Binning of rgs_train_step: pairs per bin of 2^shift x 2^shift tiles
+ * (0: the reference's 16x16 tiles; 1, the default: 32x32, rasterised by one
+ * workgroup of four waves that share the staged splats and merge each
+ * splat's gradient before the accumulator atomics). API renders always use
+ * the reference's tiles. */
+int rgs_set_train_binning(rgs_ctx* ctx, int shift);
 int rgs_get_raster_params(rgs_ctx* ctx, int* super_tile, int* sparse_lanes);
 int rgs_autotune_raster(rgs_ctx* ctx, const int* view_ids, int n_ids, const float bg[3], float lambda_dssim,
                         int* super_tile, int* sparse_lanes, float* raster_ms);

@@ -122,6 +122,11 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ P, in
     tx1 = min(tx1, c.ntx);
     ty1 = min(ty1, c.nty);
     uint32_t ntiles = (tx1 > tx0 && ty1 > ty0) ? (uint32_t)((tx1 - tx0) * (ty1 - ty0)) : 0u;
+    if (cover && ntiles) {
+      // the tiles whose reference lists hold this splat
+      for (int32_t ty = ty0; ty < ty1; ++ty)
+        for (int32_t tx = tx0; tx < tx1; ++tx) cover[c.tile_base + ty * c.ntx + tx] = 1;
+    }
 
     // Reach of the alpha test: op * exp(-q/2) >= 1/255 needs the quadratic form
     // q <= R = 2 ln(255 op). The f32 conic and power carry relative errors that
@@ -151,20 +156,18 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ P, in
       const float fx1 = __builtin_floorf((sx + hx - 0.5f) * (1.0f / TILE)) + 1.0f;
       const float fy0 = __builtin_ceilf((sy - hy - 15.5f) * (1.0f / TILE));
       const float fy1 = __builtin_floorf((sy + hy - 0.5f) * (1.0f / TILE)) + 1.0f;
-      const int32_t rx0 = tx0, ry0 = ty0, rx1 = tx1, ry1 = ty1;  // reference rect
       // clamp in float before converting (the reference rect bounds the result)
       tx0 = max(tx0, (int32_t)fmax_(fx0, (float)tx0));
       ty0 = max(ty0, (int32_t)fmax_(fy0, (float)ty0));
       tx1 = min(tx1, (int32_t)fmin_(fx1, (float)tx1));
       ty1 = min(ty1, (int32_t)fmin_(fy1, (float)ty1));
-      const uint32_t ref_tiles = ntiles;
       ntiles = (tx1 > tx0 && ty1 > ty0) ? (uint32_t)((tx1 - tx0) * (ty1 - ty0)) : 0u;
       if (ntiles == 0) tx1 = tx0, ty1 = ty0;
-      if (cover && ntiles != ref_tiles) {
-        for (int32_t ty = ry0; ty < ry1; ++ty)
-          for (int32_t tx = rx0; tx < rx1; ++tx)
-            if (!(ty >= ty0 && ty < ty1 && tx >= tx0 && tx < tx1)) cover[c.tile_base + ty * c.ntx + tx] = 1;
-      }
+    }
+    if (c.bin_shift && ntiles) {
+      // pairs per bin of 2^shift x 2^shift tiles; the rect stays in tiles
+      const int sh = c.bin_shift;
+      ntiles = (uint32_t)(((((tx1 - 1) >> sh) + 1) - (tx0 >> sh)) * ((((ty1 - 1) >> sh) + 1) - (ty0 >> sh)));
     }
     if (tz != tz) atomicOr(status, ST_NAN_DEPTH);
     sort_keys[e] = f2u(tz);

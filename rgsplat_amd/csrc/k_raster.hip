@@ -2,19 +2,22 @@
 //
 // One wavefront per 16x16 tile (the reference's binning tile,
 // backward.rs:21): each lane owns four pixels of one column, one in each
-// 4-row pixel group. Tile lists are consumed in batches of 64 splats staged
-// through LDS (one splat per lane); a splat whose alpha-reach ellipse misses
-// a pixel group is skipped for that group with a wave-uniform branch. The wave
-// stops as soon as every pixel has saturated (T < 1e-4). The exp2 table of
-// the exact expf lives in LDS.
+// 4-row pixel group. A workgroup rasterises one bin of S x S tiles (S = 1:
+// the reference's tiles, API renders; S = 2 in the training step), wave w
+// the tile (w % S, w / S) of it. Bin lists are consumed in batches of 64
+// splats staged once in LDS for all the bin's waves; a splat whose
+// alpha-reach ellipse misses a pixel group of a wave's tile is skipped for
+// that group with a wave-uniform branch. The workgroup stops as soon as every
+// pixel of the bin has saturated (T < 1e-4). The exp2 table of the exact expf
+// lives in LDS.
 //
 // Forward follows backward.rs:296-385 (mode 0) and rasterizer.rs:72-192
 // (mode 1, RGBA). Backward follows backward.rs:391-512: each pixel walks its
 // list from last_contrib down to 0, recovering the transmittance in front of
 // each splat by division (no per-fragment storage); a lane first sums its
 // four pixels' gradients for the splat, the wave then reduces the nine sums
-// across lanes, and each tile adds them to the per-splat accumulators with
-// one FP32 atomic per value.
+// across lanes into LDS, and each bin adds the sum over its waves to the
+// per-splat accumulators with one FP32 atomic per value.
 #include "rgs_kernels.h"
 
 namespace rgs {
@@ -25,6 +28,42 @@ __device__ __forceinline__ int view_of_tile(const CamDev* cams, int n_views, int
   int v = 0;
   while (v + 1 < n_views && cams[v + 1].tile_base <= tile) ++v;
   return v;
+}
+__device__ __forceinline__ int view_of_bin(const CamDev* cams, int n_views, int bin) {
+  int v = 0;
+  while (v + 1 < n_views && cams[v + 1].bin_base <= bin) ++v;
+  return v;
+}
+
+// Whether tile (tx, ty) lies in a splat's tile rect. A bin's list holds the
+// splats whose rect meets the bin; a wave takes only those whose rect holds
+// its tile, so that it sees exactly the tile's own list (the reference
+// composites a splat only into the tiles of its rect, even where its alpha
+// reaches beyond).
+__device__ __forceinline__ bool in_rect(uint2 r, int tx, int ty) {
+  return (uint32_t)tx >= (r.x & 0xffffu) && (uint32_t)tx < (r.y & 0xffffu) && (uint32_t)ty >= (r.x >> 16) &&
+         (uint32_t)ty < (r.y >> 16);
+}
+
+// The tile that wave w of a bin's workgroup rasterises.
+struct BinTile {
+  int v;       // view
+  int tx, ty;  // tile coordinates in the view
+  int tile;    // global tile id (meaningful when ok)
+  bool ok;     // inside the view's tile grid (bins at the right and bottom
+               // edges may be partial)
+};
+template <int S>
+__device__ __forceinline__ BinTile bin_tile(const RasterIn& in, int bin, int w) {
+  BinTile b;
+  b.v = view_of_bin(in.cams, in.n_views, bin);
+  const CamDev& c = in.cams[b.v];
+  const int lb = bin - c.bin_base;
+  b.tx = (lb % c.nbx) * S + (w % S);
+  b.ty = (lb / c.nbx) * S + (w / S);
+  b.ok = b.tx < c.ntx && b.ty < c.nty;
+  b.tile = c.tile_base + b.ty * c.ntx + b.tx;
+  return b;
 }
 
 // Bit q of the result: the ellipse conic(d) <= R of the splat reaches the
@@ -74,46 +113,50 @@ __device__ __forceinline__ float alpha_fast(float power, float op, const uint64_
   return fmin_(alpha_raw, MAX_ALPHA);
 }
 
-// Forward: one wave per tile, four pixels per lane (pixel group q = tile rows
-// 4q..4q+3, lane l at column l & 15, row 4q + (l >> 4)); batches of 64 splats
-// staged in LDS; the wave stops when all 256 pixels have saturated. Modes 0
-// and 1 evaluate alpha with the exact expf (images bit-identical to the
+// Forward: four pixels per lane (pixel group q = tile rows 4q..4q+3, lane l
+// at column l & 15, row 4q + (l >> 4)); batches of 64 splats staged in LDS,
+// each of the S*S waves loading one of the four splat fields; the
+// workgroup stops when all pixels of the bin have saturated. Modes 0 and 1
+// evaluate alpha with the exact expf (images bit-identical to the
 // reference's); the training step's modes 3 (as 0) and 2 use alpha_fast
-// (images equal to float rounding, alpha decisions exact). MODE 2
-// resumes a tile flagged by MODE 0 on its back list: the state (C, T, last)
-// comes from the output buffers and list indices continue after the front
-// list.
+// (images equal to float rounding, alpha decisions exact). MODE 2 resumes a
+// bin flagged by MODE 3 on its back list: the state (C, T, last) comes from
+// the output buffers and list indices continue after the front list.
 constexpr int FT = 64;
-template <int MODE>
-__global__ __launch_bounds__(FT) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_raster_fwd(RasterIn in, float* __restrict__ out_rgb,
-                                                   float* __restrict__ out_t, int32_t* __restrict__ out_last) {
+template <int MODE, int S>
+__global__ __launch_bounds__(FT * S * S) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_raster_fwd(
+    RasterIn in, float* __restrict__ out_rgb, float* __restrict__ out_t, int32_t* __restrict__ out_last) {
+  constexpr int NW = S * S;
+  constexpr bool FAST = MODE >= 2;
+  constexpr bool RGBA = MODE == 1;
   __shared__ uint64_t s_tab[32];
   __shared__ float4 s_a[FT];
   __shared__ float4 s_b[FT];
   __shared__ float s_c[FT];
-  const int lane = threadIdx.x;
-  constexpr bool FAST = MODE >= 2;
-  constexpr bool RGBA = MODE == 1;
-  const int tile = in.tile_order[xcd_remap(blockIdx.x, gridDim.x)];
-  if (MODE == 2 && !in.flag[tile]) return;
-  if (lane < 32) s_tab[lane] = kExp2Tab[lane];
-  const int v = view_of_tile(in.cams, in.n_views, tile);
-  const CamDev& c = in.cams[v];
-  const int lt = tile - c.tile_base;
-  const int tx = lt % c.ntx, ty = lt / c.ntx;
+  __shared__ float s_r[FT];
+  __shared__ uint2 s_rect[FT];
+  __shared__ int32_t s_fin[NW];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // wave-uniform
+  const int bin = in.tile_order[xcd_remap(blockIdx.x, gridDim.x)];
+  if (MODE == 2 && !in.flag[bin]) return;  // block-uniform
+  if (threadIdx.x < 32) s_tab[threadIdx.x] = kExp2Tab[threadIdx.x];
+  const BinTile bt = bin_tile<S>(in, bin, w);
+  const CamDev& c = in.cams[bt.v];
+  const int tx = bt.tx, ty = bt.ty;
   const int px = tx * TILE + (lane & 15);
   const float fpx = (float)px + 0.5f;
-  uint32_t start = in.rstart[tile], end = in.rend[tile];
-  if (end < start) start = end;  // no pair reached this tile
-  // the list walked here: the tile's list, or for MODE 2 its back list,
-  // whose entries follow the front list's (li0) in the last indices
+  uint32_t start = in.rstart[bin], end = in.rend[bin];
+  if (end < start) start = end;  // no pair reached this bin
+  // the list walked here: the bin's list, or for MODE 2 its back list, whose
+  // entries follow the front list's (li0) in the last indices
   const uint32_t* vals = in.pair_vals;
   int32_t li0 = 0;
   uint32_t total_len = end - start;
   if (MODE == 2) {
     li0 = (int32_t)(end - start);
-    start = in.bstart[tile];
-    end = in.bend[tile];
+    start = in.bstart[bin];
+    end = in.bend[bin];
     if (end < start) start = end;
     vals = in.back_vals;
     total_len += end - start;
@@ -126,7 +169,7 @@ __global__ __launch_bounds__(FT) __attribute__((amdgpu_waves_per_eu(6, 8))) void
   for (int q = 0; q < 4; ++q) {
     const int py = ty * TILE + 4 * q + (lane >> 4);
     fpy[q] = (float)py + 0.5f;
-    const bool inside = px < c.w && py < c.h;
+    const bool inside = bt.ok && px < c.w && py < c.h;
     T[q] = 1.0f;
     C0[q] = RGBA ? bg0 : 0.0f;
     C1[q] = RGBA ? bg1 : 0.0f;
@@ -142,22 +185,32 @@ __global__ __launch_bounds__(FT) __attribute__((amdgpu_waves_per_eu(6, 8))) void
     }
     done[q] = !inside || T[q] < TRANSMITTANCE_STOP;
   }
-  int32_t fin = -1;  // list index at which the last pixel saturated
+  int32_t fin = -1;  // list index at which this wave's last pixel saturated
   for (uint32_t b0 = start; b0 < end; b0 += FT) {
-    if (__ballot(!(done[0] && done[1] && done[2] && done[3])) == 0) break;
-    __syncthreads();  // previous batch fully read (one wave: s_barrier only)
+    // stop when every pixel of the bin has saturated; the barrier also ends
+    // the previous batch's reads of the staged splats
+    if (!__syncthreads_or(!(done[0] && done[1] && done[2] && done[3]))) break;
     const uint32_t idx = b0 + lane;
-    uint32_t my_m = 0;  // group mask of this lane's staged splat
     if (idx < end) {
-      uint32_t vi = vals[idx];
-      float4 A = in.S.xy_ab[vi];
-      float4 B = in.S.c_op_rg[vi];
-      s_a[lane] = A;
-      s_b[lane] = B;
-      s_c[lane] = in.S.col_b[vi];
-      my_m = wave_mask(A.x, A.y, A.z, A.w, B.x, in.S.reach[vi], tx, ty);
+      const uint32_t vi = vals[idx];
+      if (NW == 1 || w == 0) s_a[lane] = in.S.xy_ab[vi];
+      if (NW == 1 || w == 1 % NW) s_b[lane] = in.S.c_op_rg[vi];
+      if (NW == 1 || w == 2 % NW) s_c[lane] = in.S.col_b[vi];
+      if (NW == 1 || w == 3 % NW) {
+        s_r[lane] = in.S.reach[vi];
+        if (S > 1) s_rect[lane] = in.S.rect[vi];
+      }
     }
     __syncthreads();
+    // all lanes take part in the ballot: the lanes past the list's end may be
+    // the ones with pixels left
+    const bool wave_live = __ballot(!(done[0] && done[1] && done[2] && done[3])) != 0;
+    uint32_t my_m = 0;  // group mask of this lane's staged splat for this wave's tile
+    if (idx < end && wave_live && (S == 1 || in_rect(s_rect[lane], tx, ty))) {
+      const float4 A = s_a[lane];
+      const float4 B = s_b[lane];
+      my_m = wave_mask(A.x, A.y, A.z, A.w, B.x, s_r[lane], tx, ty);
+    }
     // splats that reach some pixel group, visited front to back as set bits
     // of a scalar mask; the next one's LDS data is fetched while the current
     // one is processed
@@ -209,26 +262,31 @@ __global__ __launch_bounds__(FT) __attribute__((amdgpu_waves_per_eu(6, 8))) void
       }
     }
   }
-  const bool all_done = __ballot(!(done[0] && done[1] && done[2] && done[3])) == 0;
-  // MODE 0 under a depth cut: a list that ended with pixels left is a front
-  // list; the tile keeps its raw state and waits for its back list
+  if (lane == 0) s_fin[w] = fin;
+  const bool all_done = !__syncthreads_or(!(done[0] && done[1] && done[2] && done[3]));
+  // MODE 3 under a depth cut: a list that ended with pixels left is a front
+  // list; the bin keeps its raw state and waits for its back list
   const bool partial = (MODE == 0 || MODE == 3) && in.cut_on && *in.cut_on && !all_done;
-  if (!RGBA && in.zneed && fin >= 0) {
-    // the depth of the splat that saturated the tile, for the next step's cut
-    const uint32_t vi = vals[start + (uint32_t)(fin - li0)];
-    if (lane == 0) atomicMax(in.zneed, f2u(in.S.depth[vi]));
+  if (threadIdx.x == 0) {
+    if (!RGBA && in.zneed && all_done) {
+      // the depth of the splat that saturated the bin, for the next step's cut
+      int32_t f = -1;
+      for (int k = 0; k < NW; ++k) f = max(f, s_fin[k]);
+      if (f >= 0) atomicMax(in.zneed, f2u(in.S.depth[vals[start + (uint32_t)(f - li0)]]));
+    }
+    if (partial) {
+      in.flag[bin] = 1;
+      atomicAdd(in.n_flagged, 1u);
+    }
   }
-  if (partial && lane == 0) {
-    in.flag[tile] = 1;
-    atomicAdd(in.n_flagged, 1u);
-  }
-  // a tile without any pair keeps rgb 0 (the reference composites only the
-  // tiles its binning reached; cover marks those the tight rects emptied)
-  const bool empty = total_len == 0 && !(in.cover && in.cover[tile]);
+  // a tile the reference never binned keeps rgb 0 (it composites only the
+  // tiles its lists reach; cover says which when the lists here are not the
+  // reference's)
+  const bool empty = in.cover ? !(bt.ok && in.cover[bt.tile]) : total_len == 0;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int py = ty * TILE + 4 * q + (lane >> 4);
-    if (!(px < c.w && py < c.h)) continue;
+    if (!(bt.ok && px < c.w && py < c.h)) continue;
     const size_t pix = (size_t)c.pix_base + (size_t)py * c.w + px;
     if (!RGBA) {
       if (partial) {
@@ -256,16 +314,24 @@ __global__ __launch_bounds__(FT) __attribute__((amdgpu_waves_per_eu(6, 8))) void
   }
 }
 
-void launch_raster_fwd(const RasterIn& in, int mode, float* rgb, float* final_t, int32_t* last, hipStream_t st) {
-  if (in.total_tiles <= 0) return;
-  if (mode == 0)
-    hipLaunchKernelGGL(k_raster_fwd<0>, dim3(in.total_tiles), dim3(FT), 0, st, in, rgb, final_t, last);
-  else if (mode == 1)
-    hipLaunchKernelGGL(k_raster_fwd<1>, dim3(in.total_tiles), dim3(FT), 0, st, in, rgb, final_t, last);
-  else if (mode == 2)
-    hipLaunchKernelGGL(k_raster_fwd<2>, dim3(in.total_tiles), dim3(FT), 0, st, in, rgb, final_t, last);
+template <int MODE>
+static void fwd_launch(const RasterIn& in, float* rgb, float* final_t, int32_t* last, hipStream_t st) {
+  if (in.bin_shift == 0)
+    hipLaunchKernelGGL((k_raster_fwd<MODE, 1>), dim3(in.total_bins), dim3(FT), 0, st, in, rgb, final_t, last);
   else
-    hipLaunchKernelGGL(k_raster_fwd<3>, dim3(in.total_tiles), dim3(FT), 0, st, in, rgb, final_t, last);
+    hipLaunchKernelGGL((k_raster_fwd<MODE, 2>), dim3(in.total_bins), dim3(FT * 4), 0, st, in, rgb, final_t, last);
+}
+
+void launch_raster_fwd(const RasterIn& in, int mode, float* rgb, float* final_t, int32_t* last, hipStream_t st) {
+  if (in.total_bins <= 0) return;
+  if (mode == 1)  // API renders only: the reference's tiles
+    hipLaunchKernelGGL((k_raster_fwd<1, 1>), dim3(in.total_bins), dim3(FT), 0, st, in, rgb, final_t, last);
+  else if (mode == 0)
+    fwd_launch<0>(in, rgb, final_t, last, st);
+  else if (mode == 2)
+    fwd_launch<2>(in, rgb, final_t, last, st);
+  else
+    fwd_launch<3>(in, rgb, final_t, last, st);
 }
 
 template <int CTRL>
@@ -329,35 +395,41 @@ __device__ __forceinline__ int wave_max_i(int v) {
   return v;
 }
 
-// Backward: one wave per tile, four pixels per lane (pixel group q = tile
-// rows 4q..4q+3, lane l at column l & 15, row 4q + (l >> 4)). A lane sums its
-// four pixels' gradients for a splat in registers before the single cross-lane
-// reduction, so the reduction is paid once per 256 pixels.
+// Backward: four pixels per lane as in the forward. A lane sums its four
+// pixels' gradients for a splat in registers, the wave reduces them across
+// lanes into its LDS slot, and after each batch the bin adds its waves' sums
+// to the accumulators (one atomic per value and splat per bin).
 constexpr int BT = 64;
-__global__ __launch_bounds__(BT, 5) void k_raster_bwd(RasterIn in, const float* __restrict__ final_t,
-                                                   const int32_t* __restrict__ last_contrib,
-                                                   const float* __restrict__ dl_dpix, float* __restrict__ accum) {
+template <int S>
+__global__ __launch_bounds__(BT * S * S) void k_raster_bwd(RasterIn in, const float* __restrict__ final_t,
+                                                           const int32_t* __restrict__ last_contrib,
+                                                           const float* __restrict__ dl_dpix,
+                                                           float* __restrict__ accum) {
+  constexpr int NW = S * S;
   __shared__ uint64_t s_tab[32];
   __shared__ float4 s_a[BT];
   __shared__ float4 s_b[BT];
   __shared__ float s_c[BT];
+  __shared__ float s_r[BT];
+  __shared__ uint2 s_rect[BT];
   __shared__ uint32_t s_vi[BT];
-  __shared__ float s_g[9][BT];
-  const int lane = threadIdx.x;
-  if (lane < 32) s_tab[lane] = kExp2Tab[lane];
-  const int tile = in.tile_order[xcd_remap(blockIdx.x, gridDim.x)];
-  const int v = view_of_tile(in.cams, in.n_views, tile);
-  const CamDev& c = in.cams[v];
-  const int lt = tile - c.tile_base;
-  const int tx = lt % c.ntx, ty = lt / c.ntx;
-  uint32_t start = in.rstart[tile], end = in.rend[tile];
-  if (end < start) start = end;  // no pair reached this tile
+  __shared__ float s_g[NW][9][BT];
+  __shared__ int s_top[NW];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // wave-uniform
+  if (threadIdx.x < 32) s_tab[threadIdx.x] = kExp2Tab[threadIdx.x];
+  const int bin = in.tile_order[xcd_remap(blockIdx.x, gridDim.x)];
+  const BinTile bt = bin_tile<S>(in, bin, w);
+  const CamDev& c = in.cams[bt.v];
+  const int tx = bt.tx, ty = bt.ty;
+  uint32_t start = in.rstart[bin], end = in.rend[bin];
+  if (end < start) start = end;  // no pair reached this bin
   // list index p < nf: pair_vals[start + p]; else back_vals[bs + p - nf]
   const uint32_t nf = end - start;
   uint32_t bs = 0, nbk = 0;
   if (in.bstart) {
-    bs = in.bstart[tile];
-    const uint32_t be = in.bend[tile];
+    bs = in.bstart[bin];
+    const uint32_t be = in.bend[bin];
     nbk = be > bs ? be - bs : 0u;
   }
   const int px = tx * TILE + (lane & 15);
@@ -376,7 +448,7 @@ __global__ __launch_bounds__(BT, 5) void k_raster_bwd(RasterIn in, const float* 
     lc[q] = -1;
     T[q] = 1.0f;
     d0[q] = d1[q] = d2[q] = 0.0f;
-    if (px < c.w && py < c.h && nf + nbk != 0) {
+    if (bt.ok && px < c.w && py < c.h && nf + nbk != 0) {
       const size_t pix = (size_t)c.pix_base + (size_t)py * c.w + px;
       lc[q] = last_contrib[pix];
       T[q] = final_t[pix];
@@ -388,8 +460,13 @@ __global__ __launch_bounds__(BT, 5) void k_raster_bwd(RasterIn in, const float* 
     lc_max = max(lc_max, lc[q]);
   }
   // wave-uniform (scalar) so that the splat loops are scalar-controlled
-  const int top = __builtin_amdgcn_readfirstlane(wave_max_i(lc_max));
-  if (top < 0) return;
+  const int top_w = __builtin_amdgcn_readfirstlane(wave_max_i(lc_max));
+  if (lane == 0) s_top[w] = top_w;
+  __syncthreads();
+  int top = s_top[0];
+#pragma unroll
+  for (int k = 1; k < NW; ++k) top = max(top, s_top[k]);
+  if (top < 0) return;  // block-uniform
   // the per-pixel inputs are needed by the first contributing splat; wait for
   // them here rather than inside the loop, where the wait would also cover
   // the previous batch's accumulator atomics
@@ -401,22 +478,30 @@ __global__ __launch_bounds__(BT, 5) void k_raster_bwd(RasterIn in, const float* 
   for (int hi = top; hi >= 0; hi -= BT) {
     const int lo = max(0, hi - (BT - 1));
     const int nb = hi - lo + 1;
-    __syncthreads();  // previous batch fully read (one wave: s_barrier only)
-    uint32_t my_m = 0;  // group mask of this lane's staged splat
+    __syncthreads();  // previous batch fully read and flushed
     if (lane < nb) {
       const uint32_t p = (uint32_t)(lo + lane);
-      uint32_t vi = p < nf ? in.pair_vals[start + p] : in.back_vals[bs + (p - nf)];
-      s_vi[lane] = vi;
-      float4 A = in.S.xy_ab[vi];
-      float4 B = in.S.c_op_rg[vi];
-      s_a[lane] = A;
-      s_b[lane] = B;
-      s_c[lane] = in.S.col_b[vi];
-      my_m = wave_mask(A.x, A.y, A.z, A.w, B.x, in.S.reach[vi], tx, ty);
+      const uint32_t vi = p < nf ? in.pair_vals[start + p] : in.back_vals[bs + (p - nf)];
+      if (NW == 1 || w == 0) {
+        s_vi[lane] = vi;
+        s_a[lane] = in.S.xy_ab[vi];
+      }
+      if (NW == 1 || w == 1 % NW) s_b[lane] = in.S.c_op_rg[vi];
+      if (NW == 1 || w == 2 % NW) s_c[lane] = in.S.col_b[vi];
+      if (NW == 1 || w == 3 % NW) {
+        s_r[lane] = in.S.reach[vi];
+        if (S > 1) s_rect[lane] = in.S.rect[vi];
+      }
     }
 #pragma unroll
-    for (int k = 0; k < 9; ++k) s_g[k][lane] = 0.0f;
+    for (int k = 0; k < 9; ++k) s_g[w][k][lane] = 0.0f;
     __syncthreads();
+    uint32_t my_m = 0;  // group mask of this lane's staged splat for this wave's tile
+    if (lane < nb && lo <= top_w && (S == 1 || in_rect(s_rect[lane], tx, ty))) {
+      const float4 A = s_a[lane];
+      const float4 B = s_b[lane];
+      my_m = wave_mask(A.x, A.y, A.z, A.w, B.x, s_r[lane], tx, ty);
+    }
     // splats that reach some pixel group, visited back to front as set bits
     // of a scalar mask; the next one's LDS data is fetched while the current
     // one is processed
@@ -485,13 +570,13 @@ __global__ __launch_bounds__(BT, 5) void k_raster_bwd(RasterIn in, const float* 
       if (__popcll(bm) <= in.sparse_lanes) {
         if (contrib) {
 #pragma unroll
-          for (int k = 0; k < 9; ++k) atomicAdd(&s_g[k][j], g[k]);
+          for (int k = 0; k < 9; ++k) atomicAdd(&s_g[w][k][j], g[k]);
         }
       } else {
         const float t8 = wave_sum8_scatter(g, lane);
         const float s8 = wave_sum(g[8]);
-        if (lane < 8) s_g[rev3(lane)][j] += t8;
-        if (lane == 8) s_g[8][j] += s8;
+        if (lane < 8) s_g[w][rev3(lane)][j] += t8;
+        if (lane == 8) s_g[w][8][j] += s8;
       }
       // the accumulators live across splats and restart here, which spares
       // re-initialising them for every splat
@@ -499,40 +584,55 @@ __global__ __launch_bounds__(BT, 5) void k_raster_bwd(RasterIn in, const float* 
       for (int k = 0; k < 9; ++k) g[k] = 0.0f;
     }
     __syncthreads();
+    // flush: raw sums (dL/dcolour[3], dL/dopacity, S1..S5 with S the
+    // dL/dpower moments sum(dlp dx), sum(dlp dy), sum(dlp dx^2),
+    // sum(dlp dx dy), sum(dlp dy^2)) over the waves; dL/dconic and dL/dmean
+    // follow from the conic. Output o of splat lane goes to thread
+    // (o % NW) * 64 + lane.
     if (lane < nb) {
-      // s_g = (dL/dcolour[3], dL/dopacity, S1..S5) with S the dL/dpower
-      // moments sum(dlp dx), sum(dlp dy), sum(dlp dx^2), sum(dlp dx dy),
-      // sum(dlp dy^2); dL/dconic and dL/dmean follow from the conic.
       const uint32_t vi = s_vi[lane];
-      const float4 A = s_a[lane];
-      const float4 B = s_b[lane];
-      const float s1 = s_g[4][lane], s2 = s_g[5][lane], s3 = s_g[6][lane], s4 = s_g[7][lane], s5 = s_g[8][lane];
-      float out[9];
-      out[0] = s_g[0][lane];
-      out[1] = s_g[1][lane];
-      out[2] = s_g[2][lane];
-      out[3] = s_g[3][lane];
-      out[4] = -0.5f * s3;
-      out[5] = -s4;
-      out[6] = -0.5f * s5;
-      out[7] = A.z * s1 + A.w * s2;
-      out[8] = A.w * s1 + B.x * s2;
+      auto raw = [&](int k) {
+        float t = s_g[0][k][lane];
 #pragma unroll
-      for (int k = 0; k < 9; ++k)
-        if (out[k] != 0.0f) atomicAdd(&accum[(size_t)k * in.acc_stride + vi], out[k]);
+        for (int u = 1; u < NW; ++u) t += s_g[u][k][lane];
+        return t;
+      };
+#pragma unroll
+      for (int o = w; o < 9; o += NW) {
+        float val;
+        if (o < 4) {
+          val = raw(o);
+        } else if (o == 4) {
+          val = -0.5f * raw(6);
+        } else if (o == 5) {
+          val = -raw(7);
+        } else if (o == 6) {
+          val = -0.5f * raw(8);
+        } else {
+          const float4 A = s_a[lane];
+          const float cc = s_b[lane].x;
+          const float s1 = raw(4), s2 = raw(5);
+          val = o == 7 ? A.z * s1 + A.w * s2 : A.w * s1 + cc * s2;
+        }
+        if (val != 0.0f) atomicAdd(&accum[(size_t)o * in.acc_stride + vi], val);
+      }
     }
   }
 }
 
 void launch_raster_bwd(const RasterIn& in, const float* final_t, const int32_t* last, const float* dl_dpix,
                        float* accum, hipStream_t st) {
-  if (in.total_tiles <= 0) return;
-  hipLaunchKernelGGL(k_raster_bwd, dim3(in.total_tiles), dim3(BT), 0, st, in, final_t, last, dl_dpix, accum);
+  if (in.total_bins <= 0) return;
+  if (in.bin_shift == 0)
+    hipLaunchKernelGGL((k_raster_bwd<1>), dim3(in.total_bins), dim3(BT), 0, st, in, final_t, last, dl_dpix, accum);
+  else
+    hipLaunchKernelGGL((k_raster_bwd<2>), dim3(in.total_bins), dim3(BT * 4), 0, st, in, final_t, last, dl_dpix,
+                       accum);
 }
 
 // Algorithmic work of the last render: pairs the forward evaluates per pixel
-// (up to and including the saturating splat, else the whole tile list) and
-// pairs the backward evaluates (last contributor + 1).
+// (up to and including the saturating splat, else the whole bin list) and
+// pairs the backward evaluates (last contributor + 1). One block per tile.
 __global__ __launch_bounds__(RT) void k_work_counts(RasterIn in, const float* __restrict__ final_t,
                                                     const int32_t* __restrict__ last,
                                                     unsigned long long* __restrict__ out) {
@@ -542,16 +642,18 @@ __global__ __launch_bounds__(RT) void k_work_counts(RasterIn in, const float* __
   const int v = view_of_tile(in.cams, in.n_views, tile);
   const CamDev& c = in.cams[v];
   const int lt = tile - c.tile_base;
-  const int px = (lt % c.ntx) * TILE + (t & 15), py = (lt / c.ntx) * TILE + (t >> 4);
+  const int ttx = lt % c.ntx, tty = lt / c.ntx;
+  const int bin = c.bin_base + (tty >> in.bin_shift) * c.nbx + (ttx >> in.bin_shift);
+  const int px = ttx * TILE + (t & 15), py = tty * TILE + (t >> 4);
   unsigned long long f = 0, b = 0;
   if (px < c.w && py < c.h) {
-    const size_t pix = (size_t)c.pix_base + (size_t)py * c.w + px;
-    const uint32_t s0 = in.rstart[tile], e0 = in.rend[tile];
+    const uint32_t s0 = in.rstart[bin], e0 = in.rend[bin];
     uint32_t len = e0 > s0 ? e0 - s0 : 0u;
     if (in.bstart) {
-      const uint32_t s1 = in.bstart[tile], e1 = in.bend[tile];
+      const uint32_t s1 = in.bstart[bin], e1 = in.bend[bin];
       len += e1 > s1 ? e1 - s1 : 0u;
     }
+    const size_t pix = (size_t)c.pix_base + (size_t)py * c.w + px;
     const int32_t lc = last[pix];
     f = final_t[pix] < TRANSMITTANCE_STOP ? (unsigned long long)(lc + 1) : (unsigned long long)len;
     b = (unsigned long long)(lc + 1);
@@ -578,9 +680,9 @@ __global__ __launch_bounds__(RT) void k_work_counts(RasterIn in, const float* __
 }
 
 void launch_work_counts(const RasterIn& in, const float* final_t, const int32_t* last, unsigned long long* out,
-                        hipStream_t st) {
-  if (in.total_tiles <= 0) return;
-  hipLaunchKernelGGL(k_work_counts, dim3(in.total_tiles), dim3(RT), 0, st, in, final_t, last, out);
+                        int total_tiles, hipStream_t st) {
+  if (total_tiles <= 0) return;
+  hipLaunchKernelGGL(k_work_counts, dim3(total_tiles), dim3(RT), 0, st, in, final_t, last, out);
 }
 
 }  // namespace rgs

@@ -461,7 +461,7 @@ __global__ __launch_bounds__(GP_THREADS) void k_gen_pairs(const uint32_t* __rest
   __shared__ uint32_t s_vi[GP_THREADS];
   __shared__ uint32_t s_org[GP_THREADS];  // tx0 | ty0 << 16
   __shared__ uint32_t s_w[GP_THREADS];    // rect width in tiles
-  __shared__ uint32_t s_tb[GP_THREADS];   // tile base of the splat's view
+  __shared__ uint32_t s_tb[GP_THREADS];   // bin base of the splat's view
   __shared__ uint32_t s_ntx[GP_THREADS];
   __shared__ uint32_t s_h[256];
   const int t = threadIdx.x;
@@ -484,12 +484,12 @@ __global__ __launch_bounds__(GP_THREADS) void k_gen_pairs(const uint32_t* __rest
       uint32_t vi = sorted_vals[r];
       s_off[t] = offsets[r];
       s_vi[t] = vi;
-      uint2 rr = rect[vi];
+      const CamDev& cm = cams[vi / (uint32_t)cap];
+      const uint2 rr = bin_rect(rect[vi], cm.bin_shift);
       s_org[t] = rr.x;
       s_w[t] = cnt[vi] ? (rr.y & 0xffffu) - (rr.x & 0xffffu) : 1u;
-      const CamDev& cm = cams[vi / (uint32_t)cap];
-      s_tb[t] = (uint32_t)cm.tile_base;
-      s_ntx[t] = (uint32_t)cm.ntx;
+      s_tb[t] = (uint32_t)cm.bin_base;
+      s_ntx[t] = (uint32_t)cm.nbx;
     } else {
       s_off[t] = tot;
     }
@@ -582,11 +582,11 @@ __global__ __launch_bounds__(256) void k_back_count(const uint32_t* __restrict__
   if (*n_flagged != 0 && cut_out[2] != 0 && r >= cut_out[0]) {
     const uint32_t vi = sorted_vals[r];
     if (cnt[vi] != 0) {
-      const uint2 rr = rect[vi];
       const CamDev& cm = cams[vi / (uint32_t)cap];
+      const uint2 rr = bin_rect(rect[vi], cm.bin_shift);
       const uint32_t x0 = rr.x & 0xffffu, y0 = rr.x >> 16, x1 = rr.y & 0xffffu, y1 = rr.y >> 16;
       for (uint32_t ty = y0; ty < y1; ++ty) {
-        const uint8_t* row = flag + cm.tile_base + (size_t)ty * cm.ntx;
+        const uint8_t* row = flag + cm.bin_base + (size_t)ty * cm.nbx;
         for (uint32_t tx = x0; tx < x1; ++tx) k += row[tx];
       }
     }
@@ -625,11 +625,11 @@ __global__ __launch_bounds__(256) void k_back_gen(const uint32_t* __restrict__ s
   const size_t o_end = r + 1 < n ? offsets[r + 1] : tot;
   if (o == o_end) return;
   const uint32_t vi = sorted_vals[r];
-  const uint2 rr = rect[vi];
   const CamDev& cm = cams[vi / (uint32_t)cap];
+  const uint2 rr = bin_rect(rect[vi], cm.bin_shift);
   const uint32_t x0 = rr.x & 0xffffu, y0 = rr.x >> 16, x1 = rr.y & 0xffffu, y1 = rr.y >> 16;
   for (uint32_t ty = y0; ty < y1; ++ty) {
-    const uint32_t rowbase = (uint32_t)cm.tile_base + ty * (uint32_t)cm.ntx;
+    const uint32_t rowbase = (uint32_t)cm.bin_base + ty * (uint32_t)cm.nbx;
     for (uint32_t tx = x0; tx < x1; ++tx) {
       if (!flag[rowbase + tx]) continue;
       if (o < k_cap) {

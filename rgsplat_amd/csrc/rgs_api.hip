@@ -146,29 +146,39 @@ int make_camdev(const rgs_camera& cam, CamDev& out) {
 }
 
 // A set of views rendered by one launch sequence.
+// shift: pairs are binned per (1 << shift)^2 tiles (0: the reference's tiles,
+// API renders; the training step uses its own setting).
 struct ViewSet {
   std::vector<CamDev> cams;
+  int shift = 0;
   int total_tiles = 0;
+  int total_bins = 0;
   int total_ltiles = 0;
   size_t total_pix = 0;
   void layout() {
-    int tb = 0, lb = 0;
+    int tb = 0, bb = 0, lb = 0;
     size_t pb = 0;
     for (auto& c : cams) {
       c.tile_base = tb;
+      c.bin_shift = shift;
+      c.nbx = (c.ntx + (1 << shift) - 1) >> shift;
+      c.nby = (c.nty + (1 << shift) - 1) >> shift;
+      c.bin_base = bb;
       c.ltile_base = lb;
       c.pix_base = (int32_t)pb;
       tb += c.ntx * c.nty;
+      bb += c.nbx * c.nby;
       lb += ((c.w + LTILE - 1) / LTILE) * ((c.h + LTILE - 1) / LTILE);
       pb += (size_t)c.w * c.h;
     }
     total_tiles = tb;
+    total_bins = bb;
     total_ltiles = lb;
     total_pix = pb;
   }
   int bits() const {
     int b = 1;
-    while ((1ll << b) < (long long)total_tiles) ++b;
+    while ((1ll << b) < (long long)total_bins) ++b;
     return b;
   }
 };
@@ -232,7 +242,10 @@ struct rgs_ctx {
   CamDev* h_cams = nullptr;      // pinned, 64 entries
   uint64_t last_pairs = 0;
   size_t last_cap = 1;      // row capacity of the last render (vi = v * cap + i)
-  int last_tiles = 0;       // total tiles of the last render
+  int last_tiles = 0;       // total bins of the last render
+  int last_shift = 0;       // its bin shift (0: bins are the reference's tiles)
+  int last_ntiles = 0;      // its total tiles
+  size_t last_pix = 0;      // its total pixels
   int last_nviews = 0;      // views of the last render
   float last_bg[3] = {0, 0, 0};
 
@@ -242,11 +255,12 @@ struct rgs_ctx {
   uint64_t gen = 0;        // bumped on every device (re)allocation
   uint64_t graph_gen = 0;  // gen at capture time
   DBuf api_acc9;           // raster accumulators of the reference-API paths
-  DBuf tile_order;         // raster schedule: slot -> global tile id
+  DBuf tile_order[2];      // raster schedule per bin shift: slot -> global bin id
   DBuf train_scr;          // per-Gaussian scratch between geometry and Adam
   DBuf cover;              // per tile: composited although its tight list is empty
   std::vector<int32_t> h_order;
-  std::string order_key;
+  std::string order_key[2];
+  int bin_shift = 1;       // training step: bins of 2^shift x 2^shift tiles
   int super_tile = 8;      // tiles per side of a schedule super-tile
   int sparse_lanes = 2;    // backward: LDS atomics up to this many lanes
 
@@ -292,23 +306,24 @@ void drop_graph(rgs_ctx* c) {
 // so that the blocks one XCD holds in flight cover a compact patch of the
 // image and share the splats (and accumulator lines) of that patch in L2.
 int ensure_tile_order(rgs_ctx* c, const ViewSet& vs) {
+  const int sh = vs.shift;
   std::string key = std::to_string(c->super_tile) + ":";
-  for (const auto& cd : vs.cams) key += std::to_string(cd.ntx) + "x" + std::to_string(cd.nty) + ",";
-  size_t bytes = (size_t)std::max(vs.total_tiles, 1) * 4;
-  if (key == c->order_key && c->tile_order.bytes >= bytes) return 0;
-  c->h_order.assign(std::max(vs.total_tiles, 1), 0);
+  for (const auto& cd : vs.cams) key += std::to_string(cd.nbx) + "x" + std::to_string(cd.nby) + ",";
+  size_t bytes = (size_t)std::max(vs.total_bins, 1) * 4;
+  if (key == c->order_key[sh] && c->tile_order[sh].bytes >= bytes) return 0;
+  c->h_order.assign(std::max(vs.total_bins, 1), 0);
   const int s = std::max(c->super_tile, 1);
   size_t k = 0;
   for (const auto& cd : vs.cams)
-    for (int sy = 0; sy < cd.nty; sy += s)
-      for (int sx = 0; sx < cd.ntx; sx += s)
-        for (int ty = sy; ty < std::min(sy + s, (int)cd.nty); ++ty)
-          for (int tx = sx; tx < std::min(sx + s, (int)cd.ntx); ++tx) c->h_order[k++] = cd.tile_base + ty * cd.ntx + tx;
-  CHK(ensure(c, c->tile_order, bytes));
+    for (int sy = 0; sy < cd.nby; sy += s)
+      for (int sx = 0; sx < cd.nbx; sx += s)
+        for (int by = sy; by < std::min(sy + s, (int)cd.nby); ++by)
+          for (int bx = sx; bx < std::min(sx + s, (int)cd.nbx); ++bx) c->h_order[k++] = cd.bin_base + by * cd.nbx + bx;
+  CHK(ensure(c, c->tile_order[sh], bytes));
   HIPCHK(hipStreamSynchronize(c->st));  // a running raster may still read the old schedule
-  HIPCHK(hipMemcpy(c->tile_order.p, c->h_order.data(), bytes, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->tile_order[sh].p, c->h_order.data(), bytes, hipMemcpyHostToDevice));
   ++c->gen;  // graphs captured with the old schedule are stale
-  c->order_key = key;
+  c->order_key[sh] = key;
   return 0;
 }
 
@@ -341,11 +356,11 @@ int ensure_render(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, size_t k_cap
   CHK(ensure(c, c->hist, (size_t)256 * nb * 4));
   c->hist_nb = nb;
   CHK(ensure(c, c->rowtot, 256 * 4));
-  CHK(ensure(c, c->rstart, (size_t)vs.total_tiles * 4));
-  CHK(ensure(c, c->rend, (size_t)vs.total_tiles * 4));
-  CHK(ensure(c, c->rbstart, (size_t)vs.total_tiles * 4));
-  CHK(ensure(c, c->rbend, (size_t)vs.total_tiles * 4));
-  CHK(ensure(c, c->flag, ((size_t)vs.total_tiles + 3) / 4 * 4));
+  CHK(ensure(c, c->rstart, (size_t)vs.total_bins * 4));
+  CHK(ensure(c, c->rend, (size_t)vs.total_bins * 4));
+  CHK(ensure(c, c->rbstart, (size_t)vs.total_bins * 4));
+  CHK(ensure(c, c->rbend, (size_t)vs.total_bins * 4));
+  CHK(ensure(c, c->flag, ((size_t)vs.total_bins + 3) / 4 * 4));
   CHK(ensure(c, c->pv2, k_cap * 4));
   CHK(ensure(c, c->cntb, dn * 4));
   CHK(ensure(c, c->first_splat, ((size_t)radix_nb(k_cap) + 1) * 4));
@@ -391,23 +406,24 @@ void enqueue_project(rgs_ctx* c, const float* P, size_t cap, int n, const ViewSe
   stage_end(c, S_SCAN);
 }
 
-RasterIn raster_in_n(rgs_ctx* c, int n_views, int total_tiles, const float bg[3]) {
+RasterIn raster_in_n(rgs_ctx* c, int n_views, int total_bins, int shift, const float bg[3]) {
   RasterIn in{};
   in.rstart = P_<uint32_t>(c->rstart);
   in.rend = P_<uint32_t>(c->rend);
   in.pair_vals = c->pair_vals;
-  in.tile_order = P_<int32_t>(c->tile_order);
+  in.tile_order = P_<int32_t>(c->tile_order[shift]);
   in.S = splats(c);
   in.cams = P_<CamDev>(c->cams_d);
   in.n_views = n_views;
   in.sparse_lanes = c->sparse_lanes;
   in.acc_stride = 0;  // set by the backward's callers
-  in.total_tiles = total_tiles;
+  in.total_bins = total_bins;
+  in.bin_shift = shift;
   for (int k = 0; k < 3; ++k) in.bg[k] = bg[k];
   return in;
 }
 RasterIn raster_in(rgs_ctx* c, const ViewSet& vs, const float bg[3]) {
-  return raster_in_n(c, (int)vs.cams.size(), vs.total_tiles, bg);
+  return raster_in_n(c, (int)vs.cams.size(), vs.total_bins, vs.shift, bg);
 }
 
 // Pair generation, tile sort, ranges and the forward raster.
@@ -425,7 +441,7 @@ void enqueue_bin_and_raster(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, co
   uint32_t* keys[2] = {P_<uint32_t>(c->pk[0]), P_<uint32_t>(c->pk[1])};
   uint32_t* vals[2] = {P_<uint32_t>(c->pv[0]), P_<uint32_t>(c->pv[1])};
   stage_begin(c, S_TSORT);
-  TileRanges tr{P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend), (uint32_t)vs.total_tiles};
+  TileRanges tr{P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend), (uint32_t)vs.total_bins};
   int cur = radix_sort_tiles(keys, vals, sc + SC_K, c->k_cap, vs.bits(), rs, tr, st);
   stage_end(c, S_TSORT);
   c->pair_vals = vals[cur];
@@ -468,7 +484,7 @@ void enqueue_bin_and_raster_cut(rgs_ctx* c, size_t cap, int n, const ViewSet& vs
   uint32_t* keys[2] = {P_<uint32_t>(c->pk[0]), P_<uint32_t>(c->pk[1])};
   uint32_t* vals[2] = {P_<uint32_t>(c->pv[0]), P_<uint32_t>(c->pv[1])};
   stage_begin(c, S_TSORT);
-  TileRanges tr{P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend), (uint32_t)vs.total_tiles};
+  TileRanges tr{P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend), (uint32_t)vs.total_bins};
   int cur = radix_sort_tiles(keys, vals, sc + SC_K, c->k_cap, vs.bits(), rs, tr, st);
   stage_end(c, S_TSORT);
   c->pair_vals = vals[cur];
@@ -478,7 +494,7 @@ void enqueue_bin_and_raster_cut(rgs_ctx* c, size_t cap, int n, const ViewSet& vs
   in.flag = P_<uint8_t>(c->flag);
   in.n_flagged = sc + SC_NFLAG;
   in.zneed = &cs->zneed;
-  launch_fill_u32(P_<uint32_t>(c->flag), ((size_t)vs.total_tiles + 3) / 4, 0u, st);
+  launch_fill_u32(P_<uint32_t>(c->flag), ((size_t)vs.total_bins + 3) / 4, 0u, st);
   stage_begin(c, S_FWD);
   launch_raster_fwd(in, 3, rgb_out, P_<float>(c->final_t), P_<int32_t>(c->last), st);
   stage_end(c, S_FWD);
@@ -493,7 +509,7 @@ void enqueue_bin_and_raster_cut(rgs_ctx* c, size_t cap, int n, const ViewSet& vs
   launch_back_gen(c->sorted_vals, dn, cntb, sc + SC_BTOTAL, sc + SC_CUT, sc + SC_NFLAG, splats(c), (int)cap,
                   P_<CamDev>(c->cams_d), P_<uint8_t>(c->flag), keys[0], bvals[0], c->k_cap, sc + SC_KB,
                   sc + SC_STATUS, st);
-  TileRanges trb{P_<uint32_t>(c->rbstart), P_<uint32_t>(c->rbend), (uint32_t)vs.total_tiles};
+  TileRanges trb{P_<uint32_t>(c->rbstart), P_<uint32_t>(c->rbend), (uint32_t)vs.total_bins};
   int curb = radix_sort_tiles_full(keys, bvals, sc + SC_KB, c->k_cap, vs.bits(), rs, trb, st);
   c->back_vals = bvals[curb];
   in.bstart = P_<uint32_t>(c->rbstart);
@@ -526,7 +542,9 @@ int render_eager(rgs_ctx* c, const float* P, size_t cap, int n, ViewSet& vs, con
   c->last_pairs = K;
   c->last_cut = false;
   c->last_cap = cap;
-  c->last_tiles = vs.total_tiles;
+  c->last_tiles = vs.total_bins;
+  c->last_shift = vs.shift;
+  c->last_ntiles = vs.total_tiles;
   c->last_nviews = (int)vs.cams.size();
   return 0;
 }
@@ -637,7 +655,7 @@ void rgs_destroy(rgs_ctx* c) {
                  &c->api_target, &c->api_m1, &c->api_m2, &c->api_acc, &c->api_den, &c->api_P2, &c->dens_want,
                  &c->dens_cnt, &c->dens_W, &c->dens_packed, &c->dens_prefix, &c->dens_b32, &c->dens_b64,
                  &c->dens_t32, &c->dens_t64, &c->first_splat, &c->P, &c->P2, &c->M1, &c->M2, &c->gacc, &c->gden, &c->targets,
-                 &c->adam_d, &c->api_acc9, &c->tile_order, &c->train_scr, &c->cover, &c->depth,
+                 &c->adam_d, &c->api_acc9, &c->tile_order[0], &c->tile_order[1], &c->train_scr, &c->cover, &c->depth,
                  &c->flag, &c->rbstart, &c->rbend, &c->pv2, &c->cntb, &c->cutst};
   for (DBuf* b : all)
     if (b->p) (void)hipFree(b->p);
@@ -1078,11 +1096,12 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
     vs.cams.push_back(c->reg_cams[ids[i]]);
     key += std::to_string(c->reg_cams[ids[i]].w) + "x" + std::to_string(c->reg_cams[ids[i]].h) + ",";
   }
+  vs.shift = c->bin_shift;
   vs.layout();
   size_t cap = c->cap;
   int n = (int)c->n;
   char buf[160];
-  std::snprintf(buf, sizeof(buf), "|n%d|bg%a,%a,%a|l%a", n, bg[0], bg[1], bg[2], lambda);
+  std::snprintf(buf, sizeof(buf), "|n%d|bg%a,%a,%a|l%a|s%d", n, bg[0], bg[1], bg[2], lambda, vs.shift);
   key += buf;
 
   c->t += 1;
@@ -1180,7 +1199,10 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
     c->last_pairs = c->last_front_pairs + c->last_back_pairs;
     c->last_cut = true;
     c->last_cap = cap;
-    c->last_tiles = vs.total_tiles;
+    c->last_tiles = vs.total_bins;
+    c->last_shift = vs.shift;
+    c->last_ntiles = vs.total_tiles;
+    c->last_pix = vs.total_pix;
     c->last_nviews = (int)vs.cams.size();
     if (c->profiling) {
       for (int s = 0; s < RGS_N_STAGES; ++s) {
@@ -1211,6 +1233,7 @@ int rgs_autotune_raster(rgs_ctx* c, const int* ids, int n_ids, const float bg[3]
     if (ids[i] < 0 || ids[i] >= (int)c->reg_cams.size()) return fail(RGS_E_ARG, "view id out of range");
     vs.cams.push_back(c->reg_cams[ids[i]]);
   }
+  vs.shift = c->bin_shift;
   vs.layout();
   drop_graph(c);
   const size_t cap = c->cap;
@@ -1364,18 +1387,36 @@ int rgs_step_stats(rgs_ctx* c, uint64_t* n_pairs, uint64_t* fwd_evals, uint64_t*
     unsigned long long* d = P_<unsigned long long>(c->api_den);
     launch_fill_u32(P_<uint32_t>(c->api_den), 4, 0u, c->st);
     const float zero3[3] = {0.0f, 0.0f, 0.0f};
-    RasterIn in = raster_in_n(c, c->last_nviews, c->last_tiles, zero3);
+    RasterIn in = raster_in_n(c, c->last_nviews, c->last_tiles, c->last_shift, zero3);
     if (c->last_cut) {
       in.bstart = P_<uint32_t>(c->rbstart);
       in.bend = P_<uint32_t>(c->rbend);
     }
-    launch_work_counts(in, P_<float>(c->final_t), P_<int32_t>(c->last), d, c->st);
+    launch_work_counts(in, P_<float>(c->final_t), P_<int32_t>(c->last), d, c->last_ntiles, c->st);
     unsigned long long h[2] = {0, 0};
     HIPCHK(hipMemcpyAsync(h, d, 16, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     if (fwd_evals) *fwd_evals = h[0];
     if (bwd_evals) *bwd_evals = h[1];
   }
+  return 0;
+}
+
+int rgs_step_image(rgs_ctx* c, float* rgb, size_t n_floats) {
+  if (!c || !rgb) return fail(RGS_E_ARG, "null argument");
+  if (!c->last_cut) return fail(RGS_E_ARG, "no training step rendered yet");
+  const size_t need = c->last_pix * 3;
+  if (n_floats < need) return fail(RGS_E_ARG, "rgb too small");
+  (void)hipSetDevice(c->device);
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipMemcpy(rgb, c->rgb.p, need * 4, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int rgs_set_train_binning(rgs_ctx* c, int shift) {
+  if (!c) return fail(RGS_E_ARG, "null ctx");
+  if (shift < 0 || shift > 1) return fail(RGS_E_ARG, "bin shift must be 0 or 1");
+  c->bin_shift = shift;
   return 0;
 }
 
@@ -1426,6 +1467,7 @@ int rgs_eval_expf(rgs_ctx* c, const float* x, float* out, size_t n) {
 
 int rgs_debug_tiles(rgs_ctx* c, uint64_t* offsets, size_t n_offsets, uint32_t* gis, size_t k_max, uint64_t* k_out) {
   if (!c || !offsets || !k_out) return fail(RGS_E_ARG, "null argument");
+  if (c->last_shift != 0) return fail(RGS_E_ARG, "the last render binned per super-tile (a training step)");
   if (n_offsets < (size_t)c->last_tiles + 1) return fail(RGS_E_ARG, "offsets too small");
   (void)hipSetDevice(c->device);
   HIPCHK(hipStreamSynchronize(c->st));

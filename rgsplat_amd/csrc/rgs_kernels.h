@@ -14,8 +14,8 @@ struct SplatArrays {
   float4* xy_ab;    // sx, sy, conic a, conic b
   float4* c_op_rg;  // conic c, opacity, red, green
   float* col_b;     // blue
-  uint32_t* cnt;    // tiles touched (0 when culled)
-  uint2* rect;      // (tx0 | ty0 << 16, tx1 | ty1 << 16)
+  uint32_t* cnt;    // bins touched (0 when culled)
+  uint2* rect;      // tiles: (tx0 | ty0 << 16, tx1 | ty1 << 16)
   float* reach;     // R: alpha >= 1/255 needs conic form <= R (+inf: no bound)
   float* depth;     // camera-space depth (visible splats)
 };
@@ -29,11 +29,12 @@ enum : uint32_t { ST_NAN_DEPTH = 1u, ST_PAIR_OVERFLOW = 2u };
 // culled; value = v * cap + i) at dense index v * n + i. tight != 0: each
 // splat's tile rect is the reference's 3-sigma rect cut down to the tiles its
 // alpha-reach ellipse (alpha >= 1/255) can reach; the dropped (tile, splat)
-// pairs change no pixel's blend, only the list indices. A tile can lose all
-// its pairs, though, and the reference composites a tile (rgb = C + T bg)
-// iff its own list is non-empty: with cover non-null, every tile of a
-// splat's reference rect that the cut rect drops gets cover[tile] = 1, so
-// that "reference list non-empty" = (own list non-empty) || cover[tile].
+// pairs change no pixel's blend, only the list indices. The reference
+// composites a tile (rgb = C + T bg) iff its own list is non-empty, which a
+// tight or binned list no longer tells: with cover non-null, every tile of a
+// splat's reference rect gets cover[tile] = 1. Rects are in tiles; counts
+// and pair keys in bins of 2^bin_shift x 2^bin_shift tiles (CamDev), the
+// bins the rect meets.
 void launch_project(const float* P, int cap, int n, const CamDev* cams, int n_views, SplatArrays S,
                     uint32_t* sort_keys, uint32_t* sort_vals, uint32_t* status, int tight, uint8_t* cover,
                     hipStream_t st);
@@ -91,6 +92,13 @@ void scan_gathered_counts(const uint32_t* vals, const uint32_t* cnt, size_t n, u
 void scan_u32(const uint32_t* in, size_t n, uint32_t* out, uint32_t* block_sums, uint32_t* total, hipStream_t st);
 // Exclusive scan of a u64 array.
 void scan_u64(const uint64_t* in, size_t n, uint64_t* out, uint64_t* block_sums, uint64_t* total, hipStream_t st);
+
+// Bin rect of a splat's tile rect (bins of 2^sh x 2^sh tiles).
+__device__ __forceinline__ uint2 bin_rect(uint2 r, int sh) {
+  const uint32_t x0 = r.x & 0xffffu, y0 = r.x >> 16, x1 = r.y & 0xffffu, y1 = r.y >> 16;
+  if (!sh || x1 <= x0 || y1 <= y0) return r;
+  return make_uint2((x0 >> sh) | ((y0 >> sh) << 16), (((x1 - 1) >> sh) + 1) | ((((y1 - 1) >> sh) + 1) << 16));
+}
 
 // Writes the (tile, splat) pairs in depth order, one radix tile (RS_TILE
 // pairs) per block, and the first radix pass's per-tile digit histogram
@@ -156,9 +164,10 @@ struct RasterIn {
   int n_views;
   int sparse_lanes;  // backward: at most this many contributing lanes -> LDS atomics
   size_t acc_stride;  // backward accumulators: value k of entry vi at accum[k * acc_stride + vi]
-  int total_tiles;
+  int total_bins;   // workgroups: one per bin (2^bin_shift x 2^bin_shift tiles)
+  int bin_shift;
   float bg[3];
-  const uint8_t* cover;  // null, or tiles composited although their list is empty (launch_project)
+  const uint8_t* cover;  // null, or per tile: composited by the reference (launch_project)
   // depth-cut binning (null when every list is whole)
   const uint32_t* cut_on;     // device word: 1 when a front list may be partial
   uint8_t* flag;              // per tile: front list ended before all pixels saturated
@@ -183,9 +192,10 @@ void launch_raster_fwd(const RasterIn& in, int mode, float* rgb, float* final_t,
 void launch_raster_bwd(const RasterIn& in, const float* final_t, const int32_t* last, const float* dl_dpix,
                        float* accum, hipStream_t st);
 
-// out[0] += forward pair evaluations, out[1] += backward pair evaluations.
+// out[0] += forward pair evaluations, out[1] += backward pair evaluations
+// (list entries, of the lists the render used); one block per tile.
 void launch_work_counts(const RasterIn& in, const float* final_t, const int32_t* last, unsigned long long* out,
-                        hipStream_t st);
+                        int total_tiles, hipStream_t st);
 
 // ---- loss (k_loss.hip)
 constexpr int LTILE = 32;  // loss tile side

@@ -79,6 +79,8 @@ SIGNATURES = {
     "rgs_step_stats": [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                        ctypes.POINTER(ctypes.c_uint64)],
     "rgs_set_depth_cut": [_vp, _i, _f],
+    "rgs_set_train_binning": [_vp, _i],
+    "rgs_step_image": [_vp, _vp, _sz],
     "rgs_cut_stats": [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                       ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_float)],
     "rgs_set_raster_params": [_vp, _i, _i],
@@ -314,6 +316,16 @@ class Context:
         k, f, b = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
         _check(lib().rgs_step_stats(self.h, ctypes.byref(k), ctypes.byref(f), ctypes.byref(b)))
         return k.value, f.value, b.value
+
+    def set_train_binning(self, shift: int):
+        """train_step bins per 2^shift x 2^shift tiles (0 or 1, default 1)."""
+        _check(lib().rgs_set_train_binning(self.h, int(shift)))
+
+    def step_image(self, n_pixels: int) -> np.ndarray:
+        """The last train_step's rendered images, flat (n_pixels, 3)."""
+        out = np.zeros((n_pixels, 3), np.float32)
+        _check(lib().rgs_step_image(self.h, _p(out), out.size))
+        return out
 
     def set_depth_cut(self, mode: int, depth: float = 0.0):
         """Depth-cut binning of train_step: 0 adaptive (default), 1 off
