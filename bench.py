@@ -208,7 +208,20 @@ def main():
             ctx.train_step(ids, cfg, use_graph=use_graph)
         sync()
         out["uninstrumented_iters_per_s"] = args.steps / (time.perf_counter() - u0)
-        K, pf, pb = ctx.step_work()
+        K = ctx.step_pairs()
+        cut = ctx.cut_stats()
+        # Algorithmic work, independent of this build's binning: the (pixel,
+        # splat) pairs the reference's forward and backward evaluate on its
+        # own tile lists (each view rendered once more through the reference-
+        # binning entry point on the trained scene).
+        g_now = ctx.scene_download()
+        pf = pb = 0
+        for cam in cams:
+            ctx.render_color(g_now, cam)
+            _, f_, b_ = ctx.step_work()
+            pf += f_
+            pb += b_
+        del g_now
         npix = sum(w * h for w, h in wl.sizes)
         ng = ctx.scene_size()
         work = {
@@ -240,8 +253,10 @@ def main():
                                "frac": None, "traffic": traffic.get(dom), "ms": stages[dom]}
         out["stages_ms"] = stages
         out["stage_rooflines"] = rl
-        out["work"] = {"tile_pairs": K, "fwd_pair_evals": pf, "bwd_pair_evals": pb, "gaussians": ng,
-                       "timed_steps_with_events": n_prof, "of_which_graph_replays": n_graph}
+        out["work"] = {"sorted_pairs": K, "front_pairs": cut["front_pairs"], "back_pairs": cut["back_pairs"],
+                       "flagged_bins": cut["flagged_tiles"], "fwd_pair_evals": pf, "bwd_pair_evals": pb,
+                       "pair_evals_source": "reference tile lists (API render of the trained scene)",
+                       "gaussians": ng, "timed_steps_with_events": n_prof, "of_which_graph_replays": n_graph}
         print(f"[bench] stages {json.dumps(stages)}", file=sys.stderr, flush=True)
 
     if not args.no_plain:

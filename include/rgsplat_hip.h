@@ -188,11 +188,13 @@ int rgs_step_image(rgs_ctx* ctx, float* rgb, size_t n_floats);
  * it; the scene and optimizer state are not changed. */
 int rgs_set_raster_params(rgs_ctx* ctx, int super_tile, int sparse_lanes);
 /* Binning of rgs_train_step: pairs per bin of 2^shift x 2^shift tiles
- * (0: the reference's 16x16 tiles; 1, the default: 32x32, rasterised by one
- * workgroup of four waves that share the staged splats and merge each
- * splat's gradient before the accumulator atomics). API renders always use
- * the reference's tiles. */
-int rgs_set_train_binning(rgs_ctx* ctx, int shift);
+ * (0: the reference's 16x16 tiles; 1, the default: 32x32). The backward
+ * rasterises a bin with one workgroup of four waves that share the staged
+ * splats and merge each splat's gradient before the accumulator atomics;
+ * the forward does the same with fwd_tile_blocks = 0, or runs one
+ * single-wave workgroup per tile of the bin (1, the default). API renders
+ * always use the reference's tiles. */
+int rgs_set_train_binning(rgs_ctx* ctx, int shift, int fwd_tile_blocks);
 int rgs_get_raster_params(rgs_ctx* ctx, int* super_tile, int* sparse_lanes);
 int rgs_autotune_raster(rgs_ctx* ctx, const int* view_ids, int n_ids, const float bg[3], float lambda_dssim,
                         int* super_tile, int* sparse_lanes, float* raster_ms);

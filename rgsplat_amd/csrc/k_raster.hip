@@ -123,10 +123,13 @@ __device__ __forceinline__ float alpha_fast(float power, float op, const uint64_
 // bin flagged by MODE 3 on its back list: the state (C, T, last) comes from
 // the output buffers and list indices continue after the front list.
 constexpr int FT = 64;
-template <int MODE, int S>
-__global__ __launch_bounds__(FT * S * S) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_raster_fwd(
+// NW waves per workgroup: S * S (the bin's tiles share the staged splats) or
+// 1 (one workgroup per tile of the bin, each staging the bin list itself and
+// ending on its own).
+template <int MODE, int S, int NW>
+__global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_raster_fwd(
     RasterIn in, float* __restrict__ out_rgb, float* __restrict__ out_t, int32_t* __restrict__ out_last) {
-  constexpr int NW = S * S;
+  static_assert(NW == 1 || NW == S * S, "waves per workgroup");
   constexpr bool FAST = MODE >= 2;
   constexpr bool RGBA = MODE == 1;
   __shared__ uint64_t s_tab[32];
@@ -137,11 +140,16 @@ __global__ __launch_bounds__(FT * S * S) __attribute__((amdgpu_waves_per_eu(6, 8
   __shared__ uint2 s_rect[FT];
   __shared__ int32_t s_fin[NW];
   const int lane = threadIdx.x & 63;
-  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // wave-uniform
-  const int bin = in.tile_order[xcd_remap(blockIdx.x, gridDim.x)];
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  // this wave's tile in the bin (wave-uniform), and its LDS slot
+  const int w = NW == 1 ? lb % (S * S) : __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int ws = NW == 1 ? 0 : w;
+  const int bin = in.tile_order[NW == 1 ? lb / (S * S) : lb];
   if (MODE == 2 && !in.flag[bin]) return;  // block-uniform
-  if (threadIdx.x < 32) s_tab[threadIdx.x] = kExp2Tab[threadIdx.x];
   const BinTile bt = bin_tile<S>(in, bin, w);
+  // a single-tile workgroup resumes only a tile that kept its raw state
+  if (MODE == 2 && NW == 1 && !(bt.ok && in.tile_flag[bt.tile])) return;
+  if (threadIdx.x < 32) s_tab[threadIdx.x] = kExp2Tab[threadIdx.x];
   const CamDev& c = in.cams[bt.v];
   const int tx = bt.tx, ty = bt.ty;
   const int px = tx * TILE + (lane & 15);
@@ -193,10 +201,10 @@ __global__ __launch_bounds__(FT * S * S) __attribute__((amdgpu_waves_per_eu(6, 8
     const uint32_t idx = b0 + lane;
     if (idx < end) {
       const uint32_t vi = vals[idx];
-      if (NW == 1 || w == 0) s_a[lane] = in.S.xy_ab[vi];
-      if (NW == 1 || w == 1 % NW) s_b[lane] = in.S.c_op_rg[vi];
-      if (NW == 1 || w == 2 % NW) s_c[lane] = in.S.col_b[vi];
-      if (NW == 1 || w == 3 % NW) {
+      if (NW == 1 || ws == 0) s_a[lane] = in.S.xy_ab[vi];
+      if (NW == 1 || ws == 1 % NW) s_b[lane] = in.S.c_op_rg[vi];
+      if (NW == 1 || ws == 2 % NW) s_c[lane] = in.S.col_b[vi];
+      if (NW == 1 || ws == 3 % NW) {
         s_r[lane] = in.S.reach[vi];
         if (S > 1) s_rect[lane] = in.S.rect[vi];
       }
@@ -262,14 +270,16 @@ __global__ __launch_bounds__(FT * S * S) __attribute__((amdgpu_waves_per_eu(6, 8
       }
     }
   }
-  if (lane == 0) s_fin[w] = fin;
+  if (lane == 0) s_fin[ws] = fin;
+  // (with NW = 1: this tile's pixels)
   const bool all_done = !__syncthreads_or(!(done[0] && done[1] && done[2] && done[3]));
   // MODE 3 under a depth cut: a list that ended with pixels left is a front
-  // list; the bin keeps its raw state and waits for its back list
+  // list; the workgroup's tiles keep their raw state (tile_flag) and the bin
+  // gets its back list (flag)
   const bool partial = (MODE == 0 || MODE == 3) && in.cut_on && *in.cut_on && !all_done;
   if (threadIdx.x == 0) {
     if (!RGBA && in.zneed && all_done) {
-      // the depth of the splat that saturated the bin, for the next step's cut
+      // the depth of the splat that saturated the tiles, for the next step's cut
       int32_t f = -1;
       for (int k = 0; k < NW; ++k) f = max(f, s_fin[k]);
       if (f >= 0) atomicMax(in.zneed, f2u(in.S.depth[vals[start + (uint32_t)(f - li0)]]));
@@ -279,6 +289,7 @@ __global__ __launch_bounds__(FT * S * S) __attribute__((amdgpu_waves_per_eu(6, 8
       atomicAdd(in.n_flagged, 1u);
     }
   }
+  if (partial && lane == 0 && bt.ok) in.tile_flag[bt.tile] = 1;
   // a tile the reference never binned keeps rgb 0 (it composites only the
   // tiles its lists reach; cover says which when the lists here are not the
   // reference's)
@@ -317,15 +328,19 @@ __global__ __launch_bounds__(FT * S * S) __attribute__((amdgpu_waves_per_eu(6, 8
 template <int MODE>
 static void fwd_launch(const RasterIn& in, float* rgb, float* final_t, int32_t* last, hipStream_t st) {
   if (in.bin_shift == 0)
-    hipLaunchKernelGGL((k_raster_fwd<MODE, 1>), dim3(in.total_bins), dim3(FT), 0, st, in, rgb, final_t, last);
+    hipLaunchKernelGGL((k_raster_fwd<MODE, 1, 1>), dim3(in.total_bins), dim3(FT), 0, st, in, rgb, final_t, last);
+  else if (in.fwd_tile_blocks)
+    hipLaunchKernelGGL((k_raster_fwd<MODE, 2, 1>), dim3(in.total_bins * 4), dim3(FT), 0, st, in, rgb, final_t,
+                       last);
   else
-    hipLaunchKernelGGL((k_raster_fwd<MODE, 2>), dim3(in.total_bins), dim3(FT * 4), 0, st, in, rgb, final_t, last);
+    hipLaunchKernelGGL((k_raster_fwd<MODE, 2, 4>), dim3(in.total_bins), dim3(FT * 4), 0, st, in, rgb, final_t,
+                       last);
 }
 
 void launch_raster_fwd(const RasterIn& in, int mode, float* rgb, float* final_t, int32_t* last, hipStream_t st) {
   if (in.total_bins <= 0) return;
   if (mode == 1)  // API renders only: the reference's tiles
-    hipLaunchKernelGGL((k_raster_fwd<1, 1>), dim3(in.total_bins), dim3(FT), 0, st, in, rgb, final_t, last);
+    hipLaunchKernelGGL((k_raster_fwd<1, 1, 1>), dim3(in.total_bins), dim3(FT), 0, st, in, rgb, final_t, last);
   else if (mode == 0)
     fwd_launch<0>(in, rgb, final_t, last, st);
   else if (mode == 2)

@@ -211,7 +211,7 @@ struct rgs_ctx {
   const uint32_t* sorted_keys = nullptr;
   const uint32_t* pair_vals = nullptr;
   // depth-cut binning of the training step
-  DBuf flag, rbstart, rbend, pv2, cntb, cutst;
+  DBuf flag, tile_flag, rbstart, rbend, pv2, cntb, cutst;
   const uint32_t* back_vals = nullptr;
   int cut_mode = 0;              // CutState::mode
   float cut_depth = 0.0f;        // mode 2: fixed cut depth
@@ -261,6 +261,7 @@ struct rgs_ctx {
   std::vector<int32_t> h_order;
   std::string order_key[2];
   int bin_shift = 1;       // training step: bins of 2^shift x 2^shift tiles
+  int fwd_tile_blocks = 1; // training forward with bins: a workgroup per tile (1) or per bin (0)
   int super_tile = 8;      // tiles per side of a schedule super-tile
   int sparse_lanes = 2;    // backward: LDS atomics up to this many lanes
 
@@ -361,6 +362,7 @@ int ensure_render(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, size_t k_cap
   CHK(ensure(c, c->rbstart, (size_t)vs.total_bins * 4));
   CHK(ensure(c, c->rbend, (size_t)vs.total_bins * 4));
   CHK(ensure(c, c->flag, ((size_t)vs.total_bins + 3) / 4 * 4));
+  CHK(ensure(c, c->tile_flag, ((size_t)vs.total_tiles + 3) / 4 * 4));
   CHK(ensure(c, c->pv2, k_cap * 4));
   CHK(ensure(c, c->cntb, dn * 4));
   CHK(ensure(c, c->first_splat, ((size_t)radix_nb(k_cap) + 1) * 4));
@@ -419,6 +421,7 @@ RasterIn raster_in_n(rgs_ctx* c, int n_views, int total_bins, int shift, const f
   in.acc_stride = 0;  // set by the backward's callers
   in.total_bins = total_bins;
   in.bin_shift = shift;
+  in.fwd_tile_blocks = c->fwd_tile_blocks;
   for (int k = 0; k < 3; ++k) in.bg[k] = bg[k];
   return in;
 }
@@ -492,9 +495,11 @@ void enqueue_bin_and_raster_cut(rgs_ctx* c, size_t cap, int n, const ViewSet& vs
   in.cover = cover;
   in.cut_on = sc + SC_CUT + 2;
   in.flag = P_<uint8_t>(c->flag);
+  in.tile_flag = P_<uint8_t>(c->tile_flag);
   in.n_flagged = sc + SC_NFLAG;
   in.zneed = &cs->zneed;
   launch_fill_u32(P_<uint32_t>(c->flag), ((size_t)vs.total_bins + 3) / 4, 0u, st);
+  launch_fill_u32(P_<uint32_t>(c->tile_flag), ((size_t)vs.total_tiles + 3) / 4, 0u, st);
   stage_begin(c, S_FWD);
   launch_raster_fwd(in, 3, rgb_out, P_<float>(c->final_t), P_<int32_t>(c->last), st);
   stage_end(c, S_FWD);
@@ -656,7 +661,7 @@ void rgs_destroy(rgs_ctx* c) {
                  &c->dens_cnt, &c->dens_W, &c->dens_packed, &c->dens_prefix, &c->dens_b32, &c->dens_b64,
                  &c->dens_t32, &c->dens_t64, &c->first_splat, &c->P, &c->P2, &c->M1, &c->M2, &c->gacc, &c->gden, &c->targets,
                  &c->adam_d, &c->api_acc9, &c->tile_order[0], &c->tile_order[1], &c->train_scr, &c->cover, &c->depth,
-                 &c->flag, &c->rbstart, &c->rbend, &c->pv2, &c->cntb, &c->cutst};
+                 &c->flag, &c->tile_flag, &c->rbstart, &c->rbend, &c->pv2, &c->cntb, &c->cutst};
   for (DBuf* b : all)
     if (b->p) (void)hipFree(b->p);
   if (c->h_scalars) (void)hipHostFree(c->h_scalars);
@@ -1413,10 +1418,13 @@ int rgs_step_image(rgs_ctx* c, float* rgb, size_t n_floats) {
   return 0;
 }
 
-int rgs_set_train_binning(rgs_ctx* c, int shift) {
+int rgs_set_train_binning(rgs_ctx* c, int shift, int fwd_tile_blocks) {
   if (!c) return fail(RGS_E_ARG, "null ctx");
   if (shift < 0 || shift > 1) return fail(RGS_E_ARG, "bin shift must be 0 or 1");
+  if (fwd_tile_blocks < 0 || fwd_tile_blocks > 1) return fail(RGS_E_ARG, "fwd_tile_blocks must be 0 or 1");
   c->bin_shift = shift;
+  c->fwd_tile_blocks = fwd_tile_blocks;
+  ++c->gen;  // a captured step holds the old launch shapes
   return 0;
 }
 

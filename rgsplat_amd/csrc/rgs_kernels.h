@@ -166,12 +166,14 @@ struct RasterIn {
   size_t acc_stride;  // backward accumulators: value k of entry vi at accum[k * acc_stride + vi]
   int total_bins;   // workgroups: one per bin (2^bin_shift x 2^bin_shift tiles)
   int bin_shift;
+  int fwd_tile_blocks;  // forward with bins: one workgroup per tile instead of per bin
   float bg[3];
   const uint8_t* cover;  // null, or per tile: composited by the reference (launch_project)
   // depth-cut binning (null when every list is whole)
   const uint32_t* cut_on;     // device word: 1 when a front list may be partial
-  uint8_t* flag;              // per tile: front list ended before all pixels saturated
-  uint32_t* n_flagged;        // number of flagged tiles
+  uint8_t* flag;              // per bin: a front list ended before all its pixels saturated
+  uint8_t* tile_flag;         // per tile: kept its raw state for the resumed forward
+  uint32_t* n_flagged;        // number of flagged workgroups
   uint32_t* zneed;            // CutState::zneed
   const uint32_t* bstart;     // back lists: ranges and values
   const uint32_t* bend;

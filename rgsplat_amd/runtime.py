@@ -79,7 +79,7 @@ SIGNATURES = {
     "rgs_step_stats": [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                        ctypes.POINTER(ctypes.c_uint64)],
     "rgs_set_depth_cut": [_vp, _i, _f],
-    "rgs_set_train_binning": [_vp, _i],
+    "rgs_set_train_binning": [_vp, _i, _i],
     "rgs_step_image": [_vp, _vp, _sz],
     "rgs_cut_stats": [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                       ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_float)],
@@ -317,9 +317,10 @@ class Context:
         _check(lib().rgs_step_stats(self.h, ctypes.byref(k), ctypes.byref(f), ctypes.byref(b)))
         return k.value, f.value, b.value
 
-    def set_train_binning(self, shift: int):
-        """train_step bins per 2^shift x 2^shift tiles (0 or 1, default 1)."""
-        _check(lib().rgs_set_train_binning(self.h, int(shift)))
+    def set_train_binning(self, shift: int, fwd_tile_blocks: int = 1):
+        """train_step bins per 2^shift x 2^shift tiles (0 or 1, default 1);
+        forward workgroups per tile (1) or per bin (0)."""
+        _check(lib().rgs_set_train_binning(self.h, int(shift), int(fwd_tile_blocks)))
 
     def step_image(self, n_pixels: int) -> np.ndarray:
         """The last train_step's rendered images, flat (n_pixels, 3)."""

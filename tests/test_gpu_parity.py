@@ -349,22 +349,33 @@ def test_train_step_images(ctx):
     cfg = S.benchmark_train_config()
     bg = (0.2, 0.3, 0.1)
     ref = np.concatenate([oracle.render_color(g, c, bg).reshape(-1, 3) for c in cams])
-    imgs = []
-    for shift in (0, 1):
-        for mode, depth in [(1, 0.0), (2, 3.0)]:
+    # zero learning rates: the second step renders the scene of the first
+    frozen = dict(cfg, lr_position=0.0, lr_rotation=0.0, lr_scale=0.0, lr_opacity=0.0, lr_sh_dc=0.0,
+                  lr_sh_rest=0.0)
+    # Fixed cuts: 3.0 leaves most tiles unsaturated in front, 4.0 splits bins
+    # into saturated and unsaturated tiles. The adaptive cut acts from the
+    # second step on; that step renders the scene after the first step's
+    # quaternion renormalisation, so its images are compared with whole-list
+    # two-step runs.
+    imgs = {1: [], 2: []}
+    for shift, per_tile in [(0, 1), (1, 1), (1, 0)]:
+        for mode, depth, steps in [(1, 0.0, 1), (2, 3.0, 1), (2, 4.0, 1), (1, 0.0, 2), (0, 0.0, 2)]:
             c = runtime.Context(0)
-            c.set_train_binning(shift)
+            c.set_train_binning(shift, per_tile)
             c.set_depth_cut(mode, depth)
             c.scene_upload(g)
             c.set_views(cams, targets)
-            c.train_step([0, 1], cfg, bg=bg, use_graph=False)
-            imgs.append(c.step_image(len(ref)))
-            if mode == 2:
-                assert c.cut_stats()["flagged_tiles"] > 0
+            for _ in range(steps):
+                c.train_step([0, 1], frozen, bg=bg, use_graph=False)
+            imgs[steps].append(c.step_image(len(ref)))
+            st = c.cut_stats()
+            if mode != 1:
+                assert st["front_pairs"] < c.step_pairs() or st["flagged_tiles"] > 0
             c.close()
-    for im in imgs:
-        assert np.abs(im - ref).max() <= 1e-5
-        assert np.array_equal(im, imgs[0])
+    for group in imgs.values():
+        for im in group:
+            assert np.abs(im - ref).max() <= 1e-5
+            assert np.array_equal(im, group[0])
 
 
 @pytest.mark.parametrize("use_graph", [False, True])
