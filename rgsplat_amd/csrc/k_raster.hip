@@ -428,7 +428,9 @@ __global__ __launch_bounds__(BT * S * S) void k_raster_bwd(RasterIn in, const fl
   __shared__ float s_r[BT];
   __shared__ uint2 s_rect[BT];
   __shared__ uint32_t s_vi[BT];
-  __shared__ float s_g[NW][9][BT];
+  // rows padded to 65 floats: the eight lanes that store a splat's reduced
+  // sums (values 0-7 of one column) land in eight different banks
+  __shared__ float s_g[NW][9][BT + 1];
   __shared__ int s_top[NW];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // wave-uniform

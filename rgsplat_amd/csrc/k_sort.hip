@@ -400,11 +400,16 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(F in, size_t n, cons
   }
 }
 
+template <class T>
+__global__ void k_zero(T* p) {
+  *p = 0;
+}
+
 template <class T, class F>
 static void scan_impl(F in, size_t n, T* out, T* block_sums, T* total, hipStream_t st) {
   int nb = scan_nb(n);
   if (nb == 0) {
-    (void)hipMemsetAsync(total, 0, sizeof(T), st);
+    hipLaunchKernelGGL((k_zero<T>), dim3(1), dim3(1), 0, st, total);
     return;
   }
   hipLaunchKernelGGL((k_scan_reduce<T, F>), dim3(nb), dim3(SCAN_THREADS), 0, st, in, n, block_sums);
@@ -416,6 +421,67 @@ void scan_gathered_counts(const uint32_t* vals, const uint32_t* cnt, size_t n, u
                           uint32_t* block_sums, uint32_t* total, hipStream_t st) {
   scan_impl<uint32_t>(GatherCnt{vals, cnt}, n, out, block_sums, total, st);
 }
+// Entries below the cut (the depth-cut partition).
+struct BelowCut {
+  const uint32_t* keys;
+  const CutState* cs;
+  __device__ uint32_t operator()(size_t r) const { return keys[r] < cs->key ? 1u : 0u; }
+};
+// cnt[vals[r]] over the first *n_dev entries, 0 past them.
+struct GatherCntN {
+  const uint32_t* vals;
+  const uint32_t* cnt;
+  const uint32_t* n_dev;
+  __device__ uint32_t operator()(size_t r) const { return r < *n_dev ? cnt[vals[r]] : 0u; }
+};
+// Number of flagged bins in the bin rect of splat vi.
+__device__ __forceinline__ uint32_t flagged_bins(uint32_t vi, const uint32_t* cnt, const uint2* rect, int cap,
+                                                 const CamDev* cams, const uint8_t* flag, bool any) {
+  if (!cnt[vi]) return 0u;
+  const CamDev& cm = cams[vi / (uint32_t)cap];
+  const uint2 rr = bin_rect(rect[vi], cm.bin_shift);
+  const uint32_t x0 = rr.x & 0xffffu, y0 = rr.x >> 16, x1 = rr.y & 0xffffu, y1 = rr.y >> 16;
+  uint32_t k = 0;
+  for (uint32_t by = y0; by < y1; ++by) {
+    const uint8_t* row = flag + cm.bin_base + (size_t)by * cm.nbx;
+    for (uint32_t bx = x0; bx < x1; ++bx) {
+      k += row[bx];
+      if (any && k) return 1u;
+    }
+  }
+  return k;
+}
+// Back entries (first *n_dev) whose rect meets a flagged bin.
+struct TouchFlagged {
+  const uint32_t* vals;
+  const uint32_t* n_dev;
+  const uint32_t* n_flagged;
+  const uint32_t* cnt;
+  const uint2* rect;
+  int cap;
+  const CamDev* cams;
+  const uint8_t* flag;
+  __device__ uint32_t operator()(size_t r) const {
+    if (*n_flagged == 0 || r >= *n_dev) return 0u;
+    return flagged_bins(vals[r], cnt, rect, cap, cams, flag, true);
+  }
+};
+
+void scan_below_cut(const uint32_t* keys, size_t n, const CutState* cs, uint32_t* out, uint32_t* block_sums,
+                    uint32_t* total, hipStream_t st) {
+  scan_impl<uint32_t>(BelowCut{keys, cs}, n, out, block_sums, total, st);
+}
+void scan_gathered_counts_n(const uint32_t* vals, const uint32_t* cnt, size_t n, const uint32_t* n_dev,
+                            uint32_t* out, uint32_t* block_sums, uint32_t* total, hipStream_t st) {
+  scan_impl<uint32_t>(GatherCntN{vals, cnt, n_dev}, n, out, block_sums, total, st);
+}
+void scan_touch_flagged(const uint32_t* vals, size_t n, const uint32_t* n_dev, const uint32_t* n_flagged,
+                        const SplatArrays S, int cap, const CamDev* cams, const uint8_t* flag, uint32_t* out,
+                        uint32_t* block_sums, uint32_t* total, hipStream_t st) {
+  scan_impl<uint32_t>(TouchFlagged{vals, n_dev, n_flagged, S.cnt, S.rect, cap, cams, flag}, n, out, block_sums,
+                      total, st);
+}
+
 void scan_u32(const uint32_t* in, size_t n, uint32_t* out, uint32_t* block_sums, uint32_t* total, hipStream_t st) {
   scan_impl<uint32_t>(Plain<uint32_t>{in}, n, out, block_sums, total, st);
 }
@@ -456,7 +522,8 @@ __global__ __launch_bounds__(GP_THREADS) void k_gen_pairs(const uint32_t* __rest
                                                           uint32_t* __restrict__ pair_vals, size_t k_cap,
                                                           uint32_t* __restrict__ k_out, uint32_t* __restrict__ status,
                                                           const uint32_t* __restrict__ first_splat,
-                                                          uint32_t* __restrict__ hist, int nb_max) {
+                                                          uint32_t* __restrict__ hist, int nb_max,
+                                                          const uint32_t* __restrict__ n_dev) {
   __shared__ uint32_t s_off[GP_THREADS + 1];
   __shared__ uint32_t s_vi[GP_THREADS];
   __shared__ uint32_t s_org[GP_THREADS];  // tx0 | ty0 << 16
@@ -475,6 +542,8 @@ __global__ __launch_bounds__(GP_THREADS) void k_gen_pairs(const uint32_t* __rest
   const size_t p_lo = b * RS_TILE;
   if (p_lo >= K) return;
   const size_t p_hi = p_lo + RS_TILE < K ? p_lo + RS_TILE : K;
+  // sorted entries past *n_dev hold no valid splat (their offsets are total)
+  if (n_dev && (size_t)*n_dev < n_sorted) n_sorted = *n_dev;
   s_h[t] = 0;
   size_t rc = first_splat[b];
   for (;;) {
@@ -526,7 +595,7 @@ __global__ __launch_bounds__(GP_THREADS) void k_gen_pairs(const uint32_t* __rest
 void launch_gen_pairs(const uint32_t* sorted_vals, size_t n_sorted, const uint32_t* offsets, const uint32_t* total,
                       const SplatArrays S, int cap, const CamDev* cams, uint32_t* pair_keys, uint32_t* pair_vals,
                       size_t k_cap, uint32_t* k_out, uint32_t* status, uint32_t* first_splat, RadixScratch rs,
-                      hipStream_t st) {
+                      hipStream_t st, const uint32_t* n_dev) {
   if (n_sorted)
     hipLaunchKernelGGL(k_block_first, dim3((unsigned)((n_sorted + 255) / 256)), dim3(256), 0, st, offsets, n_sorted,
                        total, k_cap, first_splat);
@@ -534,44 +603,66 @@ void launch_gen_pairs(const uint32_t* sorted_vals, size_t n_sorted, const uint32
   if (nb == 0) nb = 1;
   hipLaunchKernelGGL(k_gen_pairs, dim3(xcd_grid(nb)), dim3(GP_THREADS), 0, st, sorted_vals, n_sorted, offsets, total,
                      S.cnt, S.rect, cap, cams, pair_keys, pair_vals, k_cap, k_out, status, first_splat, rs.hist,
-                     rs.nb_max);
+                     rs.nb_max, n_dev);
 }
 
 // ------------------------------------------------------- depth-cut binning
-// One block: S = lower bound of the cut key in the sorted keys by a 256-ary
-// search (three rounds for 4M keys), then the front pair count.
-__global__ __launch_bounds__(256) void k_cut_begin(const uint32_t* __restrict__ keys, size_t n,
-                                                   const uint32_t* __restrict__ offsets,
-                                                   const uint32_t* __restrict__ total, const CutState* __restrict__ cs,
-                                                   uint32_t* __restrict__ out) {
-  const uint32_t key = cs->key;
-  size_t lo = 0, hi = n;  // keys[lo - 1] < key <= keys[hi]: S in [lo, hi]
-  while (lo < hi) {
-    const size_t step = (hi - lo + 255) / 256;
-    const size_t p = lo + (size_t)threadIdx.x * step;
-    const int c = __syncthreads_count(p < hi && keys[p] < key);
-    if (c == 0) break;  // keys[lo] >= key: S = lo
-    const size_t nlo = lo + (size_t)(c - 1) * step + 1;
-    const size_t nhi = lo + (size_t)c * step;
-    lo = nlo;
-    hi = nhi < hi ? nhi : hi;
+// Stable partition of the projected (key, value) entries at the cut: entry r
+// goes to the front arrays at off[r] (the exclusive count of front entries
+// before it) or to the back arrays at r - off[r].
+__global__ __launch_bounds__(256) void k_partition(const uint32_t* __restrict__ keys,
+                                                   const uint32_t* __restrict__ vals, size_t n,
+                                                   const uint32_t* __restrict__ off, const CutState* __restrict__ cs,
+                                                   const uint32_t* __restrict__ n_front, uint32_t* __restrict__ kf,
+                                                   uint32_t* __restrict__ vf, uint32_t* __restrict__ kb,
+                                                   uint32_t* __restrict__ vb, uint32_t* __restrict__ n_back,
+                                                   uint32_t* __restrict__ cut_on) {
+  const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r == 0) {
+    *n_back = (uint32_t)(n - *n_front);
+    *cut_on = cs->key != 0xffffffffu ? 1u : 0u;
   }
-  if (threadIdx.x == 0) {
-    const uint32_t tot = *total;
-    const uint32_t kf = lo < n ? offsets[lo] : tot;
-    out[0] = (uint32_t)lo;
-    out[1] = kf;
-    out[2] = kf < tot ? 1u : 0u;
+  if (r >= n) return;
+  const uint32_t k = keys[r], v = vals[r], o = off[r];
+  if (k < cs->key) {
+    kf[o] = k;
+    vf[o] = v;
+  } else {
+    kb[r - o] = k;
+    vb[r - o] = v;
   }
 }
 
-void launch_cut_begin(const uint32_t* sorted_keys, size_t n, const uint32_t* offsets, const uint32_t* total,
-                      const CutState* cs, uint32_t* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_cut_begin, dim3(1), dim3(256), 0, st, sorted_keys, n, offsets, total, cs, out);
+void launch_partition(const uint32_t* keys, const uint32_t* vals, size_t n, const uint32_t* off, const CutState* cs,
+                      const uint32_t* n_front, uint32_t* kf, uint32_t* vf, uint32_t* kb, uint32_t* vb,
+                      uint32_t* n_back, uint32_t* cut_on, hipStream_t st) {
+  hipLaunchKernelGGL(k_partition, dim3((unsigned)(((n ? n : 1) + 255) / 256)), dim3(256), 0, st, keys, vals, n, off,
+                     cs, n_front, kf, vf, kb, vb, n_back, cut_on);
+}
+
+// Keeps the entries that the scan of 0/1 selections off[] (total *total)
+// selected: entry r is kept iff off[r + 1] > off[r].
+__global__ __launch_bounds__(256) void k_compact(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                                 size_t n, const uint32_t* __restrict__ off,
+                                                 const uint32_t* __restrict__ total, uint32_t* __restrict__ kc,
+                                                 uint32_t* __restrict__ vc) {
+  const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n || *total == 0) return;
+  const uint32_t o = off[r];
+  const uint32_t o1 = r + 1 < n ? off[r + 1] : *total;
+  if (o1 == o) return;
+  kc[o] = keys[r];
+  vc[o] = vals[r];
+}
+
+void launch_compact(const uint32_t* keys, const uint32_t* vals, size_t n, const uint32_t* off, const uint32_t* total,
+                    uint32_t* kc, uint32_t* vc, hipStream_t st) {
+  hipLaunchKernelGGL(k_compact, dim3((unsigned)(((n ? n : 1) + 255) / 256)), dim3(256), 0, st, keys, vals, n, off,
+                     total, kc, vc);
 }
 
 __global__ __launch_bounds__(256) void k_back_count(const uint32_t* __restrict__ sorted_vals, size_t n,
-                                                    const uint32_t* __restrict__ cut_out,
+                                                    const uint32_t* __restrict__ n_dev,
                                                     const uint32_t* __restrict__ n_flagged,
                                                     const uint32_t* __restrict__ cnt, const uint2* __restrict__ rect,
                                                     int cap, const CamDev* __restrict__ cams,
@@ -579,35 +670,24 @@ __global__ __launch_bounds__(256) void k_back_count(const uint32_t* __restrict__
   const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   uint32_t k = 0;
-  if (*n_flagged != 0 && cut_out[2] != 0 && r >= cut_out[0]) {
-    const uint32_t vi = sorted_vals[r];
-    if (cnt[vi] != 0) {
-      const CamDev& cm = cams[vi / (uint32_t)cap];
-      const uint2 rr = bin_rect(rect[vi], cm.bin_shift);
-      const uint32_t x0 = rr.x & 0xffffu, y0 = rr.x >> 16, x1 = rr.y & 0xffffu, y1 = rr.y >> 16;
-      for (uint32_t ty = y0; ty < y1; ++ty) {
-        const uint8_t* row = flag + cm.bin_base + (size_t)ty * cm.nbx;
-        for (uint32_t tx = x0; tx < x1; ++tx) k += row[tx];
-      }
-    }
-  }
+  if (*n_flagged != 0 && r < *n_dev) k = flagged_bins(sorted_vals[r], cnt, rect, cap, cams, flag, false);
   count[r] = k;
 }
 
-void launch_back_count(const uint32_t* sorted_vals, size_t n, const uint32_t* cut_out, const uint32_t* n_flagged,
+void launch_back_count(const uint32_t* sorted_vals, size_t n, const uint32_t* n_dev, const uint32_t* n_flagged,
                        const SplatArrays S, int cap, const CamDev* cams, const uint8_t* flag, uint32_t* count,
                        hipStream_t st) {
   if (!n) return;
-  hipLaunchKernelGGL(k_back_count, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, sorted_vals, n, cut_out,
+  hipLaunchKernelGGL(k_back_count, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, sorted_vals, n, n_dev,
                      n_flagged, S.cnt, S.rect, cap, cams, flag, count);
 }
 
-// One thread per sorted entry with back pairs: its flagged tiles in row-major
-// order at its offset.
+// One thread per sorted back entry with back pairs: its flagged bins in
+// row-major order at its offset.
 __global__ __launch_bounds__(256) void k_back_gen(const uint32_t* __restrict__ sorted_vals, size_t n,
+                                                  const uint32_t* __restrict__ n_dev,
                                                   const uint32_t* __restrict__ offsets,
                                                   const uint32_t* __restrict__ total,
-                                                  const uint32_t* __restrict__ cut_out,
                                                   const uint32_t* __restrict__ n_flagged,
                                                   const uint2* __restrict__ rect, int cap,
                                                   const CamDev* __restrict__ cams, const uint8_t* __restrict__ flag,
@@ -620,7 +700,7 @@ __global__ __launch_bounds__(256) void k_back_gen(const uint32_t* __restrict__ s
     *k_out = (size_t)tot < k_cap ? tot : (uint32_t)k_cap;
     if ((size_t)tot > k_cap) atomicOr(status, ST_PAIR_OVERFLOW);
   }
-  if (r >= n || *n_flagged == 0 || cut_out[2] == 0 || r < cut_out[0]) return;
+  if (r >= n || *n_flagged == 0 || r >= *n_dev) return;
   size_t o = offsets[r];
   const size_t o_end = r + 1 < n ? offsets[r + 1] : tot;
   if (o == o_end) return;
@@ -641,13 +721,12 @@ __global__ __launch_bounds__(256) void k_back_gen(const uint32_t* __restrict__ s
   }
 }
 
-void launch_back_gen(const uint32_t* sorted_vals, size_t n, const uint32_t* offsets, const uint32_t* total,
-                     const uint32_t* cut_out, const uint32_t* n_flagged, const SplatArrays S, int cap,
+void launch_back_gen(const uint32_t* sorted_vals, size_t n, const uint32_t* n_dev, const uint32_t* offsets,
+                     const uint32_t* total, const uint32_t* n_flagged, const SplatArrays S, int cap,
                      const CamDev* cams, const uint8_t* flag, uint32_t* pair_keys, uint32_t* pair_vals, size_t k_cap,
                      uint32_t* k_out, uint32_t* status, hipStream_t st) {
-  hipLaunchKernelGGL(k_back_gen, dim3((unsigned)(((n ? n : 1) + 255) / 256)), dim3(256), 0, st,
-                     sorted_vals, n, offsets, total, cut_out, n_flagged, S.rect, cap, cams, flag, pair_keys, pair_vals,
-                     k_cap, k_out, status);
+  hipLaunchKernelGGL(k_back_gen, dim3((unsigned)(((n ? n : 1) + 255) / 256)), dim3(256), 0, st, sorted_vals, n, n_dev,
+                     offsets, total, n_flagged, S.rect, cap, cams, flag, pair_keys, pair_vals, k_cap, k_out, status);
 }
 
 __global__ void k_cut_end(CutState* cs) {

@@ -183,11 +183,12 @@ struct ViewSet {
   }
 };
 
-// Device scalars of a render or step (cleared at its start). SC_CUT: three
-// words from launch_cut_begin (cut position, front pairs, cut active).
+// Device scalars of a render or step (cleared at its start). Depth cut:
+// front / back entry counts, cut active, back pairs, flagged workgroups and
+// the back entries that meet a flagged bin.
 enum {
   SC_DEPTH_N = 0, SC_TOTAL = 1, SC_K = 2, SC_STATUS = 3, SC_NOUT = 4,
-  SC_CUT = 5, SC_BTOTAL = 8, SC_KB = 9, SC_NFLAG = 10, SC_COUNT = 16
+  SC_NF = 5, SC_NB = 6, SC_CUTON = 7, SC_BTOTAL = 8, SC_KB = 9, SC_NFLAG = 10, SC_NC = 11, SC_COUNT = 16
 };
 
 }  // namespace
@@ -212,6 +213,7 @@ struct rgs_ctx {
   const uint32_t* pair_vals = nullptr;
   // depth-cut binning of the training step
   DBuf flag, tile_flag, rbstart, rbend, pv2, cntb, cutst;
+  DBuf part, kB, vB, kC[2], vC[2];  // cut partition: front offsets, back entries, their flagged subset
   const uint32_t* back_vals = nullptr;
   int cut_mode = 0;              // CutState::mode
   float cut_depth = 0.0f;        // mode 2: fixed cut depth
@@ -365,6 +367,13 @@ int ensure_render(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, size_t k_cap
   CHK(ensure(c, c->tile_flag, ((size_t)vs.total_tiles + 3) / 4 * 4));
   CHK(ensure(c, c->pv2, k_cap * 4));
   CHK(ensure(c, c->cntb, dn * 4));
+  CHK(ensure(c, c->part, dn * 4));
+  CHK(ensure(c, c->kB, dn * 4));
+  CHK(ensure(c, c->vB, dn * 4));
+  for (int i = 0; i < 2; ++i) {
+    CHK(ensure(c, c->kC[i], dn * 4));
+    CHK(ensure(c, c->vC[i], dn * 4));
+  }
   CHK(ensure(c, c->first_splat, ((size_t)radix_nb(k_cap) + 1) * 4));
   CHK(ensure(c, c->rgb, vs.total_pix * 4 * 4));
   CHK(ensure(c, c->final_t, vs.total_pix * 4));
@@ -405,6 +414,38 @@ void enqueue_project(rgs_ctx* c, const float* P, size_t cap, int n, const ViewSe
   stage_begin(c, S_SCAN);
   scan_gathered_counts(vals[cur], P_<uint32_t>(c->cnt), dn, P_<uint32_t>(c->offsets), P_<uint32_t>(c->bsums),
                        sc + SC_TOTAL, st);
+  stage_end(c, S_SCAN);
+}
+
+// The training step's projection: the entries are split at the depth cut,
+// and only the front ones are depth-sorted and scanned here (the back ones
+// that meet a bin left unsaturated are sorted after the front raster).
+void enqueue_project_cut(rgs_ctx* c, const float* P, size_t cap, int n, const ViewSet& vs, uint8_t* cover) {
+  hipStream_t st = c->st;
+  uint32_t* sc = P_<uint32_t>(c->scalars);
+  int nv = (int)vs.cams.size();
+  size_t dn = (size_t)nv * n;
+  const CutState* cs = P_<CutState>(c->cutst);
+  if (cover) launch_fill_u32(reinterpret_cast<uint32_t*>(cover), ((size_t)vs.total_tiles + 3) / 4, 0u, st);
+  stage_begin(c, S_PROJECT);
+  launch_project(P, (int)cap, n, P_<CamDev>(c->cams_d), nv, splats(c), P_<uint32_t>(c->dk[0]),
+                 P_<uint32_t>(c->dv[0]), sc + SC_STATUS, 1, cover, st);
+  stage_end(c, S_PROJECT);
+  RadixScratch rs{P_<uint32_t>(c->hist), P_<uint32_t>(c->rowtot), c->hist_nb};
+  stage_begin(c, S_DSORT);
+  scan_below_cut(P_<uint32_t>(c->dk[0]), dn, cs, P_<uint32_t>(c->part), P_<uint32_t>(c->bsums), sc + SC_NF, st);
+  launch_partition(P_<uint32_t>(c->dk[0]), P_<uint32_t>(c->dv[0]), dn, P_<uint32_t>(c->part), cs, sc + SC_NF,
+                   P_<uint32_t>(c->dk[1]), P_<uint32_t>(c->dv[1]), P_<uint32_t>(c->kB), P_<uint32_t>(c->vB),
+                   sc + SC_NB, sc + SC_CUTON, st);
+  uint32_t* keys[2] = {P_<uint32_t>(c->dk[1]), P_<uint32_t>(c->dk[0])};
+  uint32_t* vals[2] = {P_<uint32_t>(c->dv[1]), P_<uint32_t>(c->dv[0])};
+  int cur = radix_sort_pairs(keys, vals, sc + SC_NF, dn, 32, rs, st);
+  stage_end(c, S_DSORT);
+  c->sorted_vals = vals[cur];
+  c->sorted_keys = keys[cur];
+  stage_begin(c, S_SCAN);
+  scan_gathered_counts_n(vals[cur], P_<uint32_t>(c->cnt), dn, sc + SC_NF, P_<uint32_t>(c->offsets),
+                         P_<uint32_t>(c->bsums), sc + SC_TOTAL, st);
   stage_end(c, S_SCAN);
 }
 
@@ -479,10 +520,9 @@ void enqueue_bin_and_raster_cut(rgs_ctx* c, size_t cap, int n, const ViewSet& vs
   RadixScratch rs{P_<uint32_t>(c->hist), P_<uint32_t>(c->rowtot), c->hist_nb};
   CutState* cs = P_<CutState>(c->cutst);
   stage_begin(c, S_GEN);
-  launch_cut_begin(c->sorted_keys, dn, P_<uint32_t>(c->offsets), sc + SC_TOTAL, cs, sc + SC_CUT, st);
-  launch_gen_pairs(c->sorted_vals, dn, P_<uint32_t>(c->offsets), sc + SC_CUT + 1, splats(c), (int)cap,
+  launch_gen_pairs(c->sorted_vals, dn, P_<uint32_t>(c->offsets), sc + SC_TOTAL, splats(c), (int)cap,
                    P_<CamDev>(c->cams_d), P_<uint32_t>(c->pk[0]), P_<uint32_t>(c->pv[0]), c->k_cap, sc + SC_K,
-                   sc + SC_STATUS, P_<uint32_t>(c->first_splat), rs, st);
+                   sc + SC_STATUS, P_<uint32_t>(c->first_splat), rs, st, sc + SC_NF);
   stage_end(c, S_GEN);
   uint32_t* keys[2] = {P_<uint32_t>(c->pk[0]), P_<uint32_t>(c->pk[1])};
   uint32_t* vals[2] = {P_<uint32_t>(c->pv[0]), P_<uint32_t>(c->pv[1])};
@@ -493,7 +533,7 @@ void enqueue_bin_and_raster_cut(rgs_ctx* c, size_t cap, int n, const ViewSet& vs
   c->pair_vals = vals[cur];
   RasterIn in = raster_in(c, vs, bg);
   in.cover = cover;
-  in.cut_on = sc + SC_CUT + 2;
+  in.cut_on = sc + SC_CUTON;
   in.flag = P_<uint8_t>(c->flag);
   in.tile_flag = P_<uint8_t>(c->tile_flag);
   in.n_flagged = sc + SC_NFLAG;
@@ -506,12 +546,19 @@ void enqueue_bin_and_raster_cut(rgs_ctx* c, size_t cap, int n, const ViewSet& vs
   // back lists of the flagged tiles: keys ping-pong in the front sort's key
   // buffers, values in its spare value buffer and pv2
   stage_begin(c, S_BACK);
+  // the back entries that meet a flagged bin, in depth order
+  uint32_t* kC[2] = {P_<uint32_t>(c->kC[0]), P_<uint32_t>(c->kC[1])};
+  uint32_t* vC[2] = {P_<uint32_t>(c->vC[0]), P_<uint32_t>(c->vC[1])};
+  scan_touch_flagged(P_<uint32_t>(c->vB), dn, sc + SC_NB, sc + SC_NFLAG, splats(c), (int)cap, P_<CamDev>(c->cams_d),
+                     P_<uint8_t>(c->flag), P_<uint32_t>(c->part), P_<uint32_t>(c->bsums), sc + SC_NC, st);
+  launch_compact(P_<uint32_t>(c->kB), P_<uint32_t>(c->vB), dn, P_<uint32_t>(c->part), sc + SC_NC, kC[0], vC[0], st);
+  const int curc = radix_sort_pairs(kC, vC, sc + SC_NC, dn, 32, rs, st);
   uint32_t* cntb = P_<uint32_t>(c->cntb);
-  launch_back_count(c->sorted_vals, dn, sc + SC_CUT, sc + SC_NFLAG, splats(c), (int)cap, P_<CamDev>(c->cams_d),
+  launch_back_count(vC[curc], dn, sc + SC_NC, sc + SC_NFLAG, splats(c), (int)cap, P_<CamDev>(c->cams_d),
                     P_<uint8_t>(c->flag), cntb, st);
   scan_u32(cntb, dn, cntb, P_<uint32_t>(c->bsums), sc + SC_BTOTAL, st);
   uint32_t* bvals[2] = {vals[cur ^ 1], P_<uint32_t>(c->pv2)};
-  launch_back_gen(c->sorted_vals, dn, cntb, sc + SC_BTOTAL, sc + SC_CUT, sc + SC_NFLAG, splats(c), (int)cap,
+  launch_back_gen(vC[curc], dn, sc + SC_NC, cntb, sc + SC_BTOTAL, sc + SC_NFLAG, splats(c), (int)cap,
                   P_<CamDev>(c->cams_d), P_<uint8_t>(c->flag), keys[0], bvals[0], c->k_cap, sc + SC_KB,
                   sc + SC_STATUS, st);
   TileRanges trb{P_<uint32_t>(c->rbstart), P_<uint32_t>(c->rbend), (uint32_t)vs.total_bins};
@@ -661,7 +708,8 @@ void rgs_destroy(rgs_ctx* c) {
                  &c->dens_cnt, &c->dens_W, &c->dens_packed, &c->dens_prefix, &c->dens_b32, &c->dens_b64,
                  &c->dens_t32, &c->dens_t64, &c->first_splat, &c->P, &c->P2, &c->M1, &c->M2, &c->gacc, &c->gden, &c->targets,
                  &c->adam_d, &c->api_acc9, &c->tile_order[0], &c->tile_order[1], &c->train_scr, &c->cover, &c->depth,
-                 &c->flag, &c->tile_flag, &c->rbstart, &c->rbend, &c->pv2, &c->cntb, &c->cutst};
+                 &c->flag, &c->tile_flag, &c->rbstart, &c->rbend, &c->pv2, &c->cntb, &c->cutst, &c->part,
+                 &c->kB, &c->vB, &c->kC[0], &c->kC[1], &c->vC[0], &c->vC[1]};
   for (DBuf* b : all)
     if (b->p) (void)hipFree(b->p);
   if (c->h_scalars) (void)hipHostFree(c->h_scalars);
@@ -1083,7 +1131,7 @@ static void enqueue_step_rest(rgs_ctx* c, const ViewSet& vs, const float bg[3], 
 // Enqueues one whole training step on the context's stream (no host sync).
 static void enqueue_step(rgs_ctx* c, const ViewSet& vs, const float bg[3], float lambda) {
   launch_fill_u32(P_<uint32_t>(c->scalars), SC_COUNT, 0u, c->st);
-  enqueue_project(c, P_<float>(c->P), c->cap, (int)c->n, vs, true, step_cover(c, bg));
+  enqueue_project_cut(c, P_<float>(c->P), c->cap, (int)c->n, vs, step_cover(c, bg));
   enqueue_step_rest(c, vs, bg, lambda);
 }
 
@@ -1165,7 +1213,7 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
           (void)hipGetLastError();
           c->prof_graph_failed = true;
           use_graph = 0;
-          enqueue_step_rest(c, vs, bg, lambda);
+          enqueue_step(c, vs, bg, lambda);
         } else if (e != hipSuccess) {
           return fail(RGS_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
         } else {
@@ -1175,7 +1223,7 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
           launched_graph = true;
         }
       } else {
-        enqueue_step_rest(c, vs, bg, lambda);
+        enqueue_step(c, vs, bg, lambda);
       }
     } else {
       HIPCHK(hipMemcpyAsync(c->cams_d.p, c->h_cams, vs.cams.size() * sizeof(CamDev), hipMemcpyHostToDevice, c->st));

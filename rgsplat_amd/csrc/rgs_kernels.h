@@ -104,11 +104,12 @@ __device__ __forceinline__ uint2 bin_rect(uint2 r, int sh) {
 // pairs) per block, and the first radix pass's per-tile digit histogram
 // (hist[d * nb_max + b] for the low 8 key bits). K = min(total, k_cap) goes to
 // *k_out; ST_PAIR_OVERFLOW is raised when total > k_cap. first_splat needs
-// radix_nb(k_cap) entries.
+// radix_nb(k_cap) entries. n_dev (optional): only the first *n_dev sorted
+// entries are valid.
 void launch_gen_pairs(const uint32_t* sorted_vals, size_t n_sorted, const uint32_t* offsets, const uint32_t* total,
                       const SplatArrays S, int cap, const CamDev* cams, uint32_t* pair_keys, uint32_t* pair_vals,
                       size_t k_cap, uint32_t* k_out, uint32_t* status, uint32_t* first_splat, RadixScratch rs,
-                      hipStream_t st);
+                      hipStream_t st, const uint32_t* n_dev = nullptr);
 // Tile sort of the pairs from launch_gen_pairs (whose first-pass histogram is
 // already written). Only the values of the result are written (the keys are
 // implied by the ranges); the ranges come out of the last pass.
@@ -130,21 +131,37 @@ struct CutState {       // persistent device words
   uint32_t mode;        // 0 adaptive, 1 no cut, 2 fixed key
   uint32_t pad;
 };
-// out[0] = S, the number of sorted depth keys below the cut; out[1] = the
-// front pair count (offsets[S], *total when S = n); out[2] = 1 when a
-// front list may be partial (front count < *total).
-void launch_cut_begin(const uint32_t* sorted_keys, size_t n, const uint32_t* offsets, const uint32_t* total,
-                      const CutState* cs, uint32_t* out, hipStream_t st);
-// Back-pair count of every sorted entry r: for r >= S the tiles of the
-// splat's rect whose flag is set, else 0 (all 0 when no tile is flagged).
-void launch_back_count(const uint32_t* sorted_vals, size_t n, const uint32_t* cut_out, const uint32_t* n_flagged,
+// Exclusive count of the entries below the cut (keys[r] < cs->key); the
+// number of front entries goes to *total.
+void scan_below_cut(const uint32_t* keys, size_t n, const CutState* cs, uint32_t* out, uint32_t* block_sums,
+                    uint32_t* total, hipStream_t st);
+// Stable partition at the cut into front (kf, vf: *n_front entries) and back
+// (kb, vb: *n_back = n - *n_front entries) arrays; *cut_on = 1 when a cut is
+// set (a front list may then be partial).
+void launch_partition(const uint32_t* keys, const uint32_t* vals, size_t n, const uint32_t* off, const CutState* cs,
+                      const uint32_t* n_front, uint32_t* kf, uint32_t* vf, uint32_t* kb, uint32_t* vb,
+                      uint32_t* n_back, uint32_t* cut_on, hipStream_t st);
+// Exclusive scan of cnt[vals[r]] over the first *n_dev of n entries (0 past them).
+void scan_gathered_counts_n(const uint32_t* vals, const uint32_t* cnt, size_t n, const uint32_t* n_dev,
+                            uint32_t* out, uint32_t* block_sums, uint32_t* total, hipStream_t st);
+// Exclusive count of the back entries (first *n_dev) whose rect meets a
+// flagged bin (all 0 when no bin is flagged).
+void scan_touch_flagged(const uint32_t* vals, size_t n, const uint32_t* n_dev, const uint32_t* n_flagged,
+                        const SplatArrays S, int cap, const CamDev* cams, const uint8_t* flag, uint32_t* out,
+                        uint32_t* block_sums, uint32_t* total, hipStream_t st);
+// Keeps the entries a 0/1 scan (off, *total) selected, in order.
+void launch_compact(const uint32_t* keys, const uint32_t* vals, size_t n, const uint32_t* off, const uint32_t* total,
+                    uint32_t* kc, uint32_t* vc, hipStream_t st);
+// Back-pair count of the first *n_dev sorted back entries: flagged bins in
+// the splat's bin rect (0 past them and when nothing is flagged).
+void launch_back_count(const uint32_t* sorted_vals, size_t n, const uint32_t* n_dev, const uint32_t* n_flagged,
                        const SplatArrays S, int cap, const CamDev* cams, const uint8_t* flag, uint32_t* count,
                        hipStream_t st);
 // Writes the back pairs at the exclusive offsets of those counts (total in
 // *total); K = min(total, k_cap) goes to *k_out, ST_PAIR_OVERFLOW when
 // total > k_cap.
-void launch_back_gen(const uint32_t* sorted_vals, size_t n, const uint32_t* offsets, const uint32_t* total,
-                     const uint32_t* cut_out, const uint32_t* n_flagged, const SplatArrays S, int cap,
+void launch_back_gen(const uint32_t* sorted_vals, size_t n, const uint32_t* n_dev, const uint32_t* offsets,
+                     const uint32_t* total, const uint32_t* n_flagged, const SplatArrays S, int cap,
                      const CamDev* cams, const uint8_t* flag, uint32_t* pair_keys, uint32_t* pair_vals, size_t k_cap,
                      uint32_t* k_out, uint32_t* status, hipStream_t st);
 // Next cut from this step's saturation depths (mode 0); clears zneed.
