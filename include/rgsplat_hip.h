@@ -165,12 +165,16 @@ int rgs_step_stats(rgs_ctx* ctx, uint64_t* n_pairs, uint64_t* fwd_evals, uint64_
  * the tiles those lists leave unsaturated get the pairs behind the cut.
  * mode 0 (default): the cut is the previous step's saturation depth + 5%
  * (none on the first step); 1: no cut (whole lists); 2: cut at the fixed
- * camera-space depth `depth` (> 0). rgs_cut_stats reports the last step's
- * front and back pair counts, how many tiles needed their back lists, and
- * the cut depth the next step will use (+inf: none); pointers may be NULL. */
+ * camera-space depth `depth` (> 0). While the previous step needed no back
+ * list, a step runs without the back-list stage (a captured graph of its
+ * own); if it then leaves a bin unsaturated, the device skips its update and
+ * the step is re-run with the stage. rgs_cut_stats reports the last step's
+ * front and back pair counts, how many workgroups needed back lists, the cut
+ * depth the next step will use (+inf: none) and how many steps were re-run
+ * so far; pointers may be NULL. */
 int rgs_set_depth_cut(rgs_ctx* ctx, int mode, float depth);
 int rgs_cut_stats(rgs_ctx* ctx, uint64_t* front_pairs, uint64_t* back_pairs, uint64_t* flagged_tiles,
-                  float* next_cut);
+                  float* next_cut, uint64_t* reruns);
 /* The images the last rgs_train_step rendered (its views in order, h x w x 3
  * each; n_floats >= 3 x total pixels). */
 int rgs_step_image(rgs_ctx* ctx, float* rgb, size_t n_floats);

@@ -317,7 +317,7 @@ __global__ __launch_bounds__(256) void k_train_geometry(const float* __restrict_
                                                         float* __restrict__ scr, const uint32_t* __restrict__ status) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  if (*status & (ST_PAIR_OVERFLOW | ST_NAN_DEPTH)) return;
+  if (*status & (ST_PAIR_OVERFLOW | ST_NAN_DEPTH | ST_NEED_BACK)) return;
   SoaLoad L{P, (size_t)cap, (size_t)i};
   float g[11];  // flat parameters 0-10
 #pragma unroll
@@ -371,7 +371,7 @@ __global__ __launch_bounds__(256) void k_train_adam(float* __restrict__ P, float
                                                     const uint32_t* __restrict__ status) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  if (*status & (ST_PAIR_OVERFLOW | ST_NAN_DEPTH)) return;
+  if (*status & (ST_PAIR_OVERFLOW | ST_NAN_DEPTH | ST_NEED_BACK)) return;
   const AdamConsts k_ = *k_dev;
   SoaLoad L{P, (size_t)cap, (size_t)i};
   uint64_t vis = 0;

@@ -729,6 +729,14 @@ void launch_back_gen(const uint32_t* sorted_vals, size_t n, const uint32_t* n_de
                      offsets, total, n_flagged, S.rect, cap, cams, flag, pair_keys, pair_vals, k_cap, k_out, status);
 }
 
+__global__ void k_need_back(const uint32_t* n_flagged, uint32_t* status) {
+  if (*n_flagged) atomicOr(status, ST_NEED_BACK);
+}
+
+void launch_need_back(const uint32_t* n_flagged, uint32_t* status, hipStream_t st) {
+  hipLaunchKernelGGL(k_need_back, dim3(1), dim3(1), 0, st, n_flagged, status);
+}
+
 __global__ void k_cut_end(CutState* cs) {
   if (cs->mode == 0) cs->key = cs->zneed ? f2u(__uint_as_float(cs->zneed) * 1.05f) : 0xffffffffu;
   cs->zneed = 0;

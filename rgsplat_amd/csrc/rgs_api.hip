@@ -219,6 +219,7 @@ struct rgs_ctx {
   float cut_depth = 0.0f;        // mode 2: fixed cut depth
   uint64_t last_front_pairs = 0, last_back_pairs = 0, last_flagged = 0;
   bool last_cut = false;         // the last render was a training step (two-part lists)
+  bool last_back_valid = false;  // ... that ran the back-list stage
   uint32_t* h_scalars = nullptr;  // pinned
   float* h_loss = nullptr;        // pinned, 64 entries
 
@@ -252,10 +253,15 @@ struct rgs_ctx {
   float last_bg[3] = {0, 0, 0};
 
   // captured step
-  hipGraphExec_t gexec = nullptr;
-  std::string gkey;
+  // captured step, per variant: [1] with the depth cut's back lists, [0]
+  // without (used while the previous step flagged no bin; a step that does
+  // flag one is void and re-run with [1])
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  std::string gkey[2];
   uint64_t gen = 0;        // bumped on every device (re)allocation
-  uint64_t graph_gen = 0;  // gen at capture time
+  uint64_t graph_gen[2] = {0, 0};  // gen at capture time
+  bool need_back = true;   // the next step runs the back-list stage
+  uint64_t back_reruns = 0;  // steps re-run because a bin needed its back list
   DBuf api_acc9;           // raster accumulators of the reference-API paths
   DBuf tile_order[2];      // raster schedule per bin shift: slot -> global bin id
   DBuf train_scr;          // per-Gaussian scratch between geometry and Adam
@@ -297,11 +303,13 @@ inline void stage_end(rgs_ctx* c, int s) {
 }
 
 void drop_graph(rgs_ctx* c) {
-  if (c->gexec) {
-    (void)hipGraphExecDestroy(c->gexec);
-    c->gexec = nullptr;
+  for (int v = 0; v < 2; ++v) {
+    if (c->gexec[v]) {
+      (void)hipGraphExecDestroy(c->gexec[v]);
+      c->gexec[v] = nullptr;
+    }
+    c->gkey[v].clear();
   }
-  c->gkey.clear();
 }
 
 // Raster schedule for the tiles of vs: views in order, each view walked in
@@ -513,7 +521,7 @@ int write_cut_state(rgs_ctx* c) {
 // counts turn out to be (no host sync): with nothing flagged the back stage
 // reduces to a count pass, a scan and empty launches.
 void enqueue_bin_and_raster_cut(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, const float bg[3], float* rgb_out,
-                                const uint8_t* cover) {
+                                const uint8_t* cover, bool with_back) {
   hipStream_t st = c->st;
   uint32_t* sc = P_<uint32_t>(c->scalars);
   size_t dn = vs.cams.size() * (size_t)n;
@@ -543,6 +551,16 @@ void enqueue_bin_and_raster_cut(rgs_ctx* c, size_t cap, int n, const ViewSet& vs
   stage_begin(c, S_FWD);
   launch_raster_fwd(in, 3, rgb_out, P_<float>(c->final_t), P_<int32_t>(c->last), st);
   stage_end(c, S_FWD);
+  if (!with_back) {
+    // no back-list stage: a flagged bin voids the step (Adam is skipped and
+    // the host re-runs it with the stage)
+    stage_begin(c, S_BACK);
+    launch_need_back(sc + SC_NFLAG, sc + SC_STATUS, st);
+    launch_cut_end(cs, st);
+    stage_end(c, S_BACK);
+    c->back_vals = nullptr;
+    return;
+  }
   // back lists of the flagged tiles: keys ping-pong in the front sort's key
   // buffers, values in its spare value buffer and pv2
   stage_begin(c, S_BACK);
@@ -1105,20 +1123,22 @@ static uint8_t* step_cover(rgs_ctx* c, const float bg[3]) {
   return (bg[0] != 0.0f || bg[1] != 0.0f || bg[2] != 0.0f) ? P_<uint8_t>(c->cover) : nullptr;
 }
 
-static void enqueue_step_rest(rgs_ctx* c, const ViewSet& vs, const float bg[3], float lambda) {
+static void enqueue_step_rest(rgs_ctx* c, const ViewSet& vs, const float bg[3], float lambda, bool with_back) {
   int n = (int)c->n;
   size_t cap = c->cap;
   uint32_t* sc = P_<uint32_t>(c->scalars);
-  enqueue_bin_and_raster_cut(c, cap, n, vs, bg, P_<float>(c->rgb), step_cover(c, bg));
+  enqueue_bin_and_raster_cut(c, cap, n, vs, bg, P_<float>(c->rgb), step_cover(c, bg), with_back);
   stage_begin(c, S_LOSS);
   enqueue_loss(c, vs, P_<float>(c->rgb), P_<float>(c->targets), lambda);
   stage_end(c, S_LOSS);
   stage_begin(c, S_BWD);
   RasterIn in = raster_in(c, vs, bg);
   in.acc_stride = vs.cams.size() * cap;
-  in.bstart = P_<uint32_t>(c->rbstart);
-  in.bend = P_<uint32_t>(c->rbend);
-  in.back_vals = c->back_vals;
+  if (with_back) {
+    in.bstart = P_<uint32_t>(c->rbstart);
+    in.bend = P_<uint32_t>(c->rbend);
+    in.back_vals = c->back_vals;
+  }
   launch_raster_bwd(in, P_<float>(c->final_t), P_<int32_t>(c->last), P_<float>(c->dl), P_<float>(c->accum), c->st);
   stage_end(c, S_BWD);
   stage_begin(c, S_ADAM);
@@ -1129,10 +1149,10 @@ static void enqueue_step_rest(rgs_ctx* c, const ViewSet& vs, const float bg[3], 
 }
 
 // Enqueues one whole training step on the context's stream (no host sync).
-static void enqueue_step(rgs_ctx* c, const ViewSet& vs, const float bg[3], float lambda) {
+static void enqueue_step(rgs_ctx* c, const ViewSet& vs, const float bg[3], float lambda, bool with_back) {
   launch_fill_u32(P_<uint32_t>(c->scalars), SC_COUNT, 0u, c->st);
   enqueue_project_cut(c, P_<float>(c->P), c->cap, (int)c->n, vs, step_cover(c, bg));
-  enqueue_step_rest(c, vs, bg, lambda);
+  enqueue_step_rest(c, vs, bg, lambda, with_back);
 }
 
 int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* cfg, const float bg[3],
@@ -1176,11 +1196,15 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
   if (c->profiling && c->prof_graph_failed) use_graph = 0;
   if (!c->st) use_graph = 0;  // no capture on the null stream
   if (c->profiling) key += "|prof";
-  for (int attempt = 0; attempt < 3; ++attempt) {
-    bool replay = use_graph && c->gexec && c->gkey == key && c->graph_gen == c->gen;
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    const int var = c->need_back ? 1 : 0;
+    bool replay = use_graph && c->gexec[var] && c->gkey[var] == key && c->graph_gen[var] == c->gen;
     bool launched_graph = replay;
     if (!replay) {
-      drop_graph(c);
+      if (c->gexec[var]) {
+        (void)hipGraphExecDestroy(c->gexec[var]);
+        c->gexec[var] = nullptr;
+      }
       // eager: exact pair count sizes the buffers, then the rest of the step
       CHK(ensure_render(c, cap, n, vs, std::max<size_t>(c->k_cap, 1)));
       HIPCHK(hipMemcpyAsync(c->cams_d.p, c->h_cams, vs.cams.size() * sizeof(CamDev), hipMemcpyHostToDevice, c->st));
@@ -1201,34 +1225,34 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
         // capture the full step for replay (the projection is re-run inside)
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-        enqueue_step(c, vs, bg, lambda);
+        enqueue_step(c, vs, bg, lambda, var == 1);
         hipError_t e = hipStreamEndCapture(c->st, &g);
         if (e == hipSuccess) {
-          e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+          e = hipGraphInstantiate(&c->gexec[var], g, nullptr, nullptr, 0);
           (void)hipGraphDestroy(g);
-          if (e != hipSuccess) c->gexec = nullptr;
+          if (e != hipSuccess) c->gexec[var] = nullptr;
         }
         if (e != hipSuccess && c->profiling) {
           // the runtime refused event nodes: profile this context eagerly
           (void)hipGetLastError();
           c->prof_graph_failed = true;
           use_graph = 0;
-          enqueue_step(c, vs, bg, lambda);
+          enqueue_step(c, vs, bg, lambda, var == 1);
         } else if (e != hipSuccess) {
           return fail(RGS_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
         } else {
-          c->gkey = key;
-          c->graph_gen = c->gen;
-          HIPCHK(hipGraphLaunch(c->gexec, c->st));
+          c->gkey[var] = key;
+          c->graph_gen[var] = c->gen;
+          HIPCHK(hipGraphLaunch(c->gexec[var], c->st));
           launched_graph = true;
         }
       } else {
-        enqueue_step(c, vs, bg, lambda);
+        enqueue_step(c, vs, bg, lambda, var == 1);
       }
     } else {
       HIPCHK(hipMemcpyAsync(c->cams_d.p, c->h_cams, vs.cams.size() * sizeof(CamDev), hipMemcpyHostToDevice, c->st));
       HIPCHK(hipMemcpyAsync(c->adam_d.p, c->h_adam, sizeof(AdamConsts), hipMemcpyHostToDevice, c->st));
-      HIPCHK(hipGraphLaunch(c->gexec, c->st));
+      HIPCHK(hipGraphLaunch(c->gexec[var], c->st));
     }
     HIPCHK(hipMemcpyAsync(c->h_scalars, c->scalars.p, SC_COUNT * 4, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipMemcpyAsync(c->h_loss, c->loss.p, vs.cams.size() * 4, hipMemcpyDeviceToHost, c->st));
@@ -1246,6 +1270,17 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
       launch_fill_f32(P_<float>(c->accum), acc_n, 0.0f, c->st);
       continue;
     }
+    if (status & ST_NEED_BACK) {
+      // a bin was left unsaturated by its front list in a step run without
+      // the back-list stage: the update was skipped on the device; redo the
+      // step with the stage
+      c->need_back = true;
+      c->back_reruns += 1;
+      launch_fill_f32(P_<float>(c->accum), acc_n, 0.0f, c->st);
+      continue;
+    }
+    c->need_back = c->h_scalars[SC_NFLAG] != 0;
+    c->last_back_valid = var == 1;
     c->last_front_pairs = c->h_scalars[SC_K];
     c->last_back_pairs = c->h_scalars[SC_KB];
     c->last_flagged = c->h_scalars[SC_NFLAG];
@@ -1270,7 +1305,7 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
       for (int i = 0; i < n_ids; ++i) loss_out[i] = c->h_loss[i];
     return 0;
   }
-  return fail(RGS_E_HIP, "pair buffer kept overflowing");
+  return fail(RGS_E_HIP, "the step kept overflowing its pair buffers");
 }
 
 int rgs_autotune_raster(rgs_ctx* c, const int* ids, int n_ids, const float bg[3], float lambda, int* super_tile_out,
@@ -1441,7 +1476,7 @@ int rgs_step_stats(rgs_ctx* c, uint64_t* n_pairs, uint64_t* fwd_evals, uint64_t*
     launch_fill_u32(P_<uint32_t>(c->api_den), 4, 0u, c->st);
     const float zero3[3] = {0.0f, 0.0f, 0.0f};
     RasterIn in = raster_in_n(c, c->last_nviews, c->last_tiles, c->last_shift, zero3);
-    if (c->last_cut) {
+    if (c->last_cut && c->last_back_valid) {
       in.bstart = P_<uint32_t>(c->rbstart);
       in.bend = P_<uint32_t>(c->rbend);
     }
@@ -1488,8 +1523,9 @@ int rgs_set_depth_cut(rgs_ctx* c, int mode, float depth) {
 }
 
 int rgs_cut_stats(rgs_ctx* c, uint64_t* front_pairs, uint64_t* back_pairs, uint64_t* flagged_tiles,
-                  float* next_cut) {
+                  float* next_cut, uint64_t* reruns) {
   if (!c) return fail(RGS_E_ARG, "null ctx");
+  if (reruns) *reruns = c->back_reruns;
   if (front_pairs) *front_pairs = c->last_front_pairs;
   if (back_pairs) *back_pairs = c->last_back_pairs;
   if (flagged_tiles) *flagged_tiles = c->last_flagged;

@@ -21,7 +21,9 @@ struct SplatArrays {
 };
 
 // Status bits written by kernels into a device word.
-enum : uint32_t { ST_NAN_DEPTH = 1u, ST_PAIR_OVERFLOW = 2u };
+// ST_NEED_BACK: a step run without the depth cut's back lists flagged a bin
+// (its results are void; the parameter update is skipped).
+enum : uint32_t { ST_NAN_DEPTH = 1u, ST_PAIR_OVERFLOW = 2u, ST_NEED_BACK = 4u };
 
 // ---- projection (k_project.hip)
 // Projects n Gaussians (SoA params P[59][cap]) into n_views views. Writes the
@@ -164,6 +166,8 @@ void launch_back_gen(const uint32_t* sorted_vals, size_t n, const uint32_t* n_de
                      const uint32_t* total, const uint32_t* n_flagged, const SplatArrays S, int cap,
                      const CamDev* cams, const uint8_t* flag, uint32_t* pair_keys, uint32_t* pair_vals, size_t k_cap,
                      uint32_t* k_out, uint32_t* status, hipStream_t st);
+// Raises ST_NEED_BACK when *n_flagged != 0.
+void launch_need_back(const uint32_t* n_flagged, uint32_t* status, hipStream_t st);
 // Next cut from this step's saturation depths (mode 0); clears zneed.
 void launch_cut_end(CutState* cs, hipStream_t st);
 // Tile sort whose first-pass histogram is not yet computed.

@@ -82,7 +82,8 @@ SIGNATURES = {
     "rgs_set_train_binning": [_vp, _i, _i],
     "rgs_step_image": [_vp, _vp, _sz],
     "rgs_cut_stats": [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
-                      ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_float)],
+                      ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_float),
+                      ctypes.POINTER(ctypes.c_uint64)],
     "rgs_set_raster_params": [_vp, _i, _i],
     "rgs_get_raster_params": [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
     "rgs_autotune_raster": [_vp, _vp, _i, _vp, ctypes.c_float, ctypes.POINTER(ctypes.c_int),
@@ -334,12 +335,15 @@ class Context:
         _check(lib().rgs_set_depth_cut(self.h, int(mode), float(depth)))
 
     def cut_stats(self) -> dict:
-        """Last step's front / back pair counts, tiles that needed their back
-        lists, and the next step's cut depth (inf: none)."""
-        a, b, t = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        """Last step's front / back pair counts, workgroups that needed their
+        back lists, the next step's cut depth (inf: none) and the steps re-run
+        so far because a step without the back-list stage needed it."""
+        a, b, t, r = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
         d = ctypes.c_float()
-        _check(lib().rgs_cut_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(t), ctypes.byref(d)))
-        return {"front_pairs": a.value, "back_pairs": b.value, "flagged_tiles": t.value, "next_cut": d.value}
+        _check(lib().rgs_cut_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(t), ctypes.byref(d),
+                                   ctypes.byref(r)))
+        return {"front_pairs": a.value, "back_pairs": b.value, "flagged_tiles": t.value, "next_cut": d.value,
+                "reruns": r.value}
 
     def set_raster_params(self, super_tile: int, sparse_lanes: int):
         _check(lib().rgs_set_raster_params(self.h, int(super_tile), int(sparse_lanes)))

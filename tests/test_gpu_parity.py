@@ -429,6 +429,44 @@ def test_depth_cut_binning(ctx, use_graph):
                        np.concatenate([ref_g[:, :59], np.zeros((len(g), 1))], 1), 1e-3)
 
 
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_depth_cut_rerun(ctx, use_graph):
+    """A step run without the back-list stage (the previous step needed no
+    back list) that leaves a bin unsaturated is re-run with the stage. On a
+    scene where every bin saturates, the adaptive cut needs no back list;
+    a shallow fixed cut on the next step then flags bins. The re-run step's
+    losses, gradients and images equal a whole-list run's."""
+    from rgsplat_amd import runtime
+    g, cams, targets = workload("c3_1m_1080p_4views", 200_000, [(128, 96), (96, 64)])
+    cfg = S.benchmark_train_config()
+    frozen = dict(cfg, lr_position=0.0, lr_rotation=0.0, lr_scale=0.0, lr_opacity=0.0, lr_sh_dc=0.0,
+                  lr_sh_rest=0.0)
+    npx = sum(c.width * c.height for c in cams)
+    out = []
+    for whole in (False, True):
+        c = runtime.Context(0)
+        c.set_depth_cut(1 if whole else 0)
+        c.scene_upload(g)
+        c.set_views(cams, targets)
+        c.train_step([0, 1], frozen, use_graph=use_graph)
+        c.train_step([0, 1], frozen, use_graph=use_graph)
+        st2 = c.cut_stats()
+        if not whole:
+            c.set_depth_cut(2, 2.0)
+        loss = c.train_step([0, 1], frozen, use_graph=use_graph)
+        out.append((loss, c.adam_download().m1[: len(g)].astype(np.float64), c.step_image(npx), st2,
+                    c.cut_stats()))
+        c.close()
+    (la, ma, ia, s2a, s3a), (lw, mw, iw, s2w, s3w) = out
+    # the adaptive cut's step needed no back list and ran as captured without
+    # the stage; the shallow cut's step was re-run once with it
+    assert s2a["flagged_tiles"] == 0 and np.isfinite(s2a["next_cut"])
+    assert s3a["reruns"] == s2a["reruns"] + 1 and s3a["flagged_tiles"] > 0 and s3a["back_pairs"] > 0
+    assert s3w["reruns"] == 0
+    assert np.array_equal(la, lw) and np.array_equal(ia, iw)
+    assert np.linalg.norm(ma - mw) <= 1e-5 * np.linalg.norm(mw)
+
+
 def test_graph_replay_matches_eager(ctx):
     from rgsplat_amd import runtime
     g, cams, targets = workload("c0_10k_256", 5000, [(128, 96), (160, 128)])
