@@ -67,12 +67,24 @@ __device__ __forceinline__ void view_sh(const CamDev& c, Load prm, const float a
   if (bits) *bits = b;
 }
 
+// dL/d(view direction) from dk (view_sh), through the SH basis (sh.rs:68-89);
+// only used where |cam - pos| > 1e-8 (geometry_nonsh).
+template <class Load>
+__device__ __forceinline__ void view_dir_grad(const CamDev& c, Load prm, const float dk[16], float dd[3]) {
+  const float px = prm(0), py = prm(1), pz = prm(2);
+  float u0 = c.cam_pos[0] - px, u1 = c.cam_pos[1] - py, u2 = c.cam_pos[2] - pz;
+  float ulen = __builtin_sqrtf(u0 * u0 + u1 * u1 + u2 * u2);
+  float e0 = u0 / ulen, e1 = u1 / ulen, e2 = u2 / ulen;
+  sh_basis_grad_contract(e0, e1, e2, dk, dd);
+}
+
 // Gradients of everything but the SH coefficients: position (0-2), rotation
 // (3-6), log-scale (7-9), opacity logit (10) and the screen-gradient norm
-// (59) (backward.rs:529-531, 546-636). ADD accumulates over views.
+// (59) (backward.rs:529-531, 546-636); dd from view_dir_grad. ADD
+// accumulates over views.
 template <bool ADD, class Load>
 __device__ __forceinline__ void geometry_nonsh(const CamDev& c, Load prm, const float acc[9],
-                                               const float dk[16], float* out) {
+                                               const float dd[3], float* out) {
   const float px = prm(0), py = prm(1), pz = prm(2);
   const float q[4] = {prm(3), prm(4), prm(5), prm(6)};
   const float scale[3] = {rgs_expf(prm(7)), rgs_expf(prm(8)), rgs_expf(prm(9))};
@@ -183,8 +195,6 @@ __device__ __forceinline__ void geometry_nonsh(const CamDev& c, Load prm, const 
   float ulen = __builtin_sqrtf(u0 * u0 + u1 * u1 + u2 * u2);
   if (ulen > 1e-8f) {
     float e0 = u0 / ulen, e1 = u1 / ulen, e2 = u2 / ulen;
-    float dd[3];
-    sh_basis_grad_contract(e0, e1, e2, dk, dd);
     float dot = e0 * dd[0] + e1 * dd[1] + e2 * dd[2];
     const float e[3] = {e0, e1, e2};
     for (int cc = 0; cc < 3; ++cc) {
@@ -224,11 +234,12 @@ __global__ __launch_bounds__(256) void k_geometry_aos(const float* __restrict__ 
   for (int k = 0; k < 9; ++k) acc[k] = accum[(size_t)k * acc_stride + vi];
   SoaLoad L{P, (size_t)cap, (size_t)i};
   const CamDev& c = cams[view];
-  float basis[16], masked[3], dk[16];
+  float basis[16], masked[3], dk[16], dd[3];
   view_sh(c, L, acc, basis, masked, dk);
+  view_dir_grad(c, L, dk, dd);
   for (int ch = 0; ch < 3; ++ch)
     for (int k = 0; k < 16; ++k) o[P_SH + ch * 16 + k] = masked[ch] * basis[k];
-  geometry_nonsh<false>(c, L, acc, dk, o);
+  geometry_nonsh<false>(c, L, acc, dd, o);
 }
 
 void launch_geometry_aos(const float* P, int cap, int n, const CamDev* cams, int view, const uint32_t* cnt,
@@ -310,44 +321,93 @@ void launch_adam_aos(float* g, const float* grads, size_t n, float* m1, float* m
 constexpr int TG_G = 0, TG_SGN = 11, TG_BITS = 12;
 static_assert(TRAIN_SCRATCH_ROWS == 18, "scratch rows");
 
+// Position and SH coefficients of one Gaussian held in registers.
+struct ShRegs {
+  float pos[3];
+  float sh[48];
+  __device__ float operator()(int j) const { return j < 3 ? pos[j] : sh[j - P_SH]; }
+};
+
 __global__ __launch_bounds__(256) void k_train_geometry(const float* __restrict__ P, int cap, int n,
                                                         const CamDev* __restrict__ cams, int n_views,
                                                         const uint32_t* __restrict__ cnt,
                                                         const float* __restrict__ accum, size_t acc_stride,
                                                         float* __restrict__ scr, const uint32_t* __restrict__ status) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // per-thread direction gradients of the current chunk of views
+  __shared__ float s_dd[4][3][256];
+  const int t = threadIdx.x;
+  int i = blockIdx.x * blockDim.x + t;
   if (i >= n) return;
   if (*status & (ST_PAIR_OVERFLOW | ST_NAN_DEPTH | ST_NEED_BACK)) return;
-  SoaLoad L{P, (size_t)cap, (size_t)i};
   float g[11];  // flat parameters 0-10
 #pragma unroll
   for (int j = 0; j < 11; ++j) g[j] = 0.0f;
   float sgn = 0.0f;
   uint64_t clamp_in[3] = {0, 0, 0};  // per channel: views whose colour is inside (0, 1)
-  for (int v = 0; v < n_views; ++v) {
-    size_t vi = (size_t)v * cap + i;
-    if (cnt[vi] == 0) continue;
-    // reload the parameters in every view instead of keeping all 59 live
-    // across the loop (an empty asm makes the base pointer loop-variant)
-    const float* Pv = P;
-    asm volatile("" : "+s"(Pv));
-    SoaLoad Lv{Pv, (size_t)cap, (size_t)i};
-    float acc[9];
+  // Views go in chunks of VCH. The first pass reads the SH coefficients once
+  // and, per visible view, evaluates the colour clamp and the direction
+  // gradient dd (kept in LDS); the second runs the geometry backprop per view.
+  // Both keep the per-view arithmetic and the view order of the sums.
+  constexpr int VCH = 4;
+  for (int v0 = 0; v0 < n_views; v0 += VCH) {
+    const int nv = min(VCH, n_views - v0);
+    uint32_t vis = 0;
+    for (int j = 0; j < nv; ++j)
+      if (cnt[(size_t)(v0 + j) * cap + i] != 0) vis |= 1u << j;
+    if (!vis) continue;
+    {
+      // an empty asm makes the base pointer chunk-variant, so that these
+      // loads are not hoisted above the loop (51 registers live across it)
+      const float* Pv = P;
+      asm volatile("" : "+s"(Pv));
+      ShRegs R;
 #pragma unroll
-    for (int q = 0; q < 9; ++q) acc[q] = accum[(size_t)q * acc_stride + vi];
-    float basis[16], masked[3], dk[16];
-    uint32_t bits = 0;
-    view_sh(cams[v], Lv, acc, basis, masked, dk, &bits);
+      for (int q = 0; q < 3; ++q) R.pos[q] = Pv[(size_t)q * cap + i];
 #pragma unroll
-    for (int ch = 0; ch < 3; ++ch) clamp_in[ch] |= (uint64_t)(bits >> ch & 1u) << v;
-    float gg[60];
-    gg[59] = 0.0f;
+      for (int q = 0; q < 48; ++q) R.sh[q] = Pv[(size_t)(P_SH + q) * cap + i];
+#pragma unroll 1
+      for (int j = 0; j < nv; ++j) {
+        if (!(vis >> j & 1u)) continue;
+        const int v = v0 + j;
+        const size_t vi = (size_t)v * cap + i;
+        float acc[9];
 #pragma unroll
-    for (int j = 0; j < 11; ++j) gg[j] = g[j];
-    geometry_nonsh<true>(cams[v], Lv, acc, dk, gg);
+        for (int q = 0; q < 9; ++q) acc[q] = q < 3 ? accum[(size_t)q * acc_stride + vi] : 0.0f;
+        float basis[16], masked[3], dk[16], dd[3];
+        uint32_t bits = 0;
+        view_sh(cams[v], R, acc, basis, masked, dk, &bits);
 #pragma unroll
-    for (int j = 0; j < 11; ++j) g[j] = gg[j];
-    sgn += gg[59];
+        for (int ch = 0; ch < 3; ++ch) clamp_in[ch] |= (uint64_t)(bits >> ch & 1u) << v;
+        view_dir_grad(cams[v], R, dk, dd);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) s_dd[j][q][t] = dd[q];
+      }
+    }
+#pragma unroll 1
+    for (int j = 0; j < nv; ++j) {
+      if (!(vis >> j & 1u)) continue;
+      const int v = v0 + j;
+      const size_t vi = (size_t)v * cap + i;
+      // reload the flat parameters per view (an empty asm makes the base
+      // pointer loop-variant) rather than keeping them live
+      const float* Pv = P;
+      asm volatile("" : "+s"(Pv));
+      SoaLoad Lv{Pv, (size_t)cap, (size_t)i};
+      float acc[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) acc[q] = q < 3 ? 0.0f : accum[(size_t)q * acc_stride + vi];
+      float dd[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) dd[q] = s_dd[j][q][t];
+      float gg[60];
+      gg[59] = 0.0f;
+#pragma unroll
+      for (int q = 0; q < 11; ++q) gg[q] = g[q];
+      geometry_nonsh<true>(cams[v], Lv, acc, dd, gg);
+#pragma unroll
+      for (int q = 0; q < 11; ++q) g[q] = gg[q];
+      sgn += gg[59];
+    }
   }
 #pragma unroll
   for (int j = 0; j < 11; ++j) scr[(size_t)(TG_G + j) * cap + i] = g[j];
