@@ -416,7 +416,7 @@ __device__ __forceinline__ int wave_max_i(int v) {
 // to the accumulators (one atomic per value and splat per bin).
 constexpr int BT = 64;
 template <int S>
-__global__ __launch_bounds__(BT * S * S) void k_raster_bwd(RasterIn in, const float* __restrict__ final_t,
+__global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_raster_bwd(RasterIn in, const float* __restrict__ final_t,
                                                            const int32_t* __restrict__ last_contrib,
                                                            const float* __restrict__ dl_dpix,
                                                            float* __restrict__ accum) {
@@ -456,12 +456,14 @@ __global__ __launch_bounds__(BT * S * S) void k_raster_bwd(RasterIn in, const fl
   // H: sum over channels of dL/dpixel_c times the colour that the splats
   // behind the current one (and the background) add, i.e. d . h with the
   // reference's per-channel accumulators h (backward.rs:470-480)
-  float T[4], d0[4], d1[4], d2[4], H[4], fpy[4];
+  float T[4], d0[4], d1[4], d2[4], H[4];
+  // pixel-centre row of group 0; group q's is fpy0 + 4 q (exact), formed
+  // where it is used rather than held in four registers
+  const float fpy0 = (float)(ty * TILE + (lane >> 4)) + 0.5f;
   int lc_max = -1;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int py = ty * TILE + 4 * q + (lane >> 4);
-    fpy[q] = (float)py + 0.5f;
     lc[q] = -1;
     T[q] = 1.0f;
     d0[q] = d1[q] = d2[q] = 0.0f;
@@ -520,33 +522,19 @@ __global__ __launch_bounds__(BT * S * S) void k_raster_bwd(RasterIn in, const fl
       my_m = wave_mask(A.x, A.y, A.z, A.w, B.x, s_r[lane], tx, ty);
     }
     // splats that reach some pixel group, visited back to front as set bits
-    // of a scalar mask; the next one's LDS data is fetched while the current
-    // one is processed
-    uint64_t live = __ballot(my_m != 0);
-    int jn = live ? 63 - __builtin_clzll(live) : 0;
-    float4 An = s_a[jn], Bn = s_b[jn];
-    float cbn = s_c[jn];
-    while (live) {
-      const int j = jn;
-      live &= ~(1ull << j);
+    // of a scalar mask. The loop is unrolled by two so that the next splat's
+    // LDS data is fetched into the other register set while the current one
+    // is processed.
+    auto splat = [&](const int j, const float4 A, const float4 B, const float cb) {
       const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)my_m, j);
       const int li = lo + j;
-      const float4 A = An;
-      const float4 B = Bn;
-      const float cb = cbn;
-      if (live) {
-        jn = 63 - __builtin_clzll(live);
-        An = s_a[jn];
-        Bn = s_b[jn];
-        cbn = s_c[jn];
-      }
+      const float dx = fpx - A.x;
       bool contrib = false;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         if (!(m >> q & 1u)) continue;  // wave-uniform
         if (li > lc[q]) continue;
-        float dx = fpx - A.x;
-        float dy = fpy[q] - A.y;
+        float dy = (fpy0 + (float)(4 * q)) - A.y;
         float power = -0.5f * (A.z * dx * dx + 2.0f * A.w * dx * dy + B.x * dy * dy);
         if (power > 0.0f) continue;  // NaN proceeds, as in the reference
         float g_exp, alpha_raw;
@@ -570,20 +558,24 @@ __global__ __launch_bounds__(BT * S * S) void k_raster_bwd(RasterIn in, const fl
         float dl_dalpha = Tq * dcol - H[q] * inv_1ma;
         float dl_draw = alpha_raw > MAX_ALPHA ? 0.0f : dl_dalpha;
         float e = dl_draw * g_exp;
+        // dL/dpower = opacity * e; the lane's pixels share dx, so only the
+        // dy moments of e are summed here (sum e, sum e dy, sum e dy^2), and
+        // the dx factors follow per splat, the opacity at the flush
         g[3] += e;
-        // dL/dpower moments; the conic factors are applied at the flush
-        float dl_dpower = e * B.y;
-        float tx_ = dl_dpower * dx, ty_ = dl_dpower * dy;
-        g[4] += tx_;
+        float ty_ = e * dy;
         g[5] += ty_;
-        g[6] += tx_ * dx;
-        g[7] += tx_ * dy;
         g[8] += ty_ * dy;
         H[q] += weight * dcol;
         }
       }
       const uint64_t bm = __ballot(contrib);
-      if (bm == 0) continue;
+      if (bm == 0) return;
+      {
+#pragma clang fp contract(fast)
+        g[4] = dx * g[3];
+        g[6] = dx * g[4];
+        g[7] = dx * g[5];
+      }
       if (__popcll(bm) <= in.sparse_lanes) {
         if (contrib) {
 #pragma unroll
@@ -596,15 +588,40 @@ __global__ __launch_bounds__(BT * S * S) void k_raster_bwd(RasterIn in, const fl
         if (lane == 8) s_g[w][8][j] += s8;
       }
       // the accumulators live across splats and restart here, which spares
-      // re-initialising them for every splat
-#pragma unroll
-      for (int k = 0; k < 9; ++k) g[k] = 0.0f;
+      // re-initialising them for every splat (4, 6 and 7 are set above)
+      g[0] = g[1] = g[2] = g[3] = g[5] = g[8] = 0.0f;
+    };
+    uint64_t live = __ballot(my_m != 0);
+    if (live) {
+      int j0 = 63 - __builtin_clzll(live), j1 = 0;
+      float4 A0 = s_a[j0], B0 = s_b[j0], A1, B1;
+      float c0 = s_c[j0], c1 = 0.0f;
+      while (true) {
+        live &= ~(1ull << j0);
+        if (live) {
+          j1 = 63 - __builtin_clzll(live);
+          A1 = s_a[j1];
+          B1 = s_b[j1];
+          c1 = s_c[j1];
+        }
+        splat(j0, A0, B0, c0);
+        if (!live) break;
+        live &= ~(1ull << j1);
+        if (live) {
+          j0 = 63 - __builtin_clzll(live);
+          A0 = s_a[j0];
+          B0 = s_b[j0];
+          c0 = s_c[j0];
+        }
+        splat(j1, A1, B1, c1);
+        if (!live) break;
+      }
     }
     __syncthreads();
     // flush: raw sums (dL/dcolour[3], dL/dopacity, S1..S5 with S the
-    // dL/dpower moments sum(dlp dx), sum(dlp dy), sum(dlp dx^2),
-    // sum(dlp dx dy), sum(dlp dy^2)) over the waves; dL/dconic and dL/dmean
-    // follow from the conic. Output o of splat lane goes to thread
+    // moments sum(e dx), sum(e dy), sum(e dx^2), sum(e dx dy), sum(e dy^2) of
+    // e = dL/dpower / opacity) over the waves; dL/dconic and dL/dmean follow
+    // from the conic and the opacity. Output o of splat lane goes to thread
     // (o % NW) * 64 + lane.
     if (lane < nb) {
       const uint32_t vi = s_vi[lane];
@@ -619,17 +636,20 @@ __global__ __launch_bounds__(BT * S * S) void k_raster_bwd(RasterIn in, const fl
         float val;
         if (o < 4) {
           val = raw(o);
-        } else if (o == 4) {
-          val = -0.5f * raw(6);
-        } else if (o == 5) {
-          val = -raw(7);
-        } else if (o == 6) {
-          val = -0.5f * raw(8);
         } else {
-          const float4 A = s_a[lane];
-          const float cc = s_b[lane].x;
-          const float s1 = raw(4), s2 = raw(5);
-          val = o == 7 ? A.z * s1 + A.w * s2 : A.w * s1 + cc * s2;
+          const float op = s_b[lane].y;
+          if (o == 4) {
+            val = -0.5f * (op * raw(6));
+          } else if (o == 5) {
+            val = -(op * raw(7));
+          } else if (o == 6) {
+            val = -0.5f * (op * raw(8));
+          } else {
+            const float4 A = s_a[lane];
+            const float cc = s_b[lane].x;
+            const float s1 = op * raw(4), s2 = op * raw(5);
+            val = o == 7 ? A.z * s1 + A.w * s2 : A.w * s1 + cc * s2;
+          }
         }
         if (val != 0.0f) atomicAdd(&accum[(size_t)o * in.acc_stride + vi], val);
       }
