@@ -170,13 +170,18 @@ __global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(6, 8)))
     total_len += end - start;
   }
   const float bg0 = in.bg[0], bg1 = in.bg[1], bg2 = in.bg[2];
-  float T[4], C0[4], C1[4], C2[4], fpy[4];
+  // T[q] < TRANSMITTANCE_STOP marks a pixel that takes no more splats: one
+  // that saturated, or (with T = 0) one outside the image
+  float T[4], C0[4], C1[4], C2[4];
   int32_t last[4];
-  bool done[4];
+  // pixel-centre row of group 0; group q's is fpy0 + 4 q (exact)
+  const float fpy0 = (float)(ty * TILE + (lane >> 4)) + 0.5f;
+  auto pixels_left = [&]() {
+    return fmaxf(fmaxf(T[0], T[1]), fmaxf(T[2], T[3])) >= TRANSMITTANCE_STOP;
+  };
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int py = ty * TILE + 4 * q + (lane >> 4);
-    fpy[q] = (float)py + 0.5f;
     const bool inside = bt.ok && px < c.w && py < c.h;
     T[q] = 1.0f;
     C0[q] = RGBA ? bg0 : 0.0f;
@@ -191,13 +196,13 @@ __global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(6, 8)))
       C2[q] = out_rgb[pix * 3 + 2];
       last[q] = out_last[pix];
     }
-    done[q] = !inside || T[q] < TRANSMITTANCE_STOP;
+    if (!inside) T[q] = 0.0f;
   }
   int32_t fin = -1;  // list index at which this wave's last pixel saturated
   for (uint32_t b0 = start; b0 < end; b0 += FT) {
     // stop when every pixel of the bin has saturated; the barrier also ends
     // the previous batch's reads of the staged splats
-    if (!__syncthreads_or(!(done[0] && done[1] && done[2] && done[3]))) break;
+    if (!__syncthreads_or(pixels_left())) break;
     const uint32_t idx = b0 + lane;
     if (idx < end) {
       const uint32_t vi = vals[idx];
@@ -212,7 +217,7 @@ __global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(6, 8)))
     __syncthreads();
     // all lanes take part in the ballot: the lanes past the list's end may be
     // the ones with pixels left
-    const bool wave_live = __ballot(!(done[0] && done[1] && done[2] && done[3])) != 0;
+    const bool wave_live = __ballot(pixels_left()) != 0;
     uint32_t my_m = 0;  // group mask of this lane's staged splat for this wave's tile
     if (idx < end && wave_live && (S == 1 || in_rect(s_rect[lane], tx, ty))) {
       const float4 A = s_a[lane];
@@ -243,9 +248,9 @@ __global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(6, 8)))
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         if (!(m >> q & 1u)) continue;  // wave-uniform
-        if (done[q]) continue;
+        if (T[q] < TRANSMITTANCE_STOP) continue;
         float dx = fpx - A.x;
-        float dy = fpy[q] - A.y;
+        float dy = (fpy0 + (float)(4 * q)) - A.y;
         float power = -0.5f * (A.z * dx * dx + 2.0f * A.w * dx * dy + B.x * dy * dy);
         if (power > 0.0f) continue;
         float alpha;
@@ -262,9 +267,8 @@ __global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(6, 8)))
         C2[q] += cb * weight;
         T[q] *= 1.0f - alpha;
         last[q] = li;
-        if (T[q] < TRANSMITTANCE_STOP) done[q] = true;
       }
-      if (__ballot(!(done[0] && done[1] && done[2] && done[3])) == 0) {
+      if (__ballot(pixels_left()) == 0) {
         fin = li;
         break;
       }
@@ -272,7 +276,7 @@ __global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(6, 8)))
   }
   if (lane == 0) s_fin[ws] = fin;
   // (with NW = 1: this tile's pixels)
-  const bool all_done = !__syncthreads_or(!(done[0] && done[1] && done[2] && done[3]));
+  const bool all_done = !__syncthreads_or(pixels_left());
   // MODE 3 under a depth cut: a list that ended with pixels left is a front
   // list; the workgroup's tiles keep their raw state (tile_flag) and the bin
   // gets its back list (flag)
