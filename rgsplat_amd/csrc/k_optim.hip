@@ -306,20 +306,19 @@ void launch_adam_aos(float* g, const float* grads, size_t n, float* m1, float* m
                      denom, k);
 }
 
-// Training step tail, two kernels over Gaussians (SoA parameters):
-//   k_train_geometry: per view the Gaussian is visible in, the non-SH
+// Training step tail over Gaussians (SoA parameters), one thread each:
+//   train_geometry: per view the Gaussian is visible in, the non-SH
 //     geometry backprop, summed over views, and the colour-clamp bits of each
-//     (view, channel); written to a scratch of TG_ROWS rows (TG_G + 11 grads,
-//     TG_SGN screen-gradient norm, TG_BITS 3 x 64-bit clamp masks);
-//   k_train_adam: per colour channel the SH gradient sum over views (basis
-//     recomputed from the position, the clamp bit from the scratch), then
-//     Adam on all 59 rows, quaternion renormalisation, the densification
-//     statistics, and clearing of the raster accumulators.
-// The first reads the SH coefficients before the second updates them, and
-// the second reads the positions before it updates them, as in the
-// reference's order (gradients of the whole step first, then Adam).
-constexpr int TG_G = 0, TG_SGN = 11, TG_BITS = 12;
-static_assert(TRAIN_SCRATCH_ROWS == 18, "scratch rows");
+//     (view, channel);
+//   train_adam: per colour channel the SH gradient sum over views (basis
+//     recomputed from the position), then Adam on all 59 rows, quaternion
+//     renormalisation, the densification statistics, and clearing of the
+//     raster accumulators.
+// The geometry part reads the SH coefficients before the Adam part updates
+// them, and the Adam part reads the positions before it updates them, as in
+// the reference's order (gradients of the whole step first, then Adam).
+// k_train_update runs both in one thread: the step's gradients never leave
+// registers.
 
 // Position and SH coefficients of one Gaussian held in registers.
 struct ShRegs {
@@ -328,22 +327,16 @@ struct ShRegs {
   __device__ float operator()(int j) const { return j < 3 ? pos[j] : sh[j - P_SH]; }
 };
 
-__global__ __launch_bounds__(256) void k_train_geometry(const float* __restrict__ P, int cap, int n,
-                                                        const CamDev* __restrict__ cams, int n_views,
-                                                        const uint32_t* __restrict__ cnt,
-                                                        const float* __restrict__ accum, size_t acc_stride,
-                                                        float* __restrict__ scr, const uint32_t* __restrict__ status) {
-  // per-thread direction gradients of the current chunk of views
-  __shared__ float s_dd[4][3][256];
-  const int t = threadIdx.x;
-  int i = blockIdx.x * blockDim.x + t;
-  if (i >= n) return;
-  if (*status & (ST_PAIR_OVERFLOW | ST_NAN_DEPTH | ST_NEED_BACK)) return;
-  float g[11];  // flat parameters 0-10
+// s_dd: per-thread direction gradients of the current chunk of views (LDS).
+__device__ __forceinline__ void train_geometry(const float* __restrict__ P, int cap, int i,
+                                               const CamDev* __restrict__ cams, int n_views,
+                                               const uint32_t* __restrict__ cnt, const float* __restrict__ accum,
+                                               size_t acc_stride, float (*s_dd)[3][256], int t, float g[11],
+                                               float& sgn, uint64_t clamp_in[3]) {
 #pragma unroll
   for (int j = 0; j < 11; ++j) g[j] = 0.0f;
-  float sgn = 0.0f;
-  uint64_t clamp_in[3] = {0, 0, 0};  // per channel: views whose colour is inside (0, 1)
+  sgn = 0.0f;
+  clamp_in[0] = clamp_in[1] = clamp_in[2] = 0;  // per channel: views whose colour is inside (0, 1)
   // Views go in chunks of VCH. The first pass reads the SH coefficients once
   // and, per visible view, evaluates the colour clamp and the direction
   // gradient dd (kept in LDS); the second runs the geometry backprop per view.
@@ -409,45 +402,26 @@ __global__ __launch_bounds__(256) void k_train_geometry(const float* __restrict_
       sgn += gg[59];
     }
   }
-#pragma unroll
-  for (int j = 0; j < 11; ++j) scr[(size_t)(TG_G + j) * cap + i] = g[j];
-  scr[(size_t)TG_SGN * cap + i] = sgn;
-  uint32_t* bits_out = reinterpret_cast<uint32_t*>(scr);
-#pragma unroll
-  for (int ch = 0; ch < 3; ++ch) {
-    bits_out[(size_t)(TG_BITS + 2 * ch) * cap + i] = (uint32_t)clamp_in[ch];
-    bits_out[(size_t)(TG_BITS + 2 * ch + 1) * cap + i] = (uint32_t)(clamp_in[ch] >> 32);
-  }
 }
 
-__global__ __launch_bounds__(256) void k_train_adam(float* __restrict__ P, float* __restrict__ M1,
-                                                    float* __restrict__ M2, int cap, int n,
-                                                    const CamDev* __restrict__ cams, int n_views,
-                                                    const uint32_t* __restrict__ cnt, float* __restrict__ accum,
-                                                    size_t acc_stride, const float* __restrict__ scr,
-                                                    float* __restrict__ gaccum,
-                                                    uint32_t* __restrict__ gdenom,
-                                                    const AdamConsts* __restrict__ k_dev,
-                                                    const uint32_t* __restrict__ status) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (*status & (ST_PAIR_OVERFLOW | ST_NAN_DEPTH | ST_NEED_BACK)) return;
-  const AdamConsts k_ = *k_dev;
+__device__ __forceinline__ void train_adam(float* __restrict__ P, float* __restrict__ M1, float* __restrict__ M2,
+                                           int cap, int i, const CamDev* __restrict__ cams, int n_views,
+                                           const uint32_t* __restrict__ cnt, float* __restrict__ accum,
+                                           size_t acc_stride, const float g[11], float sgn,
+                                           const uint64_t clamp_in[3], float* __restrict__ gaccum,
+                                           uint32_t* __restrict__ gdenom, const AdamConsts& k_) {
   SoaLoad L{P, (size_t)cap, (size_t)i};
   uint64_t vis = 0;
   for (int v = 0; v < n_views; ++v)
     if (cnt[(size_t)v * cap + i] != 0) vis |= 1ull << v;
-  const uint32_t* bits_in = reinterpret_cast<const uint32_t*>(scr);
   for (int ch = 0; ch < 3; ++ch) {
-    const uint64_t clamp_in = (uint64_t)bits_in[(size_t)(TG_BITS + 2 * ch) * cap + i] |
-                              (uint64_t)bits_in[(size_t)(TG_BITS + 2 * ch + 1) * cap + i] << 32;
     float gs[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) gs[k] = 0.0f;
     for (int v = 0; v < n_views; ++v) {
       if (!(vis >> v & 1ull)) continue;
       size_t vi = (size_t)v * cap + i;
-      const float m = (clamp_in >> v & 1ull) ? accum[(size_t)ch * acc_stride + vi] : 0.0f;
+      const float m = (clamp_in[ch] >> v & 1ull) ? accum[(size_t)ch * acc_stride + vi] : 0.0f;
       float basis[16];
       view_basis(cams[v], L, basis);
 #pragma unroll
@@ -474,7 +448,7 @@ __global__ __launch_bounds__(256) void k_train_adam(float* __restrict__ P, float
   for (int j = 0; j < 11; ++j) {
     size_t o = (size_t)j * cap + i;
     float mm = M1[o], vv = M2[o];
-    float p = adam_one(P[o], scr[(size_t)(TG_G + j) * cap + i], mm, vv, lr_of(j, k_), k_);
+    float p = adam_one(P[o], g[j], mm, vv, lr_of(j, k_), k_);
     M1[o] = mm;
     M2[o] = vv;
     if (j >= P_ROT && j < P_ROT + 4)
@@ -484,18 +458,37 @@ __global__ __launch_bounds__(256) void k_train_adam(float* __restrict__ P, float
   }
   renorm_quat(rot[0], rot[1], rot[2], rot[3]);
   for (int q = 0; q < 4; ++q) P[(size_t)(P_ROT + q) * cap + i] = rot[q];
-  gaccum[i] += scr[(size_t)TG_SGN * cap + i];
+  gaccum[i] += sgn;
   gdenom[i] += (uint32_t)n_views;
 }
 
+constexpr uint32_t ST_SKIP_UPDATE = ST_PAIR_OVERFLOW | ST_NAN_DEPTH | ST_NEED_BACK;
+
+__global__ __launch_bounds__(256) void k_train_update(float* __restrict__ P, float* __restrict__ M1,
+                                                      float* __restrict__ M2, int cap, int n,
+                                                      const CamDev* __restrict__ cams, int n_views,
+                                                      const uint32_t* __restrict__ cnt, float* __restrict__ accum,
+                                                      size_t acc_stride, float* __restrict__ gaccum,
+                                                      uint32_t* __restrict__ gdenom,
+                                                      const AdamConsts* __restrict__ k_dev,
+                                                      const uint32_t* __restrict__ status) {
+  __shared__ float s_dd[4][3][256];
+  const int t = threadIdx.x;
+  int i = blockIdx.x * blockDim.x + t;
+  if (i >= n) return;
+  if (*status & ST_SKIP_UPDATE) return;
+  float g[11], sgn;
+  uint64_t clamp_in[3];
+  train_geometry(P, cap, i, cams, n_views, cnt, accum, acc_stride, s_dd, t, g, sgn, clamp_in);
+  train_adam(P, M1, M2, cap, i, cams, n_views, cnt, accum, acc_stride, g, sgn, clamp_in, gaccum, gdenom, *k_dev);
+}
+
 void launch_train_adam(float* P, float* M1, float* M2, int cap, int n, const CamDev* cams, int n_views,
-                       const uint32_t* cnt, float* accum, size_t acc_stride, float* scratch, float* gaccum,
-                       uint32_t* gdenom, const AdamConsts* k_dev, const uint32_t* status, hipStream_t st) {
+                       const uint32_t* cnt, float* accum, size_t acc_stride, float* gaccum, uint32_t* gdenom,
+                       const AdamConsts* k_dev, const uint32_t* status, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_train_geometry, dim3((n + 255) / 256), dim3(256), 0, st, P, cap, n, cams, n_views, cnt, accum,
-                     acc_stride, scratch, status);
-  hipLaunchKernelGGL(k_train_adam, dim3((n + 255) / 256), dim3(256), 0, st, P, M1, M2, cap, n, cams, n_views, cnt,
-                     accum, acc_stride, scratch, gaccum, gdenom, k_dev, status);
+  hipLaunchKernelGGL(k_train_update, dim3((n + 255) / 256), dim3(256), 0, st, P, M1, M2, cap, n, cams, n_views, cnt,
+                     accum, acc_stride, gaccum, gdenom, k_dev, status);
 }
 
 // ln on the device: the f64 logarithm rounded once to f32 (correctly rounded

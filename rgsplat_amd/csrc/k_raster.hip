@@ -635,8 +635,7 @@ __global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(6, 8
         for (int u = 1; u < NW; ++u) t += s_g[u][k][lane];
         return t;
       };
-#pragma unroll
-      for (int o = w; o < 9; o += NW) {
+      for (int o = w; o < 9; o += NW) {  // wave-uniform o
         float val;
         if (o < 4) {
           val = raw(o);

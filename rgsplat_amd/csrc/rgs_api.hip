@@ -264,7 +264,6 @@ struct rgs_ctx {
   uint64_t back_reruns = 0;  // steps re-run because a bin needed its back list
   DBuf api_acc9;           // raster accumulators of the reference-API paths
   DBuf tile_order[2];      // raster schedule per bin shift: slot -> global bin id
-  DBuf train_scr;          // per-Gaussian scratch between geometry and Adam
   DBuf cover;              // per tile: composited although its tight list is empty
   std::vector<int32_t> h_order;
   std::string order_key[2];
@@ -725,7 +724,7 @@ void rgs_destroy(rgs_ctx* c) {
                  &c->api_target, &c->api_m1, &c->api_m2, &c->api_acc, &c->api_den, &c->api_P2, &c->dens_want,
                  &c->dens_cnt, &c->dens_W, &c->dens_packed, &c->dens_prefix, &c->dens_b32, &c->dens_b64,
                  &c->dens_t32, &c->dens_t64, &c->first_splat, &c->P, &c->P2, &c->M1, &c->M2, &c->gacc, &c->gden, &c->targets,
-                 &c->adam_d, &c->api_acc9, &c->tile_order[0], &c->tile_order[1], &c->train_scr, &c->cover, &c->depth,
+                 &c->adam_d, &c->api_acc9, &c->tile_order[0], &c->tile_order[1], &c->cover, &c->depth,
                  &c->flag, &c->tile_flag, &c->rbstart, &c->rbend, &c->pv2, &c->cntb, &c->cutst, &c->part,
                  &c->kB, &c->vB, &c->kC[0], &c->kC[1], &c->vC[0], &c->vC[1]};
   for (DBuf* b : all)
@@ -1144,7 +1143,7 @@ static void enqueue_step_rest(rgs_ctx* c, const ViewSet& vs, const float bg[3], 
   stage_begin(c, S_ADAM);
   launch_train_adam(P_<float>(c->P), P_<float>(c->M1), P_<float>(c->M2), (int)cap, n, P_<CamDev>(c->cams_d),
                     (int)vs.cams.size(), P_<uint32_t>(c->cnt), P_<float>(c->accum), vs.cams.size() * cap,
-                    P_<float>(c->train_scr), P_<float>(c->gacc), P_<uint32_t>(c->gden), P_<AdamConsts>(c->adam_d), sc + SC_STATUS, c->st);
+                    P_<float>(c->gacc), P_<uint32_t>(c->gden), P_<AdamConsts>(c->adam_d), sc + SC_STATUS, c->st);
   stage_end(c, S_ADAM);
 }
 
@@ -1183,7 +1182,6 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
   size_t acc_n = 9 * vs.cams.size() * cap;
   bool fresh_accum = c->accum.bytes < acc_n * 4;
   CHK(ensure(c, c->accum, acc_n * 4));
-  CHK(ensure(c, c->train_scr, (size_t)TRAIN_SCRATCH_ROWS * cap * 4));
   CHK(ensure(c, c->cover, ((size_t)vs.total_tiles + 3) / 4 * 4));
   if (!c->cutst.p) {
     CHK(ensure(c, c->cutst, sizeof(CutState)));

@@ -251,13 +251,12 @@ void launch_geometry_aos(const float* P, int cap, int n, const CamDev* cams, int
 // Adam on the reference layouts: AoS params/grads, row-major moments.
 void launch_adam_aos(float* g, const float* grads, size_t n, float* m1, float* m2, float* accum, uint32_t* denom,
                      const AdamConsts& k, hipStream_t st);
-// Per-Gaussian geometry backprop over all views + Adam on SoA buffers (two
-// kernels; scratch holds TRAIN_SCRATCH_ROWS x cap floats). Skips everything
-// when *status has ST_PAIR_OVERFLOW or ST_NAN_DEPTH. Clears accum.
-constexpr int TRAIN_SCRATCH_ROWS = 18;
+// Per-Gaussian geometry backprop over all views + Adam on SoA buffers (one
+// kernel, one thread per Gaussian). Skips everything when *status has
+// ST_PAIR_OVERFLOW, ST_NAN_DEPTH or ST_NEED_BACK. Clears accum.
 void launch_train_adam(float* P, float* M1, float* M2, int cap, int n, const CamDev* cams, int n_views,
-                       const uint32_t* cnt, float* accum, size_t acc_stride, float* scratch, float* gaccum,
-                       uint32_t* gdenom, const AdamConsts* k_dev, const uint32_t* status, hipStream_t st);
+                       const uint32_t* cnt, float* accum, size_t acc_stride, float* gaccum, uint32_t* gdenom,
+                       const AdamConsts* k_dev, const uint32_t* status, hipStream_t st);
 // reset_opacities on SoA (row 10) or AoS (offset 3) params.
 void launch_reset_opacity(float* p, size_t n, size_t stride, hipStream_t st);
 // Densify/prune on SoA params: P -> P_out (compacted + children). Writes the
