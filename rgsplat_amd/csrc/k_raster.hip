@@ -353,32 +353,24 @@ void launch_raster_fwd(const RasterIn& in, int mode, float* rgb, float* final_t,
     fwd_launch<3>(in, rgb, final_t, last, st);
 }
 
+// Lane exchange inside rows of 16 (quad_perm, row_ror, row_mirror and
+// row_half_mirror patterns: every lane has a source, so bound_ctrl changes
+// nothing and lets the move fold into the consuming add).
 template <int CTRL>
 __device__ __forceinline__ float dpp_f(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
-}
-__device__ __forceinline__ float read_lane_f(float v, int lane) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
-}
-// Sum over the 64 lanes (all lanes active): butterfly inside each row of 16
-// with DPP, then the four row sums through readlane. Wave-uniform result.
-__device__ __forceinline__ float wave_sum(float v) {
-  v += dpp_f<0xB1>(v);   // quad_perm [1,0,3,2]
-  v += dpp_f<0x4E>(v);   // quad_perm [2,3,0,1]
-  v += dpp_f<0x141>(v);  // row_half_mirror
-  v += dpp_f<0x140>(v);  // row_mirror
-  return (read_lane_f(v, 0) + read_lane_f(v, 16)) + (read_lane_f(v, 32) + read_lane_f(v, 48));
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, true));
 }
 // ds_swizzle in bit-mask mode: lane i reads lane i ^ X inside groups of 32.
 template <int X>
 __device__ __forceinline__ float swz_xor(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_ds_swizzle(__builtin_bit_cast(int, v), 0x1F | (X << 10)));
 }
-// Sums eight per-lane values over the wave as a butterfly that halves the
-// number of live values at each exchange (4 + 2 + 1 exchanges instead of
-// 8 x 6). On return lane l < 8 holds the wave sum of g[rev3(l)], where
-// rev3 reverses the three low bits of l.
-__device__ __forceinline__ float wave_sum8_scatter(const float g[8], int lane) {
+// Sums eight per-lane values over each row of 16 lanes as a butterfly that
+// halves the number of live values at each exchange (4 + 2 + 1 exchanges,
+// then one for lane ^ 8, instead of 8 x 4). On return lanes l and l + 8 of
+// each row hold that row's sum of g[rev3(l & 7)], where rev3 reverses the
+// three low bits (the four rows' sums are then added in LDS).
+__device__ __forceinline__ float row_sum8_scatter(const float g[8], int lane) {
   const bool b0 = lane & 1, b1 = lane & 2, b2 = lane & 4;
   float r[4];
 #pragma unroll
@@ -397,10 +389,14 @@ __device__ __forceinline__ float wave_sum8_scatter(const float g[8], int lane) {
   float snd = b2 ? s[0] : s[1];
   float kp = b2 ? s[1] : s[0];
   float t = kp + swz_xor<4>(snd);
-  t += dpp_f<0x128>(t);  // row_ror:8 = lane ^ 8 inside a row of 16
-  t += swz_xor<16>(t);
-  t += __shfl_xor(t, 32, 64);
-  return t;
+  return t + dpp_f<0x128>(t);  // row_ror:8 = lane ^ 8 inside a row of 16
+}
+// Sum over each row of 16 lanes, in every lane of the row.
+__device__ __forceinline__ float row_sum(float v) {
+  v += dpp_f<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f<0x141>(v);  // row_half_mirror
+  return v + dpp_f<0x140>(v);  // row_mirror
 }
 __device__ __forceinline__ int rev3(int l) { return ((l & 1) << 2) | (l & 2) | ((l >> 2) & 1); }
 
@@ -498,6 +494,9 @@ __global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(6, 8
   float g[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) g[k] = 0.0f;
+  // LDS slot offset of the value this lane adds after a row reduction
+  const int lr = lane & 15;
+  const int g_slot = (lr < 8 ? rev3(lr) : 8) * (BT + 1);
   for (int hi = top; hi >= 0; hi -= BT) {
     const int lo = max(0, hi - (BT - 1));
     const int nb = hi - lo + 1;
@@ -586,10 +585,15 @@ __global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(6, 8
           for (int k = 0; k < 9; ++k) atomicAdd(&s_g[w][k][j], g[k]);
         }
       } else {
-        const float t8 = wave_sum8_scatter(g, lane);
-        const float s8 = wave_sum(g[8]);
-        if (lane < 8) s_g[w][rev3(lane)][j] += t8;
-        if (lane == 8) s_g[w][8][j] += s8;
+        // row sums; lanes 0-7 of each row add value rev3(l) and lane 8 adds
+        // value 8, four rows into each slot
+        const float t8 = row_sum8_scatter(g, lane);
+        const float s8 = row_sum(g[8]);
+        // formed here with all lanes active (the last DPP exchanges then fold
+        // into their adds) rather than sunk into the branch below
+        float add = lr < 8 ? t8 : s8;
+        asm volatile("" : "+v"(add));
+        if (lr <= 8) atomicAdd(&s_g[w][0][j] + g_slot, add);
       }
       // the accumulators live across splats and restart here, which spares
       // re-initialising them for every splat (4, 6 and 7 are set above)
