@@ -127,7 +127,7 @@ constexpr int FT = 64;
 // 1 (one workgroup per tile of the bin, each staging the bin list itself and
 // ending on its own).
 template <int MODE, int S, int NW>
-__global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_raster_fwd(
+__global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_raster_fwd(
     RasterIn in, float* __restrict__ out_rgb, float* __restrict__ out_t, int32_t* __restrict__ out_last) {
   static_assert(NW == 1 || NW == S * S, "waves per workgroup");
   constexpr bool FAST = MODE >= 2;
@@ -225,25 +225,16 @@ __global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(6, 8)))
       my_m = wave_mask(A.x, A.y, A.z, A.w, B.x, s_r[lane], tx, ty);
     }
     // splats that reach some pixel group, visited front to back as set bits
-    // of a scalar mask; the next one's LDS data is fetched while the current
-    // one is processed
+    // of a scalar mask (their LDS reads are not prefetched: holding a second
+    // register set costs more than the other waves need to cover the wait)
     uint64_t live = __ballot(my_m != 0);
-    int jn = live ? __builtin_ctzll(live) : 0;
-    float4 An = s_a[jn], Bn = s_b[jn];
-    float cbn = s_c[jn];
     while (live) {
-      const int j = jn;
+      const int j = __builtin_ctzll(live);
       live &= live - 1;
       const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)my_m, j);
-      const float4 A = An;
-      const float4 B = Bn;
-      const float cb = cbn;
-      if (live) {
-        jn = __builtin_ctzll(live);
-        An = s_a[jn];
-        Bn = s_b[jn];
-        cbn = s_c[jn];
-      }
+      const float4 A = s_a[j];
+      const float4 B = s_b[j];
+      const float cb = s_c[j];
       const int32_t li = li0 + (int32_t)(b0 - start) + j;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -416,7 +407,7 @@ __device__ __forceinline__ int wave_max_i(int v) {
 // to the accumulators (one atomic per value and splat per bin).
 constexpr int BT = 64;
 template <int S>
-__global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_raster_bwd(RasterIn in, const float* __restrict__ final_t,
+__global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_raster_bwd(RasterIn in, const float* __restrict__ final_t,
                                                            const int32_t* __restrict__ last_contrib,
                                                            const float* __restrict__ dl_dpix,
                                                            float* __restrict__ accum) {
@@ -525,9 +516,7 @@ __global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(6, 8
       my_m = wave_mask(A.x, A.y, A.z, A.w, B.x, s_r[lane], tx, ty);
     }
     // splats that reach some pixel group, visited back to front as set bits
-    // of a scalar mask. The loop is unrolled by two so that the next splat's
-    // LDS data is fetched into the other register set while the current one
-    // is processed.
+    // of a scalar mask (their LDS reads are not prefetched, as in the forward)
     auto splat = [&](const int j, const float4 A, const float4 B, const float cb) {
       const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)my_m, j);
       const int li = lo + j;
@@ -600,30 +589,10 @@ __global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(6, 8
       g[0] = g[1] = g[2] = g[3] = g[5] = g[8] = 0.0f;
     };
     uint64_t live = __ballot(my_m != 0);
-    if (live) {
-      int j0 = 63 - __builtin_clzll(live), j1 = 0;
-      float4 A0 = s_a[j0], B0 = s_b[j0], A1, B1;
-      float c0 = s_c[j0], c1 = 0.0f;
-      while (true) {
-        live &= ~(1ull << j0);
-        if (live) {
-          j1 = 63 - __builtin_clzll(live);
-          A1 = s_a[j1];
-          B1 = s_b[j1];
-          c1 = s_c[j1];
-        }
-        splat(j0, A0, B0, c0);
-        if (!live) break;
-        live &= ~(1ull << j1);
-        if (live) {
-          j0 = 63 - __builtin_clzll(live);
-          A0 = s_a[j0];
-          B0 = s_b[j0];
-          c0 = s_c[j0];
-        }
-        splat(j1, A1, B1, c1);
-        if (!live) break;
-      }
+    while (live) {
+      const int j = 63 - __builtin_clzll(live);
+      live &= ~(1ull << j);
+      splat(j, s_a[j], s_b[j], s_c[j]);
     }
     __syncthreads();
     // flush: raw sums (dL/dcolour[3], dL/dopacity, S1..S5 with S the
