@@ -78,103 +78,123 @@ __device__ __forceinline__ void view_dir_grad(const CamDev& c, Load prm, const f
   sh_basis_grad_contract(e0, e1, e2, dk, dd);
 }
 
-// Gradients of everything but the SH coefficients: position (0-2), rotation
-// (3-6), log-scale (7-9), opacity logit (10) and the screen-gradient norm
-// (59) (backward.rs:529-531, 546-636); dd from view_dir_grad. ADD
-// accumulates over views.
-template <bool ADD, class Load>
-__device__ __forceinline__ void geometry_nonsh(const CamDev& c, Load prm, const float acc[9],
-                                               const float dd[3], float* out) {
-  const float px = prm(0), py = prm(1), pz = prm(2);
-  const float q[4] = {prm(3), prm(4), prm(5), prm(6)};
-  const float scale[3] = {rgs_expf(prm(7)), rgs_expf(prm(8)), rgs_expf(prm(9))};
-  const float op = sigmoid(prm(10));
-#define OUT(j, val)          \
-  do {                       \
-    if (ADD)                 \
-      out[j] += (val);       \
-    else                     \
-      out[j] = (val);        \
-  } while (0)
-  OUT(10, acc[3] * op * (1.0f - op));
+// Gradients of everything but the SH coefficients (backward.rs:529-531,
+// 546-636), split into the view-independent inputs (GeomConst), the per-view
+// part (geometry_view: dL/dposition, the screen-gradient norm, and
+// dL/dSigma and dL/dopacity-logit factors, summed over views) and the
+// view-independent tail (geometry_finish: rotation, log-scale and opacity
+// gradients from the summed dL/dSigma, which they depend on linearly, so the
+// covariance backprop runs once per Gaussian rather than once per view).
+// Gradient-only arithmetic of the geometry backprop and the training step's
+// Adam uses hardware reciprocal, square root and exp2 (about 1 ulp) and
+// fused multiply-adds: these values feed no cull or clamp decision, and the
+// results stay within float rounding of the reference's (well inside the
+// gradient tolerance). The reference-layout Adam (k_adam_aos) keeps IEEE
+// division and square root.
+__device__ __forceinline__ float rcp_fast(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float exp_fast(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
 
+struct GeomConst {
+  float q[4];      // raw quaternion
+  float scale[3];  // exp(log_scale)
+  float op;        // sigmoid(opacity logit)
+};
+
+// Sum over views of dL/dSigma: each view's T^T G T is symmetric (G is), so
+// the upper triangle is kept; doubled() gives dsig + dsig^T.
+struct SymSum {
+  float u[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};  // 00 01 02 11 12 22
+  __device__ void add(const M3& m) {
+    u[0] += m.a[0][0];
+    u[1] += m.a[0][1];
+    u[2] += m.a[0][2];
+    u[3] += m.a[1][1];
+    u[4] += m.a[1][2];
+    u[5] += m.a[2][2];
+  }
+  __device__ M3 doubled() const {
+    M3 r;
+    r.a[0][0] = u[0] + u[0];
+    r.a[0][1] = r.a[1][0] = u[1] + u[1];
+    r.a[0][2] = r.a[2][0] = u[2] + u[2];
+    r.a[1][1] = u[3] + u[3];
+    r.a[1][2] = r.a[2][1] = u[4] + u[4];
+    r.a[2][2] = u[5] + u[5];
+    return r;
+  }
+};
+
+template <class Load>
+__device__ __forceinline__ GeomConst geom_const(Load prm) {
+  GeomConst G;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) G.q[k] = prm(P_ROT + k);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) G.scale[k] = exp_fast(prm(P_LS + k));
+  G.op = rcp_fast(1.0f + exp_fast(-prm(P_OP)));
+  return G;
+}
+
+// One view: adds dL/dSigma to dsig, dL/dposition to dpos, the screen-gradient
+// norm to sgn (backward.rs:546-636 without the covariance backprop).
+template <class Load>
+__device__ __forceinline__ void geometry_view(const CamDev& c, Load prm, const GeomConst& G, const float acc[9],
+                                              const float dd[3], SymSum& dsig, float dpos[3], float& sgn) {
+#pragma clang fp contract(fast)
+  const float px = prm(0), py = prm(1), pz = prm(2);
   float pc[3];
   to_camera(c, px, py, pz, pc);
   const float tx = pc[0], ty = pc[1], tz = pc[2];
 
-  // 2-D covariance as projected (projection.rs:24-69)
-  M3 sigma = covariance_3d(q[0], q[1], q[2], q[3], scale[0], scale[1], scale[2]);
-  M3 T = ewa_T(c, tx, ty, tz);
+  // 2-D covariance as projected (projection.rs:24-69); the 3-D covariance
+  // is rebuilt per view rather than held across views
+  const M3 sigma = covariance_3d(G.q[0], G.q[1], G.q[2], G.q[3], G.scale[0], G.scale[1], G.scale[2]);
+  const float fx = c.fx, fy = c.fy;
+  const float itz = rcp_fast(tz), itz2 = itz * itz, itz3 = itz2 * itz;
+  M3 W;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) W.a[i][j] = c.view[i * 4 + j];
+  // T = J W (projection.rs:47-61)
+  M3 J = m3_zero();
+  J.a[0][0] = fx * itz;
+  J.a[0][2] = -fx * tx * itz2;
+  J.a[1][1] = fy * itz;
+  J.a[1][2] = -fy * ty * itz2;
+  const M3 T = m3_mul(J, W);
   M3 s2d = m3_mul(m3_mul(T, sigma), m3_t(T));
   const float a2 = s2d.a[0][0] + 0.3f, b2 = s2d.a[0][1], c2 = s2d.a[1][1] + 0.3f;
   const float ga = acc[4], gb = acc[5], gc = acc[6];
   const float det = a2 * c2 - b2 * b2;
-  const float det2 = det * det;
-  const float dl_da = (-ga * c2 * c2 + gb * b2 * c2 - gc * b2 * b2) / det2;
-  const float dl_db = (2.0f * ga * b2 * c2 - gb * (det + 2.0f * b2 * b2) + 2.0f * gc * a2 * b2) / det2;
-  const float dl_dc = (-ga * b2 * b2 + gb * a2 * b2 - gc * a2 * a2) / det2;
+  const float idet2 = rcp_fast(det * det);
+  const float dl_da = (-ga * c2 * c2 + gb * b2 * c2 - gc * b2 * b2) * idet2;
+  const float dl_db = (2.0f * ga * b2 * c2 - gb * (det + 2.0f * b2 * b2) + 2.0f * gc * a2 * b2) * idet2;
+  const float dl_dc = (-ga * b2 * b2 + gb * a2 * b2 - gc * a2 * a2) * idet2;
 
-  M3 G = m3_zero();
-  G.a[0][0] = dl_da;
-  G.a[0][1] = 0.5f * dl_db;
-  G.a[1][0] = 0.5f * dl_db;
-  G.a[1][1] = dl_dc;
-  M3 dsig = m3_mul(m3_mul(m3_t(T), G), T);
-  {
-    // backprop_cov3d (covariance.rs:42-84)
-    float qn[4];
-    quat_normalize(q, qn);
-    M3 r = mat_from_quat(qn);
-    M3 m;
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) m.a[i][j] = r.a[i][j] * scale[j];
-    M3 gs;
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) gs.a[i][j] = dsig.a[i][j] + dsig.a[j][i];
-    M3 dl_dm = m3_mul(gs, m);
-    M3 dl_dr;
-    for (int j = 0; j < 3; ++j) {
-      float a = 0.0f;
-      for (int i = 0; i < 3; ++i) {
-        a += dl_dm.a[i][j] * r.a[i][j];
-        dl_dr.a[i][j] = dl_dm.a[i][j] * scale[j];
-      }
-      OUT(P_LS + j, a * scale[j]);
-    }
-    const float w = qn[0], x = qn[1], y = qn[2], z = qn[3];
-    const float jac[4][9] = {
-        {0.0f, -2.0f * z, 2.0f * y, 2.0f * z, 0.0f, -2.0f * x, -2.0f * y, 2.0f * x, 0.0f},
-        {0.0f, 2.0f * y, 2.0f * z, 2.0f * y, -4.0f * x, -2.0f * w, 2.0f * z, 2.0f * w, -4.0f * x},
-        {-4.0f * y, 2.0f * x, 2.0f * w, 2.0f * x, 0.0f, 2.0f * z, -2.0f * w, 2.0f * z, -4.0f * y},
-        {-4.0f * z, -2.0f * w, 2.0f * x, 2.0f * w, -4.0f * z, 2.0f * y, 2.0f * x, 2.0f * y, 0.0f}};
-    for (int k = 0; k < 4; ++k) {
-      float a = 0.0f;  // column-major fold of the component products
-      for (int cc = 0; cc < 3; ++cc)
-        for (int rr = 0; rr < 3; ++rr) a = a + jac[k][rr * 3 + cc] * dl_dr.a[rr][cc];
-      OUT(P_ROT + k, a);
-    }
-  }
+  M3 Gm = m3_zero();
+  Gm.a[0][0] = dl_da;
+  Gm.a[0][1] = 0.5f * dl_db;
+  Gm.a[1][0] = 0.5f * dl_db;
+  Gm.a[1][1] = dl_dc;
+  const M3 ds = m3_mul(m3_mul(m3_t(T), Gm), T);
+  dsig.add(ds);
 
   // screen mean and the mean-dependence of J -> p_cam -> position
-  const float fx = c.fx, fy = c.fy;
   const float asx = acc[7], asy = acc[8];
-  float dpc[3] = {asx * fx / tz, asy * fy / tz, asx * (-fx * tx / (tz * tz)) + asy * (-fy * ty / (tz * tz))};
-  M3 W;
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) W.a[i][j] = c.view[i * 4 + j];
+  float dpc[3] = {asx * fx * itz, asy * fy * itz, asx * (-fx * tx * itz2) + asy * (-fy * ty * itz2)};
   M3 sig_tt = m3_mul(sigma, m3_t(T));
   M3 t_sig = m3_mul(T, sigma);
   for (int k = 0; k < 3; ++k) {
     M3 dj = m3_zero();
     if (k == 0) {
-      dj.a[0][2] = -fx / (tz * tz);
+      dj.a[0][2] = -fx * itz2;
     } else if (k == 1) {
-      dj.a[1][2] = -fy / (tz * tz);
+      dj.a[1][2] = -fy * itz2;
     } else {
-      dj.a[0][0] = -fx / (tz * tz);
-      dj.a[0][2] = 2.0f * fx * tx / (tz * tz * tz);
-      dj.a[1][1] = -fy / (tz * tz);
-      dj.a[1][2] = 2.0f * fy * ty / (tz * tz * tz);
+      dj.a[0][0] = -fx * itz2;
+      dj.a[0][2] = 2.0f * fx * tx * itz3;
+      dj.a[1][1] = -fy * itz2;
+      dj.a[1][2] = 2.0f * fy * ty * itz3;
     }
     M3 dt = m3_mul(dj, W);
     M3 p1 = m3_mul(dt, sig_tt);
@@ -183,30 +203,76 @@ __device__ __forceinline__ void geometry_nonsh(const CamDev& c, Load prm, const 
     float contrib = dl_da * d00 + dl_db * d01 + dl_dc * d11;
     dpc[k] += contrib;
   }
-  float dpos[3];
+  float dp[3];
   for (int i = 0; i < 3; ++i) {
     float a = W.a[0][i] * dpc[0];
     a = a + W.a[1][i] * dpc[1];
     a = a + W.a[2][i] * dpc[2];
-    dpos[i] = a;
+    dp[i] = a;
   }
   // view-dependent colour: dir = (cam - pos)/|cam - pos|
   float u0 = c.cam_pos[0] - px, u1 = c.cam_pos[1] - py, u2 = c.cam_pos[2] - pz;
-  float ulen = __builtin_sqrtf(u0 * u0 + u1 * u1 + u2 * u2);
+  float ulen = sqrt_fast(u0 * u0 + u1 * u1 + u2 * u2);
   if (ulen > 1e-8f) {
-    float e0 = u0 / ulen, e1 = u1 / ulen, e2 = u2 / ulen;
+    const float iu = rcp_fast(ulen);
+    float e0 = u0 * iu, e1 = u1 * iu, e2 = u2 * iu;
     float dot = e0 * dd[0] + e1 * dd[1] + e2 * dd[2];
     const float e[3] = {e0, e1, e2};
     for (int cc = 0; cc < 3; ++cc) {
       float proj = dd[cc] - e[cc] * dot;
-      dpos[cc] += -(proj / ulen);
+      dp[cc] += -(proj * iu);
     }
   }
-  OUT(0, dpos[0]);
-  OUT(1, dpos[1]);
-  OUT(2, dpos[2]);
-  OUT(59, __builtin_sqrtf(asx * asx + asy * asy));
-#undef OUT
+  dpos[0] += dp[0];
+  dpos[1] += dp[1];
+  dpos[2] += dp[2];
+  sgn += sqrt_fast(asx * asx + asy * asy);
+}
+
+// Rotation (3-6), log-scale (7-9) and opacity-logit (10) gradients from the
+// dL/dSigma and dL/dopacity summed over views (covariance.rs:42-84).
+__device__ __forceinline__ void geometry_finish(const GeomConst& G, const SymSum& dsig, float acc3, float* out) {
+#pragma clang fp contract(fast)
+  const float op = G.op;
+  out[P_OP] = acc3 * op * (1.0f - op);
+  // quaternion.rs normalisation (identity below 1e-12)
+  float qn[4];
+  const float qq = G.q[0] * G.q[0] + G.q[1] * G.q[1] + G.q[2] * G.q[2] + G.q[3] * G.q[3];
+  const float qlen = sqrt_fast(qq);
+  if (qlen < 1e-12f) {
+    qn[0] = 1.0f; qn[1] = 0.0f; qn[2] = 0.0f; qn[3] = 0.0f;
+  } else {
+    const float iq = rcp_fast(qlen);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) qn[k] = G.q[k] * iq;
+  }
+  M3 r = mat_from_quat(qn);
+  M3 m;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) m.a[i][j] = r.a[i][j] * G.scale[j];
+  const M3 gs = dsig.doubled();
+  M3 dl_dm = m3_mul(gs, m);
+  M3 dl_dr;
+  for (int j = 0; j < 3; ++j) {
+    float a = 0.0f;
+    for (int i = 0; i < 3; ++i) {
+      a += dl_dm.a[i][j] * r.a[i][j];
+      dl_dr.a[i][j] = dl_dm.a[i][j] * G.scale[j];
+    }
+    out[P_LS + j] = a * G.scale[j];
+  }
+  const float w = qn[0], x = qn[1], y = qn[2], z = qn[3];
+  const float jac[4][9] = {
+      {0.0f, -2.0f * z, 2.0f * y, 2.0f * z, 0.0f, -2.0f * x, -2.0f * y, 2.0f * x, 0.0f},
+      {0.0f, 2.0f * y, 2.0f * z, 2.0f * y, -4.0f * x, -2.0f * w, 2.0f * z, 2.0f * w, -4.0f * x},
+      {-4.0f * y, 2.0f * x, 2.0f * w, 2.0f * x, 0.0f, 2.0f * z, -2.0f * w, 2.0f * z, -4.0f * y},
+      {-4.0f * z, -2.0f * w, 2.0f * x, 2.0f * w, -4.0f * z, 2.0f * y, 2.0f * x, 2.0f * y, 0.0f}};
+  for (int k = 0; k < 4; ++k) {
+    float a = 0.0f;  // column-major fold of the component products
+    for (int cc = 0; cc < 3; ++cc)
+      for (int rr = 0; rr < 3; ++rr) a = a + jac[k][rr * 3 + cc] * dl_dr.a[rr][cc];
+    out[P_ROT + k] = a;
+  }
 }
 
 struct SoaLoad {
@@ -239,7 +305,15 @@ __global__ __launch_bounds__(256) void k_geometry_aos(const float* __restrict__ 
   view_dir_grad(c, L, dk, dd);
   for (int ch = 0; ch < 3; ++ch)
     for (int k = 0; k < 16; ++k) o[P_SH + ch * 16 + k] = masked[ch] * basis[k];
-  geometry_nonsh<false>(c, L, acc, dd, o);
+  const GeomConst G = geom_const(L);
+  SymSum dsig;
+  float dpos[3] = {0.0f, 0.0f, 0.0f}, sgn = 0.0f;
+  geometry_view(c, L, G, acc, dd, dsig, dpos, sgn);
+  o[0] = dpos[0];
+  o[1] = dpos[1];
+  o[2] = dpos[2];
+  o[59] = sgn;
+  geometry_finish(G, dsig, acc[3], o);
 }
 
 void launch_geometry_aos(const float* P, int cap, int n, const CamDev* cams, int view, const uint32_t* cnt,
@@ -266,6 +340,17 @@ __device__ __forceinline__ float adam_one(float p, float g, float& m, float& v, 
   float vh = v / k.bc2;
   float upd = lr * mh / (__builtin_sqrtf(vh) + k.eps);
   return p - upd;
+}
+
+// adam_one with the bias corrections as products and the hardware
+// reciprocal and square root (see rcp_fast).
+__device__ __forceinline__ float adam_fast(float p, float g, float& m, float& v, float lr, const AdamConsts& k) {
+#pragma clang fp contract(fast)
+  m = k.beta1 * m + k.one_minus_beta1 * g;
+  v = k.beta2 * v + k.one_minus_beta2 * g * g;
+  const float mh = m * k.inv_bc1;
+  const float vh = v * k.inv_bc2;
+  return p - lr * mh * rcp_fast(sqrt_fast(vh) + k.eps);
 }
 
 __device__ __forceinline__ void renorm_quat(float& a, float& b, float& c, float& d) {
@@ -341,6 +426,10 @@ __device__ __forceinline__ void train_geometry(const float* __restrict__ P, int 
   // and, per visible view, evaluates the colour clamp and the direction
   // gradient dd (kept in LDS); the second runs the geometry backprop per view.
   // Both keep the per-view arithmetic and the view order of the sums.
+  // sums over views for geometry_finish
+  bool have_const = false;
+  SymSum dsig;
+  float acc3 = 0.0f;
   constexpr int VCH = 4;
   for (int v0 = 0; v0 < n_views; v0 += VCH) {
     const int nv = min(VCH, n_views - v0);
@@ -376,13 +465,14 @@ __device__ __forceinline__ void train_geometry(const float* __restrict__ P, int 
         for (int q = 0; q < 3; ++q) s_dd[j][q][t] = dd[q];
       }
     }
+    have_const = true;
 #pragma unroll 1
     for (int j = 0; j < nv; ++j) {
       if (!(vis >> j & 1u)) continue;
       const int v = v0 + j;
       const size_t vi = (size_t)v * cap + i;
-      // reload the flat parameters per view (an empty asm makes the base
-      // pointer loop-variant) rather than keeping them live
+      // reload the position per view (an empty asm makes the base pointer
+      // loop-variant) rather than keeping it live
       const float* Pv = P;
       asm volatile("" : "+s"(Pv));
       SoaLoad Lv{Pv, (size_t)cap, (size_t)i};
@@ -392,15 +482,16 @@ __device__ __forceinline__ void train_geometry(const float* __restrict__ P, int 
       float dd[3];
 #pragma unroll
       for (int q = 0; q < 3; ++q) dd[q] = s_dd[j][q][t];
-      float gg[60];
-      gg[59] = 0.0f;
-#pragma unroll
-      for (int q = 0; q < 11; ++q) gg[q] = g[q];
-      geometry_nonsh<true>(cams[v], Lv, acc, dd, gg);
-#pragma unroll
-      for (int q = 0; q < 11; ++q) g[q] = gg[q];
-      sgn += gg[59];
+      // the view-independent inputs are rebuilt per view from the reloaded
+      // parameters rather than held across views (register pressure)
+      geometry_view(cams[v], Lv, geom_const(Lv), acc, dd, dsig, g, sgn);
+      acc3 += acc[3];
     }
+  }
+  if (have_const) {
+    const float* Pv = P;
+    asm volatile("" : "+s"(Pv));
+    geometry_finish(geom_const(SoaLoad{Pv, (size_t)cap, (size_t)i}), dsig, acc3, g);
   }
 }
 
@@ -431,7 +522,7 @@ __device__ __forceinline__ void train_adam(float* __restrict__ P, float* __restr
     for (int k = 0; k < 16; ++k) {
       size_t o = (size_t)(P_SH + ch * 16 + k) * cap + i;
       float mm = M1[o], vv = M2[o];
-      P[o] = adam_one(P[o], gs[k], mm, vv, lr_of(P_SH + ch * 16 + k, k_), k_);
+      P[o] = adam_fast(P[o], gs[k], mm, vv, lr_of(P_SH + ch * 16 + k, k_), k_);
       M1[o] = mm;
       M2[o] = vv;
     }
@@ -448,7 +539,7 @@ __device__ __forceinline__ void train_adam(float* __restrict__ P, float* __restr
   for (int j = 0; j < 11; ++j) {
     size_t o = (size_t)j * cap + i;
     float mm = M1[o], vv = M2[o];
-    float p = adam_one(P[o], g[j], mm, vv, lr_of(j, k_), k_);
+    float p = adam_fast(P[o], g[j], mm, vv, lr_of(j, k_), k_);
     M1[o] = mm;
     M2[o] = vv;
     if (j >= P_ROT && j < P_ROT + 4)

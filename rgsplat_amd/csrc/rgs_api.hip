@@ -674,6 +674,8 @@ AdamConsts adam_consts(const rgs_train_cfg& cfg, uint64_t t) {
   k.bc1 = 1.0f - powi(cfg.beta1, (int)t);
   k.bc2 = 1.0f - powi(cfg.beta2, (int)t);
   k.eps = cfg.eps;
+  k.inv_bc1 = 1.0f / k.bc1;
+  k.inv_bc2 = 1.0f / k.bc2;
   return k;
 }
 

@@ -241,6 +241,7 @@ struct AdamConsts {
   float lr[5];  // position, rotation, scale, opacity, sh_dc
   float lr_sh_rest;
   float beta1, beta2, one_minus_beta1, one_minus_beta2, bc1, bc2, eps;
+  float inv_bc1, inv_bc2;  // 1 / bc1, 1 / bc2 (the training step's Adam)
 };
 // AoS (Gaussian3D, 59 floats) <-> SoA (P[59][cap], Adam flat order).
 void launch_aos_to_soa(const float* aos, size_t n, float* P, size_t cap, hipStream_t st);
