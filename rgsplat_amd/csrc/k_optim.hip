@@ -730,29 +730,47 @@ void launch_zero_rows(float* M, size_t cap, size_t r0, size_t r1, hipStream_t st
   hipLaunchKernelGGL(k_zero_rows, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, M, cap, r0, r1);
 }
 
-__global__ void k_aos_to_soa(const float* __restrict__ aos, size_t n, float* __restrict__ P, size_t cap) {
-  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n * P_N) return;
-  size_t j = idx / n, i = idx % n;
-  P[j * cap + i] = aos[i * 59 + aos_offset((int)j)];
+// Layout transposes between Gaussian3D records (59 floats) and the SoA rows,
+// through LDS: a block moves 64 records, reading and writing both sides in
+// contiguous runs. The record stride (59, odd) makes the column accesses to
+// the LDS tile conflict-free.
+constexpr int TR_G = 64;
+
+__global__ __launch_bounds__(256) void k_aos_to_soa(const float* __restrict__ aos, size_t n, float* __restrict__ P,
+                                                    size_t cap) {
+  __shared__ float s[TR_G * P_N];
+  const size_t i0 = (size_t)blockIdx.x * TR_G;
+  const int cnt = (int)min((size_t)TR_G, n - i0);
+  const float* src = aos + i0 * P_N;
+  for (int t = threadIdx.x; t < cnt * P_N; t += blockDim.x) s[t] = src[t];
+  __syncthreads();
+  for (int t = threadIdx.x; t < P_N * TR_G; t += blockDim.x) {
+    const int j = t / TR_G, l = t % TR_G;
+    if (l < cnt) P[(size_t)j * cap + i0 + l] = s[l * P_N + aos_offset(j)];
+  }
 }
-__global__ void k_soa_to_aos(const float* __restrict__ P, size_t cap, size_t n, float* __restrict__ aos) {
-  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n * P_N) return;
-  size_t i = idx / P_N, o = idx % P_N;
-  // o is the Gaussian3D offset; find its flat index
-  int j = o < 3 ? (int)o : (o == 3 ? 10 : (o < 11 ? (int)o - 1 : (int)o));
-  aos[i * 59 + o] = P[(size_t)j * cap + i];
+
+__global__ __launch_bounds__(256) void k_soa_to_aos(const float* __restrict__ P, size_t cap, size_t n,
+                                                    float* __restrict__ aos) {
+  __shared__ float s[TR_G * P_N];
+  const size_t i0 = (size_t)blockIdx.x * TR_G;
+  const int cnt = (int)min((size_t)TR_G, n - i0);
+  for (int t = threadIdx.x; t < P_N * TR_G; t += blockDim.x) {
+    const int j = t / TR_G, l = t % TR_G;
+    if (l < cnt) s[l * P_N + aos_offset(j)] = P[(size_t)j * cap + i0 + l];
+  }
+  __syncthreads();
+  float* dst = aos + i0 * P_N;
+  for (int t = threadIdx.x; t < cnt * P_N; t += blockDim.x) dst[t] = s[t];
 }
+
 void launch_aos_to_soa(const float* aos, size_t n, float* P, size_t cap, hipStream_t st) {
   if (!n) return;
-  size_t tot = n * P_N;
-  hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, aos, n, P, cap);
+  hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)((n + TR_G - 1) / TR_G)), dim3(256), 0, st, aos, n, P, cap);
 }
 void launch_soa_to_aos(const float* P, size_t cap, size_t n, float* aos, hipStream_t st) {
   if (!n) return;
-  size_t tot = n * P_N;
-  hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, P, cap, n, aos);
+  hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)((n + TR_G - 1) / TR_G)), dim3(256), 0, st, P, cap, n, aos);
 }
 
 }  // namespace rgs
