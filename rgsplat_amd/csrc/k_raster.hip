@@ -96,21 +96,38 @@ __device__ __forceinline__ uint32_t wave_mask(float sx, float sy, float ca, floa
   return m;
 }
 
-// alpha = min(op exp(power), 0.99) from the hardware exp2 (relative error
-// < 1e-6 where alpha can reach either threshold: power in [-5.6, 0]); within
-// 1e-5 of a threshold the exact expf decides, so the alpha tests and the
-// clamp agree with the reference's. The training forward and the backward
-// both use it, with the same operations, so they agree with each other.
-__device__ __forceinline__ float alpha_fast(float power, float op, const uint64_t* s_tab, float& g_exp,
-                                            float& alpha_raw) {
-  g_exp = __builtin_amdgcn_exp2f(power * 1.44269504088896341f);
+// The raster takes the quadratic form qf = a dx^2 + 2 b dx dy + c dy^2 of the
+// reference's power = -0.5 qf (summed in the reference's order). The
+// reference skips power > 0, which holds exactly when qf < -2^-149 (the
+// product by -0.5 rounds -2^-149 to +0 and keeps every other negative qf
+// positive); NaN proceeds either way. And exp2(qf * (-0.5 log2 e)) is
+// bit-identical to exp2(power * log2 e): the product by -0.5 is exact for
+// every qf whose power matters (for |qf| < 2^-125 both arguments give 1).
+constexpr float POWER_POSITIVE_QF = -0x1p-149f;
+constexpr float NEG_HALF_LOG2E = -0.5f * 1.44269504088896341f;
+
+// alpha_raw = op exp(power) from the hardware exp2 (relative error < 1e-6
+// where alpha can reach either threshold: power in [-5.6, 0]); within 1e-5
+// of a threshold the exact expf decides, so the alpha tests and the clamp
+// agree with the reference's. The training forward and the backward both
+// use it, with the same operations, so they agree with each other. The
+// caller clamps: alpha = min(alpha_raw, 0.99) (clamp_alpha).
+__device__ __forceinline__ void alpha_fast(float qf, float op, const uint64_t* s_tab, float& g_exp,
+                                           float& alpha_raw) {
+  g_exp = __builtin_amdgcn_exp2f(qf * NEG_HALF_LOG2E);
   alpha_raw = op * g_exp;
   if (__builtin_fabsf(alpha_raw - ALPHA_THRESHOLD) <= 1e-5f * ALPHA_THRESHOLD ||
       __builtin_fabsf(alpha_raw - MAX_ALPHA) <= 1e-5f * MAX_ALPHA) {
-    g_exp = expf_tab(power, s_tab);
+    g_exp = expf_tab(-0.5f * qf, s_tab);
     alpha_raw = op * g_exp;
   }
-  return fmin_(alpha_raw, MAX_ALPHA);
+}
+// f32::min(alpha_raw, 0.99) (NaN -> 0.99); *clamped is set when the clamp
+// applied, which is also when the reference zeroes dL/d(alpha_raw).
+__device__ __forceinline__ float clamp_alpha(float alpha_raw, bool* clamped = nullptr) {
+  const bool c = !(alpha_raw <= MAX_ALPHA);
+  if (clamped) *clamped = c;
+  return c ? MAX_ALPHA : alpha_raw;
 }
 
 // Forward: four pixels per lane (pixel group q = tile rows 4q..4q+3, lane l
@@ -242,14 +259,15 @@ __global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(7, 8)))
         if (T[q] < TRANSMITTANCE_STOP) continue;
         float dx = fpx - A.x;
         float dy = (fpy0 + (float)(4 * q)) - A.y;
-        float power = -0.5f * (A.z * dx * dx + 2.0f * A.w * dx * dy + B.x * dy * dy);
-        if (power > 0.0f) continue;
+        const float qf = A.z * dx * dx + 2.0f * A.w * dx * dy + B.x * dy * dy;
+        if (qf < POWER_POSITIVE_QF) continue;  // power > 0
         float alpha;
         if (FAST) {
           float g_exp, alpha_raw;
-          alpha = alpha_fast(power, B.y, s_tab, g_exp, alpha_raw);
+          alpha_fast(qf, B.y, s_tab, g_exp, alpha_raw);
+          alpha = clamp_alpha(alpha_raw);
         } else {
-          alpha = fmin_(B.y * expf_tab(power, s_tab), MAX_ALPHA);
+          alpha = fmin_(B.y * expf_tab(-0.5f * qf, s_tab), MAX_ALPHA);
         }
         if (alpha < ALPHA_THRESHOLD) continue;
         float weight = alpha * T[q];
@@ -527,10 +545,12 @@ __global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(7, 8
         if (!(m >> q & 1u)) continue;  // wave-uniform
         if (li > lc[q]) continue;
         float dy = (fpy0 + (float)(4 * q)) - A.y;
-        float power = -0.5f * (A.z * dx * dx + 2.0f * A.w * dx * dy + B.x * dy * dy);
-        if (power > 0.0f) continue;  // NaN proceeds, as in the reference
+        const float qf = A.z * dx * dx + 2.0f * A.w * dx * dy + B.x * dy * dy;
+        if (qf < POWER_POSITIVE_QF) continue;  // power > 0; NaN proceeds, as in the reference
         float g_exp, alpha_raw;
-        const float alpha = alpha_fast(power, B.y, s_tab, g_exp, alpha_raw);
+        alpha_fast(qf, B.y, s_tab, g_exp, alpha_raw);
+        bool clamped;
+        const float alpha = clamp_alpha(alpha_raw, &clamped);
         if (!(alpha >= ALPHA_THRESHOLD)) continue;
         contrib = true;
         {
@@ -548,7 +568,7 @@ __global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(7, 8
         g[2] += d2[q] * weight;
         const float dcol = d0[q] * B.z + d1[q] * B.w + d2[q] * cb;
         float dl_dalpha = Tq * dcol - H[q] * inv_1ma;
-        float dl_draw = alpha_raw > MAX_ALPHA ? 0.0f : dl_dalpha;
+        float dl_draw = clamped ? 0.0f : dl_dalpha;
         float e = dl_draw * g_exp;
         // dL/dpower = opacity * e; the lane's pixels share dx, so only the
         // dy moments of e are summed here (sum e, sum e dy, sum e dy^2), and
