@@ -212,11 +212,16 @@ int rgs_autotune_raster(rgs_ctx* ctx, const int* view_ids, int n_ids, const floa
  * 8 geometry + Adam, 9 back lists of the depth cut (count, scan, generation,
  * sort, resumed forward). Under the depth cut, stages 3-5 cover the front
  * pairs (and 3 the cut itself).
+ * rgs_set_profiling_stages brackets only the stages whose bit is set in
+ * stage_mask (bit s = stage s; 0 turns profiling off), so a timed run can
+ * carry just the events it reads; rgs_set_profiling(ctx, enable) sets all or
+ * none.
  * rgs_stage_times returns the mean milliseconds per stage since profiling
- * was enabled, the number of profiled steps and how many of them were graph
- * replays. */
+ * was enabled (0 for stages outside the mask), the number of profiled steps
+ * and how many of them were graph replays. */
 #define RGS_N_STAGES 10
 int rgs_set_profiling(rgs_ctx* ctx, int enable);
+int rgs_set_profiling_stages(rgs_ctx* ctx, uint32_t stage_mask);
 int rgs_stage_times(rgs_ctx* ctx, float* mean_ms, int n_stages, uint64_t* steps, uint64_t* graph_steps);
 
 /* ---- Diagnostics (no reference counterpart; used by the parity tests).

@@ -274,6 +274,7 @@ struct rgs_ctx {
 
   // stage timing
   bool profiling = false;
+  uint32_t prof_mask = 0;  // stages bracketed by events while profiling
   hipEvent_t ev_b[RGS_N_STAGES] = {}, ev_e[RGS_N_STAGES] = {};
   double stage_ms[RGS_N_STAGES] = {};
   uint64_t prof_steps = 0;
@@ -295,10 +296,10 @@ inline void record(rgs_ctx* c, hipEvent_t e) {
   else (void)hipEventRecord(e, c->st);
 }
 inline void stage_begin(rgs_ctx* c, int s) {
-  if (c->profiling) record(c, c->ev_b[s]);
+  if (c->profiling && (c->prof_mask >> s & 1u)) record(c, c->ev_b[s]);
 }
 inline void stage_end(rgs_ctx* c, int s) {
-  if (c->profiling) record(c, c->ev_e[s]);
+  if (c->profiling && (c->prof_mask >> s & 1u)) record(c, c->ev_e[s]);
 }
 
 void drop_graph(rgs_ctx* c) {
@@ -1195,7 +1196,7 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
 
   if (c->profiling && c->prof_graph_failed) use_graph = 0;
   if (!c->st) use_graph = 0;  // no capture on the null stream
-  if (c->profiling) key += "|prof";
+  if (c->profiling) key += "|prof" + std::to_string(c->prof_mask);
   for (int attempt = 0; attempt < 4; ++attempt) {
     const int var = c->need_back ? 1 : 0;
     bool replay = use_graph && c->gexec[var] && c->gkey[var] == key && c->graph_gen[var] == c->gen;
@@ -1294,6 +1295,7 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
     c->last_nviews = (int)vs.cams.size();
     if (c->profiling) {
       for (int s = 0; s < RGS_N_STAGES; ++s) {
+        if (!(c->prof_mask >> s & 1u)) continue;
         float ms = 0.0f;
         HIPCHK(hipEventElapsedTime(&ms, c->ev_b[s], c->ev_e[s]));
         c->stage_ms[s] += ms;
@@ -1442,15 +1444,21 @@ int rgs_scene_reset_opacities(rgs_ctx* c) {
 }
 
 int rgs_set_profiling(rgs_ctx* c, int enable) {
+  return rgs_set_profiling_stages(c, enable ? (1u << RGS_N_STAGES) - 1u : 0u);
+}
+
+int rgs_set_profiling_stages(rgs_ctx* c, uint32_t stage_mask) {
   if (!c) return fail(RGS_E_ARG, "null ctx");
   (void)hipSetDevice(c->device);
-  if (enable && !c->ev_b[0]) {
+  stage_mask &= (1u << RGS_N_STAGES) - 1u;
+  if (stage_mask && !c->ev_b[0]) {
     for (int s = 0; s < RGS_N_STAGES; ++s) {
       HIPCHK(hipEventCreate(&c->ev_b[s]));
       HIPCHK(hipEventCreate(&c->ev_e[s]));
     }
   }
-  c->profiling = enable != 0;
+  c->profiling = stage_mask != 0;
+  c->prof_mask = stage_mask;
   for (int s = 0; s < RGS_N_STAGES; ++s) c->stage_ms[s] = 0.0;
   c->prof_steps = 0;
   c->prof_graph_steps = 0;

@@ -90,6 +90,7 @@ SIGNATURES = {
                             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)],
     "rgs_eval_expf": [_vp, _vp, _vp, _sz],
     "rgs_set_profiling": [_vp, _i],
+    "rgs_set_profiling_stages": [_vp, ctypes.c_uint32],
     "rgs_stage_times": [_vp, _vp, _i, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)],
     "rgs_debug_tiles": [_vp, _vp, _sz, _vp, _sz, ctypes.POINTER(ctypes.c_uint64)],
 }
@@ -369,6 +370,13 @@ class Context:
 
     def set_profiling(self, enable: bool):
         _check(lib().rgs_set_profiling(self.h, int(bool(enable))))
+
+    def set_profiling_stages(self, stages):
+        """Stage events around the named stages only (names as in STAGES)."""
+        mask = 0
+        for name in stages:
+            mask |= 1 << self.STAGES.index(name)
+        _check(lib().rgs_set_profiling_stages(self.h, mask))
 
     def stage_times(self) -> tuple[dict, int, int]:
         """(mean ms per stage, profiled steps, of which graph replays)."""
