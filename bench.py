@@ -129,6 +129,36 @@ def plain_path(ctx, g, cams, targets, cfg):
     return steps / (t1 - t0)
 
 
+def model_files(runtime, wl, reps):
+    """Wall time of the model-file entry points on the workload's scene (the
+    device scene to a file and back; files under the temp directory)."""
+    import tempfile
+    g = wl.scene()
+    ctx = runtime.Context(0)
+    ctx.scene_upload(g)
+    d = tempfile.mkdtemp()
+    ply, spl = os.path.join(d, "scene.ply"), os.path.join(d, "scene.splat")
+    res = {}
+    for name, fn in (("scene_save_ply", lambda: ctx.scene_save_ply(ply)),
+                     ("scene_load_ply", lambda: ctx.scene_load_ply(ply)),
+                     ("scene_save_splat", lambda: ctx.scene_save_splat(spl))):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        res[name + "_ms"] = (time.perf_counter() - t0) * 1000.0 / reps
+        print(f"[bench] {name} {res[name + '_ms']:.1f} ms", file=sys.stderr, flush=True)
+    res["ply_bytes"] = os.path.getsize(ply)
+    res["splat_bytes"] = os.path.getsize(spl)
+    ok = np.array_equal(ctx.scene_download().view(np.uint32), g.view(np.uint32))
+    ctx.close()
+    os.remove(ply)
+    os.remove(spl)
+    os.rmdir(d)
+    print(json.dumps({"metric": "model files (wall ms per call, file I/O included)", "gaussians": len(g),
+                      "reps": reps, "roundtrip_bit_exact": bool(ok), **res}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +170,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path measurement")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-stage event timing")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default raster work decomposition")
+    ap.add_argument("--io", action="store_true",
+                    help="time the model files instead (PLY save/load and .splat save of the workload's scene)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the step shards by views "
@@ -150,6 +182,9 @@ def main():
     from rgsplat_amd import synthetic as S
 
     wl = S.WORKLOADS[args.workload]
+    if args.io:
+        model_files(runtime, wl, args.steps)
+        return
     print(f"[bench] workload {wl.name}: {wl.n} Gaussians, views {wl.sizes}", file=sys.stderr, flush=True)
     g = wl.scene()
     cams = wl.cameras()

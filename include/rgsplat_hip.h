@@ -25,7 +25,9 @@ enum {
   RGS_E_HIP = 2,      /* HIP runtime failure */
   RGS_E_CAMERA = 3,   /* c2w not invertible (math/mod.rs:155-160 panics) */
   RGS_E_NAN = 4,      /* NaN depth (backward.rs:191 panics in the sort) */
-  RGS_E_CAPACITY = 5  /* densify result exceeds the caller's capacity */
+  RGS_E_CAPACITY = 5, /* densify result exceeds the caller's capacity */
+  RGS_E_IO = 6,       /* a model file could not be opened, read or written */
+  RGS_E_FORMAT = 7    /* not a PLY the reference reads (io/ply.rs:116-170 bails) */
 };
 
 /* Gaussian3D (math/mod.rs:27-46), 59 x f32 = 236 bytes. */
@@ -202,6 +204,26 @@ int rgs_set_train_binning(rgs_ctx* ctx, int shift, int fwd_tile_blocks);
 int rgs_get_raster_params(rgs_ctx* ctx, int* super_tile, int* sparse_lanes);
 int rgs_autotune_raster(rgs_ctx* ctx, const int* view_ids, int n_ids, const float bg[3], float lambda_dssim,
                         int* super_tile, int* sparse_lanes, float* raster_ms);
+
+/* ---- Model files (io/ply.rs). The records move between the device
+ * parameter rows and the file as one block; headers are text on the host.
+ * rgs_save_ply replaces save_ply (io/ply.rs:35-102): binary little-endian
+ * 3DGS layout, byte-identical to the reference's file. rgs_scene_save_ply
+ * writes the device scene (after rgs_scene_upload / training) without a host
+ * copy of the records in Gaussian3D layout.
+ * rgs_load_ply replaces load_ply (io/ply.rs:116-247): binary (either
+ * endianness) or ASCII, any property order, float/double/uchar/ushort/int
+ * properties decoded as the reference does, Gaussian3D::new defaults for
+ * absent properties. With out == NULL or capacity < count it returns
+ * RGS_E_CAPACITY and *n_out = count. rgs_scene_load_ply loads into the device
+ * scene (as rgs_scene_upload: moments and statistics cleared).
+ * rgs_save_splat replaces save_splat (io/ply.rs:258-303): 32-byte records. */
+int rgs_save_ply(rgs_ctx* ctx, const rgs_gaussian* g, size_t n, const char* path);
+int rgs_scene_save_ply(rgs_ctx* ctx, const char* path);
+int rgs_load_ply(rgs_ctx* ctx, const char* path, rgs_gaussian* out, size_t capacity, size_t* n_out);
+int rgs_scene_load_ply(rgs_ctx* ctx, const char* path, size_t capacity);
+int rgs_save_splat(rgs_ctx* ctx, const rgs_gaussian* g, size_t n, const char* path);
+int rgs_scene_save_splat(rgs_ctx* ctx, const char* path);
 
 /* ---- Stage timing (no reference counterpart). With profiling on,
  * rgs_train_step brackets each stage with HIP events on the context's stream;

@@ -109,6 +109,12 @@ def _declare(L):
                                         ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(TrainCfg), ctypes.c_float]
     L.orc_set_threads.argtypes = [i32]
     L.orc_expf_batch.argtypes = [vp, vp, sz]
+    L.orc_save_ply.restype = i32
+    L.orc_save_ply.argtypes = [vp, sz, ctypes.c_char_p]
+    L.orc_load_ply.restype = i32
+    L.orc_load_ply.argtypes = [ctypes.c_char_p, vp, sz, ctypes.POINTER(ctypes.c_size_t)]
+    L.orc_save_splat.restype = i32
+    L.orc_save_splat.argtypes = [vp, sz, ctypes.c_char_p]
 
 
 def _p(a):
@@ -319,3 +325,34 @@ def call3(name, *arrays, out_shape):
     out = np.zeros(out_shape, np.float32)
     getattr(lib(), name)(*[_p(f32vec(a)) for a in arrays], _p(out))
     return out
+
+
+# ---- model files (io/ply.rs) -------------------------------------------------
+def save_ply(g, path):
+    """save_ply (io/ply.rs:35-102)."""
+    g = np.ascontiguousarray(g, np.float32).reshape(-1, 59)
+    if lib().orc_save_ply(_p(g), len(g), str(path).encode()):
+        raise OSError(f"oracle save_ply failed: {path}")
+
+
+def load_ply(path, capacity=None):
+    """load_ply (io/ply.rs:116-247): (n, 59) Gaussian3D records. Raises
+    ValueError for what the reference rejects (bad header or short body)."""
+    n = ctypes.c_size_t(0)
+    # first pass sizes the buffer
+    rc = lib().orc_load_ply(str(path).encode(), None, 0, ctypes.byref(n))
+    if rc not in (0, 4):
+        raise ValueError(f"oracle load_ply failed ({rc}): {path}")
+    cap = n.value if capacity is None else capacity
+    out = np.zeros((max(cap, 1), 59), np.float32)
+    rc = lib().orc_load_ply(str(path).encode(), _p(out), cap, ctypes.byref(n))
+    if rc:
+        raise ValueError(f"oracle load_ply failed ({rc}): {path}")
+    return out[: n.value]
+
+
+def save_splat(g, path):
+    """save_splat (io/ply.rs:258-303)."""
+    g = np.ascontiguousarray(g, np.float32).reshape(-1, 59)
+    if lib().orc_save_splat(_p(g), len(g), str(path).encode()):
+        raise OSError(f"oracle save_splat failed: {path}")

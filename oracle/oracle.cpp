@@ -29,10 +29,16 @@
 // finite differences of every gradient, brute-force binning, a closed-form
 // on-axis isotropic image, direct 2-D convolution SSIM and textbook Adam.
 
+#include <strings.h>
+
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -1332,6 +1338,229 @@ int orc_densify_and_prune(float* g, size_t* n_io, size_t cap, float* m1, float* 
   std::fill(denom, denom + *rows, 0u);
   *n_io = new_n;
   return 0;
+}
+
+// ---- io/ply.rs: model files. Records are Gaussian3D (AoS, 59 floats).
+// save_ply (io/ply.rs:35-102): binary little-endian 3DGS layout.
+int orc_save_ply(const float* g, size_t n, const char* path) {
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return 1;
+  std::fprintf(f, "ply\nformat binary_little_endian 1.0\nelement vertex %zu\n", n);
+  const char* xyz[6] = {"x", "y", "z", "nx", "ny", "nz"};
+  for (int i = 0; i < 6; ++i) std::fprintf(f, "property float %s\n", xyz[i]);
+  for (int i = 0; i < 3; ++i) std::fprintf(f, "property float f_dc_%d\n", i);
+  for (int i = 0; i < 45; ++i) std::fprintf(f, "property float f_rest_%d\n", i);
+  std::fprintf(f, "property float opacity\n");
+  for (int i = 0; i < 3; ++i) std::fprintf(f, "property float scale_%d\n", i);
+  for (int i = 0; i < 4; ++i) std::fprintf(f, "property float rot_%d\n", i);
+  std::fprintf(f, "end_header\n");
+  std::vector<float> rec(62);
+  for (size_t i = 0; i < n; ++i) {
+    const float* q = g + i * G_N;
+    int k = 0;
+    for (int d = 0; d < 3; ++d) rec[k++] = q[G_POS + d];
+    for (int d = 0; d < 3; ++d) rec[k++] = 0.0f;
+    for (int ch = 0; ch < 3; ++ch) rec[k++] = q[G_SH + ch * 16];
+    for (int ch = 0; ch < 3; ++ch)
+      for (int j = 1; j < 16; ++j) rec[k++] = q[G_SH + ch * 16 + j];
+    rec[k++] = q[G_OPL];
+    for (int d = 0; d < 3; ++d) rec[k++] = q[G_LS + d];
+    for (int d = 0; d < 4; ++d) rec[k++] = q[G_ROT + d];
+    std::fwrite(rec.data(), 4, 62, f);  // little-endian host
+  }
+  return std::fclose(f) == 0 ? 0 : 1;
+}
+
+// load_ply (io/ply.rs:116-247). Returns 0, or 1 I/O error, 2 bad header,
+// 3 short body, 4 capacity. *n gets the vertex count (also on code 4).
+static void ply_assign(float* g, const char* name, float v) {  // gaussian_from_props (:216-247)
+  if (!std::strcmp(name, "x")) g[G_POS + 0] = v;
+  else if (!std::strcmp(name, "y")) g[G_POS + 1] = v;
+  else if (!std::strcmp(name, "z")) g[G_POS + 2] = v;
+  else if (!std::strcmp(name, "f_dc_0")) g[G_SH + 0] = v;
+  else if (!std::strcmp(name, "f_dc_1")) g[G_SH + 16] = v;
+  else if (!std::strcmp(name, "f_dc_2")) g[G_SH + 32] = v;
+  else if (!std::strcmp(name, "opacity")) g[G_OPL] = v;
+  else if (!std::strcmp(name, "scale_0")) g[G_LS + 0] = v;
+  else if (!std::strcmp(name, "scale_1")) g[G_LS + 1] = v;
+  else if (!std::strcmp(name, "scale_2")) g[G_LS + 2] = v;
+  else if (!std::strcmp(name, "rot_0")) g[G_ROT + 0] = v;
+  else if (!std::strcmp(name, "rot_1")) g[G_ROT + 1] = v;
+  else if (!std::strcmp(name, "rot_2")) g[G_ROT + 2] = v;
+  else if (!std::strcmp(name, "rot_3")) g[G_ROT + 3] = v;
+  else if (!std::strncmp(name, "f_rest_", 7)) {
+    // usize::parse: decimal digits only (a leading '+' is accepted)
+    const char* d = name + 7;
+    if (*d == '+') ++d;
+    if (!*d) return;
+    unsigned long long idx = 0;
+    for (const char* c = d; *c; ++c) {
+      if (*c < '0' || *c > '9') return;
+      idx = idx * 10 + (unsigned)(*c - '0');
+      if (idx > (1ull << 40)) return;
+    }
+    const unsigned long long ch = idx / 15, k = idx % 15;
+    if (ch < 3) g[G_SH + ch * 16 + 1 + k] = v;
+  }
+}
+static void ply_default(float* g) {  // Gaussian3D::new([0, 0, 0]) (math/mod.rs:52-62)
+  std::fill(g, g + G_N, 0.0f);
+  g[G_ROT] = 1.0f;
+  g[G_LS + 0] = g[G_LS + 1] = g[G_LS + 2] = -3.0f;
+  g[G_OPL] = -2.0f;
+}
+// Rust f32::from_str on a whitespace-free token: decimal (with exponent),
+// inf/infinity/nan (any case, optional sign); anything else fails (-> 0).
+static float rust_parse_f32(const char* t) {
+  const char* d = t;
+  if (*d == '+' || *d == '-') ++d;
+  if (!strcasecmp(d, "inf") || !strcasecmp(d, "infinity")) return t[0] == '-' ? -INFINITY : INFINITY;
+  if (!strcasecmp(d, "nan")) return NAN;
+  for (const char* c = d; *c; ++c)
+    if (!((*c >= '0' && *c <= '9') || *c == '.' || *c == 'e' || *c == 'E' || *c == '+' || *c == '-')) return 0.0f;
+  char* end = nullptr;
+  float v = std::strtof(t, &end);
+  if (end == t || *end) return 0.0f;
+  return v;
+}
+int orc_load_ply(const char* path, float* out, size_t cap, size_t* n_out) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return 1;
+  auto read_line = [&](std::string& line) {
+    line.clear();
+    int ch;
+    while ((ch = std::fgetc(f)) != EOF) {
+      line.push_back((char)ch);
+      if (ch == '\n') break;
+    }
+    return !line.empty();
+  };
+  auto split = [](const std::string& l) {
+    std::vector<std::string> t;
+    size_t i = 0;
+    while (i < l.size()) {
+      while (i < l.size() && std::isspace((unsigned char)l[i])) ++i;
+      size_t j = i;
+      while (j < l.size() && !std::isspace((unsigned char)l[j])) ++j;
+      if (j > i) t.push_back(l.substr(i, j - i));
+      i = j;
+    }
+    return t;
+  };
+  std::string line;
+  auto trim = [](const std::string& l) {
+    size_t a = 0, b = l.size();
+    while (a < b && std::isspace((unsigned char)l[a])) ++a;
+    while (b > a && std::isspace((unsigned char)l[b - 1])) --b;
+    return l.substr(a, b - a);
+  };
+  if (!read_line(line) || trim(line) != "ply") { std::fclose(f); return 2; }
+  int fmt = 2;  // 0 binary LE, 1 binary BE, 2 ascii
+  size_t count = 0;
+  std::vector<std::string> names;
+  std::vector<int> widths;
+  while (true) {
+    if (!read_line(line)) { std::fclose(f); return 2; }
+    auto t = split(line);
+    if (t.size() == 3 && t[0] == "format") {
+      if (t[1] == "binary_little_endian") fmt = 0;
+      else if (t[1] == "binary_big_endian") fmt = 1;
+      else if (t[1] == "ascii") fmt = 2;
+      else { std::fclose(f); return 2; }
+    } else if (t.size() == 3 && t[0] == "element" && t[1] == "vertex") {
+      char* end = nullptr;
+      const char* s0 = t[2].c_str();
+      if (*s0 == '-' || !*s0) { std::fclose(f); return 2; }
+      unsigned long long v = std::strtoull(s0, &end, 10);
+      if (*end) { std::fclose(f); return 2; }
+      count = (size_t)v;
+    } else if (t.size() == 3 && t[0] == "property") {
+      const std::string& ty = t[1];
+      int w;
+      if (ty == "float" || ty == "float32") w = 4;
+      else if (ty == "double" || ty == "float64") w = 8;
+      else if (ty == "uchar" || ty == "uint8" || ty == "char" || ty == "int8") w = 1;
+      else if (ty == "short" || ty == "ushort" || ty == "int16" || ty == "uint16") w = 2;
+      else if (ty == "int" || ty == "uint" || ty == "int32" || ty == "uint32") w = 4;
+      else { std::fclose(f); return 2; }
+      names.push_back(t[2]);
+      widths.push_back(w);
+    } else if (t.size() == 1 && t[0] == "end_header") {
+      break;
+    }
+  }
+  *n_out = count;
+  if (count > cap) { std::fclose(f); return 4; }
+  const size_t np = names.size();
+  if (fmt == 2) {
+    for (size_t i = 0; i < count; ++i) {
+      read_line(line);  // EOF leaves an empty line: every value 0
+      auto t = split(line);
+      float* g = out + i * G_N;
+      ply_default(g);
+      for (size_t k = 0; k < np; ++k) ply_assign(g, names[k].c_str(), k < t.size() ? rust_parse_f32(t[k].c_str()) : 0.0f);
+    }
+  } else {
+    size_t rb = 0;
+    for (int w : widths) rb += (size_t)w;
+    std::vector<unsigned char> rec(rb);
+    for (size_t i = 0; i < count; ++i) {
+      if (rb && std::fread(rec.data(), 1, rb, f) != rb) { std::fclose(f); return 3; }
+      float* g = out + i * G_N;
+      ply_default(g);
+      size_t off = 0;
+      for (size_t k = 0; k < np; ++k) {
+        const int w = widths[k];
+        unsigned char b[8];
+        for (int q = 0; q < w; ++q) b[q] = rec[off + (fmt == 0 ? q : w - 1 - q)];  // to little-endian
+        float v = 0.0f;
+        if (w == 4) { std::memcpy(&v, b, 4); }
+        else if (w == 8) { double d; std::memcpy(&d, b, 8); v = (float)d; }
+        else if (w == 1) { v = (float)b[0]; }
+        else if (w == 2) { uint16_t u; std::memcpy(&u, b, 2); v = (float)u; }
+        ply_assign(g, names[k].c_str(), v);
+        off += (size_t)w;
+      }
+    }
+  }
+  std::fclose(f);
+  return 0;
+}
+
+// save_splat (io/ply.rs:258-303): 32 bytes per Gaussian.
+static uint8_t rust_as_u8(float x) {  // Rust `as u8`: saturate, NaN -> 0
+  if (x != x) return 0;
+  if (x <= 0.0f) return 0;
+  if (x >= 255.0f) return 255;
+  return (uint8_t)x;
+}
+static float rust_clamp(float x, float lo, float hi) {  // f32::clamp (NaN stays NaN)
+  if (x < lo) x = lo;
+  if (x > hi) x = hi;
+  return x;
+}
+int orc_save_splat(const float* g, size_t n, const char* path) {
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return 1;
+  const float SH_C0_SPLAT = 0.2820948f;  // io/ply.rs:25
+  for (size_t i = 0; i < n; ++i) {
+    const float* q = g + i * G_N;
+    unsigned char rec[32];
+    int off = 0;
+    for (int d = 0; d < 3; ++d) { std::memcpy(rec + off, &q[G_POS + d], 4); off += 4; }
+    for (int d = 0; d < 3; ++d) { float s = EXP(q[G_LS + d]); std::memcpy(rec + off, &s, 4); off += 4; }
+    auto to_u8 = [](float x) { return rust_as_u8(std::round(rust_clamp(x, 0.0f, 1.0f) * 255.0f)); };
+    rec[off + 0] = to_u8(0.5f + q[G_SH + 0] * SH_C0_SPLAT);
+    rec[off + 1] = to_u8(0.5f + q[G_SH + 16] * SH_C0_SPLAT);
+    rec[off + 2] = to_u8(0.5f + q[G_SH + 32] * SH_C0_SPLAT);
+    rec[off + 3] = to_u8(sigmoid(q[G_OPL]));
+    off += 4;
+    float qn[4];
+    q_normalize(q + G_ROT, qn);
+    for (int d = 0; d < 4; ++d) rec[off++] = rust_as_u8(std::round(rust_clamp(qn[d] * 128.0f + 128.0f, 0.0f, 255.0f)));
+    std::fwrite(rec, 1, 32, f);
+  }
+  return std::fclose(f) == 0 ? 0 : 1;
 }
 
 }  // extern "C"

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -235,6 +236,7 @@ struct rgs_ctx {
 
   // device-resident scene
   DBuf P, P2, M1, M2, gacc, gden;
+  DBuf io_soa, io_rec, io_fields;  // model files: rows of host records, records, PLY property table
   size_t n = 0, cap = 0, m_rows = 0;
   uint64_t t = 0;
   std::vector<CamDev> reg_cams;
@@ -727,6 +729,7 @@ void rgs_destroy(rgs_ctx* c) {
                  &c->api_target, &c->api_m1, &c->api_m2, &c->api_acc, &c->api_den, &c->api_P2, &c->dens_want,
                  &c->dens_cnt, &c->dens_W, &c->dens_packed, &c->dens_prefix, &c->dens_b32, &c->dens_b64,
                  &c->dens_t32, &c->dens_t64, &c->first_splat, &c->P, &c->P2, &c->M1, &c->M2, &c->gacc, &c->gden, &c->targets,
+                 &c->io_soa, &c->io_rec, &c->io_fields,
                  &c->adam_d, &c->api_acc9, &c->tile_order[0], &c->tile_order[1], &c->cover, &c->depth,
                  &c->flag, &c->tile_flag, &c->rbstart, &c->rbend, &c->pv2, &c->cntb, &c->cutst, &c->part,
                  &c->kB, &c->vB, &c->kC[0], &c->kC[1], &c->vC[0], &c->vC[1]};
@@ -1023,12 +1026,13 @@ int rgs_backward_dev(rgs_ctx* c, const float* dl_dev, float* grads_dev) {
 }
 
 // ---- device-resident scene
-int rgs_scene_upload(rgs_ctx* c, const rgs_gaussian* g, size_t n, size_t capacity) {
-  if (!c || (n && !g)) return fail(RGS_E_ARG, "null argument");
+namespace {
+// A fresh device scene of n Gaussians (rows not written here) with room for
+// capacity: moments and densification statistics cleared, Adam's t reset.
+int scene_reset(rgs_ctx* c, size_t n, size_t capacity) {
   if (capacity < n) capacity = n;
   if (capacity == 0) capacity = 1;
   if (capacity >= (1u << 31)) return fail(RGS_E_ARG, "capacity too large");
-  (void)hipSetDevice(c->device);
   drop_graph(c);
   CHK(ensure(c, c->P, capacity * 59 * 4));
   CHK(ensure(c, c->P2, capacity * 59 * 4));
@@ -1045,10 +1049,316 @@ int rgs_scene_upload(rgs_ctx* c, const rgs_gaussian* g, size_t n, size_t capacit
   launch_fill_f32(P_<float>(c->M2), capacity * 59, 0.0f, c->st);
   launch_fill_f32(P_<float>(c->gacc), capacity, 0.0f, c->st);
   launch_fill_u32(P_<uint32_t>(c->gden), capacity, 0u, c->st);
+  return 0;
+}
+}  // namespace
+
+int rgs_scene_upload(rgs_ctx* c, const rgs_gaussian* g, size_t n, size_t capacity) {
+  if (!c || (n && !g)) return fail(RGS_E_ARG, "null argument");
+  (void)hipSetDevice(c->device);
+  CHK(scene_reset(c, n, capacity));
   if (n) {
     HIPCHK(hipMemcpyAsync(c->api_aos.p, g, n * 59 * 4, hipMemcpyHostToDevice, c->st));
-    launch_aos_to_soa(P_<float>(c->api_aos), n, P_<float>(c->P), capacity, c->st);
+    launch_aos_to_soa(P_<float>(c->api_aos), n, P_<float>(c->P), c->cap, c->st);
   }
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+// ---- model files (io/ply.rs) ----------------------------------------------
+}  // extern "C"
+namespace {
+
+// The reference's header text (io/ply.rs:43-64).
+std::string ply_header(size_t n) {
+  std::string h = "ply\nformat binary_little_endian 1.0\nelement vertex " + std::to_string(n) + "\n";
+  for (const char* p : {"x", "y", "z", "nx", "ny", "nz"}) h += std::string("property float ") + p + "\n";
+  for (int i = 0; i < 3; ++i) h += "property float f_dc_" + std::to_string(i) + "\n";
+  for (int i = 0; i < 45; ++i) h += "property float f_rest_" + std::to_string(i) + "\n";
+  h += "property float opacity\n";
+  for (int i = 0; i < 3; ++i) h += "property float scale_" + std::to_string(i) + "\n";
+  for (int i = 0; i < 4; ++i) h += "property float rot_" + std::to_string(i) + "\n";
+  h += "end_header\n";
+  return h;
+}
+
+// Rows [59][cap] of n host Gaussian3D records in c->io_soa (cap = n).
+int host_records_to_rows(rgs_ctx* c, const rgs_gaussian* g, size_t n) {
+  CHK(ensure(c, c->api_aos, (n ? n : 1) * 59 * 4));
+  CHK(ensure(c, c->io_soa, (n ? n : 1) * 59 * 4));
+  if (n) {
+    HIPCHK(hipMemcpyAsync(c->api_aos.p, g, n * 59 * 4, hipMemcpyHostToDevice, c->st));
+    launch_aos_to_soa(P_<float>(c->api_aos), n, P_<float>(c->io_soa), n, c->st);
+  }
+  return 0;
+}
+
+int write_file(const char* path, const std::string& head, const void* body, size_t bytes) {
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(RGS_E_IO, std::string("cannot create ") + path);
+  bool ok = std::fwrite(head.data(), 1, head.size(), f) == head.size();
+  if (ok && bytes) ok = std::fwrite(body, 1, bytes, f) == bytes;
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? 0 : fail(RGS_E_IO, std::string("cannot write ") + path);
+}
+
+// PLY records of n Gaussians held as rows [59][cap], written to path.
+int save_ply_rows(rgs_ctx* c, const float* rows, size_t cap, size_t n, const char* path) {
+  std::vector<float> rec(n * PLY_FIELDS);
+  if (n) {
+    CHK(ensure(c, c->io_rec, n * PLY_FIELDS * 4));
+    launch_ply_encode(rows, cap, n, P_<float>(c->io_rec), c->st);
+    HIPCHK(hipMemcpyAsync(rec.data(), c->io_rec.p, n * PLY_FIELDS * 4, hipMemcpyDeviceToHost, c->st));
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  return write_file(path, ply_header(n), rec.data(), n * PLY_FIELDS * 4);  // little-endian host
+}
+
+int save_splat_rows(rgs_ctx* c, const float* rows, size_t cap, size_t n, const char* path) {
+  std::vector<uint8_t> rec(n * 32);
+  if (n) {
+    CHK(ensure(c, c->io_rec, n * 32));
+    launch_splat_encode(rows, cap, n, c->io_rec.p, c->st);
+    HIPCHK(hipMemcpyAsync(rec.data(), c->io_rec.p, n * 32, hipMemcpyDeviceToHost, c->st));
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  return write_file(path, std::string(), rec.data(), n * 32);
+}
+
+// gaussian_from_props (io/ply.rs:216-247): the SoA row a property sets, -1 if none.
+int ply_row(const std::string& name) {
+  static const std::pair<const char*, int> named[] = {
+      {"x", P_POS + 0}, {"y", P_POS + 1}, {"z", P_POS + 2}, {"f_dc_0", P_SH + 0}, {"f_dc_1", P_SH + 16},
+      {"f_dc_2", P_SH + 32}, {"opacity", P_OP}, {"scale_0", P_LS + 0}, {"scale_1", P_LS + 1},
+      {"scale_2", P_LS + 2}, {"rot_0", P_ROT + 0}, {"rot_1", P_ROT + 1}, {"rot_2", P_ROT + 2},
+      {"rot_3", P_ROT + 3}};
+  for (const auto& kv : named)
+    if (name == kv.first) return kv.second;
+  if (name.compare(0, 7, "f_rest_") != 0) return -1;
+  // usize::from_str: optional '+', then decimal digits only
+  size_t i = 7;
+  if (i < name.size() && name[i] == '+') ++i;
+  if (i == name.size()) return -1;
+  unsigned long long idx = 0;
+  for (; i < name.size(); ++i) {
+    if (name[i] < '0' || name[i] > '9') return -1;
+    idx = idx * 10 + (unsigned)(name[i] - '0');
+    if (idx > (1ull << 40)) return -1;  // far past the 45 coefficients
+  }
+  const unsigned long long ch = idx / 15, k = idx % 15;  // channel-major, 15 per channel
+  return ch < 3 ? P_SH + (int)(ch * 16 + 1 + k) : -1;
+}
+
+struct PlyFile {
+  int format = 2;  // 0 binary LE, 1 binary BE, 2 ASCII
+  size_t count = 0;
+  std::vector<std::string> names;
+  std::vector<int> widths;
+  std::vector<uint8_t> body;  // count records of the properties' widths
+  int rec_bytes = 0;
+};
+
+bool read_text_line(FILE* f, std::string& line) {
+  line.clear();
+  int ch;
+  while ((ch = std::fgetc(f)) != EOF) {
+    line.push_back((char)ch);
+    if (ch == '\n') break;
+  }
+  return !line.empty();
+}
+std::vector<std::string> split_ws(const std::string& l) {
+  std::vector<std::string> t;
+  size_t i = 0;
+  while (i < l.size()) {
+    while (i < l.size() && std::isspace((unsigned char)l[i])) ++i;
+    size_t j = i;
+    while (j < l.size() && !std::isspace((unsigned char)l[j])) ++j;
+    if (j > i) t.push_back(l.substr(i, j - i));
+    i = j;
+  }
+  return t;
+}
+// f32::from_str of one token (decimal with exponent, inf/infinity/nan in any
+// case, optional sign); the reference maps a failure to 0.
+float parse_f32(const std::string& t) {
+  std::string d = t;
+  bool neg = false;
+  if (!d.empty() && (d[0] == '+' || d[0] == '-')) {
+    neg = d[0] == '-';
+    d = d.substr(1);
+  }
+  std::string low = d;
+  for (char& ch : low) ch = (char)std::tolower((unsigned char)ch);
+  if (low == "inf" || low == "infinity") return neg ? -INFINITY : INFINITY;
+  if (low == "nan") return NAN;
+  for (char ch : d)
+    if (!((ch >= '0' && ch <= '9') || ch == '.' || ch == 'e' || ch == 'E' || ch == '+' || ch == '-')) return 0.0f;
+  char* end = nullptr;
+  const float v = std::strtof(t.c_str(), &end);
+  return (end == t.c_str() || *end) ? 0.0f : v;
+}
+
+// Header and body of a PLY as load_ply reads them (io/ply.rs:121-189).
+int read_ply(const char* path, PlyFile& pf) {
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(RGS_E_IO, std::string("cannot open ") + path);
+  struct Closer {
+    FILE* f;
+    ~Closer() { std::fclose(f); }
+  } closer{f};
+  std::string line;
+  auto trimmed = [](const std::string& l) {
+    size_t a = 0, b = l.size();
+    while (a < b && std::isspace((unsigned char)l[a])) ++a;
+    while (b > a && std::isspace((unsigned char)l[b - 1])) --b;
+    return l.substr(a, b - a);
+  };
+  if (!read_text_line(f, line) || trimmed(line) != "ply") return fail(RGS_E_FORMAT, "not a PLY file (missing magic)");
+  while (true) {
+    if (!read_text_line(f, line)) return fail(RGS_E_FORMAT, "unexpected EOF in PLY header");
+    const auto t = split_ws(line);
+    if (t.size() == 3 && t[0] == "format") {
+      if (t[1] == "binary_little_endian") pf.format = 0;
+      else if (t[1] == "binary_big_endian") pf.format = 1;
+      else if (t[1] == "ascii") pf.format = 2;
+      else return fail(RGS_E_FORMAT, "unsupported PLY format: " + t[1]);
+    } else if (t.size() == 3 && t[0] == "element" && t[1] == "vertex") {
+      const char* s0 = t[2].c_str();
+      char* end = nullptr;
+      if (*s0 == '-' || !*s0) return fail(RGS_E_FORMAT, "vertex count");
+      const unsigned long long v = std::strtoull(s0, &end, 10);
+      if (*end) return fail(RGS_E_FORMAT, "vertex count");
+      pf.count = (size_t)v;
+    } else if (t.size() == 3 && t[0] == "property") {
+      const std::string& ty = t[1];
+      int w;
+      if (ty == "float" || ty == "float32") w = 4;
+      else if (ty == "double" || ty == "float64") w = 8;
+      else if (ty == "uchar" || ty == "uint8" || ty == "char" || ty == "int8") w = 1;
+      else if (ty == "short" || ty == "ushort" || ty == "int16" || ty == "uint16") w = 2;
+      else if (ty == "int" || ty == "uint" || ty == "int32" || ty == "uint32") w = 4;
+      else return fail(RGS_E_FORMAT, "unsupported PLY property type: " + ty);
+      pf.names.push_back(t[2]);
+      pf.widths.push_back(w);
+    } else if (t.size() == 1 && t[0] == "end_header") {
+      break;
+    }  // other elements, comments, obj_info: ignored
+  }
+  const size_t np = pf.names.size();
+  if (pf.count > ((size_t)1 << 31)) return fail(RGS_E_FORMAT, "vertex count too large");
+  if (pf.format == 2) {
+    // one line per vertex, values in property order; a missing value (or
+    // a missing line at EOF) reads as 0; stored as little-endian floats
+    pf.rec_bytes = (int)(4 * np);
+    pf.body.assign(pf.count * np * 4, 0);
+    for (size_t i = 0; i < pf.count; ++i) {
+      read_text_line(f, line);
+      const auto t = split_ws(line);
+      for (size_t k = 0; k < np; ++k) {
+        const float v = k < t.size() ? parse_f32(t[k]) : 0.0f;
+        std::memcpy(&pf.body[(i * np + k) * 4], &v, 4);
+      }
+    }
+  } else {
+    size_t rb = 0;
+    for (int w : pf.widths) rb += (size_t)w;
+    if (rb > (1u << 20)) return fail(RGS_E_FORMAT, "PLY record too large");
+    pf.rec_bytes = (int)rb;
+    pf.body.resize(pf.count * rb);
+    if (!pf.body.empty() && std::fread(pf.body.data(), 1, pf.body.size(), f) != pf.body.size())
+      return fail(RGS_E_FORMAT, "PLY body shorter than its vertex count");
+  }
+  return 0;
+}
+
+// Decode a read PLY into rows [59][cap] (count <= cap).
+int decode_ply(rgs_ctx* c, const PlyFile& pf, float* rows, size_t cap) {
+  if (!pf.count) return 0;
+  std::vector<PlyField> fields;
+  int off = 0;
+  for (size_t k = 0; k < pf.names.size(); ++k) {
+    const int w = pf.format == 2 ? 4 : pf.widths[k];
+    fields.push_back(PlyField{off, w, ply_row(pf.names[k])});
+    off += w;
+  }
+  CHK(ensure(c, c->io_rec, pf.body.empty() ? 4 : pf.body.size()));
+  CHK(ensure(c, c->io_fields, (std::max<size_t>(fields.size(), P_N) + 1) * sizeof(PlyField)));
+  if (!pf.body.empty())
+    HIPCHK(hipMemcpyAsync(c->io_rec.p, pf.body.data(), pf.body.size(), hipMemcpyHostToDevice, c->st));
+  if (ply_decode_tiled_fits(pf.rec_bytes)) {
+    // per row, the last property that sets it
+    std::vector<PlyField> row_field(P_N, PlyField{0, 0, -1});
+    for (const PlyField& f : fields)
+      if (f.row >= 0) row_field[f.row] = f;
+    HIPCHK(hipMemcpyAsync(c->io_fields.p, row_field.data(), P_N * sizeof(PlyField), hipMemcpyHostToDevice, c->st));
+    launch_ply_decode_tiled(P_<uint8_t>(c->io_rec), pf.count, pf.rec_bytes, P_<PlyField>(c->io_fields),
+                            pf.format == 1, rows, cap, c->st);
+  } else {
+    if (!fields.empty())
+      HIPCHK(hipMemcpyAsync(c->io_fields.p, fields.data(), fields.size() * sizeof(PlyField), hipMemcpyHostToDevice,
+                            c->st));
+    launch_ply_decode(P_<uint8_t>(c->io_rec), pf.count, pf.rec_bytes, P_<PlyField>(c->io_fields),
+                      (int)fields.size(), pf.format == 1, rows, cap, c->st);
+  }
+  // the host buffers above are read by the copies: wait before they go
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+}  // namespace
+extern "C" {
+
+int rgs_save_ply(rgs_ctx* c, const rgs_gaussian* g, size_t n, const char* path) {
+  if (!c || !path || (n && !g)) return fail(RGS_E_ARG, "null argument");
+  (void)hipSetDevice(c->device);
+  CHK(host_records_to_rows(c, g, n));
+  return save_ply_rows(c, P_<float>(c->io_soa), n, n, path);
+}
+
+int rgs_scene_save_ply(rgs_ctx* c, const char* path) {
+  if (!c || !path) return fail(RGS_E_ARG, "null argument");
+  (void)hipSetDevice(c->device);
+  return save_ply_rows(c, P_<float>(c->P), c->cap, c->n, path);
+}
+
+int rgs_save_splat(rgs_ctx* c, const rgs_gaussian* g, size_t n, const char* path) {
+  if (!c || !path || (n && !g)) return fail(RGS_E_ARG, "null argument");
+  (void)hipSetDevice(c->device);
+  CHK(host_records_to_rows(c, g, n));
+  return save_splat_rows(c, P_<float>(c->io_soa), n, n, path);
+}
+
+int rgs_scene_save_splat(rgs_ctx* c, const char* path) {
+  if (!c || !path) return fail(RGS_E_ARG, "null argument");
+  (void)hipSetDevice(c->device);
+  return save_splat_rows(c, P_<float>(c->P), c->cap, c->n, path);
+}
+
+int rgs_load_ply(rgs_ctx* c, const char* path, rgs_gaussian* out, size_t capacity, size_t* n_out) {
+  if (!c || !path || !n_out) return fail(RGS_E_ARG, "null argument");
+  (void)hipSetDevice(c->device);
+  PlyFile pf;
+  CHK(read_ply(path, pf));
+  *n_out = pf.count;
+  if (pf.count && (!out || capacity < pf.count)) return fail(RGS_E_CAPACITY, "output capacity smaller than the file");
+  if (!pf.count) return 0;
+  CHK(ensure(c, c->io_soa, pf.count * 59 * 4));
+  CHK(ensure(c, c->api_aos, pf.count * 59 * 4));
+  CHK(decode_ply(c, pf, P_<float>(c->io_soa), pf.count));
+  launch_soa_to_aos(P_<float>(c->io_soa), pf.count, pf.count, P_<float>(c->api_aos), c->st);
+  HIPCHK(hipMemcpyAsync(out, c->api_aos.p, pf.count * 59 * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int rgs_scene_load_ply(rgs_ctx* c, const char* path, size_t capacity) {
+  if (!c || !path) return fail(RGS_E_ARG, "null argument");
+  (void)hipSetDevice(c->device);
+  PlyFile pf;
+  CHK(read_ply(path, pf));
+  CHK(scene_reset(c, pf.count, capacity));
+  CHK(decode_ply(c, pf, P_<float>(c->P), c->cap));
   HIPCHK(hipStreamSynchronize(c->st));
   return 0;
 }

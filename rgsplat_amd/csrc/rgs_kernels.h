@@ -258,6 +258,21 @@ void launch_adam_aos(float* g, const float* grads, size_t n, float* m1, float* m
 void launch_train_adam(float* P, float* M1, float* M2, int cap, int n, const CamDev* cams, int n_views,
                        const uint32_t* cnt, float* accum, size_t acc_stride, float* gaccum, uint32_t* gdenom,
                        const AdamConsts* k_dev, const uint32_t* status, hipStream_t st);
+// Model files (k_io.hip). A PLY vertex property: byte offset in the record,
+// byte width (1, 2, 4 or 8) and the SoA row it sets (-1: ignored).
+struct PlyField {
+  int32_t offset, width, row;
+};
+constexpr int PLY_FIELDS = 62;  // floats per record written by save_ply
+void launch_ply_encode(const float* P, size_t cap, size_t n, float* rec, hipStream_t st);
+void launch_ply_decode(const uint8_t* body, size_t n, int rec_bytes, const PlyField* fields, int n_fields,
+                       int big_endian, float* P, size_t cap, hipStream_t st);
+// Tiled decoder for records of at most ply_decode_tiled_fits bytes;
+// row_field[59]: per SoA row the last property that sets it (width 0: none).
+int ply_decode_tiled_fits(int rec_bytes);
+void launch_ply_decode_tiled(const uint8_t* body, size_t n, int rec_bytes, const PlyField* row_field, int big_endian,
+                             float* P, size_t cap, hipStream_t st);
+void launch_splat_encode(const float* P, size_t cap, size_t n, void* out /* n x 32 bytes */, hipStream_t st);
 // reset_opacities on SoA (row 10) or AoS (offset 3) params.
 void launch_reset_opacity(float* p, size_t n, size_t stride, hipStream_t st);
 // Densify/prune on SoA params: P -> P_out (compacted + children). Writes the

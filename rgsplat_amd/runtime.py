@@ -71,6 +71,12 @@ SIGNATURES = {
     "rgs_scene_upload": [_vp, _vp, _sz, _sz],
     "rgs_scene_size": [_vp, ctypes.POINTER(_sz)],
     "rgs_scene_download": [_vp, _vp, _sz, ctypes.POINTER(_sz)],
+    "rgs_save_ply": [_vp, _vp, _sz, ctypes.c_char_p],
+    "rgs_scene_save_ply": [_vp, ctypes.c_char_p],
+    "rgs_load_ply": [_vp, ctypes.c_char_p, _vp, _sz, ctypes.POINTER(_sz)],
+    "rgs_scene_load_ply": [_vp, ctypes.c_char_p, _sz],
+    "rgs_save_splat": [_vp, _vp, _sz, ctypes.c_char_p],
+    "rgs_scene_save_splat": [_vp, ctypes.c_char_p],
     "rgs_adam_download": [_vp, _ADAM],
     "rgs_set_views": [_vp, _CAM, ctypes.POINTER(ctypes.c_void_p), _i],
     "rgs_train_step": [_vp, ctypes.POINTER(_i), _i, _CFG, _vp, _f, _i, _vp],
@@ -276,6 +282,38 @@ class Context:
         nn = ctypes.c_size_t()
         _check(lib().rgs_scene_download(self.h, _p(out), len(out), ctypes.byref(nn)))
         return out[: nn.value]
+
+    # ------------------------------------------------------------ model files
+    def save_ply(self, gaussians, path):
+        """save_ply (io/ply.rs:35-102) of (n, 59) Gaussian3D records."""
+        g = _f32(gaussians).reshape(-1, 59)
+        _check(lib().rgs_save_ply(self.h, _p(g), len(g), os.fsencode(path)))
+
+    def scene_save_ply(self, path):
+        """The device scene as a PLY (save_ply's file)."""
+        _check(lib().rgs_scene_save_ply(self.h, os.fsencode(path)))
+
+    def load_ply(self, path) -> np.ndarray:
+        """load_ply (io/ply.rs:116-247): (n, 59) Gaussian3D records."""
+        n = ctypes.c_size_t()
+        rc = lib().rgs_load_ply(self.h, os.fsencode(path), None, 0, ctypes.byref(n))
+        if rc not in (0, 5):  # 5: RGS_E_CAPACITY, the count is in n
+            _check(rc)
+        out = np.zeros((max(n.value, 1), 59), np.float32)
+        _check(lib().rgs_load_ply(self.h, os.fsencode(path), _p(out), len(out), ctypes.byref(n)))
+        return out[: n.value]
+
+    def scene_load_ply(self, path, capacity: int = 0):
+        """A PLY into the device scene (as scene_upload)."""
+        _check(lib().rgs_scene_load_ply(self.h, os.fsencode(path), int(capacity)))
+
+    def save_splat(self, gaussians, path):
+        """save_splat (io/ply.rs:258-303) of (n, 59) Gaussian3D records."""
+        g = _f32(gaussians).reshape(-1, 59)
+        _check(lib().rgs_save_splat(self.h, _p(g), len(g), os.fsencode(path)))
+
+    def scene_save_splat(self, path):
+        _check(lib().rgs_scene_save_splat(self.h, os.fsencode(path)))
 
     def adam_download(self, capacity: int | None = None) -> AdamState:
         n = self.scene_size()
