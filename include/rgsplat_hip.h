@@ -146,6 +146,19 @@ int rgs_scene_download(rgs_ctx* ctx, rgs_gaussian* g, size_t capacity, size_t* n
 int rgs_adam_download(rgs_ctx* ctx, rgs_adam* st);
 /* Registers the training views and their targets (w*h*4 f32 each). */
 int rgs_set_views(rgs_ctx* ctx, const rgs_camera* cams, const float* const* targets_rgba, int n_views);
+/* As rgs_set_views from decoded RGBA8 images of the cameras' sizes (w*h*4
+ * bytes each), converted to linear RGBA on the device as load_target does
+ * after decoding and resizing (io/images.rs:63-69); 4x less to upload. */
+int rgs_set_views_rgba8(rgs_ctx* ctx, const rgs_camera* cams, const uint8_t* const* images_rgba8, int n_views);
+/* load_target's conversion (io/images.rs:63-69, srgb_to_linear :76-82) of
+ * n_pixels RGBA8 pixels to linear RGBA f32, bit-identical (the sRGB table is
+ * computed with the libm powf the reference calls). */
+int rgs_rgba8_to_linear(rgs_ctx* ctx, const uint8_t* rgba8, size_t n_pixels, float* out_rgba);
+/* Framebuffer::to_u8_srgb (cpu/rasterizer.rs:38-49) of n_pixels RGBA f32
+ * pixels. linear_to_srgb (:271-279) takes powf from an f64 pow rounded to
+ * f32, so a byte can differ from the reference's only where the scaled value
+ * lies within an ulp of an integer. */
+int rgs_to_u8_srgb(rgs_ctx* ctx, const float* rgba, size_t n_pixels, uint8_t* out_rgba8);
 /* One training iteration over the views listed in view_ids: forward and
  * backward of every listed view in single launches, gradients summed over the
  * views, one Adam step. loss_out (may be NULL) receives one loss per listed

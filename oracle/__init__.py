@@ -109,6 +109,8 @@ def _declare(L):
                                         ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(TrainCfg), ctypes.c_float]
     L.orc_set_threads.argtypes = [i32]
     L.orc_expf_batch.argtypes = [vp, vp, sz]
+    L.orc_rgba8_to_linear.argtypes = [vp, sz, vp]
+    L.orc_to_u8_srgb.argtypes = [vp, sz, vp]
     L.orc_save_ply.restype = i32
     L.orc_save_ply.argtypes = [vp, sz, ctypes.c_char_p]
     L.orc_load_ply.restype = i32
@@ -356,3 +358,18 @@ def save_splat(g, path):
     g = np.ascontiguousarray(g, np.float32).reshape(-1, 59)
     if lib().orc_save_splat(_p(g), len(g), str(path).encode()):
         raise OSError(f"oracle save_splat failed: {path}")
+
+
+# ---- colour conversions (io/images.rs:63-82, cpu/rasterizer.rs:38-49) -------
+def rgba8_to_linear(img):
+    img = np.ascontiguousarray(img, np.uint8)
+    out = np.zeros(img.shape, np.float32)
+    lib().orc_rgba8_to_linear(_p(img), img.size // 4, _p(out))
+    return out
+
+
+def to_u8_srgb(rgba):
+    rgba = np.ascontiguousarray(rgba, np.float32)
+    out = np.zeros(rgba.shape, np.uint8)
+    lib().orc_to_u8_srgb(_p(rgba), rgba.size // 4, _p(out))
+    return out

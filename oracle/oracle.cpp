@@ -1171,6 +1171,20 @@ int orc_aabb(float mx, float my, const float s2[9], float n_sigma, uint32_t w, u
 // rasterizer.rs:273-279 and io/images.rs:76-82
 float orc_linear_to_srgb(float x) { return x <= 0.0031308f ? 12.92f * x : 1.055f * powf(x, 1.0f / 2.4f) - 0.055f; }
 float orc_srgb_to_linear(float x) { return x <= 0.04045f ? x / 12.92f : powf((x + 0.055f) / 1.055f, 2.4f); }
+// load_target's loop (io/images.rs:63-69) over decoded RGBA8 pixels
+void orc_rgba8_to_linear(const uint8_t* px, size_t n, float* out) {
+  for (size_t i = 0; i < 4 * n; i += 4) {
+    for (int c = 0; c < 3; ++c) out[i + c] = orc_srgb_to_linear((float)px[i + c] / 255.0f);
+    out[i + 3] = (float)px[i + 3] / 255.0f;
+  }
+}
+// Framebuffer::to_u8_srgb (rasterizer.rs:38-49): `as u8` truncates and saturates
+void orc_to_u8_srgb(const float* v, size_t n, uint8_t* out) {
+  auto as_u8 = [](float x) -> uint8_t { return (x != x || x <= 0.0f) ? 0 : (x >= 255.0f ? 255 : (uint8_t)x); };
+  auto clamp01 = [](float x) { return x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x); };
+  for (size_t i = 0; i < 4 * n; ++i)
+    out[i] = (i % 4 == 3) ? as_u8(clamp01(v[i]) * 255.0f) : as_u8(orc_linear_to_srgb(clamp01(v[i])) * 255.0f);
+}
 
 // ---- optimizer.rs:116-163 (+ :165-195). Parameters AoS (Gaussian3D), grads
 // AoS (GaussianGrad), moments n x 59 in the flat order

@@ -221,3 +221,64 @@ void launch_splat_encode(const float* P, size_t cap, size_t n, void* out, hipStr
 }
 
 }  // namespace rgs
+
+namespace rgs {
+
+// ---- Colour conversions ----------------------------------------------------
+// load_target's per-pixel loop (io/images.rs:63-69): channel c of an RGBA8
+// pixel -> srgb_to_linear(c / 255) for RGB, c / 255 for alpha. The 256 RGB
+// values come from a table the host fills with the same libm powf the
+// reference calls (srgb8_table), so the kernel is a lookup; 4 pixels per
+// thread (16 bytes in, 64 bytes out).
+__global__ __launch_bounds__(256) void k_rgba8_to_linear(const uint32_t* __restrict__ px, size_t n,
+                                                         const float* __restrict__ lut, float4* __restrict__ out) {
+  __shared__ float s_lut[512];  // [0, 256): RGB, [256, 512): alpha (c / 255)
+  for (int t = threadIdx.x; t < 512; t += blockDim.x) s_lut[t] = lut[t];
+  __syncthreads();
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t p = px[i];
+  out[i] = make_float4(s_lut[p & 255u], s_lut[(p >> 8) & 255u], s_lut[(p >> 16) & 255u], s_lut[256 + (p >> 24)]);
+}
+
+void launch_rgba8_to_linear(const uint8_t* rgba8, size_t n_pixels, const float* lut, float* out, hipStream_t st) {
+  if (!n_pixels) return;
+  hipLaunchKernelGGL(k_rgba8_to_linear, dim3((unsigned)((n_pixels + 255) / 256)), dim3(256), 0, st,
+                     reinterpret_cast<const uint32_t*>(rgba8), n_pixels, lut, reinterpret_cast<float4*>(out));
+}
+
+// Framebuffer::to_u8_srgb (cpu/rasterizer.rs:38-49): alpha (clamp(v) * 255)
+// as u8; RGB (linear_to_srgb(clamp(v)) * 255) as u8 with linear_to_srgb
+// (rasterizer.rs:271-279). powf(x, 1/2.4) is the f64 pow of the f32 inputs
+// rounded to f32 (the correctly rounded value but for double-rounding
+// corner cases; glibc's powf is within 0.82 ulp of it): a byte can differ
+// only where the scaled value lies within an ulp of an integer.
+__device__ __forceinline__ float linear_to_srgb(float x) {
+  if (x <= 0.0031308f) return 12.92f * x;
+  return 1.055f * (float)pow((double)x, (double)(1.0f / 2.4f)) - 0.055f;
+}
+__device__ __forceinline__ uint32_t as_u8_trunc(float x) {  // Rust `as u8`
+  if (!(x > 0.0f)) return 0u;
+  if (x >= 255.0f) return 255u;
+  return (uint32_t)x;
+}
+__global__ __launch_bounds__(256) void k_to_u8_srgb(const float4* __restrict__ in, size_t n, uint32_t* __restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 v = in[i];
+  const float c[4] = {v.x, v.y, v.z, v.w};
+  uint32_t r = 0;
+  for (int k = 0; k < 4; ++k) {
+    const float x = clamp_rs(c[k], 0.0f, 1.0f);
+    r |= as_u8_trunc((k == 3 ? x : linear_to_srgb(x)) * 255.0f) << (8 * k);
+  }
+  out[i] = r;
+}
+
+void launch_to_u8_srgb(const float* rgba, size_t n_pixels, uint8_t* out, hipStream_t st) {
+  if (!n_pixels) return;
+  hipLaunchKernelGGL(k_to_u8_srgb, dim3((unsigned)((n_pixels + 255) / 256)), dim3(256), 0, st,
+                     reinterpret_cast<const float4*>(rgba), n_pixels, reinterpret_cast<uint32_t*>(out));
+}
+
+}  // namespace rgs

@@ -237,6 +237,7 @@ struct rgs_ctx {
   // device-resident scene
   DBuf P, P2, M1, M2, gacc, gden;
   DBuf io_soa, io_rec, io_fields;  // model files: rows of host records, records, PLY property table
+  DBuf srgb_lut;                   // RGBA8 -> linear table (512 floats), filled once
   size_t n = 0, cap = 0, m_rows = 0;
   uint64_t t = 0;
   std::vector<CamDev> reg_cams;
@@ -729,7 +730,7 @@ void rgs_destroy(rgs_ctx* c) {
                  &c->api_target, &c->api_m1, &c->api_m2, &c->api_acc, &c->api_den, &c->api_P2, &c->dens_want,
                  &c->dens_cnt, &c->dens_W, &c->dens_packed, &c->dens_prefix, &c->dens_b32, &c->dens_b64,
                  &c->dens_t32, &c->dens_t64, &c->first_splat, &c->P, &c->P2, &c->M1, &c->M2, &c->gacc, &c->gden, &c->targets,
-                 &c->io_soa, &c->io_rec, &c->io_fields,
+                 &c->io_soa, &c->io_rec, &c->io_fields, &c->srgb_lut,
                  &c->adam_d, &c->api_acc9, &c->tile_order[0], &c->tile_order[1], &c->cover, &c->depth,
                  &c->flag, &c->tile_flag, &c->rbstart, &c->rbend, &c->pv2, &c->cntb, &c->cutst, &c->part,
                  &c->kB, &c->vB, &c->kC[0], &c->kC[1], &c->vC[0], &c->vC[1]};
@@ -1405,6 +1406,53 @@ int rgs_adam_download(rgs_ctx* c, rgs_adam* st_) {
   return 0;
 }
 
+// ---- colour conversions (io/images.rs:63-82, cpu/rasterizer.rs:38-49) -------
+}  // extern "C"
+namespace {
+// srgb_to_linear (io/images.rs:76-82) of c / 255 for every byte c, with the
+// libm powf the reference calls; then c / 255 (alpha).
+int srgb_table(rgs_ctx* c) {
+  if (c->srgb_lut.p) return 0;
+  float t[512];
+  for (int v = 0; v < 256; ++v) {
+    const float x = (float)v / 255.0f;
+    t[v] = x <= 0.04045f ? x / 12.92f : powf((x + 0.055f) / 1.055f, 2.4f);
+    t[256 + v] = x;
+  }
+  CHK(ensure(c, c->srgb_lut, sizeof(t)));
+  HIPCHK(hipMemcpy(c->srgb_lut.p, t, sizeof(t), hipMemcpyHostToDevice));
+  return 0;
+}
+}  // namespace
+extern "C" {
+
+int rgs_rgba8_to_linear(rgs_ctx* c, const uint8_t* rgba8, size_t n_pixels, float* out) {
+  if (!c || (n_pixels && (!rgba8 || !out))) return fail(RGS_E_ARG, "null argument");
+  if (!n_pixels) return 0;
+  (void)hipSetDevice(c->device);
+  CHK(srgb_table(c));
+  CHK(ensure(c, c->io_soa, n_pixels * 4));
+  CHK(ensure(c, c->io_rec, n_pixels * 16));
+  HIPCHK(hipMemcpyAsync(c->io_soa.p, rgba8, n_pixels * 4, hipMemcpyHostToDevice, c->st));
+  launch_rgba8_to_linear(P_<uint8_t>(c->io_soa), n_pixels, P_<float>(c->srgb_lut), P_<float>(c->io_rec), c->st);
+  HIPCHK(hipMemcpyAsync(out, c->io_rec.p, n_pixels * 16, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int rgs_to_u8_srgb(rgs_ctx* c, const float* rgba, size_t n_pixels, uint8_t* out) {
+  if (!c || (n_pixels && (!rgba || !out))) return fail(RGS_E_ARG, "null argument");
+  if (!n_pixels) return 0;
+  (void)hipSetDevice(c->device);
+  CHK(ensure(c, c->io_rec, n_pixels * 16));
+  CHK(ensure(c, c->io_soa, n_pixels * 4));
+  HIPCHK(hipMemcpyAsync(c->io_rec.p, rgba, n_pixels * 16, hipMemcpyHostToDevice, c->st));
+  launch_to_u8_srgb(P_<float>(c->io_rec), n_pixels, P_<uint8_t>(c->io_soa), c->st);
+  HIPCHK(hipMemcpyAsync(out, c->io_soa.p, n_pixels * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
 int rgs_set_views(rgs_ctx* c, const rgs_camera* cams, const float* const* targets, int nv) {
   if (!c || nv <= 0 || !cams || !targets) return fail(RGS_E_ARG, "null argument or no views");
   (void)hipSetDevice(c->device);
@@ -1422,6 +1470,32 @@ int rgs_set_views(rgs_ctx* c, const rgs_camera* cams, const float* const* target
   for (int i = 0; i < nv; ++i)
     HIPCHK(hipMemcpyAsync(P_<float>(c->targets) + (size_t)v[i].tgt_base * 4, targets[i],
                           (size_t)cams[i].width * cams[i].height * 16, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->reg_cams = v;
+  c->target_pix = tot;
+  return 0;
+}
+
+int rgs_set_views_rgba8(rgs_ctx* c, const rgs_camera* cams, const uint8_t* const* images, int nv) {
+  if (!c || nv <= 0 || !cams || !images) return fail(RGS_E_ARG, "null argument or no views");
+  (void)hipSetDevice(c->device);
+  drop_graph(c);
+  std::vector<CamDev> v(nv);
+  size_t tot = 0;
+  for (int i = 0; i < nv; ++i) {
+    if (!images[i]) return fail(RGS_E_ARG, "null image");
+    CHK(make_camdev(cams[i], v[i]));
+    v[i].tgt_base = (uint32_t)tot;
+    tot += (size_t)cams[i].width * cams[i].height;
+  }
+  if (tot >= (1ull << 32)) return fail(RGS_E_ARG, "targets too large");
+  CHK(srgb_table(c));
+  CHK(ensure(c, c->targets, tot * 16));
+  CHK(ensure(c, c->io_soa, tot * 4));
+  for (int i = 0; i < nv; ++i)
+    HIPCHK(hipMemcpyAsync(P_<uint8_t>(c->io_soa) + (size_t)v[i].tgt_base * 4, images[i],
+                          (size_t)cams[i].width * cams[i].height * 4, hipMemcpyHostToDevice, c->st));
+  launch_rgba8_to_linear(P_<uint8_t>(c->io_soa), tot, P_<float>(c->srgb_lut), P_<float>(c->targets), c->st);
   HIPCHK(hipStreamSynchronize(c->st));
   c->reg_cams = v;
   c->target_pix = tot;

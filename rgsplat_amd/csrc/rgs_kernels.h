@@ -273,6 +273,10 @@ int ply_decode_tiled_fits(int rec_bytes);
 void launch_ply_decode_tiled(const uint8_t* body, size_t n, int rec_bytes, const PlyField* row_field, int big_endian,
                              float* P, size_t cap, hipStream_t st);
 void launch_splat_encode(const float* P, size_t cap, size_t n, void* out /* n x 32 bytes */, hipStream_t st);
+// Colour conversions (k_io.hip). lut: 512 floats, [0, 256) sRGB -> linear of
+// c / 255, [256, 512) c / 255.
+void launch_rgba8_to_linear(const uint8_t* rgba8, size_t n_pixels, const float* lut, float* out, hipStream_t st);
+void launch_to_u8_srgb(const float* rgba, size_t n_pixels, uint8_t* out, hipStream_t st);
 // reset_opacities on SoA (row 10) or AoS (offset 3) params.
 void launch_reset_opacity(float* p, size_t n, size_t stride, hipStream_t st);
 // Densify/prune on SoA params: P -> P_out (compacted + children). Writes the

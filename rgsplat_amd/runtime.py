@@ -71,6 +71,9 @@ SIGNATURES = {
     "rgs_scene_upload": [_vp, _vp, _sz, _sz],
     "rgs_scene_size": [_vp, ctypes.POINTER(_sz)],
     "rgs_scene_download": [_vp, _vp, _sz, ctypes.POINTER(_sz)],
+    "rgs_set_views_rgba8": [_vp, _vp, _vp, _i],
+    "rgs_rgba8_to_linear": [_vp, _vp, _sz, _vp],
+    "rgs_to_u8_srgb": [_vp, _vp, _sz, _vp],
     "rgs_save_ply": [_vp, _vp, _sz, ctypes.c_char_p],
     "rgs_scene_save_ply": [_vp, ctypes.c_char_p],
     "rgs_load_ply": [_vp, ctypes.c_char_p, _vp, _sz, ctypes.POINTER(_sz)],
@@ -282,6 +285,32 @@ class Context:
         nn = ctypes.c_size_t()
         _check(lib().rgs_scene_download(self.h, _p(out), len(out), ctypes.byref(nn)))
         return out[: nn.value]
+
+    # ------------------------------------------------------ colour conversions
+    def rgba8_to_linear(self, img) -> np.ndarray:
+        """load_target's conversion of decoded RGBA8 (h, w, 4) to linear f32."""
+        img = np.ascontiguousarray(img, np.uint8)
+        out = np.zeros(img.shape, np.float32)
+        _check(lib().rgs_rgba8_to_linear(self.h, _p(img), img.size // 4, _p(out)))
+        return out
+
+    def to_u8_srgb(self, rgba) -> np.ndarray:
+        """Framebuffer::to_u8_srgb of an RGBA f32 (h, w, 4) image."""
+        rgba = _f32(rgba)
+        out = np.zeros(rgba.shape, np.uint8)
+        _check(lib().rgs_to_u8_srgb(self.h, _p(rgba), rgba.size // 4, _p(out)))
+        return out
+
+    def set_views_rgba8(self, cams, images):
+        """set_views from decoded RGBA8 images of the cameras' sizes."""
+        cs = (Camera * len(cams))(*[camera(c) for c in cams])
+        ims = [np.ascontiguousarray(im, np.uint8) for im in images]
+        for c, im in zip(cams, ims):
+            if im.shape != (c.height, c.width, 4):
+                raise ValueError(f"image {im.shape} does not match camera {(c.height, c.width, 4)}")
+        ptrs = (ctypes.c_void_p * len(ims))(*[im.ctypes.data for im in ims])
+        _check(lib().rgs_set_views_rgba8(self.h, cs, ptrs, len(ims)))
+        self._views_keepalive = ims
 
     # ------------------------------------------------------------ model files
     def save_ply(self, gaussians, path):
