@@ -159,6 +159,42 @@ def model_files(runtime, wl, reps):
                       "reps": reps, "roundtrip_bit_exact": bool(ok), **res}), flush=True)
 
 
+def sfm_init(runtime, wl, reps, cpu=True):
+    """Wall time of the SfM initialisation (gaussians_from_sfm) on a point
+    cloud of the workload's size: the workload's scene positions as f64 SfM
+    points with random colours, host arrays in, device scene out; the oracle
+    on the same cloud on the host cores beside it."""
+    g = wl.scene()
+    rng = np.random.default_rng(11)
+    xyz = g[:, 0:3].astype(np.float64)
+    rgb = rng.integers(0, 256, (len(g), 3), dtype=np.uint8)
+    del g
+    ctx = runtime.Context(0)
+    ctx.scene_from_sfm(xyz, rgb)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.scene_from_sfm(xyz, rgb)
+    ms = (time.perf_counter() - t0) * 1000.0 / reps
+    print(f"[bench] scene_from_sfm {ms:.1f} ms", file=sys.stderr, flush=True)
+    res = {"scene_from_sfm_ms": ms}
+    if cpu:
+        import oracle
+        t0 = time.perf_counter()
+        want, _ = oracle.gaussians_from_sfm(xyz, rgb)
+        res["cpu_oracle_ms"] = (time.perf_counter() - t0) * 1000.0
+        res["cpu_threads"] = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        got = ctx.scene_download()
+        a, b = got.view(np.uint32), want.view(np.uint32)
+        other = np.ones(59, bool)
+        other[8:11] = False
+        res["bit_exact_except_log_scale"] = bool(np.array_equal(a[:, other], b[:, other]))
+        res["log_scale_max_ulp"] = int(np.abs(a[:, 8:11].astype(np.int64) - b[:, 8:11].astype(np.int64)).max())
+        print(f"[bench] oracle gaussians_from_sfm {res['cpu_oracle_ms']:.0f} ms", file=sys.stderr, flush=True)
+    ctx.close()
+    print(json.dumps({"metric": "SfM initialisation (wall ms per call, host arrays in, device scene out)",
+                      "points": len(xyz), "reps": reps, **res}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,6 +208,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default raster work decomposition")
     ap.add_argument("--io", action="store_true",
                     help="time the model files instead (PLY save/load and .splat save of the workload's scene)")
+    ap.add_argument("--sfm", action="store_true",
+                    help="time the SfM initialisation instead (gaussians_from_sfm on a cloud of the workload's size)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the step shards by views "
@@ -184,6 +222,9 @@ def main():
     wl = S.WORKLOADS[args.workload]
     if args.io:
         model_files(runtime, wl, args.steps)
+        return
+    if args.sfm:
+        sfm_init(runtime, wl, args.steps, cpu=not args.no_cpu)
         return
     print(f"[bench] workload {wl.name}: {wl.n} Gaussians, views {wl.sizes}", file=sys.stderr, flush=True)
     g = wl.scene()

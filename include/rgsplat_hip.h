@@ -218,6 +218,18 @@ int rgs_get_raster_params(rgs_ctx* ctx, int* super_tile, int* sparse_lanes);
 int rgs_autotune_raster(rgs_ctx* ctx, const int* view_ids, int n_ids, const float bg[3], float lambda_dssim,
                         int* super_tile, int* sparse_lanes, float* raster_ms);
 
+/* ---- SfM initialisation (pipeline/sfm.rs:365-455). gaussians_from_sfm for
+ * n points (xyz: n x 3 f64, rgb: n x 3 bytes; SfMPoint's other fields are
+ * not used): positions, sRGB-to-linear DC colours, identity rotations,
+ * opacity logit(0.1) and isotropic log-scales from the mean distance to the
+ * 3 nearest of a 50-point window in x order (estimate_initial_scales). The
+ * log is the f64 log rounded to f32 (within 1 ulp of the reference's logf);
+ * every other value is the reference's. RGS_E_NAN where the reference panics
+ * (a NaN x, or a NaN distance in a window). rgs_scene_from_sfm builds the
+ * device scene directly (as rgs_scene_upload). */
+int rgs_gaussians_from_sfm(rgs_ctx* ctx, const double* xyz, const uint8_t* rgb, size_t n, rgs_gaussian* out);
+int rgs_scene_from_sfm(rgs_ctx* ctx, const double* xyz, const uint8_t* rgb, size_t n, size_t capacity);
+
 /* ---- Model files (io/ply.rs). The records move between the device
  * parameter rows and the file as one block; headers are text on the host.
  * rgs_save_ply replaces save_ply (io/ply.rs:35-102): binary little-endian

@@ -109,6 +109,8 @@ def _declare(L):
                                         ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(TrainCfg), ctypes.c_float]
     L.orc_set_threads.argtypes = [i32]
     L.orc_expf_batch.argtypes = [vp, vp, sz]
+    L.orc_gaussians_from_sfm.restype = i32
+    L.orc_gaussians_from_sfm.argtypes = [vp, vp, sz, vp, vp]
     L.orc_rgba8_to_linear.argtypes = [vp, sz, vp]
     L.orc_to_u8_srgb.argtypes = [vp, sz, vp]
     L.orc_save_ply.restype = i32
@@ -373,3 +375,16 @@ def to_u8_srgb(rgba):
     out = np.zeros(rgba.shape, np.uint8)
     lib().orc_to_u8_srgb(_p(rgba), rgba.size // 4, _p(out))
     return out
+
+
+# ---- SfM initialisation (pipeline/sfm.rs:365-455) ---------------------------
+def gaussians_from_sfm(xyz, rgb):
+    """(Gaussian3D records (n, 59), per-point KNN scales (n,))."""
+    xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+    rgb = np.ascontiguousarray(rgb, np.uint8).reshape(-1, 3)
+    n = len(xyz)
+    out = np.zeros((max(n, 1), 59), np.float32)
+    sc = np.zeros(max(n, 1), np.float32)
+    if lib().orc_gaussians_from_sfm(_p(xyz), _p(rgb), n, _p(out), _p(sc)):
+        raise ValueError("NaN coordinate (the reference panics)")
+    return out[:n], sc[:n]

@@ -1354,6 +1354,90 @@ int orc_densify_and_prune(float* g, size_t* n_io, size_t cap, float* m1, float* 
   return 0;
 }
 
+// ---- pipeline/sfm.rs:365-455: Gaussians from an SfM point cloud.
+// Rust's slice::binary_search_by as in the std of edition-2024 toolchains
+// (no early exit; on equal keys it ends on the last of them), for the search
+// of a point's own x in the x-sorted points (:425-427).
+static size_t rust_binary_search_x(const std::vector<std::pair<size_t, double>>& s, double key) {
+  size_t size = s.size();
+  if (size == 0) return 0;  // Err(0) -> unwrap_or(0)
+  size_t base = 0;
+  while (size > 1) {
+    const size_t half = size / 2, mid = base + half;
+    base = (s[mid].second > key) ? base : mid;  // cmp == Greater keeps base
+    size -= half;
+  }
+  const double v = s[base].second;
+  if (v == key) return base;                  // Ok(base)
+  return 0;                                   // Err(_) -> unwrap_or(0)
+}
+// Returns 0, or 1 when the reference panics (a NaN in partial_cmp().unwrap()).
+int orc_gaussians_from_sfm(const double* xyz, const uint8_t* rgb, size_t n, float* out, float* scales_out) {
+  // estimate_initial_scales (:410-455)
+  std::vector<std::pair<size_t, double>> sorted(n);
+  for (size_t i = 0; i < n; ++i) {
+    if (xyz[3 * i] != xyz[3 * i]) return 1;
+    sorted[i] = {i, xyz[3 * i]};
+  }
+  std::stable_sort(sorted.begin(), sorted.end(), [](const auto& a, const auto& b) { return a.second < b.second; });
+  std::vector<float> scales(n, 0.1f);
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(| : bad)
+  for (long ii = 0; ii < (long)n; ++ii) {
+    const size_t i = (size_t)ii;
+    const double* p = xyz + 3 * i;
+    const size_t window = std::min<size_t>(50, n);
+    const size_t idx = rust_binary_search_x(sorted, p[0]);
+    const size_t start = idx >= window / 2 ? idx - window / 2 : 0;
+    const size_t end = std::min(start + window, n);
+    std::vector<double> d;
+    for (size_t k = start; k < end; ++k) {
+      if (sorted[k].first == i) continue;
+      const double* q = xyz + 3 * sorted[k].first;
+      const double dx = q[0] - p[0], dy = q[1] - p[1], dz = q[2] - p[2];
+      d.push_back(std::sqrt(dx * dx + dy * dy + dz * dz));
+    }
+    // sort_by(partial_cmp().unwrap()) panics on a NaN as soon as it compares
+    // anything: from two distances on
+    if (d.size() >= 2) {
+      bool nan = false;
+      for (double v : d) nan |= v != v;
+      if (nan) {
+        bad = 1;
+        continue;
+      }
+    }
+    std::sort(d.begin(), d.end());
+    if (d.empty()) {
+      scales[i] = 0.01f;
+    } else {
+      const size_t k = std::min<size_t>(3, d.size());
+      double sum = -0.0;  // f64 Sum
+      for (size_t q = 0; q < k; ++q) sum += d[q];
+      scales[i] = (float)(sum / (double)k);
+    }
+  }
+  if (bad) return 1;
+  // gaussians_from_sfm (:365-407)
+  const float SH_C0_INV = 1.0f / 0.28209479177387814f;
+  const float op = logf(0.1f / (1.0f - 0.1f));  // math::logit(0.1)
+  for (size_t i = 0; i < n; ++i) {
+    float* g = out + i * G_N;
+    std::fill(g, g + G_N, 0.0f);
+    for (int d = 0; d < 3; ++d) g[G_POS + d] = (float)xyz[3 * i + d];
+    g[G_ROT] = 1.0f;
+    for (int ch = 0; ch < 3; ++ch) {
+      const float c = (float)rgb[3 * i + ch] / 255.0f;
+      g[G_SH + 16 * ch] = (powf(c, 2.2f) - 0.5f) * SH_C0_INV;
+    }
+    const float s = logf(fmaxf(scales[i], 1e-4f));
+    g[G_LS + 0] = g[G_LS + 1] = g[G_LS + 2] = s;
+    g[G_OPL] = op;
+    if (scales_out) scales_out[i] = scales[i];
+  }
+  return 0;
+}
+
 // ---- io/ply.rs: model files. Records are Gaussian3D (AoS, 59 floats).
 // save_ply (io/ply.rs:35-102): binary little-endian 3DGS layout.
 int orc_save_ply(const float* g, size_t n, const char* path) {

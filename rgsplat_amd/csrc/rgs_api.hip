@@ -238,6 +238,7 @@ struct rgs_ctx {
   DBuf P, P2, M1, M2, gacc, gden;
   DBuf io_soa, io_rec, io_fields;  // model files: rows of host records, records, PLY property table
   DBuf srgb_lut;                   // RGBA8 -> linear table (512 floats), filled once
+  DBuf sfm_xyz, sfm_rgb, sfm_k[2], sfm_v[2], sfm_khi, sfm_sx, sfm_scale, sfm_hist, sfm_rowtot, sfm_misc;
   size_t n = 0, cap = 0, m_rows = 0;
   uint64_t t = 0;
   std::vector<CamDev> reg_cams;
@@ -730,7 +731,9 @@ void rgs_destroy(rgs_ctx* c) {
                  &c->api_target, &c->api_m1, &c->api_m2, &c->api_acc, &c->api_den, &c->api_P2, &c->dens_want,
                  &c->dens_cnt, &c->dens_W, &c->dens_packed, &c->dens_prefix, &c->dens_b32, &c->dens_b64,
                  &c->dens_t32, &c->dens_t64, &c->first_splat, &c->P, &c->P2, &c->M1, &c->M2, &c->gacc, &c->gden, &c->targets,
-                 &c->io_soa, &c->io_rec, &c->io_fields, &c->srgb_lut,
+                 &c->io_soa, &c->io_rec, &c->io_fields, &c->srgb_lut, &c->sfm_xyz, &c->sfm_rgb, &c->sfm_k[0],
+                 &c->sfm_k[1], &c->sfm_v[0], &c->sfm_v[1], &c->sfm_khi, &c->sfm_sx, &c->sfm_scale, &c->sfm_hist,
+                 &c->sfm_rowtot, &c->sfm_misc,
                  &c->adam_d, &c->api_acc9, &c->tile_order[0], &c->tile_order[1], &c->cover, &c->depth,
                  &c->flag, &c->tile_flag, &c->rbstart, &c->rbend, &c->pv2, &c->cntb, &c->cutst, &c->part,
                  &c->kB, &c->vB, &c->kC[0], &c->kC[1], &c->vC[0], &c->vC[1]};
@@ -1403,6 +1406,91 @@ int rgs_adam_download(rgs_ctx* c, rgs_adam* st_) {
   }
   st_->rows = rows;
   st_->t = c->t;
+  return 0;
+}
+
+// ---- SfM initialisation (pipeline/sfm.rs:365-455) ---------------------------
+}  // extern "C"
+namespace {
+// Rows [59][cap] of the Gaussians gaussians_from_sfm makes from n points.
+int sfm_rows(rgs_ctx* c, const double* xyz, const uint8_t* rgb, size_t n, float* rows, size_t cap) {
+  if (!n) return 0;
+  if (n >= (1u << 31)) return fail(RGS_E_ARG, "too many points");
+  const uint32_t n32 = (uint32_t)n;
+  const int nbm = radix_nb_max(n);
+  CHK(ensure(c, c->sfm_xyz, n * 24));
+  CHK(ensure(c, c->sfm_rgb, (n * 3 + 3) / 4 * 4));
+  for (int b = 0; b < 2; ++b) {
+    CHK(ensure(c, c->sfm_k[b], n * 4));
+    CHK(ensure(c, c->sfm_v[b], n * 4));
+  }
+  CHK(ensure(c, c->sfm_khi, n * 4));
+  CHK(ensure(c, c->sfm_sx, n * 24));
+  CHK(ensure(c, c->sfm_scale, n * 4));
+  CHK(ensure(c, c->sfm_hist, (size_t)256 * nbm * 4));
+  CHK(ensure(c, c->sfm_rowtot, 256 * 4));
+  CHK(ensure(c, c->sfm_misc, (2 + 256) * 4));  // n, status, dc table
+  // dc[c] = (powf(c / 255, 2.2) - 0.5) * SH_C0_INV with the reference's f32
+  // constant 1 / 0.28209479177387814 and libm's powf; logit(0.1) likewise
+  const float sh_c0_inv = 1.0f / 0.28209479177387814f;
+  uint32_t misc[2 + 256];
+  misc[0] = n32;
+  misc[1] = 0;
+  for (int v = 0; v < 256; ++v) {
+    const float dcv = (powf((float)v / 255.0f, 2.2f) - 0.5f) * sh_c0_inv;
+    std::memcpy(&misc[2 + v], &dcv, 4);
+  }
+  const float opacity_logit = logf(0.1f / (1.0f - 0.1f));
+  HIPCHK(hipMemcpyAsync(c->sfm_xyz.p, xyz, n * 24, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->sfm_rgb.p, rgb, n * 3, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->sfm_misc.p, misc, sizeof(misc), hipMemcpyHostToDevice, c->st));
+  uint32_t* n_dev = P_<uint32_t>(c->sfm_misc);
+  uint32_t* status = n_dev + 1;
+  const float* dc = reinterpret_cast<const float*>(n_dev + 2);
+  const double* X = P_<double>(c->sfm_xyz);
+  RadixScratch rs{P_<uint32_t>(c->sfm_hist), P_<uint32_t>(c->sfm_rowtot), nbm};
+  // stable sort by x: low key word, then high key word
+  launch_sfm_keys(X, n32, P_<uint32_t>(c->sfm_k[0]), P_<uint32_t>(c->sfm_khi), P_<uint32_t>(c->sfm_v[0]), status,
+                  c->st);
+  uint32_t* ka[2] = {P_<uint32_t>(c->sfm_k[0]), P_<uint32_t>(c->sfm_k[1])};
+  uint32_t* va[2] = {P_<uint32_t>(c->sfm_v[0]), P_<uint32_t>(c->sfm_v[1])};
+  const int a = radix_sort_pairs(ka, va, n_dev, n, 32, rs, c->st);
+  launch_sfm_gather_hi(P_<uint32_t>(c->sfm_khi), va[a], n32, ka[1 - a], c->st);
+  uint32_t* kb[2] = {ka[1 - a], ka[a]};
+  uint32_t* vb[2] = {va[a], va[1 - a]};
+  const int b = radix_sort_pairs(kb, vb, n_dev, n, 32, rs, c->st);
+  const uint32_t* order = vb[b];
+  launch_sfm_sorted_xyz(X, order, n32, P_<double>(c->sfm_sx), c->st);
+  launch_sfm_scale(P_<double>(c->sfm_sx), order, n32, P_<float>(c->sfm_scale), status, c->st);
+  uint32_t st_h = 0;
+  HIPCHK(hipMemcpyAsync(&st_h, status, 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (st_h) return fail(RGS_E_NAN, "NaN coordinate or point distance (the reference panics in its sorts)");
+  launch_sfm_gaussians(X, P_<uint8_t>(c->sfm_rgb), P_<float>(c->sfm_scale), n32, dc, opacity_logit, rows, cap, c->st);
+  return 0;
+}
+}  // namespace
+extern "C" {
+
+int rgs_gaussians_from_sfm(rgs_ctx* c, const double* xyz, const uint8_t* rgb, size_t n, rgs_gaussian* out) {
+  if (!c || (n && (!xyz || !rgb || !out))) return fail(RGS_E_ARG, "null argument");
+  if (!n) return 0;
+  (void)hipSetDevice(c->device);
+  CHK(ensure(c, c->io_soa, n * 59 * 4));
+  CHK(ensure(c, c->api_aos, n * 59 * 4));
+  CHK(sfm_rows(c, xyz, rgb, n, P_<float>(c->io_soa), n));
+  launch_soa_to_aos(P_<float>(c->io_soa), n, n, P_<float>(c->api_aos), c->st);
+  HIPCHK(hipMemcpyAsync(out, c->api_aos.p, n * 59 * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int rgs_scene_from_sfm(rgs_ctx* c, const double* xyz, const uint8_t* rgb, size_t n, size_t capacity) {
+  if (!c || (n && (!xyz || !rgb))) return fail(RGS_E_ARG, "null argument");
+  (void)hipSetDevice(c->device);
+  CHK(scene_reset(c, n, capacity));
+  CHK(sfm_rows(c, xyz, rgb, n, P_<float>(c->P), c->cap));
+  HIPCHK(hipStreamSynchronize(c->st));
   return 0;
 }
 

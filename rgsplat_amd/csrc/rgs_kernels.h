@@ -273,6 +273,15 @@ int ply_decode_tiled_fits(int rec_bytes);
 void launch_ply_decode_tiled(const uint8_t* body, size_t n, int rec_bytes, const PlyField* row_field, int big_endian,
                              float* P, size_t cap, hipStream_t st);
 void launch_splat_encode(const float* P, size_t cap, size_t n, void* out /* n x 32 bytes */, hipStream_t st);
+// SfM initialisation (k_sfm.hip). status bit 1: a NaN coordinate or distance.
+void launch_sfm_keys(const double* xyz, uint32_t n, uint32_t* klo, uint32_t* khi, uint32_t* idx, uint32_t* status,
+                     hipStream_t st);
+void launch_sfm_gather_hi(const uint32_t* khi, const uint32_t* order, uint32_t n, uint32_t* keys, hipStream_t st);
+void launch_sfm_sorted_xyz(const double* xyz, const uint32_t* order, uint32_t n, double* sxyz, hipStream_t st);
+void launch_sfm_scale(const double* sxyz, const uint32_t* order, uint32_t n, float* scale, uint32_t* status,
+                      hipStream_t st);
+void launch_sfm_gaussians(const double* xyz, const uint8_t* rgb, const float* scale, uint32_t n, const float* dc,
+                          float opacity_logit, float* P, size_t cap, hipStream_t st);
 // Colour conversions (k_io.hip). lut: 512 floats, [0, 256) sRGB -> linear of
 // c / 255, [256, 512) c / 255.
 void launch_rgba8_to_linear(const uint8_t* rgba8, size_t n_pixels, const float* lut, float* out, hipStream_t st);

@@ -80,6 +80,8 @@ SIGNATURES = {
     "rgs_scene_load_ply": [_vp, ctypes.c_char_p, _sz],
     "rgs_save_splat": [_vp, _vp, _sz, ctypes.c_char_p],
     "rgs_scene_save_splat": [_vp, ctypes.c_char_p],
+    "rgs_gaussians_from_sfm": [_vp, _vp, _vp, _sz, _vp],
+    "rgs_scene_from_sfm": [_vp, _vp, _vp, _sz, _sz],
     "rgs_adam_download": [_vp, _ADAM],
     "rgs_set_views": [_vp, _CAM, ctypes.POINTER(ctypes.c_void_p), _i],
     "rgs_train_step": [_vp, ctypes.POINTER(_i), _i, _CFG, _vp, _f, _i, _vp],
@@ -335,6 +337,25 @@ class Context:
     def scene_load_ply(self, path, capacity: int = 0):
         """A PLY into the device scene (as scene_upload)."""
         _check(lib().rgs_scene_load_ply(self.h, os.fsencode(path), int(capacity)))
+
+    def gaussians_from_sfm(self, xyz, rgb) -> np.ndarray:
+        """gaussians_from_sfm (pipeline/sfm.rs:365-407) of n SfM points
+        (xyz (n, 3) f64, rgb (n, 3) u8): (n, 59) Gaussian3D records."""
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        rgb = np.ascontiguousarray(rgb, np.uint8).reshape(-1, 3)
+        if len(xyz) != len(rgb):
+            raise ValueError("xyz and rgb differ in length")
+        out = np.zeros((max(len(xyz), 1), 59), np.float32)
+        _check(lib().rgs_gaussians_from_sfm(self.h, _p(xyz), _p(rgb), len(xyz), _p(out)))
+        return out[: len(xyz)]
+
+    def scene_from_sfm(self, xyz, rgb, capacity: int = 0):
+        """The Gaussians of gaussians_from_sfm straight into the device scene."""
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        rgb = np.ascontiguousarray(rgb, np.uint8).reshape(-1, 3)
+        if len(xyz) != len(rgb):
+            raise ValueError("xyz and rgb differ in length")
+        _check(lib().rgs_scene_from_sfm(self.h, _p(xyz), _p(rgb), len(xyz), int(capacity)))
 
     def save_splat(self, gaussians, path):
         """save_splat (io/ply.rs:258-303) of (n, 59) Gaussian3D records."""
