@@ -27,7 +27,14 @@
 // (sh.rs:145-156, quaternion.rs:77-106, covariance.rs:91-121,
 // backward.rs:848-982, tests/training_reduces_loss.rs:38-84), fp64 central
 // finite differences of every gradient, brute-force binning, a closed-form
-// on-axis isotropic image, direct 2-D convolution SSIM and textbook Adam.
+// on-axis isotropic image, direct 2-D convolution SSIM and textbook Adam;
+// for the model files, the reference's io/ply.rs tests and hand-derived
+// records of crafted files; for the colour conversions, the reference's
+// images.rs/rasterizer.rs tests and closed forms; for the SfM initialisation
+// (pipeline/sfm.rs:365-455), points on a line, brute-force 3-NN and a numpy
+// restatement of the window search. Rust's std sort and binary search are
+// restated from the std library's documented behaviour (stable sort_by;
+// binary_search_by as shipped since Rust 1.82, ending on the last equal key).
 
 #include <strings.h>
 
