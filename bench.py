@@ -195,6 +195,37 @@ def sfm_init(runtime, wl, reps, cpu=True):
                       "points": len(xyz), "reps": reps, **res}), flush=True)
 
 
+def preview(runtime, wl, reps, cpu=True):
+    """Wall time of the preview render (GpuRenderer::render) of the
+    workload's scene, resident on the device, at each of its views; the
+    oracle on the first view beside it."""
+    g = wl.scene()
+    cams = wl.cameras()
+    ctx = runtime.Context(0)
+    ctx.scene_upload(g)
+    bg = (0.0, 0.0, 0.0)
+    img = ctx.scene_preview_render(cams[0], bg)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for cam in cams:
+            ctx.scene_preview_render(cam, bg)
+    ms = (time.perf_counter() - t0) * 1000.0 / (reps * len(cams))
+    print(f"[bench] scene_preview_render {ms:.2f} ms per view", file=sys.stderr, flush=True)
+    res = {"scene_preview_render_ms_per_view": ms}
+    if cpu:
+        import oracle
+        t0 = time.perf_counter()
+        want = oracle.preview_render(g, cams[0], bg)
+        res["cpu_oracle_ms_first_view"] = (time.perf_counter() - t0) * 1000.0
+        res["cpu_threads"] = 1
+        res["bit_exact_first_view"] = bool(np.array_equal(img.view(np.uint32), want.view(np.uint32)))
+        print(f"[bench] oracle preview_render {res['cpu_oracle_ms_first_view']:.0f} ms", file=sys.stderr, flush=True)
+    ctx.close()
+    print(json.dumps({"metric": "preview render (wall ms per view, device scene in, host image out)",
+                      "gaussians": len(g), "views": [[c.width, c.height] for c in cams], "reps": reps, **res}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,6 +241,8 @@ def main():
                     help="time the model files instead (PLY save/load and .splat save of the workload's scene)")
     ap.add_argument("--sfm", action="store_true",
                     help="time the SfM initialisation instead (gaussians_from_sfm on a cloud of the workload's size)")
+    ap.add_argument("--preview", action="store_true",
+                    help="time the preview renderer instead (GpuRenderer::render of the workload's scene)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the step shards by views "
@@ -225,6 +258,9 @@ def main():
         return
     if args.sfm:
         sfm_init(runtime, wl, args.steps, cpu=not args.no_cpu)
+        return
+    if args.preview:
+        preview(runtime, wl, args.steps, cpu=not args.no_cpu)
         return
     print(f"[bench] workload {wl.name}: {wl.n} Gaussians, views {wl.sizes}", file=sys.stderr, flush=True)
     g = wl.scene()

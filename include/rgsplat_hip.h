@@ -218,6 +218,19 @@ int rgs_get_raster_params(rgs_ctx* ctx, int* super_tile, int* sparse_lanes);
 int rgs_autotune_raster(rgs_ctx* ctx, const int* view_ids, int n_ids, const float bg[3], float lambda_dssim,
                         int* super_tile, int* sparse_lanes, float* raster_ms);
 
+/* ---- Preview renderer (gpu/compute.rs:138-297, gpu/splat.wgsl:120-190).
+ * GpuRenderer's API: project gives n SplatInfo2D records of 10 floats (sx,
+ * sy, conic a, b, c, colour r, g, b, alpha, depth; all zero where the shader
+ * culls), render gives linear RGB h*w*3: DC colour only, the shader's
+ * covariance (no perspective terms) and composite's 3-sigma rects, nearest
+ * first (equal depths in index order). rgs_scene_preview_render renders the
+ * device scene (a viewer during training, no host copy of the scene).
+ * WGSL math runs as IEEE f32 operations in source order. */
+int rgs_preview_project(rgs_ctx* ctx, const rgs_gaussian* g, size_t n, const rgs_camera* cam, float* splats_out);
+int rgs_preview_render(rgs_ctx* ctx, const rgs_gaussian* g, size_t n, const rgs_camera* cam, const float bg[3],
+                       float* rgb_out);
+int rgs_scene_preview_render(rgs_ctx* ctx, const rgs_camera* cam, const float bg[3], float* rgb_out);
+
 /* ---- SfM initialisation (pipeline/sfm.rs:365-455). gaussians_from_sfm for
  * n points (xyz: n x 3 f64, rgb: n x 3 bytes; SfMPoint's other fields are
  * not used): positions, sRGB-to-linear DC colours, identity rotations,

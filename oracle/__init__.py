@@ -111,6 +111,11 @@ def _declare(L):
     L.orc_expf_batch.argtypes = [vp, vp, sz]
     L.orc_gaussians_from_sfm.restype = i32
     L.orc_gaussians_from_sfm.argtypes = [vp, vp, sz, vp, vp]
+    for f in ("orc_preview_project",):
+        getattr(L, f).restype = i32
+        getattr(L, f).argtypes = [vp, sz, ctypes.POINTER(Camera), vp]
+    L.orc_preview_render.restype = i32
+    L.orc_preview_render.argtypes = [vp, sz, ctypes.POINTER(Camera), vp, vp]
     L.orc_rgba8_to_linear.argtypes = [vp, sz, vp]
     L.orc_to_u8_srgb.argtypes = [vp, sz, vp]
     L.orc_save_ply.restype = i32
@@ -388,3 +393,25 @@ def gaussians_from_sfm(xyz, rgb):
     if lib().orc_gaussians_from_sfm(_p(xyz), _p(rgb), n, _p(out), _p(sc)):
         raise ValueError("NaN coordinate (the reference panics)")
     return out[:n], sc[:n]
+
+
+def preview_project(g, cam):
+    """GpuRenderer::project (gpu/compute.rs:138-207): (n, 10) SplatInfo2D
+    records (sx, sy, conic a, b, c, colour r, g, b, alpha, depth)."""
+    g = np.ascontiguousarray(g, np.float32).reshape(-1, 59)
+    out = np.zeros((max(len(g), 1), 10), np.float32)
+    c = camera(cam)
+    if lib().orc_preview_project(_p(g), len(g), ctypes.byref(c), _p(out)):
+        raise ValueError("singular camera")
+    return out[: len(g)]
+
+
+def preview_render(g, cam, bg=(0.0, 0.0, 0.0)):
+    """GpuRenderer::render (gpu/compute.rs:210-218): (h, w, 3) linear RGB."""
+    g = np.ascontiguousarray(g, np.float32).reshape(-1, 59)
+    out = np.zeros((cam.height, cam.width, 3), np.float32)
+    bgv = np.ascontiguousarray(bg, np.float32)
+    c = camera(cam)
+    if lib().orc_preview_render(_p(g), len(g), ctypes.byref(c), _p(bgv), _p(out)):
+        raise ValueError("singular camera")
+    return out

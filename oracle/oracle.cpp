@@ -1445,6 +1445,157 @@ int orc_gaussians_from_sfm(const double* xyz, const uint8_t* rgb, size_t n, floa
   return 0;
 }
 
+// ---- gpu/compute.rs + gpu/splat.wgsl: the preview renderer (GpuRenderer).
+// SplatInfo2D (compute.rs:65-78): sx, sy, conic_a, conic_b, conic_c, color
+// r, g, b, alpha, depth.
+//
+// splat.wgsl `project` (:120-190) with every WGSL operation as one IEEE f32
+// operation in source order (WGSL allows exp 3 + 2|x| ulp and division
+// 2.5 ulp; this restatement takes the correctly rounded / glibc results).
+// WGSL matrix constructors take columns: quat_to_mat3 is the usual rotation
+// matrix of the unnormalised quaternion, the Jacobian built as
+// mat3x3(vec3(fx/tz, 0, -fx tx/tz^2), vec3(0, fy/tz, -fy ty/tz^2), 0) is
+// the transpose of the perspective Jacobian (so the 2-D covariance keeps no
+// perspective terms), and matrix products sum over k in order.
+static void wgsl_project(const float* g, const float view[16], const Camera& cam, float out[10]) {
+  for (int k = 0; k < 10; ++k) out[k] = 0.0f;  // wgpu output buffers start zeroed
+  float pc[3];
+  for (int r = 0; r < 3; ++r) {  // camera.view * vec4(position, 1)
+    float acc = view[r * 4 + 0] * g[G_POS + 0];
+    acc = acc + view[r * 4 + 1] * g[G_POS + 1];
+    acc = acc + view[r * 4 + 2] * g[G_POS + 2];
+    acc = acc + view[r * 4 + 3] * 1.0f;
+    pc[r] = acc;
+  }
+  const float tx = pc[0], ty = pc[1], tz = pc[2];
+  if (tz <= 0.001f) return;
+  const float alpha = 1.0f / (1.0f + expf(-g[G_OPL]));
+  if (alpha < 1.0f / 255.0f) return;
+  const float sx = cam.fx * tx / tz + cam.cx;
+  const float sy = cam.fy * ty / tz + cam.cy;
+  // quat_to_mat3, q = [w, x, y, z]; R[row][col]
+  const float w = g[G_ROT], x = g[G_ROT + 1], y = g[G_ROT + 2], z = g[G_ROT + 3];
+  float R[3][3];
+  R[0][0] = 1.0f - 2.0f * (y * y + z * z);
+  R[1][0] = 2.0f * (x * y + w * z);
+  R[2][0] = 2.0f * (x * z - w * y);
+  R[0][1] = 2.0f * (x * y - w * z);
+  R[1][1] = 1.0f - 2.0f * (x * x + z * z);
+  R[2][1] = 2.0f * (y * z + w * x);
+  R[0][2] = 2.0f * (x * z + w * y);
+  R[1][2] = 2.0f * (y * z - w * x);
+  R[2][2] = 1.0f - 2.0f * (x * x + y * y);
+  float S[3][3] = {{expf(g[G_LS]), 0.0f, 0.0f}, {0.0f, expf(g[G_LS + 1]), 0.0f}, {0.0f, 0.0f, expf(g[G_LS + 2])}};
+  auto mul = [](const float A[3][3], const float B[3][3], float C[3][3]) {
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) C[r][c] = A[r][0] * B[0][c] + A[r][1] * B[1][c] + A[r][2] * B[2][c];
+  };
+  auto transpose = [](const float A[3][3], float B[3][3]) {
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) B[r][c] = A[c][r];
+  };
+  float RS[3][3], RSt[3][3], sig3[3][3];
+  mul(R, S, RS);
+  transpose(RS, RSt);
+  mul(RS, RSt, sig3);
+  float J[3][3] = {{cam.fx / tz, 0.0f, 0.0f}, {0.0f, cam.fy / tz, 0.0f},
+                   {-cam.fx * tx / (tz * tz), -cam.fy * ty / (tz * tz), 0.0f}};
+  float W[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) W[r][c] = view[r * 4 + c];
+  float T[3][3], Tt[3][3], TS[3][3], sig2[3][3];
+  mul(J, W, T);
+  transpose(T, Tt);
+  mul(T, sig3, TS);
+  mul(TS, Tt, sig2);
+  sig2[0][0] += 0.3f;
+  sig2[1][1] += 0.3f;
+  const float a = sig2[0][0], b = sig2[1][0], c = sig2[1][1];  // WGSL sigma2d[0][1]: column 0, row 1
+  const float det = a * c - b * b;
+  if (fabsf(det) < 1e-8f) return;
+  const float inv_det = 1.0f / det;
+  const float SH_C0 = 0.28209479177387814f;
+  auto clamp01 = [](float v) { return fminf(fmaxf(v, 0.0f), 1.0f); };
+  out[0] = sx;
+  out[1] = sy;
+  out[2] = c * inv_det;
+  out[3] = -b * inv_det;
+  out[4] = a * inv_det;
+  out[5] = clamp01(0.5f + g[G_SH + 0] * SH_C0);
+  out[6] = clamp01(0.5f + g[G_SH + 16] * SH_C0);
+  out[7] = clamp01(0.5f + g[G_SH + 32] * SH_C0);
+  out[8] = alpha;
+  out[9] = tz;
+}
+
+// GpuRenderer::project (compute.rs:138-207): n SplatInfo2D records.
+int orc_preview_project(const float* g, size_t n, const Camera* cam, float* out) {
+  float view[16];
+  if (!view_matrix<float>(*cam, view)) return 1;
+  for (size_t i = 0; i < n; ++i) wgsl_project(g + i * G_N, view, *cam, out + i * 10);
+  return 0;
+}
+
+// Rust `f as usize` of a value already floored and >= 0 (NaN -> 0), capped
+// at w: beyond it the pixel loops are empty either way.
+static size_t as_index_capped(float v, size_t w) {
+  if (!(v > 0.0f)) return 0;
+  if (v >= (float)w) return w;
+  return (size_t)v;
+}
+
+// composite (compute.rs:243-297) over depth_order (sort.rs:10-17): the
+// splats with alpha >= 1/255 and depth > 0, nearest first. sort_unstable_by
+// leaves equal depths unordered; they are taken in index order here.
+int orc_preview_render(const float* g, size_t n, const Camera* cam, const float bg[3], float* rgb) {
+  std::vector<float> sp(n * 10);
+  if (orc_preview_project(g, n, cam, sp.data())) return 1;
+  const size_t w = cam->width, h = cam->height;
+  std::vector<size_t> order;
+  for (size_t i = 0; i < n; ++i)
+    if (sp[i * 10 + 8] >= 1.0f / 255.0f && sp[i * 10 + 9] > 0.0f) order.push_back(i);
+  std::stable_sort(order.begin(), order.end(), [&](size_t p, size_t q) { return sp[p * 10 + 9] < sp[q * 10 + 9]; });
+  std::vector<float> trans(w * h, 1.0f);
+  for (size_t k = 0; k < w * h * 3; ++k) rgb[k] = 0.0f;
+  for (size_t i : order) {
+    const float* s = &sp[i * 10];
+    const float a = s[2], b = s[3], c = s[4];
+    const float mid = 0.5f * (a + c);
+    const float disc = sqrtf(0.25f * (a - c) * (a - c) + b * b);
+    const float lambda_min = fmaxf(mid - disc, 1e-6f);
+    const float rad = 3.0f * sqrtf(1.0f / lambda_min);
+    const size_t x0 = as_index_capped(fmaxf(floorf(s[0] - rad), 0.0f), w);
+    const size_t y0 = as_index_capped(fmaxf(floorf(s[1] - rad), 0.0f), h);
+    const float ex = ceilf(s[0] + rad), ey = ceilf(s[1] + rad);  // `as i64`, then clamp(0, w)
+    const size_t x1 = ex != ex ? 0 : (size_t)fminf(fmaxf(ex, 0.0f), (float)w);
+    const size_t y1 = ey != ey ? 0 : (size_t)fminf(fmaxf(ey, 0.0f), (float)h);
+    for (size_t py = y0; py < y1; ++py)
+      for (size_t px = x0; px < x1; ++px) {
+        const size_t pix = py * w + px;
+        const float t = trans[pix];
+        if (t < 1e-4f) continue;
+        const float dx = (float)px + 0.5f - s[0];
+        const float dy = (float)py + 0.5f - s[1];
+        const float power = -0.5f * (a * dx * dx + 2.0f * b * dx * dy + c * dy * dy);
+        if (power > 0.0f) continue;
+        const float alpha = fminf(s[8] * expf(power), 0.99f);
+        if (alpha < 1.0f / 255.0f) continue;
+        const float weight = alpha * t;
+        rgb[pix * 3] += s[5] * weight;
+        rgb[pix * 3 + 1] += s[6] * weight;
+        rgb[pix * 3 + 2] += s[7] * weight;
+        trans[pix] = t * (1.0f - alpha);
+      }
+  }
+  for (size_t pix = 0; pix < w * h; ++pix) {
+    const float t = trans[pix];
+    rgb[pix * 3] += t * bg[0];
+    rgb[pix * 3 + 1] += t * bg[1];
+    rgb[pix * 3 + 2] += t * bg[2];
+  }
+  return 0;
+}
+
 // ---- io/ply.rs: model files. Records are Gaussian3D (AoS, 59 floats).
 // save_ply (io/ply.rs:35-102): binary little-endian 3DGS layout.
 int orc_save_ply(const float* g, size_t n, const char* path) {

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.environ.get("RGS_BUILD_DIR", os.path.join(HERE, "_build"))
 LIB = os.environ.get("RGS_LIB_OUT", os.path.join(HERE, "librgsplat_hip.so"))
-SOURCES = ["k_project.hip", "k_sort.hip", "k_raster.hip", "k_loss.hip", "k_optim.hip", "k_io.hip", "k_sfm.hip", "rgs_api.hip"]
+SOURCES = ["k_project.hip", "k_sort.hip", "k_raster.hip", "k_loss.hip", "k_optim.hip", "k_io.hip", "k_sfm.hip", "k_preview.hip", "rgs_api.hip"]
 ARCH = os.environ.get("RGS_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

@@ -239,6 +239,10 @@ struct rgs_ctx {
   DBuf io_soa, io_rec, io_fields;  // model files: rows of host records, records, PLY property table
   DBuf srgb_lut;                   // RGBA8 -> linear table (512 floats), filled once
   DBuf sfm_xyz, sfm_rgb, sfm_k[2], sfm_v[2], sfm_khi, sfm_sx, sfm_scale, sfm_hist, sfm_rowtot, sfm_misc;
+  uint32_t* pv_pairs = nullptr;  // the tile-sorted pair values of the last preview render
+  uint64_t pv_last_pairs = 0;
+  DBuf pv_sp, pv_k[2], pv_v[2], pv_rect, pv_cnt, pv_off, pv_bs, pv_misc, pv_pk[2], pv_pv[2], pv_hist, pv_rowtot,
+      pv_rs, pv_re, pv_rgb;
   size_t n = 0, cap = 0, m_rows = 0;
   uint64_t t = 0;
   std::vector<CamDev> reg_cams;
@@ -733,7 +737,9 @@ void rgs_destroy(rgs_ctx* c) {
                  &c->dens_t32, &c->dens_t64, &c->first_splat, &c->P, &c->P2, &c->M1, &c->M2, &c->gacc, &c->gden, &c->targets,
                  &c->io_soa, &c->io_rec, &c->io_fields, &c->srgb_lut, &c->sfm_xyz, &c->sfm_rgb, &c->sfm_k[0],
                  &c->sfm_k[1], &c->sfm_v[0], &c->sfm_v[1], &c->sfm_khi, &c->sfm_sx, &c->sfm_scale, &c->sfm_hist,
-                 &c->sfm_rowtot, &c->sfm_misc,
+                 &c->sfm_rowtot, &c->sfm_misc, &c->pv_sp, &c->pv_k[0], &c->pv_k[1], &c->pv_v[0], &c->pv_v[1],
+                 &c->pv_rect, &c->pv_cnt, &c->pv_off, &c->pv_bs, &c->pv_misc, &c->pv_pk[0], &c->pv_pk[1],
+                 &c->pv_pv[0], &c->pv_pv[1], &c->pv_hist, &c->pv_rowtot, &c->pv_rs, &c->pv_re, &c->pv_rgb,
                  &c->adam_d, &c->api_acc9, &c->tile_order[0], &c->tile_order[1], &c->cover, &c->depth,
                  &c->flag, &c->tile_flag, &c->rbstart, &c->rbend, &c->pv2, &c->cntb, &c->cutst, &c->part,
                  &c->kB, &c->vB, &c->kC[0], &c->kC[1], &c->vC[0], &c->vC[1]};
@@ -1407,6 +1413,128 @@ int rgs_adam_download(rgs_ctx* c, rgs_adam* st_) {
   st_->rows = rows;
   st_->t = c->t;
   return 0;
+}
+
+// ---- preview renderer (gpu/compute.rs:138-297) ------------------------------
+}  // extern "C"
+namespace {
+// GpuRenderer::project of the n Gaussians in rows P (stride cap) into
+// c->pv_sp, then, when rgb is set, composite into rgb (host, h*w*3).
+int preview_run(rgs_ctx* c, const float* P, size_t cap, size_t n, const rgs_camera* cam, const float* bg,
+                float* splats_host, float* rgb) {
+  if (!cam->width || !cam->height) return fail(RGS_E_ARG, "camera with zero width or height");
+  float view[16];
+  if (!view_from_c2w(cam->c2w, view)) return fail(RGS_E_CAMERA, "c2w must be invertible");
+  PreviewCam pc;
+  for (int k = 0; k < 12; ++k) pc.view[k] = view[k];
+  pc.fx = cam->fx;
+  pc.fy = cam->fy;
+  pc.cx = cam->cx;
+  pc.cy = cam->cy;
+  const int w = (int)cam->width, h = (int)cam->height;
+  const int ntx = (w + TILE - 1) / TILE, nty = (h + TILE - 1) / TILE;
+  const size_t ntiles = (size_t)ntx * nty;
+  if (ntiles >= (1u << 24)) return fail(RGS_E_ARG, "image too large for the preview renderer");
+  const uint32_t n32 = (uint32_t)n;
+  const size_t nn = n ? n : 1;
+  CHK(ensure(c, c->pv_sp, nn * 40));
+  for (int b = 0; b < 2; ++b) {
+    CHK(ensure(c, c->pv_k[b], nn * 4));
+    CHK(ensure(c, c->pv_v[b], nn * 4));
+  }
+  CHK(ensure(c, c->pv_misc, 16));
+  launch_preview_project(P, (int)cap, n32, pc, P_<float>(c->pv_sp), P_<uint32_t>(c->pv_k[0]),
+                         P_<uint32_t>(c->pv_v[0]), c->st);
+  if (splats_host) {
+    if (n) HIPCHK(hipMemcpyAsync(splats_host, c->pv_sp.p, n * 40, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return 0;
+  }
+  // depth order
+  int nbm = radix_nb_max(nn);
+  CHK(ensure(c, c->pv_hist, (size_t)256 * nbm * 4));
+  CHK(ensure(c, c->pv_rowtot, 256 * 4));
+  uint32_t* n_dev = P_<uint32_t>(c->pv_misc);
+  uint32_t misc[4] = {n32, 0, 0, 0};
+  HIPCHK(hipMemcpyAsync(n_dev, misc, 16, hipMemcpyHostToDevice, c->st));
+  uint32_t* ka[2] = {P_<uint32_t>(c->pv_k[0]), P_<uint32_t>(c->pv_k[1])};
+  uint32_t* va[2] = {P_<uint32_t>(c->pv_v[0]), P_<uint32_t>(c->pv_v[1])};
+  const int a = n ? radix_sort_pairs(ka, va, n_dev, nn, 32, RadixScratch{P_<uint32_t>(c->pv_hist),
+                                                                           P_<uint32_t>(c->pv_rowtot), nbm},
+                                     c->st)
+                  : 0;
+  // rects, pair offsets, pairs
+  CHK(ensure(c, c->pv_rect, nn * 16));
+  CHK(ensure(c, c->pv_cnt, nn * 8));
+  CHK(ensure(c, c->pv_off, nn * 8));
+  CHK(ensure(c, c->pv_bs, ((size_t)scan_nb(nn) + 2) * 8));
+  int4* rects = P_<int4>(c->pv_rect);
+  launch_preview_rects(P_<float>(c->pv_sp), ka[a], va[a], n32, w, h, rects, P_<uint64_t>(c->pv_cnt), c->st);
+  uint64_t* total_d = P_<uint64_t>(c->pv_bs) + scan_nb(nn) + 1;
+  uint64_t total = 0;
+  if (n) {
+    scan_u64(P_<uint64_t>(c->pv_cnt), n, P_<uint64_t>(c->pv_off), P_<uint64_t>(c->pv_bs), total_d, c->st);
+    HIPCHK(hipMemcpyAsync(&total, total_d, 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+  }
+  if (total >= (1ull << 31)) return fail(RGS_E_CAPACITY, "more than 2^31 (tile, splat) pairs");
+  const size_t m = total ? total : 1;
+  for (int b = 0; b < 2; ++b) {
+    CHK(ensure(c, c->pv_pk[b], m * 4));
+    CHK(ensure(c, c->pv_pv[b], m * 4));
+  }
+  CHK(ensure(c, c->pv_rs, ntiles * 4));
+  CHK(ensure(c, c->pv_re, ntiles * 4));
+  launch_fill_u32(P_<uint32_t>(c->pv_rs), ntiles, 0u, c->st);
+  launch_fill_u32(P_<uint32_t>(c->pv_re), ntiles, 0u, c->st);
+  if (total) {
+    uint32_t* pk[2] = {P_<uint32_t>(c->pv_pk[0]), P_<uint32_t>(c->pv_pk[1])};
+    uint32_t* pv[2] = {P_<uint32_t>(c->pv_pv[0]), P_<uint32_t>(c->pv_pv[1])};
+    launch_preview_pairs(rects, P_<uint64_t>(c->pv_off), n32, ntx, pk[0], pv[0], c->st);
+    misc[1] = (uint32_t)total;
+    HIPCHK(hipMemcpyAsync(n_dev, misc, 16, hipMemcpyHostToDevice, c->st));
+    int bits = 8;
+    while (bits < 32 && (ntiles - 1) >> bits) bits += 8;
+    nbm = radix_nb_max(m);
+    CHK(ensure(c, c->pv_hist, (size_t)256 * nbm * 4));
+    const int q = radix_sort_pairs(pk, pv, n_dev + 1, m, bits,
+                                   RadixScratch{P_<uint32_t>(c->pv_hist), P_<uint32_t>(c->pv_rowtot), nbm}, c->st);
+    launch_preview_ranges(pk[q], (uint32_t)total, P_<uint32_t>(c->pv_rs), P_<uint32_t>(c->pv_re), c->st);
+    c->pv_pairs = pv[q];
+  }
+  CHK(ensure(c, c->pv_rgb, (size_t)w * h * 12));
+  launch_preview_composite(P_<float>(c->pv_sp), va[a], rects, total ? c->pv_pairs : P_<uint32_t>(c->pv_pv[0]),
+                           P_<uint32_t>(c->pv_rs), P_<uint32_t>(c->pv_re), w, h, ntx, nty, bg, P_<float>(c->pv_rgb),
+                           c->st);
+  HIPCHK(hipMemcpyAsync(rgb, c->pv_rgb.p, (size_t)w * h * 12, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->pv_last_pairs = total;
+  return 0;
+}
+}  // namespace
+extern "C" {
+
+int rgs_preview_project(rgs_ctx* c, const rgs_gaussian* g, size_t n, const rgs_camera* cam, float* splats_out) {
+  CHK(check_common(c, g, n, cam));
+  if (!n) return 0;
+  if (!splats_out) return fail(RGS_E_ARG, "null output");
+  CHK(upload_api_scene(c, g, n, false));
+  return preview_run(c, P_<float>(c->api_P), n, n, cam, nullptr, splats_out, nullptr);
+}
+
+int rgs_preview_render(rgs_ctx* c, const rgs_gaussian* g, size_t n, const rgs_camera* cam, const float bg[3],
+                       float* rgb_out) {
+  CHK(check_common(c, g, n, cam));
+  if (!rgb_out || !bg) return fail(RGS_E_ARG, "null output or bg");
+  CHK(upload_api_scene(c, g, n, false));
+  return preview_run(c, P_<float>(c->api_P), n ? n : 1, n, cam, bg, nullptr, rgb_out);
+}
+
+int rgs_scene_preview_render(rgs_ctx* c, const rgs_camera* cam, const float bg[3], float* rgb_out) {
+  if (!c || !cam) return fail(RGS_E_ARG, "null argument");
+  if (!rgb_out || !bg) return fail(RGS_E_ARG, "null output or bg");
+  (void)hipSetDevice(c->device);
+  return preview_run(c, P_<float>(c->P), c->cap, c->n, cam, bg, nullptr, rgb_out);
 }
 
 // ---- SfM initialisation (pipeline/sfm.rs:365-455) ---------------------------

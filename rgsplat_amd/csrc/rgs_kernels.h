@@ -273,6 +273,21 @@ int ply_decode_tiled_fits(int rec_bytes);
 void launch_ply_decode_tiled(const uint8_t* body, size_t n, int rec_bytes, const PlyField* row_field, int big_endian,
                              float* P, size_t cap, hipStream_t st);
 void launch_splat_encode(const float* P, size_t cap, size_t n, void* out /* n x 32 bytes */, hipStream_t st);
+// Preview renderer (k_preview.hip): GpuRenderer's project and composite.
+struct PreviewCam {
+  float view[12];  // rows 0-2 of the world-to-camera matrix, row-major
+  float fx, fy, cx, cy;
+};
+void launch_preview_project(const float* P, int cap, uint32_t n, const PreviewCam& cam, float* splats,
+                            uint32_t* keys, uint32_t* vals, hipStream_t st);
+void launch_preview_rects(const float* splats, const uint32_t* keys, const uint32_t* order, uint32_t n, int w, int h,
+                          int4* rects, uint64_t* cnt, hipStream_t st);
+void launch_preview_pairs(const int4* rects, const uint64_t* off, uint32_t n, int ntx, uint32_t* pk, uint32_t* pv,
+                          hipStream_t st);
+void launch_preview_ranges(const uint32_t* pk, uint32_t m, uint32_t* rs, uint32_t* re, hipStream_t st);
+void launch_preview_composite(const float* splats, const uint32_t* order, const int4* rects, const uint32_t* pv,
+                              const uint32_t* rs, const uint32_t* re, int w, int h, int ntx, int nty, const float bg[3],
+                              float* rgb, hipStream_t st);
 // SfM initialisation (k_sfm.hip). status bit 1: a NaN coordinate or distance.
 void launch_sfm_keys(const double* xyz, uint32_t n, uint32_t* klo, uint32_t* khi, uint32_t* idx, uint32_t* status,
                      hipStream_t st);

@@ -80,6 +80,9 @@ SIGNATURES = {
     "rgs_scene_load_ply": [_vp, ctypes.c_char_p, _sz],
     "rgs_save_splat": [_vp, _vp, _sz, ctypes.c_char_p],
     "rgs_scene_save_splat": [_vp, ctypes.c_char_p],
+    "rgs_preview_project": [_vp, _vp, _sz, _CAM, _vp],
+    "rgs_preview_render": [_vp, _vp, _sz, _CAM, _vp, _vp],
+    "rgs_scene_preview_render": [_vp, _CAM, _vp, _vp],
     "rgs_gaussians_from_sfm": [_vp, _vp, _vp, _sz, _vp],
     "rgs_scene_from_sfm": [_vp, _vp, _vp, _sz, _sz],
     "rgs_adam_download": [_vp, _ADAM],
@@ -231,6 +234,29 @@ class Context:
         out = np.zeros((cam.height, cam.width, 4), np.float32)
         c = camera(cam)
         _check(lib().rgs_rasterize_rgba(self.h, _p(g), len(g), ctypes.byref(c), _p(_f32(bg)), _p(out)))
+        return out
+
+    def preview_project(self, gaussians, cam):
+        """GpuRenderer::project (gpu/compute.rs:138-207) -> (n, 10) SplatInfo2D."""
+        g = _f32(gaussians).reshape(-1, 59)
+        out = np.zeros((max(len(g), 1), 10), np.float32)
+        c = camera(cam)
+        _check(lib().rgs_preview_project(self.h, _p(g), len(g), ctypes.byref(c), _p(out)))
+        return out[: len(g)]
+
+    def preview_render(self, gaussians, cam, bg=(0.0, 0.0, 0.0)):
+        """GpuRenderer::render (gpu/compute.rs:210-218) -> (h, w, 3) linear RGB."""
+        g = _f32(gaussians).reshape(-1, 59)
+        out = np.zeros((cam.height, cam.width, 3), np.float32)
+        c = camera(cam)
+        _check(lib().rgs_preview_render(self.h, _p(g), len(g), ctypes.byref(c), _p(_f32(bg)), _p(out)))
+        return out
+
+    def scene_preview_render(self, cam, bg=(0.0, 0.0, 0.0)):
+        """The preview render of the device scene."""
+        out = np.zeros((cam.height, cam.width, 3), np.float32)
+        c = camera(cam)
+        _check(lib().rgs_scene_preview_render(self.h, ctypes.byref(c), _p(_f32(bg)), _p(out)))
         return out
 
     def photometric_loss_grad(self, rendered, target_rgba, w, h, lambda_dssim=0.2):
