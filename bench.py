@@ -226,6 +226,34 @@ def preview(runtime, wl, reps, cpu=True):
           flush=True)
 
 
+def target_ingestion(runtime, wl, reps, cpu=True):
+    """Wall time of registering the workload's views from decoded RGBA8
+    images 1.5x the cameras' sizes: upload, resize (image 0.25's Triangle
+    filter) and sRGB-to-linear conversion on the device; the oracle's resize
+    and conversion of the first view beside it."""
+    cams = wl.cameras()
+    rng = np.random.default_rng(12)
+    ims = [rng.integers(0, 256, (c.height * 3 // 2, c.width * 3 // 2, 4), dtype=np.uint8) for c in cams]
+    ctx = runtime.Context(0)
+    ctx.set_views_rgba8_sized(cams, ims)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.set_views_rgba8_sized(cams, ims)
+    ms = (time.perf_counter() - t0) * 1000.0 / reps
+    print(f"[bench] set_views_rgba8_sized {ms:.1f} ms for {len(cams)} views", file=sys.stderr, flush=True)
+    res = {"set_views_rgba8_sized_ms": ms}
+    if cpu:
+        import oracle
+        t0 = time.perf_counter()
+        oracle.rgba8_to_linear(oracle.resize_rgba8(ims[0], cams[0].width, cams[0].height))
+        res["cpu_oracle_ms_first_view"] = (time.perf_counter() - t0) * 1000.0
+        res["cpu_threads"] = 1
+    ctx.close()
+    print(json.dumps({"metric": "target ingestion (wall ms per registration of all views, RGBA8 host images in)",
+                      "views": [[c.width, c.height] for c in cams],
+                      "source": [[im.shape[1], im.shape[0]] for im in ims], "reps": reps, **res}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -243,6 +271,8 @@ def main():
                     help="time the SfM initialisation instead (gaussians_from_sfm on a cloud of the workload's size)")
     ap.add_argument("--preview", action="store_true",
                     help="time the preview renderer instead (GpuRenderer::render of the workload's scene)")
+    ap.add_argument("--targets", action="store_true",
+                    help="time the target ingestion instead (RGBA8 images resized and converted on the device)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the step shards by views "
@@ -261,6 +291,9 @@ def main():
         return
     if args.preview:
         preview(runtime, wl, args.steps, cpu=not args.no_cpu)
+        return
+    if args.targets:
+        target_ingestion(runtime, wl, args.steps, cpu=not args.no_cpu)
         return
     print(f"[bench] workload {wl.name}: {wl.n} Gaussians, views {wl.sizes}", file=sys.stderr, flush=True)
     g = wl.scene()

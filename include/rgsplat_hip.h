@@ -150,6 +150,17 @@ int rgs_set_views(rgs_ctx* ctx, const rgs_camera* cams, const float* const* targ
  * bytes each), converted to linear RGBA on the device as load_target does
  * after decoding and resizing (io/images.rs:63-69); 4x less to upload. */
 int rgs_set_views_rgba8(rgs_ctx* ctx, const rgs_camera* cams, const uint8_t* const* images_rgba8, int n_views);
+/* As rgs_set_views_rgba8 from decoded RGBA8 images of any size (sizes: w, h
+ * per view): an image whose size differs from its camera's is resized on the
+ * device as load_target does (io/images.rs:57-61, image 0.25's
+ * resize_exact(.., FilterType::Triangle)) before the conversion. */
+int rgs_set_views_rgba8_sized(rgs_ctx* ctx, const rgs_camera* cams, const uint8_t* const* images_rgba8,
+                              const uint32_t* sizes, int n_views);
+/* image 0.25's resize_exact(.., FilterType::Triangle) of an RGBA8 image
+ * (imageops/sample.rs: a vertical pass to f32, then a horizontal pass that
+ * rounds to bytes), on the device. */
+int rgs_resize_rgba8(rgs_ctx* ctx, const uint8_t* src, uint32_t src_w, uint32_t src_h, uint8_t* dst, uint32_t dst_w,
+                     uint32_t dst_h);
 /* load_target's conversion (io/images.rs:63-69, srgb_to_linear :76-82) of
  * n_pixels RGBA8 pixels to linear RGBA f32, bit-identical (the sRGB table is
  * computed with the libm powf the reference calls). */

@@ -114,6 +114,8 @@ def _declare(L):
     for f in ("orc_preview_project",):
         getattr(L, f).restype = i32
         getattr(L, f).argtypes = [vp, sz, ctypes.POINTER(Camera), vp]
+    L.orc_resize_rgba8.restype = i32
+    L.orc_resize_rgba8.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, vp, ctypes.c_uint32, ctypes.c_uint32]
     L.orc_preview_render.restype = i32
     L.orc_preview_render.argtypes = [vp, sz, ctypes.POINTER(Camera), vp, vp]
     L.orc_rgba8_to_linear.argtypes = [vp, sz, vp]
@@ -414,4 +416,14 @@ def preview_render(g, cam, bg=(0.0, 0.0, 0.0)):
     c = camera(cam)
     if lib().orc_preview_render(_p(g), len(g), ctypes.byref(c), _p(bgv), _p(out)):
         raise ValueError("singular camera")
+    return out
+
+
+def resize_rgba8(img, width, height):
+    """load_target's resize_exact(width, height, Triangle) (io/images.rs:57-61,
+    image 0.25.10) of an (h, w, 4) RGBA8 image."""
+    img = np.ascontiguousarray(img, np.uint8)
+    out = np.zeros((height, width, 4), np.uint8)
+    if lib().orc_resize_rgba8(_p(img), img.shape[1], img.shape[0], _p(out), width, height):
+        raise ValueError("zero image size")
     return out

@@ -1445,6 +1445,72 @@ int orc_gaussians_from_sfm(const double* xyz, const uint8_t* rgb, size_t n, floa
   return 0;
 }
 
+// ---- io/images.rs:57-61: img.resize_exact(w, h, FilterType::Triangle) of
+// the image crate 0.25.10 (Cargo.lock), which is not vendored in the
+// reference. Restated from the crate's published imageops/sample.rs:
+// `resize` runs vertical_sample (rows to an f32 image) and then
+// horizontal_sample (columns, then clamp to [0, 255] and round half away
+// from zero). Each pass maps output index o to the centre (o + 0.5) * ratio
+// (ratio = src / dst), takes the source indices [left, right) with left =
+// clamp(floor(centre - support), 0, src - 1), right = clamp(ceil(centre +
+// support), left + 1, src), support = sratio = max(ratio, 1), weights
+// triangle((i - (centre - 0.5)) / sratio) normalised by their f32 sum, and
+// sums value * weight over the window in order, in f32.
+static void resize_pass_weights(uint32_t src, uint32_t dst, uint32_t o, int64_t* left_out,
+                                std::vector<float>* w) {
+  const float ratio = (float)src / (float)dst;
+  const float sratio = ratio < 1.0f ? 1.0f : ratio;
+  const float support = 1.0f * sratio;
+  const float centre = ((float)o + 0.5f) * ratio;
+  int64_t left = (int64_t)floorf(centre - support);
+  left = std::min<int64_t>(std::max<int64_t>(left, 0), (int64_t)src - 1);
+  int64_t right = (int64_t)ceilf(centre + support);
+  right = std::min<int64_t>(std::max<int64_t>(right, left + 1), (int64_t)src);
+  const float c = centre - 0.5f;
+  w->clear();
+  float sum = 0.0f;
+  for (int64_t i = left; i < right; ++i) {
+    const float x = fabsf(((float)i - c) / sratio);
+    const float v = x < 1.0f ? 1.0f - x : 0.0f;
+    w->push_back(v);
+    sum += v;
+  }
+  for (float& v : *w) v /= sum;
+  *left_out = left;
+}
+
+int orc_resize_rgba8(const uint8_t* src, uint32_t sw, uint32_t sh, uint8_t* dst, uint32_t dw, uint32_t dh) {
+  if (!sw || !sh || !dw || !dh) return 1;
+  if (sw == dw && sh == dh) {
+    memcpy(dst, src, (size_t)sw * sh * 4);
+    return 0;
+  }
+  std::vector<float> tmp((size_t)sw * dh * 4);
+  std::vector<float> w;
+  for (uint32_t y = 0; y < dh; ++y) {  // vertical_sample
+    int64_t left;
+    resize_pass_weights(sh, dh, y, &left, &w);
+    for (uint32_t x = 0; x < sw; ++x)
+      for (int ch = 0; ch < 4; ++ch) {
+        float t = 0.0f;
+        for (size_t i = 0; i < w.size(); ++i) t += (float)src[((size_t)(left + i) * sw + x) * 4 + ch] * w[i];
+        tmp[((size_t)y * sw + x) * 4 + ch] = t;
+      }
+  }
+  for (uint32_t x = 0; x < dw; ++x) {  // horizontal_sample
+    int64_t left;
+    resize_pass_weights(sw, dw, x, &left, &w);
+    for (uint32_t y = 0; y < dh; ++y)
+      for (int ch = 0; ch < 4; ++ch) {
+        float t = 0.0f;
+        for (size_t i = 0; i < w.size(); ++i) t += tmp[((size_t)y * sw + left + i) * 4 + ch] * w[i];
+        const float cl = t < 0.0f ? 0.0f : (t > 255.0f ? 255.0f : t);
+        dst[((size_t)y * dw + x) * 4 + ch] = (uint8_t)roundf(cl);
+      }
+  }
+  return 0;
+}
+
 // ---- gpu/compute.rs + gpu/splat.wgsl: the preview renderer (GpuRenderer).
 // SplatInfo2D (compute.rs:65-78): sx, sy, conic_a, conic_b, conic_c, color
 // r, g, b, alpha, depth.

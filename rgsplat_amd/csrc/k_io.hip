@@ -282,3 +282,68 @@ void launch_to_u8_srgb(const float* rgba, size_t n_pixels, uint8_t* out, hipStre
 }
 
 }  // namespace rgs
+
+namespace rgs {
+
+// image 0.25's resize (imageops/sample.rs: vertical_sample, then
+// horizontal_sample) of RGBA8 with the Triangle filter. Each output row /
+// column has a window of taps [left, left + cnt) with normalised f32
+// weights, computed on the host (see resize_taps); the sums run over the
+// taps in order from 0, in f32.
+//
+// Vertical pass: RGBA8 (sw x sh) -> f32 RGBA (sw x dh).
+__global__ __launch_bounds__(256) void k_resize_v(const uchar4* __restrict__ src, uint32_t sw, uint32_t dh,
+                                                  const int2* __restrict__ taps, const float* __restrict__ w, int K,
+                                                  float4* __restrict__ tmp) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= sw || y >= dh) return;
+  const int2 tp = taps[y];
+  float4 t = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  for (int i = 0; i < tp.y; ++i) {
+    const uchar4 p = src[(size_t)(tp.x + i) * sw + x];
+    const float wi = w[(size_t)y * K + i];
+    t.x += (float)p.x * wi;
+    t.y += (float)p.y * wi;
+    t.z += (float)p.z * wi;
+    t.w += (float)p.w * wi;
+  }
+  tmp[(size_t)y * sw + x] = t;
+}
+
+// FloatNearest(clamp(t, 0, 255)) as u8: round half away from zero.
+__device__ __forceinline__ unsigned char nearest_u8(float t) {
+  const float c = t < 0.0f ? 0.0f : (t > 255.0f ? 255.0f : t);
+  return (unsigned char)__builtin_roundf(c);
+}
+
+// Horizontal pass: f32 RGBA (sw x dh) -> RGBA8 (dw x dh).
+__global__ __launch_bounds__(256) void k_resize_h(const float4* __restrict__ tmp, uint32_t sw, uint32_t dw,
+                                                  uint32_t dh, const int2* __restrict__ taps,
+                                                  const float* __restrict__ w, int K, uchar4* __restrict__ dst) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= dw || y >= dh) return;
+  const int2 tp = taps[x];
+  float4 t = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  for (int i = 0; i < tp.y; ++i) {
+    const float4 p = tmp[(size_t)y * sw + tp.x + i];
+    const float wi = w[(size_t)x * K + i];
+    t.x += p.x * wi;
+    t.y += p.y * wi;
+    t.z += p.z * wi;
+    t.w += p.w * wi;
+  }
+  dst[(size_t)y * dw + x] = make_uchar4(nearest_u8(t.x), nearest_u8(t.y), nearest_u8(t.z), nearest_u8(t.w));
+}
+
+void launch_resize_rgba8(const uint8_t* src, uint32_t sw, uint32_t sh, uint8_t* dst, uint32_t dw, uint32_t dh,
+                         const int2* vtaps, const float* vw, int vK, const int2* htaps, const float* hw, int hK,
+                         float* tmp, hipStream_t st) {
+  (void)sh;
+  if (!sw || !dw || !dh) return;
+  hipLaunchKernelGGL(k_resize_v, dim3((sw + 255) / 256, dh), dim3(256), 0, st, reinterpret_cast<const uchar4*>(src),
+                     sw, dh, vtaps, vw, vK, reinterpret_cast<float4*>(tmp));
+  hipLaunchKernelGGL(k_resize_h, dim3((dw + 255) / 256, dh), dim3(256), 0, st, reinterpret_cast<const float4*>(tmp),
+                     sw, dw, dh, htaps, hw, hK, reinterpret_cast<uchar4*>(dst));
+}
+
+}  // namespace rgs

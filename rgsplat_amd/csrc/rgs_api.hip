@@ -239,6 +239,7 @@ struct rgs_ctx {
   DBuf io_soa, io_rec, io_fields;  // model files: rows of host records, records, PLY property table
   DBuf srgb_lut;                   // RGBA8 -> linear table (512 floats), filled once
   DBuf sfm_xyz, sfm_rgb, sfm_k[2], sfm_v[2], sfm_khi, sfm_sx, sfm_scale, sfm_hist, sfm_rowtot, sfm_misc;
+  DBuf rs_taps, rs_tmp, rs_src, rs_dst;  // resize: taps and weights, f32 rows, source and result
   uint32_t* pv_pairs = nullptr;  // the tile-sorted pair values of the last preview render
   uint64_t pv_last_pairs = 0;
   DBuf pv_sp, pv_k[2], pv_v[2], pv_rect, pv_cnt, pv_off, pv_bs, pv_misc, pv_pk[2], pv_pv[2], pv_hist, pv_rowtot,
@@ -740,6 +741,7 @@ void rgs_destroy(rgs_ctx* c) {
                  &c->sfm_rowtot, &c->sfm_misc, &c->pv_sp, &c->pv_k[0], &c->pv_k[1], &c->pv_v[0], &c->pv_v[1],
                  &c->pv_rect, &c->pv_cnt, &c->pv_off, &c->pv_bs, &c->pv_misc, &c->pv_pk[0], &c->pv_pk[1],
                  &c->pv_pv[0], &c->pv_pv[1], &c->pv_hist, &c->pv_rowtot, &c->pv_rs, &c->pv_re, &c->pv_rgb,
+                 &c->rs_taps, &c->rs_tmp, &c->rs_src, &c->rs_dst,
                  &c->adam_d, &c->api_acc9, &c->tile_order[0], &c->tile_order[1], &c->cover, &c->depth,
                  &c->flag, &c->tile_flag, &c->rbstart, &c->rbend, &c->pv2, &c->cntb, &c->cutst, &c->part,
                  &c->kB, &c->vB, &c->kC[0], &c->kC[1], &c->vC[0], &c->vC[1]};
@@ -1639,8 +1641,133 @@ int srgb_table(rgs_ctx* c) {
   HIPCHK(hipMemcpy(c->srgb_lut.p, t, sizeof(t), hipMemcpyHostToDevice));
   return 0;
 }
+
+// The taps of one pass of image 0.25's resize (imageops/sample.rs,
+// vertical_sample / horizontal_sample) with the Triangle filter (support 1):
+// for output index o, the source window [left, right) around the centre
+// (o + 0.5) * ratio and the normalised weights triangle((i - centre + 0.5) /
+// sratio), all in f32 as the crate computes them.
+struct Taps {
+  std::vector<int32_t> lc;  // (left, count) per output index
+  std::vector<float> w;     // K per output index
+  int K = 1;
+};
+Taps resize_taps(uint32_t src, uint32_t dst) {
+  Taps t;
+  const float ratio = (float)src / (float)dst;
+  const float sratio = ratio < 1.0f ? 1.0f : ratio;
+  const float support = 1.0f * sratio;
+  auto clampi = [](int64_t a, int64_t lo, int64_t hi) { return a < lo ? lo : (a > hi ? hi : a); };
+  auto as_i64 = [](float v) -> int64_t {  // Rust `as i64` of a floored / ceiled value
+    if (v != v) return 0;
+    if (v >= 9.2e18f) return INT64_MAX;
+    if (v <= -9.2e18f) return INT64_MIN;
+    return (int64_t)v;
+  };
+  std::vector<std::vector<float>> ws(dst);
+  t.lc.resize(2 * (size_t)dst);
+  for (uint32_t o = 0; o < dst; ++o) {
+    float centre = ((float)o + 0.5f) * ratio;
+    const int64_t left = clampi(as_i64(floorf(centre - support)), 0, (int64_t)src - 1);
+    const int64_t right = clampi(as_i64(ceilf(centre + support)), left + 1, (int64_t)src);
+    centre = centre - 0.5f;
+    float sum = 0.0f;
+    for (int64_t i = left; i < right; ++i) {
+      const float x = ((float)i - centre) / sratio;
+      const float ax = fabsf(x);
+      const float wv = ax < 1.0f ? 1.0f - ax : 0.0f;
+      ws[o].push_back(wv);
+      sum += wv;
+    }
+    for (float& wv : ws[o]) wv /= sum;
+    t.lc[2 * o] = (int32_t)left;
+    t.lc[2 * o + 1] = (int32_t)(right - left);
+    t.K = std::max(t.K, (int)(right - left));
+  }
+  t.w.assign((size_t)dst * t.K, 0.0f);
+  for (uint32_t o = 0; o < dst; ++o)
+    for (size_t i = 0; i < ws[o].size(); ++i) t.w[(size_t)o * t.K + i] = ws[o][i];
+  return t;
+}
+
+// Resizes the RGBA8 image at src (device, sw x sh) into dst (device, dw x
+// dh) with image 0.25's resize_exact(.., Triangle).
+int resize_device(rgs_ctx* c, const uint8_t* src, uint32_t sw, uint32_t sh, uint8_t* dst, uint32_t dw,
+                  uint32_t dh) {
+  const Taps v = resize_taps(sh, dh), h = resize_taps(sw, dw);
+  const size_t vb = v.lc.size() * 4, vwb = v.w.size() * 4, hb = h.lc.size() * 4, hwb = h.w.size() * 4;
+  CHK(ensure(c, c->rs_taps, vb + vwb + hb + hwb));
+  CHK(ensure(c, c->rs_tmp, (size_t)sw * dh * 16));
+  std::vector<uint8_t> blob(vb + vwb + hb + hwb);
+  std::memcpy(blob.data(), v.lc.data(), vb);
+  std::memcpy(blob.data() + vb, v.w.data(), vwb);
+  std::memcpy(blob.data() + vb + vwb, h.lc.data(), hb);
+  std::memcpy(blob.data() + vb + vwb + hb, h.w.data(), hwb);
+  HIPCHK(hipMemcpyAsync(c->rs_taps.p, blob.data(), blob.size(), hipMemcpyHostToDevice, c->st));
+  uint8_t* base = P_<uint8_t>(c->rs_taps);
+  launch_resize_rgba8(src, sw, sh, dst, dw, dh, reinterpret_cast<const int2*>(base),
+                      reinterpret_cast<const float*>(base + vb), v.K, reinterpret_cast<const int2*>(base + vb + vwb),
+                      reinterpret_cast<const float*>(base + vb + vwb + hb), h.K, P_<float>(c->rs_tmp), c->st);
+  HIPCHK(hipStreamSynchronize(c->st));  // the tap blob is pageable host memory
+  return 0;
+}
 }  // namespace
 extern "C" {
+
+int rgs_resize_rgba8(rgs_ctx* c, const uint8_t* src, uint32_t sw, uint32_t sh, uint8_t* dst, uint32_t dw,
+                     uint32_t dh) {
+  if (!c || !src || !dst) return fail(RGS_E_ARG, "null argument");
+  if (!sw || !sh || !dw || !dh) return fail(RGS_E_ARG, "zero image size");
+  (void)hipSetDevice(c->device);
+  const size_t sb = (size_t)sw * sh * 4, db = (size_t)dw * dh * 4;
+  CHK(ensure(c, c->rs_src, sb));
+  CHK(ensure(c, c->rs_dst, db));
+  HIPCHK(hipMemcpyAsync(c->rs_src.p, src, sb, hipMemcpyHostToDevice, c->st));
+  if (sw == dw && sh == dh) {  // resize copies when the size is unchanged
+    HIPCHK(hipMemcpyAsync(dst, c->rs_src.p, sb, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return 0;
+  }
+  CHK(resize_device(c, P_<uint8_t>(c->rs_src), sw, sh, P_<uint8_t>(c->rs_dst), dw, dh));
+  HIPCHK(hipMemcpy(dst, c->rs_dst.p, db, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int rgs_set_views_rgba8_sized(rgs_ctx* c, const rgs_camera* cams, const uint8_t* const* images,
+                              const uint32_t* sizes, int nv) {
+  if (!c || nv <= 0 || !cams || !images || !sizes) return fail(RGS_E_ARG, "null argument or no views");
+  (void)hipSetDevice(c->device);
+  drop_graph(c);
+  std::vector<CamDev> v(nv);
+  size_t tot = 0;
+  for (int i = 0; i < nv; ++i) {
+    if (!images[i]) return fail(RGS_E_ARG, "null image");
+    if (!sizes[2 * i] || !sizes[2 * i + 1]) return fail(RGS_E_ARG, "zero image size");
+    CHK(make_camdev(cams[i], v[i]));
+    v[i].tgt_base = (uint32_t)tot;
+    tot += (size_t)cams[i].width * cams[i].height;
+  }
+  if (tot >= (1ull << 32)) return fail(RGS_E_ARG, "targets too large");
+  CHK(srgb_table(c));
+  CHK(ensure(c, c->targets, tot * 16));
+  CHK(ensure(c, c->io_soa, tot * 4));
+  for (int i = 0; i < nv; ++i) {
+    const uint32_t sw = sizes[2 * i], sh = sizes[2 * i + 1];
+    uint8_t* dst = P_<uint8_t>(c->io_soa) + (size_t)v[i].tgt_base * 4;
+    if (sw == cams[i].width && sh == cams[i].height) {
+      HIPCHK(hipMemcpyAsync(dst, images[i], (size_t)sw * sh * 4, hipMemcpyHostToDevice, c->st));
+    } else {
+      CHK(ensure(c, c->rs_src, (size_t)sw * sh * 4));
+      HIPCHK(hipMemcpyAsync(c->rs_src.p, images[i], (size_t)sw * sh * 4, hipMemcpyHostToDevice, c->st));
+      CHK(resize_device(c, P_<uint8_t>(c->rs_src), sw, sh, dst, cams[i].width, cams[i].height));
+    }
+  }
+  launch_rgba8_to_linear(P_<uint8_t>(c->io_soa), tot, P_<float>(c->srgb_lut), P_<float>(c->targets), c->st);
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->reg_cams = v;
+  c->target_pix = tot;
+  return 0;
+}
 
 int rgs_rgba8_to_linear(rgs_ctx* c, const uint8_t* rgba8, size_t n_pixels, float* out) {
   if (!c || (n_pixels && (!rgba8 || !out))) return fail(RGS_E_ARG, "null argument");

@@ -273,6 +273,11 @@ int ply_decode_tiled_fits(int rec_bytes);
 void launch_ply_decode_tiled(const uint8_t* body, size_t n, int rec_bytes, const PlyField* row_field, int big_endian,
                              float* P, size_t cap, hipStream_t st);
 void launch_splat_encode(const float* P, size_t cap, size_t n, void* out /* n x 32 bytes */, hipStream_t st);
+// image 0.25's resize_exact with the Triangle filter, RGBA8 (k_io.hip):
+// taps are (left, count) per output row (v) / column (h), weights K per row.
+void launch_resize_rgba8(const uint8_t* src, uint32_t sw, uint32_t sh, uint8_t* dst, uint32_t dw, uint32_t dh,
+                         const int2* vtaps, const float* vw, int vK, const int2* htaps, const float* hw, int hK,
+                         float* tmp, hipStream_t st);
 // Preview renderer (k_preview.hip): GpuRenderer's project and composite.
 struct PreviewCam {
   float view[12];  // rows 0-2 of the world-to-camera matrix, row-major

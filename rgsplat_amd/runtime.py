@@ -80,6 +80,8 @@ SIGNATURES = {
     "rgs_scene_load_ply": [_vp, ctypes.c_char_p, _sz],
     "rgs_save_splat": [_vp, _vp, _sz, ctypes.c_char_p],
     "rgs_scene_save_splat": [_vp, ctypes.c_char_p],
+    "rgs_resize_rgba8": [_vp, _vp, ctypes.c_uint32, ctypes.c_uint32, _vp, ctypes.c_uint32, ctypes.c_uint32],
+    "rgs_set_views_rgba8_sized": [_vp, _CAM, ctypes.POINTER(ctypes.c_void_p), _vp, _i],
     "rgs_preview_project": [_vp, _vp, _sz, _CAM, _vp],
     "rgs_preview_render": [_vp, _vp, _sz, _CAM, _vp, _vp],
     "rgs_scene_preview_render": [_vp, _CAM, _vp, _vp],
@@ -339,6 +341,26 @@ class Context:
         ptrs = (ctypes.c_void_p * len(ims))(*[im.ctypes.data for im in ims])
         _check(lib().rgs_set_views_rgba8(self.h, cs, ptrs, len(ims)))
         self._views_keepalive = ims
+
+    def set_views_rgba8_sized(self, cams, images):
+        """set_views from decoded RGBA8 images (h, w, 4) of any size, resized
+        on the device to the cameras' sizes as load_target does."""
+        cs = (Camera * len(cams))(*[camera(c) for c in cams])
+        ims = [np.ascontiguousarray(im, np.uint8) for im in images]
+        for im in ims:
+            if im.ndim != 3 or im.shape[2] != 4:
+                raise ValueError(f"image {im.shape} is not (h, w, 4)")
+        sizes = np.array([[im.shape[1], im.shape[0]] for im in ims], np.uint32)
+        ptrs = (ctypes.c_void_p * len(ims))(*[im.ctypes.data for im in ims])
+        _check(lib().rgs_set_views_rgba8_sized(self.h, cs, ptrs, _p(sizes), len(ims)))
+        self._views_keepalive = ims
+
+    def resize_rgba8(self, img, width, height):
+        """image 0.25's resize_exact(width, height, Triangle) of (h, w, 4) RGBA8."""
+        img = np.ascontiguousarray(img, np.uint8)
+        out = np.zeros((height, width, 4), np.uint8)
+        _check(lib().rgs_resize_rgba8(self.h, _p(img), img.shape[1], img.shape[0], _p(out), width, height))
+        return out
 
     # ------------------------------------------------------------ model files
     def save_ply(self, gaussians, path):
