@@ -134,15 +134,29 @@ __device__ __forceinline__ int index_capped(float v, int w) {
   return (int)v;
 }
 
+// Pixel range [lo, hi) of the centres px + 0.5 within [s - hw, s + hw],
+// cut to [r0, r1) (hw finite and >= 0; huge values saturate).
+__device__ __forceinline__ void reach_range(float s, float hw, int r0, int r1, int& lo, int& hi) {
+  const float flo = ceilf(s - hw - 0.5f), fhi = floorf(s + hw - 0.5f) + 1.0f;
+  lo = !(flo > (float)r0) ? r0 : (flo >= (float)r1 ? r1 : (int)flo);
+  hi = !(fhi < (float)r1) ? r1 : (fhi <= (float)r0 ? r0 : (int)fhi);
+}
+
 // composite's rect (compute.rs:249-260) for the splat at each sorted
-// position, and the number of tiles it meets.
+// position, and the tiles to list it in: those of the rect that the splat's
+// alpha-reach ellipse meets. Outside the ellipse q <= R (q the conic form,
+// R = 2 ln(255 alpha) with a margin for the f32 evaluation of q, which
+// grows with the conic's condition number; no cut when that is too large)
+// alpha stays below 1/255, so composite skips those pixels anyway: the cut
+// drops no pixel's splat. The composite still tests the full rect per pixel.
 __global__ __launch_bounds__(256) void k_preview_rects(const float* __restrict__ splats,
                                                        const uint32_t* __restrict__ keys,
                                                        const uint32_t* __restrict__ order, uint32_t n, int w, int h,
-                                                       int4* __restrict__ rects, uint64_t* __restrict__ cnt) {
+                                                       int4* __restrict__ rects, int4* __restrict__ trects,
+                                                       uint64_t* __restrict__ cnt) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  int4 r = make_int4(0, 0, 0, 0);
+  int4 r = make_int4(0, 0, 0, 0), tr = make_int4(0, 0, 0, 0);
   uint64_t tiles = 0;
   if (keys[p] != 0xffffffffu) {
     const float* s = splats + (size_t)order[p] * 10;
@@ -156,42 +170,98 @@ __global__ __launch_bounds__(256) void k_preview_rects(const float* __restrict__
     const float ex = ceilf(s[0] + rad), ey = ceilf(s[1] + rad);  // `as i64`, then clamp(0, w)
     r.z = ex != ex ? 0 : (int)fmin_(fmax_(ex, 0.0f), (float)w);
     r.w = ey != ey ? 0 : (int)fmin_(fmax_(ey, 0.0f), (float)h);
-    if (r.z > r.x && r.w > r.y)
-      tiles = (uint64_t)((r.z - 1) / TILE - r.x / TILE + 1) * (uint64_t)((r.w - 1) / TILE - r.y / TILE + 1);
+    // the alpha-reach box, in pixels, within the rect
+    int x0 = r.x, x1 = r.z, y0 = r.y, y1 = r.w;
+    const float l1 = mid + disc, l2 = mid - disc;  // conic eigenvalues
+    const float q_err = 1e-6f * (fmax_(l1, 0.0f) / fmax_(l2, 1e-30f));
+    const float det = a * c - b * b;
+    if (q_err <= 0.02f && l2 > 0.0f && det > 0.0f) {
+      const float R = fmax_(2.0f * logf(255.0f * s[8]) * (1.02f + 20.0f * q_err) + 1e-3f, 1e-3f);
+      const float hx = __builtin_sqrtf(R * (c / det)) + 0.5f, hy = __builtin_sqrtf(R * (a / det)) + 0.5f;
+      if (__builtin_isfinite(hx) && __builtin_isfinite(hy) && __builtin_isfinite(s[0]) && __builtin_isfinite(s[1])) {
+        reach_range(s[0], hx, r.x, r.z, x0, x1);
+        reach_range(s[1], hy, r.y, r.w, y0, y1);
+      }
+    }
+    if (x1 > x0 && y1 > y0) {
+      tr = make_int4(x0 / TILE, y0 / TILE, (x1 - 1) / TILE + 1, (y1 - 1) / TILE + 1);
+      tiles = (uint64_t)(tr.z - tr.x) * (uint64_t)(tr.w - tr.y);
+    }
   }
   rects[p] = r;
+  trects[p] = tr;
   cnt[p] = tiles;
 }
 
 void launch_preview_rects(const float* splats, const uint32_t* keys, const uint32_t* order, uint32_t n, int w, int h,
-                          int4* rects, uint64_t* cnt, hipStream_t st) {
+                          int4* rects, int4* trects, uint64_t* cnt, hipStream_t st) {
   if (!n) return;
   hipLaunchKernelGGL(k_preview_rects, dim3((n + 255) / 256), dim3(256), 0, st, splats, keys, order, n, w, h, rects,
-                     cnt);
+                     trects, cnt);
 }
 
-// (tile, sorted position) pairs; splat p's pairs start at off[p], tiles in
-// row-major order.
-__global__ __launch_bounds__(256) void k_preview_pairs(const int4* __restrict__ rects,
-                                                       const uint64_t* __restrict__ off, uint32_t n, int ntx,
+// (tile, sorted position) pairs, one thread per pair: pair q belongs to the
+// last splat p with off[p] <= q (off is the exclusive scan of the tile
+// counts, so splats without tiles are passed over), and is that splat's
+// (q - off[p])-th tile of [tx0, tx1) x [ty0, ty1) in row-major order. Two
+// searches over off bound the splats of the workgroup's 256 pairs; their
+// offsets are staged in LDS and each thread searches there (in global
+// memory when more than 256 splats share the 256 pairs).
+__device__ __forceinline__ uint32_t last_at_or_below(const uint64_t* off, uint32_t lo, uint32_t hi, uint64_t q) {
+  // off[lo] <= q < off[hi] (off[hi] may be the virtual end)
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (off[mid] <= q) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// blk[b] = the splat of pair min(256 b, m - 1), for b = 0 .. nb.
+__global__ __launch_bounds__(256) void k_preview_pair_blocks(const uint64_t* __restrict__ off, uint32_t n, uint32_t m,
+                                                             uint32_t nb, uint32_t* __restrict__ blk) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nb) return;
+  blk[b] = last_at_or_below(off, 0, n, (uint64_t)min(b * 256u, m - 1));
+}
+
+__global__ __launch_bounds__(256) void k_preview_pairs(const int4* __restrict__ trects,
+                                                       const uint64_t* __restrict__ off, uint32_t n, uint32_t m,
+                                                       const uint32_t* __restrict__ blk, int ntx,
                                                        uint32_t* __restrict__ pk, uint32_t* __restrict__ pv) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const int4 r = rects[p];
-  if (!(r.z > r.x && r.w > r.y)) return;
-  uint64_t o = off[p];
-  for (int ty = r.y / TILE; ty <= (r.w - 1) / TILE; ++ty)
-    for (int tx = r.x / TILE; tx <= (r.z - 1) / TILE; ++tx) {
-      pk[o] = (uint32_t)(ty * ntx + tx);
-      pv[o] = p;
-      ++o;
+  __shared__ uint64_t s_off[256];
+  const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t lo = blk[blockIdx.x], span = blk[blockIdx.x + 1] - lo + 1;
+  if (span <= 256 && threadIdx.x < span) s_off[threadIdx.x] = off[lo + threadIdx.x];
+  __syncthreads();
+  if (q >= m) return;
+  uint32_t p;
+  uint64_t first;
+  if (span <= 256) {
+    uint32_t a = 0, b = span;  // s_off[a] <= q < s_off[b] (s_off[span] virtual)
+    while (b - a > 1) {
+      const uint32_t mid = (a + b) / 2;
+      if (s_off[mid] <= q) a = mid;
+      else b = mid;
     }
+    p = lo + a;
+    first = s_off[a];
+  } else {
+    p = last_at_or_below(off, lo, lo + span, q);
+    first = off[p];
+  }
+  const int4 r = trects[p];
+  const uint32_t k = q - (uint32_t)first, wt = (uint32_t)(r.z - r.x);
+  pk[q] = (uint32_t)((r.y + (int)(k / wt)) * ntx + r.x + (int)(k % wt));
+  pv[q] = p;
 }
 
-void launch_preview_pairs(const int4* rects, const uint64_t* off, uint32_t n, int ntx, uint32_t* pk, uint32_t* pv,
-                          hipStream_t st) {
-  if (!n) return;
-  hipLaunchKernelGGL(k_preview_pairs, dim3((n + 255) / 256), dim3(256), 0, st, rects, off, n, ntx, pk, pv);
+void launch_preview_pairs(const int4* trects, const uint64_t* off, uint32_t n, uint32_t m, int ntx, uint32_t* blk,
+                          uint32_t* pk, uint32_t* pv, hipStream_t st) {
+  if (!n || !m) return;
+  const uint32_t nb = (m + 255) / 256;
+  hipLaunchKernelGGL(k_preview_pair_blocks, dim3(nb / 256 + 1), dim3(256), 0, st, off, n, m, nb, blk);
+  hipLaunchKernelGGL(k_preview_pairs, dim3(nb), dim3(256), 0, st, trects, off, n, m, blk, ntx, pk, pv);
 }
 
 // [start, end) of every tile's run in the tile-sorted pairs (tiles without
@@ -227,8 +297,11 @@ __global__ __launch_bounds__(256) void k_preview_composite(const float* __restri
   const uint32_t b = rs[tile], e = re[tile];
   float cr = 0.0f, cg = 0.0f, cb = 0.0f, tr = 1.0f;
   const float fx = (float)px + 0.5f, fy = (float)py + 0.5f;
+  const bool inside = px < w && py < h;
   for (uint32_t base = b; base < e; base += 256) {
-    __syncthreads();
+    // composite skips a pixel once T < 1e-4 (T never grows): stop when every
+    // pixel of the tile is there
+    if (__syncthreads_count(inside && !(tr < TRANSMITTANCE_STOP)) == 0) break;
     if (base + t < e) {
       const uint32_t p = pv[base + t];
       const float* s = splats + (size_t)order[p] * 10;
@@ -257,7 +330,7 @@ __global__ __launch_bounds__(256) void k_preview_composite(const float* __restri
       tr = T * (1.0f - alpha);
     }
   }
-  if (px < w && py < h) {
+  if (inside) {
     float* o = rgb + ((size_t)py * w + px) * 3;
     o[0] = cr + tr * bg0;
     o[1] = cg + tr * bg1;

@@ -239,6 +239,7 @@ struct rgs_ctx {
   DBuf io_soa, io_rec, io_fields;  // model files: rows of host records, records, PLY property table
   DBuf srgb_lut;                   // RGBA8 -> linear table (512 floats), filled once
   DBuf sfm_xyz, sfm_rgb, sfm_k[2], sfm_v[2], sfm_khi, sfm_sx, sfm_scale, sfm_hist, sfm_rowtot, sfm_misc;
+  DBuf pv_blk;                           // preview: first splat of every 256 pairs
   DBuf rs_taps, rs_tmp, rs_src, rs_dst;  // resize: taps and weights, f32 rows, source and result
   uint32_t* pv_pairs = nullptr;  // the tile-sorted pair values of the last preview render
   uint64_t pv_last_pairs = 0;
@@ -741,7 +742,7 @@ void rgs_destroy(rgs_ctx* c) {
                  &c->sfm_rowtot, &c->sfm_misc, &c->pv_sp, &c->pv_k[0], &c->pv_k[1], &c->pv_v[0], &c->pv_v[1],
                  &c->pv_rect, &c->pv_cnt, &c->pv_off, &c->pv_bs, &c->pv_misc, &c->pv_pk[0], &c->pv_pk[1],
                  &c->pv_pv[0], &c->pv_pv[1], &c->pv_hist, &c->pv_rowtot, &c->pv_rs, &c->pv_re, &c->pv_rgb,
-                 &c->rs_taps, &c->rs_tmp, &c->rs_src, &c->rs_dst,
+                 &c->rs_taps, &c->rs_tmp, &c->rs_src, &c->rs_dst, &c->pv_blk,
                  &c->adam_d, &c->api_acc9, &c->tile_order[0], &c->tile_order[1], &c->cover, &c->depth,
                  &c->flag, &c->tile_flag, &c->rbstart, &c->rbend, &c->pv2, &c->cntb, &c->cutst, &c->part,
                  &c->kB, &c->vB, &c->kC[0], &c->kC[1], &c->vC[0], &c->vC[1]};
@@ -1466,12 +1467,13 @@ int preview_run(rgs_ctx* c, const float* P, size_t cap, size_t n, const rgs_came
                                      c->st)
                   : 0;
   // rects, pair offsets, pairs
-  CHK(ensure(c, c->pv_rect, nn * 16));
+  CHK(ensure(c, c->pv_rect, nn * 32));  // composite rects, then the tile ranges to list
   CHK(ensure(c, c->pv_cnt, nn * 8));
   CHK(ensure(c, c->pv_off, nn * 8));
   CHK(ensure(c, c->pv_bs, ((size_t)scan_nb(nn) + 2) * 8));
   int4* rects = P_<int4>(c->pv_rect);
-  launch_preview_rects(P_<float>(c->pv_sp), ka[a], va[a], n32, w, h, rects, P_<uint64_t>(c->pv_cnt), c->st);
+  int4* trects = rects + nn;
+  launch_preview_rects(P_<float>(c->pv_sp), ka[a], va[a], n32, w, h, rects, trects, P_<uint64_t>(c->pv_cnt), c->st);
   uint64_t* total_d = P_<uint64_t>(c->pv_bs) + scan_nb(nn) + 1;
   uint64_t total = 0;
   if (n) {
@@ -1492,7 +1494,9 @@ int preview_run(rgs_ctx* c, const float* P, size_t cap, size_t n, const rgs_came
   if (total) {
     uint32_t* pk[2] = {P_<uint32_t>(c->pv_pk[0]), P_<uint32_t>(c->pv_pk[1])};
     uint32_t* pv[2] = {P_<uint32_t>(c->pv_pv[0]), P_<uint32_t>(c->pv_pv[1])};
-    launch_preview_pairs(rects, P_<uint64_t>(c->pv_off), n32, ntx, pk[0], pv[0], c->st);
+    CHK(ensure(c, c->pv_blk, ((size_t)(total + 255) / 256 + 1) * 4));
+    launch_preview_pairs(trects, P_<uint64_t>(c->pv_off), n32, (uint32_t)total, ntx, P_<uint32_t>(c->pv_blk), pk[0],
+                         pv[0], c->st);
     misc[1] = (uint32_t)total;
     HIPCHK(hipMemcpyAsync(n_dev, misc, 16, hipMemcpyHostToDevice, c->st));
     int bits = 8;

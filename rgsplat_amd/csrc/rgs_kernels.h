@@ -286,9 +286,10 @@ struct PreviewCam {
 void launch_preview_project(const float* P, int cap, uint32_t n, const PreviewCam& cam, float* splats,
                             uint32_t* keys, uint32_t* vals, hipStream_t st);
 void launch_preview_rects(const float* splats, const uint32_t* keys, const uint32_t* order, uint32_t n, int w, int h,
-                          int4* rects, uint64_t* cnt, hipStream_t st);
-void launch_preview_pairs(const int4* rects, const uint64_t* off, uint32_t n, int ntx, uint32_t* pk, uint32_t* pv,
-                          hipStream_t st);
+                          int4* rects, int4* trects, uint64_t* cnt, hipStream_t st);
+// blk: (m + 255) / 256 + 1 entries of scratch
+void launch_preview_pairs(const int4* trects, const uint64_t* off, uint32_t n, uint32_t m, int ntx, uint32_t* blk,
+                          uint32_t* pk, uint32_t* pv, hipStream_t st);
 void launch_preview_ranges(const uint32_t* pk, uint32_t m, uint32_t* rs, uint32_t* re, hipStream_t st);
 void launch_preview_composite(const float* splats, const uint32_t* order, const int4* rects, const uint32_t* pv,
                               const uint32_t* rs, const uint32_t* re, int w, int h, int ntx, int nty, const float bg[3],
