@@ -275,8 +275,10 @@ def main():
                     help="time the target ingestion instead (RGBA8 images resized and converted on the device)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the step shards by views "
-              "across GPUs as described in DESIGN.md, but no multi-GPU path is built", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the step would shard by views "
+              "(each GPU renders its share of the views, then a reduce-scatter of the gradient rows, Adam on "
+              "each GPU's shard and an all-gather of the parameters), but no multi-GPU path is built",
+              file=sys.stderr)
         sys.exit(2)
 
     from rgsplat_amd import runtime
