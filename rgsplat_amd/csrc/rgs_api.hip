@@ -1698,6 +1698,7 @@ Taps resize_taps(uint32_t src, uint32_t dst) {
 // dh) with image 0.25's resize_exact(.., Triangle).
 int resize_device(rgs_ctx* c, const uint8_t* src, uint32_t sw, uint32_t sh, uint8_t* dst, uint32_t dw,
                   uint32_t dh) {
+  if (dh > 65535) return fail(RGS_E_ARG, "resize: output taller than 65535 rows");  // grid rows
   const Taps v = resize_taps(sh, dh), h = resize_taps(sw, dw);
   const size_t vb = v.lc.size() * 4, vwb = v.w.size() * 4, hb = h.lc.size() * 4, hwb = h.w.size() * 4;
   CHK(ensure(c, c->rs_taps, vb + vwb + hb + hwb));
