@@ -14,10 +14,12 @@
 // of its 3-sigma rect) becomes a loop per 16x16 tile over the splats whose
 // rect meets the tile, in depth order, with the rect test per pixel: every
 // pixel sees the same splats in the same order, so the f32 results equal
-// composite's. Depth order: a stable radix sort of the depth keys (ties in
-// index order; the reference's sort_unstable_by leaves them unspecified).
-// (tile, sorted position) pairs are emitted per splat in depth order and
-// stably sorted by tile, so each tile's list stays in depth order.
+// composite's. A tile whose pixels a splat's alpha-reach ellipse cannot
+// reach (alpha < 1/255 there, which composite skips) does not list it.
+// Depth order: a stable radix sort of the depth keys (ties in index order;
+// the reference's sort_unstable_by leaves them unspecified). (tile, sorted
+// position) pairs are emitted in splat-major order and stably sorted by
+// tile, so each tile's list stays in depth order.
 #include "rgs_kernels.h"
 
 namespace rgs {
