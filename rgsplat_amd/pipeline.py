@@ -48,7 +48,8 @@ def train(ctx: runtime.Context, gaussians, cameras, images, cfg: dict | None = N
           use_graph: bool = True) -> tuple[np.ndarray, TrainLog]:
     """Runs cfg['num_iterations'] iterations of train_blocking's loop on the
     device and returns the trained Gaussians (n, 59) and a log. `images` are
-    the cameras' linear RGBA targets (h, w, 4). The scene's capacity is
+    the cameras' linear RGBA targets (h, w, 4), or None when the views are
+    already registered with the context. The scene's capacity is
     cfg['max_gaussians'] (densify never grows past it, as in the reference).
     save_interval 0 writes no checkpoints."""
     cfg = dict(cfg or synthetic.train_config())
@@ -56,7 +57,8 @@ def train(ctx: runtime.Context, gaussians, cameras, images, cfg: dict | None = N
     extent = runtime.scene_extent(g)  # estimate_scene_extent, once (pipeline/mod.rs:214)
     cap = max(int(cfg["max_gaussians"]), len(g), 1)
     ctx.scene_upload(g, capacity=cap)
-    ctx.set_views(cameras, images)
+    if images is not None:
+        ctx.set_views(cameras, images)
     cams = camera_sequence(len(cameras), int(cfg["num_iterations"]))
     log = TrainLog()
     for it, cam in enumerate(cams):
@@ -73,3 +75,22 @@ def train(ctx: runtime.Context, gaussians, cameras, images, cfg: dict | None = N
             log.checkpoints.append(path)
         log.sizes.append(ctx.scene_size())
     return ctx.scene_download(), log
+
+
+def train_from_sfm(ctx: runtime.Context, xyz, rgb, cameras, images_rgba8, cfg: dict | None = None,
+                   bg=(0.0, 0.0, 0.0), save_interval: int = 1, out_dir: str = ".",
+                   use_graph: bool = True) -> tuple[np.ndarray, TrainLog]:
+    """Stages 3-5 of the reference's pipeline (pipeline/mod.rs:140-172,
+    177-193, 270-296) from SfM points and decoded images: gaussians_from_sfm
+    on the device, the targets resized and converted on the device
+    (load_target after decoding), train_blocking's loop, then output.ply and
+    output.splat in out_dir. As in the reference, save_interval is at least
+    1. Parsing COLMAP's binary files and decoding the images stay with the
+    caller."""
+    g = ctx.gaussians_from_sfm(xyz, rgb)
+    ctx.set_views_rgba8_sized(cameras, images_rgba8)
+    out, log = train(ctx, g, cameras, None, cfg, bg=bg, save_interval=max(int(save_interval), 1),
+                     out_dir=out_dir, use_graph=use_graph)
+    ctx.scene_save_ply(os.path.join(out_dir, "output.ply"))
+    ctx.scene_save_splat(os.path.join(out_dir, "output.splat"))
+    return out, log

@@ -38,3 +38,23 @@ def test_train_driver(tmp_path):
     assert np.isfinite(ck).all()
     assert np.all(1.0 / (1.0 + np.exp(-ck[:, 3].astype(np.float64))) <= 0.01 + 1e-6)
     ctx.close()
+
+
+def test_train_from_sfm(tmp_path):
+    """SfM points and RGBA8 images of another size to output.ply / .splat."""
+    from rgsplat_amd import runtime
+    from tests import sfm_points
+    xyz, rgb = sfm_points.cloud(2000, 11)
+    xyz *= 0.3
+    cams = S.make_cameras(5, [(64, 48), (80, 64)], focal_scale=1.0)
+    rng = np.random.default_rng(2)
+    ims = [rng.integers(0, 256, (c.height * 2, c.width * 2, 4), dtype=np.uint8) for c in cams]
+    cfg = S.benchmark_train_config()
+    cfg.update(num_iterations=30, max_gaussians=4000)
+    ctx = runtime.Context(0)
+    out, log = pipeline.train_from_sfm(ctx, xyz, rgb, cams, ims, cfg, save_interval=10, out_dir=str(tmp_path))
+    assert len(log.checkpoints) == 2 and np.all(np.isfinite(log.losses))
+    assert np.array_equal(oracle.load_ply(tmp_path / "output.ply").view(np.uint32), out.view(np.uint32))
+    oracle.save_splat(out, tmp_path / "cpu.splat")
+    assert (tmp_path / "output.splat").read_bytes() == (tmp_path / "cpu.splat").read_bytes()
+    ctx.close()
