@@ -241,6 +241,10 @@ int rgs_preview_project(rgs_ctx* ctx, const rgs_gaussian* g, size_t n, const rgs
 int rgs_preview_render(rgs_ctx* ctx, const rgs_gaussian* g, size_t n, const rgs_camera* cam, const float bg[3],
                        float* rgb_out);
 int rgs_scene_preview_render(rgs_ctx* ctx, const rgs_camera* cam, const float bg[3], float* rgb_out);
+/* As rgs_scene_preview_render into device memory (h*w*3 floats on the
+ * context's device, written in stream order on the context's stream): a
+ * viewer's frame without the copy to the host. */
+int rgs_scene_preview_render_device(rgs_ctx* ctx, const rgs_camera* cam, const float bg[3], float* rgb_dev);
 
 /* ---- SfM initialisation (pipeline/sfm.rs:365-455). gaussians_from_sfm for
  * n points (xyz: n x 3 f64, rgb: n x 3 bytes; SfMPoint's other fields are

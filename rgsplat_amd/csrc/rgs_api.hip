@@ -1424,7 +1424,7 @@ namespace {
 // GpuRenderer::project of the n Gaussians in rows P (stride cap) into
 // c->pv_sp, then, when rgb is set, composite into rgb (host, h*w*3).
 int preview_run(rgs_ctx* c, const float* P, size_t cap, size_t n, const rgs_camera* cam, const float* bg,
-                float* splats_host, float* rgb) {
+                float* splats_host, float* rgb, bool rgb_on_device = false) {
   if (!cam->width || !cam->height) return fail(RGS_E_ARG, "camera with zero width or height");
   float view[16];
   if (!view_from_c2w(cam->c2w, view)) return fail(RGS_E_CAMERA, "c2w must be invertible");
@@ -1508,11 +1508,12 @@ int preview_run(rgs_ctx* c, const float* P, size_t cap, size_t n, const rgs_came
     launch_preview_ranges(pk[q], (uint32_t)total, P_<uint32_t>(c->pv_rs), P_<uint32_t>(c->pv_re), c->st);
     c->pv_pairs = pv[q];
   }
-  CHK(ensure(c, c->pv_rgb, (size_t)w * h * 12));
+  if (!rgb_on_device) CHK(ensure(c, c->pv_rgb, (size_t)w * h * 12));
   launch_preview_composite(P_<float>(c->pv_sp), va[a], rects, total ? c->pv_pairs : P_<uint32_t>(c->pv_pv[0]),
-                           P_<uint32_t>(c->pv_rs), P_<uint32_t>(c->pv_re), w, h, ntx, nty, bg, P_<float>(c->pv_rgb),
-                           c->st);
-  HIPCHK(hipMemcpyAsync(rgb, c->pv_rgb.p, (size_t)w * h * 12, hipMemcpyDeviceToHost, c->st));
+                           P_<uint32_t>(c->pv_rs), P_<uint32_t>(c->pv_re), w, h, ntx, nty, bg,
+                           rgb_on_device ? rgb : P_<float>(c->pv_rgb), c->st);
+  if (!rgb_on_device)
+    HIPCHK(hipMemcpyAsync(rgb, c->pv_rgb.p, (size_t)w * h * 12, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   c->pv_last_pairs = total;
   return 0;
@@ -1541,6 +1542,13 @@ int rgs_scene_preview_render(rgs_ctx* c, const rgs_camera* cam, const float bg[3
   if (!rgb_out || !bg) return fail(RGS_E_ARG, "null output or bg");
   (void)hipSetDevice(c->device);
   return preview_run(c, P_<float>(c->P), c->cap, c->n, cam, bg, nullptr, rgb_out);
+}
+
+int rgs_scene_preview_render_device(rgs_ctx* c, const rgs_camera* cam, const float bg[3], float* rgb_dev) {
+  if (!c || !cam) return fail(RGS_E_ARG, "null argument");
+  if (!rgb_dev || !bg) return fail(RGS_E_ARG, "null output or bg");
+  (void)hipSetDevice(c->device);
+  return preview_run(c, P_<float>(c->P), c->cap, c->n, cam, bg, nullptr, rgb_dev, true);
 }
 
 // ---- SfM initialisation (pipeline/sfm.rs:365-455) ---------------------------

@@ -85,6 +85,7 @@ SIGNATURES = {
     "rgs_preview_project": [_vp, _vp, _sz, _CAM, _vp],
     "rgs_preview_render": [_vp, _vp, _sz, _CAM, _vp, _vp],
     "rgs_scene_preview_render": [_vp, _CAM, _vp, _vp],
+    "rgs_scene_preview_render_device": [_vp, _CAM, _vp, _vp],
     "rgs_gaussians_from_sfm": [_vp, _vp, _vp, _sz, _vp],
     "rgs_scene_from_sfm": [_vp, _vp, _vp, _sz, _sz],
     "rgs_adam_download": [_vp, _ADAM],
@@ -259,6 +260,18 @@ class Context:
         out = np.zeros((cam.height, cam.width, 3), np.float32)
         c = camera(cam)
         _check(lib().rgs_scene_preview_render(self.h, ctypes.byref(c), _p(_f32(bg)), _p(out)))
+        return out
+
+    def scene_preview_render_device(self, cam, out, bg=(0.0, 0.0, 0.0)):
+        """The preview render of the device scene into `out`, a contiguous
+        float32 torch tensor of shape (h, w, 3) on this context's device."""
+        if tuple(out.shape) != (cam.height, cam.width, 3) or not out.is_contiguous():
+            raise ValueError("out must be a contiguous (h, w, 3) tensor")
+        if str(out.dtype) != "torch.float32" or out.device.type != "cuda":
+            raise ValueError("out must be a float32 tensor on the GPU")
+        c = camera(cam)
+        _check(lib().rgs_scene_preview_render_device(self.h, ctypes.byref(c), _p(_f32(bg)),
+                                                     ctypes.c_void_p(out.data_ptr())))
         return out
 
     def photometric_loss_grad(self, rendered, target_rgba, w, h, lambda_dssim=0.2):

@@ -80,3 +80,13 @@ def test_scene_preview_and_edges(ctx):
     bad.c2w = np.zeros((4, 4), np.float32)
     with pytest.raises(RgsError, match="error 3"):
         ctx.preview_render(g[:10], bad)
+
+
+def test_scene_preview_to_device(ctx):
+    import torch
+    g = C.scene(5000, 12, opacity=(0.05, 0.99))
+    cam = S.look_at_origin(np.array([0.1, 0.9, 0.6]), 3.0, 200, 120, 180.0)
+    ctx.scene_upload(g)
+    out = torch.empty((120, 200, 3), dtype=torch.float32, device="cuda:0")
+    ctx.scene_preview_render_device(cam, out, (0.3, 0.2, 0.1))
+    assert same(out.cpu().numpy(), oracle.preview_render(g, cam, (0.3, 0.2, 0.1)))
