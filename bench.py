@@ -43,6 +43,10 @@ FP32_PEAK_TFLOPS = 157.3
 FWD_OPS_PER_PAIR = 26
 BWD_OPS_PER_PAIR = 62
 SSIM_FLOP_PER_PIXEL = 1440
+# The VALU kernels use no matrix cores. 157.3 TFLOP/s is MI355X's published
+# FP32 vector peak, which counts packed FP32 (two FMAs per lane per cycle);
+# this build disables packed FP32, so its FP32 issue ceiling is half of it.
+FP32_SCALAR_TFLOPS = FP32_PEAK_TFLOPS / 2
 
 
 def measured_traffic(workload: str) -> dict:
@@ -74,45 +78,103 @@ def with_alpha(rgb):
     return np.concatenate([rgb, np.ones(rgb.shape[:2] + (1,), np.float32)], -1)
 
 
-def grad_rel_err(ctx) -> float:
-    """Max over parameter groups of ||g_gpu - g_cpu|| / ||g_cpu|| at the
-    10k-Gaussian 256x256 parity size."""
-    import oracle
-    from rgsplat_amd import synthetic as S
-    wl = S.WORKLOADS["c0_10k_256"]
-    g = wl.scene()
-    cam = wl.cameras()[0]
-    tgt = with_alpha(oracle.render_color(wl.target_scene(), cam))
-    _, gr = ctx.render_and_backward(g, cam, tgt)
-    _, ref = oracle.render_and_backward(g, cam, tgt)
-    worst = 0.0
-    for sl in [slice(0, 3), slice(3, 7), slice(7, 10), slice(10, 11), slice(11, 59)]:
-        d = np.linalg.norm(ref[:, sl])
-        worst = max(worst, float(np.linalg.norm(gr[:, sl] - ref[:, sl]) / d))
-    return worst
+def cpu_info() -> dict:
+    """Host CPU model and the threads this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count() or 1, "allowed": allowed}
 
 
-def cpu_baseline(g, cams, targets, cfg, threads):
-    """The oracle (C++ restatement of the reference CPU path, OpenMP where the
-    reference uses rayon) on one view of the workload plus one Adam step."""
+def step_check_and_cpu_baseline(ctx, ids, cams, targets, cfg, threads, cpu=True):
+    """One captured step of the timed configuration (graph replay, the
+    adaptive depth cut) against the oracle, and the oracle's own step time.
+
+    The step's summed gradients are compared with the oracle's
+    render_and_backward of every view at the pre-step parameters (max over
+    parameter groups of ||g - g_ref|| / ||g_ref||, and the per-component
+    maximum), and with the oracle's backward driven by the loss gradient of
+    the step's own image (the L1 term's signum flips where rounding moves a
+    pixel across its target, which on a trained scene moves the gradient by
+    ~1e-4 relative). The same oracle work times the CPU path: 1 warm-up view,
+    then 3 full steps (every view's render_and_backward plus one adam_step),
+    median of the 3."""
     import oracle
+    from tests.step_parity import group_rel_err, component_rel_err
     oracle.set_threads(threads)
-    gg = np.ascontiguousarray(g.copy())
+    g0 = ctx.scene_download()
+    ctx.set_grad_capture(True)
+    ctx.train_step(ids, cfg, use_graph=True)
+    grads = ctx.step_grads()
+    img = ctx.step_image(sum(c.width * c.height for c in cams))
+    ctx.set_grad_capture(False)
+    out = {}
     t0 = time.perf_counter()
-    _, grads = oracle.render_and_backward(gg, cams[0], targets[0])
-    t1 = time.perf_counter()
-    st = oracle.AdamState(len(gg))
-    oracle.adam_step(gg, grads, st, cfg)
-    t2 = time.perf_counter()
-    per_step = len(cams) * (t1 - t0) + (t2 - t1)
-    return {"value": 1.0 / per_step, "unit": "iters/s", "cores": threads, "kind": "port",
-            "sample": f"1 of the {len(cams)} views (render_and_backward) + 1 adam_step at the full workload size; "
-                      f"step time = {len(cams)} x {t1 - t0:.2f} s + {t2 - t1:.2f} s"}
+    oracle.render_and_backward(g0, cams[0], targets[0])
+    warm = time.perf_counter() - t0
+    steps = []
+    ref = ref_dev = None
+    img_err = 0.0
+    for rep in range(3 if cpu else 1):
+        t0 = time.perf_counter()
+        tot = np.zeros((len(g0), 60), np.float32 if rep else np.float64)
+        extra = 0.0
+        base = 0
+        for c, t in zip(cams, targets):
+            r = oracle.Render(g0, c, t)
+            tot += r.grads()
+            if rep == 0:
+                e0 = time.perf_counter()
+                npx = c.width * c.height
+                rgb = r.image()[0]
+                dimg = img[base:base + npx].reshape(c.height, c.width, 3)
+                base += npx
+                img_err = max(img_err, float(np.abs(dimg - rgb).max()))
+                _, dl = oracle.loss_grad(dimg, t, c.width, c.height, 0.2)
+                g_dev = r.backward_with_dl(dl)
+                ref_dev = g_dev.astype(np.float64) if ref_dev is None else ref_dev + g_dev
+                extra += time.perf_counter() - e0
+            del r
+        gg = np.ascontiguousarray(g0.copy())
+        st = oracle.AdamState(len(gg))
+        oracle.adam_step(gg, tot.astype(np.float32), st, cfg)
+        steps.append(time.perf_counter() - t0 - extra)
+        if ref is None:
+            ref = tot
+        print(f"[bench] oracle step {rep}: {steps[-1]:.2f} s", file=sys.stderr, flush=True)
+    groups = group_rel_err(grads, ref)
+    out["grad_rel_err"] = max(groups.values())
+    out["grad_rel_err_groups"] = groups
+    out["grad_component_rel_err"] = component_rel_err(grads, ref)
+    same = group_rel_err(grads, ref_dev)
+    out["grad_rel_err_same_dl"] = max(same.values())
+    out["image_max_abs_err"] = img_err
+    if cpu:
+        med = float(np.median(steps))
+        info = cpu_info()
+        out["cpu_baseline"] = {
+            "value": 1.0 / med, "unit": "iters/s", "cores": threads, "kind": "port",
+            "cpu_model": info["model"], "host_nproc": info["nproc"], "host_allowed_cpus": info["allowed"],
+            "sample": f"the oracle (OpenMP, {threads} threads) on this workload's full step: render_and_backward "
+                      f"of all {len(cams)} views + 1 adam_step at the pre-step parameters of a timed-configuration "
+                      f"step; 1 warm-up view ({warm:.2f} s), then 3 steps, median "
+                      f"({', '.join(f'{x:.2f}' for x in steps)} s)"}
+    return out
 
 
-def plain_path(ctx, g, cams, targets, cfg):
+def plain_path_api(ctx, g, cams, targets, cfg):
     """The reference-API path on the GPU: one render_and_backward call per
-    view (host buffers in and out) and one adam_step call."""
+    view (host buffers in and out) and one adam_step call: PCIe-bound."""
     from rgsplat_amd.runtime import AdamState
     gg = np.ascontiguousarray(g.copy())
     st = AdamState(len(gg))
@@ -126,6 +188,27 @@ def plain_path(ctx, g, cams, targets, cfg):
             tot += gr
         ctx.adam_step(gg, tot, st, cfg)
     t1 = time.perf_counter()
+    return steps / (t1 - t0)
+
+
+def plain_path_device(runtime, g, cams, targets, cfg, steps=3):
+    """The plain device path: the scene resident in HBM, one view per launch
+    sequence (train_step of a single view, eager, no graph), its own Adam
+    step per view as the reference's loop does. Reported per iteration of the
+    same views as the batched step (one eager single-view step per view)."""
+    ctx = runtime.Context(0)
+    ctx.scene_upload(g, capacity=len(g))
+    ctx.set_views(cams, targets)
+    for v in range(len(cams)):
+        ctx.train_step([v], cfg, use_graph=False)
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for v in range(len(cams)):
+            ctx.train_step([v], cfg, use_graph=False)
+    sync()
+    t1 = time.perf_counter()
+    ctx.close()
     return steps / (t1 - t0)
 
 
@@ -303,11 +386,24 @@ def main():
     ctx = runtime.Context(0)
     tscene = wl.target_scene()
     targets = [with_alpha(ctx.render_color(tscene, c)) for c in cams]
+    # The oracle check and CPU baseline run on the timed views, except for a
+    # clustered scene, whose 1080p oracle backward would hold ~1.5e9 (tile,
+    # splat) accumulators per view: there they run on a 128x128 window of
+    # each view around its densest cluster (registered beside the timed views).
+    check_cams, check_targets = cams, targets
+    if wl.clusters:
+        check_cams = S.cluster_crops(wl)
+        check_targets = [with_alpha(ctx.render_color(tscene, c)) for c in check_cams]
     del tscene
     cfg = S.benchmark_train_config()
     ctx.scene_upload(g, capacity=len(g))
-    ctx.set_views(cams, targets)
     ids = list(range(len(cams)))
+    if check_cams is cams:
+        ctx.set_views(cams, targets)
+        check_ids = ids
+    else:
+        ctx.set_views(cams + check_cams, targets + check_targets)
+        check_ids = list(range(len(cams), len(cams) + len(check_cams)))
     use_graph = not args.no_graph
     tuned = None
     if not args.no_autotune:
@@ -387,9 +483,9 @@ def main():
         npix = sum(w * h for w, h in wl.sizes)
         ng = ctx.scene_size()
         work = {
-            "raster_fwd": ("mfma", FWD_OPS_PER_PAIR * pf, "TFLOP/s"),
-            "raster_bwd": ("mfma", BWD_OPS_PER_PAIR * pb, "TFLOP/s"),
-            "loss_ssim": ("mfma", SSIM_FLOP_PER_PIXEL * npix, "TFLOP/s"),
+            "raster_fwd": ("valu", FWD_OPS_PER_PAIR * pf, "TFLOP/s"),
+            "raster_bwd": ("valu", BWD_OPS_PER_PAIR * pb, "TFLOP/s"),
+            "loss_ssim": ("valu", SSIM_FLOP_PER_PIXEL * npix, "TFLOP/s"),
             "project": ("hbm", 236 * ng + 60 * ng * len(cams), "GB/s"),
             "geometry_adam": ("hbm", 1652 * ng, "GB/s"),
         }
@@ -405,10 +501,17 @@ def main():
                 peak = FP32_PEAK_TFLOPS
             rl[s] = {"bound": bound, "achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak,
                      "traffic": traffic.get(s), "ms": stages[s]}
+            if bound == "valu":
+                rl[s]["frac_scalar_issue"] = ach / FP32_SCALAR_TFLOPS
+                rl[s]["achieved_basis"] = ("reference-equivalent: the reference algorithm's evaluated (pixel, splat) "
+                                           "pairs x ops per pair; the kernel culls most of them"
+                                           if s != "loss_ssim" else "FLOP per pixel x pixels")
         if dom in rl:
             out["roofline"] = dict(rl[dom], kernel=dom)
             if traffic.get(dom) is not None:
-                out["roofline"]["traffic_source"] = f"profiles/hbm_traffic_{wl.name}.md (rocprofv3 FETCH_SIZE/WRITE_SIZE)"
+                out["roofline"]["traffic_source"] = (f"profiles/hbm_traffic_{wl.name}.md (rocprofv3 FETCH_SIZE/"
+                                                     "WRITE_SIZE: bytes between L2 and the memory fabric, which "
+                                                     "the Infinity Cache may serve; an upper bound on HBM bytes)")
         else:
             out["roofline"] = {"kernel": dom, "bound": None, "achieved": None, "peak": None, "unit": None,
                                "frac": None, "traffic": traffic.get(dom), "ms": stages[dom]}
@@ -422,19 +525,40 @@ def main():
                        "gaussians": ng, "timed_steps_with_events": n_prof, "of_which_graph_replays": n_graph}
         print(f"[bench] stages {json.dumps(stages)}", file=sys.stderr, flush=True)
 
-    if not args.no_plain:
-        print("[bench] plain path", file=sys.stderr, flush=True)
-        out["plain_gpu_path_iters_per_s"] = plain_path(ctx, g, cams, targets, cfg)
+    threads = min(16, cpu_info()["allowed"])
     try:
-        out["grad_rel_err"] = grad_rel_err(ctx)
-    except Exception as e:  # the oracle may be unbuilt on a bare box
+        chk = step_check_and_cpu_baseline(ctx, check_ids, check_cams, check_targets, cfg, threads,
+                                          cpu=not args.no_cpu)
+        out.update(chk)
+        if check_cams is not cams:
+            out["check_views"] = ("128x128 windows of the timed views around their densest clusters (the full "
+                                  "3M-Gaussian scene, each tile listing tens of thousands of Gaussians)")
+        out["grad_rel_err_source"] = ("one captured step of the timed configuration (graph replay, depth cut) at "
+                                      "the full workload size after the timed steps, against the oracle's "
+                                      "render_and_backward of every view, max over parameter groups; "
+                                      "grad_rel_err_same_dl: against the oracle's backward of the loss gradient of "
+                                      "the step's own image")
+    except ImportError as e:  # the oracle may be unbuilt on a bare box
         out["grad_rel_err"] = None
         print(f"[bench] grad check skipped: {e}", file=sys.stderr)
-    if not args.no_cpu:
-        print("[bench] cpu baseline (oracle on the host)", file=sys.stderr, flush=True)
-        threads = min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(g, cams, targets, cfg, threads)
-    ctx.close()
+    if not args.no_plain:
+        print("[bench] plain paths", file=sys.stderr, flush=True)
+        ctx.close()
+        ctx = None
+        plain = plain_path_device(runtime, g, cams, targets, cfg)
+        out["plain_gpu_path_iters_per_s"] = plain
+        out["plain_gpu_path"] = ("device-resident eager train_step of one view per launch sequence, one per view "
+                                 "of the batch (each with its Adam step, as the reference's loop); per iteration "
+                                 "over the batch's views")
+        out["speedup_vs_plain_gpu_path"] = out["value"] / plain
+        ctx = runtime.Context(0)
+        out["api_gpu_path_iters_per_s"] = plain_path_api(ctx, g, cams, targets, cfg)
+        out["api_gpu_path"] = ("render_and_backward per view with host buffers + adam_step (PCIe-bound: "
+                               "parameters and gradients cross PCIe per view)")
+    if "cpu_baseline" in out:
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    if ctx is not None:
+        ctx.close()
     print(json.dumps(out), flush=True)
 
 

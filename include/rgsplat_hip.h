@@ -25,7 +25,8 @@ enum {
   RGS_E_HIP = 2,      /* HIP runtime failure */
   RGS_E_CAMERA = 3,   /* c2w not invertible (math/mod.rs:155-160 panics) */
   RGS_E_NAN = 4,      /* NaN depth (backward.rs:191 panics in the sort) */
-  RGS_E_CAPACITY = 5, /* densify result exceeds the caller's capacity */
+  RGS_E_CAPACITY = 5, /* densify result exceeds the caller's capacity, or a
+                         render needs 2^31 - 1 or more (tile, splat) pairs */
   RGS_E_IO = 6,       /* a model file could not be opened, read or written */
   RGS_E_FORMAT = 7    /* not a PLY the reference reads (io/ply.rs:116-170 bails) */
 };
@@ -307,6 +308,15 @@ int rgs_stage_times(rgs_ctx* ctx, float* mean_ms, int n_stages, uint64_t* steps,
 int rgs_eval_expf(rgs_ctx* ctx, const float* x, float* out, size_t n);
 int rgs_debug_tiles(rgs_ctx* ctx, uint64_t* offsets, size_t n_offsets, uint32_t* gis, size_t k_max,
                     uint64_t* k_out);
+/* Gradient capture of rgs_train_step: with enable != 0, each step also
+ * writes the gradients its Adam update consumed (summed over the step's
+ * views, GaussianGrad order; screen_grad_norm is the sum of the views'
+ * norms), and rgs_step_grads returns the last step's for the n = scene size
+ * Gaussians. Replaces nothing in the reference: it exposes the values the
+ * fused geometry/Adam kernel keeps in registers, so that the step's
+ * gradients and its Adam update can each be compared with the CPU path. */
+int rgs_set_grad_capture(rgs_ctx* ctx, int enable);
+int rgs_step_grads(rgs_ctx* ctx, rgs_grad* grads_out, size_t n);
 
 #ifdef __cplusplus
 }

@@ -67,10 +67,15 @@ def _declare(L):
     L.orc_get_image.argtypes = [vp, i32, vp, vp, vp]
     L.orc_get_loss.restype = ctypes.c_double
     L.orc_get_loss.argtypes = [vp, i32]
+    L.orc_get_loss_exact.restype = ctypes.c_double
+    L.orc_get_loss_exact.argtypes = [vp, i32]
     L.orc_get_dl.argtypes = [vp, i32, vp]
     L.orc_get_grads.argtypes = [vp, i32, vp]
+    L.orc_backward_with_dl_f32.argtypes = [vp, vp, sz, vp, vp, vp]
     L.orc_rasterize_rgba.restype = i32
     L.orc_rasterize_rgba.argtypes = [vp, sz, ctypes.POINTER(Camera), vp, vp]
+    L.orc_loss_grad_f32_exact.restype = ctypes.c_double
+    L.orc_loss_grad_f32_exact.argtypes = [vp, vp, sz, sz, ctypes.c_float, vp, ctypes.POINTER(ctypes.c_double)]
     L.orc_loss_grad_f32.restype = ctypes.c_double
     L.orc_loss_grad_f32.argtypes = [vp, vp, sz, sz, ctypes.c_float, vp]
     L.orc_loss_grad_f64.restype = ctypes.c_double
@@ -225,6 +230,12 @@ class Render:
     def loss(self):
         return lib().orc_get_loss(self.h_, self.f64)
 
+    @property
+    def loss_exact(self):
+        """The loss from the same per-pixel terms added in double (the
+        reference adds them serially in the working precision)."""
+        return lib().orc_get_loss_exact(self.h_, self.f64)
+
     def dl(self):
         d = np.zeros((self.h, self.w, 3), self.dtype)
         lib().orc_get_dl(self.h_, self.f64, _p(d))
@@ -233,6 +244,17 @@ class Render:
     def grads(self):
         gr = np.zeros((self.n, 60), self.dtype)
         lib().orc_get_grads(self.h_, self.f64, _p(gr))
+        return gr
+
+    def backward_with_dl(self, dl):
+        """The reference backward (backward_pass + geometry_backprop) of this
+        forward with the given dL/dpixel (h, w, 3) instead of its own loss's
+        (f32 handles only)."""
+        assert not self.f64
+        dl = np.ascontiguousarray(dl, np.float32).reshape(-1)
+        assert dl.size == self.w * self.h * 3
+        gr = np.zeros((self.n, 60), np.float32)
+        lib().orc_backward_with_dl_f32(self.h_, _p(self._g), self.n, _p(self._bg), _p(dl), _p(gr))
         return gr
 
 
@@ -265,6 +287,17 @@ def loss_grad(rgb, target_rgba, w, h, lam, f64=False):
     fn = lib().orc_loss_grad_f64 if f64 else lib().orc_loss_grad_f32
     loss = fn(_p(rgb), _p(tg), w, h, lam, _p(dl))
     return loss, dl
+
+
+def loss_grad_exact(rgb, target_rgba, w, h, lam):
+    """loss_grad in f32 -> (loss, the same per-pixel terms added in double,
+    dL/drgb)."""
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    tg = np.ascontiguousarray(target_rgba, np.float32)
+    dl = np.zeros(w * h * 3, np.float32)
+    ex = ctypes.c_double()
+    loss = lib().orc_loss_grad_f32_exact(_p(rgb), _p(tg), w, h, lam, _p(dl), ctypes.byref(ex))
+    return loss, ex.value, dl
 
 
 def view_matrix(cam) -> np.ndarray:

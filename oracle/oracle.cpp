@@ -795,8 +795,11 @@ template <class F> void blur(const F* src, size_t w, size_t h, const F* k, int k
     }
 }
 // backward.rs:697-756
+// exact_sum (optional): the same per-pixel SSIM terms added in double
+// (the reference adds them serially in f32).
 template <class F>
-F ssim_channel(const F* x, const F* y, size_t w, size_t h, const F* k, int klen, F* grad) {
+F ssim_channel(const F* x, const F* y, size_t w, size_t h, const F* k, int klen, F* grad,
+               double* exact_sum = nullptr) {
   const F C1 = F(0.01) * F(0.01);
   const F C2 = F(0.03) * F(0.03);
   size_t n = w * h;
@@ -825,6 +828,7 @@ F ssim_channel(const F* x, const F* y, size_t w, size_t h, const F* k, int klen,
     F num = a1 * a2;
     F den = b1 * b2;
     ssim_sum += num / den;
+    if (exact_sum) *exact_sum += (double)(num / den);
     F inv_den = F(1) / den;
     F pc = inv_den * (F(2) * my * a2) - (num / (den * den)) * (F(2) * mx * b2);
     F qc = inv_den * (F(2) * a1);
@@ -850,9 +854,12 @@ template <class F> inline F signum(F d) {
   if (d != d) return d;
   return std::signbit(d) ? F(-1) : F(1);
 }
-// backward.rs:646-693
+// backward.rs:646-693. exact_loss (optional): the loss from the same f32
+// per-pixel terms added in double, i.e. without the rounding of the
+// reference's serial f32 sums.
 template <class F>
-F photometric_loss_grad(const F* rendered, const F* target_rgba, size_t w, size_t h, F lambda, F* dl) {
+F photometric_loss_grad(const F* rendered, const F* target_rgba, size_t w, size_t h, F lambda, F* dl,
+                        double* exact_loss = nullptr) {
   size_t n = w * h;
   F m_total = (F)(n * 3);
   std::vector<F> x[3], y[3];
@@ -866,10 +873,12 @@ F photometric_loss_grad(const F* rendered, const F* target_rgba, size_t w, size_
       y[ch][p] = target_rgba[p * 4 + ch];
     }
   F l1 = F(0);
+  double l1_exact = 0.0, ssim_exact = 0.0;
   for (size_t p = 0; p < n; ++p)
     for (int ch = 0; ch < 3; ++ch) {
       F d = rendered[p * 3 + ch] - target_rgba[p * 4 + ch];
       l1 += std::fabs(d);
+      l1_exact += (double)std::fabs(d);
       dl[p * 3 + ch] = (F(1) - lambda) * signum(d) / m_total;
     }
   l1 = l1 / m_total;
@@ -878,11 +887,15 @@ F photometric_loss_grad(const F* rendered, const F* target_rgba, size_t w, size_
   F ssim_sum = F(0);
   std::vector<F> grad(n);
   for (int ch = 0; ch < 3; ++ch) {
-    F s = ssim_channel(x[ch].data(), y[ch].data(), w, h, k, 11, grad.data());
+    F s = ssim_channel(x[ch].data(), y[ch].data(), w, h, k, 11, grad.data(), &ssim_exact);
     ssim_sum += s;
     for (size_t p = 0; p < n; ++p) dl[p * 3 + ch] += -lambda * grad[p] / m_total;
   }
   F ssim_mean = ssim_sum / m_total;
+  if (exact_loss) {
+    const double m = (double)n * 3.0, lam = (double)lambda;
+    *exact_loss = (1.0 - lam) * (l1_exact / m) + lam * (1.0 - ssim_exact / m);
+  }
   return (F(1) - lambda) * l1 + lambda * (F(1) - ssim_mean);
 }
 
@@ -894,6 +907,7 @@ template <class F> struct Run {
   std::vector<int32_t> last;
   std::vector<F> grads;
   F loss = F(0);
+  double loss_exact = 0.0;  // the same per-pixel terms added in double
   uint64_t pairs_fwd = 0;
 };
 
@@ -913,7 +927,7 @@ Run<F>* run(const F* g, size_t n, const Camera* cam, const F* target, const F bg
                &r->pairs_fwd);
   if (!want_backward || !target) return r;
   r->dl.assign(w * h * 3, F(0));
-  r->loss = photometric_loss_grad(r->rgb.data(), target, w, h, lambda, r->dl.data());
+  r->loss = photometric_loss_grad(r->rgb.data(), target, w, h, lambda, r->dl.data(), &r->loss_exact);
   auto acc = backward_pass(r->b, cam->width, cam->height, bg, r->final_t.data(), r->last.data(), r->dl.data());
   r->grads.assign(n * D_N, F(0));
   for (size_t si = 0; si < r->b.splats.size(); ++si) {
@@ -1008,6 +1022,20 @@ void* orc_run_f64(const double* g, size_t n, const Camera* cam, const double* ta
                   double lambda, int want_backward, int* err) {
   return run<double>(g, n, cam, target, bg, lambda, want_backward, err);
 }
+// backward_pass + geometry_backprop (backward.rs:131-146) of a forward
+// handle with a given dL/dpixel (w*h*3) instead of the photometric loss's:
+// the reference's backward of another loss gradient, e.g. the one the GPU's
+// own image implies. g, n and bg are the handle's inputs; grads is n x 60.
+void orc_backward_with_dl_f32(void* h, const float* g, size_t n, const float* bg, const float* dl, float* grads) {
+  auto* r = (Run<float>*)h;
+  auto acc = backward_pass(r->b, r->v.w, r->v.h, bg, r->final_t.data(), r->last.data(), dl);
+  std::fill(grads, grads + n * D_N, 0.0f);
+  for (size_t si = 0; si < r->b.splats.size(); ++si) {
+    const auto& s = r->b.splats[si];
+    geometry_backprop(s, acc[si], g + s.gi * G_N, r->v, grads + s.gi * D_N);
+  }
+}
+
 void orc_free(void* h, int is_f64) {
   if (is_f64) delete (Run<double>*)h;
   else delete (Run<float>*)h;
@@ -1080,6 +1108,11 @@ void orc_get_image(void* h, int f64, void* rgb, void* final_t, int32_t* last) {
     std::memcpy(last, r->last.data(), r->last.size() * sizeof(int32_t));
   });
 }
+double orc_get_loss_exact(void* h, int f64) {
+  double v = 0.0;
+  ORC_DISPATCH(h, f64, v = r->loss_exact);
+  return v;
+}
 double orc_get_loss(void* h, int f64) {
   double v = 0;
   ORC_DISPATCH(h, f64, v = (double)r->loss);
@@ -1103,6 +1136,10 @@ int orc_rasterize_rgba(const float* g, size_t n, const Camera* cam, const float*
 // ---- backward.rs:646-693
 double orc_loss_grad_f32(const float* rgb, const float* target, size_t w, size_t h, float lambda, float* dl) {
   return photometric_loss_grad<float>(rgb, target, w, h, lambda, dl);
+}
+double orc_loss_grad_f32_exact(const float* rgb, const float* target, size_t w, size_t h, float lambda, float* dl,
+                               double* exact) {
+  return photometric_loss_grad<float>(rgb, target, w, h, lambda, dl, exact);
 }
 double orc_loss_grad_f64(const double* rgb, const double* target, size_t w, size_t h, double lambda, double* dl) {
   return photometric_loss_grad<double>(rgb, target, w, h, lambda, dl);

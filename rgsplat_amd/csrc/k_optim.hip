@@ -500,7 +500,8 @@ __device__ __forceinline__ void train_adam(float* __restrict__ P, float* __restr
                                            const uint32_t* __restrict__ cnt, float* __restrict__ accum,
                                            size_t acc_stride, const float g[11], float sgn,
                                            const uint64_t clamp_in[3], float* __restrict__ gaccum,
-                                           uint32_t* __restrict__ gdenom, const AdamConsts& k_) {
+                                           uint32_t* __restrict__ gdenom, const AdamConsts& k_,
+                                           float* __restrict__ gout) {
   SoaLoad L{P, (size_t)cap, (size_t)i};
   uint64_t vis = 0;
   for (int v = 0; v < n_views; ++v)
@@ -521,6 +522,7 @@ __device__ __forceinline__ void train_adam(float* __restrict__ P, float* __restr
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       size_t o = (size_t)(P_SH + ch * 16 + k) * cap + i;
+      if (gout) gout[o] = gs[k];
       float mm = M1[o], vv = M2[o];
       P[o] = adam_fast(P[o], gs[k], mm, vv, lr_of(P_SH + ch * 16 + k, k_), k_);
       M1[o] = mm;
@@ -538,6 +540,7 @@ __device__ __forceinline__ void train_adam(float* __restrict__ P, float* __restr
 #pragma unroll
   for (int j = 0; j < 11; ++j) {
     size_t o = (size_t)j * cap + i;
+    if (gout) gout[o] = g[j];
     float mm = M1[o], vv = M2[o];
     float p = adam_fast(P[o], g[j], mm, vv, lr_of(j, k_), k_);
     M1[o] = mm;
@@ -549,6 +552,7 @@ __device__ __forceinline__ void train_adam(float* __restrict__ P, float* __restr
   }
   renorm_quat(rot[0], rot[1], rot[2], rot[3]);
   for (int q = 0; q < 4; ++q) P[(size_t)(P_ROT + q) * cap + i] = rot[q];
+  if (gout) gout[(size_t)59 * cap + i] = sgn;
   gaccum[i] += sgn;
   gdenom[i] += (uint32_t)n_views;
 }
@@ -562,7 +566,8 @@ __global__ __launch_bounds__(256) void k_train_update(float* __restrict__ P, flo
                                                       size_t acc_stride, float* __restrict__ gaccum,
                                                       uint32_t* __restrict__ gdenom,
                                                       const AdamConsts* __restrict__ k_dev,
-                                                      const uint32_t* __restrict__ status) {
+                                                      const uint32_t* __restrict__ status,
+                                                      float* __restrict__ gout) {
   __shared__ float s_dd[4][3][256];
   const int t = threadIdx.x;
   int i = blockIdx.x * blockDim.x + t;
@@ -571,15 +576,16 @@ __global__ __launch_bounds__(256) void k_train_update(float* __restrict__ P, flo
   float g[11], sgn;
   uint64_t clamp_in[3];
   train_geometry(P, cap, i, cams, n_views, cnt, accum, acc_stride, s_dd, t, g, sgn, clamp_in);
-  train_adam(P, M1, M2, cap, i, cams, n_views, cnt, accum, acc_stride, g, sgn, clamp_in, gaccum, gdenom, *k_dev);
+  train_adam(P, M1, M2, cap, i, cams, n_views, cnt, accum, acc_stride, g, sgn, clamp_in, gaccum, gdenom, *k_dev,
+             gout);
 }
 
 void launch_train_adam(float* P, float* M1, float* M2, int cap, int n, const CamDev* cams, int n_views,
                        const uint32_t* cnt, float* accum, size_t acc_stride, float* gaccum, uint32_t* gdenom,
-                       const AdamConsts* k_dev, const uint32_t* status, hipStream_t st) {
+                       const AdamConsts* k_dev, const uint32_t* status, hipStream_t st, float* grad_out) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_train_update, dim3((n + 255) / 256), dim3(256), 0, st, P, M1, M2, cap, n, cams, n_views, cnt,
-                     accum, acc_stride, gaccum, gdenom, k_dev, status);
+                     accum, acc_stride, gaccum, gdenom, k_dev, status, grad_out);
 }
 
 // ln on the device: the f64 logarithm rounded once to f32 (correctly rounded

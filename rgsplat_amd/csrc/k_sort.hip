@@ -361,14 +361,25 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(F in, size_t n, T*
   if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
 }
 
+// The total of a 32-bit scan saturates: a true total of 2^32 - 1 or more is
+// written as SCAN_SATURATED (the offsets have wrapped; callers treat the scan
+// as failed), so a count past the 32-bit range is never taken for a small one.
 template <class T>
 __global__ __launch_bounds__(1024) void k_scan_top(T* block_sums, int nb, T* total) {
   __shared__ T s_tmp[16];
+  __shared__ unsigned long long s_wide;
   int per = (nb + 1023) / 1024;
   int beg = threadIdx.x * per;
   int end = min(beg + per, nb);
   T sum = 0;
-  for (int j = beg; j < end; ++j) sum += block_sums[j];
+  unsigned long long wide = 0;
+  for (int j = beg; j < end; ++j) {
+    sum += block_sums[j];
+    wide += (unsigned long long)block_sums[j];
+  }
+  if (threadIdx.x == 0) s_wide = 0;
+  __syncthreads();
+  if (sizeof(T) == 4) atomicAdd(&s_wide, wide);
   T tot;
   T run = block_excl_scan_generic<T>(sum, s_tmp, &tot);
   for (int j = beg; j < end; ++j) {
@@ -376,7 +387,10 @@ __global__ __launch_bounds__(1024) void k_scan_top(T* block_sums, int nb, T* tot
     block_sums[j] = run;
     run += c;
   }
-  if (threadIdx.x == 0) *total = tot;
+  if (threadIdx.x == 0) {
+    if (sizeof(T) == 4 && s_wide >= (unsigned long long)SCAN_SATURATED) tot = (T)SCAN_SATURATED;
+    *total = tot;
+  }
 }
 
 template <class T, class F>

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <string>
 #include <vector>
 
@@ -73,6 +74,18 @@ int ensure(rgs_ctx* c, DBuf& b, size_t bytes) {
 template <class T>
 T* P_(DBuf& b) {
   return reinterpret_cast<T*>(b.p);
+}
+
+// Largest (tile, splat) pair count a render takes. Pair offsets, sort
+// positions and tile ranges are 32-bit; a 32-bit scan total saturates at
+// SCAN_SATURATED, so a larger count reads as at least this and is refused.
+constexpr uint64_t PAIR_LIMIT = 0x7fffffffull;
+int check_pairs(uint64_t k) {
+  if (k < PAIR_LIMIT) return 0;
+  return fail(RGS_E_CAPACITY, "the render needs " +
+                                  (k >= rgs::SCAN_SATURATED ? std::string("2^32 or more")
+                                                            : std::to_string((unsigned long long)k)) +
+                                  " (tile, splat) pairs; at most 2^31 - 2 are supported");
 }
 
 // Matrix4::try_inverse of the row-major c2w (math/mod.rs:155-160), evaluated
@@ -281,6 +294,8 @@ struct rgs_ctx {
   int fwd_tile_blocks = 1; // training forward with bins: a workgroup per tile (1) or per bin (0)
   int super_tile = 8;      // tiles per side of a schedule super-tile
   int sparse_lanes = 2;    // backward: LDS atomics up to this many lanes
+  bool grad_capture = false;  // the step writes its summed gradients into gcap
+  DBuf gcap;                  // 60 rows x cap: gradient rows 0-58, screen-gradient norm
 
   // stage timing
   bool profiling = false;
@@ -614,6 +629,7 @@ int render_eager(rgs_ctx* c, const float* P, size_t cap, int n, ViewSet& vs, con
   HIPCHK(hipStreamSynchronize(c->st));
   if (c->h_scalars[SC_STATUS] & ST_NAN_DEPTH) return fail(RGS_E_NAN, "NaN depth in projection");
   size_t K = c->h_scalars[SC_TOTAL];
+  CHK(check_pairs(K));
   if (K > c->k_cap) {
     size_t want = K + K / 4 + 1024;
     CHK(ensure_render(c, cap, n, vs, want));
@@ -1886,7 +1902,8 @@ static void enqueue_step_rest(rgs_ctx* c, const ViewSet& vs, const float bg[3], 
   stage_begin(c, S_ADAM);
   launch_train_adam(P_<float>(c->P), P_<float>(c->M1), P_<float>(c->M2), (int)cap, n, P_<CamDev>(c->cams_d),
                     (int)vs.cams.size(), P_<uint32_t>(c->cnt), P_<float>(c->accum), vs.cams.size() * cap,
-                    P_<float>(c->gacc), P_<uint32_t>(c->gden), P_<AdamConsts>(c->adam_d), sc + SC_STATUS, c->st);
+                    P_<float>(c->gacc), P_<uint32_t>(c->gden), P_<AdamConsts>(c->adam_d), sc + SC_STATUS, c->st,
+                    c->grad_capture ? P_<float>(c->gcap) : nullptr);
   stage_end(c, S_ADAM);
 }
 
@@ -1897,9 +1914,29 @@ static void enqueue_step(rgs_ctx* c, const ViewSet& vs, const float bg[3], float
   enqueue_step_rest(c, vs, bg, lambda, with_back);
 }
 
+static int train_step_impl(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* cfg, const float bg[3],
+                           float lambda, int use_graph, float* loss_out);
+
+// A failed step leaves the parameters untouched (the device skips the update
+// on every status the host turns into an error), so the Adam step count goes
+// back too: the next step's bias corrections are those it would have had.
 int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* cfg, const float bg[3],
                    float lambda, int use_graph, float* loss_out) {
-  if (!c || !ids || n_ids <= 0 || !cfg || !bg) return fail(RGS_E_ARG, "null argument");
+  if (!c) return fail(RGS_E_ARG, "null ctx");
+  const uint64_t t0 = c->t;
+  int rc;
+  try {
+    rc = train_step_impl(c, ids, n_ids, cfg, bg, lambda, use_graph, loss_out);
+  } catch (const std::exception& e) {
+    rc = fail(RGS_E_HIP, std::string("host failure in the training step: ") + e.what());
+  }
+  if (rc) c->t = t0;
+  return rc;
+}
+
+static int train_step_impl(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* cfg, const float bg[3],
+                           float lambda, int use_graph, float* loss_out) {
+  if (!ids || n_ids <= 0 || !cfg || !bg) return fail(RGS_E_ARG, "null argument");
   c->fwd_valid = false;
   if (n_ids > 64) return fail(RGS_E_ARG, "at most 64 views per step");
   if (!c->P.p) return fail(RGS_E_ARG, "no scene uploaded");
@@ -1937,6 +1974,10 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
   if (c->profiling && c->prof_graph_failed) use_graph = 0;
   if (!c->st) use_graph = 0;  // no capture on the null stream
   if (c->profiling) key += "|prof" + std::to_string(c->prof_mask);
+  if (c->grad_capture) {
+    CHK(ensure(c, c->gcap, (size_t)60 * cap * 4));
+    key += "|gcap";
+  }
   for (int attempt = 0; attempt < 4; ++attempt) {
     const int var = c->need_back ? 1 : 0;
     bool replay = use_graph && c->gexec[var] && c->gkey[var] == key && c->graph_gen[var] == c->gen;
@@ -1955,11 +1996,9 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
       enqueue_project(c, P_<float>(c->P), cap, n, vs, true, step_cover(c, bg));
       HIPCHK(hipMemcpyAsync(c->h_scalars, sc, SC_COUNT * 4, hipMemcpyDeviceToHost, c->st));
       HIPCHK(hipStreamSynchronize(c->st));
-      if (c->h_scalars[SC_STATUS] & ST_NAN_DEPTH) {
-        c->t -= 1;
-        return fail(RGS_E_NAN, "NaN depth in projection");
-      }
+      if (c->h_scalars[SC_STATUS] & ST_NAN_DEPTH) return fail(RGS_E_NAN, "NaN depth in projection");
       size_t K = c->h_scalars[SC_TOTAL];
+      CHK(check_pairs(K));
       size_t want = K + K / 4 + 4096;  // headroom for the following steps
       if (K > c->k_cap) CHK(ensure_render(c, cap, n, vs, std::max(want, c->k_cap)));
       if (use_graph) {
@@ -1999,13 +2038,11 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
     HIPCHK(hipMemcpyAsync(c->h_loss, c->loss.p, vs.cams.size() * 4, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     uint32_t status = c->h_scalars[SC_STATUS];
-    if (status & ST_NAN_DEPTH) {
-      c->t -= 1;
-      return fail(RGS_E_NAN, "NaN depth in projection");
-    }
+    if (status & ST_NAN_DEPTH) return fail(RGS_E_NAN, "NaN depth in projection");
     if (status & ST_PAIR_OVERFLOW) {
       // the step was skipped on the device (parameters untouched): grow and redo
-      size_t K = c->h_scalars[SC_TOTAL];
+      size_t K = std::max<size_t>(c->h_scalars[SC_TOTAL], c->h_scalars[SC_BTOTAL]);
+      CHK(check_pairs(K));
       drop_graph(c);
       CHK(ensure_render(c, cap, n, vs, K + K / 2 + 4096));
       launch_fill_f32(P_<float>(c->accum), acc_n, 0.0f, c->st);
@@ -2082,6 +2119,7 @@ int rgs_autotune_raster(rgs_ctx* c, const int* ids, int n_ids, const float bg[3]
   HIPCHK(hipMemcpyAsync(c->h_scalars, sc, SC_COUNT * 4, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   size_t K = c->h_scalars[SC_TOTAL];
+  CHK(check_pairs(K));
   if (K > c->k_cap) CHK(ensure_render(c, cap, n, vs, K + K / 4 + 4096));
   enqueue_bin_and_raster(c, cap, n, vs, bg, 0, P_<float>(c->rgb));
   enqueue_loss(c, vs, P_<float>(c->rgb), P_<float>(c->targets), lambda);
@@ -2302,6 +2340,27 @@ int rgs_eval_expf(rgs_ctx* c, const float* x, float* out, size_t n) {
   launch_eval_expf(P_<float>(c->api_aos), P_<float>(c->api_aos) + n, n, c->st);
   HIPCHK(hipMemcpyAsync(out, P_<float>(c->api_aos) + n, n * 4, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int rgs_set_grad_capture(rgs_ctx* c, int enable) {
+  if (!c) return fail(RGS_E_ARG, "null ctx");
+  c->grad_capture = enable != 0;
+  return 0;
+}
+
+int rgs_step_grads(rgs_ctx* c, rgs_grad* out, size_t n) {
+  if (!c || !out) return fail(RGS_E_ARG, "null argument");
+  if (!c->grad_capture || !c->gcap.p) return fail(RGS_E_ARG, "gradient capture is off or no step has run");
+  if (n != c->n) return fail(RGS_E_ARG, "n must be the scene size");
+  if (c->gcap.bytes < (size_t)60 * c->cap * 4) return fail(RGS_E_ARG, "no step ran since the scene grew");
+  (void)hipSetDevice(c->device);
+  HIPCHK(hipStreamSynchronize(c->st));
+  std::vector<float> rows((size_t)60 * c->cap);
+  HIPCHK(hipMemcpy(rows.data(), c->gcap.p, rows.size() * 4, hipMemcpyDeviceToHost));
+  float* o = reinterpret_cast<float*>(out);
+  for (size_t i = 0; i < n; ++i)
+    for (size_t j = 0; j < 60; ++j) o[i * 60 + j] = rows[j * c->cap + i];
   return 0;
 }
 

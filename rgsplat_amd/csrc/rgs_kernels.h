@@ -85,6 +85,8 @@ int radix_sort_pairs(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev
                      RadixScratch rs, hipStream_t st);
 
 constexpr int SCAN_TILE = 2048;
+// Total written by a 32-bit scan whose true total is 2^32 - 1 or more.
+constexpr uint32_t SCAN_SATURATED = 0xffffffffu;
 inline int scan_nb(size_t n) { return (int)((n + SCAN_TILE - 1) / SCAN_TILE); }
 // Exclusive scan of cnt[vals[r]] over r < n (pair offsets in depth order).
 // block_sums needs scan_nb(n) + 1 entries; total written to *total.
@@ -254,10 +256,12 @@ void launch_adam_aos(float* g, const float* grads, size_t n, float* m1, float* m
                      const AdamConsts& k, hipStream_t st);
 // Per-Gaussian geometry backprop over all views + Adam on SoA buffers (one
 // kernel, one thread per Gaussian). Skips everything when *status has
-// ST_PAIR_OVERFLOW, ST_NAN_DEPTH or ST_NEED_BACK. Clears accum.
+// ST_PAIR_OVERFLOW, ST_NAN_DEPTH or ST_NEED_BACK. Clears accum. grad_out
+// (optional, 60 rows of cap floats): the step's gradient of each of the 59
+// parameter rows, summed over the views, and the screen-gradient norm sum.
 void launch_train_adam(float* P, float* M1, float* M2, int cap, int n, const CamDev* cams, int n_views,
                        const uint32_t* cnt, float* accum, size_t acc_stride, float* gaccum, uint32_t* gdenom,
-                       const AdamConsts* k_dev, const uint32_t* status, hipStream_t st);
+                       const AdamConsts* k_dev, const uint32_t* status, hipStream_t st, float* grad_out = nullptr);
 // Model files (k_io.hip). A PLY vertex property: byte offset in the record,
 // byte width (1, 2, 4 or 8) and the SoA row it sets (-1: ignored).
 struct PlyField {

@@ -110,6 +110,8 @@ SIGNATURES = {
     "rgs_set_profiling_stages": [_vp, ctypes.c_uint32],
     "rgs_stage_times": [_vp, _vp, _i, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)],
     "rgs_debug_tiles": [_vp, _vp, _sz, _vp, _sz, ctypes.POINTER(ctypes.c_uint64)],
+    "rgs_set_grad_capture": [_vp, _i],
+    "rgs_step_grads": [_vp, _vp, _sz],
 }
 
 
@@ -541,6 +543,18 @@ class Context:
         out = np.zeros_like(x)
         _check(lib().rgs_eval_expf(self.h, _p(x), _p(out), len(x)))
         return out
+
+    def set_grad_capture(self, enable: bool):
+        """train_step also keeps the gradients its Adam update consumed."""
+        _check(lib().rgs_set_grad_capture(self.h, int(bool(enable))))
+
+    def step_grads(self) -> np.ndarray:
+        """The last train_step's gradients, (n, 60) GaussianGrad records
+        summed over its views (needs set_grad_capture(True) before the step)."""
+        n = self.scene_size()
+        out = np.zeros((max(n, 1), 60), np.float32)
+        _check(lib().rgs_step_grads(self.h, _p(out), n))
+        return out[:n]
 
     def debug_tiles(self, n_tiles: int, k_max: int):
         off = np.zeros(n_tiles + 1, np.uint64)

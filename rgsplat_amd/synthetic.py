@@ -186,6 +186,41 @@ def scaled(w: Workload, n: int | None = None, sizes=None) -> Workload:
     return out
 
 
+def crop_camera(cam: Camera, x0: int, y0: int, width: int, height: int) -> Camera:
+    """The window [x0, x0 + width) x [y0, y0 + height) of a camera's image as
+    a camera of its own: same pose and focal lengths, principal point moved
+    by the window's origin."""
+    return Camera(int(width), int(height), cam.fx, cam.fy, cam.cx - x0, cam.cy - y0, cam.c2w.copy())
+
+
+def cluster_crops(w: Workload, size: int = 128):
+    """For a clustered workload, one size x size window per view, centred on
+    the projection of the cluster centre nearest to that view's camera among
+    those projecting at least size / 2 inside its image: the densest part of
+    the view (every tile there lists tens of thousands of Gaussians).
+    Returns the cropped cameras."""
+    assert w.clusters > 0
+    centres = Stream(1000 + w.index).normal((w.clusters, 3))  # make_scene's first draw
+    out = []
+    for cam in w.cameras():
+        c2w = cam.c2w.astype(np.float64)
+        rot, t = c2w[:3, :3], c2w[:3, 3]
+        pc = (centres - t) @ rot  # world -> camera: R^T (x - t)
+        z = pc[:, 2]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            sx = cam.fx * pc[:, 0] / z + cam.cx
+            sy = cam.fy * pc[:, 1] / z + cam.cy
+        m = size / 2
+        ok = (z > 0.5) & (sx > m) & (sx < cam.width - m) & (sy > m) & (sy < cam.height - m)
+        if not ok.any():
+            raise ValueError("no cluster centre projects inside the view")
+        k = np.where(ok)[0][np.argmin(z[ok])]
+        x0 = int(round(sx[k])) - size // 2
+        y0 = int(round(sy[k])) - size // 2
+        out.append(crop_camera(cam, x0, y0, size, size))
+    return out
+
+
 def train_config(**over) -> dict:
     """TrainConfig defaults (pipeline/optimizer.rs:42-63)."""
     cfg = dict(lr_position=0.00016, lr_rotation=0.001, lr_scale=0.005, lr_opacity=0.05, lr_sh_dc=0.0025,

@@ -295,6 +295,49 @@ def test_parameter_gradients_fd_f64_all_params(orc):
         np.testing.assert_allclose(tang(grads[i, 3:7]), tang(nr), atol=1e-6 * max(np.abs(nr).max(), 1e-3))
 
 
+def test_loss_exact_sum(orc):
+    """The loss with the reference's per-pixel f32 terms added in double.
+    With lambda = 0 it is the mean of |x - y| of the f32 differences, which
+    numpy sums in double as well; with D-SSIM it stays within f32 rounding of
+    the all-f64 loss, while the reference's serial f32 sum of 3 w h terms
+    drifts at 600x600 (its rounding, not the terms')."""
+    r = S.Stream(31)
+    w, h = 600, 600
+    x = (r.normal((h, w, 3)) * 0.2 + 0.5).astype(np.float32)
+    y = np.concatenate([(r.normal((h, w, 3)) * 0.2 + 0.5), np.ones((h, w, 1))], -1).astype(np.float32)
+    loss0, exact0, _ = orc.loss_grad_exact(x, y, w, h, 0.0)
+    want = np.mean(np.abs(x - y[..., :3]), dtype=np.float64)
+    assert abs(exact0 - want) <= 1e-13 * want
+    loss, exact, dl = orc.loss_grad_exact(x, y, w, h, 0.2)
+    l64, _ = orc.loss_grad(x, y, w, h, 0.2, f64=True)
+    assert abs(exact - l64) <= 1e-5 * l64
+    assert abs(loss - exact) > 1e-5 * exact  # the serial f32 sum has drifted (2.3e-5 here)
+    _, dl32 = orc.loss_grad(x, y, w, h, 0.2)
+    assert np.array_equal(dl, dl32)
+
+
+def test_backward_with_given_loss_gradient(orc):
+    """Render.backward_with_dl drives the same backward_pass and
+    geometry_backprop (backward.rs:131-146) with a given dL/dpixel: with the
+    handle's own loss gradient it gives its gradients bit for bit, the
+    backward is linear in dL/dpixel (2 dl gives exactly 2 g: scaling by a
+    power of two commutes with every rounding), and a dL/dpixel that is zero
+    except on one pixel reaches only the Gaussians that pixel's blend used."""
+    gs, cam, tgt = _fd_scene()
+    bg = (0.1, 0.05, 0.2)
+    g32 = gs.astype(np.float32)
+    r = orc.Render(g32, cam, tgt.astype(np.float32), bg=bg, lam=0.2)
+    dl = r.dl()
+    assert np.array_equal(r.backward_with_dl(dl), r.grads())
+    assert np.array_equal(r.backward_with_dl(2 * dl), 2 * r.grads())
+    one = np.zeros_like(dl)
+    one[16, 20, 1] = 1.0
+    g1 = r.backward_with_dl(one)
+    assert np.all(g1[:, 11:27] == 0) and np.all(g1[:, 43:59] == 0)  # red and blue SH untouched
+    assert np.any(g1[:, 27:43] != 0)
+    assert np.all(r.backward_with_dl(np.zeros_like(dl)) == 0)
+
+
 def test_screen_grad_norm_is_pixel_mean_gradient(orc):
     """screen_grad_norm = |dL/d(sx, sy)|. On the optical axis with an axis-
     aligned covariance and DC-only colour, moving the centre by (dx, dy) in
