@@ -115,6 +115,22 @@ int rgs_rasterize_rgba(rgs_ctx* ctx, const rgs_gaussian* g, size_t n, const rgs_
 int rgs_loss_grad(rgs_ctx* ctx, const float* rgb, const float* target_rgba, uint32_t w, uint32_t h,
                   float lambda_dssim, float* loss_out, float* dl_drgb_out);
 
+/* photometric_loss_grad on device buffers, allocating nothing. No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). rgb_dev (w*h*3), target_rgba_dev (w*h*4), dl_drgb_dev
+ * (w*h*3) and loss_dev (1 float) are device memory on the context's device;
+ * the work is enqueued on the context's stream (rgs_set_stream) and complete
+ * in stream order, without a host synchronisation. dL/drgb is bit-identical
+ * to rgs_loss_grad's; the loss agrees to float rounding (another summation
+ * order). The kernels' partial sums live in a static array of 1.5 MiB per
+ * GPU: images of at most 65536 tiles of 32x32 pixels (e.g. 8192 x 8192);
+ * larger ones return RGS_E_CAPACITY. Every call on one GPU shares that
+ * array, so calls that overlap in time on one GPU (other streams, other
+ * contexts, other threads) race; calls in one stream order do not. */
+int rgs_loss_grad_dev_noalloc(rgs_ctx* ctx, const float* rgb_dev, const float* target_rgba_dev, uint32_t w,
+                              uint32_t h, float lambda_dssim, float* loss_dev, float* dl_drgb_dev);
+
 /* adam_step (pipeline/optimizer.rs:116-163): updates g and st in place. */
 int rgs_adam_step(rgs_ctx* ctx, rgs_gaussian* g, const rgs_grad* grads, size_t n, rgs_adam* st,
                   const rgs_train_cfg* cfg);

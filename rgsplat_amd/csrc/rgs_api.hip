@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <exception>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -294,6 +295,7 @@ struct rgs_ctx {
   int fwd_tile_blocks = 1; // training forward with bins: a workgroup per tile (1) or per bin (0)
   int super_tile = 8;      // tiles per side of a schedule super-tile
   int sparse_lanes = 2;    // backward: LDS atomics up to this many lanes
+  float* scratch_dev = nullptr;  // this device's copy of the no-alloc scratch array
   bool grad_capture = false;  // the step writes its summed gradients into gcap
   DBuf gcap;                  // 60 rows x cap: gradient rows 0-58, screen-gradient norm
 
@@ -706,6 +708,34 @@ AdamConsts adam_consts(const rgs_train_cfg& cfg, uint64_t t) {
   return k;
 }
 
+// Scratch of rgs_loss_grad_dev_noalloc: the per-tile partial sums of the
+// fused loss kernel, for at most NOALLOC_MAX_TILES 32x32 tiles (an image of
+// up to 8192 x 8192 pixels), 1.5 MiB per GPU for the life of the process.
+// Host code never uses the array's name as an address: each device's copy is
+// resolved once with hipGetSymbolAddress (scratch_on) and that pointer is
+// passed to the kernels.
+constexpr size_t NOALLOC_MAX_TILES = 65536;
+static __device__ __align__(256) float scratch[NOALLOC_MAX_TILES * 6];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+float* g_scratch[MAX_DEVICES] = {};
+
+// The address of the scratch array on `device`, which must be current.
+int scratch_on(int device, float** out) {
+  if (device < 0 || device >= MAX_DEVICES) return fail(RGS_E_ARG, "device id beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(RGS_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[device] = static_cast<float*>(ptr);
+  }
+  *out = g_scratch[device];
+  return 0;
+}
+
 }  // namespace
 
 // ============================================================== C ABI
@@ -723,8 +753,11 @@ int rgs_create(int device, rgs_ctx** out) {
     HIPCHK(hipGetDevice(&device));
   }
   HIPCHK(hipSetDevice(device));
+  float* scratch_dev = nullptr;
+  CHK(scratch_on(device, &scratch_dev));
   auto* c = new rgs_ctx();
   c->device = device;
+  c->scratch_dev = scratch_dev;
   if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return fail(RGS_E_HIP, "hipStreamCreate failed");
@@ -909,6 +942,30 @@ int rgs_loss_grad(rgs_ctx* c, const float* rgb, const float* target_rgba, uint32
   HIPCHK(hipMemcpyAsync(c->h_loss, c->loss.p, 4, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   if (loss_out) *loss_out = c->h_loss[0];
+  return 0;
+}
+
+int rgs_loss_grad_dev_noalloc(rgs_ctx* c, const float* rgb_dev, const float* target_rgba_dev, uint32_t w,
+                              uint32_t h, float lambda, float* loss_dev, float* dl_drgb_dev) {
+  if (!c || !rgb_dev || !target_rgba_dev || !loss_dev || !dl_drgb_dev) return fail(RGS_E_ARG, "null argument");
+  if (!w || !h) return fail(RGS_E_ARG, "empty image");
+  const size_t nlx = (w + LTILE - 1) / LTILE, nly = (h + LTILE - 1) / LTILE;
+  if (nlx * nly > NOALLOC_MAX_TILES)
+    return fail(RGS_E_CAPACITY, "image beyond the no-alloc scratch (at most 65536 tiles of 32x32 pixels)");
+  if (!c->scratch_dev) return fail(RGS_E_HIP, "no scratch on this context's device");
+  (void)hipSetDevice(c->device);
+  LossIn li{};
+  li.rgb = rgb_dev;
+  li.target = target_rgba_dev;
+  li.cams = nullptr;
+  li.n_views = 1;
+  li.total_ltiles = (int)(nlx * nly);
+  li.total_pix = (size_t)w * h;
+  li.lambda = lambda;
+  li.cam0.w = (int32_t)w;
+  li.cam0.h = (int32_t)h;
+  launch_loss_fused(li, dl_drgb_dev, c->scratch_dev, loss_dev, c->st);
+  HIPCHK(hipGetLastError());
   return 0;
 }
 

@@ -227,16 +227,20 @@ constexpr int LTILE = 32;  // loss tile side
 struct LossIn {
   const float* rgb;     // total_pix * 3
   const float* target;  // target store, RGBA; view v starts at pixel cams[v].tgt_base
-  const CamDev* cams;
+  const CamDev* cams;   // null: one view, described by cam0
   int n_views;
   int total_ltiles;     // 32x32 loss tiles over all views
   size_t total_pix;
   float lambda;
+  CamDev cam0;          // the view when cams is null (w, h, pix_base, tgt_base, ltile_base)
 };
-// ssim_maps: SSIM_MAPS * 3 * total_pix floats; partial: 2 * 3 * total_ltiles
-// floats; loss_out: n_views floats.
+// dl: 3 * total_pix floats; ssim_maps: SSIM_MAPS * 3 * total_pix floats and
+// partial: 2 * 3 * total_ltiles floats (scratch); loss_out: n_views floats.
 constexpr int SSIM_MAPS = 3;
 void launch_loss(const LossIn& in, float* dl, float* ssim_maps, float* partial, float* loss_out, hipStream_t st);
+// The same dL/dpixel in one pass per tile, without the map buffer (the maps
+// stay in LDS); the loss agrees with launch_loss's to float rounding.
+void launch_loss_fused(const LossIn& in, float* dl, float* partial, float* loss_out, hipStream_t st);
 
 // ---- optimizer and parameter kernels (k_optim.hip)
 struct AdamConsts {

@@ -111,6 +111,7 @@ SIGNATURES = {
     "rgs_stage_times": [_vp, _vp, _i, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)],
     "rgs_debug_tiles": [_vp, _vp, _sz, _vp, _sz, ctypes.POINTER(ctypes.c_uint64)],
     "rgs_set_grad_capture": [_vp, _i],
+    "rgs_loss_grad_dev_noalloc": [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32, _f, _vp, _vp],
     "rgs_step_grads": [_vp, _vp, _sz],
 }
 
@@ -286,6 +287,22 @@ class Context:
         loss = ctypes.c_float()
         _check(lib().rgs_loss_grad(self.h, _p(r), _p(t), w, h, float(lambda_dssim), ctypes.byref(loss), _p(dl)))
         return float(loss.value), dl
+
+    def set_stream(self, hip_stream: int | None):
+        """Run later work on an external HIP stream handle (e.g. a torch
+        stream's cuda_stream); None returns to the context's own stream."""
+        if hip_stream is None:
+            _check(lib().rgs_use_own_stream(self.h))
+        else:
+            _check(lib().rgs_set_stream(self.h, ctypes.c_void_p(int(hip_stream))))
+
+    def loss_grad_dev_noalloc(self, rgb_dev_ptr: int, target_dev_ptr: int, w: int, h: int, lambda_dssim: float,
+                              loss_dev_ptr: int, dl_dev_ptr: int):
+        """photometric_loss_grad on device pointers, allocating nothing
+        (rgs_loss_grad_dev_noalloc); enqueued on the context's stream."""
+        _check(lib().rgs_loss_grad_dev_noalloc(self.h, ctypes.c_void_p(rgb_dev_ptr), ctypes.c_void_p(target_dev_ptr),
+                                               int(w), int(h), float(lambda_dssim), ctypes.c_void_p(loss_dev_ptr),
+                                               ctypes.c_void_p(dl_dev_ptr)))
 
     def adam_step(self, gaussians: np.ndarray, grads, state: AdamState, cfg: dict | None = None):
         """optimizer.rs:116-163; updates `gaussians` (float32, C-contiguous) in place."""
