@@ -28,7 +28,8 @@ enum {
   RGS_E_CAPACITY = 5, /* densify result exceeds the caller's capacity, or a
                          render needs 2^31 - 1 or more (tile, splat) pairs */
   RGS_E_IO = 6,       /* a model file could not be opened, read or written */
-  RGS_E_FORMAT = 7    /* not a PLY the reference reads (io/ply.rs:116-170 bails) */
+  RGS_E_FORMAT = 7,   /* not a PLY the reference reads (io/ply.rs:116-170 bails) */
+  RGS_E_HOST = 8      /* a host-side failure, e.g. host memory exhausted */
 };
 
 /* Gaussian3D (math/mod.rs:27-46), 59 x f32 = 236 bytes. */

@@ -61,8 +61,11 @@ void launch_ply_encode(const float* P, size_t cap, size_t n, float* rec, hipStre
 
 // One thread per Gaussian. Fields are applied in header order, so a later
 // property of the same name wins, as in gaussian_from_props.
-__global__ __launch_bounds__(256) void k_ply_decode(const uint8_t* __restrict__ body, size_t n, int rec_bytes,
-                                                    const PlyField* __restrict__ fields, int n_fields,
+// Records from n_body on are all zero bytes: an ASCII file whose lines end
+// before its vertex count (read_line at EOF gives an empty line, whose
+// properties read as 0).
+__global__ __launch_bounds__(256) void k_ply_decode(const uint8_t* __restrict__ body, size_t n, size_t n_body,
+                                                    int rec_bytes, const PlyField* __restrict__ fields, int n_fields,
                                                     int big_endian, float* __restrict__ P, size_t cap) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -79,7 +82,7 @@ __global__ __launch_bounds__(256) void k_ply_decode(const uint8_t* __restrict__ 
     const PlyField f = fields[k];
     if (f.row < 0) continue;
     uint8_t b[8];
-    for (int q = 0; q < f.width; ++q) b[q] = r[f.offset + (big_endian ? f.width - 1 - q : q)];
+    for (int q = 0; q < f.width; ++q) b[q] = i < n_body ? r[f.offset + (big_endian ? f.width - 1 - q : q)] : 0;
     float v = 0.0f;  // decode_scalar (io/ply.rs:194-212)
     if (f.width == 4) {
       v = __builtin_bit_cast(float, (uint32_t)b[0] | (uint32_t)b[1] << 8 | (uint32_t)b[2] << 16 |
@@ -104,9 +107,10 @@ __global__ __launch_bounds__(256) void k_ply_decode(const uint8_t* __restrict__ 
 // and width; width 0 keeps the default), which is what applying them all in
 // header order leaves.
 constexpr int PLY_TILE_BYTES = 32768;
-__global__ __launch_bounds__(256) void k_ply_decode_tiled(const uint8_t* __restrict__ body, size_t n, int rec_bytes,
-                                                          const PlyField* __restrict__ row_field, int big_endian,
-                                                          float* __restrict__ P, size_t cap, int per_block) {
+__global__ __launch_bounds__(256) void k_ply_decode_tiled(const uint8_t* __restrict__ body, size_t n, size_t n_body,
+                                                          int rec_bytes, const PlyField* __restrict__ row_field,
+                                                          int big_endian, float* __restrict__ P, size_t cap,
+                                                          int per_block) {
   __shared__ uint32_t s_w[PLY_TILE_BYTES / 4 + 2];
   const size_t i0 = (size_t)blockIdx.x * per_block;
   const int cnt = (int)min((size_t)per_block, n - i0);
@@ -114,15 +118,16 @@ __global__ __launch_bounds__(256) void k_ply_decode_tiled(const uint8_t* __restr
   const size_t b0 = i0 * (size_t)rec_bytes, b1 = b0 + (size_t)cnt * rec_bytes;
   const size_t w0 = b0 / 4, w1 = (b1 + 3) / 4;
   const int skew = (int)(b0 - w0 * 4);
-  const size_t body_words = (n * (size_t)rec_bytes + 3) / 4;
+  const size_t body_bytes = n_body * (size_t)rec_bytes;  // bytes past it read as 0
+  const size_t body_words = (body_bytes + 3) / 4;
   for (size_t w = w0 + threadIdx.x; w < w1; w += blockDim.x) {
     uint32_t v = 0;
-    if (w + 1 <= (n * (size_t)rec_bytes) / 4) {
+    if (w + 1 <= body_bytes / 4) {
       v = reinterpret_cast<const uint32_t*>(body)[w];  // whole word inside the body
     } else if (w < body_words) {
       for (int q = 0; q < 4; ++q) {  // the body's last, partial word
         const size_t b = w * 4 + q;
-        if (b < n * (size_t)rec_bytes) v |= (uint32_t)body[b] << (8 * q);
+        if (b < body_bytes) v |= (uint32_t)body[b] << (8 * q);
       }
     }
     s_w[w - w0] = v;
@@ -161,21 +166,21 @@ __global__ __launch_bounds__(256) void k_ply_decode_tiled(const uint8_t* __restr
   }
 }
 
-void launch_ply_decode(const uint8_t* body, size_t n, int rec_bytes, const PlyField* fields, int n_fields,
-                       int big_endian, float* P, size_t cap, hipStream_t st) {
+void launch_ply_decode(const uint8_t* body, size_t n, size_t n_body, int rec_bytes, const PlyField* fields,
+                       int n_fields, int big_endian, float* P, size_t cap, hipStream_t st) {
   if (!n) return;
-  hipLaunchKernelGGL(k_ply_decode, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, body, n, rec_bytes, fields,
-                     n_fields, big_endian, P, cap);
+  hipLaunchKernelGGL(k_ply_decode, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, body, n, n_body, rec_bytes,
+                     fields, n_fields, big_endian, P, cap);
 }
 
 int ply_decode_tiled_fits(int rec_bytes) { return rec_bytes > 0 && rec_bytes <= PLY_TILE_BYTES / 16; }
 
-void launch_ply_decode_tiled(const uint8_t* body, size_t n, int rec_bytes, const PlyField* row_field, int big_endian,
-                             float* P, size_t cap, hipStream_t st) {
+void launch_ply_decode_tiled(const uint8_t* body, size_t n, size_t n_body, int rec_bytes, const PlyField* row_field,
+                             int big_endian, float* P, size_t cap, hipStream_t st) {
   if (!n) return;
   const int per_block = min(64, (PLY_TILE_BYTES - 8) / rec_bytes);
   hipLaunchKernelGGL(k_ply_decode_tiled, dim3((unsigned)((n + per_block - 1) / per_block)), dim3(256), 0, st, body, n,
-                     rec_bytes, row_field, big_endian, P, cap, per_block);
+                     n_body, rec_bytes, row_field, big_endian, P, cap, per_block);
 }
 
 // Rust `as u8`: saturating, NaN -> 0.

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <exception>
 #include <mutex>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -32,6 +33,19 @@ thread_local std::string g_err;
 int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
+}
+
+// Runs f, turning a host exception (an allocation that fails) into an error
+// code: no exception crosses the C ABI.
+template <class F>
+int guarded(const char* what, F&& f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    return fail(RGS_E_HOST, std::string(what) + ": out of host memory");
+  } catch (const std::exception& e) {
+    return fail(RGS_E_HOST, std::string(what) + ": " + e.what());
+  }
 }
 
 #define HIPCHK(x)                                                                              \
@@ -1189,7 +1203,11 @@ int write_file(const char* path, const std::string& head, const void* body, size
 }
 
 // PLY records of n Gaussians held as rows [59][cap], written to path.
+int save_ply_rows_impl(rgs_ctx* c, const float* rows, size_t cap, size_t n, const char* path);
 int save_ply_rows(rgs_ctx* c, const float* rows, size_t cap, size_t n, const char* path) {
+  return guarded("writing the model file", [&] { return save_ply_rows_impl(c, rows, cap, n, path); });
+}
+int save_ply_rows_impl(rgs_ctx* c, const float* rows, size_t cap, size_t n, const char* path) {
   std::vector<float> rec(n * PLY_FIELDS);
   if (n) {
     CHK(ensure(c, c->io_rec, n * PLY_FIELDS * 4));
@@ -1200,7 +1218,11 @@ int save_ply_rows(rgs_ctx* c, const float* rows, size_t cap, size_t n, const cha
   return write_file(path, ply_header(n), rec.data(), n * PLY_FIELDS * 4);  // little-endian host
 }
 
+int save_splat_rows_impl(rgs_ctx* c, const float* rows, size_t cap, size_t n, const char* path);
 int save_splat_rows(rgs_ctx* c, const float* rows, size_t cap, size_t n, const char* path) {
+  return guarded("writing the model file", [&] { return save_splat_rows_impl(c, rows, cap, n, path); });
+}
+int save_splat_rows_impl(rgs_ctx* c, const float* rows, size_t cap, size_t n, const char* path) {
   std::vector<uint8_t> rec(n * 32);
   if (n) {
     CHK(ensure(c, c->io_rec, n * 32));
@@ -1240,7 +1262,8 @@ struct PlyFile {
   size_t count = 0;
   std::vector<std::string> names;
   std::vector<int> widths;
-  std::vector<uint8_t> body;  // count records of the properties' widths
+  std::vector<uint8_t> body;  // n_body records of the properties' widths
+  size_t n_body = 0;          // records stored; those up to count are zeros
   int rec_bytes = 0;
 };
 
@@ -1286,7 +1309,12 @@ float parse_f32(const std::string& t) {
 }
 
 // Header and body of a PLY as load_ply reads them (io/ply.rs:121-189).
+int read_ply_impl(const char* path, PlyFile& pf);
 int read_ply(const char* path, PlyFile& pf) {
+  return guarded("reading the PLY", [&] { return read_ply_impl(path, pf); });
+}
+
+int read_ply_impl(const char* path, PlyFile& pf) {
   FILE* f = std::fopen(path, "rb");
   if (!f) return fail(RGS_E_IO, std::string("cannot open ") + path);
   struct Closer {
@@ -1336,24 +1364,37 @@ int read_ply(const char* path, PlyFile& pf) {
   if (pf.format == 2) {
     // one line per vertex, values in property order; a missing value (or
     // a missing line at EOF) reads as 0; stored as little-endian floats
+    // stored as little-endian floats. Only the lines present are stored:
+    // the records after the last line are zeros (n_body), so host memory is
+    // bounded by the file, whatever the header claims.
     pf.rec_bytes = (int)(4 * np);
-    pf.body.assign(pf.count * np * 4, 0);
-    for (size_t i = 0; i < pf.count; ++i) {
-      read_text_line(f, line);
+    size_t i = 0;
+    for (; i < pf.count; ++i) {
+      if (!read_text_line(f, line)) break;
       const auto t = split_ws(line);
+      pf.body.resize((i + 1) * np * 4);
       for (size_t k = 0; k < np; ++k) {
         const float v = k < t.size() ? parse_f32(t[k]) : 0.0f;
         std::memcpy(&pf.body[(i * np + k) * 4], &v, 4);
       }
     }
+    pf.n_body = i;
   } else {
     size_t rb = 0;
     for (int w : pf.widths) rb += (size_t)w;
     if (rb > (1u << 20)) return fail(RGS_E_FORMAT, "PLY record too large");
     pf.rec_bytes = (int)rb;
+    // the body must hold count records before anything is allocated for it
+    const long here = std::ftell(f);
+    if (here < 0 || std::fseek(f, 0, SEEK_END) != 0) return fail(RGS_E_IO, "cannot seek in the PLY file");
+    const long end = std::ftell(f);
+    if (end < here || std::fseek(f, here, SEEK_SET) != 0) return fail(RGS_E_IO, "cannot seek in the PLY file");
+    if (rb && (unsigned long long)(end - here) / rb < pf.count)
+      return fail(RGS_E_FORMAT, "PLY body shorter than its vertex count");
     pf.body.resize(pf.count * rb);
     if (!pf.body.empty() && std::fread(pf.body.data(), 1, pf.body.size(), f) != pf.body.size())
       return fail(RGS_E_FORMAT, "PLY body shorter than its vertex count");
+    pf.n_body = pf.count;
   }
   return 0;
 }
@@ -1378,13 +1419,13 @@ int decode_ply(rgs_ctx* c, const PlyFile& pf, float* rows, size_t cap) {
     for (const PlyField& f : fields)
       if (f.row >= 0) row_field[f.row] = f;
     HIPCHK(hipMemcpyAsync(c->io_fields.p, row_field.data(), P_N * sizeof(PlyField), hipMemcpyHostToDevice, c->st));
-    launch_ply_decode_tiled(P_<uint8_t>(c->io_rec), pf.count, pf.rec_bytes, P_<PlyField>(c->io_fields),
+    launch_ply_decode_tiled(P_<uint8_t>(c->io_rec), pf.count, pf.n_body, pf.rec_bytes, P_<PlyField>(c->io_fields),
                             pf.format == 1, rows, cap, c->st);
   } else {
     if (!fields.empty())
       HIPCHK(hipMemcpyAsync(c->io_fields.p, fields.data(), fields.size() * sizeof(PlyField), hipMemcpyHostToDevice,
                             c->st));
-    launch_ply_decode(P_<uint8_t>(c->io_rec), pf.count, pf.rec_bytes, P_<PlyField>(c->io_fields),
+    launch_ply_decode(P_<uint8_t>(c->io_rec), pf.count, pf.n_body, pf.rec_bytes, P_<PlyField>(c->io_fields),
                       (int)fields.size(), pf.format == 1, rows, cap, c->st);
   }
   // the host buffers above are read by the copies: wait before they go
@@ -1985,7 +2026,7 @@ int rgs_train_step(rgs_ctx* c, const int* ids, int n_ids, const rgs_train_cfg* c
   try {
     rc = train_step_impl(c, ids, n_ids, cfg, bg, lambda, use_graph, loss_out);
   } catch (const std::exception& e) {
-    rc = fail(RGS_E_HIP, std::string("host failure in the training step: ") + e.what());
+    rc = fail(RGS_E_HOST, std::string("host failure in the training step: ") + e.what());
   }
   if (rc) c->t = t0;
   return rc;
@@ -2406,7 +2447,11 @@ int rgs_set_grad_capture(rgs_ctx* c, int enable) {
   return 0;
 }
 
+static int step_grads_impl(rgs_ctx* c, rgs_grad* out, size_t n);
 int rgs_step_grads(rgs_ctx* c, rgs_grad* out, size_t n) {
+  return guarded("rgs_step_grads", [&] { return step_grads_impl(c, out, n); });
+}
+static int step_grads_impl(rgs_ctx* c, rgs_grad* out, size_t n) {
   if (!c || !out) return fail(RGS_E_ARG, "null argument");
   if (!c->grad_capture || !c->gcap.p) return fail(RGS_E_ARG, "gradient capture is off or no step has run");
   if (n != c->n) return fail(RGS_E_ARG, "n must be the scene size");
@@ -2421,7 +2466,13 @@ int rgs_step_grads(rgs_ctx* c, rgs_grad* out, size_t n) {
   return 0;
 }
 
+static int debug_tiles_impl(rgs_ctx* c, uint64_t* offsets, size_t n_offsets, uint32_t* gis, size_t k_max,
+                            uint64_t* k_out);
 int rgs_debug_tiles(rgs_ctx* c, uint64_t* offsets, size_t n_offsets, uint32_t* gis, size_t k_max, uint64_t* k_out) {
+  return guarded("rgs_debug_tiles", [&] { return debug_tiles_impl(c, offsets, n_offsets, gis, k_max, k_out); });
+}
+static int debug_tiles_impl(rgs_ctx* c, uint64_t* offsets, size_t n_offsets, uint32_t* gis, size_t k_max,
+                            uint64_t* k_out) {
   if (!c || !offsets || !k_out) return fail(RGS_E_ARG, "null argument");
   if (c->last_shift != 0) return fail(RGS_E_ARG, "the last render binned per super-tile (a training step)");
   if (n_offsets < (size_t)c->last_tiles + 1) return fail(RGS_E_ARG, "offsets too small");

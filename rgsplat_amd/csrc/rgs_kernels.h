@@ -273,13 +273,14 @@ struct PlyField {
 };
 constexpr int PLY_FIELDS = 62;  // floats per record written by save_ply
 void launch_ply_encode(const float* P, size_t cap, size_t n, float* rec, hipStream_t st);
-void launch_ply_decode(const uint8_t* body, size_t n, int rec_bytes, const PlyField* fields, int n_fields,
-                       int big_endian, float* P, size_t cap, hipStream_t st);
+// n records; those from n_body on are all zero bytes (not read from body).
+void launch_ply_decode(const uint8_t* body, size_t n, size_t n_body, int rec_bytes, const PlyField* fields,
+                       int n_fields, int big_endian, float* P, size_t cap, hipStream_t st);
 // Tiled decoder for records of at most ply_decode_tiled_fits bytes;
 // row_field[59]: per SoA row the last property that sets it (width 0: none).
 int ply_decode_tiled_fits(int rec_bytes);
-void launch_ply_decode_tiled(const uint8_t* body, size_t n, int rec_bytes, const PlyField* row_field, int big_endian,
-                             float* P, size_t cap, hipStream_t st);
+void launch_ply_decode_tiled(const uint8_t* body, size_t n, size_t n_body, int rec_bytes, const PlyField* row_field,
+                             int big_endian, float* P, size_t cap, hipStream_t st);
 void launch_splat_encode(const float* P, size_t cap, size_t n, void* out /* n x 32 bytes */, hipStream_t st);
 // image 0.25's resize_exact with the Triangle filter, RGBA8 (k_io.hip):
 // taps are (left, count) per output row (v) / column (h), weights K per row.

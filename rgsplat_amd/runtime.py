@@ -273,6 +273,11 @@ class Context:
         if str(out.dtype) != "torch.float32" or out.device.type != "cuda":
             raise ValueError("out must be a float32 tensor on the GPU")
         c = camera(cam)
+        # The render runs on this context's stream, which is not ordered
+        # after torch's: wait for the work torch has queued (which may still
+        # write or read `out`). The call returns with the image complete.
+        import torch
+        torch.cuda.current_stream(out.device).synchronize()
         _check(lib().rgs_scene_preview_render_device(self.h, ctypes.byref(c), _p(_f32(bg)),
                                                      ctypes.c_void_p(out.data_ptr())))
         return out

@@ -101,3 +101,39 @@ def test_checkpoint_after_training(ctx, tmp_path):
     c.scene_load_ply(tmp_path / "ckpt.ply")
     assert same(c.scene_download(), trained)
     c.close()
+
+
+def test_load_ply_short_ascii_and_huge_counts(ctx, tmp_path):
+    """An ASCII body with fewer lines than its vertex count: read_line at EOF
+    gives empty lines, whose properties read as 0 (io/ply.rs:170-179). A
+    header that claims 2^31 vertices costs no host memory: a binary body is
+    checked against the file's length before anything is allocated
+    (RGS_E_FORMAT), an ASCII one stores only its lines and reports the count
+    to a caller without room for it (RGS_E_CAPACITY); no exception leaves the
+    library."""
+    import ctypes
+    from rgsplat_amd import runtime
+    props = ["x", "y", "z", "opacity", "f_dc_0", "rot_1"]
+    head = ("ply\nformat ascii 1.0\nelement vertex {n}\n" + "".join(f"property float {p}\n" for p in props)
+            + "end_header\n")
+    short = tmp_path / "short.ply"
+    short.write_text(head.format(n=5) + "1 2 3 0.5 0.1 0.2\n4 5 6\n")
+    got = ctx.load_ply(short)
+    assert len(got) == 5
+    assert same(got, oracle.load_ply(short))
+    ctx.scene_load_ply(short)
+    assert same(ctx.scene_download(), got)
+    L = runtime.lib()
+    n = ctypes.c_size_t()
+    big_ascii = tmp_path / "big_ascii.ply"
+    big_ascii.write_text(head.format(n=2 ** 31) + "1 2 3 0.5 0.1 0.2\n")
+    assert L.rgs_load_ply(ctx.h, bytes(big_ascii), None, 0, ctypes.byref(n)) == 5 and n.value == 2 ** 31
+    small = np.zeros((4, 59), np.float32)
+    assert L.rgs_load_ply(ctx.h, bytes(big_ascii), small.ctypes.data_as(ctypes.c_void_p), 4, ctypes.byref(n)) == 5
+    big_bin = tmp_path / "big_bin.ply"
+    big_bin.write_bytes(head.format(n=2 ** 31).replace("ascii", "binary_little_endian").encode()
+                        + np.arange(12, dtype=np.float32).tobytes())
+    assert L.rgs_load_ply(ctx.h, bytes(big_bin), None, 0, ctypes.byref(n)) == 7
+    assert L.rgs_scene_load_ply(ctx.h, bytes(big_bin), 0) == 7
+    # the context still works
+    assert same(ctx.load_ply(short), got)

@@ -601,8 +601,11 @@ def test_training_reduces_loss_on_gpu(ctx):
 
 
 def test_full_size_properties(ctx):
-    """1M Gaussians at 1080p: the oracle is too slow here, so check
-    size-independent properties of the binning and the step."""
+    """1M Gaussians at 1080p through the reference-binning API render: the
+    whole-image tile lists (132M pairs) are checked by size-independent
+    properties (every list in depth order, ties by index, pair counts that
+    agree); the images, losses and gradients of the training step at this
+    size are compared with the oracle in tests/test_gpu_fullsize.py."""
     from rgsplat_amd import runtime
     wl = S.WORKLOADS["c3_1m_1080p_4views"]
     g = wl.scene()
@@ -666,3 +669,21 @@ def test_raster_params_and_autotune(ctx):
     for k in ["position", "rotation", "log_scale", "opacity", "sh"]:
         assert errs[k] <= 1e-3, (k, errs)
     c.close()
+
+
+def test_create_on_a_missing_device():
+    """rgs_create on a device id the machine does not have: RGS_E_ARG and a
+    message, no context; the current device (-1) and device 0 still work."""
+    import ctypes
+    from rgsplat_amd import runtime
+    L = runtime.lib()
+    h = ctypes.c_void_p()
+    assert L.rgs_create(4096, ctypes.byref(h)) == 1 and not h.value
+    assert b"out of range" in L.rgs_last_error()
+    with pytest.raises(runtime.RgsError):
+        runtime.Context(4096)
+    for dev in (-1, 0):
+        c = runtime.Context(dev)
+        g, cams, _ = workload("c0_10k_256", 200, [(40, 24)])
+        assert np.array_equal(c.render_color(g, cams[0]), oracle.render_color(g, cams[0]))
+        c.close()

@@ -90,3 +90,21 @@ def test_scene_preview_to_device(ctx):
     out = torch.empty((120, 200, 3), dtype=torch.float32, device="cuda:0")
     ctx.scene_preview_render_device(cam, out, (0.3, 0.2, 0.1))
     assert same(out.cpu().numpy(), oracle.preview_render(g, cam, (0.3, 0.2, 0.1)))
+
+
+def test_scene_preview_to_device_after_torch_work(ctx):
+    """The render into a torch tensor is ordered after the work torch has
+    queued on its own stream: a fill of `out` held back by a spin kernel
+    must land before the image, not over it."""
+    import torch
+    g = C.scene(3000, 13, opacity=(0.05, 0.99))
+    cam = S.look_at_origin(np.array([0.4, -0.7, 0.5]), 3.0, 160, 96, 150.0)
+    ctx.scene_upload(g)
+    want = oracle.preview_render(g, cam, (0.0, 0.0, 0.0))
+    out = torch.empty((96, 160, 3), dtype=torch.float32, device="cuda:0")
+    for _ in range(3):
+        torch.cuda._sleep(20_000_000)  # ~10 ms of spinning on torch's stream
+        out.fill_(7.0)
+        ctx.scene_preview_render_device(cam, out)
+        torch.cuda.synchronize()
+        assert same(out.cpu().numpy(), want)
