@@ -337,6 +337,53 @@ def target_ingestion(runtime, wl, reps, cpu=True):
                       "source": [[im.shape[1], im.shape[0]] for im in ims], "reps": reps, **res}), flush=True)
 
 
+def densifying_run(runtime, wl, iters, grad_threshold=None):
+    """The workload's densifying schedule: `iters` iterations of one batched
+    step over all its views, densify_and_prune every 100 iterations from 500
+    (pipeline/mod.rs:240-242), the scene growing in place up to
+    max_gaussians. Reports iterations/s including the densifies, their cost
+    and N(t)."""
+    from rgsplat_amd import pipeline
+    g = wl.scene()
+    cams = wl.cameras()
+    ctx = runtime.Context(0)
+    tscene = wl.target_scene()
+    targets = [with_alpha(ctx.render_color(tscene, c)) for c in cams]
+    del tscene
+    cfg = workload_train_config(num_iterations=iters)
+    if grad_threshold is not None:
+        cfg["densify_grad_threshold"] = grad_threshold
+    ctx.scene_upload(g, capacity=int(cfg["max_gaussians"]))
+    ctx.set_views(cams, targets)
+    tuned = ctx.autotune_raster(list(range(len(cams))))
+    sync()
+    t0 = time.perf_counter()
+    _, log = pipeline.train(ctx, g, cams, None, cfg, all_views=True)
+    sync()
+    wall = time.perf_counter() - t0
+    dens_s = sum(d[3] for d in log.densify)
+    ctx.close()
+    out = {"metric": "densifying training run (iterations/s including densify)", "value": iters / wall,
+           "unit": "iters/s", "iterations": iters, "wall_s": wall, "densify_s": dens_s,
+           "config": {"workload": wl.name, "views_per_step": len(cams), "densify_every": 100,
+                      "densify_from_iter": cfg["densify_from_iter"], "max_gaussians": cfg["max_gaussians"],
+                      "densify_grad_threshold": cfg["densify_grad_threshold"],
+                      "raster_params": tuned},
+           "n_t": [[d[0], d[1], d[2], round(d[3] * 1000.0, 2)] for d in log.densify],
+           "n_t_fields": ["iteration", "gaussians_before", "gaussians_after", "densify_ms"],
+           "final_gaussians": log.sizes[-1], "loss_first_last": [log.losses[0], log.losses[-1]]}
+    print(f"[bench] densifying run {iters} iterations in {wall:.1f} s ({dens_s:.2f} s densify)", file=sys.stderr,
+          flush=True)
+    print(json.dumps(out), flush=True)
+
+
+def workload_train_config(**over):
+    from rgsplat_amd import synthetic as S
+    cfg = S.benchmark_train_config()
+    cfg.update(over)
+    return cfg
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -356,6 +403,10 @@ def main():
                     help="time the preview renderer instead (GpuRenderer::render of the workload's scene)")
     ap.add_argument("--targets", action="store_true",
                     help="time the target ingestion instead (RGBA8 images resized and converted on the device)")
+    ap.add_argument("--densify", type=int, default=0, metavar="ITERS",
+                    help="run the workload's densifying schedule for ITERS iterations instead (C3: 2000)")
+    ap.add_argument("--densify-grad-threshold", type=float, default=None,
+                    help="densify_grad_threshold of the densifying run (default: the reference's 2e-4)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the step would shard by views "
@@ -379,6 +430,9 @@ def main():
         return
     if args.targets:
         target_ingestion(runtime, wl, args.steps, cpu=not args.no_cpu)
+        return
+    if args.densify:
+        densifying_run(runtime, wl, args.densify, args.densify_grad_threshold)
         return
     print(f"[bench] workload {wl.name}: {wl.n} Gaussians, views {wl.sizes}", file=sys.stderr, flush=True)
     g = wl.scene()

@@ -196,6 +196,13 @@ int rgs_train_step(rgs_ctx* ctx, const int* view_ids, int n_ids, const rgs_train
                    float lambda_dssim, int use_graph, float* loss_out);
 /* densify_and_prune / reset_opacities on the device-resident scene. */
 int rgs_scene_densify(rgs_ctx* ctx, const rgs_train_cfg* cfg, float scene_extent);
+/* The split of densify (rgs_scene_densify and rgs_densify_and_prune). 0, the
+ * default, is split_gaussian (optimizer.rs:274-301): the two children's
+ * log_scale[0] becomes ln(max(s0 / sqrt 2, 1e-6)), the other axes keep
+ * theirs. A divisor d > 1 sets every child log-scale to ln(max(s_j / d,
+ * 1e-6)) -- the "scale / 1.6" split of 3DGS practice -- with the
+ * reference's offsets and opacity. No counterpart in the reference. */
+int rgs_set_split_scale_divisor(rgs_ctx* ctx, float divisor);
 int rgs_scene_reset_opacities(rgs_ctx* ctx);
 /* Last render's (tile, splat) pair count, summed over its views, and the
  * (pixel, splat) pairs its forward and backward passes evaluate (each output

@@ -112,6 +112,10 @@ def _declare(L):
     L.orc_densify_and_prune.restype = i32
     L.orc_densify_and_prune.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t), sz, vp, vp, vp, vp,
                                         ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(TrainCfg), ctypes.c_float]
+    L.orc_densify_and_prune_split.restype = i32
+    L.orc_densify_and_prune_split.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t), sz, vp, vp, vp, vp,
+                                              ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(TrainCfg),
+                                              ctypes.c_float, ctypes.c_float]
     L.orc_set_threads.argtypes = [i32]
     L.orc_expf_batch.argtypes = [vp, vp, sz]
     L.orc_gaussians_from_sfm.restype = i32
@@ -340,14 +344,16 @@ def scene_extent(g):
     return lib().orc_scene_extent(_p(g), len(g))
 
 
-def densify_and_prune(g_buf, n, st: AdamState, cfg: dict, extent: float):
-    """g_buf: (cap, 59) float32 buffer whose first n rows are live. Returns new n."""
+def densify_and_prune(g_buf, n, st: AdamState, cfg: dict, extent: float, split_div: float = 0.0):
+    """g_buf: (cap, 59) float32 buffer whose first n rows are live. Returns new n.
+    split_div > 0 replaces the reference's split scale (s0 / sqrt 2 on axis
+    0) by every scale / split_div."""
     nn = ctypes.c_size_t(n)
     rows = ctypes.c_size_t(st.rows)
     c = train_cfg(cfg)
-    rc = lib().orc_densify_and_prune(_p(g_buf), ctypes.byref(nn), len(g_buf), _p(st.m1), _p(st.m2),
-                                     _p(st.accum), _p(st.denom), ctypes.byref(rows), ctypes.byref(c),
-                                     float(extent))
+    rc = lib().orc_densify_and_prune_split(_p(g_buf), ctypes.byref(nn), len(g_buf), _p(st.m1), _p(st.m2),
+                                           _p(st.accum), _p(st.denom), ctypes.byref(rows), ctypes.byref(c),
+                                           float(extent), float(split_div))
     if rc:
         raise ValueError("capacity exceeded")
     st.rows = rows.value

@@ -1318,8 +1318,18 @@ float orc_scene_extent(const float* g, size_t n) {
 // (rows) have capacity `cap` rows; *rows is AdamState's current row count
 // (it only grows, so moments stay aligned to row indices, not Gaussians).
 // Returns 1 if the result would exceed `cap`.
+// split_div (an extension, not the reference's): 0 for split_gaussian; > 0
+// sets every child log-scale to ln(max(s_j / split_div, 1e-6)).
+int orc_densify_and_prune_split(float* g, size_t* n_io, size_t cap, float* m1, float* m2, float* accum,
+                                uint32_t* denom, size_t* rows, const orc_train_cfg* cfg, float extent,
+                                float split_div);
 int orc_densify_and_prune(float* g, size_t* n_io, size_t cap, float* m1, float* m2, float* accum, uint32_t* denom,
                           size_t* rows, const orc_train_cfg* cfg, float extent) {
+  return orc_densify_and_prune_split(g, n_io, cap, m1, m2, accum, denom, rows, cfg, extent, 0.0f);
+}
+int orc_densify_and_prune_split(float* g, size_t* n_io, size_t cap, float* m1, float* m2, float* accum,
+                                uint32_t* denom, size_t* rows, const orc_train_cfg* cfg, float extent,
+                                float split_div) {
   size_t n = *n_io;
   std::vector<float> to_add;
   std::vector<uint8_t> to_remove(n, 0);
@@ -1352,7 +1362,9 @@ int orc_densify_and_prune(float* g, size_t* n_io, size_t cap, float* m1, float* 
         float nrm = sqrtf(a0 + a1 + a2);
         float off[3];
         for (int k = 0; k < 3; ++k) off[k] = ax[k] / nrm * s[0];
-        float new_scale = logf(fmaxf(s[0] / 1.41421356237309504880f, 1e-6f));
+        float new_ls[3] = {logf(fmaxf(s[0] / 1.41421356237309504880f, 1e-6f)), gp[G_LS + 1], gp[G_LS + 2]};
+        if (split_div > 0.0f)
+          for (int k = 0; k < 3; ++k) new_ls[k] = logf(fmaxf(s[k] / split_div, 1e-6f));
         float new_op = clampf(alpha * 0.8f, 1e-6f, 0.999f);
         float new_logit = logit(new_op);
         for (int c = 0; c < 2; ++c) {
@@ -1362,7 +1374,7 @@ int orc_densify_and_prune(float* g, size_t* n_io, size_t cap, float* m1, float* 
             if (c == 0) to_add[base + k] += off[k];
             else to_add[base + k] -= off[k];
           }
-          to_add[base + G_LS] = new_scale;
+          for (int k = 0; k < 3; ++k) to_add[base + G_LS + k] = new_ls[k];
           to_add[base + G_OPL] = new_logit;
         }
       } else {

@@ -645,7 +645,8 @@ __global__ void k_densify_decide(const uint32_t* __restrict__ want, const uint32
 __global__ void k_densify_scatter(const float* __restrict__ P, float* __restrict__ Q, int cap, int n,
                                   const uint64_t* __restrict__ packed_in, const uint64_t* __restrict__ prefix,
                                   const uint64_t* __restrict__ total, const float* __restrict__ gaccum,
-                                  const uint32_t* __restrict__ gdenom, uint32_t* __restrict__ n_out) {
+                                  const uint32_t* __restrict__ gdenom, uint32_t* __restrict__ n_out,
+                                  float split_div) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t tot = *total;
   uint32_t kept_total = (uint32_t)(tot & 0xffffffffu);
@@ -688,7 +689,17 @@ __global__ void k_densify_scatter(const float* __restrict__ P, float* __restrict
   float ax0 = t0 * w + c0 + v0, ax1 = t1 * w + c1 + v1, ax2 = t2 * w + c2 + v2;
   float nrm = __builtin_sqrtf(ax0 * ax0 + ax1 * ax1 + ax2 * ax2);
   float off[3] = {ax0 / nrm * s0, ax1 / nrm * s0, ax2 / nrm * s0};
-  float new_scale = logf_dev(fmax_(s0 / 1.41421356237309504880f, 1e-6f));
+  // the reference halves the variance along axis 0 only; with split_div > 0
+  // every axis is divided by it (the workload table's "scale / 1.6")
+  float new_ls[3];
+  if (split_div > 0.0f) {
+    for (int k = 0; k < 3; ++k)
+      new_ls[k] = logf_dev(fmax_(rgs_expf(P[(size_t)(P_LS + k) * cap + i]) / split_div, 1e-6f));
+  } else {
+    new_ls[0] = logf_dev(fmax_(s0 / 1.41421356237309504880f, 1e-6f));
+    new_ls[1] = P[(size_t)(P_LS + 1) * cap + i];
+    new_ls[2] = P[(size_t)(P_LS + 2) * cap + i];
+  }
   float alpha = sigmoid(P[(size_t)P_OP * cap + i]);
   float op = alpha * 0.8f;
   if (op < 1e-6f) op = 1e-6f;
@@ -699,7 +710,7 @@ __global__ void k_densify_scatter(const float* __restrict__ P, float* __restrict
     for (int j = 0; j < P_N; ++j) {
       float xv = P[(size_t)j * cap + i];
       if (j < 3) xv = ci == 0 ? xv + off[j] : xv - off[j];
-      if (j == P_LS) xv = new_scale;
+      if (j >= P_LS && j < P_LS + 3) xv = new_ls[j - P_LS];
       if (j == P_OP) xv = new_logit;
       Q[(size_t)j * cap + o] = xv;
     }
@@ -708,7 +719,7 @@ __global__ void k_densify_scatter(const float* __restrict__ P, float* __restrict
 
 void launch_densify(const float* P, float* P_out, int cap, int n, const float* gaccum, const uint32_t* gdenom,
                     float grad_thresh, float prune_thresh, uint64_t max_gaussians, float extent, DensifyScratch ds,
-                    uint32_t* n_out, hipStream_t st) {
+                    uint32_t* n_out, hipStream_t st, float split_div) {
   if (n <= 0) {
     launch_fill_u32(n_out, 1, 0u, st);
     return;
@@ -720,7 +731,7 @@ void launch_densify(const float* P, float* P_out, int cap, int n, const float* g
   hipLaunchKernelGGL(k_densify_decide, g, b, 0, st, ds.want, ds.W, n, max_gaussians, ds.packed);
   scan_u64(ds.packed, (size_t)n, ds.prefix, ds.bsum64, ds.total64, st);
   hipLaunchKernelGGL(k_densify_scatter, g, b, 0, st, P, P_out, cap, n, ds.packed, ds.prefix, ds.total64, gaccum,
-                     gdenom, n_out);
+                     gdenom, n_out, split_div);
 }
 
 __global__ void k_zero_rows(float* M, size_t cap, size_t r0, size_t r1) {

@@ -310,6 +310,7 @@ struct rgs_ctx {
   int super_tile = 8;      // tiles per side of a schedule super-tile
   int sparse_lanes = 2;    // backward: LDS atomics up to this many lanes
   float* scratch_dev = nullptr;  // this device's copy of the no-alloc scratch array
+  float split_div = 0.0f;         // densify: 0 the reference's split, > 0 every scale / split_div
   bool grad_capture = false;  // the step writes its summed gradients into gcap
   DBuf gcap;                  // 60 rows x cap: gradient rows 0-58, screen-gradient norm
 
@@ -1056,7 +1057,7 @@ int rgs_densify_and_prune(rgs_ctx* c, rgs_gaussian* g, size_t* n_io, size_t capa
   uint32_t* sc = P_<uint32_t>(c->scalars);
   launch_densify(P_<float>(c->api_P), P_<float>(c->api_P2), (int)cap, (int)n, P_<float>(c->api_acc),
                  P_<uint32_t>(c->api_den), cfg->densify_grad_threshold, cfg->prune_opacity_thresh,
-                 cfg->max_gaussians, extent, dscratch(c), sc + SC_NOUT, c->st);
+                 cfg->max_gaussians, extent, dscratch(c), sc + SC_NOUT, c->st, c->split_div);
   HIPCHK(hipMemcpyAsync(c->h_scalars, sc, SC_COUNT * 4, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   size_t nn = c->h_scalars[SC_NOUT];
@@ -2292,7 +2293,7 @@ int rgs_scene_densify(rgs_ctx* c, const rgs_train_cfg* cfg, float extent) {
   uint32_t* sc = P_<uint32_t>(c->scalars);
   launch_densify(P_<float>(c->P), P_<float>(c->P2), (int)c->cap, (int)n, P_<float>(c->gacc), P_<uint32_t>(c->gden),
                  cfg->densify_grad_threshold, cfg->prune_opacity_thresh, cfg->max_gaussians, extent, dscratch(c),
-                 sc + SC_NOUT, c->st);
+                 sc + SC_NOUT, c->st, c->split_div);
   HIPCHK(hipMemcpyAsync(c->h_scalars, sc, SC_COUNT * 4, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   size_t nn = c->h_scalars[SC_NOUT];
@@ -2438,6 +2439,14 @@ int rgs_eval_expf(rgs_ctx* c, const float* x, float* out, size_t n) {
   launch_eval_expf(P_<float>(c->api_aos), P_<float>(c->api_aos) + n, n, c->st);
   HIPCHK(hipMemcpyAsync(out, P_<float>(c->api_aos) + n, n * 4, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+int rgs_set_split_scale_divisor(rgs_ctx* c, float divisor) {
+  if (!c) return fail(RGS_E_ARG, "null ctx");
+  if (!(divisor == 0.0f || (divisor > 1.0f && divisor < 1e6f)))
+    return fail(RGS_E_ARG, "split scale divisor must be 0 (the reference's split) or in (1, 1e6)");
+  c->split_div = divisor;
   return 0;
 }
 

@@ -331,9 +331,11 @@ struct DensifyScratch {
   uint32_t* total32;     // 1
   uint64_t* total64;     // 1
 };
+// split_div: 0 for the reference's split (log_scale[0] of the children
+// becomes ln(max(s0 / sqrt 2, 1e-6))); > 0 divides every scale by it.
 void launch_densify(const float* P, float* P_out, int cap, int n, const float* gaccum, const uint32_t* gdenom,
                     float grad_thresh, float prune_thresh, uint64_t max_gaussians, float extent, DensifyScratch ds,
-                    uint32_t* n_out, hipStream_t st);
+                    uint32_t* n_out, hipStream_t st, float split_div = 0.0f);
 void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t st);
 void launch_fill_f32(float* p, size_t n, float v, hipStream_t st);
 // Zero rows [r0, r1) of each of the 59 rows of a SoA moment buffer.

@@ -13,6 +13,7 @@ scene.
 from __future__ import annotations
 
 import os
+import time
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -41,17 +42,21 @@ class TrainLog:
     cameras: list = field(default_factory=list)    # per iteration
     sizes: list = field(default_factory=list)      # Gaussians after each iteration's densify
     checkpoints: list = field(default_factory=list)
+    densify: list = field(default_factory=list)    # (iteration, Gaussians before, after, seconds)
 
 
 def train(ctx: runtime.Context, gaussians, cameras, images, cfg: dict | None = None, bg=(0.0, 0.0, 0.0),
           save_interval: int = 0, out_dir: str | None = None, lambda_dssim: float = 0.2,
-          use_graph: bool = True) -> tuple[np.ndarray, TrainLog]:
+          use_graph: bool = True, all_views: bool = False) -> tuple[np.ndarray, TrainLog]:
     """Runs cfg['num_iterations'] iterations of train_blocking's loop on the
     device and returns the trained Gaussians (n, 59) and a log. `images` are
     the cameras' linear RGBA targets (h, w, 4), or None when the views are
     already registered with the context. The scene's capacity is
     cfg['max_gaussians'] (densify never grows past it, as in the reference).
-    save_interval 0 writes no checkpoints."""
+    save_interval 0 writes no checkpoints. all_views: every iteration is one
+    step over all the cameras (their gradients summed, one Adam step) instead
+    of the reference's one camera per iteration; the logged loss is their
+    mean."""
     cfg = dict(cfg or synthetic.train_config())
     g = np.ascontiguousarray(gaussians, np.float32).reshape(-1, 59)
     extent = runtime.scene_extent(g)  # estimate_scene_extent, once (pipeline/mod.rs:214)
@@ -61,12 +66,17 @@ def train(ctx: runtime.Context, gaussians, cameras, images, cfg: dict | None = N
         ctx.set_views(cameras, images)
     cams = camera_sequence(len(cameras), int(cfg["num_iterations"]))
     log = TrainLog()
+    batch = list(range(len(cameras)))
     for it, cam in enumerate(cams):
-        loss = ctx.train_step([cam], cfg, bg=bg, lambda_dssim=lambda_dssim, use_graph=use_graph)
-        log.losses.append(float(np.asarray(loss).reshape(-1)[0]))
-        log.cameras.append(cam)
+        ids = batch if all_views else [cam]
+        loss = ctx.train_step(ids, cfg, bg=bg, lambda_dssim=lambda_dssim, use_graph=use_graph)
+        log.losses.append(float(np.mean(np.asarray(loss))))
+        log.cameras.append(-1 if all_views else cam)
         if cfg["densify_from_iter"] <= it <= cfg["densify_until_iter"] and it % 100 == 0 and it > 0:
+            n0 = ctx.scene_size()
+            t0 = time.perf_counter()
             ctx.scene_densify(cfg, extent)
+            log.densify.append((it, n0, ctx.scene_size(), time.perf_counter() - t0))
         if it % cfg["opacity_reset_iter"] == 0 and it > 0:
             ctx.scene_reset_opacities()
         if save_interval and it % save_interval == 0 and it > 0:

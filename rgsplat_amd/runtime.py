@@ -111,6 +111,7 @@ SIGNATURES = {
     "rgs_stage_times": [_vp, _vp, _i, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)],
     "rgs_debug_tiles": [_vp, _vp, _sz, _vp, _sz, ctypes.POINTER(ctypes.c_uint64)],
     "rgs_set_grad_capture": [_vp, _i],
+    "rgs_set_split_scale_divisor": [_vp, _f],
     "rgs_loss_grad_dev_noalloc": [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32, _f, _vp, _vp],
     "rgs_step_grads": [_vp, _vp, _sz],
 }
@@ -476,6 +477,11 @@ class Context:
     def scene_densify(self, cfg: dict | None, extent: float):
         c = train_cfg(cfg)
         _check(lib().rgs_scene_densify(self.h, ctypes.byref(c), float(extent)))
+
+    def set_split_scale_divisor(self, divisor: float):
+        """Densify's split: 0 (default) the reference's (s0 / sqrt 2 on axis
+        0); d > 1 divides every scale of the children by d."""
+        _check(lib().rgs_set_split_scale_divisor(self.h, float(divisor)))
 
     def scene_reset_opacities(self):
         _check(lib().rgs_scene_reset_opacities(self.h))

@@ -37,6 +37,14 @@ def test_train_driver(tmp_path):
     ck = oracle.load_ply(log.checkpoints[-1])
     assert np.isfinite(ck).all()
     assert np.all(1.0 / (1.0 + np.exp(-ck[:, 3].astype(np.float64))) <= 0.01 + 1e-6)
+    # one densify, at iteration 100, logged with the sizes around it
+    assert [d[0] for d in log.densify] == [100]
+    assert log.densify[0][1] == log.sizes[99] and log.densify[0][2] == log.sizes[100]
+    # all views per iteration (one batched step each)
+    cfg.update(num_iterations=20)
+    _, log2 = pipeline.train(ctx, g, cams, targets, cfg, all_views=True)
+    assert log2.cameras == [-1] * 20 and np.all(np.isfinite(log2.losses))
+    assert np.mean(log2.losses[-5:]) < np.mean(log2.losses[:5])
     ctx.close()
 
 

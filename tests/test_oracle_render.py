@@ -395,9 +395,10 @@ def test_adam_matches_textbook_f64(orc):
     assert np.all(st.denom[:n] == 5)
 
 
-def _densify_python(g, n, st, cfg, extent):
+def _densify_python(g, n, st, cfg, extent, split_div=0.0):
     """Line-by-line Python restatement of optimizer.rs:206-311 in fp64 math,
-    rounded to f32 at the same points."""
+    rounded to f32 at the same points; split_div > 0: every child log-scale is
+    ln(max(s_j / split_div, 1e-6)) instead of the reference's axis-0 split."""
     f = np.float32
     sig = lambda x: f(1) / (f(1) + f(math.exp(-float(x))))
     to_add, remove = [], [False] * n
@@ -429,6 +430,8 @@ def _densify_python(g, n, st, cfg, extent):
                 lg = math.log(op / (1 - op))
                 for c in (c1, c2):
                     c[8] = ns
+                    if split_div > 0:
+                        c[8:11] = [math.log(max(s[k] / split_div, 1e-6)) for k in range(3)]
                     c[3] = lg
                 to_add += [c1, c2]
             else:
@@ -465,6 +468,36 @@ def test_densify_matches_python_restatement(orc):
     assert np.all(st.accum[:st.rows] == 0) and np.all(st.denom[:st.rows] == 0)
     # the max_gaussians cap is reached: exactly one addition sequence fits
     assert nn <= n + 120 + 2
+
+
+def test_densify_split_scale_divisor(orc):
+    """The split-factor knob (not the reference's): children of a split get
+    every scale divided by the divisor; everything else, and the decisions,
+    are the reference's."""
+    n = 400
+    g = S.make_scene(n, 17, log_scale_mean=-3.0, log_scale_std=1.0)
+    extent = orc.scene_extent(g)
+    cfg = S.train_config()
+    acc = np.abs(S.Stream(3).normal((n,), std=4e-4)).astype(np.float32)
+    out = {}
+    for d in (0.0, 1.6):
+        st = orc.AdamState(n, cap=3 * n)
+        st.accum[:n] = acc
+        st.denom[:n] = 1
+        buf = np.zeros((3 * n, 59), np.float32)
+        buf[:n] = g
+        nn = orc.densify_and_prune(buf, n, st, cfg, extent, split_div=d)
+        st2 = orc.AdamState(n, cap=3 * n)
+        st2.accum[:n] = acc
+        st2.denom[:n] = 1
+        ref = _densify_python(g, n, st2, cfg, extent, split_div=d)
+        assert nn == len(ref)
+        np.testing.assert_allclose(buf[:nn], ref, rtol=2e-6, atol=2e-6)
+        out[d] = buf[:nn]
+    a, b = out[0.0], out[1.6]
+    assert a.shape == b.shape
+    differ = np.any(a != b, axis=1)
+    assert differ.sum() > 0 and np.array_equal(np.delete(a[differ], [8, 9, 10], 1), np.delete(b[differ], [8, 9, 10], 1))
 
 
 def test_reset_opacities(orc):
