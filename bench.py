@@ -522,6 +522,12 @@ def main():
         out["uninstrumented_iters_per_s"] = args.steps / (time.perf_counter() - u0)
         K = ctx.step_pairs()
         cut = ctx.cut_stats()
+        # the raster kernels' own work on the last step's lists (counting
+        # instantiations of the same kernels)
+        try:
+            kernel_counts = ctx.raster_work_counts()
+        except runtime.RgsError as e:
+            kernel_counts = {"unavailable": str(e)}
         # Algorithmic work, independent of this build's binning: the (pixel,
         # splat) pairs the reference's forward and backward evaluate on its
         # own tile lists (each view rendered once more through the reference-
@@ -576,6 +582,7 @@ def main():
         out["work"] = {"sorted_pairs": K, "front_pairs": cut["front_pairs"], "back_pairs": cut["back_pairs"],
                        "flagged_bins": cut["flagged_tiles"], "fwd_pair_evals": pf, "bwd_pair_evals": pb,
                        "pair_evals_source": "reference tile lists (API render of the trained scene)",
+                       "kernel_counts": kernel_counts,
                        "gaussians": ng, "timed_steps_with_events": n_prof, "of_which_graph_replays": n_graph}
         print(f"[bench] stages {json.dumps(stages)}", file=sys.stderr, flush=True)
 

@@ -234,7 +234,10 @@ int rgs_step_image(rgs_ctx* ctx, float* rgb, size_t n_floats);
  * depend on it, gradients only through the order of float summation).
  * super_tile: side, in 16x16 tiles, of the square patches the raster
  * schedule walks (1..1024; large = row-major), which sets what the blocks
- * resident on one XCD share in L2. sparse_lanes: in the backward, a splat
+ * resident on one XCD share in L2; 0: row-major, except the training
+ * step's backward, which takes its bins longest first (in descending order
+ * of how far down their lists the step's forward composited), so that the
+ * launch ends on short bins. sparse_lanes: in the backward, a splat
  * whose gradient reaches at most this many lanes of a wave is accumulated
  * with LDS atomics instead of a cross-lane reduction (0..64).
  * rgs_autotune_raster renders the registered views ids on the device-resident
@@ -340,6 +343,15 @@ int rgs_debug_tiles(rgs_ctx* ctx, uint64_t* offsets, size_t n_offsets, uint32_t*
  * fused geometry/Adam kernel keeps in registers, so that the step's
  * gradients and its Adam update can each be compared with the CPU path. */
 int rgs_set_grad_capture(rgs_ctx* ctx, int enable);
+/* The work the last rgs_train_step's raster kernels did, counted by running
+ * their counting instantiations on the same lists (images, losses and the
+ * scene are unchanged): counts[0] forward lane evaluations of a (pixel,
+ * splat) pair, [1] forward wave visits of a (splat, 4-row pixel group) that
+ * its alpha-reach test kept, [2] forward composites (alpha >= 1/255), [3]-[5]
+ * the same for the backward (contributing pairs in [5]). Dividing a
+ * profiler's VALU instruction count by [1] or [4] gives instructions per
+ * visit. Not for a step whose depth cut resumed tiles on back lists. */
+int rgs_raster_work_counts(rgs_ctx* ctx, uint64_t* counts);
 int rgs_step_grads(rgs_ctx* ctx, rgs_grad* grads_out, size_t n);
 
 #ifdef __cplusplus

@@ -96,6 +96,16 @@ __device__ __forceinline__ uint32_t wave_mask(float sx, float sy, float ca, floa
   return m;
 }
 
+__device__ __forceinline__ int wave_max_i(int v) {
+  v = max(v, __shfl_xor(v, 1, 64));
+  v = max(v, __shfl_xor(v, 2, 64));
+  v = max(v, __shfl_xor(v, 4, 64));
+  v = max(v, __shfl_xor(v, 8, 64));
+  v = max(v, __shfl_xor(v, 16, 64));
+  v = max(v, __shfl_xor(v, 32, 64));
+  return v;
+}
+
 // The raster takes the quadratic form qf = a dx^2 + 2 b dx dy + c dy^2 of the
 // reference's power = -0.5 qf (summed in the reference's order). The
 // reference skips power > 0, which holds exactly when qf < -2^-149 (the
@@ -140,10 +150,30 @@ __device__ __forceinline__ float clamp_alpha(float alpha_raw, bool* clamped = nu
 // bin flagged by MODE 3 on its back list: the state (C, T, last) comes from
 // the output buffers and list indices continue after the front list.
 constexpr int FT = 64;
+// Adds a wave's work counts to counts[0..2]: lane evaluations (summed over
+// the lanes), wave visits (lane 0's count, the same in every lane) and lane
+// composites.
+__device__ __forceinline__ void count_add(unsigned long long* counts, uint32_t n_eval, uint32_t n_visit,
+                                          uint32_t n_comp, int lane) {
+  unsigned long long e = n_eval, c = n_comp;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    e += __shfl_xor(e, o, 64);
+    c += __shfl_xor(c, o, 64);
+  }
+  if (lane == 0) {
+    atomicAdd(&counts[0], e);
+    atomicAdd(&counts[1], (unsigned long long)n_visit);
+    atomicAdd(&counts[2], c);
+  }
+}
 // NW waves per workgroup: S * S (the bin's tiles share the staged splats) or
 // 1 (one workgroup per tile of the bin, each staging the bin list itself and
 // ending on its own).
-template <int MODE, int S, int NW>
+// COUNT: the same pass, also adding this kernel's own work to in.counts
+// (0: lane evaluations of a (pixel, splat) pair, 1: wave visits of a
+// (splat, pixel group), 2: pairs that composite).
+template <int MODE, int S, int NW, bool COUNT = false>
 __global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_raster_fwd(
     RasterIn in, float* __restrict__ out_rgb, float* __restrict__ out_t, int32_t* __restrict__ out_last) {
   static_assert(NW == 1 || NW == S * S, "waves per workgroup");
@@ -157,8 +187,8 @@ __global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(7, 8)))
   __shared__ uint2 s_rect[FT];
   __shared__ int32_t s_fin[NW];
   const int lane = threadIdx.x & 63;
-  const int lb = xcd_remap(blockIdx.x, gridDim.x);
   // this wave's tile in the bin (wave-uniform), and its LDS slot
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const int w = NW == 1 ? lb % (S * S) : __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int ws = NW == 1 ? 0 : w;
   const int bin = in.tile_order[NW == 1 ? lb / (S * S) : lb];
@@ -216,6 +246,7 @@ __global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(7, 8)))
     if (!inside) T[q] = 0.0f;
   }
   int32_t fin = -1;  // list index at which this wave's last pixel saturated
+  uint32_t n_eval = 0, n_visit = 0, n_comp = 0;
   for (uint32_t b0 = start; b0 < end; b0 += FT) {
     // stop when every pixel of the bin has saturated; the barrier also ends
     // the previous batch's reads of the staged splats
@@ -256,7 +287,9 @@ __global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(7, 8)))
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         if (!(m >> q & 1u)) continue;  // wave-uniform
+        if (COUNT) n_visit += 1;
         if (T[q] < TRANSMITTANCE_STOP) continue;
+        if (COUNT) n_eval += 1;
         float dx = fpx - A.x;
         float dy = (fpy0 + (float)(4 * q)) - A.y;
         const float qf = A.z * dx * dx + 2.0f * A.w * dx * dy + B.x * dy * dy;
@@ -270,6 +303,7 @@ __global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(7, 8)))
           alpha = fmin_(B.y * expf_tab(-0.5f * qf, s_tab), MAX_ALPHA);
         }
         if (alpha < ALPHA_THRESHOLD) continue;
+        if (COUNT) n_comp += 1;
         float weight = alpha * T[q];
         C0[q] += B.z * weight;
         C1[q] += B.w * weight;
@@ -282,6 +316,13 @@ __global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(7, 8)))
         break;
       }
     }
+  }
+  if (COUNT) count_add(in.counts, n_eval, n_visit, n_comp, lane);
+  if (!RGBA && in.tile_cost && bt.ok) {
+    // the tile's cost for the backward's longest-first schedule: how far
+    // down the list its pixels composited (the backward walks that far back)
+    const int lm = wave_max_i(max(max(last[0], last[1]), max(last[2], last[3])) + 1);
+    if (lane == 0) in.tile_cost[bt.tile] = (uint32_t)lm;
   }
   if (lane == 0) s_fin[ws] = fin;
   // (with NW = 1: this tile's pixels)
@@ -409,22 +450,14 @@ __device__ __forceinline__ float row_sum(float v) {
 }
 __device__ __forceinline__ int rev3(int l) { return ((l & 1) << 2) | (l & 2) | ((l >> 2) & 1); }
 
-__device__ __forceinline__ int wave_max_i(int v) {
-  v = max(v, __shfl_xor(v, 1, 64));
-  v = max(v, __shfl_xor(v, 2, 64));
-  v = max(v, __shfl_xor(v, 4, 64));
-  v = max(v, __shfl_xor(v, 8, 64));
-  v = max(v, __shfl_xor(v, 16, 64));
-  v = max(v, __shfl_xor(v, 32, 64));
-  return v;
-}
-
 // Backward: four pixels per lane as in the forward. A lane sums its four
 // pixels' gradients for a splat in registers, the wave reduces them across
 // lanes into its LDS slot, and after each batch the bin adds its waves' sums
 // to the accumulators (one atomic per value and splat per bin).
 constexpr int BT = 64;
-template <int S>
+// COUNT: adds this kernel's own work to in.counts (3: lane evaluations, 4:
+// wave visits, 5: contributing pairs) and writes no accumulator.
+template <int S, bool COUNT = false>
 __global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_raster_bwd(RasterIn in, const float* __restrict__ final_t,
                                                            const int32_t* __restrict__ last_contrib,
                                                            const float* __restrict__ dl_dpix,
@@ -444,7 +477,7 @@ __global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(7, 8
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // wave-uniform
   if (threadIdx.x < 32) s_tab[threadIdx.x] = kExp2Tab[threadIdx.x];
-  const int bin = in.tile_order[xcd_remap(blockIdx.x, gridDim.x)];
+  const int bin = in.lpt_bins ? in.lpt_bins[blockIdx.x] : in.tile_order[xcd_remap(blockIdx.x, gridDim.x)];
   const BinTile bt = bin_tile<S>(in, bin, w);
   const CamDev& c = in.cams[bt.v];
   const int tx = bt.tx, ty = bt.ty;
@@ -506,6 +539,7 @@ __global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(7, 8
   // LDS slot offset of the value this lane adds after a row reduction
   const int lr = lane & 15;
   const int g_slot = (lr < 8 ? rev3(lr) : 8) * (BT + 1);
+  uint32_t n_eval = 0, n_visit = 0, n_comp = 0;
   for (int hi = top; hi >= 0; hi -= BT) {
     const int lo = max(0, hi - (BT - 1));
     const int nb = hi - lo + 1;
@@ -543,7 +577,9 @@ __global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(7, 8
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         if (!(m >> q & 1u)) continue;  // wave-uniform
+        if (COUNT) n_visit += 1;
         if (li > lc[q]) continue;
+        if (COUNT) n_eval += 1;
         float dy = (fpy0 + (float)(4 * q)) - A.y;
         const float qf = A.z * dx * dx + 2.0f * A.w * dx * dy + B.x * dy * dy;
         if (qf < POWER_POSITIVE_QF) continue;  // power > 0; NaN proceeds, as in the reference
@@ -552,6 +588,7 @@ __global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(7, 8
         bool clamped;
         const float alpha = clamp_alpha(alpha_raw, &clamped);
         if (!(alpha >= ALPHA_THRESHOLD)) continue;
+        if (COUNT) n_comp += 1;
         contrib = true;
         {
         // Gradient arithmetic only (the decisions above are uncontracted,
@@ -647,9 +684,105 @@ __global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(7, 8
             val = o == 7 ? A.z * s1 + A.w * s2 : A.w * s1 + cc * s2;
           }
         }
-        if (val != 0.0f) atomicAdd(&accum[(size_t)o * in.acc_stride + vi], val);
+        if (!COUNT && val != 0.0f) atomicAdd(&accum[(size_t)o * in.acc_stride + vi], val);
       }
     }
+  }
+  if (COUNT) count_add(in.counts + 3, n_eval, n_visit, n_comp, lane);
+}
+
+// ---- longest-first schedule of the backward
+// Orders the bins by cost (the largest tile cost among their tiles: the
+// bin's waves walk that far back), descending, by a counting sort over
+// LPT_BUCKETS buckets scaled to the largest cost (order inside a bucket is
+// arbitrary). Every bin appears exactly once, so the table is a permutation
+// whatever the costs hold. Block b of the backward then takes the b-th
+// costliest bin; the hardware deals blocks to the XCDs in turn, so each XCD
+// gets a descending share and the launch ends on short bins.
+constexpr int LPT_THREADS = 1024;
+constexpr int LPT_BUCKETS = 4096;
+__global__ __launch_bounds__(LPT_THREADS) void k_lpt_order(const uint32_t* __restrict__ tile_cost,
+                                                           const CamDev* __restrict__ cams, int n_views,
+                                                           int total_bins, int shift, int32_t* __restrict__ lpt_bins) {
+  __shared__ uint32_t s_cnt[LPT_BUCKETS];
+  __shared__ uint32_t s_max[LPT_THREADS / 64];
+  __shared__ uint32_t s_part[LPT_THREADS / 64];
+  const int n = total_bins;
+  const int t = threadIdx.x;
+  auto cost = [&](int id) -> uint32_t {
+    const int v = view_of_bin(cams, n_views, id);
+    const CamDev& c = cams[v];
+    const int lb = id - c.bin_base, S = 1 << shift;
+    const int bx = lb % c.nbx, by = lb / c.nbx;
+    uint32_t m = 0;
+    for (int j = 0; j < S; ++j)
+      for (int i = 0; i < S; ++i) {
+        const int tx = bx * S + i, ty = by * S + j;
+        if (tx < c.ntx && ty < c.nty) m = max(m, tile_cost[c.tile_base + ty * c.ntx + tx]);
+      }
+    return m;
+  };
+  for (int k = t; k < LPT_BUCKETS; k += LPT_THREADS) s_cnt[k] = 0;
+  uint32_t m = 0;
+  for (int id = t; id < n; id += LPT_THREADS) m = max(m, cost(id));
+  for (int o = 32; o >= 1; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
+  if ((t & 63) == 0) s_max[t >> 6] = m;
+  __syncthreads();
+  uint32_t cmax = 0;
+  for (int k = 0; k < LPT_THREADS / 64; ++k) cmax = max(cmax, s_max[k]);
+  // bucket LPT_BUCKETS - 1 - floor(cost (LPT_BUCKETS - 1) / cmax): costliest first
+  auto bucket = [&](uint32_t c) -> int {
+    return cmax ? LPT_BUCKETS - 1 - (int)(((uint64_t)c * (LPT_BUCKETS - 1)) / cmax) : 0;
+  };
+  for (int id = t; id < n; id += LPT_THREADS) atomicAdd(&s_cnt[bucket(cost(id))], 1u);
+  __syncthreads();
+  // exclusive scan of the bucket counts: each thread owns PER consecutive buckets
+  constexpr int PER = LPT_BUCKETS / LPT_THREADS;
+  uint32_t v[PER], sum = 0;
+  for (int k = 0; k < PER; ++k) {
+    v[k] = s_cnt[t * PER + k];
+    sum += v[k];
+  }
+  uint32_t x = sum;
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)x, d, 64);
+    if ((t & 63) >= d) x += y;
+  }
+  if ((t & 63) == 63) s_part[t >> 6] = x;
+  __syncthreads();
+  uint32_t base = x - sum;
+  for (int k = 0; k < (t >> 6); ++k) base += s_part[k];
+  for (int k = 0; k < PER; ++k) {
+    s_cnt[t * PER + k] = base;
+    base += v[k];
+  }
+  __syncthreads();
+  for (int id = t; id < n; id += LPT_THREADS) lpt_bins[atomicAdd(&s_cnt[bucket(cost(id))], 1u)] = id;
+}
+
+void launch_lpt_order(const uint32_t* tile_cost, const CamDev* cams, int n_views, int total_bins, int shift,
+                      int32_t* lpt_bins, hipStream_t st) {
+  if (total_bins <= 0) return;
+  hipLaunchKernelGGL(k_lpt_order, dim3(1), dim3(LPT_THREADS), 0, st, tile_cost, cams, n_views, total_bins, shift,
+                     lpt_bins);
+}
+
+void launch_raster_counts(const RasterIn& in, float* rgb, float* final_t, int32_t* last, const float* dl_dpix,
+                          hipStream_t st) {
+  if (in.total_bins <= 0 || !in.counts) return;
+  if (in.bin_shift == 0) {
+    hipLaunchKernelGGL((k_raster_fwd<3, 1, 1, true>), dim3(in.total_bins), dim3(FT), 0, st, in, rgb, final_t, last);
+    hipLaunchKernelGGL((k_raster_bwd<1, true>), dim3(in.total_bins), dim3(BT), 0, st, in, final_t, last, dl_dpix,
+                       nullptr);
+  } else {
+    if (in.fwd_tile_blocks)
+      hipLaunchKernelGGL((k_raster_fwd<3, 2, 1, true>), dim3(in.total_bins * 4), dim3(FT), 0, st, in, rgb, final_t,
+                         last);
+    else
+      hipLaunchKernelGGL((k_raster_fwd<3, 2, 4, true>), dim3(in.total_bins), dim3(FT * 4), 0, st, in, rgb, final_t,
+                         last);
+    hipLaunchKernelGGL((k_raster_bwd<2, true>), dim3(in.total_bins), dim3(BT * 4), 0, st, in, final_t, last, dl_dpix,
+                       nullptr);
   }
 }
 

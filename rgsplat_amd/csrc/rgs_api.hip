@@ -307,12 +307,16 @@ struct rgs_ctx {
   std::string order_key[2];
   int bin_shift = 1;       // training step: bins of 2^shift x 2^shift tiles
   int fwd_tile_blocks = 1; // training forward with bins: a workgroup per tile (1) or per bin (0)
-  int super_tile = 8;      // tiles per side of a schedule super-tile
+  int super_tile = 8;      // tiles per side of a schedule super-tile; 0: the backward longest first
+  // longest-first schedule of the training step's backward: per-tile costs
+  // of the step's forward and the bins ordered by them
+  DBuf tile_cost, lpt_bins;
   int sparse_lanes = 2;    // backward: LDS atomics up to this many lanes
   float* scratch_dev = nullptr;  // this device's copy of the no-alloc scratch array
   float split_div = 0.0f;         // densify: 0 the reference's split, > 0 every scale / split_div
   bool grad_capture = false;  // the step writes its summed gradients into gcap
   DBuf gcap;                  // 60 rows x cap: gradient rows 0-58, screen-gradient norm
+  DBuf work_counts;           // rgs_raster_work_counts: 6 u64
 
   // stage timing
   bool profiling = false;
@@ -378,6 +382,27 @@ int ensure_tile_order(rgs_ctx* c, const ViewSet& vs) {
   ++c->gen;  // graphs captured with the old schedule are stale
   c->order_key[sh] = key;
   return 0;
+}
+
+// Buffers of the backward's longest-first schedule for the view set vs (the
+// forward of the same step fills the costs and the order kernel the table
+// before the backward reads them).
+int ensure_lpt(rgs_ctx* c, const ViewSet& vs) {
+  CHK(ensure(c, c->tile_cost, (size_t)std::max(vs.total_tiles, 1) * 4));
+  CHK(ensure(c, c->lpt_bins, (size_t)std::max(vs.total_bins, 1) * 4));
+  return 0;
+}
+// The forward records tile costs (when the schedule is longest first).
+void lpt_forward(rgs_ctx* c, RasterIn& in) {
+  if (c->super_tile == 0) in.tile_cost = P_<uint32_t>(c->tile_cost);
+}
+// Orders the bins for the backward and points its input at the table.
+void lpt_backward(rgs_ctx* c, const ViewSet& vs, RasterIn& in) {
+  if (c->super_tile != 0) return;
+  launch_lpt_order(P_<uint32_t>(c->tile_cost), P_<CamDev>(c->cams_d), (int)vs.cams.size(), vs.total_bins, vs.shift,
+                   P_<int32_t>(c->lpt_bins), c->st);
+  in.tile_cost = nullptr;
+  in.lpt_bins = P_<int32_t>(c->lpt_bins);
 }
 
 // Sizes every buffer the render sequence touches for n Gaussians (row
@@ -588,6 +613,7 @@ void enqueue_bin_and_raster_cut(rgs_ctx* c, size_t cap, int n, const ViewSet& vs
   in.tile_flag = P_<uint8_t>(c->tile_flag);
   in.n_flagged = sc + SC_NFLAG;
   in.zneed = &cs->zneed;
+  lpt_forward(c, in);
   launch_fill_u32(P_<uint32_t>(c->flag), ((size_t)vs.total_bins + 3) / 4, 0u, st);
   launch_fill_u32(P_<uint32_t>(c->tile_flag), ((size_t)vs.total_tiles + 3) / 4, 0u, st);
   stage_begin(c, S_FWD);
@@ -807,7 +833,7 @@ void rgs_destroy(rgs_ctx* c) {
                  &c->pv_rect, &c->pv_cnt, &c->pv_off, &c->pv_bs, &c->pv_misc, &c->pv_pk[0], &c->pv_pk[1],
                  &c->pv_pv[0], &c->pv_pv[1], &c->pv_hist, &c->pv_rowtot, &c->pv_rs, &c->pv_re, &c->pv_rgb,
                  &c->rs_taps, &c->rs_tmp, &c->rs_src, &c->rs_dst, &c->pv_blk,
-                 &c->adam_d, &c->api_acc9, &c->tile_order[0], &c->tile_order[1], &c->cover, &c->depth,
+                 &c->adam_d, &c->api_acc9, &c->tile_order[0], &c->tile_order[1], &c->tile_cost, &c->lpt_bins, &c->cover, &c->depth,
                  &c->flag, &c->tile_flag, &c->rbstart, &c->rbend, &c->pv2, &c->cntb, &c->cutst, &c->part,
                  &c->kB, &c->vB, &c->kC[0], &c->kC[1], &c->vC[0], &c->vC[1]};
   for (DBuf* b : all)
@@ -840,7 +866,8 @@ int rgs_use_own_stream(rgs_ctx* c) {
 
 int rgs_set_raster_params(rgs_ctx* c, int super_tile, int sparse_lanes) {
   if (!c) return fail(RGS_E_ARG, "null ctx");
-  if (super_tile < 1 || super_tile > 1024) return fail(RGS_E_ARG, "super_tile must be in [1, 1024]");
+  if (super_tile < 0 || super_tile > 1024)
+    return fail(RGS_E_ARG, "super_tile must be 0 (longest first) or in [1, 1024]");
   if (sparse_lanes < 0 || sparse_lanes > 64) return fail(RGS_E_ARG, "sparse_lanes must be in [0, 64]");
   if (super_tile != c->super_tile || sparse_lanes != c->sparse_lanes) drop_graph(c);
   c->super_tile = super_tile;
@@ -1990,6 +2017,7 @@ static void enqueue_step_rest(rgs_ctx* c, const ViewSet& vs, const float bg[3], 
   stage_end(c, S_LOSS);
   stage_begin(c, S_BWD);
   RasterIn in = raster_in(c, vs, bg);
+  lpt_backward(c, vs, in);
   in.acc_stride = vs.cams.size() * cap;
   if (with_back) {
     in.bstart = P_<uint32_t>(c->rbstart);
@@ -2062,6 +2090,7 @@ static int train_step_impl(rgs_ctx* c, const int* ids, int n_ids, const rgs_trai
   bool fresh_accum = c->accum.bytes < acc_n * 4;
   CHK(ensure(c, c->accum, acc_n * 4));
   CHK(ensure(c, c->cover, ((size_t)vs.total_tiles + 3) / 4 * 4));
+  if (c->super_tile == 0) CHK(ensure_lpt(c, vs));
   if (!c->cutst.p) {
     CHK(ensure(c, c->cutst, sizeof(CutState)));
     CHK(write_cut_state(c));
@@ -2169,6 +2198,7 @@ static int train_step_impl(rgs_ctx* c, const int* ids, int n_ids, const rgs_trai
     c->last_ntiles = vs.total_tiles;
     c->last_pix = vs.total_pix;
     c->last_nviews = (int)vs.cams.size();
+    for (int k = 0; k < 3; ++k) c->last_bg[k] = bg[k];
     if (c->profiling) {
       for (int s = 0; s < RGS_N_STAGES; ++s) {
         if (!(c->prof_mask >> s & 1u)) continue;
@@ -2226,12 +2256,15 @@ int rgs_autotune_raster(rgs_ctx* c, const int* ids, int n_ids, const float bg[3]
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   auto time_once = [&](float* ms) -> int {
     CHK(ensure_tile_order(c, vs));
+    if (c->super_tile == 0) CHK(ensure_lpt(c, vs));
     RasterIn in = raster_in(c, vs, bg);
     in.acc_stride = vs.cams.size() * cap;
+    lpt_forward(c, in);
     HIPCHK(hipEventRecord(ev[0], c->st));
     launch_raster_fwd(in, 3, P_<float>(c->rgb), P_<float>(c->final_t), P_<int32_t>(c->last), c->st);
     HIPCHK(hipEventRecord(ev[1], c->st));
     HIPCHK(hipEventRecord(ev[2], c->st));
+    lpt_backward(c, vs, in);
     launch_raster_bwd(in, P_<float>(c->final_t), P_<int32_t>(c->last), P_<float>(c->dl), P_<float>(c->accum), c->st);
     HIPCHK(hipEventRecord(ev[3], c->st));
     HIPCHK(hipStreamSynchronize(c->st));
@@ -2256,7 +2289,7 @@ int rgs_autotune_raster(rgs_ctx* c, const int* ids, int n_ids, const float bg[3]
   float best_ms = 0.0f;
   int rc = score(best_st, best_sl, &best_ms);
   // coordinate search: schedule patch size, then the sparse/dense switch
-  const int st_cand[] = {2, 4, 8, 16, 1024};
+  const int st_cand[] = {0, 2, 4, 8, 16, 1024};
   const int sl_cand[] = {0, 2, 4, 8, 16, 32};
   for (int st_ : st_cand) {
     if (rc) break;
@@ -2447,6 +2480,30 @@ int rgs_set_split_scale_divisor(rgs_ctx* c, float divisor) {
   if (!(divisor == 0.0f || (divisor > 1.0f && divisor < 1e6f)))
     return fail(RGS_E_ARG, "split scale divisor must be 0 (the reference's split) or in (1, 1e6)");
   c->split_div = divisor;
+  return 0;
+}
+
+int rgs_raster_work_counts(rgs_ctx* c, uint64_t* counts) {
+  if (!c || !counts) return fail(RGS_E_ARG, "null argument");
+  if (!c->last_cut) return fail(RGS_E_ARG, "no training step rendered yet");
+  if (c->last_back_valid && c->last_back_pairs)
+    return fail(RGS_E_ARG, "the last step resumed tiles on back lists; the counts cover front lists only");
+  (void)hipSetDevice(c->device);
+  CHK(ensure(c, c->work_counts, 6 * sizeof(unsigned long long)));
+  launch_fill_u32(P_<uint32_t>(c->work_counts), 12, 0u, c->st);
+  RasterIn in = raster_in_n(c, c->last_nviews, c->last_tiles, c->last_shift, c->last_bg);
+  in.acc_stride = (size_t)c->last_nviews * c->last_cap;
+  in.cover = step_cover(c, c->last_bg);
+  if (c->last_back_valid) {
+    in.bstart = P_<uint32_t>(c->rbstart);
+    in.bend = P_<uint32_t>(c->rbend);
+    in.back_vals = c->back_vals;
+  }
+  in.counts = P_<unsigned long long>(c->work_counts);
+  launch_raster_counts(in, P_<float>(c->rgb), P_<float>(c->final_t), P_<int32_t>(c->last), P_<float>(c->dl), c->st);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(counts, c->work_counts.p, 6 * sizeof(uint64_t), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
   return 0;
 }
 

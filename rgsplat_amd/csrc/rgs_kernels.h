@@ -201,6 +201,12 @@ struct RasterIn {
   const uint32_t* bstart;     // back lists: ranges and values
   const uint32_t* bend;
   const uint32_t* back_vals;
+  unsigned long long* counts; // launch_raster_counts only: 6 work counters
+  // longest-first schedule of the backward (null: the static tile_order):
+  // the forward writes each tile's cost (how far down its list it
+  // composited); backward block b takes bin lpt_bins[b]
+  uint32_t* tile_cost;
+  const int32_t* lpt_bins;
 };
 // mode 0: training forward (rgb = C + T bg; final T; last contributor); with
 //   cut_on set, a tile whose (front) list ends before all its pixels
@@ -216,6 +222,20 @@ void launch_raster_fwd(const RasterIn& in, int mode, float* rgb, float* final_t,
 // lines (L2 channels) instead of queueing on one.
 void launch_raster_bwd(const RasterIn& in, const float* final_t, const int32_t* last, const float* dl_dpix,
                        float* accum, hipStream_t st);
+
+// The backward's longest-first bin table from the tile costs of a forward:
+// lpt_bins (total_bins entries), a permutation, costliest bin first.
+void launch_lpt_order(const uint32_t* tile_cost, const CamDev* cams, int n_views, int total_bins, int shift,
+                      int32_t* lpt_bins, hipStream_t st);
+
+// The training step's forward (mode 3) and backward raster again, counting
+// their own work into in.counts: [0] forward lane evaluations of a (pixel,
+// splat) pair, [1] forward wave visits of a (splat, 4-row pixel group), [2]
+// forward composites, [3]-[5] the same for the backward (contributing pairs
+// in [5]). The forward rewrites its outputs with the same values (in.zneed
+// and in.cut_on must be null); the backward writes no accumulator.
+void launch_raster_counts(const RasterIn& in, float* rgb, float* final_t, int32_t* last, const float* dl_dpix,
+                          hipStream_t st);
 
 // out[0] += forward pair evaluations, out[1] += backward pair evaluations
 // (list entries, of the lists the render used); one block per tile.

@@ -111,6 +111,7 @@ SIGNATURES = {
     "rgs_stage_times": [_vp, _vp, _i, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)],
     "rgs_debug_tiles": [_vp, _vp, _sz, _vp, _sz, ctypes.POINTER(ctypes.c_uint64)],
     "rgs_set_grad_capture": [_vp, _i],
+    "rgs_raster_work_counts": [_vp, _vp],
     "rgs_set_split_scale_divisor": [_vp, _f],
     "rgs_loss_grad_dev_noalloc": [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32, _f, _vp, _vp],
     "rgs_step_grads": [_vp, _vp, _sz],
@@ -571,6 +572,15 @@ class Context:
         out = np.zeros_like(x)
         _check(lib().rgs_eval_expf(self.h, _p(x), _p(out), len(x)))
         return out
+
+    def raster_work_counts(self) -> dict:
+        """The last train_step's raster work, counted by the kernels
+        (rgs_raster_work_counts)."""
+        out = np.zeros(6, np.uint64)
+        _check(lib().rgs_raster_work_counts(self.h, _p(out)))
+        keys = ["fwd_lane_evals", "fwd_wave_visits", "fwd_composites", "bwd_lane_evals", "bwd_wave_visits",
+                "bwd_contributions"]
+        return dict(zip(keys, (int(x) for x in out)))
 
     def set_grad_capture(self, enable: bool):
         """train_step also keeps the gradients its Adam update consumed."""

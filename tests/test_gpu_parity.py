@@ -641,7 +641,7 @@ def test_raster_params_and_autotune(ctx):
     ref_loss, ref_g = _oracle_step_grads(g, cams, targets, (0, 0, 0), 0.2)
     c = runtime.Context(0)
     assert c.raster_params() == (8, 2)
-    for st, sl in [(1, 0), (1024, 64), (3, 5)]:
+    for st, sl in [(1, 0), (1024, 64), (3, 5), (0, 2)]:
         c.set_raster_params(st, sl)
         assert c.raster_params() == (st, sl)
         img = c.render_color(g, cams[1])
@@ -649,14 +649,14 @@ def test_raster_params_and_autotune(ctx):
         _, gr = c.render_and_backward(g, cams[0], targets[0])
         _, ref0 = oracle.render_and_backward(g, cams[0], targets[0])
         assert_grads_close(gr, ref0, 1e-3)
-    for bad in [(0, 2), (8, 65), (2000, 2)]:
+    for bad in [(-1, 2), (8, 65), (2000, 2)]:
         with pytest.raises(runtime.RgsError):
             c.set_raster_params(*bad)
     c.scene_upload(g)
     c.set_views(cams, targets)
     best = c.autotune_raster([0, 1])
     # a candidate, or the setting it started from (3, 5) when nothing beat it
-    assert best["super_tile"] in (2, 4, 8, 16, 1024, 3) and best["sparse_lanes"] in (0, 2, 4, 8, 16, 32, 5)
+    assert best["super_tile"] in (0, 2, 4, 8, 16, 1024, 3) and best["sparse_lanes"] in (0, 2, 4, 8, 16, 32, 5)
     assert best["raster_ms"] > 0
     assert c.raster_params() == (best["super_tile"], best["sparse_lanes"])
     assert np.array_equal(c.scene_download(), g)
@@ -687,3 +687,73 @@ def test_create_on_a_missing_device():
         g, cams, _ = workload("c0_10k_256", 200, [(40, 24)])
         assert np.array_equal(c.render_color(g, cams[0]), oracle.render_color(g, cams[0]))
         c.close()
+
+
+def test_raster_work_counts():
+    """The training step's raster kernels counting their own work: the
+    backward revisits exactly the (pixel, splat) pairs the forward
+    composited (the same alpha decisions), lane evaluations lie between the
+    composites and the reference's evaluated pairs on its own lists, and the
+    counting pass changes no image."""
+    from rgsplat_amd import runtime
+    g, cams, targets = workload("c3_1m_1080p_4views", 60_000, [(256, 192), (224, 160)])
+    cfg = S.benchmark_train_config()
+    c = runtime.Context(0)
+    c.set_depth_cut(1)
+    c.scene_upload(g)
+    c.set_views(cams, targets)
+    g0 = c.scene_download()
+    c.train_step([0, 1], cfg, use_graph=True)
+    npx = sum(cm.width * cm.height for cm in cams)
+    img = c.step_image(npx)
+    k = c.raster_work_counts()
+    assert np.array_equal(c.step_image(npx), img)
+    assert k["fwd_composites"] == k["bwd_contributions"] > 0
+    assert k["fwd_composites"] <= k["fwd_lane_evals"] <= 64 * k["fwd_wave_visits"]
+    assert k["bwd_contributions"] <= k["bwd_lane_evals"] <= 64 * k["bwd_wave_visits"]
+    ref_pairs = sum(oracle.Render(g0, cm, None, backward=False).pairs_fwd for cm in cams)
+    assert k["fwd_lane_evals"] <= ref_pairs
+    # a step that resumed tiles on back lists is refused, not miscounted
+    c.set_depth_cut(2, 2.0)
+    c.train_step([0, 1], cfg, use_graph=False)
+    if c.cut_stats()["back_pairs"]:
+        with pytest.raises(runtime.RgsError):
+            c.raster_work_counts()
+    c.close()
+
+
+@pytest.mark.parametrize("fwd_tile_blocks", [1, 0])
+def test_longest_first_schedule(fwd_tile_blocks):
+    """The backward's longest-first schedule (super_tile 0: bins in
+    descending order of the step's forward walk lengths) against the static
+    one over four graph-replayed steps of three uneven views: the same images
+    and losses bit for bit on the first step, gradients to the atomics'
+    summation order, and the gradients against the oracle."""
+    from rgsplat_amd import runtime
+    g, cams, targets = workload("c2_500k_8views", 30_000, [(160, 120), (240, 136), (320, 180)])
+    cfg = S.benchmark_train_config()
+    npx = sum(c.width * c.height for c in cams)
+    out = {}
+    for st in (8, 0):
+        c = runtime.Context(0)
+        c.set_train_binning(1, fwd_tile_blocks)
+        c.set_raster_params(st, 2)
+        c.scene_upload(g)
+        c.set_views(cams, targets)
+        c.set_grad_capture(True)
+        rec = []
+        for _ in range(4):
+            g_before = c.scene_download()
+            loss = c.train_step([0, 1, 2], cfg, use_graph=True)
+            rec.append((g_before, loss, c.step_image(npx), c.step_grads()))
+        out[st] = rec
+        c.close()
+    for (g0, l0, i0, gr0), (g1, l1, i1, gr1) in zip(out[8], out[0]):
+        assert np.abs(g0 - g1).max() < 1e-5
+    # the first step's scenes are equal: bit-identical forward
+    assert np.array_equal(out[8][0][1], out[0][0][1]) and np.array_equal(out[8][0][2], out[0][0][2])
+    for k in range(4):
+        gs, gl = out[8][k][3].astype(np.float64), out[0][k][3].astype(np.float64)
+        assert np.linalg.norm(gl - gs) <= 1e-5 * np.linalg.norm(gs), k
+    ref_loss, ref_g = _oracle_step_grads(out[0][3][0], cams, targets, (0, 0, 0), 0.2)
+    assert_grads_close(out[0][3][3], ref_g.astype(np.float32), 1e-3)
