@@ -542,12 +542,20 @@ def main():
         del g_now
         npix = sum(w * h for w, h in wl.sizes)
         ng = ctx.scene_size()
+        # the tile sort's LSD passes: 8 bits each over the global 32x32-bin ids
+        n_bins = sum(((w + 15) // 16 + 1) // 2 * (((h + 15) // 16 + 1) // 2) for w, h in wl.sizes)
+        sort_passes = max(1, (int(n_bins - 1).bit_length() + 7) // 8)
         work = {
             "raster_fwd": ("valu", FWD_OPS_PER_PAIR * pf, "TFLOP/s"),
             "raster_bwd": ("valu", BWD_OPS_PER_PAIR * pb, "TFLOP/s"),
             "loss_ssim": ("valu", SSIM_FLOP_PER_PIXEL * npix, "TFLOP/s"),
             "project": ("hbm", 236 * ng + 60 * ng * len(cams), "GB/s"),
             "geometry_adam": ("hbm", 1652 * ng, "GB/s"),
+            # pair generation writes 8 B per (bin, splat) pair; each tile-sort
+            # pass reads and writes key and value (16 B) and the histogram
+            # pass reads the key again (4 B)
+            "pair_gen": ("hbm", 8 * cut["front_pairs"], "GB/s"),
+            "tile_sort": ("hbm", 20 * sort_passes * cut["front_pairs"], "GB/s"),
         }
         traffic = measured_traffic(wl.name)
         rl = {}
@@ -616,7 +624,7 @@ def main():
         out["api_gpu_path_iters_per_s"] = plain_path_api(ctx, g, cams, targets, cfg)
         out["api_gpu_path"] = ("render_and_backward per view with host buffers + adam_step (PCIe-bound: "
                                "parameters and gradients cross PCIe per view)")
-    if "cpu_baseline" in out:
+    if "cpu_baseline" in out and check_cams is cams:
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     if ctx is not None:
         ctx.close()
