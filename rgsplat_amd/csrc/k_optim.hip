@@ -23,7 +23,7 @@ __device__ __forceinline__ int aos_offset(int j) {
 // Per view: SH basis at the view direction and the clamp-masked dL/dcolour
 // (backward.rs:533-544, sh.rs:39-61,122-139).
 template <class Load>
-__device__ __forceinline__ void view_basis(const CamDev& c, Load prm, float basis[16]) {
+__device__ __forceinline__ void view_basis(const CamDev& c, Load prm, float basis[16], float dir[3] = nullptr) {
   const float px = prm(0), py = prm(1), pz = prm(2);
   float u0 = c.cam_pos[0] - px, u1 = c.cam_pos[1] - py, u2 = c.cam_pos[2] - pz;
   float ulen = __builtin_sqrtf(u0 * u0 + u1 * u1 + u2 * u2);
@@ -34,6 +34,7 @@ __device__ __forceinline__ void view_basis(const CamDev& c, Load prm, float basi
     d0 = u0 / ulen; d1 = u1 / ulen; d2 = u2 / ulen;
   }
   sh_basis(d0, d1, d2, basis);
+  if (dir) dir[0] = d0, dir[1] = d1, dir[2] = d2;
 }
 // bits: bit ch set when channel ch's colour is inside the clamp (0, 1).
 // Also dk[k] = sum_ch masked[ch] * sh[ch][k], the colour gradient contracted
@@ -105,14 +106,6 @@ struct GeomConst {
 // the upper triangle is kept; doubled() gives dsig + dsig^T.
 struct SymSum {
   float u[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};  // 00 01 02 11 12 22
-  __device__ void add(const M3& m) {
-    u[0] += m.a[0][0];
-    u[1] += m.a[0][1];
-    u[2] += m.a[0][2];
-    u[3] += m.a[1][1];
-    u[4] += m.a[1][2];
-    u[5] += m.a[2][2];
-  }
   __device__ M3 doubled() const {
     M3 r;
     r.a[0][0] = u[0] + u[0];
@@ -136,34 +129,72 @@ __device__ __forceinline__ GeomConst geom_const(Load prm) {
   return G;
 }
 
+// 3-D covariance, upper triangle (00 01 02 11 12 22): Sigma = (R S)(R S)^T
+// with covariance_3d's quaternion normalisation and rotation formula
+// (math/mod.rs:89-96); R S formed column by column (S is diagonal).
+struct Sym3 {
+  float m[6];
+};
+__device__ __forceinline__ Sym3 cov_sym(const GeomConst& G) {
+#pragma clang fp contract(fast)
+  const float qw = G.q[0], qx = G.q[1], qy = G.q[2], qz = G.q[3];
+  const float in = rcp_fast(sqrt_fast((qx * qx + qz * qz) + (qy * qy + qw * qw)));
+  const float i = qx * in, j = qy * in, k = qz * in, w = qw * in;
+  const float ww = w * w, ii = i * i, jj = j * j, kk = k * k;
+  const float ij = i * j * 2.0f, wk = w * k * 2.0f, wj = w * j * 2.0f;
+  const float ik = i * k * 2.0f, jk = j * k * 2.0f, wi = w * i * 2.0f;
+  const float s0 = G.scale[0], s1 = G.scale[1], s2 = G.scale[2];
+  const float r0[3] = {(ww + ii - jj - kk) * s0, (ij - wk) * s1, (wj + ik) * s2};
+  const float r1[3] = {(ij + wk) * s0, (ww - ii + jj - kk) * s1, (jk - wi) * s2};
+  const float r2[3] = {(ik - wj) * s0, (wi + jk) * s1, (ww - ii - jj + kk) * s2};
+  Sym3 S;
+  S.m[0] = r0[0] * r0[0] + r0[1] * r0[1] + r0[2] * r0[2];
+  S.m[1] = r0[0] * r1[0] + r0[1] * r1[1] + r0[2] * r1[2];
+  S.m[2] = r0[0] * r2[0] + r0[1] * r2[1] + r0[2] * r2[2];
+  S.m[3] = r1[0] * r1[0] + r1[1] * r1[1] + r1[2] * r1[2];
+  S.m[4] = r1[0] * r2[0] + r1[1] * r2[1] + r1[2] * r2[2];
+  S.m[5] = r2[0] * r2[0] + r2[1] * r2[1] + r2[2] * r2[2];
+  return S;
+}
+
+__device__ __forceinline__ float dot3(const float* a, const float* b) {
+#pragma clang fp contract(fast)
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
 // One view: adds dL/dSigma to dsig, dL/dposition to dpos, the screen-gradient
-// norm to sgn (backward.rs:546-636 without the covariance backprop).
-template <class Load>
-__device__ __forceinline__ void geometry_view(const CamDev& c, Load prm, const GeomConst& G, const float acc[9],
+// norm to sgn (backward.rs:546-636 without the covariance backprop). The
+// EWA Jacobian J has zero third row, so only rows T0, T1 of T = J W enter:
+// the 2-D covariance is (T_r . Sigma T_s) for r, s in {0, 1}, dL/dSigma is
+// T^T G T with G the 2x2 loss gradient, and each p_cam component's dJ
+// (projection.rs:47-61) touches at most two entries of those rows, so the
+// position path needs only the six products W_r . Sigma T_s.
+__device__ __forceinline__ void geometry_view(const CamDev& c, const float p[3], const Sym3& S, const float acc[9],
                                               const float dd[3], SymSum& dsig, float dpos[3], float& sgn) {
 #pragma clang fp contract(fast)
-  const float px = prm(0), py = prm(1), pz = prm(2);
   float pc[3];
-  to_camera(c, px, py, pz, pc);
+  to_camera(c, p[0], p[1], p[2], pc);
   const float tx = pc[0], ty = pc[1], tz = pc[2];
-
-  // 2-D covariance as projected (projection.rs:24-69); the 3-D covariance
-  // is rebuilt per view rather than held across views
-  const M3 sigma = covariance_3d(G.q[0], G.q[1], G.q[2], G.q[3], G.scale[0], G.scale[1], G.scale[2]);
   const float fx = c.fx, fy = c.fy;
   const float itz = rcp_fast(tz), itz2 = itz * itz, itz3 = itz2 * itz;
-  M3 W;
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) W.a[i][j] = c.view[i * 4 + j];
-  // T = J W (projection.rs:47-61)
-  M3 J = m3_zero();
-  J.a[0][0] = fx * itz;
-  J.a[0][2] = -fx * tx * itz2;
-  J.a[1][1] = fy * itz;
-  J.a[1][2] = -fy * ty * itz2;
-  const M3 T = m3_mul(J, W);
-  M3 s2d = m3_mul(m3_mul(T, sigma), m3_t(T));
-  const float a2 = s2d.a[0][0] + 0.3f, b2 = s2d.a[0][1], c2 = s2d.a[1][1] + 0.3f;
+  const float* W0 = c.view;
+  const float* W1 = c.view + 4;
+  const float* W2 = c.view + 8;
+  const float j00 = fx * itz, j02 = -fx * tx * itz2, j11 = fy * itz, j12 = -fy * ty * itz2;
+  float T0[3], T1[3], u0[3], u1[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    T0[m] = j00 * W0[m] + j02 * W2[m];
+    T1[m] = j11 * W1[m] + j12 * W2[m];
+  }
+  // u_r = Sigma T_r
+  u0[0] = S.m[0] * T0[0] + S.m[1] * T0[1] + S.m[2] * T0[2];
+  u0[1] = S.m[1] * T0[0] + S.m[3] * T0[1] + S.m[4] * T0[2];
+  u0[2] = S.m[2] * T0[0] + S.m[4] * T0[1] + S.m[5] * T0[2];
+  u1[0] = S.m[0] * T1[0] + S.m[1] * T1[1] + S.m[2] * T1[2];
+  u1[1] = S.m[1] * T1[0] + S.m[3] * T1[1] + S.m[4] * T1[2];
+  u1[2] = S.m[2] * T1[0] + S.m[4] * T1[1] + S.m[5] * T1[2];
+  const float a2 = dot3(T0, u0) + 0.3f, b2 = dot3(T1, u0), c2 = dot3(T1, u1) + 0.3f;
   const float ga = acc[4], gb = acc[5], gc = acc[6];
   const float det = a2 * c2 - b2 * b2;
   const float idet2 = rcp_fast(det * det);
@@ -171,57 +202,49 @@ __device__ __forceinline__ void geometry_view(const CamDev& c, Load prm, const G
   const float dl_db = (2.0f * ga * b2 * c2 - gb * (det + 2.0f * b2 * b2) + 2.0f * gc * a2 * b2) * idet2;
   const float dl_dc = (-ga * b2 * b2 + gb * a2 * b2 - gc * a2 * a2) * idet2;
 
-  M3 Gm = m3_zero();
-  Gm.a[0][0] = dl_da;
-  Gm.a[0][1] = 0.5f * dl_db;
-  Gm.a[1][0] = 0.5f * dl_db;
-  Gm.a[1][1] = dl_dc;
-  const M3 ds = m3_mul(m3_mul(m3_t(T), Gm), T);
-  dsig.add(ds);
+  // dL/dSigma = T^T G T, G = [[dl_da, dl_db/2], [dl_db/2, dl_dc]]:
+  // entry (i, j) = T0_i A_j + T1_i B_j with (A; B) = G (T0; T1)
+  const float hb = 0.5f * dl_db;
+  float A[3], B[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    A[m] = dl_da * T0[m] + hb * T1[m];
+    B[m] = hb * T0[m] + dl_dc * T1[m];
+  }
+  dsig.u[0] += T0[0] * A[0] + T1[0] * B[0];
+  dsig.u[1] += T0[0] * A[1] + T1[0] * B[1];
+  dsig.u[2] += T0[0] * A[2] + T1[0] * B[2];
+  dsig.u[3] += T0[1] * A[1] + T1[1] * B[1];
+  dsig.u[4] += T0[1] * A[2] + T1[1] * B[2];
+  dsig.u[5] += T0[2] * A[2] + T1[2] * B[2];
 
-  // screen mean and the mean-dependence of J -> p_cam -> position
+  // screen mean and the mean-dependence of J -> p_cam -> position. For p_cam
+  // component k, dT = dJ_k W and d(a, b, c) = (2 dT0.u0, dT0.u1 + dT1.u0,
+  // 2 dT1.u1); dJ_x = e at (0,2); dJ_y = f at (1,2); dJ_z = e at (0,0),
+  // gx at (0,2), f at (1,1), gy at (1,2).
   const float asx = acc[7], asy = acc[8];
-  float dpc[3] = {asx * fx * itz, asy * fy * itz, asx * (-fx * tx * itz2) + asy * (-fy * ty * itz2)};
-  M3 sig_tt = m3_mul(sigma, m3_t(T));
-  M3 t_sig = m3_mul(T, sigma);
-  for (int k = 0; k < 3; ++k) {
-    M3 dj = m3_zero();
-    if (k == 0) {
-      dj.a[0][2] = -fx * itz2;
-    } else if (k == 1) {
-      dj.a[1][2] = -fy * itz2;
-    } else {
-      dj.a[0][0] = -fx * itz2;
-      dj.a[0][2] = 2.0f * fx * tx * itz3;
-      dj.a[1][1] = -fy * itz2;
-      dj.a[1][2] = 2.0f * fy * ty * itz3;
-    }
-    M3 dt = m3_mul(dj, W);
-    M3 p1 = m3_mul(dt, sig_tt);
-    M3 p2 = m3_mul(t_sig, m3_t(dt));
-    float d00 = p1.a[0][0] + p2.a[0][0], d01 = p1.a[0][1] + p2.a[0][1], d11 = p1.a[1][1] + p2.a[1][1];
-    float contrib = dl_da * d00 + dl_db * d01 + dl_dc * d11;
-    dpc[k] += contrib;
-  }
+  const float w0u0 = dot3(W0, u0), w0u1 = dot3(W0, u1), w1u0 = dot3(W1, u0), w1u1 = dot3(W1, u1);
+  const float w2u0 = dot3(W2, u0), w2u1 = dot3(W2, u1);
+  const float e = -fx * itz2, f = -fy * itz2, gx = 2.0f * fx * tx * itz3, gy = 2.0f * fy * ty * itz3;
+  const float da2 = 2.0f * (e * w0u0 + gx * w2u0);
+  const float db2 = (e * w0u1 + gx * w2u1) + (f * w1u0 + gy * w2u0);
+  const float dc2 = 2.0f * (f * w1u1 + gy * w2u1);
+  float dpc[3];
+  dpc[0] = asx * j00 + (dl_da * (2.0f * e * w2u0) + dl_db * (e * w2u1));
+  dpc[1] = asy * j11 + (dl_db * (f * w2u0) + dl_dc * (2.0f * f * w2u1));
+  dpc[2] = (asx * j02 + asy * j12) + (dl_da * da2 + dl_db * db2 + dl_dc * dc2);
   float dp[3];
-  for (int i = 0; i < 3; ++i) {
-    float a = W.a[0][i] * dpc[0];
-    a = a + W.a[1][i] * dpc[1];
-    a = a + W.a[2][i] * dpc[2];
-    dp[i] = a;
-  }
+#pragma unroll
+  for (int m = 0; m < 3; ++m) dp[m] = W0[m] * dpc[0] + W1[m] * dpc[1] + W2[m] * dpc[2];
   // view-dependent colour: dir = (cam - pos)/|cam - pos|
-  float u0 = c.cam_pos[0] - px, u1 = c.cam_pos[1] - py, u2 = c.cam_pos[2] - pz;
-  float ulen = sqrt_fast(u0 * u0 + u1 * u1 + u2 * u2);
+  float u0d = c.cam_pos[0] - p[0], u1d = c.cam_pos[1] - p[1], u2d = c.cam_pos[2] - p[2];
+  float ulen = sqrt_fast(u0d * u0d + u1d * u1d + u2d * u2d);
   if (ulen > 1e-8f) {
     const float iu = rcp_fast(ulen);
-    float e0 = u0 * iu, e1 = u1 * iu, e2 = u2 * iu;
-    float dot = e0 * dd[0] + e1 * dd[1] + e2 * dd[2];
-    const float e[3] = {e0, e1, e2};
-    for (int cc = 0; cc < 3; ++cc) {
-      float proj = dd[cc] - e[cc] * dot;
-      dp[cc] += -(proj * iu);
-    }
+    const float ev[3] = {u0d * iu, u1d * iu, u2d * iu};
+    const float dot = ev[0] * dd[0] + ev[1] * dd[1] + ev[2] * dd[2];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) dp[m] += -((dd[m] - ev[m] * dot) * iu);
   }
   dpos[0] += dp[0];
   dpos[1] += dp[1];
@@ -275,6 +298,13 @@ __device__ __forceinline__ void geometry_finish(const GeomConst& G, const SymSum
   }
 }
 
+// Position of one Gaussian held in registers, as a parameter loader for
+// view_basis (rows 0-2 only).
+struct ShPos {
+  const float* p;
+  __device__ float operator()(int j) const { return p[j]; }
+};
+
 struct SoaLoad {
   const float* P;
   size_t cap;
@@ -306,9 +336,10 @@ __global__ __launch_bounds__(256) void k_geometry_aos(const float* __restrict__ 
   for (int ch = 0; ch < 3; ++ch)
     for (int k = 0; k < 16; ++k) o[P_SH + ch * 16 + k] = masked[ch] * basis[k];
   const GeomConst G = geom_const(L);
+  const float p[3] = {L(0), L(1), L(2)};
   SymSum dsig;
   float dpos[3] = {0.0f, 0.0f, 0.0f}, sgn = 0.0f;
-  geometry_view(c, L, G, acc, dd, dsig, dpos, sgn);
+  geometry_view(c, p, cov_sym(G), acc, dd, dsig, dpos, sgn);
   o[0] = dpos[0];
   o[1] = dpos[1];
   o[2] = dpos[2];
@@ -391,158 +422,146 @@ void launch_adam_aos(float* g, const float* grads, size_t n, float* m1, float* m
                      denom, k);
 }
 
-// Training step tail over Gaussians (SoA parameters), one thread each:
-//   train_geometry: per view the Gaussian is visible in, the non-SH
-//     geometry backprop, summed over views, and the colour-clamp bits of each
-//     (view, channel);
-//   train_adam: per colour channel the SH gradient sum over views (basis
-//     recomputed from the position), then Adam on all 59 rows, quaternion
-//     renormalisation, the densification statistics, and clearing of the
-//     raster accumulators.
-// The geometry part reads the SH coefficients before the Adam part updates
-// them, and the Adam part reads the positions before it updates them, as in
-// the reference's order (gradients of the whole step first, then Adam).
-// k_train_update runs both in one thread: the step's gradients never leave
-// registers.
+// Training step tail over Gaussians (SoA parameters): one 256-thread
+// workgroup per 64 Gaussians, one role per wave, lane = Gaussian:
+//   waves 0-2, colour channel ch: per view the Gaussian is visible in, the
+//     SH basis at the view direction, channel ch's colour clamp
+//     (backward.rs:533-544), the channel's share of the direction gradient
+//     (to LDS), and the SH gradient summed over views; then Adam on the
+//     channel's 16 coefficient rows;
+//   wave 3, geometry: per visible view, the non-SH geometry backprop with
+//     the direction gradient summed over the three channels from LDS, summed
+//     over views; the covariance tail; Adam on rows 0-10, quaternion
+//     renormalisation and the densification statistics.
+// Every wave clears the raster accumulator rows it consumed. The colour
+// waves read the SH coefficients and positions before any wave updates
+// them (positions are updated by the geometry wave after the last barrier),
+// as in the reference's order: gradients of the whole step first, then
+// Adam (pipeline/mod.rs:203-265, optimizer.rs:116-163). Views go through
+// LDS in chunks of TU_VCH, two barriers per chunk on every wave.
+constexpr int TU_G = 64;
+constexpr int TU_VCH = 4;
 
-// Position and SH coefficients of one Gaussian held in registers.
-struct ShRegs {
-  float pos[3];
-  float sh[48];
-  __device__ float operator()(int j) const { return j < 3 ? pos[j] : sh[j - P_SH]; }
-};
-
-// s_dd: per-thread direction gradients of the current chunk of views (LDS).
-__device__ __forceinline__ void train_geometry(const float* __restrict__ P, int cap, int i,
-                                               const CamDev* __restrict__ cams, int n_views,
-                                               const uint32_t* __restrict__ cnt, const float* __restrict__ accum,
-                                               size_t acc_stride, float (*s_dd)[3][256], int t, float g[11],
-                                               float& sgn, uint64_t clamp_in[3]) {
+__device__ __forceinline__ void train_colour(float* __restrict__ P, float* __restrict__ M1, float* __restrict__ M2,
+                                             int cap, int i, bool live, int ch, const CamDev* __restrict__ cams,
+                                             int n_views, const uint32_t* __restrict__ cnt,
+                                             float* __restrict__ accum, size_t acc_stride,
+                                             float (*s_dd)[3][3][TU_G], int lane, const AdamConsts* __restrict__ kd,
+                                             float* __restrict__ gout) {
+  float pos[3] = {0.0f, 0.0f, 0.0f}, cf[16], gs[16];
 #pragma unroll
-  for (int j = 0; j < 11; ++j) g[j] = 0.0f;
-  sgn = 0.0f;
-  clamp_in[0] = clamp_in[1] = clamp_in[2] = 0;  // per channel: views whose colour is inside (0, 1)
-  // Views go in chunks of VCH. The first pass reads the SH coefficients once
-  // and, per visible view, evaluates the colour clamp and the direction
-  // gradient dd (kept in LDS); the second runs the geometry backprop per view.
-  // Both keep the per-view arithmetic and the view order of the sums.
-  // sums over views for geometry_finish
-  bool have_const = false;
-  SymSum dsig;
-  float acc3 = 0.0f;
-  constexpr int VCH = 4;
-  for (int v0 = 0; v0 < n_views; v0 += VCH) {
-    const int nv = min(VCH, n_views - v0);
-    uint32_t vis = 0;
-    for (int j = 0; j < nv; ++j)
-      if (cnt[(size_t)(v0 + j) * cap + i] != 0) vis |= 1u << j;
-    if (!vis) continue;
-    {
-      // an empty asm makes the base pointer chunk-variant, so that these
-      // loads are not hoisted above the loop (51 registers live across it)
-      const float* Pv = P;
-      asm volatile("" : "+s"(Pv));
-      ShRegs R;
+  for (int k = 0; k < 16; ++k) cf[k] = 0.0f, gs[k] = 0.0f;
+  const size_t row0 = (size_t)(P_SH + ch * 16);
+  if (live) {
 #pragma unroll
-      for (int q = 0; q < 3; ++q) R.pos[q] = Pv[(size_t)q * cap + i];
+    for (int q = 0; q < 3; ++q) pos[q] = P[(size_t)q * cap + i];
 #pragma unroll
-      for (int q = 0; q < 48; ++q) R.sh[q] = Pv[(size_t)(P_SH + q) * cap + i];
-#pragma unroll 1
-      for (int j = 0; j < nv; ++j) {
-        if (!(vis >> j & 1u)) continue;
-        const int v = v0 + j;
-        const size_t vi = (size_t)v * cap + i;
-        float acc[9];
-#pragma unroll
-        for (int q = 0; q < 9; ++q) acc[q] = q < 3 ? accum[(size_t)q * acc_stride + vi] : 0.0f;
-        float basis[16], masked[3], dk[16], dd[3];
-        uint32_t bits = 0;
-        view_sh(cams[v], R, acc, basis, masked, dk, &bits);
-#pragma unroll
-        for (int ch = 0; ch < 3; ++ch) clamp_in[ch] |= (uint64_t)(bits >> ch & 1u) << v;
-        view_dir_grad(cams[v], R, dk, dd);
-#pragma unroll
-        for (int q = 0; q < 3; ++q) s_dd[j][q][t] = dd[q];
-      }
-    }
-    have_const = true;
+    for (int k = 0; k < 16; ++k) cf[k] = P[(row0 + k) * cap + i];
+  }
+  const ShPos L{pos};
+  for (int v0 = 0; v0 < n_views; v0 += TU_VCH) {
+    const int nv = min(TU_VCH, n_views - v0);
 #pragma unroll 1
     for (int j = 0; j < nv; ++j) {
-      if (!(vis >> j & 1u)) continue;
       const int v = v0 + j;
       const size_t vi = (size_t)v * cap + i;
-      // reload the position per view (an empty asm makes the base pointer
-      // loop-variant) rather than keeping it live
-      const float* Pv = P;
-      asm volatile("" : "+s"(Pv));
-      SoaLoad Lv{Pv, (size_t)cap, (size_t)i};
+      float dd[3] = {0.0f, 0.0f, 0.0f};
+      if (live && cnt[vi] != 0) {
+        const float a = accum[(size_t)ch * acc_stride + vi];
+        float basis[16], d[3];
+        view_basis(cams[v], L, basis, d);
+        float s = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) s += basis[k] * cf[k];
+        const float raw = 0.5f + s;
+        const float m = (raw > 0.0f && raw < 1.0f) ? a : 0.0f;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) gs[k] += m * basis[k];
+        float t[3];
+        sh_basis_grad_contract(d[0], d[1], d[2], cf, t);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) dd[q] = m * t[q];
+      }
+#pragma unroll
+      for (int q = 0; q < 3; ++q) s_dd[j][ch][q][lane] = dd[q];
+    }
+    __syncthreads();  // direction gradients of the chunk are in LDS
+    __syncthreads();  // the geometry wave has read them
+  }
+  if (!live) return;
+  const AdamConsts K = *kd;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const size_t o = (row0 + k) * cap + i;
+    if (gout) gout[o] = gs[k];
+    float mm = M1[o], vv = M2[o];
+    P[o] = adam_fast(cf[k], gs[k], mm, vv, lr_of(P_SH + ch * 16 + k, K), K);
+    M1[o] = mm;
+    M2[o] = vv;
+  }
+  for (int v = 0; v < n_views; ++v) {
+    const size_t vi = (size_t)v * cap + i;
+    if (cnt[vi] != 0) accum[(size_t)ch * acc_stride + vi] = 0.0f;
+  }
+}
+
+__device__ __forceinline__ void train_geometry(float* __restrict__ P, float* __restrict__ M1,
+                                               float* __restrict__ M2, int cap, int i, bool live,
+                                               const CamDev* __restrict__ cams, int n_views,
+                                               const uint32_t* __restrict__ cnt, float* __restrict__ accum,
+                                               size_t acc_stride, float (*s_dd)[3][3][TU_G], int lane,
+                                               float* __restrict__ gaccum, uint32_t* __restrict__ gdenom,
+                                               const AdamConsts* __restrict__ kd, float* __restrict__ gout) {
+  float pos[3] = {0.0f, 0.0f, 0.0f}, prm[11];
+  if (live) {
+#pragma unroll
+    for (int j = 0; j < 11; ++j) prm[j] = P[(size_t)j * cap + i];
+  } else {
+#pragma unroll
+    for (int j = 0; j < 11; ++j) prm[j] = 0.0f;
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) pos[q] = prm[q];
+  const GeomConst G = geom_const([&](int j) { return prm[j]; });
+  const Sym3 S = cov_sym(G);
+  SymSum dsig;
+  float g[11], sgn = 0.0f, acc3 = 0.0f;
+#pragma unroll
+  for (int j = 0; j < 11; ++j) g[j] = 0.0f;
+  bool any = false;
+  for (int v0 = 0; v0 < n_views; v0 += TU_VCH) {
+    const int nv = min(TU_VCH, n_views - v0);
+    __syncthreads();  // direction gradients of the chunk are in LDS
+#pragma unroll 1
+    for (int j = 0; j < nv; ++j) {
+      const int v = v0 + j;
+      const size_t vi = (size_t)v * cap + i;
+      if (!live || cnt[vi] == 0) continue;
       float acc[9];
 #pragma unroll
       for (int q = 0; q < 9; ++q) acc[q] = q < 3 ? 0.0f : accum[(size_t)q * acc_stride + vi];
       float dd[3];
 #pragma unroll
-      for (int q = 0; q < 3; ++q) dd[q] = s_dd[j][q][t];
-      // the view-independent inputs are rebuilt per view from the reloaded
-      // parameters rather than held across views (register pressure)
-      geometry_view(cams[v], Lv, geom_const(Lv), acc, dd, dsig, g, sgn);
+      for (int q = 0; q < 3; ++q) dd[q] = (s_dd[j][0][q][lane] + s_dd[j][1][q][lane]) + s_dd[j][2][q][lane];
+      geometry_view(cams[v], pos, S, acc, dd, dsig, g, sgn);
       acc3 += acc[3];
+      any = true;
     }
+    __syncthreads();  // the chunk's LDS slots may be rewritten
   }
-  if (have_const) {
-    const float* Pv = P;
-    asm volatile("" : "+s"(Pv));
-    geometry_finish(geom_const(SoaLoad{Pv, (size_t)cap, (size_t)i}), dsig, acc3, g);
-  }
-}
-
-__device__ __forceinline__ void train_adam(float* __restrict__ P, float* __restrict__ M1, float* __restrict__ M2,
-                                           int cap, int i, const CamDev* __restrict__ cams, int n_views,
-                                           const uint32_t* __restrict__ cnt, float* __restrict__ accum,
-                                           size_t acc_stride, const float g[11], float sgn,
-                                           const uint64_t clamp_in[3], float* __restrict__ gaccum,
-                                           uint32_t* __restrict__ gdenom, const AdamConsts& k_,
-                                           float* __restrict__ gout) {
-  SoaLoad L{P, (size_t)cap, (size_t)i};
-  uint64_t vis = 0;
-  for (int v = 0; v < n_views; ++v)
-    if (cnt[(size_t)v * cap + i] != 0) vis |= 1ull << v;
-  for (int ch = 0; ch < 3; ++ch) {
-    float gs[16];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) gs[k] = 0.0f;
-    for (int v = 0; v < n_views; ++v) {
-      if (!(vis >> v & 1ull)) continue;
-      size_t vi = (size_t)v * cap + i;
-      const float m = (clamp_in[ch] >> v & 1ull) ? accum[(size_t)ch * acc_stride + vi] : 0.0f;
-      float basis[16];
-      view_basis(cams[v], L, basis);
-#pragma unroll
-      for (int k = 0; k < 16; ++k) gs[k] += m * basis[k];
-    }
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      size_t o = (size_t)(P_SH + ch * 16 + k) * cap + i;
-      if (gout) gout[o] = gs[k];
-      float mm = M1[o], vv = M2[o];
-      P[o] = adam_fast(P[o], gs[k], mm, vv, lr_of(P_SH + ch * 16 + k, k_), k_);
-      M1[o] = mm;
-      M2[o] = vv;
-    }
-  }
-  // the raster accumulators of this Gaussian are consumed: clear them
-  for (int v = 0; v < n_views; ++v) {
-    if (!(vis >> v & 1ull)) continue;
-    size_t vi = (size_t)v * cap + i;
-#pragma unroll
-    for (int q = 0; q < 9; ++q) accum[(size_t)q * acc_stride + vi] = 0.0f;
-  }
+  if (!live) return;
+  if (any) geometry_finish(G, dsig, acc3, g);
+  const AdamConsts K = *kd;
   float rot[4];
 #pragma unroll
   for (int j = 0; j < 11; ++j) {
-    size_t o = (size_t)j * cap + i;
+    const size_t o = (size_t)j * cap + i;
     if (gout) gout[o] = g[j];
+    // positions and the quaternion are still in registers; the log-scales
+    // and the opacity logit are read again
+    const float old = j < 3 ? pos[j] : j < 7 ? G.q[j - 3] : P[o];
     float mm = M1[o], vv = M2[o];
-    float p = adam_fast(P[o], g[j], mm, vv, lr_of(j, k_), k_);
+    const float p = adam_fast(old, g[j], mm, vv, lr_of(j, K), K);
     M1[o] = mm;
     M2[o] = vv;
     if (j >= P_ROT && j < P_ROT + 4)
@@ -551,10 +570,17 @@ __device__ __forceinline__ void train_adam(float* __restrict__ P, float* __restr
       P[o] = p;
   }
   renorm_quat(rot[0], rot[1], rot[2], rot[3]);
+#pragma unroll
   for (int q = 0; q < 4; ++q) P[(size_t)(P_ROT + q) * cap + i] = rot[q];
   if (gout) gout[(size_t)59 * cap + i] = sgn;
   gaccum[i] += sgn;
   gdenom[i] += (uint32_t)n_views;
+  for (int v = 0; v < n_views; ++v) {
+    const size_t vi = (size_t)v * cap + i;
+    if (cnt[vi] == 0) continue;
+#pragma unroll
+    for (int q = 3; q < 9; ++q) accum[(size_t)q * acc_stride + vi] = 0.0f;
+  }
 }
 
 constexpr uint32_t ST_SKIP_UPDATE = ST_PAIR_OVERFLOW | ST_NAN_DEPTH | ST_NEED_BACK;
@@ -568,24 +594,27 @@ __global__ __launch_bounds__(256) void k_train_update(float* __restrict__ P, flo
                                                       const AdamConsts* __restrict__ k_dev,
                                                       const uint32_t* __restrict__ status,
                                                       float* __restrict__ gout) {
-  __shared__ float s_dd[4][3][256];
-  const int t = threadIdx.x;
-  int i = blockIdx.x * blockDim.x + t;
-  if (i >= n) return;
+  __shared__ float s_dd[TU_VCH][3][3][TU_G];  // [view in chunk][channel][component][Gaussian]
+  // the same word for every thread: the whole grid returns or none of it
   if (*status & ST_SKIP_UPDATE) return;
-  float g[11], sgn;
-  uint64_t clamp_in[3];
-  train_geometry(P, cap, i, cams, n_views, cnt, accum, acc_stride, s_dd, t, g, sgn, clamp_in);
-  train_adam(P, M1, M2, cap, i, cams, n_views, cnt, accum, acc_stride, g, sgn, clamp_in, gaccum, gdenom, *k_dev,
-             gout);
+  const int lane = threadIdx.x & 63;
+  // wave-uniform role (scalar), so that each wave runs one role's barriers
+  const int role = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i = blockIdx.x * TU_G + lane;
+  const bool live = i < n;
+  if (role < 3)
+    train_colour(P, M1, M2, cap, i, live, role, cams, n_views, cnt, accum, acc_stride, s_dd, lane, k_dev, gout);
+  else
+    train_geometry(P, M1, M2, cap, i, live, cams, n_views, cnt, accum, acc_stride, s_dd, lane, gaccum, gdenom,
+                   k_dev, gout);
 }
 
 void launch_train_adam(float* P, float* M1, float* M2, int cap, int n, const CamDev* cams, int n_views,
                        const uint32_t* cnt, float* accum, size_t acc_stride, float* gaccum, uint32_t* gdenom,
                        const AdamConsts* k_dev, const uint32_t* status, hipStream_t st, float* grad_out) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_train_update, dim3((n + 255) / 256), dim3(256), 0, st, P, M1, M2, cap, n, cams, n_views, cnt,
-                     accum, acc_stride, gaccum, gdenom, k_dev, status, grad_out);
+  hipLaunchKernelGGL(k_train_update, dim3((n + TU_G - 1) / TU_G), dim3(256), 0, st, P, M1, M2, cap, n, cams, n_views,
+                     cnt, accum, acc_stride, gaccum, gdenom, k_dev, status, grad_out);
 }
 
 // ln on the device: the f64 logarithm rounded once to f32 (correctly rounded

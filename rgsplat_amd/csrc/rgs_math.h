@@ -240,7 +240,12 @@ RGS_HD void sh_basis(float x, float y, float z, float o[16]) {
 }
 
 // sh.rs:68-89, contracted with dL/dbasis: returns sum_k dk[k] * d basis_k / d dir.
+// Gradient-only (fused multiply-adds); the Jacobian's structural zeros
+// (NZ) are skipped at compile time rather than multiplied.
 RGS_HD void sh_basis_grad_contract(float x, float y, float z, const float dk[16], float dd[3]) {
+#pragma clang fp contract(fast)
+  constexpr bool NZ[16][3] = {{0, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 0}, {1, 1, 0}, {0, 1, 1}, {1, 1, 1}, {1, 0, 1},
+                              {1, 1, 0}, {1, 1, 0}, {1, 1, 1}, {1, 1, 1}, {1, 1, 1}, {1, 1, 1}, {1, 1, 1}, {1, 1, 0}};
   float xx = x * x, yy = y * y, zz = z * z;
   const float g[16][3] = {
       {0.0f, 0.0f, 0.0f},
@@ -260,8 +265,11 @@ RGS_HD void sh_basis_grad_contract(float x, float y, float z, const float dk[16]
       {SH_C3_5 * 2.0f * x * z, SH_C3_5 * -2.0f * y * z, SH_C3_5 * (xx - yy)},
       {SH_C3_6 * 3.0f * (xx - yy), SH_C3_6 * -6.0f * x * y, 0.0f}};
   dd[0] = 0.0f; dd[1] = 0.0f; dd[2] = 0.0f;
+#pragma unroll
   for (int k = 0; k < 16; ++k)
-    for (int c = 0; c < 3; ++c) dd[c] += dk[k] * g[k][c];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      if (NZ[k][c]) dd[c] += dk[k] * g[k][c];
 }
 
 // Per-view camera constants for the kernels. view is the world-to-camera
