@@ -122,11 +122,11 @@ int rgs_loss_grad(rgs_ctx* ctx, const float* rgb, const float* target_rgba, uint
  * forbid hipMalloc). rgb_dev (w*h*3), target_rgba_dev (w*h*4), dl_drgb_dev
  * (w*h*3) and loss_dev (1 float) are device memory on the context's device;
  * the work is enqueued on the context's stream (rgs_set_stream) and complete
- * in stream order, without a host synchronisation. dL/drgb is bit-identical
- * to rgs_loss_grad's; the loss agrees to float rounding (another summation
- * order). The kernels' partial sums live in a static array of 1.5 MiB per
- * GPU: images of at most 65536 tiles of 32x32 pixels (e.g. 8192 x 8192);
- * larger ones return RGS_E_CAPACITY. Every call on one GPU shares that
+ * in stream order, without a host synchronisation. dL/drgb and the loss
+ * are those of rgs_loss_grad (the same kernels). The kernels' partial sums
+ * live in a static array of 1.5 MiB per GPU: images of at most 65536 loss
+ * units of 54 x 155 pixels (any image up to 16384 x 16384); larger ones
+ * return RGS_E_CAPACITY. Every call on one GPU shares that
  * array, so calls that overlap in time on one GPU (other streams, other
  * contexts, other threads) race; calls in one stream order do not. */
 int rgs_loss_grad_dev_noalloc(rgs_ctx* ctx, const float* rgb_dev, const float* target_rgba_dev, uint32_t w,

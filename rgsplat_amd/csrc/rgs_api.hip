@@ -170,7 +170,7 @@ int make_camdev(const rgs_camera& cam, CamDev& out) {
   out.tile_base = 0;
   out.pix_base = 0;
   out.tgt_base = 0;
-  out.ltile_base = 0;
+  out.lunit_base = 0;
   return 0;
 }
 
@@ -182,7 +182,7 @@ struct ViewSet {
   int shift = 0;
   int total_tiles = 0;
   int total_bins = 0;
-  int total_ltiles = 0;
+  int total_lunits = 0;
   size_t total_pix = 0;
   void layout() {
     int tb = 0, bb = 0, lb = 0;
@@ -193,16 +193,16 @@ struct ViewSet {
       c.nbx = (c.ntx + (1 << shift) - 1) >> shift;
       c.nby = (c.nty + (1 << shift) - 1) >> shift;
       c.bin_base = bb;
-      c.ltile_base = lb;
+      c.lunit_base = lb;
       c.pix_base = (int32_t)pb;
       tb += c.ntx * c.nty;
       bb += c.nbx * c.nby;
-      lb += ((c.w + LTILE - 1) / LTILE) * ((c.h + LTILE - 1) / LTILE);
+      lb += loss_units(c.w, c.h);
       pb += (size_t)c.w * c.h;
     }
     total_tiles = tb;
     total_bins = bb;
-    total_ltiles = lb;
+    total_lunits = lb;
     total_pix = pb;
   }
   int bits() const {
@@ -231,7 +231,7 @@ struct rgs_ctx {
   DBuf cams_d, xy_ab, c_op_rg, col_b, cnt, rect, reach, depth;
   DBuf dk[2], dv[2], offsets, bsums, scalars;
   DBuf pk[2], pv[2], hist, rowtot;
-  DBuf rstart, rend, rgb, final_t, last, dl, maps, partial, loss;
+  DBuf rstart, rend, rgb, final_t, last, dl, partial, loss;
   DBuf first_splat;
   DBuf accum;
   size_t k_cap = 0;
@@ -706,15 +706,14 @@ int upload_api_scene(rgs_ctx* c, const void* g, size_t n, bool device_src) {
 
 int ensure_loss(rgs_ctx* c, const ViewSet& vs) {
   CHK(ensure(c, c->dl, vs.total_pix * 3 * 4));
-  CHK(ensure(c, c->maps, vs.total_pix * SSIM_MAPS * 3 * 4));
-  CHK(ensure(c, c->partial, (size_t)vs.total_ltiles * 6 * 4));
+  CHK(ensure(c, c->partial, (size_t)vs.total_lunits * 6 * 4));
   CHK(ensure(c, c->loss, 64 * 4));
   return 0;
 }
 
 void enqueue_loss(rgs_ctx* c, const ViewSet& vs, const float* rgb, const float* target, float lambda) {
-  LossIn li{rgb, target, P_<CamDev>(c->cams_d), (int)vs.cams.size(), vs.total_ltiles, vs.total_pix, lambda};
-  launch_loss(li, P_<float>(c->dl), P_<float>(c->maps), P_<float>(c->partial), P_<float>(c->loss), c->st);
+  LossIn li{rgb, target, P_<CamDev>(c->cams_d), (int)vs.cams.size(), vs.total_lunits, vs.total_pix, lambda};
+  launch_loss(li, P_<float>(c->dl), P_<float>(c->partial), P_<float>(c->loss), c->st);
 }
 
 AdamConsts adam_consts(const rgs_train_cfg& cfg, uint64_t t) {
@@ -749,14 +748,15 @@ AdamConsts adam_consts(const rgs_train_cfg& cfg, uint64_t t) {
   return k;
 }
 
-// Scratch of rgs_loss_grad_dev_noalloc: the per-tile partial sums of the
-// fused loss kernel, for at most NOALLOC_MAX_TILES 32x32 tiles (an image of
-// up to 8192 x 8192 pixels), 1.5 MiB per GPU for the life of the process.
+// Scratch of rgs_loss_grad_dev_noalloc: the per-unit partial sums of the
+// loss kernel, for at most NOALLOC_MAX_UNITS loss units of LSW x LSG pixels
+// (any image up to 16384 x 16384 pixels), 1.5 MiB per GPU for the life of
+// the process.
 // Host code never uses the array's name as an address: each device's copy is
 // resolved once with hipGetSymbolAddress (scratch_on) and that pointer is
 // passed to the kernels.
-constexpr size_t NOALLOC_MAX_TILES = 65536;
-static __device__ __align__(256) float scratch[NOALLOC_MAX_TILES * 6];
+constexpr size_t NOALLOC_MAX_UNITS = 65536;
+static __device__ __align__(256) float scratch[NOALLOC_MAX_UNITS * 6];
 
 constexpr int MAX_DEVICES = 64;
 std::mutex g_scratch_mu;
@@ -823,7 +823,7 @@ void rgs_destroy(rgs_ctx* c) {
   DBuf* all[] = {&c->cams_d, &c->reach, &c->xy_ab, &c->c_op_rg, &c->col_b, &c->cnt, &c->rect, &c->dk[0], &c->dk[1],
                  &c->dv[0], &c->dv[1], &c->offsets, &c->bsums, &c->scalars, &c->pk[0], &c->pk[1], &c->pv[0],
                  &c->pv[1], &c->hist, &c->rowtot, &c->rstart, &c->rend, &c->rgb, &c->final_t, &c->last, &c->dl,
-                 &c->maps, &c->partial, &c->loss, &c->accum, &c->api_aos, &c->api_P, &c->api_grads,
+                 &c->partial, &c->loss, &c->accum, &c->api_aos, &c->api_P, &c->api_grads,
                  &c->api_target, &c->api_m1, &c->api_m2, &c->api_acc, &c->api_den, &c->api_P2, &c->dens_want,
                  &c->dens_cnt, &c->dens_W, &c->dens_packed, &c->dens_prefix, &c->dens_b32, &c->dens_b64,
                  &c->dens_t32, &c->dens_t64, &c->first_splat, &c->P, &c->P2, &c->M1, &c->M2, &c->gacc, &c->gden, &c->targets,
@@ -991,9 +991,12 @@ int rgs_loss_grad_dev_noalloc(rgs_ctx* c, const float* rgb_dev, const float* tar
                               uint32_t h, float lambda, float* loss_dev, float* dl_drgb_dev) {
   if (!c || !rgb_dev || !target_rgba_dev || !loss_dev || !dl_drgb_dev) return fail(RGS_E_ARG, "null argument");
   if (!w || !h) return fail(RGS_E_ARG, "empty image");
-  const size_t nlx = (w + LTILE - 1) / LTILE, nly = (h + LTILE - 1) / LTILE;
-  if (nlx * nly > NOALLOC_MAX_TILES)
-    return fail(RGS_E_CAPACITY, "image beyond the no-alloc scratch (at most 65536 tiles of 32x32 pixels)");
+  if (w > (uint32_t)INT32_MAX || h > (uint32_t)INT32_MAX)
+    return fail(RGS_E_CAPACITY, "image beyond the no-alloc scratch");
+  const size_t units = (size_t)((w + LSW - 1) / LSW) * (size_t)((h + LSG - 1) / LSG);
+  if (units > NOALLOC_MAX_UNITS)
+    return fail(RGS_E_CAPACITY,
+                "image beyond the no-alloc scratch (at most 65536 loss units of 54 x 155 pixels, e.g. 16384 x 16384)");
   if (!c->scratch_dev) return fail(RGS_E_HIP, "no scratch on this context's device");
   (void)hipSetDevice(c->device);
   LossIn li{};
@@ -1001,12 +1004,12 @@ int rgs_loss_grad_dev_noalloc(rgs_ctx* c, const float* rgb_dev, const float* tar
   li.target = target_rgba_dev;
   li.cams = nullptr;
   li.n_views = 1;
-  li.total_ltiles = (int)(nlx * nly);
+  li.total_lunits = (int)units;
   li.total_pix = (size_t)w * h;
   li.lambda = lambda;
   li.cam0.w = (int32_t)w;
   li.cam0.h = (int32_t)h;
-  launch_loss_fused(li, dl_drgb_dev, c->scratch_dev, loss_dev, c->st);
+  launch_loss(li, dl_drgb_dev, c->scratch_dev, loss_dev, c->st);
   HIPCHK(hipGetLastError());
   return 0;
 }

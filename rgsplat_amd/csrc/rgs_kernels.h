@@ -243,24 +243,24 @@ void launch_work_counts(const RasterIn& in, const float* final_t, const int32_t*
                         int total_tiles, hipStream_t st);
 
 // ---- loss (k_loss.hip)
-constexpr int LTILE = 32;  // loss tile side
+// The loss sweeps each view in units of LSW output columns x LSG output rows
+// (one workgroup per unit and colour channel).
+constexpr int LSW = 54;
+constexpr int LSG = 155;
+RGS_HD int loss_units(int w, int h) { return ((w + LSW - 1) / LSW) * ((h + LSG - 1) / LSG); }
 struct LossIn {
   const float* rgb;     // total_pix * 3
   const float* target;  // target store, RGBA; view v starts at pixel cams[v].tgt_base
   const CamDev* cams;   // null: one view, described by cam0
   int n_views;
-  int total_ltiles;     // 32x32 loss tiles over all views
+  int total_lunits;     // loss units over all views
   size_t total_pix;
   float lambda;
-  CamDev cam0;          // the view when cams is null (w, h, pix_base, tgt_base, ltile_base)
+  CamDev cam0;          // the view when cams is null (w, h, pix_base, tgt_base, lunit_base)
 };
-// dl: 3 * total_pix floats; ssim_maps: SSIM_MAPS * 3 * total_pix floats and
-// partial: 2 * 3 * total_ltiles floats (scratch); loss_out: n_views floats.
-constexpr int SSIM_MAPS = 3;
-void launch_loss(const LossIn& in, float* dl, float* ssim_maps, float* partial, float* loss_out, hipStream_t st);
-// The same dL/dpixel in one pass per tile, without the map buffer (the maps
-// stay in LDS); the loss agrees with launch_loss's to float rounding.
-void launch_loss_fused(const LossIn& in, float* dl, float* partial, float* loss_out, hipStream_t st);
+// dl: 3 * total_pix floats; partial: 2 * 3 * total_lunits floats (scratch);
+// loss_out: n_views floats. Allocates nothing.
+void launch_loss(const LossIn& in, float* dl, float* partial, float* loss_out, hipStream_t st);
 
 // ---- optimizer and parameter kernels (k_optim.hip)
 struct AdamConsts {

@@ -282,7 +282,7 @@ struct CamDev {
   int32_t tile_base;  // first global tile id of this view
   int32_t pix_base;   // first pixel of this view in the batched image arrays
   uint32_t tgt_base;  // first pixel of this view's target in the target store
-  int32_t ltile_base; // first 32x32 loss tile of this view
+  int32_t lunit_base; // first loss unit of this view
   // binning grid: bins of (1 << bin_shift)^2 tiles, nbx x nby per view
   int32_t bin_shift, nbx, nby;
   int32_t bin_base;   // first global bin id of this view

@@ -13,8 +13,9 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: longer CPU test")
     # torch bundles its own HIP runtime; a process that uses torch tensors
     # with librgsplat_hip must load torch's runtime first so both share it
+    # (every run that may select GPU tests: -m gpu, no -m, named files)
     markexpr = config.getoption("-m") or ""
-    if "gpu" in markexpr and "not gpu" not in markexpr:
+    if "not gpu" not in markexpr:
         try:
             import torch  # noqa: F401
         except Exception:

@@ -45,7 +45,7 @@ def test_loss_grad_noalloc_matches_loss_grad():
             assert abs(loss - loss_ref) <= 1e-6 * abs(loss_ref), (w, h, loss, loss_ref)
         # beyond the scratch: an error, not a launch
         with pytest.raises(runtime.RgsError):
-            ctx.loss_grad_dev_noalloc(rgb_d.data_ptr(), tgt_d.data_ptr(), 8193, 8193, 0.2, loss_d.data_ptr(),
+            ctx.loss_grad_dev_noalloc(rgb_d.data_ptr(), tgt_d.data_ptr(), 32768, 32768, 0.2, loss_d.data_ptr(),
                                       dl_d.data_ptr())
     finally:
         ctx.set_stream(None)
