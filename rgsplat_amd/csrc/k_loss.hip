@@ -60,14 +60,33 @@ constexpr int LRING = LRB + 10;         // ring rows: a step's 8 plus the window
 constexpr int LIN = LRB * LIC;          // staged inputs per image per step
 constexpr int LIPT = (LIN + LT - 1) / LT;
 static_assert(LMC == 64 && LT == 4 * 64 && LRB == 2 * (LT / 64), "a wave per row pair, a lane per map column");
-static_assert(2 * LIN <= 3 * LRB * LMC, "inputs and maps share the staging area");
 
-__global__ __launch_bounds__(LT) void k_ssim_sweep(LossIn in, Kern11 K, float* __restrict__ dl,
+// Sweep modes:
+//   SW_FULL: the five products blurred (x, y staged; products formed in B);
+//   SW_CACHED: the target's two blurs (mu_y, E[y^2]) read from a statistics
+//     buffer made once per set of targets; x, x^2, xy staged and blurred;
+//   SW_STATS: that buffer: y, y^2 staged and blurred, written for the unit's
+//     own pixels (no maps, no dL/dpixel).
+// The blurs of y and y^2 are the same fused multiply-add chains in SW_FULL
+// and SW_STATS, so SW_CACHED computes the values SW_FULL does.
+enum { SW_FULL = 0, SW_CACHED = 1, SW_STATS = 2 };
+template <int MODE>
+struct Sweep {
+  static constexpr int NP = MODE == SW_FULL ? 5 : MODE == SW_CACHED ? 3 : 2;  // blurred products
+  static constexpr int NS = MODE == SW_FULL ? 2 : NP;                         // staged values per input
+  static constexpr bool MAPS = MODE != SW_STATS;
+  static constexpr int ST = (NS * LIN > 3 * LRB * LMC || !MAPS) ? NS * LIN : 3 * LRB * LMC;
+};
+
+template <int MODE>
+__global__ __launch_bounds__(LT) __attribute__((amdgpu_waves_per_eu(MODE == SW_CACHED ? 5 : 1))) void k_ssim_sweep(LossIn in, Kern11 K, float* __restrict__ out,
                                                    float* __restrict__ partial) {
 #pragma clang fp contract(fast)
-  __shared__ float s_hp[5][LRING][LMC];  // product ring: x, y, x^2, y^2, xy blurred along rows
-  __shared__ float s_mh[3][LRING][LSW];  // map ring: the maps blurred along rows
-  __shared__ float s_st[3 * LRB * LMC];  // the step's inputs [x | y][8][74], then its maps [3][8][64]
+  using SP = Sweep<MODE>;
+  constexpr int NP = SP::NP, NS = SP::NS;
+  __shared__ float s_hp[NP][LRING][LMC];             // product ring: the products blurred along rows
+  __shared__ float s_mh[SP::MAPS ? 3 : 1][LRING][LSW];  // map ring: the maps blurred along rows
+  __shared__ float s_st[SP::ST];                     // the step's staged inputs [NS][8][74], then its maps [3][8][64]
   __shared__ float s_tmp[4];
   const int t = threadIdx.x, lane = t & 63;
   const int wv = __builtin_amdgcn_readfirstlane(t >> 6);  // row pair of C and E
@@ -96,9 +115,13 @@ __global__ __launch_bounds__(LT) void k_ssim_sweep(LossIn in, Kern11 K, float* _
   // -lambda / M scales the SSIM term
   const float l1_pos = (1.0f - lam) * 1.0f / m_total, l1_neg = (1.0f - lam) * -1.0f / m_total;
   const float s_scale = -lam / m_total;
+  // the target statistics of channel ch: mu_y, then E[y^2]
+  const float* ts_mu = in.tstats + ((size_t)ch * 2 + 0) * in.tgt_total + tb;
+  const float* ts_yy = in.tstats + ((size_t)ch * 2 + 1) * in.tgt_total + tb;
 
-  for (int i = t; i < 5 * LRING * LMC; i += LT) (&s_hp[0][0][0])[i] = 0.0f;
-  for (int i = t; i < 3 * LRING * LSW; i += LT) (&s_mh[0][0][0])[i] = 0.0f;
+  for (int i = t; i < NP * LRING * LMC; i += LT) (&s_hp[0][0][0])[i] = 0.0f;
+  if (SP::MAPS)
+    for (int i = t; i < 3 * LRING * LSW; i += LT) (&s_mh[0][0][0])[i] = 0.0f;
 
   // the staged inputs this thread loads: row ir[i] of the step, image column ix[i]
   int ir[LIPT], ix[LIPT];
@@ -117,7 +140,7 @@ __global__ __launch_bounds__(LT) void k_ssim_sweep(LossIn in, Kern11 K, float* _
       const int gy = Y + LRB * k + ir[i];
       const bool ok = cok[i] && gy >= 0 && gy < H;
       const size_t q = (size_t)gy * W + ix[i];
-      px[i] = ok ? in.rgb[(pb + q) * 3 + ch] : 0.0f;
+      px[i] = (MODE != SW_STATS && ok) ? in.rgb[(pb + q) * 3 + ch] : 0.0f;
       py[i] = ok ? in.target[(tb + q) * 4 + ch] : 0.0f;
     }
   };
@@ -128,34 +151,83 @@ __global__ __launch_bounds__(LT) void k_ssim_sweep(LossIn in, Kern11 K, float* _
     for (int i = 0; i < LIPT; ++i) {
       const int idx = t + i * LT;
       if (idx < LIN) {
-        s_st[idx] = px[i];
-        s_st[LIN + idx] = py[i];
+        const float xv = px[i], yv = py[i];
+        if (MODE == SW_FULL) {
+          s_st[idx] = xv;
+          s_st[LIN + idx] = yv;
+        } else if (MODE == SW_CACHED) {
+          s_st[idx] = xv;
+          s_st[LIN + idx] = xv * xv;
+          s_st[2 * LIN + idx] = xv * yv;
+        } else {
+          s_st[idx] = yv;
+          s_st[LIN + idx] = yv * yv;
+        }
       }
     }
     __syncthreads();
     if (k + 1 < nk) fetch(k + 1);
+    // the target statistics of this step's map rows (stage C), loaded ahead
+    float t_mu[2] = {0.0f, 0.0f}, t_yy[2] = {0.0f, 0.0f};
+    if (MODE == SW_CACHED && k > 0) {
+      const int gx = x0 - 5 + lane;
+#pragma unroll
+      for (int o = 0; o < 2; ++o) {
+        const int gy = Y + LRB * k - 5 + 2 * wv + o;
+        if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+          const size_t q = (size_t)gy * W + gx;
+          t_mu[o] = ts_mu[q];
+          t_yy[o] = ts_yy[q];
+        }
+      }
+    }
+    // the image and target values of this step's output pixels (stage E),
+    // loaded ahead
+    const int or0 = LRB * k - 10 + 2 * wv;
+    const bool e_rows = SP::MAPS && Y + or0 + 1 >= y0 && lane < LSW;
+    float e_x[2] = {0.0f, 0.0f}, e_y[2] = {0.0f, 0.0f};
+    if (e_rows) {
+      const int gx = x0 + lane;
+#pragma unroll
+      for (int o = 0; o < 2; ++o) {
+        const int gy = Y + or0 + o;
+        if (gx < W && gy >= y0 && gy < yend) {
+          const size_t q = (size_t)gy * W + gx;
+          e_x[o] = in.rgb[(pb + q) * 3 + ch];
+          e_y[o] = in.target[(tb + q) * 4 + ch];
+        }
+      }
+    }
     // B: products of input row r blurred along the row, map columns c0, c0 + 1
     {
       const int r = t >> 5, c0 = (t & 31) * 2;
-      float a[5][2];
+      float a[NP][2];
 #pragma unroll
-      for (int m = 0; m < 5; ++m) a[m][0] = a[m][1] = 0.0f;
+      for (int m = 0; m < NP; ++m) a[m][0] = a[m][1] = 0.0f;
 #pragma unroll
       for (int w = 0; w < 12; ++w) {
-        const float xv = s_st[r * LIC + c0 + w], yv = s_st[LIN + r * LIC + c0 + w];
-        const float pr[5] = {xv, yv, xv * xv, yv * yv, xv * yv};
+        float pr[NP];
+        if (MODE == SW_FULL) {
+          const float xv = s_st[r * LIC + c0 + w], yv = s_st[LIN + r * LIC + c0 + w];
+          const float p5[5] = {xv, yv, xv * xv, yv * yv, xv * yv};
+#pragma unroll
+          for (int m = 0; m < NP; ++m) pr[m] = p5[m];
+        } else {
+#pragma unroll
+          for (int m = 0; m < NP; ++m) pr[m] = s_st[m * LIN + r * LIC + c0 + w];
+        }
 #pragma unroll
         for (int o = 0; o < 2; ++o) {
           const int ki = w - o;
           if (ki < 0 || ki > 10) continue;
           const float kk = K.k[ki];
 #pragma unroll
-          for (int m = 0; m < 5; ++m) a[m][o] += kk * pr[m];
+          for (int m = 0; m < NP; ++m) a[m][o] += kk * pr[m];
         }
       }
       const int slot = (LRB * k + r) % LRING;
 #pragma unroll
-      for (int m = 0; m < 5; ++m) {
+      for (int m = 0; m < NP; ++m) {
         s_hp[m][slot][c0] = a[m][0];
         s_hp[m][slot][c0 + 1] = a[m][1];
       }
@@ -165,22 +237,22 @@ __global__ __launch_bounds__(LT) void k_ssim_sweep(LossIn in, Kern11 K, float* _
     // mr0 - 5 .. mr0 + 6
     if (k > 0) {
       const int mr0 = LRB * k - 5 + 2 * wv;
-      float mu[5][2];
+      float mu[NP][2];
 #pragma unroll
-      for (int m = 0; m < 5; ++m) mu[m][0] = mu[m][1] = 0.0f;
+      for (int m = 0; m < NP; ++m) mu[m][0] = mu[m][1] = 0.0f;
 #pragma unroll
       for (int w = 0; w < 12; ++w) {
         const int slot = (mr0 - 5 + w + 2 * LRING) % LRING;
-        float hv[5];
+        float hv[NP];
 #pragma unroll
-        for (int m = 0; m < 5; ++m) hv[m] = s_hp[m][slot][lane];
+        for (int m = 0; m < NP; ++m) hv[m] = s_hp[m][slot][lane];
 #pragma unroll
         for (int o = 0; o < 2; ++o) {
           const int ki = w - o;
           if (ki < 0 || ki > 10) continue;
           const float kk = K.k[ki];
 #pragma unroll
-          for (int m = 0; m < 5; ++m) mu[m][o] += kk * hv[m];
+          for (int m = 0; m < NP; ++m) mu[m][o] += kk * hv[m];
         }
       }
       const int gx = x0 - 5 + lane;
@@ -188,10 +260,26 @@ __global__ __launch_bounds__(LT) void k_ssim_sweep(LossIn in, Kern11 K, float* _
       for (int o = 0; o < 2; ++o) {
         const int gy = Y + mr0 + o;
         const bool in_img = gx >= 0 && gx < W && gy >= 0 && gy < H;
-        const float mx = mu[0][o], my = mu[1][o];
-        const float sx2 = mu[2][o] - mx * mx;
-        const float sy2 = mu[3][o] - my * my;
-        const float sxy = mu[4][o] - mx * my;
+        const bool own = in_img && gy >= y0 && gy < yend && lane >= 5 && lane < 5 + LSW;
+        if (MODE == SW_STATS) {
+          if (own) {
+            const size_t q = (size_t)gy * W + gx;
+            out[((size_t)ch * 2 + 0) * in.tgt_total + tb + q] = mu[0][o];
+            out[((size_t)ch * 2 + 1) * in.tgt_total + tb + q] = mu[1][o];
+          }
+          continue;
+        }
+        float mx, my, exx, eyy, exy;
+        if (MODE == SW_FULL) {
+          mx = mu[0][o], my = mu[1][o], exx = mu[2][o], eyy = mu[3][o], exy = mu[4][o];
+        } else {
+          mx = mu[0][o], exx = mu[1][o], exy = mu[2][o];
+          my = t_mu[o];
+          eyy = t_yy[o];
+        }
+        const float sx2 = exx - mx * mx;
+        const float sy2 = eyy - my * my;
+        const float sxy = exy - mx * my;
         const float a1 = 2.0f * mx * my + C1;
         const float a2 = 2.0f * sxy + C2;
         const float b1 = mx * mx + my * my + C1;
@@ -210,8 +298,12 @@ __global__ __launch_bounds__(LT) void k_ssim_sweep(LossIn in, Kern11 K, float* _
         s_st[(0 * LRB + row) * LMC + lane] = in_img ? pc - qc * my - 2.0f * (rc * mx) : 0.0f;
         s_st[(1 * LRB + row) * LMC + lane] = in_img ? qc : 0.0f;
         s_st[(2 * LRB + row) * LMC + lane] = in_img ? rc : 0.0f;
-        if (in_img && gy >= y0 && gy < yend && lane >= 5 && lane < 5 + LSW) ss += ssim;
+        if (own) ss += ssim;
       }
+    }
+    if (!SP::MAPS) {
+      __syncthreads();  // the product ring's oldest rows may be rewritten
+      continue;
     }
     __syncthreads();
     // D: map row r blurred along the row, output columns c0, c0 + 1
@@ -246,8 +338,7 @@ __global__ __launch_bounds__(LT) void k_ssim_sweep(LossIn in, Kern11 K, float* _
     __syncthreads();
     // E: output rows or0, or0 + 1 of output column `lane`, from map rows
     // or0 - 5 .. or0 + 6; the first rows of the sweep are ahead of the unit
-    const int or0 = LRB * k - 10 + 2 * wv;
-    if (Y + or0 + 1 >= y0 && lane < LSW) {
+    if (e_rows) {
       float b[3][2];
 #pragma unroll
       for (int m = 0; m < 3; ++m) b[m][0] = b[m][1] = 0.0f;
@@ -272,16 +363,17 @@ __global__ __launch_bounds__(LT) void k_ssim_sweep(LossIn in, Kern11 K, float* _
         const int gy = Y + or0 + o;
         if (gx >= W || gy < y0 || gy >= yend) continue;
         const size_t q = (size_t)gy * W + gx;
-        const float xv = in.rgb[(pb + q) * 3 + ch], yv = in.target[(tb + q) * 4 + ch];
+        const float xv = e_x[o], yv = e_y[o];
         const float d = xv - yv;
         l1 += __builtin_fabsf(d);
         // Rust f32::signum: +1 for +0.0, NaN for NaN
         const float l1_term = d != d ? d : (__builtin_signbitf(d) ? l1_neg : l1_pos);
         const float grad = b[0][o] + yv * b[1][o] + 2.0f * xv * b[2][o];
-        dl[(pb + q) * 3 + ch] = l1_term + s_scale * grad;
+        out[(pb + q) * 3 + ch] = l1_term + s_scale * grad;
       }
     }
   }
+  if (!SP::MAPS) return;
   const float bl1 = block_sum_256(l1, s_tmp);
   const float bss = block_sum_256(ss, s_tmp);
   if (t == 0) {
@@ -340,8 +432,17 @@ static Kern11 make_kernel() {
 void launch_loss(const LossIn& in, float* dl, float* partial, float* loss_out, hipStream_t st) {
   if (in.total_lunits <= 0) return;
   static const Kern11 K = make_kernel();
-  hipLaunchKernelGGL(k_ssim_sweep, dim3(in.total_lunits * 3), dim3(LT), 0, st, in, K, dl, partial);
+  if (in.tstats)
+    hipLaunchKernelGGL(k_ssim_sweep<SW_CACHED>, dim3(in.total_lunits * 3), dim3(LT), 0, st, in, K, dl, partial);
+  else
+    hipLaunchKernelGGL(k_ssim_sweep<SW_FULL>, dim3(in.total_lunits * 3), dim3(LT), 0, st, in, K, dl, partial);
   hipLaunchKernelGGL(k_loss_reduce, dim3(in.n_views), dim3(256), 0, st, in, partial, loss_out);
+}
+
+void launch_target_stats(const LossIn& in, float* tstats, hipStream_t st) {
+  if (in.total_lunits <= 0) return;
+  static const Kern11 K = make_kernel();
+  hipLaunchKernelGGL(k_ssim_sweep<SW_STATS>, dim3(in.total_lunits * 3), dim3(LT), 0, st, in, K, tstats, nullptr);
 }
 
 }  // namespace rgs

@@ -277,6 +277,7 @@ struct rgs_ctx {
   uint64_t t = 0;
   std::vector<CamDev> reg_cams;
   DBuf targets;
+  DBuf tstats;  // the targets' blurred statistics for the loss (launch_target_stats)
   size_t target_pix = 0;
   DBuf adam_d;
   AdamConsts* h_adam = nullptr;  // pinned
@@ -711,9 +712,35 @@ int ensure_loss(rgs_ctx* c, const ViewSet& vs) {
   return 0;
 }
 
-void enqueue_loss(rgs_ctx* c, const ViewSet& vs, const float* rgb, const float* target, float lambda) {
+// tstats: null, or the statistics of the target store `target` is (the
+// registered views' targets).
+void enqueue_loss(rgs_ctx* c, const ViewSet& vs, const float* rgb, const float* target, float lambda,
+                  const float* tstats = nullptr) {
   LossIn li{rgb, target, P_<CamDev>(c->cams_d), (int)vs.cams.size(), vs.total_lunits, vs.total_pix, lambda};
+  li.tstats = tstats;
+  li.tgt_total = c->target_pix;
   launch_loss(li, P_<float>(c->dl), P_<float>(c->partial), P_<float>(c->loss), c->st);
+}
+
+// After the target store changed: registers the views and makes the
+// targets' loss statistics (one sweep per view, stream order).
+int finish_views(rgs_ctx* c, const std::vector<CamDev>& v, size_t tot) {
+  CHK(ensure(c, c->tstats, tot * 6 * 4));
+  for (const CamDev& cam : v) {
+    LossIn li{};
+    li.target = P_<float>(c->targets);
+    li.n_views = 1;
+    li.total_lunits = loss_units(cam.w, cam.h);
+    li.cam0 = cam;
+    li.cam0.lunit_base = 0;
+    li.tgt_total = tot;
+    launch_target_stats(li, P_<float>(c->tstats), c->st);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->reg_cams = v;
+  c->target_pix = tot;
+  return 0;
 }
 
 AdamConsts adam_consts(const rgs_train_cfg& cfg, uint64_t t) {
@@ -826,7 +853,7 @@ void rgs_destroy(rgs_ctx* c) {
                  &c->partial, &c->loss, &c->accum, &c->api_aos, &c->api_P, &c->api_grads,
                  &c->api_target, &c->api_m1, &c->api_m2, &c->api_acc, &c->api_den, &c->api_P2, &c->dens_want,
                  &c->dens_cnt, &c->dens_W, &c->dens_packed, &c->dens_prefix, &c->dens_b32, &c->dens_b64,
-                 &c->dens_t32, &c->dens_t64, &c->first_splat, &c->P, &c->P2, &c->M1, &c->M2, &c->gacc, &c->gden, &c->targets,
+                 &c->dens_t32, &c->dens_t64, &c->first_splat, &c->P, &c->P2, &c->M1, &c->M2, &c->gacc, &c->gden, &c->targets, &c->tstats,
                  &c->io_soa, &c->io_rec, &c->io_fields, &c->srgb_lut, &c->sfm_xyz, &c->sfm_rgb, &c->sfm_k[0],
                  &c->sfm_k[1], &c->sfm_v[0], &c->sfm_v[1], &c->sfm_khi, &c->sfm_sx, &c->sfm_scale, &c->sfm_hist,
                  &c->sfm_rowtot, &c->sfm_misc, &c->pv_sp, &c->pv_k[0], &c->pv_k[1], &c->pv_v[0], &c->pv_v[1],
@@ -1921,10 +1948,7 @@ int rgs_set_views_rgba8_sized(rgs_ctx* c, const rgs_camera* cams, const uint8_t*
     }
   }
   launch_rgba8_to_linear(P_<uint8_t>(c->io_soa), tot, P_<float>(c->srgb_lut), P_<float>(c->targets), c->st);
-  HIPCHK(hipStreamSynchronize(c->st));
-  c->reg_cams = v;
-  c->target_pix = tot;
-  return 0;
+  return finish_views(c, v, tot);
 }
 
 int rgs_rgba8_to_linear(rgs_ctx* c, const uint8_t* rgba8, size_t n_pixels, float* out) {
@@ -1971,10 +1995,7 @@ int rgs_set_views(rgs_ctx* c, const rgs_camera* cams, const float* const* target
   for (int i = 0; i < nv; ++i)
     HIPCHK(hipMemcpyAsync(P_<float>(c->targets) + (size_t)v[i].tgt_base * 4, targets[i],
                           (size_t)cams[i].width * cams[i].height * 16, hipMemcpyHostToDevice, c->st));
-  HIPCHK(hipStreamSynchronize(c->st));
-  c->reg_cams = v;
-  c->target_pix = tot;
-  return 0;
+  return finish_views(c, v, tot);
 }
 
 int rgs_set_views_rgba8(rgs_ctx* c, const rgs_camera* cams, const uint8_t* const* images, int nv) {
@@ -1997,10 +2018,7 @@ int rgs_set_views_rgba8(rgs_ctx* c, const rgs_camera* cams, const uint8_t* const
     HIPCHK(hipMemcpyAsync(P_<uint8_t>(c->io_soa) + (size_t)v[i].tgt_base * 4, images[i],
                           (size_t)cams[i].width * cams[i].height * 4, hipMemcpyHostToDevice, c->st));
   launch_rgba8_to_linear(P_<uint8_t>(c->io_soa), tot, P_<float>(c->srgb_lut), P_<float>(c->targets), c->st);
-  HIPCHK(hipStreamSynchronize(c->st));
-  c->reg_cams = v;
-  c->target_pix = tot;
-  return 0;
+  return finish_views(c, v, tot);
 }
 
 // Enqueues the step after the projection/depth-sort/scan prefix.
@@ -2016,7 +2034,7 @@ static void enqueue_step_rest(rgs_ctx* c, const ViewSet& vs, const float bg[3], 
   uint32_t* sc = P_<uint32_t>(c->scalars);
   enqueue_bin_and_raster_cut(c, cap, n, vs, bg, P_<float>(c->rgb), step_cover(c, bg), with_back);
   stage_begin(c, S_LOSS);
-  enqueue_loss(c, vs, P_<float>(c->rgb), P_<float>(c->targets), lambda);
+  enqueue_loss(c, vs, P_<float>(c->rgb), P_<float>(c->targets), lambda, P_<float>(c->tstats));
   stage_end(c, S_LOSS);
   stage_begin(c, S_BWD);
   RasterIn in = raster_in(c, vs, bg);
@@ -2254,7 +2272,7 @@ int rgs_autotune_raster(rgs_ctx* c, const int* ids, int n_ids, const float bg[3]
   CHK(check_pairs(K));
   if (K > c->k_cap) CHK(ensure_render(c, cap, n, vs, K + K / 4 + 4096));
   enqueue_bin_and_raster(c, cap, n, vs, bg, 0, P_<float>(c->rgb));
-  enqueue_loss(c, vs, P_<float>(c->rgb), P_<float>(c->targets), lambda);
+  enqueue_loss(c, vs, P_<float>(c->rgb), P_<float>(c->targets), lambda, P_<float>(c->tstats));
   hipEvent_t ev[4];
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   auto time_once = [&](float* ms) -> int {

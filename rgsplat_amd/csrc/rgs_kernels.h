@@ -257,10 +257,17 @@ struct LossIn {
   size_t total_pix;
   float lambda;
   CamDev cam0;          // the view when cams is null (w, h, pix_base, tgt_base, lunit_base)
+  // null, or the target store's statistics (launch_target_stats), planar
+  // [channel][mu_y | E[y^2]][tgt_total]
+  const float* tstats;
+  size_t tgt_total;     // pixels of the target store
 };
 // dl: 3 * total_pix floats; partial: 2 * 3 * total_lunits floats (scratch);
 // loss_out: n_views floats. Allocates nothing.
 void launch_loss(const LossIn& in, float* dl, float* partial, float* loss_out, hipStream_t st);
+// The blurred target and squared target of in's views into tstats
+// (6 * tgt_total floats), for later launch_loss calls on those targets.
+void launch_target_stats(const LossIn& in, float* tstats, hipStream_t st);
 
 // ---- optimizer and parameter kernels (k_optim.hip)
 struct AdamConsts {

@@ -142,6 +142,58 @@ def test_loss_grad_parity(ctx):
     assert abs(loss - loss_ref) <= 1e-5 * abs(loss_ref)
 
 
+@pytest.mark.parametrize("w,h", [(333, 400), (1, 170), (170, 1), (54, 155), (55, 156)])
+def test_loss_grad_parity_unit_edges(ctx, w, h):
+    """The loss sweeps 54-column x 155-row units eight rows per step: images
+    spanning several units, one pixel wide or tall, and on and just past the
+    unit size, against the oracle (same bars as test_loss_grad_parity)."""
+    r = S.Stream(23)
+    rgb = np.clip(r.normal((h, w, 3)) * 0.2 + 0.5, 0, 1).astype(np.float32)
+    tg = with_alpha(np.clip(rgb + r.normal((h, w, 3)) * 0.05, 0, 1).astype(np.float32))
+    loss, dl = ctx.photometric_loss_grad(rgb, tg, w, h, 0.2)
+    # the reference's loss is a serial f32 sum (drifting by ~1e-5 relative
+    # at 400k terms); exact: the reference's own f32 terms added in double
+    _, loss_exact, dl_ref = oracle.loss_grad_exact(rgb, tg, w, h, 0.2)
+    d = (dl - dl_ref).astype(np.float64)
+    assert np.linalg.norm(d) <= 1e-5 * np.linalg.norm(dl_ref)
+    assert np.abs(d).max() <= 1e-4 * np.abs(dl_ref).max()
+    assert abs(loss - loss_exact) <= 1e-6 * abs(loss_exact)
+
+
+def test_step_loss_uses_target_statistics():
+    """The training step reads the registered targets' blurred statistics
+    (mu_y and E[y^2], made once per set_views) instead of blurring the targets
+    every step; the values are the same fused multiply-add chains, so each
+    view's loss equals, bit for bit, rgs_loss_grad's (which blurs the target
+    itself) on the step's own image. Registering other targets replaces the
+    statistics."""
+    from rgsplat_amd import runtime
+    g, cams, targets = workload("c0_10k_256", 20_000, [(333, 400), (256, 192), (55, 156)])
+    cfg = S.benchmark_train_config()
+    frozen = dict(cfg, lr_position=0.0, lr_rotation=0.0, lr_scale=0.0, lr_opacity=0.0, lr_sh_dc=0.0,
+                  lr_sh_rest=0.0)
+    npx = sum(c.width * c.height for c in cams)
+    c = runtime.Context(0)
+    try:
+        c.scene_upload(g)
+        for round_ in range(2):
+            if round_ == 1:  # other targets: the statistics must follow
+                r = S.Stream(5)
+                targets = [with_alpha(np.clip(t[..., :3] + r.normal(t[..., :3].shape) * 0.1, 0, 1)
+                                      .astype(np.float32)) for t in targets]
+            c.set_views(cams, targets)
+            loss = c.train_step([0, 1, 2], frozen, use_graph=False)
+            img = c.step_image(npx)
+            off = 0
+            for v, cam in enumerate(cams):
+                n = cam.width * cam.height
+                api_loss, _ = c.photometric_loss_grad(img[off:off + n], targets[v], cam.width, cam.height, 0.2)
+                assert np.float32(api_loss) == loss[v], (round_, v, api_loss, loss[v])
+                off += n
+    finally:
+        c.close()
+
+
 # -------------------------------------------------------------- backward ---
 @pytest.mark.parametrize("bg,lam", [((0.0, 0.0, 0.0), 0.2), ((0.1, 0.4, 0.2), 0.5)])
 def test_render_and_backward_parity(ctx, bg, lam):
