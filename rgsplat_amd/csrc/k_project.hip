@@ -26,11 +26,16 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ P, in
   const float px = P[(size_t)0 * cap + i], py = P[(size_t)1 * cap + i], pz = P[(size_t)2 * cap + i];
   bool sigma_done = false;
   M3 sigma;
+  // rotation and log-scales loaded up front (used from the first visible view)
+  const float qw_ = P[(size_t)(P_ROT + 0) * cap + i], qx_ = P[(size_t)(P_ROT + 1) * cap + i];
+  const float qy_ = P[(size_t)(P_ROT + 2) * cap + i], qz_ = P[(size_t)(P_ROT + 3) * cap + i];
+  const float l0_ = P[(size_t)(P_LS + 0) * cap + i], l1_ = P[(size_t)(P_LS + 1) * cap + i];
+  const float l2_ = P[(size_t)(P_LS + 2) * cap + i];
   // Views go in chunks of VCH: the first pass projects each view of the chunk
   // and keeps its view direction (in LDS, one slot per thread); the second
-  // reads the 48 SH coefficients once and evaluates the colour of every
-  // visible view (the per-view sums keep sh.rs's k = 0..15 order, so colours
-  // do not depend on the chunking).
+  // evaluates the colour of every visible view channel by channel, each
+  // channel's 16 SH coefficients read once (the per-view sums keep sh.rs's
+  // k = 0..15 order, so colours do not depend on the chunking).
   constexpr int VCH = 4;
   const int t = threadIdx.x;
   for (int v0 = 0; v0 < n_views; v0 += VCH) {
@@ -66,12 +71,10 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ P, in
         continue;
       }
       if (!sigma_done) {
-        float qw = P[(size_t)(P_ROT + 0) * cap + i], qx = P[(size_t)(P_ROT + 1) * cap + i];
-        float qy = P[(size_t)(P_ROT + 2) * cap + i], qz = P[(size_t)(P_ROT + 3) * cap + i];
-        float s0 = rgs_expf(P[(size_t)(P_LS + 0) * cap + i]);
-        float s1 = rgs_expf(P[(size_t)(P_LS + 1) * cap + i]);
-        float s2 = rgs_expf(P[(size_t)(P_LS + 2) * cap + i]);
-        sigma = covariance_3d(qw, qx, qy, qz, s0, s1, s2);
+        float s0 = rgs_expf(l0_);
+        float s1 = rgs_expf(l1_);
+        float s2 = rgs_expf(l2_);
+        sigma = covariance_3d(qw_, qx_, qy_, qz_, s0, s1, s2);
         sigma_done = true;
       }
       // project_covariance (projection.rs:24-69): tz clamped at 0.001 (no-op here)
@@ -178,32 +181,41 @@ __global__ __launch_bounds__(256) void k_project(const float* __restrict__ P, in
       vis |= 1u << j;
     }
     if (!vis) continue;
-    // an empty asm makes the base pointer chunk-variant, so that the loads
-    // are not hoisted above the first pass (48 registers live across it)
-    const float* Pv = P;
-    asm volatile("" : "+s"(Pv));
-    float coef[48];
+    // colours channel by channel: 16 coefficients live at a time, the basis
+    // rebuilt from the view direction per channel (the sums keep sh.rs's
+    // k = 0..15 order)
+    float col[VCH][3];
 #pragma unroll
-    for (int q = 0; q < 48; ++q) coef[q] = Pv[(size_t)(P_SH + q) * cap + i];
-    for (int j = 0; j < VCH; ++j) {
-      if (!((vis >> j) & 1u)) continue;
-      float basis[16];
-      sh_basis(s_dir[j][0][t], s_dir[j][1][t], s_dir[j][2][t], basis);
-      float col[3];
+    for (int ch = 0; ch < 3; ++ch) {
+      // an empty asm makes the base pointer channel-variant, so that the
+      // loads are not hoisted above the first pass
+      const float* Pv = P;
+      asm volatile("" : "+s"(Pv));
+      float cf[16];
 #pragma unroll
-      for (int ch = 0; ch < 3; ++ch) {
+      for (int k = 0; k < 16; ++k) cf[k] = Pv[(size_t)(P_SH + ch * 16 + k) * cap + i];
+#pragma unroll
+      for (int j = 0; j < VCH; ++j) {
+        col[j][ch] = 0.0f;
+        if (!((vis >> j) & 1u)) continue;
+        float basis[16];
+        sh_basis(s_dir[j][0][t], s_dir[j][1][t], s_dir[j][2][t], basis);
         float acc = 0.0f;
 #pragma unroll
-        for (int k = 0; k < 16; ++k) acc += basis[k] * coef[ch * 16 + k];
+        for (int k = 0; k < 16; ++k) acc += basis[k] * cf[k];
         float raw = 0.5f + acc;
         // f32::clamp(0, 1)
         if (raw < 0.0f) raw = 0.0f;
         if (raw > 1.0f) raw = 1.0f;
-        col[ch] = raw;
+        col[j][ch] = raw;
       }
+    }
+#pragma unroll
+    for (int j = 0; j < VCH; ++j) {
+      if (!((vis >> j) & 1u)) continue;
       const size_t vi = (size_t)(v0 + j) * cap + i;
-      S.c_op_rg[vi] = make_float4(s_dir[j][3][t], opacity, col[0], col[1]);
-      S.col_b[vi] = col[2];
+      S.c_op_rg[vi] = make_float4(s_dir[j][3][t], opacity, col[j][0], col[j][1]);
+      S.col_b[vi] = col[j][2];
     }
   }
 }
