@@ -549,8 +549,15 @@ def main():
             "raster_fwd": ("valu", FWD_OPS_PER_PAIR * pf, "TFLOP/s"),
             "raster_bwd": ("valu", BWD_OPS_PER_PAIR * pb, "TFLOP/s"),
             "loss_ssim": ("valu", SSIM_FLOP_PER_PIXEL * npix, "TFLOP/s"),
-            "project": ("hbm", 236 * ng + 60 * ng * len(cams), "GB/s"),
-            "geometry_adam": ("hbm", 1652 * ng, "GB/s"),
+            # the projection reads 59 parameters per Gaussian and writes 64 B
+            # per (Gaussian, view): sort key and value, depth, screen mean and
+            # conic, pair count, tile rect, conic a / opacity / colour
+            "project": ("hbm", 236 * ng + 64 * ng * len(cams), "GB/s"),
+            # the update reads and writes the 59 parameters and both Adam
+            # moments (1416 B), reads and clears the nine raster accumulators
+            # and reads the pair count of every view (76 B per view), and
+            # updates the two densification statistics (16 B)
+            "geometry_adam": ("hbm", (1432 + 76 * len(cams)) * ng, "GB/s"),
             # pair generation writes 8 B per (bin, splat) pair; each tile-sort
             # pass reads and writes key and value (16 B) and the histogram
             # pass reads the key again (4 B)
