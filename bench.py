@@ -498,7 +498,8 @@ def main():
                       "resolutions": sorted(set(f"{w}x{h}" for w, h in wl.sizes)),
                       "execution": "device-resident step, views batched per launch, "
                                    + ("hipGraph replay" if use_graph else "eager"),
-                      "raster_params": dict(zip(("super_tile", "sparse_lanes"), ctx.raster_params())),
+                      "raster_params": dict(zip(("super_tile", "sparse_lanes"), ctx.raster_params()),
+                                            batch=ctx.raster_batch()),
                       "raster_autotuned": tuned is not None}}
 
     if profile:

@@ -242,9 +242,16 @@ int rgs_step_image(rgs_ctx* ctx, float* rgb, size_t n_floats);
  * with LDS atomics instead of a cross-lane reduction (0..64).
  * rgs_autotune_raster renders the registered views ids on the device-resident
  * scene and times the forward + backward raster under candidate settings
- * (coordinate search, best of two runs each), keeps the fastest and reports
- * it; the scene and optimizer state are not changed. */
+ * (coordinate search over super_tile, sparse_lanes and, with binning, the
+ * staging batch; best of two runs each), keeps the fastest and reports it
+ * (the batch through rgs_get_raster_batch); the scene and optimizer state
+ * are not changed. */
 int rgs_set_raster_params(rgs_ctx* ctx, int super_tile, int sparse_lanes);
+/* Splats the binned training raster stages in LDS per batch: 32, 64 (the
+ * default) or 128 (no reference counterpart; images do not depend on it,
+ * gradients only through the order of float summation). */
+int rgs_set_raster_batch(rgs_ctx* ctx, int batch);
+int rgs_get_raster_batch(rgs_ctx* ctx, int* batch);
 /* Binning of rgs_train_step: pairs per bin of 2^shift x 2^shift tiles
  * (0: the reference's 16x16 tiles; 1, the default: 32x32). The backward
  * rasterises a bin with one workgroup of four waves that share the staged

@@ -174,18 +174,23 @@ __device__ __forceinline__ void count_add(unsigned long long* counts, uint32_t n
 // COUNT: the same pass, also adding this kernel's own work to in.counts
 // (0: lane evaluations of a (pixel, splat) pair, 1: wave visits of a
 // (splat, pixel group), 2: pairs that composite).
-template <int MODE, int S, int NW, bool COUNT = false>
+// FB: splats staged per batch (32, 64 or 128); a batch of 128 is staged as
+// two halves of 64 (one per lane each) and walked half by half.
+template <int MODE, int S, int NW, bool COUNT = false, int FB = FT>
 __global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_raster_fwd(
     RasterIn in, float* __restrict__ out_rgb, float* __restrict__ out_t, int32_t* __restrict__ out_last) {
   static_assert(NW == 1 || NW == S * S, "waves per workgroup");
+  static_assert(FB == 32 || FB == 64 || FB == 128, "batch");
   constexpr bool FAST = MODE >= 2;
   constexpr bool RGBA = MODE == 1;
+  constexpr int NH = FB > FT ? FB / FT : 1;  // halves of 64 per batch
+  constexpr int LW = FB < FT ? FB : FT;      // lanes staging each half
   __shared__ uint64_t s_tab[32];
-  __shared__ float4 s_a[FT];
-  __shared__ float4 s_b[FT];
-  __shared__ float s_c[FT];
-  __shared__ float s_r[FT];
-  __shared__ uint2 s_rect[FT];
+  __shared__ float4 s_a[NH * FT];
+  __shared__ float4 s_b[NH * FT];
+  __shared__ float s_c[NH * FT];
+  __shared__ float s_r[NH * FT];
+  __shared__ uint2 s_rect[NH * FT];
   __shared__ int32_t s_fin[NW];
   const int lane = threadIdx.x & 63;
   // this wave's tile in the bin (wave-uniform), and its LDS slot
@@ -248,39 +253,48 @@ __global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(7, 8)))
   }
   int32_t fin = -1;  // list index at which this wave's last pixel saturated
   uint32_t n_eval = 0, n_visit = 0, n_comp = 0;
-  for (uint32_t b0 = start; b0 < end; b0 += FT) {
+  for (uint32_t b0 = start; b0 < end; b0 += FB) {
     // stop when every pixel of the bin has saturated; the barrier also ends
     // the previous batch's reads of the staged splats
     if (!__syncthreads_or(pixels_left())) break;
-    const uint32_t idx = b0 + lane;
-    if (idx < end) {
-      const uint32_t vi = vals[idx];
-      if (NW == 1 || ws == 0) s_a[lane] = in.S.xy_ab[vi];
-      if (NW == 1 || ws == 1 % NW) s_b[lane] = in.S.c_op_rg[vi];
-      if (NW == 1 || ws == 2 % NW) s_c[lane] = in.S.col_b[vi];
-      if (NW == 1 || ws == 3 % NW) {
-        s_r[lane] = in.S.reach[vi];
-        if (S > 1) s_rect[lane] = in.S.rect[vi];
+#pragma unroll
+    for (int h = 0; h < NH; ++h) {
+      const uint32_t idx = b0 + h * FT + lane;
+      if (lane < LW && idx < end) {
+        const uint32_t vi = vals[idx];
+        const int sl = h * FT + lane;
+        if (NW == 1 || ws == 0) s_a[sl] = in.S.xy_ab[vi];
+        if (NW == 1 || ws == 1 % NW) s_b[sl] = in.S.c_op_rg[vi];
+        if (NW == 1 || ws == 2 % NW) s_c[sl] = in.S.col_b[vi];
+        if (NW == 1 || ws == 3 % NW) {
+          s_r[sl] = in.S.reach[vi];
+          if (S > 1) s_rect[sl] = in.S.rect[vi];
+        }
       }
     }
     __syncthreads();
     // all lanes take part in the ballot: the lanes past the list's end may be
     // the ones with pixels left
     const bool wave_live = __ballot(pixels_left()) != 0;
+    bool done = false;
+#pragma unroll 1
+    for (int h = 0; h < NH && !done; ++h) {
+    const uint32_t idx = b0 + h * FT + lane;
     uint32_t my_m = 0;  // group mask of this lane's staged splat for this wave's tile
-    if (idx < end && wave_live && (S == 1 || in_rect(s_rect[lane], tx, ty))) {
-      const float4 A = s_a[lane];
-      const float4 B = s_b[lane];
-      my_m = wave_mask(A.x, A.y, A.z, A.w, B.x, s_r[lane], tx, ty);
+    if (lane < LW && idx < end && wave_live && (S == 1 || in_rect(s_rect[h * FT + lane], tx, ty))) {
+      const float4 A = s_a[h * FT + lane];
+      const float4 B = s_b[h * FT + lane];
+      my_m = wave_mask(A.x, A.y, A.z, A.w, B.x, s_r[h * FT + lane], tx, ty);
     }
     // splats that reach some pixel group, visited front to back as set bits
     // of a scalar mask (their LDS reads are not prefetched: holding a second
     // register set costs more than the other waves need to cover the wait)
     uint64_t live = __ballot(my_m != 0);
     while (live) {
-      const int j = __builtin_ctzll(live);
+      const int jl = __builtin_ctzll(live);
       live &= live - 1;
-      const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)my_m, j);
+      const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)my_m, jl);
+      const int j = h * FT + jl;
       const float4 A = s_a[j];
       const float4 B = s_b[j];
       const float cb = s_c[j];
@@ -323,8 +337,10 @@ __global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(7, 8)))
       }
       if (__ballot(pixels_left()) == 0) {
         fin = li;
+        done = true;
         break;
       }
+    }
     }
   }
   if (COUNT) count_add(in.counts, n_eval, n_visit, n_comp, lane);
@@ -389,16 +405,26 @@ __global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(7, 8)))
   }
 }
 
+// The binned training forward (MODE 2, 3) in the staging batch in.batch.
+template <int MODE, int NW>
+static void fwd_binned(const RasterIn& in, dim3 grid, float* rgb, float* final_t, int32_t* last, hipStream_t st) {
+  const int b = MODE >= 2 ? in.batch : FT;
+  if (b == 32)
+    hipLaunchKernelGGL((k_raster_fwd<MODE, 2, NW, false, 32>), grid, dim3(FT * NW), 0, st, in, rgb, final_t, last);
+  else if (b == 128)
+    hipLaunchKernelGGL((k_raster_fwd<MODE, 2, NW, false, 128>), grid, dim3(FT * NW), 0, st, in, rgb, final_t, last);
+  else
+    hipLaunchKernelGGL((k_raster_fwd<MODE, 2, NW>), grid, dim3(FT * NW), 0, st, in, rgb, final_t, last);
+}
+
 template <int MODE>
 static void fwd_launch(const RasterIn& in, float* rgb, float* final_t, int32_t* last, hipStream_t st) {
   if (in.bin_shift == 0)
     hipLaunchKernelGGL((k_raster_fwd<MODE, 1, 1>), dim3(in.total_bins), dim3(FT), 0, st, in, rgb, final_t, last);
   else if (in.fwd_tile_blocks)
-    hipLaunchKernelGGL((k_raster_fwd<MODE, 2, 1>), dim3(in.total_bins * 4), dim3(FT), 0, st, in, rgb, final_t,
-                       last);
+    fwd_binned<MODE, 1>(in, dim3(in.total_bins * 4), rgb, final_t, last, st);
   else
-    hipLaunchKernelGGL((k_raster_fwd<MODE, 2, 4>), dim3(in.total_bins), dim3(FT * 4), 0, st, in, rgb, final_t,
-                       last);
+    fwd_binned<MODE, 4>(in, dim3(in.total_bins), rgb, final_t, last, st);
 }
 
 void launch_raster_fwd(const RasterIn& in, int mode, float* rgb, float* final_t, int32_t* last, hipStream_t st) {
@@ -467,22 +493,29 @@ __device__ __forceinline__ int rev3(int l) { return ((l & 1) << 2) | (l & 2) | (
 constexpr int BT = 64;
 // COUNT: adds this kernel's own work to in.counts (3: lane evaluations, 4:
 // wave visits, 5: contributing pairs) and writes no accumulator.
-template <int S, bool COUNT = false>
+// FB: splats staged per batch (32, 64 or 128; 128 as two halves of 64,
+// walked back to front).
+template <int S, bool COUNT = false, int FB = BT>
 __global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_raster_bwd(RasterIn in, const float* __restrict__ final_t,
                                                            const int32_t* __restrict__ last_contrib,
                                                            const float* __restrict__ dl_dpix,
                                                            float* __restrict__ accum) {
+  static_assert(FB == 32 || FB == 64 || FB == 128, "batch");
   constexpr int NW = S * S;
+  constexpr int NH = FB > BT ? FB / BT : 1;  // halves of 64 per batch
+  constexpr int LW = FB < BT ? FB : BT;      // lanes staging each half
+  constexpr int GS = NH * BT + 1;            // s_g row stride
   __shared__ uint64_t s_tab[32];
-  __shared__ float4 s_a[BT];
-  __shared__ float4 s_b[BT];
-  __shared__ float s_c[BT];
-  __shared__ float s_r[BT];
-  __shared__ uint2 s_rect[BT];
-  __shared__ uint32_t s_vi[BT];
-  // rows padded to 65 floats: the eight lanes that store a splat's reduced
-  // sums (values 0-7 of one column) land in eight different banks
-  __shared__ float s_g[NW][9][BT + 1];
+  __shared__ float4 s_a[NH * BT];
+  __shared__ float4 s_b[NH * BT];
+  __shared__ float s_c[NH * BT];
+  __shared__ float s_r[NH * BT];
+  __shared__ uint2 s_rect[NH * BT];
+  __shared__ uint32_t s_vi[NH * BT];
+  // rows padded to an odd number of floats: the eight lanes that store a
+  // splat's reduced sums (values 0-7 of one column) land in eight different
+  // banks
+  __shared__ float s_g[NW][9][GS];
   __shared__ int s_top[NW];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // wave-uniform
@@ -548,39 +581,49 @@ __global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(7, 8
   for (int k = 0; k < 9; ++k) g[k] = 0.0f;
   // LDS slot offset of the value this lane adds after a row reduction
   const int lr = lane & 15;
-  const int g_slot = (lr < 8 ? rev3(lr) : 8) * (BT + 1);
+  const int g_slot = (lr < 8 ? rev3(lr) : 8) * GS;
   uint32_t n_eval = 0, n_visit = 0, n_comp = 0;
-  for (int hi = top; hi >= 0; hi -= BT) {
-    const int lo = max(0, hi - (BT - 1));
+  for (int hi = top; hi >= 0; hi -= FB) {
+    const int lo = max(0, hi - (FB - 1));
     const int nb = hi - lo + 1;
     __syncthreads();  // previous batch fully read and flushed
-    if (lane < nb) {
-      const uint32_t p = (uint32_t)(lo + lane);
-      const uint32_t vi = p < nf ? in.pair_vals[start + p] : in.back_vals[bs + (p - nf)];
-      if (NW == 1 || w == 0) {
-        s_vi[lane] = vi;
-        s_a[lane] = in.S.xy_ab[vi];
-      }
-      if (NW == 1 || w == 1 % NW) s_b[lane] = in.S.c_op_rg[vi];
-      if (NW == 1 || w == 2 % NW) s_c[lane] = in.S.col_b[vi];
-      if (NW == 1 || w == 3 % NW) {
-        s_r[lane] = in.S.reach[vi];
-        if (S > 1) s_rect[lane] = in.S.rect[vi];
-      }
-    }
 #pragma unroll
-    for (int k = 0; k < 9; ++k) s_g[w][k][lane] = 0.0f;
+    for (int h = 0; h < NH; ++h) {
+      const int sl = h * BT + lane;
+      if (lane < LW && sl < nb) {
+        const uint32_t p = (uint32_t)(lo + sl);
+        const uint32_t vi = p < nf ? in.pair_vals[start + p] : in.back_vals[bs + (p - nf)];
+        if (NW == 1 || w == 0) {
+          s_vi[sl] = vi;
+          s_a[sl] = in.S.xy_ab[vi];
+        }
+        if (NW == 1 || w == 1 % NW) s_b[sl] = in.S.c_op_rg[vi];
+        if (NW == 1 || w == 2 % NW) s_c[sl] = in.S.col_b[vi];
+        if (NW == 1 || w == 3 % NW) {
+          s_r[sl] = in.S.reach[vi];
+          if (S > 1) s_rect[sl] = in.S.rect[vi];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 9; ++k) s_g[w][k][sl] = 0.0f;
+    }
     __syncthreads();
+    // the halves back to front
+#pragma unroll 1
+    for (int h = NH - 1; h >= 0; --h) {
+    const int sl = h * BT + lane;
     uint32_t my_m = 0;  // group mask of this lane's staged splat for this wave's tile
-    if (lane < nb && lo <= top_w && (S == 1 || in_rect(s_rect[lane], tx, ty))) {
-      const float4 A = s_a[lane];
-      const float4 B = s_b[lane];
-      my_m = wave_mask(A.x, A.y, A.z, A.w, B.x, s_r[lane], tx, ty);
+    if (lane < LW && sl < nb && lo + h * BT <= top_w && (S == 1 || in_rect(s_rect[sl], tx, ty))) {
+      const float4 A = s_a[sl];
+      const float4 B = s_b[sl];
+      my_m = wave_mask(A.x, A.y, A.z, A.w, B.x, s_r[sl], tx, ty);
     }
     // splats that reach some pixel group, visited back to front as set bits
     // of a scalar mask (their LDS reads are not prefetched, as in the forward)
-    auto splat = [&](const int j, const float4 A, const float4 B, const float cb) {
-      const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)my_m, j);
+    // (jl: the splat's lane in the half, j: its slot in the batch)
+    auto splat = [&](const int jl, const float4 A, const float4 B, const float cb) {
+      const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)my_m, jl);
+      const int j = h * BT + jl;
       const int li = lo + j;
       const float dx = fpx - A.x;
       bool contrib = false;
@@ -657,22 +700,26 @@ __global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(7, 8
     };
     uint64_t live = __ballot(my_m != 0);
     while (live) {
-      const int j = 63 - __builtin_clzll(live);
-      live &= ~(1ull << j);
-      splat(j, s_a[j], s_b[j], s_c[j]);
+      const int jl = 63 - __builtin_clzll(live);
+      live &= ~(1ull << jl);
+      splat(jl, s_a[h * BT + jl], s_b[h * BT + jl], s_c[h * BT + jl]);
+    }
     }
     __syncthreads();
     // flush: raw sums (dL/dcolour[3], dL/dopacity, S1..S5 with S the
     // moments sum(e dx), sum(e dy), sum(e dx^2), sum(e dx dy), sum(e dy^2) of
     // e = dL/dpower / opacity) over the waves; dL/dconic and dL/dmean follow
     // from the conic and the opacity. Output o of splat lane goes to thread
-    // (o % NW) * 64 + lane.
-    if (lane < nb) {
-      const uint32_t vi = s_vi[lane];
-      auto raw = [&](int k) {
-        float t = s_g[0][k][lane];
+    // (o % NW) * 64 + lane, for each half.
 #pragma unroll
-        for (int u = 1; u < NW; ++u) t += s_g[u][k][lane];
+    for (int h = 0; h < NH; ++h) {
+    const int sl = h * BT + lane;
+    if (lane < LW && sl < nb) {
+      const uint32_t vi = s_vi[sl];
+      auto raw = [&](int k) {
+        float t = s_g[0][k][sl];
+#pragma unroll
+        for (int u = 1; u < NW; ++u) t += s_g[u][k][sl];
         return t;
       };
       for (int o = w; o < 9; o += NW) {  // wave-uniform o
@@ -680,7 +727,7 @@ __global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(7, 8
         if (o < 4) {
           val = raw(o);
         } else {
-          const float op = s_b[lane].y;
+          const float op = s_b[sl].y;
           if (o == 4) {
             val = -0.5f * (op * raw(6));
           } else if (o == 5) {
@@ -688,14 +735,15 @@ __global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(7, 8
           } else if (o == 6) {
             val = -0.5f * (op * raw(8));
           } else {
-            const float4 A = s_a[lane];
-            const float cc = s_b[lane].x;
+            const float4 A = s_a[sl];
+            const float cc = s_b[sl].x;
             const float s1 = op * raw(4), s2 = op * raw(5);
             val = o == 7 ? A.z * s1 + A.w * s2 : A.w * s1 + cc * s2;
           }
         }
         if (!COUNT && val != 0.0f) atomicAdd(&accum[(size_t)o * in.acc_stride + vi], val);
       }
+    }
     }
   }
   if (COUNT) count_add(in.counts + 3, n_eval, n_visit, n_comp, lane);
@@ -801,6 +849,12 @@ void launch_raster_bwd(const RasterIn& in, const float* final_t, const int32_t* 
   if (in.total_bins <= 0) return;
   if (in.bin_shift == 0)
     hipLaunchKernelGGL((k_raster_bwd<1>), dim3(in.total_bins), dim3(BT), 0, st, in, final_t, last, dl_dpix, accum);
+  else if (in.batch == 32)
+    hipLaunchKernelGGL((k_raster_bwd<2, false, 32>), dim3(in.total_bins), dim3(BT * 4), 0, st, in, final_t, last,
+                       dl_dpix, accum);
+  else if (in.batch == 128)
+    hipLaunchKernelGGL((k_raster_bwd<2, false, 128>), dim3(in.total_bins), dim3(BT * 4), 0, st, in, final_t, last,
+                       dl_dpix, accum);
   else
     hipLaunchKernelGGL((k_raster_bwd<2>), dim3(in.total_bins), dim3(BT * 4), 0, st, in, final_t, last, dl_dpix,
                        accum);

@@ -313,6 +313,7 @@ struct rgs_ctx {
   // of the step's forward and the bins ordered by them
   DBuf tile_cost, lpt_bins;
   int sparse_lanes = 2;    // backward: LDS atomics up to this many lanes
+  int raster_batch = 64;   // binned training raster: splats staged per batch
   float* scratch_dev = nullptr;  // this device's copy of the no-alloc scratch array
   float split_div = 0.0f;         // densify: 0 the reference's split, > 0 every scale / split_div
   bool grad_capture = false;  // the step writes its summed gradients into gcap
@@ -535,6 +536,7 @@ RasterIn raster_in_n(rgs_ctx* c, int n_views, int total_bins, int shift, const f
   in.cams = P_<CamDev>(c->cams_d);
   in.n_views = n_views;
   in.sparse_lanes = c->sparse_lanes;
+  in.batch = c->raster_batch;
   in.acc_stride = 0;  // set by the backward's callers
   in.total_bins = total_bins;
   in.bin_shift = shift;
@@ -906,6 +908,20 @@ int rgs_get_raster_params(rgs_ctx* c, int* super_tile, int* sparse_lanes) {
   if (!c) return fail(RGS_E_ARG, "null ctx");
   if (super_tile) *super_tile = c->super_tile;
   if (sparse_lanes) *sparse_lanes = c->sparse_lanes;
+  return 0;
+}
+
+int rgs_set_raster_batch(rgs_ctx* c, int batch) {
+  if (!c) return fail(RGS_E_ARG, "null ctx");
+  if (batch != 32 && batch != 64 && batch != 128) return fail(RGS_E_ARG, "batch must be 32, 64 or 128");
+  if (batch != c->raster_batch) drop_graph(c);
+  c->raster_batch = batch;
+  return 0;
+}
+
+int rgs_get_raster_batch(rgs_ctx* c, int* batch) {
+  if (!c || !batch) return fail(RGS_E_ARG, "null argument");
+  *batch = c->raster_batch;
   return 0;
 }
 
@@ -2295,9 +2311,10 @@ int rgs_autotune_raster(rgs_ctx* c, const int* ids, int n_ids, const float bg[3]
     *ms = a + b;
     return 0;
   };
-  auto score = [&](int st_, int sl, float* best) -> int {
+  auto score = [&](int st_, int sl, int fb, float* best) -> int {
     c->super_tile = st_;
     c->sparse_lanes = sl;
+    c->raster_batch = fb;
     float t = 0.0f, m = 3.4e38f;
     for (int r = 0; r < 2; ++r) {
       CHK(time_once(&t));
@@ -2306,28 +2323,37 @@ int rgs_autotune_raster(rgs_ctx* c, const int* ids, int n_ids, const float bg[3]
     *best = m;
     return 0;
   };
-  int best_st = c->super_tile, best_sl = c->sparse_lanes;
+  int best_st = c->super_tile, best_sl = c->sparse_lanes, best_fb = c->raster_batch;
   float best_ms = 0.0f;
-  int rc = score(best_st, best_sl, &best_ms);
-  // coordinate search: schedule patch size, then the sparse/dense switch
+  int rc = score(best_st, best_sl, best_fb, &best_ms);
+  // coordinate search: schedule patch size, the sparse/dense switch, then
+  // the staging batch (the binned training raster only)
   const int st_cand[] = {0, 2, 4, 8, 16, 1024};
   const int sl_cand[] = {0, 2, 4, 8, 16, 32};
+  const int fb_cand[] = {32, 64, 128};
   for (int st_ : st_cand) {
     if (rc) break;
     float m = 0.0f;
-    rc = score(st_, best_sl, &m);
+    rc = score(st_, best_sl, best_fb, &m);
     if (!rc && m < best_ms) best_ms = m, best_st = st_;
   }
   for (int sl : sl_cand) {
     if (rc) break;
     float m = 0.0f;
-    rc = score(best_st, sl, &m);
+    rc = score(best_st, sl, best_fb, &m);
     if (!rc && m < best_ms) best_ms = m, best_sl = sl;
+  }
+  for (int fb : fb_cand) {
+    if (rc || c->bin_shift == 0) break;
+    float m = 0.0f;
+    rc = score(best_st, best_sl, fb, &m);
+    if (!rc && m < best_ms) best_ms = m, best_fb = fb;
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
   c->profiling = prof;
   c->super_tile = best_st;
   c->sparse_lanes = best_sl;
+  c->raster_batch = best_fb;
   if (rc) return rc;
   CHK(ensure_tile_order(c, vs));
   launch_fill_f32(P_<float>(c->accum), acc_n, 0.0f, c->st);

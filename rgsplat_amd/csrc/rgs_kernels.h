@@ -186,6 +186,7 @@ struct RasterIn {
   const CamDev* cams;
   int n_views;
   int sparse_lanes;  // backward: at most this many contributing lanes -> LDS atomics
+  int batch;         // binned training raster: splats staged per batch (32, 64, 128; 0 = 64)
   size_t acc_stride;  // backward accumulators: value k of entry vi at accum[k * acc_stride + vi]
   int total_bins;   // workgroups: one per bin (2^bin_shift x 2^bin_shift tiles)
   int bin_shift;

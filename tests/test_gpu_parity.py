@@ -704,11 +704,16 @@ def test_raster_params_and_autotune(ctx):
     for bad in [(-1, 2), (8, 65), (2000, 2)]:
         with pytest.raises(runtime.RgsError):
             c.set_raster_params(*bad)
+    assert c.raster_batch() == 64
+    for bad in [0, 16, 96, 256]:
+        with pytest.raises(runtime.RgsError):
+            c.set_raster_batch(bad)
     c.scene_upload(g)
     c.set_views(cams, targets)
     best = c.autotune_raster([0, 1])
     # a candidate, or the setting it started from (3, 5) when nothing beat it
     assert best["super_tile"] in (0, 2, 4, 8, 16, 1024, 3) and best["sparse_lanes"] in (0, 2, 4, 8, 16, 32, 5)
+    assert best["batch"] in (32, 64, 128) and c.raster_batch() == best["batch"]
     assert best["raster_ms"] > 0
     assert c.raster_params() == (best["super_tile"], best["sparse_lanes"])
     assert np.array_equal(c.scene_download(), g)
@@ -772,6 +777,38 @@ def test_raster_work_counts():
         with pytest.raises(runtime.RgsError):
             c.raster_work_counts()
     c.close()
+
+
+@pytest.mark.parametrize("fwd_tile_blocks,cut", [(1, None), (0, None), (1, 3.0)])
+def test_raster_batch_sizes(fwd_tile_blocks, cut):
+    """The binned training raster stages 32, 64 or 128 splats per batch
+    (128 as two halves of 64): the splats are composited in the same order,
+    so images and losses are bit-identical, and gradients differ only by the
+    order of float summation; with a fixed depth cut the resumed back lists
+    take the same path. Against the oracle as well."""
+    from rgsplat_amd import runtime
+    g, cams, targets = workload("c2_500k_8views", 30_000, [(160, 120), (240, 136), (320, 180)])
+    cfg = S.benchmark_train_config()
+    npx = sum(c.width * c.height for c in cams)
+    out = {}
+    for fb in (64, 32, 128):
+        c = runtime.Context(0)
+        c.set_train_binning(1, fwd_tile_blocks)
+        if cut is not None:
+            c.set_depth_cut(2, cut)
+        c.set_raster_batch(fb)
+        c.scene_upload(g)
+        c.set_views(cams, targets)
+        c.set_grad_capture(True)
+        loss = c.train_step([0, 1, 2], cfg, use_graph=True)
+        out[fb] = (loss, c.step_image(npx), c.step_grads(), c.cut_stats())
+        c.close()
+    for fb in (32, 128):
+        assert np.array_equal(out[fb][0], out[64][0]) and np.array_equal(out[fb][1], out[64][1]), fb
+        gs, gl = out[64][2].astype(np.float64), out[fb][2].astype(np.float64)
+        assert np.linalg.norm(gl - gs) <= 1e-5 * np.linalg.norm(gs), fb
+    ref_loss, ref_g = _oracle_step_grads(g, cams, targets, (0, 0, 0), 0.2)
+    assert_grads_close(out[128][2], ref_g.astype(np.float32), 1e-3)
 
 
 @pytest.mark.parametrize("fwd_tile_blocks", [1, 0])
