@@ -413,6 +413,9 @@ int ensure_render(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, size_t k_cap
   size_t nv = vs.cams.size();
   size_t vc = nv * cap;
   size_t dn = nv * (size_t)n;
+  // sort values and the projection's thread index are 32-bit (v * cap + i)
+  if (vc >= (1ull << 32))
+    return fail(RGS_E_CAPACITY, "views x Gaussian capacity must stay below 2^32");
   CHK(ensure(c, c->cams_d, nv * sizeof(CamDev)));
   CHK(ensure(c, c->xy_ab, vc * sizeof(float4)));
   CHK(ensure(c, c->c_op_rg, vc * sizeof(float4)));
