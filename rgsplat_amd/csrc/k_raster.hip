@@ -145,9 +145,8 @@ __device__ __forceinline__ float clamp_alpha(float alpha_raw, bool* clamped = nu
 // each of the S*S waves loading one of the four splat fields; the
 // workgroup stops when all pixels of the bin have saturated. Modes 0 and 1
 // evaluate alpha with the exact expf (images bit-identical to the
-// reference's); the training step's modes 3 (as 0) and 2 use alpha_fast and
-// fused multiply-adds in the composite (images equal to float rounding,
-// alpha decisions exact). MODE 2 resumes a
+// reference's); the training step's modes 3 (as 0) and 2 use alpha_fast
+// (images equal to float rounding, alpha decisions exact). MODE 2 resumes a
 // bin flagged by MODE 3 on its back list: the state (C, T, last) comes from
 // the output buffers and list indices continue after the front list.
 constexpr int FT = 64;
@@ -319,20 +318,15 @@ __global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(7, 8)))
         }
         if (alpha < ALPHA_THRESHOLD) continue;
         if (COUNT) n_comp += 1;
+        // the reference's operations (no fused multiply-adds): the training
+        // forward's images then differ from the reference's only where the
+        // hardware exp moved alpha, which keeps the L1 term's sign where a
+        // pixel sits on its target
         float weight = alpha * T[q];
-        if (FAST) {
-          // the training forward (images equal to float rounding): fused
-          // multiply-adds for the colour sums and T (1 - alpha)
-          C0[q] = __builtin_fmaf(B.z, weight, C0[q]);
-          C1[q] = __builtin_fmaf(B.w, weight, C1[q]);
-          C2[q] = __builtin_fmaf(cb, weight, C2[q]);
-          T[q] = __builtin_fmaf(-alpha, T[q], T[q]);
-        } else {
-          C0[q] += B.z * weight;
-          C1[q] += B.w * weight;
-          C2[q] += cb * weight;
-          T[q] *= 1.0f - alpha;
-        }
+        C0[q] += B.z * weight;
+        C1[q] += B.w * weight;
+        C2[q] += cb * weight;
+        T[q] *= 1.0f - alpha;
         last[q] = li;
       }
       if (__ballot(pixels_left()) == 0) {
