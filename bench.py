@@ -499,7 +499,7 @@ def main():
                       "execution": "device-resident step, views batched per launch, "
                                    + ("hipGraph replay" if use_graph else "eager"),
                       "raster_params": dict(zip(("super_tile", "sparse_lanes"), ctx.raster_params()),
-                                            batch=ctx.raster_batch()),
+                                            batch=ctx.raster_batch(), queue=ctx.raster_queue()),
                       "raster_autotuned": tuned is not None}}
 
     if profile:

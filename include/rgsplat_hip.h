@@ -252,6 +252,16 @@ int rgs_set_raster_params(rgs_ctx* ctx, int super_tile, int sparse_lanes);
  * gradients only through the order of float summation). */
 int rgs_set_raster_batch(rgs_ctx* ctx, int batch);
 int rgs_get_raster_batch(rgs_ctx* ctx, int* batch);
+/* Persistent tile queue of the training step's raster (no reference
+ * counterpart; images do not depend on it, gradients only through the order
+ * of float summation). Bit 0: the forward, bit 1: the backward run as
+ * persistent kernels, a grid of as many workgroups as the device holds at
+ * once, each taking bins from per-XCD queues (its own XCD's share of the
+ * schedule first, then the others') until none is left; 0 (the default):
+ * one workgroup per bin. rgs_autotune_raster searches it with the binned
+ * training raster. */
+int rgs_set_raster_queue(rgs_ctx* ctx, int mode);
+int rgs_get_raster_queue(rgs_ctx* ctx, int* mode);
 /* Binning of rgs_train_step: pairs per bin of 2^shift x 2^shift tiles
  * (0: the reference's 16x16 tiles; 1, the default: 32x32). The backward
  * rasterises a bin with one workgroup of four waves that share the staged

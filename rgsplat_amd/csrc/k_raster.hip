@@ -18,6 +18,11 @@
 // four pixels' gradients for the splat, the wave then reduces the nine sums
 // across lanes into LDS, and each bin adds the sum over its waves to the
 // per-splat accumulators with one FP32 atomic per value.
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "rgs_kernels.h"
 
 namespace rgs {
@@ -175,9 +180,10 @@ __device__ __forceinline__ void count_add(unsigned long long* counts, uint32_t n
 // (splat, pixel group), 2: pairs that composite).
 // FB: splats staged per batch (32, 64 or 128); a batch of 128 is staged as
 // two halves of 64 (one per lane each) and walked half by half.
-template <int MODE, int S, int NW, bool COUNT = false, int FB = FT>
-__global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_raster_fwd(
-    RasterIn in, float* __restrict__ out_rgb, float* __restrict__ out_t, int32_t* __restrict__ out_last) {
+// lb: the schedule slot (NW == 1: bin slot * S * S + the tile in the bin).
+template <int MODE, int S, int NW, bool COUNT, int FB>
+__device__ __forceinline__ void fwd_body(const RasterIn& in, const int lb, float* __restrict__ out_rgb,
+                                         float* __restrict__ out_t, int32_t* __restrict__ out_last) {
   static_assert(NW == 1 || NW == S * S, "waves per workgroup");
   static_assert(FB == 32 || FB == 64 || FB == 128, "batch");
   constexpr bool FAST = MODE >= 2;
@@ -193,7 +199,6 @@ __global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(7, 8)))
   __shared__ int32_t s_fin[NW];
   const int lane = threadIdx.x & 63;
   // this wave's tile in the bin (wave-uniform), and its LDS slot
-  const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const int w = NW == 1 ? lb % (S * S) : __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int ws = NW == 1 ? 0 : w;
   const int bin = in.tile_order[NW == 1 ? lb / (S * S) : lb];
@@ -399,16 +404,118 @@ __global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(7, 8)))
   }
 }
 
-// The binned training forward (MODE 2, 3) in the staging batch in.batch.
+// Persistent raster launches (in.queue set): a grid of the workgroups the
+// chip holds at once, each taking schedule slots from in.queue until none is
+// left. The n slots are split into eight shares, one per XCD (the hardware
+// deals workgroups to the XCDs in turn, so workgroup b runs on XCD b % 8):
+// a contiguous run of the schedule (strided = false; the run's bins are
+// neighbours that share splats in the XCD's L2) or every eighth slot
+// (strided = true; a descending share of a longest-first table). A
+// workgroup drains its own XCD's share first, then takes from the other
+// shares in turn, so an XCD that runs ahead helps the others with their
+// last slots. Counter x lives at queue[32 x] (its own 128-byte line) and
+// starts at zero. Every slot is handed out exactly once, whatever the grid
+// size and however many of its workgroups are resident.
+__device__ __forceinline__ int queue_slot(int x, int n, bool strided, uint32_t k) {
+  if (strided) {
+    const uint32_t s = (uint32_t)x + 8u * k;
+    return s < (uint32_t)n ? (int)s : -1;
+  }
+  const int per = n >> 3, rem = n & 7;
+  const uint32_t len = (uint32_t)(per + (x < rem ? 1 : 0));
+  return k < len ? x * per + min(x, rem) + (int)k : -1;
+}
+// The next slot for a workgroup of XCD x0 (from: the share it drains, x0 at
+// first); -1 when every share is drained.
+__device__ __forceinline__ int queue_pop(uint32_t* queue, int n, bool strided, int x0, int& from) {
+  for (;;) {
+    const int s = queue_slot(from, n, strided, atomicAdd(&queue[32 * from], 1u));
+    if (s >= 0) return s;
+    from = (from + 1) & 7;
+    if (from == x0) return -1;
+  }
+}
+// The slot of the workgroup's next turn, the same in every thread (NW waves).
+template <int NW>
+__device__ __forceinline__ int queue_next(uint32_t* queue, int n, bool strided, int x0, int& from) {
+  if (NW == 1) {
+    int v = 0;
+    if ((threadIdx.x & 63) == 0) v = queue_pop(queue, n, strided, x0, from);
+    return __builtin_amdgcn_readlane(v, 0);
+  }
+  __shared__ int s_slot;
+  if (threadIdx.x == 0) s_slot = queue_pop(queue, n, strided, x0, from);
+  __syncthreads();
+  const int v = s_slot;
+  __syncthreads();  // every thread has read it before thread 0 writes the next
+  return v;
+}
+
+// Q: persistent (slots from in.queue) instead of one workgroup per slot.
+template <int MODE, int S, int NW, bool COUNT = false, int FB = FT, bool Q = false>
+__global__ __launch_bounds__(FT * NW) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_raster_fwd(
+    RasterIn in, float* __restrict__ out_rgb, float* __restrict__ out_t, int32_t* __restrict__ out_last) {
+  if (!Q) {
+    fwd_body<MODE, S, NW, COUNT, FB>(in, xcd_remap(blockIdx.x, gridDim.x), out_rgb, out_t, out_last);
+    return;
+  }
+  const int n = in.total_bins * (NW == 1 ? S * S : 1);
+  const int x0 = (int)(blockIdx.x & 7);
+  int from = x0;
+  for (;;) {
+    const int slot = queue_next<NW>(in.queue, n, false, x0, from);
+    if (slot < 0) break;
+    fwd_body<MODE, S, NW, COUNT, FB>(in, slot, out_rgb, out_t, out_last);
+  }
+}
+
+// Workgroups of kernel fn (threads per workgroup) that the device holds at
+// once, rounded down to a multiple of 8 (cached per kernel and device): the
+// grid of a persistent launch.
+static int resident_grid(const void* fn, int threads) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, int> cache;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lock(mu);
+  const auto it = cache.find({fn, dev});
+  if (it != cache.end()) return it->second;
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 8;
+  const int g = std::max(8, per_cu * cus / 8 * 8);
+  cache[{fn, dev}] = g;
+  return g;
+}
+// Grid of a persistent launch over n slots.
+static dim3 queue_grid(const void* fn, int threads, int n) {
+  return dim3((unsigned)std::min(resident_grid(fn, threads), (n + 7) / 8 * 8));
+}
+
+// The binned training forward (MODE 2, 3) in staging batch FB; the training
+// forward (MODE 3) runs persistent when in.queue is set.
+template <int MODE, int NW, int FB>
+static void fwd_binned_fb(const RasterIn& in, dim3 grid, float* rgb, float* final_t, int32_t* last,
+                          hipStream_t st) {
+  if constexpr (MODE == 3) {
+    if (in.queue) {
+      const auto k = k_raster_fwd<MODE, 2, NW, false, FB, true>;
+      hipLaunchKernelGGL(k, queue_grid((const void*)k, FT * NW, (int)grid.x), dim3(FT * NW), 0, st, in, rgb,
+                         final_t, last);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_raster_fwd<MODE, 2, NW, false, FB>), grid, dim3(FT * NW), 0, st, in, rgb, final_t, last);
+}
 template <int MODE, int NW>
 static void fwd_binned(const RasterIn& in, dim3 grid, float* rgb, float* final_t, int32_t* last, hipStream_t st) {
   const int b = MODE >= 2 ? in.batch : FT;
   if (b == 32)
-    hipLaunchKernelGGL((k_raster_fwd<MODE, 2, NW, false, 32>), grid, dim3(FT * NW), 0, st, in, rgb, final_t, last);
+    fwd_binned_fb<MODE, NW, 32>(in, grid, rgb, final_t, last, st);
   else if (b == 128)
-    hipLaunchKernelGGL((k_raster_fwd<MODE, 2, NW, false, 128>), grid, dim3(FT * NW), 0, st, in, rgb, final_t, last);
+    fwd_binned_fb<MODE, NW, 128>(in, grid, rgb, final_t, last, st);
   else
-    hipLaunchKernelGGL((k_raster_fwd<MODE, 2, NW>), grid, dim3(FT * NW), 0, st, in, rgb, final_t, last);
+    fwd_binned_fb<MODE, NW, FT>(in, grid, rgb, final_t, last, st);
 }
 
 template <int MODE>
@@ -489,11 +596,10 @@ constexpr int BT = 64;
 // wave visits, 5: contributing pairs) and writes no accumulator.
 // FB: splats staged per batch (32, 64 or 128; 128 as two halves of 64,
 // walked back to front).
-template <int S, bool COUNT = false, int FB = BT>
-__global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_raster_bwd(RasterIn in, const float* __restrict__ final_t,
-                                                           const int32_t* __restrict__ last_contrib,
-                                                           const float* __restrict__ dl_dpix,
-                                                           float* __restrict__ accum) {
+template <int S, bool COUNT, int FB>
+__device__ __forceinline__ void bwd_body(const RasterIn& in, const int bin, const float* __restrict__ final_t,
+                                         const int32_t* __restrict__ last_contrib,
+                                         const float* __restrict__ dl_dpix, float* __restrict__ accum) {
   static_assert(FB == 32 || FB == 64 || FB == 128, "batch");
   constexpr int NW = S * S;
   constexpr int NH = FB > BT ? FB / BT : 1;  // halves of 64 per batch
@@ -514,7 +620,6 @@ __global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(7, 8
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // wave-uniform
   if (threadIdx.x < 32) s_tab[threadIdx.x] = kExp2Tab[threadIdx.x];
-  const int bin = in.lpt_bins ? in.lpt_bins[blockIdx.x] : in.tile_order[xcd_remap(blockIdx.x, gridDim.x)];
   const BinTile bt = bin_tile<S>(in, bin, w);
   const CamDev& c = in.cams[bt.v];
   const int tx = bt.tx, ty = bt.ty;
@@ -743,6 +848,29 @@ __global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(7, 8
   if (COUNT) count_add(in.counts + 3, n_eval, n_visit, n_comp, lane);
 }
 
+// Q: persistent (slots from in.queue; every eighth slot per XCD under the
+// longest-first table, a contiguous run of the schedule otherwise) instead
+// of one workgroup per slot.
+template <int S, bool COUNT = false, int FB = BT, bool Q = false>
+__global__ __launch_bounds__(BT * S * S) __attribute__((amdgpu_waves_per_eu(7, 8))) void k_raster_bwd(
+    RasterIn in, const float* __restrict__ final_t, const int32_t* __restrict__ last_contrib,
+    const float* __restrict__ dl_dpix, float* __restrict__ accum) {
+  if (!Q) {
+    const int bin = in.lpt_bins ? in.lpt_bins[blockIdx.x] : in.tile_order[xcd_remap(blockIdx.x, gridDim.x)];
+    bwd_body<S, COUNT, FB>(in, bin, final_t, last_contrib, dl_dpix, accum);
+    return;
+  }
+  const bool strided = in.lpt_bins != nullptr;
+  const int x0 = (int)(blockIdx.x & 7);
+  int from = x0;
+  for (;;) {
+    const int slot = queue_next<S * S>(in.queue, in.total_bins, strided, x0, from);
+    if (slot < 0) break;
+    const int bin = strided ? in.lpt_bins[slot] : in.tile_order[slot];
+    bwd_body<S, COUNT, FB>(in, bin, final_t, last_contrib, dl_dpix, accum);
+  }
+}
+
 // ---- longest-first schedule of the backward
 // Orders the bins by cost (the largest tile cost among their tiles: the
 // bin's waves walk that far back), descending, by a counting sort over
@@ -838,20 +966,32 @@ void launch_raster_counts(const RasterIn& in, float* rgb, float* final_t, int32_
   }
 }
 
+// The binned training backward in staging batch FB, persistent when
+// in.queue is set.
+template <int FB>
+static void bwd_binned(const RasterIn& in, const float* final_t, const int32_t* last, const float* dl_dpix,
+                       float* accum, hipStream_t st) {
+  if (in.queue) {
+    const auto k = k_raster_bwd<2, false, FB, true>;
+    hipLaunchKernelGGL(k, queue_grid((const void*)k, BT * 4, in.total_bins), dim3(BT * 4), 0, st, in, final_t,
+                       last, dl_dpix, accum);
+  } else {
+    hipLaunchKernelGGL((k_raster_bwd<2, false, FB>), dim3(in.total_bins), dim3(BT * 4), 0, st, in, final_t, last,
+                       dl_dpix, accum);
+  }
+}
+
 void launch_raster_bwd(const RasterIn& in, const float* final_t, const int32_t* last, const float* dl_dpix,
                        float* accum, hipStream_t st) {
   if (in.total_bins <= 0) return;
   if (in.bin_shift == 0)
     hipLaunchKernelGGL((k_raster_bwd<1>), dim3(in.total_bins), dim3(BT), 0, st, in, final_t, last, dl_dpix, accum);
   else if (in.batch == 32)
-    hipLaunchKernelGGL((k_raster_bwd<2, false, 32>), dim3(in.total_bins), dim3(BT * 4), 0, st, in, final_t, last,
-                       dl_dpix, accum);
+    bwd_binned<32>(in, final_t, last, dl_dpix, accum, st);
   else if (in.batch == 128)
-    hipLaunchKernelGGL((k_raster_bwd<2, false, 128>), dim3(in.total_bins), dim3(BT * 4), 0, st, in, final_t, last,
-                       dl_dpix, accum);
+    bwd_binned<128>(in, final_t, last, dl_dpix, accum, st);
   else
-    hipLaunchKernelGGL((k_raster_bwd<2>), dim3(in.total_bins), dim3(BT * 4), 0, st, in, final_t, last, dl_dpix,
-                       accum);
+    bwd_binned<BT>(in, final_t, last, dl_dpix, accum, st);
 }
 
 // Algorithmic work of the last render: pairs the forward evaluates per pixel

@@ -314,6 +314,8 @@ struct rgs_ctx {
   DBuf tile_cost, lpt_bins;
   int sparse_lanes = 2;    // backward: LDS atomics up to this many lanes
   int raster_batch = 64;   // binned training raster: splats staged per batch
+  int raster_queue = 0;    // persistent training raster: bit 0 the forward, bit 1 the backward
+  DBuf rqueue;             // their slot counters: forward at [0, 256), backward at [256, 512)
   float* scratch_dev = nullptr;  // this device's copy of the no-alloc scratch array
   float split_div = 0.0f;         // densify: 0 the reference's split, > 0 every scale / split_div
   bool grad_capture = false;  // the step writes its summed gradients into gcap
@@ -407,6 +409,12 @@ void lpt_backward(rgs_ctx* c, const ViewSet& vs, RasterIn& in) {
   in.lpt_bins = P_<int32_t>(c->lpt_bins);
 }
 
+// The persistent training raster's slot counters (zeroed by the step before
+// its forward): forward and backward.
+constexpr size_t RQ_WORDS = 512;
+uint32_t* fwd_queue(rgs_ctx* c) { return (c->raster_queue & 1) ? P_<uint32_t>(c->rqueue) : nullptr; }
+uint32_t* bwd_queue(rgs_ctx* c) { return (c->raster_queue & 2) ? P_<uint32_t>(c->rqueue) + 256 : nullptr; }
+
 // Sizes every buffer the render sequence touches for n Gaussians (row
 // capacity cap) in the view set vs, pairs capacity k_cap.
 int ensure_render(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, size_t k_cap) {
@@ -417,6 +425,7 @@ int ensure_render(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, size_t k_cap
   if (vc >= (1ull << 32))
     return fail(RGS_E_CAPACITY, "views x Gaussian capacity must stay below 2^32");
   CHK(ensure(c, c->cams_d, nv * sizeof(CamDev)));
+  CHK(ensure(c, c->rqueue, RQ_WORDS * 4));
   CHK(ensure(c, c->xy_ab, vc * sizeof(float4)));
   CHK(ensure(c, c->c_op_rg, vc * sizeof(float4)));
   CHK(ensure(c, c->col_b, vc * sizeof(float)));
@@ -620,6 +629,8 @@ void enqueue_bin_and_raster_cut(rgs_ctx* c, size_t cap, int n, const ViewSet& vs
   in.n_flagged = sc + SC_NFLAG;
   in.zneed = &cs->zneed;
   lpt_forward(c, in);
+  in.queue = fwd_queue(c);
+  if (c->raster_queue) launch_fill_u32(P_<uint32_t>(c->rqueue), RQ_WORDS, 0u, st);
   launch_fill_u32(P_<uint32_t>(c->flag), ((size_t)vs.total_bins + 3) / 4, 0u, st);
   launch_fill_u32(P_<uint32_t>(c->tile_flag), ((size_t)vs.total_tiles + 3) / 4, 0u, st);
   stage_begin(c, S_FWD);
@@ -865,9 +876,9 @@ void rgs_destroy(rgs_ctx* c) {
                  &c->pv_rect, &c->pv_cnt, &c->pv_off, &c->pv_bs, &c->pv_misc, &c->pv_pk[0], &c->pv_pk[1],
                  &c->pv_pv[0], &c->pv_pv[1], &c->pv_hist, &c->pv_rowtot, &c->pv_rs, &c->pv_re, &c->pv_rgb,
                  &c->rs_taps, &c->rs_tmp, &c->rs_src, &c->rs_dst, &c->pv_blk,
-                 &c->adam_d, &c->api_acc9, &c->tile_order[0], &c->tile_order[1], &c->tile_cost, &c->lpt_bins, &c->cover, &c->depth,
+                 &c->adam_d, &c->api_acc9, &c->tile_order[0], &c->tile_order[1], &c->tile_cost, &c->lpt_bins, &c->rqueue, &c->cover, &c->depth,
                  &c->flag, &c->tile_flag, &c->rbstart, &c->rbend, &c->pv2, &c->cntb, &c->cutst, &c->part,
-                 &c->kB, &c->vB, &c->kC[0], &c->kC[1], &c->vC[0], &c->vC[1]};
+                 &c->kB, &c->vB, &c->kC[0], &c->kC[1], &c->vC[0], &c->vC[1], &c->gcap, &c->work_counts};
   for (DBuf* b : all)
     if (b->p) (void)hipFree(b->p);
   if (c->h_scalars) (void)hipHostFree(c->h_scalars);
@@ -925,6 +936,20 @@ int rgs_set_raster_batch(rgs_ctx* c, int batch) {
 int rgs_get_raster_batch(rgs_ctx* c, int* batch) {
   if (!c || !batch) return fail(RGS_E_ARG, "null argument");
   *batch = c->raster_batch;
+  return 0;
+}
+
+int rgs_set_raster_queue(rgs_ctx* c, int mode) {
+  if (!c) return fail(RGS_E_ARG, "null ctx");
+  if (mode < 0 || mode > 3) return fail(RGS_E_ARG, "queue mode must be in [0, 3]");
+  if (mode != c->raster_queue) drop_graph(c);
+  c->raster_queue = mode;
+  return 0;
+}
+
+int rgs_get_raster_queue(rgs_ctx* c, int* mode) {
+  if (!c || !mode) return fail(RGS_E_ARG, "null argument");
+  *mode = c->raster_queue;
   return 0;
 }
 
@@ -2058,6 +2083,7 @@ static void enqueue_step_rest(rgs_ctx* c, const ViewSet& vs, const float bg[3], 
   stage_begin(c, S_BWD);
   RasterIn in = raster_in(c, vs, bg);
   lpt_backward(c, vs, in);
+  in.queue = bwd_queue(c);
   in.acc_stride = vs.cams.size() * cap;
   if (with_back) {
     in.bstart = P_<uint32_t>(c->rbstart);
@@ -2300,11 +2326,14 @@ int rgs_autotune_raster(rgs_ctx* c, const int* ids, int n_ids, const float bg[3]
     RasterIn in = raster_in(c, vs, bg);
     in.acc_stride = vs.cams.size() * cap;
     lpt_forward(c, in);
+    launch_fill_u32(P_<uint32_t>(c->rqueue), RQ_WORDS, 0u, c->st);
+    in.queue = fwd_queue(c);
     HIPCHK(hipEventRecord(ev[0], c->st));
     launch_raster_fwd(in, 3, P_<float>(c->rgb), P_<float>(c->final_t), P_<int32_t>(c->last), c->st);
     HIPCHK(hipEventRecord(ev[1], c->st));
     HIPCHK(hipEventRecord(ev[2], c->st));
     lpt_backward(c, vs, in);
+    in.queue = bwd_queue(c);
     launch_raster_bwd(in, P_<float>(c->final_t), P_<int32_t>(c->last), P_<float>(c->dl), P_<float>(c->accum), c->st);
     HIPCHK(hipEventRecord(ev[3], c->st));
     HIPCHK(hipStreamSynchronize(c->st));
@@ -2314,10 +2343,11 @@ int rgs_autotune_raster(rgs_ctx* c, const int* ids, int n_ids, const float bg[3]
     *ms = a + b;
     return 0;
   };
-  auto score = [&](int st_, int sl, int fb, float* best) -> int {
+  auto score = [&](int st_, int sl, int fb, int q, float* best) -> int {
     c->super_tile = st_;
     c->sparse_lanes = sl;
     c->raster_batch = fb;
+    c->raster_queue = q;
     float t = 0.0f, m = 3.4e38f;
     for (int r = 0; r < 2; ++r) {
       CHK(time_once(&t));
@@ -2326,37 +2356,54 @@ int rgs_autotune_raster(rgs_ctx* c, const int* ids, int n_ids, const float bg[3]
     *best = m;
     return 0;
   };
-  int best_st = c->super_tile, best_sl = c->sparse_lanes, best_fb = c->raster_batch;
+  int best_st = c->super_tile, best_sl = c->sparse_lanes, best_fb = c->raster_batch, best_q = c->raster_queue;
   float best_ms = 0.0f;
-  int rc = score(best_st, best_sl, best_fb, &best_ms);
+  int rc = score(best_st, best_sl, best_fb, best_q, &best_ms);
   // coordinate search: schedule patch size, the sparse/dense switch, then
-  // the staging batch (the binned training raster only)
+  // (the binned training raster only) the staging batch and the persistent
+  // queue, then the schedule once more under the chosen queue (a queue
+  // changes which bins run together)
   const int st_cand[] = {0, 2, 4, 8, 16, 1024};
   const int sl_cand[] = {0, 2, 4, 8, 16, 32};
   const int fb_cand[] = {32, 64, 128};
+  const int q_cand[] = {0, 1, 2, 3};
   for (int st_ : st_cand) {
     if (rc) break;
     float m = 0.0f;
-    rc = score(st_, best_sl, best_fb, &m);
+    rc = score(st_, best_sl, best_fb, best_q, &m);
     if (!rc && m < best_ms) best_ms = m, best_st = st_;
   }
   for (int sl : sl_cand) {
     if (rc) break;
     float m = 0.0f;
-    rc = score(best_st, sl, best_fb, &m);
+    rc = score(best_st, sl, best_fb, best_q, &m);
     if (!rc && m < best_ms) best_ms = m, best_sl = sl;
   }
   for (int fb : fb_cand) {
     if (rc || c->bin_shift == 0) break;
     float m = 0.0f;
-    rc = score(best_st, best_sl, fb, &m);
+    rc = score(best_st, best_sl, fb, best_q, &m);
     if (!rc && m < best_ms) best_ms = m, best_fb = fb;
+  }
+  const int q0 = best_q;
+  for (int q : q_cand) {
+    if (rc || c->bin_shift == 0) break;
+    float m = 0.0f;
+    rc = score(best_st, best_sl, best_fb, q, &m);
+    if (!rc && m < best_ms) best_ms = m, best_q = q;
+  }
+  for (int st_ : st_cand) {
+    if (rc || best_q == q0) break;
+    float m = 0.0f;
+    rc = score(st_, best_sl, best_fb, best_q, &m);
+    if (!rc && m < best_ms) best_ms = m, best_st = st_;
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
   c->profiling = prof;
   c->super_tile = best_st;
   c->sparse_lanes = best_sl;
   c->raster_batch = best_fb;
+  c->raster_queue = best_q;
   if (rc) return rc;
   CHK(ensure_tile_order(c, vs));
   launch_fill_f32(P_<float>(c->accum), acc_n, 0.0f, c->st);

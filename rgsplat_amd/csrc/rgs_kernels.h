@@ -208,6 +208,11 @@ struct RasterIn {
   // composited); backward block b takes bin lpt_bins[b]
   uint32_t* tile_cost;
   const int32_t* lpt_bins;
+  // persistent training raster (null: one workgroup per schedule slot): 8
+  // per-XCD slot counters at queue[0], queue[32], ..., queue[224], zero at
+  // launch; the training forward (mode 3) and the binned backward take
+  // their bins from them
+  uint32_t* queue;
 };
 // mode 0: training forward (rgb = C + T bg; final T; last contributor); with
 //   cut_on set, a tile whose (front) list ends before all its pixels

@@ -105,6 +105,8 @@ SIGNATURES = {
     "rgs_get_raster_params": [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
     "rgs_set_raster_batch": [_vp, _i],
     "rgs_get_raster_batch": [_vp, ctypes.POINTER(ctypes.c_int)],
+    "rgs_set_raster_queue": [_vp, _i],
+    "rgs_get_raster_queue": [_vp, ctypes.POINTER(ctypes.c_int)],
     "rgs_autotune_raster": [_vp, _vp, _i, _vp, ctypes.c_float, ctypes.POINTER(ctypes.c_int),
                             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)],
     "rgs_eval_expf": [_vp, _vp, _vp, _sz],
@@ -545,6 +547,16 @@ class Context:
         _check(lib().rgs_get_raster_batch(self.h, ctypes.byref(b)))
         return b.value
 
+    def set_raster_queue(self, mode: int):
+        """Persistent tile queue of the training raster: bit 0 the forward,
+        bit 1 the backward (0: one workgroup per bin)."""
+        _check(lib().rgs_set_raster_queue(self.h, int(mode)))
+
+    def raster_queue(self) -> int:
+        q = ctypes.c_int()
+        _check(lib().rgs_get_raster_queue(self.h, ctypes.byref(q)))
+        return q.value
+
     def autotune_raster(self, view_ids, bg=(0.0, 0.0, 0.0), lambda_dssim=0.2) -> dict:
         """Times the raster under candidate work decompositions on the
         registered views and keeps the fastest (rgs_autotune_raster)."""
@@ -553,7 +565,8 @@ class Context:
         a, b, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
         _check(lib().rgs_autotune_raster(self.h, _p(ids), len(ids), _p(bgv), float(lambda_dssim), ctypes.byref(a),
                                          ctypes.byref(b), ctypes.byref(ms)))
-        return {"super_tile": a.value, "sparse_lanes": b.value, "batch": self.raster_batch(), "raster_ms": ms.value}
+        return {"super_tile": a.value, "sparse_lanes": b.value, "batch": self.raster_batch(),
+                "queue": self.raster_queue(), "raster_ms": ms.value}
 
     # ---------------------------------------------------------- stage timing
     STAGES = ["project", "depth_sort", "pair_scan", "pair_gen", "tile_sort", "raster_fwd", "loss_ssim",

@@ -714,6 +714,7 @@ def test_raster_params_and_autotune(ctx):
     # a candidate, or the setting it started from (3, 5) when nothing beat it
     assert best["super_tile"] in (0, 2, 4, 8, 16, 1024, 3) and best["sparse_lanes"] in (0, 2, 4, 8, 16, 32, 5)
     assert best["batch"] in (32, 64, 128) and c.raster_batch() == best["batch"]
+    assert best["queue"] in (0, 1, 2, 3) and c.raster_queue() == best["queue"]
     assert best["raster_ms"] > 0
     assert c.raster_params() == (best["super_tile"], best["sparse_lanes"])
     assert np.array_equal(c.scene_download(), g)
@@ -809,6 +810,55 @@ def test_raster_batch_sizes(fwd_tile_blocks, cut):
         assert np.linalg.norm(gl - gs) <= 1e-5 * np.linalg.norm(gs), fb
     ref_loss, ref_g = _oracle_step_grads(g, cams, targets, (0, 0, 0), 0.2)
     assert_grads_close(out[128][2], ref_g.astype(np.float32), 1e-3)
+
+
+@pytest.mark.parametrize("fwd_tile_blocks,super_tile,cut", [(1, 8, None), (0, 4, None), (1, 0, None), (0, 0, 3.0)])
+def test_raster_queue_modes(fwd_tile_blocks, super_tile, cut):
+    """The persistent training raster (bins from per-XCD queues; bit 0 the
+    forward, bit 1 the backward) against one workgroup per bin over two
+    graph-replayed steps of three uneven views, under the static schedule
+    and the longest-first one (strided queue shares), with and without
+    resumed back lists: every bin is rasterised exactly once, so images and
+    losses are bit-identical and gradients differ only by the atomics'
+    summation order; the gradients against the oracle as well."""
+    from rgsplat_amd import runtime
+    g, cams, targets = workload("c2_500k_8views", 30_000, [(160, 120), (240, 136), (320, 180)])
+    cfg = S.benchmark_train_config()
+    npx = sum(c.width * c.height for c in cams)
+    out = {}
+    for q in (0, 1, 2, 3):
+        c = runtime.Context(0)
+        c.set_train_binning(1, fwd_tile_blocks)
+        c.set_raster_params(super_tile, 2)
+        if cut is not None:
+            c.set_depth_cut(2, cut)
+        c.set_raster_queue(q)
+        assert c.raster_queue() == q
+        c.scene_upload(g)
+        c.set_views(cams, targets)
+        c.set_grad_capture(True)
+        rec = []
+        for _ in range(2):
+            loss = c.train_step([0, 1, 2], cfg, use_graph=True)
+            rec.append((loss, c.step_image(npx), c.step_grads()))
+        out[q] = rec
+        c.close()
+    for q in (1, 2, 3):
+        l0, i0, g0 = out[0][0]
+        l1, i1, g1 = out[q][0]
+        assert np.array_equal(l1, l0) and np.array_equal(i1, i0), q
+        gs, gq = g0.astype(np.float64), g1.astype(np.float64)
+        assert np.linalg.norm(gq - gs) <= 1e-5 * np.linalg.norm(gs), q
+        # the second step starts from scenes that differ by the first
+        # step's summation order only
+        np.testing.assert_allclose(out[q][1][0], out[0][1][0], rtol=1e-5)
+    c = runtime.Context(0)
+    for bad in (-1, 4):
+        with pytest.raises(runtime.RgsError):
+            c.set_raster_queue(bad)
+    c.close()
+    ref_loss, ref_g = _oracle_step_grads(g, cams, targets, (0, 0, 0), 0.2)
+    assert_grads_close(out[3][0][2], ref_g.astype(np.float32), 1e-3)
 
 
 @pytest.mark.parametrize("fwd_tile_blocks", [1, 0])
