@@ -438,49 +438,75 @@ void launch_adam_aos(float* g, const float* grads, size_t n, float* m1, float* m
 // them (positions are updated by the geometry wave after the last barrier),
 // as in the reference's order: gradients of the whole step first, then
 // Adam (pipeline/mod.rs:203-265, optimizer.rs:116-163). Views go through
-// LDS in chunks of TU_VCH, two barriers per chunk on every wave.
+// LDS in chunks of TU_VCH. A view's pair count and accumulator values are
+// loaded together, one view ahead of the one being computed: the raster
+// accumulators are zero wherever the count is (the backward adds only to
+// Gaussians with pairs, every consumed value is cleared here, and the host
+// clears them all after a void step), so their loads need not wait for the
+// count. The geometry wave issues its first view's loads before it waits
+// for the colour waves' direction gradients, and both roles issue their
+// Adam moments' loads before the last computation that precedes their use.
+// One barrier per chunk, and a second only when another chunk follows (the
+// chunk's LDS slots are rewritten then), so the colour waves go on to their
+// Adam while the geometry wave computes. The update runs only when the
+// step's status word allows it: the same word for every thread, read once
+// and tested before each role's first store or barrier, so the whole grid
+// updates or none of it.
 constexpr int TU_G = 64;
 constexpr int TU_VCH = 4;
+constexpr uint32_t ST_SKIP_UPDATE = ST_PAIR_OVERFLOW | ST_NAN_DEPTH | ST_NEED_BACK;
 
 __device__ __forceinline__ void train_colour(float* __restrict__ P, float* __restrict__ M1, float* __restrict__ M2,
                                              int cap, int i, bool live, int ch, const CamDev* __restrict__ cams,
                                              int n_views, const uint32_t* __restrict__ cnt,
                                              float* __restrict__ accum, size_t acc_stride,
                                              float (*s_dd)[3][3][TU_G], int lane, const AdamConsts* __restrict__ kd,
-                                             float* __restrict__ gout) {
+                                             float* __restrict__ gout, uint32_t stv) {
   float pos[3] = {0.0f, 0.0f, 0.0f}, cf[16], gs[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) cf[k] = 0.0f, gs[k] = 0.0f;
   const size_t row0 = (size_t)(P_SH + ch * 16);
+  float* arow = accum + (size_t)ch * acc_stride + i;
+  // the next view's count and accumulator value (zero for lanes past the scene)
+  uint32_t cn = 0;
+  float an = 0.0f;
   if (live) {
 #pragma unroll
     for (int q = 0; q < 3; ++q) pos[q] = P[(size_t)q * cap + i];
 #pragma unroll
     for (int k = 0; k < 16; ++k) cf[k] = P[(row0 + k) * cap + i];
+    cn = cnt[i];
+    an = arow[0];
   }
+  if (stv & ST_SKIP_UPDATE) return;
   const ShPos L{pos};
   for (int v0 = 0; v0 < n_views; v0 += TU_VCH) {
     const int nv = min(TU_VCH, n_views - v0);
 #pragma unroll 1
     for (int j = 0; j < nv; ++j) {
       const int v = v0 + j;
-      const size_t vi = (size_t)v * cap + i;
+      const uint32_t cj = cn;
+      const float aj = an;
+      if (live && v + 1 < n_views) {
+        cn = cnt[(size_t)(v + 1) * cap + i];
+        an = arow[(size_t)(v + 1) * cap];
+      }
       float dd[3] = {0.0f, 0.0f, 0.0f};
-      if (live && cnt[vi] != 0) {
-        const float a = accum[(size_t)ch * acc_stride + vi];
+      if (cj != 0) {
         float basis[16], d[3];
         view_basis(cams[v], L, basis, d);
-        float s = 0.0f;
+        float sum = 0.0f;
 #pragma unroll
-        for (int k = 0; k < 16; ++k) s += basis[k] * cf[k];
-        const float raw = 0.5f + s;
-        const float m = (raw > 0.0f && raw < 1.0f) ? a : 0.0f;
+        for (int k = 0; k < 16; ++k) sum += basis[k] * cf[k];
+        const float raw = 0.5f + sum;
+        const float m = (raw > 0.0f && raw < 1.0f) ? aj : 0.0f;
 #pragma unroll
         for (int k = 0; k < 16; ++k) gs[k] += m * basis[k];
         float t[3];
         sh_basis_grad_contract(d[0], d[1], d[2], cf, t);
 #pragma unroll
         for (int q = 0; q < 3; ++q) dd[q] = m * t[q];
+        arow[(size_t)v * cap] = 0.0f;  // consumed
       }
 #pragma unroll
       for (int q = 0; q < 3; ++q) s_dd[j][ch][q][lane] = dd[q];
@@ -499,20 +525,19 @@ __device__ __forceinline__ void train_colour(float* __restrict__ P, float* __res
     M1[o] = mm;
     M2[o] = vv;
   }
-  for (int v = 0; v < n_views; ++v) {
-    const size_t vi = (size_t)v * cap + i;
-    if (cnt[vi] != 0) accum[(size_t)ch * acc_stride + vi] = 0.0f;
-  }
 }
 
+// s_ga[j][0][lane]: view v0 + j's pair count (as bits), [1..6]: its
+// accumulator values 3-8, staged by the geometry wave for itself.
 __device__ __forceinline__ void train_geometry(float* __restrict__ P, float* __restrict__ M1,
                                                float* __restrict__ M2, int cap, int i, bool live,
                                                const CamDev* __restrict__ cams, int n_views,
                                                const uint32_t* __restrict__ cnt, float* __restrict__ accum,
-                                               size_t acc_stride, float (*s_dd)[3][3][TU_G], int lane,
-                                               float* __restrict__ gaccum, uint32_t* __restrict__ gdenom,
-                                               const AdamConsts* __restrict__ kd, float* __restrict__ gout) {
-  float pos[3] = {0.0f, 0.0f, 0.0f}, prm[11];
+                                               size_t acc_stride, float (*s_dd)[3][3][TU_G],
+                                               float (*s_ga)[7][TU_G], int lane, float* __restrict__ gaccum,
+                                               uint32_t* __restrict__ gdenom, const AdamConsts* __restrict__ kd,
+                                               float* __restrict__ gout, uint32_t stv) {
+  float prm[11];
   if (live) {
 #pragma unroll
     for (int j = 0; j < 11; ++j) prm[j] = P[(size_t)j * cap + i];
@@ -520,6 +545,8 @@ __device__ __forceinline__ void train_geometry(float* __restrict__ P, float* __r
 #pragma unroll
     for (int j = 0; j < 11; ++j) prm[j] = 0.0f;
   }
+  if (stv & ST_SKIP_UPDATE) return;
+  float pos[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) pos[q] = prm[q];
   const GeomConst G = geom_const([&](int j) { return prm[j]; });
@@ -531,21 +558,37 @@ __device__ __forceinline__ void train_geometry(float* __restrict__ P, float* __r
   bool any = false;
   for (int v0 = 0; v0 < n_views; v0 += TU_VCH) {
     const int nv = min(TU_VCH, n_views - v0);
+    // the chunk's counts and accumulator values, all in flight together
+    // while the colour waves work, then staged in LDS
+#pragma unroll
+    for (int j = 0; j < TU_VCH; ++j) {
+      const bool in = live && j < nv;
+      const size_t vi = (size_t)(v0 + j) * cap + i;
+      s_ga[j][0][lane] = __builtin_bit_cast(float, in ? cnt[vi] : 0u);
+#pragma unroll
+      for (int q = 0; q < 6; ++q) s_ga[j][1 + q][lane] = in ? accum[(size_t)(q + 3) * acc_stride + vi] : 0.0f;
+    }
     __syncthreads();  // direction gradients of the chunk are in LDS
 #pragma unroll 1
     for (int j = 0; j < nv; ++j) {
-      const int v = v0 + j;
-      const size_t vi = (size_t)v * cap + i;
-      if (!live || cnt[vi] == 0) continue;
+      if (__builtin_bit_cast(uint32_t, s_ga[j][0][lane]) == 0u) continue;
       float acc[9];
 #pragma unroll
-      for (int q = 0; q < 9; ++q) acc[q] = q < 3 ? 0.0f : accum[(size_t)q * acc_stride + vi];
+      for (int q = 0; q < 9; ++q) acc[q] = q < 3 ? 0.0f : s_ga[j][q - 2][lane];
       float dd[3];
 #pragma unroll
       for (int q = 0; q < 3; ++q) dd[q] = (s_dd[j][0][q][lane] + s_dd[j][1][q][lane]) + s_dd[j][2][q][lane];
-      geometry_view(cams[v], pos, S, acc, dd, dsig, g, sgn);
+      geometry_view(cams[v0 + j], pos, S, acc, dd, dsig, g, sgn);
       acc3 += acc[3];
       any = true;
+    }
+    // the chunk's accumulators 3-8 are consumed
+#pragma unroll 1
+    for (int j = 0; j < nv; ++j) {
+      if (__builtin_bit_cast(uint32_t, s_ga[j][0][lane]) == 0u) continue;
+      const size_t vi = (size_t)(v0 + j) * cap + i;
+#pragma unroll
+      for (int q = 3; q < 9; ++q) accum[(size_t)q * acc_stride + vi] = 0.0f;
     }
     __syncthreads();  // the chunk's LDS slots may be rewritten
   }
@@ -575,38 +618,26 @@ __device__ __forceinline__ void train_geometry(float* __restrict__ P, float* __r
   if (gout) gout[(size_t)59 * cap + i] = sgn;
   gaccum[i] += sgn;
   gdenom[i] += (uint32_t)n_views;
-  for (int v = 0; v < n_views; ++v) {
-    const size_t vi = (size_t)v * cap + i;
-    if (cnt[vi] == 0) continue;
-#pragma unroll
-    for (int q = 3; q < 9; ++q) accum[(size_t)q * acc_stride + vi] = 0.0f;
-  }
 }
 
-constexpr uint32_t ST_SKIP_UPDATE = ST_PAIR_OVERFLOW | ST_NAN_DEPTH | ST_NEED_BACK;
-
-__global__ __launch_bounds__(256) void k_train_update(float* __restrict__ P, float* __restrict__ M1,
-                                                      float* __restrict__ M2, int cap, int n,
-                                                      const CamDev* __restrict__ cams, int n_views,
-                                                      const uint32_t* __restrict__ cnt, float* __restrict__ accum,
-                                                      size_t acc_stride, float* __restrict__ gaccum,
-                                                      uint32_t* __restrict__ gdenom,
-                                                      const AdamConsts* __restrict__ k_dev,
-                                                      const uint32_t* __restrict__ status,
-                                                      float* __restrict__ gout) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_train_update(
+    float* __restrict__ P, float* __restrict__ M1, float* __restrict__ M2, int cap, int n,
+    const CamDev* __restrict__ cams, int n_views, const uint32_t* __restrict__ cnt, float* __restrict__ accum,
+    size_t acc_stride, float* __restrict__ gaccum, uint32_t* __restrict__ gdenom,
+    const AdamConsts* __restrict__ k_dev, const uint32_t* __restrict__ status, float* __restrict__ gout) {
   __shared__ float s_dd[TU_VCH][3][3][TU_G];  // [view in chunk][channel][component][Gaussian]
-  // the same word for every thread: the whole grid returns or none of it
-  if (*status & ST_SKIP_UPDATE) return;
+  __shared__ float s_ga[TU_VCH][7][TU_G];     // the geometry wave's counts and accumulator values
+  const uint32_t stv = *status;
   const int lane = threadIdx.x & 63;
   // wave-uniform role (scalar), so that each wave runs one role's barriers
   const int role = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int i = blockIdx.x * TU_G + lane;
   const bool live = i < n;
   if (role < 3)
-    train_colour(P, M1, M2, cap, i, live, role, cams, n_views, cnt, accum, acc_stride, s_dd, lane, k_dev, gout);
+    train_colour(P, M1, M2, cap, i, live, role, cams, n_views, cnt, accum, acc_stride, s_dd, lane, k_dev, gout, stv);
   else
-    train_geometry(P, M1, M2, cap, i, live, cams, n_views, cnt, accum, acc_stride, s_dd, lane, gaccum, gdenom,
-                   k_dev, gout);
+    train_geometry(P, M1, M2, cap, i, live, cams, n_views, cnt, accum, acc_stride, s_dd, s_ga, lane, gaccum, gdenom,
+                   k_dev, gout, stv);
 }
 
 void launch_train_adam(float* P, float* M1, float* M2, int cap, int n, const CamDev* cams, int n_views,

@@ -2232,7 +2232,12 @@ static int train_step_impl(rgs_ctx* c, const int* ids, int n_ids, const rgs_trai
     HIPCHK(hipMemcpyAsync(c->h_loss, c->loss.p, vs.cams.size() * 4, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     uint32_t status = c->h_scalars[SC_STATUS];
-    if (status & ST_NAN_DEPTH) return fail(RGS_E_NAN, "NaN depth in projection");
+    if (status & ST_NAN_DEPTH) {
+      // the update was skipped on the device; the raster's sums are void
+      launch_fill_f32(P_<float>(c->accum), acc_n, 0.0f, c->st);
+      (void)hipStreamSynchronize(c->st);
+      return fail(RGS_E_NAN, "NaN depth in projection");
+    }
     if (status & ST_PAIR_OVERFLOW) {
       // the step was skipped on the device (parameters untouched): grow and redo
       size_t K = std::max<size_t>(c->h_scalars[SC_TOTAL], c->h_scalars[SC_BTOTAL]);
