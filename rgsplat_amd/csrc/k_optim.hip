@@ -454,7 +454,7 @@ void launch_adam_aos(float* g, const float* grads, size_t n, float* m1, float* m
 // updates or none of it.
 constexpr int TU_G = 64;
 constexpr int TU_VCH = 4;
-constexpr uint32_t ST_SKIP_UPDATE = ST_PAIR_OVERFLOW | ST_NAN_DEPTH | ST_NEED_BACK;
+constexpr uint32_t ST_SKIP_UPDATE = ST_PAIR_OVERFLOW | ST_NAN_DEPTH | ST_NEED_BACK | ST_SCAN_STALL;
 
 __device__ __forceinline__ void train_colour(float* __restrict__ P, float* __restrict__ M1, float* __restrict__ M2,
                                              int cap, int i, bool live, int ch, const CamDev* __restrict__ cams,

@@ -435,19 +435,6 @@ void scan_gathered_counts(const uint32_t* vals, const uint32_t* cnt, size_t n, u
                           uint32_t* block_sums, uint32_t* total, hipStream_t st) {
   scan_impl<uint32_t>(GatherCnt{vals, cnt}, n, out, block_sums, total, st);
 }
-// Entries below the cut (the depth-cut partition).
-struct BelowCut {
-  const uint32_t* keys;
-  const CutState* cs;
-  __device__ uint32_t operator()(size_t r) const { return keys[r] < cs->key ? 1u : 0u; }
-};
-// cnt[vals[r]] over the first *n_dev entries, 0 past them.
-struct GatherCntN {
-  const uint32_t* vals;
-  const uint32_t* cnt;
-  const uint32_t* n_dev;
-  __device__ uint32_t operator()(size_t r) const { return r < *n_dev ? cnt[vals[r]] : 0u; }
-};
 // Number of flagged bins in the bin rect of splat vi.
 __device__ __forceinline__ uint32_t flagged_bins(uint32_t vi, const uint32_t* cnt, const uint2* rect, int cap,
                                                  const CamDev* cams, const uint8_t* flag, bool any) {
@@ -481,14 +468,6 @@ struct TouchFlagged {
   }
 };
 
-void scan_below_cut(const uint32_t* keys, size_t n, const CutState* cs, uint32_t* out, uint32_t* block_sums,
-                    uint32_t* total, hipStream_t st) {
-  scan_impl<uint32_t>(BelowCut{keys, cs}, n, out, block_sums, total, st);
-}
-void scan_gathered_counts_n(const uint32_t* vals, const uint32_t* cnt, size_t n, const uint32_t* n_dev,
-                            uint32_t* out, uint32_t* block_sums, uint32_t* total, hipStream_t st) {
-  scan_impl<uint32_t>(GatherCntN{vals, cnt, n_dev}, n, out, block_sums, total, st);
-}
 void scan_touch_flagged(const uint32_t* vals, size_t n, const uint32_t* n_dev, const uint32_t* n_flagged,
                         const SplatArrays S, int cap, const CamDev* cams, const uint8_t* flag, uint32_t* out,
                         uint32_t* block_sums, uint32_t* total, hipStream_t st) {
@@ -501,6 +480,248 @@ void scan_u32(const uint32_t* in, size_t n, uint32_t* out, uint32_t* block_sums,
 }
 void scan_u64(const uint64_t* in, size_t n, uint64_t* out, uint64_t* block_sums, uint64_t* total, hipStream_t st) {
   scan_impl<uint64_t>(Plain<uint64_t>{in}, n, out, block_sums, total, st);
+}
+
+// ------------------------------------------- single-pass scans (look-back)
+// An exclusive scan in one launch: each workgroup takes the next tile of
+// LB_TILE entries from a ticket counter, publishes its tile's sum, then one
+// wave looks back over the tiles before it (256 per round, four per lane)
+// for the sum of everything in front of it, publishes its inclusive prefix
+// and the workgroup writes its entries. Tiles are numbered in the order
+// workgroups start, so a tile only waits for tiles whose workgroups are
+// already running, and every running workgroup publishes its own sum before
+// it waits for anything: the look-back always ends. Its wait is bounded all
+// the same (LB_SPIN_LIMIT polls); past the bound the workgroup flags
+// ST_SCAN_STALL in the status word and finishes with a wrong prefix, and the
+// caller turns the flag into an error. Only the workgroups whose tiles hold
+// entries take a ticket. Tile words: bits 62-63 the flag (0: nothing yet,
+// LB_AGG: the tile's own sum, LB_INC: the inclusive prefix through the
+// tile), bits 0-61 the value. The state (LB_HEAD words, the first holding
+// the ticket, then a word per tile) must be zero when the launch starts.
+constexpr uint64_t LB_AGG = 1ull << 62, LB_INC = 2ull << 62, LB_VAL = (1ull << 62) - 1;
+constexpr uint32_t LB_SPIN_LIMIT = 1u << 16;
+constexpr int LB_WIN = 4;  // look-back words per lane and round
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+// The sum of the tiles before `tile` (tw: the tile words), by one whole wave.
+__device__ __forceinline__ uint64_t look_back(uint64_t* tw, int tile, uint32_t* status) {
+  const int lane = (int)(threadIdx.x & 63);
+  uint64_t prefix = 0;
+  int hi = tile - 1;  // the nearest tile not yet added
+  uint32_t polls = 0;
+  while (hi >= 0) {
+    // word j of this lane: tile hi - (64 j + lane), i.e. distance 64 j + lane
+    uint64_t w[LB_WIN];
+#pragma unroll
+    for (int j = 0; j < LB_WIN; ++j) {
+      const int idx = hi - (64 * j + lane);
+      w[j] = idx >= 0 ? __hip_atomic_load(&tw[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : LB_INC;
+    }
+    // the nearest inclusive prefix (word jf, lane kf); everything nearer it
+    // must have published at least its own sum
+    int jf = LB_WIN, kf = 63;
+    bool missing = false;
+#pragma unroll
+    for (int j = 0; j < LB_WIN; ++j) {
+      if (jf < LB_WIN) break;
+      const uint64_t fl = w[j] & ~LB_VAL;
+      const uint64_t inc = __ballot(fl == LB_INC);
+      const uint64_t none = __ballot(fl == 0);
+      const int k = inc ? __builtin_ctzll(inc) : 63;
+      const uint64_t upto = k == 63 ? ~0ull : ((2ull << k) - 1ull);
+      if (none & upto) {
+        missing = true;
+        break;
+      }
+      if (inc) jf = j, kf = k;
+    }
+    if (missing) {
+      if (++polls > LB_SPIN_LIMIT) {
+        if (lane == 0) atomicOr(status, ST_SCAN_STALL);
+        return prefix;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    uint64_t x = 0;
+#pragma unroll
+    for (int j = 0; j < LB_WIN; ++j)
+      if (j < jf || (j == jf && lane <= kf)) x += w[j] & LB_VAL;
+    prefix += wave_sum_u64(x);
+    if (jf < LB_WIN) break;
+    hi -= 64 * LB_WIN;
+  }
+  return prefix;
+}
+
+// The entries' counts and what is written for them. F::Item holds what an
+// entry's output needs; F::count gives its count; F::put writes entry e
+// given the exclusive prefix of the counts; F::last gets the total. TILE:
+// entries per workgroup (a multiple of SCAN_THREADS; fewer, larger tiles
+// shorten the look-back, smaller ones need fewer registers).
+template <class F, int TILE>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_lookback(F f, size_t n_max, const uint32_t* __restrict__ n_dev,
+                                                                uint64_t* __restrict__ state,
+                                                                uint32_t* __restrict__ total,
+                                                                uint32_t* __restrict__ status) {
+  __shared__ uint64_t s_tmp[8];
+  __shared__ uint32_t s_tile;
+  __shared__ uint64_t s_prefix;
+  const size_t n = n_dev ? min((size_t)*n_dev, n_max) : n_max;
+  constexpr int LB_ITEMS = TILE / SCAN_THREADS;
+  const uint32_t needed = max((uint32_t)((n + TILE - 1) / TILE), 1u);
+  if (blockIdx.x >= needed) return;  // exactly `needed` workgroups take tickets 0..needed-1
+  if (threadIdx.x == 0) s_tile = atomicAdd(reinterpret_cast<uint32_t*>(state), 1u);
+  __syncthreads();
+  const uint32_t tile = s_tile;
+  uint64_t* tw = state + LB_HEAD;
+  const size_t base = (size_t)tile * TILE;
+  typename F::Item it[LB_ITEMS];
+  // thread t holds entries base + k * SCAN_THREADS + t (row k): coalesced
+  // loads and stores; entries past n hold the item of count 0
+#pragma unroll
+  for (int k = 0; k < LB_ITEMS; ++k) {
+    const size_t e = base + (size_t)k * SCAN_THREADS + threadIdx.x;
+    it[k] = e < n ? f.item(e) : F::empty();
+  }
+  // exclusive prefix within the tile: within the row's wave, then the row's
+  // earlier waves, then the earlier rows
+  uint32_t rank[LB_ITEMS];
+#pragma unroll
+  for (int k = 0; k < LB_ITEMS; ++k) {
+    const uint32_t c = f.count(it[k]);
+    uint32_t x = c;
+    const int lane = (int)(threadIdx.x & 63);
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(x, d, 64);
+      if (lane >= d) x += y;
+    }
+    rank[k] = x - c;  // exclusive within the wave
+  }
+  // per (row, wave) totals -> offsets
+  __shared__ uint32_t s_wt[LB_ITEMS][SCAN_THREADS / 64];
+  if ((threadIdx.x & 63) == 63) {
+#pragma unroll
+    for (int k = 0; k < LB_ITEMS; ++k) s_wt[k][threadIdx.x >> 6] = rank[k] + f.count(it[k]);
+  }
+  __syncthreads();
+  uint64_t tsum = 0;
+  {
+    const int w = (int)(threadIdx.x >> 6);
+    uint64_t run = 0;
+#pragma unroll
+    for (int k = 0; k < LB_ITEMS; ++k) {
+      uint64_t before = 0, row = 0;
+#pragma unroll
+      for (int u = 0; u < SCAN_THREADS / 64; ++u) {
+        const uint32_t t = s_wt[k][u];
+        if (u < w) before += t;
+        row += t;
+      }
+      rank[k] = (uint32_t)(run + before) + rank[k];  // exclusive within the tile (< 2^32 per tile)
+      run += row;
+    }
+    tsum = run;
+  }
+  if (threadIdx.x < 64) {
+    if (threadIdx.x == 0)
+      __hip_atomic_store(&tw[tile], (tile == 0 ? LB_INC : LB_AGG) | (tsum & LB_VAL), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t prefix = 0;
+    if (tile > 0) {
+      prefix = look_back(tw, (int)tile, status);
+      if (threadIdx.x == 0)
+        __hip_atomic_store(&tw[tile], LB_INC | ((prefix + tsum) & LB_VAL), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (threadIdx.x == 0) s_prefix = prefix;
+  }
+  __syncthreads();
+  const uint64_t prefix = s_prefix;
+#pragma unroll
+  for (int k = 0; k < LB_ITEMS; ++k) {
+    const size_t e = base + (size_t)k * SCAN_THREADS + threadIdx.x;
+    if (e < n) f.put(e, it[k], prefix + rank[k]);
+  }
+  if (threadIdx.x == 0 && tile == needed - 1) {
+    const uint64_t t = prefix + tsum;
+    *total = t >= (uint64_t)SCAN_SATURATED ? SCAN_SATURATED : (uint32_t)t;
+    f.last(t);
+  }
+}
+
+// The depth-cut partition: entry r (key, value) goes to the front arrays at
+// its exclusive count of front entries, or (with back arrays) to the back
+// arrays at r minus that count; the totals and the cut switch go to n_back
+// and cut_on.
+struct PartitionAtCut {
+  const uint32_t* keys;
+  const uint32_t* vals;
+  const CutState* cs;
+  uint32_t *kf, *vf, *kb, *vb;
+  size_t n;
+  uint32_t* n_back;
+  uint32_t* cut_on;
+  struct Item {
+    uint32_t k, v;
+  };
+  __device__ Item item(size_t r) const { return {keys[r], vals[r]}; }
+  __device__ static Item empty() { return {0xffffffffu, 0u}; }  // at or past any cut
+  __device__ uint32_t count(const Item& it) const { return it.k < cs->key ? 1u : 0u; }
+  __device__ void put(size_t r, const Item& it, uint64_t o) const {
+    if (it.k < cs->key) {
+      kf[o] = it.k;
+      vf[o] = it.v;
+    } else if (kb) {
+      kb[r - o] = it.k;
+      vb[r - o] = it.v;
+    }
+  }
+  __device__ void last(uint64_t t) const {
+    *n_back = (uint32_t)(n - t);
+    *cut_on = cs->key != 0xffffffffu ? 1u : 0u;
+  }
+};
+// Pair offsets: the exclusive scan of cnt[vals[r]] over the first *n_dev
+// sorted entries.
+struct PairOffsets {
+  const uint32_t* vals;
+  const uint32_t* cnt;
+  uint32_t* out;
+  struct Item {
+    uint32_t c;
+  };
+  __device__ Item item(size_t r) const { return {cnt[vals[r]]}; }
+  __device__ static Item empty() { return {0u}; }
+  __device__ uint32_t count(const Item& it) const { return it.c; }
+  __device__ void put(size_t r, const Item&, uint64_t o) const { out[r] = (uint32_t)o; }
+  __device__ void last(uint64_t) const {}
+};
+
+void partition_at_cut(const uint32_t* keys, const uint32_t* vals, size_t n, const CutState* cs, uint32_t* kf,
+                      uint32_t* vf, uint32_t* kb, uint32_t* vb, uint32_t* n_front, uint32_t* n_back, uint32_t* cut_on,
+                      uint64_t* state, uint32_t* status, hipStream_t st) {
+  // 4096-entry tiles: every tile is live here, and fewer of them shorten
+  // the look-back
+  constexpr int TILE = 2 * LB_TILE;
+  const int nb = std::max((int)((n + TILE - 1) / TILE), 1);
+  PartitionAtCut f{keys, vals, cs, kf, vf, kb, vb, n, n_back, cut_on};
+  hipLaunchKernelGGL((k_scan_lookback<PartitionAtCut, TILE>), dim3(nb), dim3(SCAN_THREADS), 0, st, f, n, nullptr, state,
+                     n_front, status);
+}
+
+void scan_pair_offsets(const uint32_t* vals, const uint32_t* cnt, size_t n_max, const uint32_t* n_dev,
+                       uint32_t* out, uint32_t* total, uint64_t* state, uint32_t* status, hipStream_t st) {
+  const int nb = std::max(lookback_tiles(n_max), 1);
+  PairOffsets f{vals, cnt, out};
+  hipLaunchKernelGGL((k_scan_lookback<PairOffsets, LB_TILE>), dim3(nb), dim3(SCAN_THREADS), 0, st, f, n_max, n_dev, state,
+                     total, status);
 }
 
 // ------------------------------------------------------------ pairs/ranges
@@ -621,39 +842,6 @@ void launch_gen_pairs(const uint32_t* sorted_vals, size_t n_sorted, const uint32
 }
 
 // ------------------------------------------------------- depth-cut binning
-// Stable partition of the projected (key, value) entries at the cut: entry r
-// goes to the front arrays at off[r] (the exclusive count of front entries
-// before it) or to the back arrays at r - off[r].
-__global__ __launch_bounds__(256) void k_partition(const uint32_t* __restrict__ keys,
-                                                   const uint32_t* __restrict__ vals, size_t n,
-                                                   const uint32_t* __restrict__ off, const CutState* __restrict__ cs,
-                                                   const uint32_t* __restrict__ n_front, uint32_t* __restrict__ kf,
-                                                   uint32_t* __restrict__ vf, uint32_t* __restrict__ kb,
-                                                   uint32_t* __restrict__ vb, uint32_t* __restrict__ n_back,
-                                                   uint32_t* __restrict__ cut_on) {
-  const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r == 0) {
-    *n_back = (uint32_t)(n - *n_front);
-    *cut_on = cs->key != 0xffffffffu ? 1u : 0u;
-  }
-  if (r >= n) return;
-  const uint32_t k = keys[r], v = vals[r], o = off[r];
-  if (k < cs->key) {
-    kf[o] = k;
-    vf[o] = v;
-  } else {
-    kb[r - o] = k;
-    vb[r - o] = v;
-  }
-}
-
-void launch_partition(const uint32_t* keys, const uint32_t* vals, size_t n, const uint32_t* off, const CutState* cs,
-                      const uint32_t* n_front, uint32_t* kf, uint32_t* vf, uint32_t* kb, uint32_t* vb,
-                      uint32_t* n_back, uint32_t* cut_on, hipStream_t st) {
-  hipLaunchKernelGGL(k_partition, dim3((unsigned)(((n ? n : 1) + 255) / 256)), dim3(256), 0, st, keys, vals, n, off,
-                     cs, n_front, kf, vf, kb, vb, n_back, cut_on);
-}
-
 // Keeps the entries that the scan of 0/1 selections off[] (total *total)
 // selected: entry r is kept iff off[r + 1] > off[r].
 __global__ __launch_bounds__(256) void k_compact(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,

@@ -442,7 +442,8 @@ int ensure_render(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, size_t k_cap
   c->depth_cap = dn;
   CHK(ensure(c, c->offsets, dn * 4));
   CHK(ensure(c, c->bsums, ((size_t)scan_nb(dn) + 1) * 8));
-  CHK(ensure(c, c->scalars, SC_COUNT * 4));
+  // the scalars, then the training step's two look-back scan states
+  CHK(ensure(c, c->scalars, SC_COUNT * 4 + 2 * lookback_words(dn) * 8));
   int nb = std::max(radix_nb_max(dn), radix_nb_max(k_cap));
   if (nb < 256) nb = 256;
   CHK(ensure(c, c->hist, (size_t)256 * nb * 4));
@@ -509,7 +510,19 @@ void enqueue_project(rgs_ctx* c, const float* P, size_t cap, int n, const ViewSe
 // The training step's projection: the entries are split at the depth cut,
 // and only the front ones are depth-sorted and scanned here (the back ones
 // that meet a bin left unsaturated are sorted after the front raster).
-void enqueue_project_cut(rgs_ctx* c, const float* P, size_t cap, int n, const ViewSet& vs, uint8_t* cover) {
+// The training step's look-back scan states (k = 0: the partition at the
+// cut, 1: the pair offsets), after the scalars; the step's first fill clears
+// them with the scalars.
+size_t step_scalar_words(const rgs_ctx* c) { return SC_COUNT + 4 * lookback_words(c->depth_cap); }
+uint64_t* lb_state(rgs_ctx* c, int k) {
+  return reinterpret_cast<uint64_t*>(P_<uint32_t>(c->scalars) + SC_COUNT) + (size_t)k * lookback_words(c->depth_cap);
+}
+
+// The training step's projection, partition at the cut (back entries only
+// when the step has the back-list stage), depth sort of the front entries
+// and pair offsets.
+void enqueue_project_cut(rgs_ctx* c, const float* P, size_t cap, int n, const ViewSet& vs, uint8_t* cover,
+                         bool with_back) {
   hipStream_t st = c->st;
   uint32_t* sc = P_<uint32_t>(c->scalars);
   int nv = (int)vs.cams.size();
@@ -522,10 +535,10 @@ void enqueue_project_cut(rgs_ctx* c, const float* P, size_t cap, int n, const Vi
   stage_end(c, S_PROJECT);
   RadixScratch rs{P_<uint32_t>(c->hist), P_<uint32_t>(c->rowtot), c->hist_nb};
   stage_begin(c, S_DSORT);
-  scan_below_cut(P_<uint32_t>(c->dk[0]), dn, cs, P_<uint32_t>(c->part), P_<uint32_t>(c->bsums), sc + SC_NF, st);
-  launch_partition(P_<uint32_t>(c->dk[0]), P_<uint32_t>(c->dv[0]), dn, P_<uint32_t>(c->part), cs, sc + SC_NF,
-                   P_<uint32_t>(c->dk[1]), P_<uint32_t>(c->dv[1]), P_<uint32_t>(c->kB), P_<uint32_t>(c->vB),
-                   sc + SC_NB, sc + SC_CUTON, st);
+  partition_at_cut(P_<uint32_t>(c->dk[0]), P_<uint32_t>(c->dv[0]), dn, cs, P_<uint32_t>(c->dk[1]),
+                   P_<uint32_t>(c->dv[1]), with_back ? P_<uint32_t>(c->kB) : nullptr,
+                   with_back ? P_<uint32_t>(c->vB) : nullptr, sc + SC_NF, sc + SC_NB, sc + SC_CUTON, lb_state(c, 0),
+                   sc + SC_STATUS, st);
   uint32_t* keys[2] = {P_<uint32_t>(c->dk[1]), P_<uint32_t>(c->dk[0])};
   uint32_t* vals[2] = {P_<uint32_t>(c->dv[1]), P_<uint32_t>(c->dv[0])};
   int cur = radix_sort_pairs(keys, vals, sc + SC_NF, dn, 32, rs, st);
@@ -533,8 +546,8 @@ void enqueue_project_cut(rgs_ctx* c, const float* P, size_t cap, int n, const Vi
   c->sorted_vals = vals[cur];
   c->sorted_keys = keys[cur];
   stage_begin(c, S_SCAN);
-  scan_gathered_counts_n(vals[cur], P_<uint32_t>(c->cnt), dn, sc + SC_NF, P_<uint32_t>(c->offsets),
-                         P_<uint32_t>(c->bsums), sc + SC_TOTAL, st);
+  scan_pair_offsets(vals[cur], P_<uint32_t>(c->cnt), dn, sc + SC_NF, P_<uint32_t>(c->offsets), sc + SC_TOTAL,
+                    lb_state(c, 1), sc + SC_STATUS, st);
   stage_end(c, S_SCAN);
 }
 
@@ -2102,8 +2115,8 @@ static void enqueue_step_rest(rgs_ctx* c, const ViewSet& vs, const float bg[3], 
 
 // Enqueues one whole training step on the context's stream (no host sync).
 static void enqueue_step(rgs_ctx* c, const ViewSet& vs, const float bg[3], float lambda, bool with_back) {
-  launch_fill_u32(P_<uint32_t>(c->scalars), SC_COUNT, 0u, c->st);
-  enqueue_project_cut(c, P_<float>(c->P), c->cap, (int)c->n, vs, step_cover(c, bg));
+  launch_fill_u32(P_<uint32_t>(c->scalars), step_scalar_words(c), 0u, c->st);
+  enqueue_project_cut(c, P_<float>(c->P), c->cap, (int)c->n, vs, step_cover(c, bg), with_back);
   enqueue_step_rest(c, vs, bg, lambda, with_back);
 }
 
@@ -2232,11 +2245,12 @@ static int train_step_impl(rgs_ctx* c, const int* ids, int n_ids, const rgs_trai
     HIPCHK(hipMemcpyAsync(c->h_loss, c->loss.p, vs.cams.size() * 4, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     uint32_t status = c->h_scalars[SC_STATUS];
-    if (status & ST_NAN_DEPTH) {
+    if (status & (ST_NAN_DEPTH | ST_SCAN_STALL)) {
       // the update was skipped on the device; the raster's sums are void
       launch_fill_f32(P_<float>(c->accum), acc_n, 0.0f, c->st);
       (void)hipStreamSynchronize(c->st);
-      return fail(RGS_E_NAN, "NaN depth in projection");
+      if (status & ST_NAN_DEPTH) return fail(RGS_E_NAN, "NaN depth in projection");
+      return fail(RGS_E_HIP, "a single-pass scan's look-back gave up waiting");
     }
     if (status & ST_PAIR_OVERFLOW) {
       // the step was skipped on the device (parameters untouched): grow and redo

@@ -2,6 +2,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <stddef.h>
 #include <stdint.h>
 
@@ -23,7 +25,7 @@ struct SplatArrays {
 // Status bits written by kernels into a device word.
 // ST_NEED_BACK: a step run without the depth cut's back lists flagged a bin
 // (its results are void; the parameter update is skipped).
-enum : uint32_t { ST_NAN_DEPTH = 1u, ST_PAIR_OVERFLOW = 2u, ST_NEED_BACK = 4u };
+enum : uint32_t { ST_NAN_DEPTH = 1u, ST_PAIR_OVERFLOW = 2u, ST_NEED_BACK = 4u, ST_SCAN_STALL = 8u };
 
 // ---- projection (k_project.hip)
 // Projects n Gaussians (SoA params P[59][cap]) into n_views views. Writes the
@@ -135,19 +137,24 @@ struct CutState {       // persistent device words
   uint32_t mode;        // 0 adaptive, 1 no cut, 2 fixed key
   uint32_t pad;
 };
-// Exclusive count of the entries below the cut (keys[r] < cs->key); the
-// number of front entries goes to *total.
-void scan_below_cut(const uint32_t* keys, size_t n, const CutState* cs, uint32_t* out, uint32_t* block_sums,
-                    uint32_t* total, hipStream_t st);
-// Stable partition at the cut into front (kf, vf: *n_front entries) and back
-// (kb, vb: *n_back = n - *n_front entries) arrays; *cut_on = 1 when a cut is
-// set (a front list may then be partial).
-void launch_partition(const uint32_t* keys, const uint32_t* vals, size_t n, const uint32_t* off, const CutState* cs,
-                      const uint32_t* n_front, uint32_t* kf, uint32_t* vf, uint32_t* kb, uint32_t* vb,
-                      uint32_t* n_back, uint32_t* cut_on, hipStream_t st);
-// Exclusive scan of cnt[vals[r]] over the first *n_dev of n entries (0 past them).
-void scan_gathered_counts_n(const uint32_t* vals, const uint32_t* cnt, size_t n, const uint32_t* n_dev,
-                            uint32_t* out, uint32_t* block_sums, uint32_t* total, hipStream_t st);
+// The training step's partition at the cut and its pair offsets, each a
+// single-pass (look-back) scan. state: lookback_words(n) u64 words, zero
+// when the call starts; status gets ST_SCAN_STALL if a look-back gave up.
+constexpr int LB_HEAD = 1;
+constexpr int LB_TILE = 2048;  // entries per look-back tile (at most)
+inline int lookback_tiles(size_t n) { return (int)((n + LB_TILE - 1) / LB_TILE); }
+inline size_t lookback_words(size_t n) { return (size_t)LB_HEAD + (size_t)std::max(lookback_tiles(n), 1); }
+// Stable partition of the projected (key, value) entries at the cut into
+// front (kf, vf: *n_front entries, keys[r] < cs->key) and back (kb, vb:
+// *n_back = n - *n_front entries; may be null: nothing written) arrays;
+// *cut_on = 1 when a cut is set (a front list may then be partial).
+void partition_at_cut(const uint32_t* keys, const uint32_t* vals, size_t n, const CutState* cs, uint32_t* kf,
+                      uint32_t* vf, uint32_t* kb, uint32_t* vb, uint32_t* n_front, uint32_t* n_back, uint32_t* cut_on,
+                      uint64_t* state, uint32_t* status, hipStream_t st);
+// Exclusive scan of cnt[vals[r]] over the first *n_dev of n_max entries
+// (pair offsets in depth order); the total goes to *total.
+void scan_pair_offsets(const uint32_t* vals, const uint32_t* cnt, size_t n_max, const uint32_t* n_dev,
+                       uint32_t* out, uint32_t* total, uint64_t* state, uint32_t* status, hipStream_t st);
 // Exclusive count of the back entries (first *n_dev) whose rect meets a
 // flagged bin (all 0 when no bin is flagged).
 void scan_touch_flagged(const uint32_t* vals, size_t n, const uint32_t* n_dev, const uint32_t* n_flagged,
