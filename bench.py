@@ -438,6 +438,10 @@ def main():
     g = wl.scene()
     cams = wl.cameras()
     ctx = runtime.Context(0)
+    # the practical ceiling of the memory-bound stages: a plain streaming
+    # device copy of 1 GiB (read + written bytes per second)
+    copy_gbs = ctx.hbm_copy_probe(1 << 30, 10)
+    print(f"[bench] HBM copy probe {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
     tscene = wl.target_scene()
     targets = [with_alpha(ctx.render_color(tscene, c)) for c in cams]
     # The oracle check and CPU baseline run on the timed views, except for a
@@ -577,6 +581,8 @@ def main():
                 peak = FP32_PEAK_TFLOPS
             rl[s] = {"bound": bound, "achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak,
                      "traffic": traffic.get(s), "ms": stages[s]}
+            if bound == "hbm":
+                rl[s]["frac_of_copy"] = ach / copy_gbs
             if bound == "valu":
                 rl[s]["frac_scalar_issue"] = ach / FP32_SCALAR_TFLOPS
                 rl[s]["achieved_basis"] = ("reference-equivalent: the reference algorithm's evaluated (pixel, splat) "
@@ -591,6 +597,10 @@ def main():
         else:
             out["roofline"] = {"kernel": dom, "bound": None, "achieved": None, "peak": None, "unit": None,
                                "frac": None, "traffic": traffic.get(dom), "ms": stages[dom]}
+        out["hbm_copy_probe"] = {"gbs": copy_gbs, "bytes": 1 << 30, "frac_of_peak": copy_gbs / HBM_PEAK_GBS,
+                                 "what": "plain streaming device copy (16-byte loads and stores), read + written "
+                                         "bytes per second: the practical ceiling of the memory-bound stages "
+                                         "(frac_of_copy)"}
         out["stages_ms"] = stages
         out["stages_source"] = (f"{dom}: the timed steps (events around it only); the others: {args.steps} "
                                 "further steps with events around every stage")

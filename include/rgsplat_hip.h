@@ -370,6 +370,14 @@ int rgs_set_grad_capture(rgs_ctx* ctx, int enable);
  * visit. Not for a step whose depth cut resumed tiles on back lists. */
 int rgs_raster_work_counts(rgs_ctx* ctx, uint64_t* counts);
 int rgs_step_grads(rgs_ctx* ctx, rgs_grad* grads_out, size_t n);
+/* Measurement aid (no reference counterpart): the HBM bandwidth a plain
+ * streaming copy reaches on the context's device, the practical ceiling of
+ * the memory-bound kernels. Copies `bytes` (rounded down to 16 B) from one
+ * fresh device buffer to another with 16-byte loads and stores, `reps` timed
+ * launches after two untimed ones, and returns read + written bytes per
+ * second of the mean launch in *gbs (GB/s). Allocates and frees its two
+ * buffers. */
+int rgs_hbm_copy_probe(rgs_ctx* ctx, size_t bytes, int reps, float* gbs);
 
 #ifdef __cplusplus
 }

@@ -2599,6 +2599,42 @@ int rgs_set_split_scale_divisor(rgs_ctx* c, float divisor) {
   return 0;
 }
 
+int rgs_hbm_copy_probe(rgs_ctx* c, size_t bytes, int reps, float* gbs) {
+  if (!c || !gbs || reps <= 0) return fail(RGS_E_ARG, "null argument or no repetitions");
+  const size_t n16 = bytes / 16;
+  if (n16 == 0) return fail(RGS_E_ARG, "probe size below 16 bytes");
+  if (!c->st) return fail(RGS_E_ARG, "the probe needs a non-null stream");
+  (void)hipSetDevice(c->device);
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, n16 * 16) != hipSuccess) return fail(RGS_E_HIP, "hipMalloc failed");
+  if (hipMalloc(&b, n16 * 16) != hipSuccess) {
+    (void)hipFree(a);
+    return fail(RGS_E_HIP, "hipMalloc failed");
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = 0;
+  float ms = 0.0f;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    rc = fail(RGS_E_HIP, "hipEventCreate failed");
+  } else {
+    launch_fill_u32(static_cast<uint32_t*>(a), n16 * 4, 0u, c->st);
+    for (int r = 0; r < 2; ++r) launch_copy16(a, b, n16, c->st);
+    (void)hipEventRecord(e0, c->st);
+    for (int r = 0; r < reps; ++r) launch_copy16(a, b, n16, c->st);
+    (void)hipEventRecord(e1, c->st);
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess || ms <= 0.0f)
+      rc = fail(RGS_E_HIP, "probe timing failed");
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(c->st);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (rc) return rc;
+  *gbs = (float)(2.0 * (double)(n16 * 16) / ((double)ms / reps * 1e-3) / 1e9);
+  return 0;
+}
+
 int rgs_raster_work_counts(rgs_ctx* c, uint64_t* counts) {
   if (!c || !counts) return fail(RGS_E_ARG, "null argument");
   if (!c->last_cut) return fail(RGS_E_ARG, "no training step rendered yet");

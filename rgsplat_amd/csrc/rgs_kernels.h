@@ -377,6 +377,8 @@ void launch_densify(const float* P, float* P_out, int cap, int n, const float* g
                     float grad_thresh, float prune_thresh, uint64_t max_gaussians, float extent, DensifyScratch ds,
                     uint32_t* n_out, hipStream_t st, float split_div = 0.0f);
 void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t st);
+// b[i] = a[i] for n16 16-byte elements (the HBM copy probe).
+void launch_copy16(const void* a, void* b, size_t n16, hipStream_t st);
 void launch_fill_f32(float* p, size_t n, float v, hipStream_t st);
 // Zero rows [r0, r1) of each of the 59 rows of a SoA moment buffer.
 void launch_zero_rows(float* M, size_t cap, size_t r0, size_t r1, hipStream_t st);

@@ -116,6 +116,7 @@ SIGNATURES = {
     "rgs_debug_tiles": [_vp, _vp, _sz, _vp, _sz, ctypes.POINTER(ctypes.c_uint64)],
     "rgs_set_grad_capture": [_vp, _i],
     "rgs_raster_work_counts": [_vp, _vp],
+    "rgs_hbm_copy_probe": [_vp, ctypes.c_size_t, _i, ctypes.POINTER(ctypes.c_float)],
     "rgs_set_split_scale_divisor": [_vp, _f],
     "rgs_loss_grad_dev_noalloc": [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32, _f, _vp, _vp],
     "rgs_step_grads": [_vp, _vp, _sz],
@@ -556,6 +557,13 @@ class Context:
         q = ctypes.c_int()
         _check(lib().rgs_get_raster_queue(self.h, ctypes.byref(q)))
         return q.value
+
+    def hbm_copy_probe(self, nbytes: int = 1 << 30, reps: int = 10) -> float:
+        """GB/s (read + written) of a plain streaming device copy of nbytes:
+        the practical ceiling of the memory-bound kernels."""
+        g = ctypes.c_float()
+        _check(lib().rgs_hbm_copy_probe(self.h, int(nbytes), int(reps), ctypes.byref(g)))
+        return g.value
 
     def autotune_raster(self, view_ids, bg=(0.0, 0.0, 0.0), lambda_dssim=0.2) -> dict:
         """Times the raster under candidate work decompositions on the

@@ -729,6 +729,18 @@ def test_raster_params_and_autotune(ctx):
     c.close()
 
 
+def test_hbm_copy_probe(ctx):
+    """The streaming-copy probe behind the memory-bound stages' frac_of_copy:
+    a plausible rate (below the 8 TB/s peak, with margin for the clock) and
+    its argument checks."""
+    from rgsplat_amd import runtime
+    gbs = ctx.hbm_copy_probe(64 << 20, 3)
+    assert 100.0 < gbs < 9000.0, gbs
+    for nbytes, reps in [(8, 3), (1 << 20, 0)]:
+        with pytest.raises(runtime.RgsError):
+            ctx.hbm_copy_probe(nbytes, reps)
+
+
 def test_create_on_a_missing_device():
     """rgs_create on a device id the machine does not have: RGS_E_ARG and a
     message, no context; the current device (-1) and device 0 still work."""
