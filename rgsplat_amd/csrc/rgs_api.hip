@@ -2603,6 +2603,7 @@ int rgs_hbm_copy_probe(rgs_ctx* c, size_t bytes, int reps, float* gbs) {
   if (!c || !gbs || reps <= 0) return fail(RGS_E_ARG, "null argument or no repetitions");
   const size_t n16 = bytes / 16;
   if (n16 == 0) return fail(RGS_E_ARG, "probe size below 16 bytes");
+  if (n16 > ((size_t)1 << 36)) return fail(RGS_E_ARG, "probe size above 1 TiB");
   if (!c->st) return fail(RGS_E_ARG, "the probe needs a non-null stream");
   (void)hipSetDevice(c->device);
   void *a = nullptr, *b = nullptr;
