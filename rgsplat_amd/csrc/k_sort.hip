@@ -561,7 +561,9 @@ __device__ __forceinline__ uint64_t look_back(uint64_t* tw, int tile, uint32_t* 
 
 // The entries' counts and what is written for them. F::Item holds what an
 // entry's output needs; F::count gives its count; F::put writes entry e
-// given the exclusive prefix of the counts; F::last gets the total. TILE:
+// given the exclusive prefix of the counts; F::last gets the total. With
+// F::kTally, F::tally(item) is also summed over the entries into
+// *F::tally_out (one atomic per workgroup). TILE:
 // entries per workgroup (a multiple of SCAN_THREADS; fewer, larger tiles
 // shorten the look-back, smaller ones need fewer registers).
 template <class F, int TILE>
@@ -649,6 +651,22 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_lookback(F f, size_t n_ma
     const size_t e = base + (size_t)k * SCAN_THREADS + threadIdx.x;
     if (e < n) f.put(e, it[k], prefix + rank[k]);
   }
+  if constexpr (F::kTally) {
+    uint32_t t = 0;
+#pragma unroll
+    for (int k = 0; k < LB_ITEMS; ++k) t += f.tally(it[k]);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) t += __shfl_xor(t, o, 64);
+    __shared__ uint32_t s_tally[SCAN_THREADS / 64];
+    if ((threadIdx.x & 63) == 0) s_tally[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t sum = 0;
+#pragma unroll
+      for (int w = 0; w < SCAN_THREADS / 64; ++w) sum += s_tally[w];
+      if (sum) atomicAdd(f.tally_out, sum);
+    }
+  }
   if (threadIdx.x == 0 && tile == needed - 1) {
     const uint64_t t = prefix + tsum;
     *total = t >= (uint64_t)SCAN_SATURATED ? SCAN_SATURATED : (uint32_t)t;
@@ -659,8 +677,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_lookback(F f, size_t n_ma
 // The depth-cut partition: entry r (key, value) goes to the front arrays at
 // its exclusive count of front entries, or (with back arrays) to the back
 // arrays at r minus that count; the totals and the cut switch go to n_back
-// and cut_on.
+// and cut_on, the number of visible entries behind the cut (culled entries
+// carry the key 0xffffffff) to tally_out.
 struct PartitionAtCut {
+  static constexpr bool kTally = true;
   const uint32_t* keys;
   const uint32_t* vals;
   const CutState* cs;
@@ -668,12 +688,14 @@ struct PartitionAtCut {
   size_t n;
   uint32_t* n_back;
   uint32_t* cut_on;
+  uint32_t* tally_out;
   struct Item {
     uint32_t k, v;
   };
   __device__ Item item(size_t r) const { return {keys[r], vals[r]}; }
   __device__ static Item empty() { return {0xffffffffu, 0u}; }  // at or past any cut
   __device__ uint32_t count(const Item& it) const { return it.k < cs->key ? 1u : 0u; }
+  __device__ uint32_t tally(const Item& it) const { return (it.k >= cs->key && it.k != 0xffffffffu) ? 1u : 0u; }
   __device__ void put(size_t r, const Item& it, uint64_t o) const {
     if (it.k < cs->key) {
       kf[o] = it.k;
@@ -691,6 +713,7 @@ struct PartitionAtCut {
 // Pair offsets: the exclusive scan of cnt[vals[r]] over the first *n_dev
 // sorted entries.
 struct PairOffsets {
+  static constexpr bool kTally = false;
   const uint32_t* vals;
   const uint32_t* cnt;
   uint32_t* out;
@@ -702,16 +725,18 @@ struct PairOffsets {
   __device__ uint32_t count(const Item& it) const { return it.c; }
   __device__ void put(size_t r, const Item&, uint64_t o) const { out[r] = (uint32_t)o; }
   __device__ void last(uint64_t) const {}
+  __device__ uint32_t tally(const Item&) const { return 0u; }
+  uint32_t* tally_out = nullptr;
 };
 
 void partition_at_cut(const uint32_t* keys, const uint32_t* vals, size_t n, const CutState* cs, uint32_t* kf,
                       uint32_t* vf, uint32_t* kb, uint32_t* vb, uint32_t* n_front, uint32_t* n_back, uint32_t* cut_on,
-                      uint64_t* state, uint32_t* status, hipStream_t st) {
+                      uint32_t* n_back_visible, uint64_t* state, uint32_t* status, hipStream_t st) {
   // 4096-entry tiles: every tile is live here, and fewer of them shorten
   // the look-back
   constexpr int TILE = 2 * LB_TILE;
   const int nb = std::max((int)((n + TILE - 1) / TILE), 1);
-  PartitionAtCut f{keys, vals, cs, kf, vf, kb, vb, n, n_back, cut_on};
+  PartitionAtCut f{keys, vals, cs, kf, vf, kb, vb, n, n_back, cut_on, n_back_visible};
   hipLaunchKernelGGL((k_scan_lookback<PartitionAtCut, TILE>), dim3(nb), dim3(SCAN_THREADS), 0, st, f, n, nullptr, state,
                      n_front, status);
 }
@@ -719,7 +744,7 @@ void partition_at_cut(const uint32_t* keys, const uint32_t* vals, size_t n, cons
 void scan_pair_offsets(const uint32_t* vals, const uint32_t* cnt, size_t n_max, const uint32_t* n_dev,
                        uint32_t* out, uint32_t* total, uint64_t* state, uint32_t* status, hipStream_t st) {
   const int nb = std::max(lookback_tiles(n_max), 1);
-  PairOffsets f{vals, cnt, out};
+  PairOffsets f{vals, cnt, out, nullptr};
   hipLaunchKernelGGL((k_scan_lookback<PairOffsets, LB_TILE>), dim3(nb), dim3(SCAN_THREADS), 0, st, f, n_max, n_dev, state,
                      total, status);
 }

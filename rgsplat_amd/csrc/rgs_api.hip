@@ -217,7 +217,8 @@ struct ViewSet {
 // the back entries that meet a flagged bin.
 enum {
   SC_DEPTH_N = 0, SC_TOTAL = 1, SC_K = 2, SC_STATUS = 3, SC_NOUT = 4,
-  SC_NF = 5, SC_NB = 6, SC_CUTON = 7, SC_BTOTAL = 8, SC_KB = 9, SC_NFLAG = 10, SC_NC = 11, SC_COUNT = 16
+  SC_NF = 5, SC_NB = 6, SC_CUTON = 7, SC_BTOTAL = 8, SC_KB = 9, SC_NFLAG = 10, SC_NC = 11, SC_NBV = 12,
+  SC_COUNT = 16
 };
 
 }  // namespace
@@ -300,6 +301,11 @@ struct rgs_ctx {
   uint64_t gen = 0;        // bumped on every device (re)allocation
   uint64_t graph_gen[2] = {0, 0};  // gen at capture time
   bool need_back = true;   // the next step runs the back-list stage
+  // adaptive cut paused (steps left; the device runs without a cut), and
+  // steps that run the back-list stage whatever the last one flagged (after
+  // a pause, so the first step with the cut again is not voided)
+  int cut_paused = 0;
+  int force_back = 0;
   uint64_t back_reruns = 0;  // steps re-run because a bin needed its back list
   DBuf api_acc9;           // raster accumulators of the reference-API paths
   DBuf tile_order[2];      // raster schedule per bin shift: slot -> global bin id
@@ -537,8 +543,8 @@ void enqueue_project_cut(rgs_ctx* c, const float* P, size_t cap, int n, const Vi
   stage_begin(c, S_DSORT);
   partition_at_cut(P_<uint32_t>(c->dk[0]), P_<uint32_t>(c->dv[0]), dn, cs, P_<uint32_t>(c->dk[1]),
                    P_<uint32_t>(c->dv[1]), with_back ? P_<uint32_t>(c->kB) : nullptr,
-                   with_back ? P_<uint32_t>(c->vB) : nullptr, sc + SC_NF, sc + SC_NB, sc + SC_CUTON, lb_state(c, 0),
-                   sc + SC_STATUS, st);
+                   with_back ? P_<uint32_t>(c->vB) : nullptr, sc + SC_NF, sc + SC_NB, sc + SC_CUTON, sc + SC_NBV,
+                   lb_state(c, 0), sc + SC_STATUS, st);
   uint32_t* keys[2] = {P_<uint32_t>(c->dk[1]), P_<uint32_t>(c->dk[0])};
   uint32_t* vals[2] = {P_<uint32_t>(c->dv[1]), P_<uint32_t>(c->dv[0])};
   int cur = radix_sort_pairs(keys, vals, sc + SC_NF, dn, 32, rs, st);
@@ -602,7 +608,7 @@ void enqueue_bin_and_raster(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, co
 
 // (Re)writes the persistent cut state from the host settings.
 int write_cut_state(rgs_ctx* c) {
-  CutState cs{0xffffffffu, 0u, (uint32_t)c->cut_mode, 0u};
+  CutState cs{0xffffffffu, 0u, (uint32_t)(c->cut_paused > 0 ? 1 : c->cut_mode), 0u};
   if (c->cut_mode == 2) cs.key = f2u(c->cut_depth);
   HIPCHK(hipMemcpyAsync(c->cutst.p, &cs, sizeof(cs), hipMemcpyHostToDevice, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
@@ -2113,6 +2119,32 @@ static void enqueue_step_rest(rgs_ctx* c, const ViewSet& vs, const float bg[3], 
   stage_end(c, S_ADAM);
 }
 
+// The adaptive cut pays for itself by sorting and binning only the entries
+// in front of it; when some tiles never saturate, their bins need the
+// back-list stage every step (about 30 launches, ~0.13 ms). After a step
+// that needed it while the cut left fewer than an eighth as many visible
+// entries behind it as in front (little saved), the cut pauses for CUT_PAUSE steps
+// (whole lists, no back-list stage), then resumes with the back-list stage
+// on its first two steps (no step voided for want of it).
+constexpr int CUT_PAUSE = 256;
+static int adapt_cut(rgs_ctx* c) {
+  if (c->cut_mode != 0) return 0;
+  if (c->cut_paused > 0) {
+    if (--c->cut_paused == 0) {
+      CHK(write_cut_state(c));
+      c->force_back = 2;
+    }
+    return 0;
+  }
+  const uint64_t nf = c->h_scalars[SC_NF], nbv = c->h_scalars[SC_NBV];
+  if (c->h_scalars[SC_NFLAG] != 0 && nbv * 8 <= nf) {
+    c->cut_paused = CUT_PAUSE;
+    c->need_back = false;
+    CHK(write_cut_state(c));
+  }
+  return 0;
+}
+
 // Enqueues one whole training step on the context's stream (no host sync).
 static void enqueue_step(rgs_ctx* c, const ViewSet& vs, const float bg[3], float lambda, bool with_back) {
   launch_fill_u32(P_<uint32_t>(c->scalars), step_scalar_words(c), 0u, c->st);
@@ -2186,7 +2218,7 @@ static int train_step_impl(rgs_ctx* c, const int* ids, int n_ids, const rgs_trai
     key += "|gcap";
   }
   for (int attempt = 0; attempt < 4; ++attempt) {
-    const int var = c->need_back ? 1 : 0;
+    const int var = (c->need_back || c->force_back > 0) ? 1 : 0;
     bool replay = use_graph && c->gexec[var] && c->gkey[var] == key && c->graph_gen[var] == c->gen;
     bool launched_graph = replay;
     if (!replay) {
@@ -2271,6 +2303,8 @@ static int train_step_impl(rgs_ctx* c, const int* ids, int n_ids, const rgs_trai
       continue;
     }
     c->need_back = c->h_scalars[SC_NFLAG] != 0;
+    if (c->force_back > 0) --c->force_back;
+    CHK(adapt_cut(c));
     c->last_back_valid = var == 1;
     c->last_front_pairs = c->h_scalars[SC_K];
     c->last_back_pairs = c->h_scalars[SC_KB];
@@ -2552,6 +2586,8 @@ int rgs_set_depth_cut(rgs_ctx* c, int mode, float depth) {
   (void)hipSetDevice(c->device);
   c->cut_mode = mode;
   c->cut_depth = depth;
+  c->cut_paused = 0;
+  c->force_back = 0;
   if (c->cutst.p) CHK(write_cut_state(c));
   return 0;
 }

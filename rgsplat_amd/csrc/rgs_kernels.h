@@ -147,10 +147,11 @@ inline size_t lookback_words(size_t n) { return (size_t)LB_HEAD + (size_t)std::m
 // Stable partition of the projected (key, value) entries at the cut into
 // front (kf, vf: *n_front entries, keys[r] < cs->key) and back (kb, vb:
 // *n_back = n - *n_front entries; may be null: nothing written) arrays;
-// *cut_on = 1 when a cut is set (a front list may then be partial).
+// *cut_on = 1 when a cut is set (a front list may then be partial); the
+// back entries that are not culled are added to *n_back_visible.
 void partition_at_cut(const uint32_t* keys, const uint32_t* vals, size_t n, const CutState* cs, uint32_t* kf,
                       uint32_t* vf, uint32_t* kb, uint32_t* vb, uint32_t* n_front, uint32_t* n_back, uint32_t* cut_on,
-                      uint64_t* state, uint32_t* status, hipStream_t st);
+                      uint32_t* n_back_visible, uint64_t* state, uint32_t* status, hipStream_t st);
 // Exclusive scan of cnt[vals[r]] over the first *n_dev of n_max entries
 // (pair offsets in depth order); the total goes to *total.
 void scan_pair_offsets(const uint32_t* vals, const uint32_t* cnt, size_t n_max, const uint32_t* n_dev,

@@ -519,6 +519,43 @@ def test_depth_cut_rerun(ctx, use_graph):
     assert np.linalg.norm(ma - mw) <= 1e-5 * np.linalg.norm(mw)
 
 
+def test_adaptive_cut_pause():
+    """The single-view 100k-Gaussian workload at 800x800 leaves tiles that
+    never saturate, so the adaptive cut needs its back-list stage on every
+    step while leaving few entries behind it: after the first such step the
+    cut pauses (whole lists, no back lists, nothing flagged) for 256 steps,
+    then resumes with the back-list stage on its first steps, so no step is
+    re-run. Losses equal a whole-list run's throughout (frozen scene)."""
+    from rgsplat_amd import runtime
+    g, cams, targets = workload("c1_100k_800")
+    cfg = S.benchmark_train_config()
+    frozen = dict(cfg, lr_position=0.0, lr_rotation=0.0, lr_scale=0.0, lr_opacity=0.0, lr_sh_dc=0.0,
+                  lr_sh_rest=0.0)
+    runs = {}
+    for mode in (0, 1):
+        c = runtime.Context(0)
+        c.set_depth_cut(mode)
+        c.scene_upload(g)
+        c.set_views(cams, targets)
+        rec = []
+        for k in range(261):
+            loss = c.train_step([0], frozen, use_graph=True)
+            if k in (0, 1, 2, 3, 257, 258, 259, 260):
+                rec.append((k, loss, c.cut_stats()))
+        runs[mode] = rec
+        c.close()
+    st = {k: s for k, _, s in runs[0]}
+    assert st[1]["flagged_tiles"] > 0  # the cut's first step needed back lists
+    for k in (2, 3, 257, 260):  # paused (257: the pause ends after it)
+        assert st[k]["flagged_tiles"] == 0 and st[k]["back_pairs"] == 0 and not np.isfinite(st[k]["next_cut"]), k
+    # resumed: no cut on step 258, its cut on 259, which ran with the back
+    # lists (no re-run) and paused the cut again
+    assert st[258]["flagged_tiles"] == 0
+    assert st[259]["flagged_tiles"] > 0 and st[259]["reruns"] == st[1]["reruns"]
+    for (k, la, _), (_, lw, _) in zip(runs[0], runs[1]):
+        assert np.array_equal(la, lw), k
+
+
 def test_graph_replay_matches_eager(ctx):
     from rgsplat_amd import runtime
     g, cams, targets = workload("c0_10k_256", 5000, [(128, 96), (160, 128)])
