@@ -758,6 +758,7 @@ void scan_pair_offsets(const uint32_t* vals, const uint32_t* cnt, size_t n_max, 
 // LDS on the way. first_splat[b] (the splat holding pair b*RS_TILE) comes
 // from k_block_first.
 constexpr int GP_THREADS = 256;
+constexpr int GP_PER = RS_TILE / GP_THREADS;  // window slots scanned per thread
 
 __global__ __launch_bounds__(256) void k_block_first(const uint32_t* __restrict__ offsets, size_t n_sorted,
                                                      const uint32_t* __restrict__ total, size_t k_cap,
@@ -790,6 +791,9 @@ __global__ __launch_bounds__(GP_THREADS) void k_gen_pairs(const uint32_t* __rest
   __shared__ uint32_t s_w[GP_THREADS];    // rect width in tiles
   __shared__ uint32_t s_tb[GP_THREADS];   // bin base of the splat's view
   __shared__ uint32_t s_ntx[GP_THREADS];
+  __shared__ float s_rw[GP_THREADS];       // 1 / rect width
+  __shared__ uint32_t s_map[RS_TILE];     // window slot -> splat of the pair
+  __shared__ uint32_t s_wmax[GP_THREADS / 64];
   __shared__ uint32_t s_h[256];
   const int t = threadIdx.x;
   const uint32_t tot = *total;
@@ -817,6 +821,7 @@ __global__ __launch_bounds__(GP_THREADS) void k_gen_pairs(const uint32_t* __rest
       const uint2 rr = bin_rect(rect[vi], cm.bin_shift);
       s_org[t] = rr.x;
       s_w[t] = cnt[vi] ? (rr.y & 0xffffu) - (rr.x & 0xffffu) : 1u;
+      s_rw[t] = 1.0f / (float)s_w[t];
       s_tb[t] = (uint32_t)cm.bin_base;
       s_ntx[t] = (uint32_t)cm.nbx;
     } else {
@@ -827,22 +832,65 @@ __global__ __launch_bounds__(GP_THREADS) void k_gen_pairs(const uint32_t* __rest
     const size_t c_lo = (size_t)s_off[0] > p_lo ? (size_t)s_off[0] : p_lo;
     const size_t c_end = (size_t)s_off[GP_THREADS];
     const size_t c_hi = c_end < p_hi ? c_end : p_hi;
-    for (size_t p = c_lo + t; p < c_hi; p += GP_THREADS) {
-      // largest j with s_off[j] <= p (skips splats with no tiles)
-      int lo = 0, hi = GP_THREADS - 1;
-      while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if ((size_t)s_off[mid] <= p) lo = mid;
-        else hi = mid - 1;
+    // the window's splat of every pair in [c_lo, c_hi) (at most RS_TILE):
+    // each splat marks the slot of its first pair there (the splat holding
+    // c_lo, the last to start at or before it, marks slot 0; of the splats
+    // that start at one offset the last, which has the pairs, wins), and an
+    // inclusive max-scan carries the marks over the following slots
+    const uint32_t m = c_hi > c_lo ? (uint32_t)(c_hi - c_lo) : 0u;
+    for (uint32_t k = (uint32_t)t; k < m; k += GP_THREADS) s_map[k] = 0u;
+    __syncthreads();
+    if ((size_t)s_off[t] < c_hi) {
+      const size_t o = (size_t)s_off[t];
+      atomicMax(&s_map[o > c_lo ? (uint32_t)(o - c_lo) : 0u], (uint32_t)t);
+    }
+    __syncthreads();
+    {
+      // thread t scans slots [GP_PER t, GP_PER (t + 1)) in registers, then
+      // the threads' maxima are scanned across the block
+      uint32_t v[GP_PER], run = 0;
+#pragma unroll
+      for (int k = 0; k < GP_PER; ++k) {
+        const uint32_t slot = (uint32_t)t * GP_PER + k;
+        v[k] = slot < m ? s_map[slot] : 0u;
+        run = max(run, v[k]);
+        v[k] = run;
       }
-      const uint32_t local = (uint32_t)(p - s_off[lo]);
-      const uint32_t w = s_w[lo];
-      const uint32_t org = s_org[lo];
-      const uint32_t ty = (org >> 16) + local / w;
-      const uint32_t tx = (org & 0xffffu) + local % w;
-      const uint32_t key = s_tb[lo] + ty * s_ntx[lo] + tx;
+      uint32_t x = run;
+      const int lane = t & 63;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, d, 64);
+        if (lane >= d) x = max(x, y);
+      }
+      if (lane == 63) s_wmax[t >> 6] = x;
+      __syncthreads();
+      uint32_t before = (uint32_t)__shfl_up((int)x, 1, 64);
+      if (lane == 0) before = 0u;
+      for (int w = 0; w < (t >> 6); ++w) before = max(before, s_wmax[w]);
+#pragma unroll
+      for (int k = 0; k < GP_PER; ++k) {
+        const uint32_t slot = (uint32_t)t * GP_PER + k;
+        if (slot < m) s_map[slot] = max(before, v[k]);
+      }
+    }
+    __syncthreads();
+    for (uint32_t k = (uint32_t)t; k < m; k += GP_THREADS) {
+      const size_t p = c_lo + k;
+      const uint32_t j = s_map[k];
+      const uint32_t local = (uint32_t)(p - s_off[j]);
+      const uint32_t w = s_w[j];
+      // local / w through the float reciprocal, corrected to the exact
+      // quotient (local < 2^24: a bin rect holds fewer pairs)
+      uint32_t q = (uint32_t)((float)local * s_rw[j]);
+      if (q * w > local) --q;
+      else if ((q + 1) * w <= local) ++q;
+      const uint32_t org = s_org[j];
+      const uint32_t ty = (org >> 16) + q;
+      const uint32_t tx = (org & 0xffffu) + (local - q * w);
+      const uint32_t key = s_tb[j] + ty * s_ntx[j] + tx;
       pair_keys[p] = key;
-      pair_vals[p] = s_vi[lo];
+      pair_vals[p] = s_vi[j];
       atomicAdd(&s_h[key & 255u], 1u);
     }
     if (c_end >= p_hi) break;  // uniform: every thread read the same s_off
