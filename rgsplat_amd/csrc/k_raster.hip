@@ -139,10 +139,16 @@ __device__ __forceinline__ void alpha_fast(float qf, float op, const uint64_t* s
 }
 // f32::min(alpha_raw, 0.99) (NaN -> 0.99); *clamped is set when the clamp
 // applied, which is also when the reference zeroes dL/d(alpha_raw).
+// With the flag wanted (the backward), the clamp is that flag's select on
+// the bit patterns, one instruction; a float select would be matched to a
+// minimum and cost a canonicalising maximum besides.
 __device__ __forceinline__ float clamp_alpha(float alpha_raw, bool* clamped = nullptr) {
   const bool c = !(alpha_raw <= MAX_ALPHA);
-  if (clamped) *clamped = c;
-  return c ? MAX_ALPHA : alpha_raw;
+  if (!clamped) return c ? MAX_ALPHA : alpha_raw;
+  *clamped = c;
+  uint32_t bits = __float_as_uint(alpha_raw);
+  asm volatile("" : "+v"(bits));
+  return __uint_as_float(c ? __float_as_uint(MAX_ALPHA) : bits);
 }
 
 // Forward: four pixels per lane (pixel group q = tile rows 4q..4q+3, lane l
