@@ -50,7 +50,7 @@ __device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* s_
 
 __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t* __restrict__ keys,
                                                          const uint32_t* __restrict__ n_dev, int shift,
-                                                         uint32_t* __restrict__ hist, int nb_max) {
+                                                         uint32_t mask, uint32_t* __restrict__ hist, int nb_max) {
   __shared__ uint32_t h[256];
   uint32_t n = *n_dev;
   const uint32_t b = xcd_chunked(blockIdx.x);
@@ -62,7 +62,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t* __res
 #pragma unroll 4
   for (int k = 0; k < RS_ITEMS; ++k) {
     size_t e = base + (size_t)k * RS_THREADS + threadIdx.x;
-    if (e < n) atomicAdd(&h[(keys[e] >> shift) & 255u], 1u);
+    if (e < n) atomicAdd(&h[(keys[e] >> shift) & mask], 1u);
   }
   __syncthreads();
   hist[(size_t)threadIdx.x * nb_max + b] = h[threadIdx.x];
@@ -73,10 +73,16 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t* __res
 // loads; rows are 16-byte aligned as nb_max is a multiple of 256).
 constexpr int RSCAN_THREADS = 1024;
 constexpr int RSCAN_ITEMS = 8;
+// Rows of digits the pass does not have (d >= n_digits) hold zeros: their
+// scan is zeros too and is skipped.
 __global__ __launch_bounds__(RSCAN_THREADS) void k_radix_rowscan(uint32_t* __restrict__ hist,
                                                                const uint32_t* __restrict__ n_dev, int nb_max,
-                                                               uint32_t* __restrict__ row_total) {
+                                                               int n_digits, uint32_t* __restrict__ row_total) {
   __shared__ uint32_t s_w[RSCAN_THREADS / 64];
+  if ((int)blockIdx.x >= n_digits) {
+    if (threadIdx.x == 0) row_total[blockIdx.x] = 0u;
+    return;
+  }
   const uint32_t n = *n_dev;
   const uint32_t nb = (n + RS_TILE - 1) / RS_TILE;
   uint32_t* row = hist + (size_t)blockIdx.x * nb_max;
@@ -134,17 +140,22 @@ __global__ __launch_bounds__(RSCAN_THREADS) void k_radix_rowscan(uint32_t* __res
   if (threadIdx.x == 0) row_total[blockIdx.x] = carry;
 }
 
-// Stable scatter of one radix tile. Keys are ranked with wave64 ballot
-// matching, the tile is staged in LDS in digit order and each digit's run is
-// written contiguously (keys first, then the values through the same LDS).
+// Stable scatter of one radix tile over the digit of `dbits` (<= 8) bits at
+// `shift`. Keys are ranked with wave64 ballot matching, the tile is staged
+// in LDS in digit order and each digit's run is written contiguously (keys
+// first, then the values through the same LDS).
 // LAST: the final pass of the tile sort; keys are not written, and the first
 // and last position of every key segment inside the tile go to the ranges
 // (atomicMin / atomicMax), which yields each key's global [start, end).
-template <bool LAST>
+// D8: the digit is 8 bits wide (dbits is ignored; the matching loop is
+// fully unrolled).
+template <bool LAST, bool D8>
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ keys_out,
-    uint32_t* __restrict__ vals_out, const uint32_t* __restrict__ n_dev, int shift,
+    uint32_t* __restrict__ vals_out, const uint32_t* __restrict__ n_dev, int shift, int dbits,
     const uint32_t* __restrict__ hist, const uint32_t* __restrict__ row_total, int nb_max, TileRanges tr) {
+  if (D8) dbits = 8;
+  const uint32_t mask = (1u << dbits) - 1u;
   __shared__ uint32_t s_cnt[RS_WAVES][256];
   __shared__ uint32_t s_blk_start[256];
   __shared__ uint32_t s_glob[256];
@@ -184,10 +195,11 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
 #pragma unroll
   for (int r = 0; r < RS_ITEMS; ++r) {
     bool valid = (uint32_t)r * 64 + lane < w_n;
-    uint32_t d = (key[r] >> shift) & 255u;
+    uint32_t d = (key[r] >> shift) & mask;
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int bit = 0; bit < 8; ++bit) {
+      if (!D8 && bit >= dbits) break;  // uniform
       uint64_t bb = __ballot((d >> bit) & 1u);
       peers &= ((d >> bit) & 1u) ? bb : ~bb;
     }
@@ -226,7 +238,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     // across the scan (an empty asm makes the key opaque to CSE)
     asm volatile("" : "+v"(key[r]));
     if ((uint32_t)r * 64 + lane < w_n) {
-      uint32_t d = (key[r] >> shift) & 255u;
+      uint32_t d = (key[r] >> shift) & mask;
       pos[r] += s_cnt[w][d];
       s_stage[pos[r]] = key[r];
     }
@@ -237,7 +249,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     uint32_t p = (uint32_t)r * RS_THREADS + t;
     if (p < blk_n) {
       uint32_t k = s_stage[p];
-      uint32_t d = (k >> shift) & 255u;
+      uint32_t d = (k >> shift) & mask;
       s_dig[p] = (uint8_t)d;
       uint32_t o = s_glob[d] + (p - s_blk_start[d]);
       if (!LAST) {
@@ -275,20 +287,28 @@ static int radix_sort_impl(uint32_t* keys[2], uint32_t* vals[2], const uint32_t*
     launch_fill_u32(tr->rstart, tr->n_keys, 0xffffffffu, st);
     launch_fill_u32(tr->rend, tr->n_keys, 0u, st);
   }
-  for (int shift = 0; shift < bits; shift += 8) {
-    const bool last = shift + 8 >= bits;
+  const int db = radix_digit_bits(bits);
+  for (int shift = 0; shift < bits; shift += db) {
+    const bool last = shift + db >= bits;
+    const int w = std::min(db, bits - shift);
     if (!(shift == 0 && first_hist_done))
-      hipLaunchKernelGGL(k_radix_hist, dim3(grid), dim3(RS_THREADS), 0, st, keys[cur], n_dev, shift, rs.hist,
-                         rs.nb_max);
-    hipLaunchKernelGGL(k_radix_rowscan, dim3(256), dim3(RSCAN_THREADS), 0, st, rs.hist, n_dev, rs.nb_max,
+      hipLaunchKernelGGL(k_radix_hist, dim3(grid), dim3(RS_THREADS), 0, st, keys[cur], n_dev, shift,
+                         (1u << w) - 1u, rs.hist, rs.nb_max);
+    hipLaunchKernelGGL(k_radix_rowscan, dim3(256), dim3(RSCAN_THREADS), 0, st, rs.hist, n_dev, rs.nb_max, 1 << w,
                        rs.row_total);
-    if (last && tr)
-      hipLaunchKernelGGL(k_radix_scatter<true>, dim3(grid), dim3(RS_THREADS), 0, st, keys[cur], vals[cur],
-                         keys[cur ^ 1], vals[cur ^ 1], n_dev, shift, rs.hist, rs.row_total, rs.nb_max, *tr);
+    const TileRanges none{nullptr, nullptr, 0u};
+    if (last && tr && w == 8)
+      hipLaunchKernelGGL((k_radix_scatter<true, true>), dim3(grid), dim3(RS_THREADS), 0, st, keys[cur], vals[cur],
+                         keys[cur ^ 1], vals[cur ^ 1], n_dev, shift, w, rs.hist, rs.row_total, rs.nb_max, *tr);
+    else if (last && tr)
+      hipLaunchKernelGGL((k_radix_scatter<true, false>), dim3(grid), dim3(RS_THREADS), 0, st, keys[cur], vals[cur],
+                         keys[cur ^ 1], vals[cur ^ 1], n_dev, shift, w, rs.hist, rs.row_total, rs.nb_max, *tr);
+    else if (w == 8)
+      hipLaunchKernelGGL((k_radix_scatter<false, true>), dim3(grid), dim3(RS_THREADS), 0, st, keys[cur], vals[cur],
+                         keys[cur ^ 1], vals[cur ^ 1], n_dev, shift, w, rs.hist, rs.row_total, rs.nb_max, none);
     else
-      hipLaunchKernelGGL(k_radix_scatter<false>, dim3(grid), dim3(RS_THREADS), 0, st, keys[cur], vals[cur],
-                         keys[cur ^ 1], vals[cur ^ 1], n_dev, shift, rs.hist, rs.row_total, rs.nb_max,
-                         TileRanges{nullptr, nullptr, 0u});
+      hipLaunchKernelGGL((k_radix_scatter<false, false>), dim3(grid), dim3(RS_THREADS), 0, st, keys[cur], vals[cur],
+                         keys[cur ^ 1], vals[cur ^ 1], n_dev, shift, w, rs.hist, rs.row_total, rs.nb_max, none);
     cur ^= 1;
   }
   return cur;
@@ -784,7 +804,7 @@ __global__ __launch_bounds__(GP_THREADS) void k_gen_pairs(const uint32_t* __rest
                                                           uint32_t* __restrict__ k_out, uint32_t* __restrict__ status,
                                                           const uint32_t* __restrict__ first_splat,
                                                           uint32_t* __restrict__ hist, int nb_max,
-                                                          const uint32_t* __restrict__ n_dev) {
+                                                          const uint32_t* __restrict__ n_dev, uint32_t hmask) {
   __shared__ uint32_t s_off[GP_THREADS + 1];
   __shared__ uint32_t s_vi[GP_THREADS];
   __shared__ uint32_t s_org[GP_THREADS];  // tx0 | ty0 << 16
@@ -891,7 +911,7 @@ __global__ __launch_bounds__(GP_THREADS) void k_gen_pairs(const uint32_t* __rest
       const uint32_t key = s_tb[j] + ty * s_ntx[j] + tx;
       pair_keys[p] = key;
       pair_vals[p] = s_vi[j];
-      atomicAdd(&s_h[key & 255u], 1u);
+      atomicAdd(&s_h[key & hmask], 1u);
     }
     if (c_end >= p_hi) break;  // uniform: every thread read the same s_off
     rc += GP_THREADS;
@@ -903,7 +923,7 @@ __global__ __launch_bounds__(GP_THREADS) void k_gen_pairs(const uint32_t* __rest
 void launch_gen_pairs(const uint32_t* sorted_vals, size_t n_sorted, const uint32_t* offsets, const uint32_t* total,
                       const SplatArrays S, int cap, const CamDev* cams, uint32_t* pair_keys, uint32_t* pair_vals,
                       size_t k_cap, uint32_t* k_out, uint32_t* status, uint32_t* first_splat, RadixScratch rs,
-                      hipStream_t st, const uint32_t* n_dev) {
+                      int key_bits, hipStream_t st, const uint32_t* n_dev) {
   if (n_sorted)
     hipLaunchKernelGGL(k_block_first, dim3((unsigned)((n_sorted + 255) / 256)), dim3(256), 0, st, offsets, n_sorted,
                        total, k_cap, first_splat);
@@ -911,7 +931,7 @@ void launch_gen_pairs(const uint32_t* sorted_vals, size_t n_sorted, const uint32
   if (nb == 0) nb = 1;
   hipLaunchKernelGGL(k_gen_pairs, dim3(xcd_grid(nb)), dim3(GP_THREADS), 0, st, sorted_vals, n_sorted, offsets, total,
                      S.cnt, S.rect, cap, cams, pair_keys, pair_vals, k_cap, k_out, status, first_splat, rs.hist,
-                     rs.nb_max, n_dev);
+                     rs.nb_max, n_dev, (1u << radix_digit_bits(key_bits)) - 1u);
 }
 
 // ------------------------------------------------------- depth-cut binning

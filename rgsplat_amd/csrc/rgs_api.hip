@@ -589,7 +589,7 @@ void enqueue_bin_and_raster(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, co
   stage_begin(c, S_GEN);
   launch_gen_pairs(c->sorted_vals, dn, P_<uint32_t>(c->offsets), sc + SC_TOTAL, splats(c), (int)cap,
                    P_<CamDev>(c->cams_d), P_<uint32_t>(c->pk[0]), P_<uint32_t>(c->pv[0]), c->k_cap, sc + SC_K,
-                   sc + SC_STATUS, P_<uint32_t>(c->first_splat), rs, st);
+                   sc + SC_STATUS, P_<uint32_t>(c->first_splat), rs, vs.bits(), st);
   stage_end(c, S_GEN);
   uint32_t* keys[2] = {P_<uint32_t>(c->pk[0]), P_<uint32_t>(c->pk[1])};
   uint32_t* vals[2] = {P_<uint32_t>(c->pv[0]), P_<uint32_t>(c->pv[1])};
@@ -631,7 +631,7 @@ void enqueue_bin_and_raster_cut(rgs_ctx* c, size_t cap, int n, const ViewSet& vs
   stage_begin(c, S_GEN);
   launch_gen_pairs(c->sorted_vals, dn, P_<uint32_t>(c->offsets), sc + SC_TOTAL, splats(c), (int)cap,
                    P_<CamDev>(c->cams_d), P_<uint32_t>(c->pk[0]), P_<uint32_t>(c->pv[0]), c->k_cap, sc + SC_K,
-                   sc + SC_STATUS, P_<uint32_t>(c->first_splat), rs, st, sc + SC_NF);
+                   sc + SC_STATUS, P_<uint32_t>(c->first_splat), rs, vs.bits(), st, sc + SC_NF);
   stage_end(c, S_GEN);
   uint32_t* keys[2] = {P_<uint32_t>(c->pk[0]), P_<uint32_t>(c->pk[1])};
   uint32_t* vals[2] = {P_<uint32_t>(c->pv[0]), P_<uint32_t>(c->pv[1])};

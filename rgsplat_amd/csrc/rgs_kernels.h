@@ -80,6 +80,13 @@ struct TileRanges {
   uint32_t n_keys;
 };
 
+// Digit width of the LSD radix sorts over `bits` key bits: the fewest passes
+// of at most 8 bits, equally wide (a 13-bit key: 7 + 6). A narrower digit
+// gives each radix tile fewer, longer runs to write.
+inline int radix_digit_bits(int bits) {
+  const int passes = (bits + 7) / 8;
+  return passes ? (bits + passes - 1) / passes : 8;
+}
 // Stable LSD radix sort of (key, value) pairs over bits [0, bits); count read
 // from device memory (*n_dev <= n_max). Ping-pongs between the two buffer
 // pairs; returns the index (0/1) of the buffer pair holding the result.
@@ -115,7 +122,7 @@ __device__ __forceinline__ uint2 bin_rect(uint2 r, int sh) {
 void launch_gen_pairs(const uint32_t* sorted_vals, size_t n_sorted, const uint32_t* offsets, const uint32_t* total,
                       const SplatArrays S, int cap, const CamDev* cams, uint32_t* pair_keys, uint32_t* pair_vals,
                       size_t k_cap, uint32_t* k_out, uint32_t* status, uint32_t* first_splat, RadixScratch rs,
-                      hipStream_t st, const uint32_t* n_dev = nullptr);
+                      int key_bits, hipStream_t st, const uint32_t* n_dev = nullptr);
 // Tile sort of the pairs from launch_gen_pairs (whose first-pass histogram is
 // already written). Only the values of the result are written (the keys are
 // implied by the ranges); the ranges come out of the last pass.
