@@ -48,7 +48,10 @@ __device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* s_
   return base + x - v;
 }
 
-__global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t* __restrict__ keys,
+// K: the key type (uint32_t, or uint16_t for the tile sort of at most 2^16
+// bins: a quarter less traffic per pair and pass).
+template <class K>
+__global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const K* __restrict__ keys,
                                                          const uint32_t* __restrict__ n_dev, int shift,
                                                          uint32_t mask, uint32_t* __restrict__ hist, int nb_max) {
   __shared__ uint32_t h[256];
@@ -149,9 +152,9 @@ __global__ __launch_bounds__(RSCAN_THREADS) void k_radix_rowscan(uint32_t* __res
 // (atomicMin / atomicMax), which yields each key's global [start, end).
 // D8: the digit is 8 bits wide (dbits is ignored; the matching loop is
 // fully unrolled).
-template <bool LAST, bool D8>
+template <bool LAST, bool D8, class K>
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
-    const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ keys_out,
+    const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, K* __restrict__ keys_out,
     uint32_t* __restrict__ vals_out, const uint32_t* __restrict__ n_dev, int shift, int dbits,
     const uint32_t* __restrict__ hist, const uint32_t* __restrict__ row_total, int nb_max, TileRanges tr) {
   if (D8) dbits = 8;
@@ -185,12 +188,12 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
   // items of this wave: wave-local index r * 64 + lane < w_n
   const uint32_t w_lo = w * RS_WAVE_ITEMS;
   const uint32_t w_n = blk_n > w_lo ? min(blk_n - w_lo, (uint32_t)RS_WAVE_ITEMS) : 0u;
-  const uint32_t* kin = keys_in + base + w_lo + lane;
+  const K* kin = keys_in + base + w_lo + lane;
   const uint32_t* vin = vals_in + base + w_lo + lane;
 #pragma unroll
   for (int r = 0; r < RS_ITEMS; ++r) {
     bool valid = (uint32_t)r * 64 + lane < w_n;
-    key[r] = valid ? kin[r * 64] : 0xffffffffu;
+    key[r] = valid ? (uint32_t)kin[r * 64] : 0xffffffffu;
   }
 #pragma unroll
   for (int r = 0; r < RS_ITEMS; ++r) {
@@ -253,7 +256,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
       s_dig[p] = (uint8_t)d;
       uint32_t o = s_glob[d] + (p - s_blk_start[d]);
       if (!LAST) {
-        keys_out[o] = k;
+        keys_out[o] = (K)k;
       } else if (k < tr.n_keys) {
         // within a digit run the keys are sorted (lower digits were sorted
         // by the earlier passes), so equal keys form one segment here
@@ -277,7 +280,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
   }
 }
 
-static int radix_sort_impl(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev, size_t n_max, int bits,
+template <class K>
+static int radix_sort_impl(K* keys[2], uint32_t* vals[2], const uint32_t* n_dev, size_t n_max, int bits,
                            RadixScratch rs, hipStream_t st, bool first_hist_done, const TileRanges* tr) {
   int cur = 0;
   int nb = radix_nb(n_max);
@@ -292,22 +296,22 @@ static int radix_sort_impl(uint32_t* keys[2], uint32_t* vals[2], const uint32_t*
     const bool last = shift + db >= bits;
     const int w = std::min(db, bits - shift);
     if (!(shift == 0 && first_hist_done))
-      hipLaunchKernelGGL(k_radix_hist, dim3(grid), dim3(RS_THREADS), 0, st, keys[cur], n_dev, shift,
+      hipLaunchKernelGGL(k_radix_hist<K>, dim3(grid), dim3(RS_THREADS), 0, st, keys[cur], n_dev, shift,
                          (1u << w) - 1u, rs.hist, rs.nb_max);
     hipLaunchKernelGGL(k_radix_rowscan, dim3(256), dim3(RSCAN_THREADS), 0, st, rs.hist, n_dev, rs.nb_max, 1 << w,
                        rs.row_total);
     const TileRanges none{nullptr, nullptr, 0u};
     if (last && tr && w == 8)
-      hipLaunchKernelGGL((k_radix_scatter<true, true>), dim3(grid), dim3(RS_THREADS), 0, st, keys[cur], vals[cur],
+      hipLaunchKernelGGL((k_radix_scatter<true, true, K>), dim3(grid), dim3(RS_THREADS), 0, st, keys[cur], vals[cur],
                          keys[cur ^ 1], vals[cur ^ 1], n_dev, shift, w, rs.hist, rs.row_total, rs.nb_max, *tr);
     else if (last && tr)
-      hipLaunchKernelGGL((k_radix_scatter<true, false>), dim3(grid), dim3(RS_THREADS), 0, st, keys[cur], vals[cur],
+      hipLaunchKernelGGL((k_radix_scatter<true, false, K>), dim3(grid), dim3(RS_THREADS), 0, st, keys[cur], vals[cur],
                          keys[cur ^ 1], vals[cur ^ 1], n_dev, shift, w, rs.hist, rs.row_total, rs.nb_max, *tr);
     else if (w == 8)
-      hipLaunchKernelGGL((k_radix_scatter<false, true>), dim3(grid), dim3(RS_THREADS), 0, st, keys[cur], vals[cur],
+      hipLaunchKernelGGL((k_radix_scatter<false, true, K>), dim3(grid), dim3(RS_THREADS), 0, st, keys[cur], vals[cur],
                          keys[cur ^ 1], vals[cur ^ 1], n_dev, shift, w, rs.hist, rs.row_total, rs.nb_max, none);
     else
-      hipLaunchKernelGGL((k_radix_scatter<false, false>), dim3(grid), dim3(RS_THREADS), 0, st, keys[cur], vals[cur],
+      hipLaunchKernelGGL((k_radix_scatter<false, false, K>), dim3(grid), dim3(RS_THREADS), 0, st, keys[cur], vals[cur],
                          keys[cur ^ 1], vals[cur ^ 1], n_dev, shift, w, rs.hist, rs.row_total, rs.nb_max, none);
     cur ^= 1;
   }
@@ -320,6 +324,10 @@ int radix_sort_pairs(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev
 }
 int radix_sort_tiles(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev, size_t n_max, int bits,
                      RadixScratch rs, TileRanges tr, hipStream_t st) {
+  if (tile_keys_16(bits)) {
+    uint16_t* k16[2] = {reinterpret_cast<uint16_t*>(keys[0]), reinterpret_cast<uint16_t*>(keys[1])};
+    return radix_sort_impl(k16, vals, n_dev, n_max, bits, rs, st, true, &tr);
+  }
   return radix_sort_impl(keys, vals, n_dev, n_max, bits, rs, st, true, &tr);
 }
 int radix_sort_tiles_full(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev, size_t n_max, int bits,
@@ -793,13 +801,14 @@ __global__ __launch_bounds__(256) void k_block_first(const uint32_t* __restrict_
   for (uint64_t b = (lo + RS_TILE - 1) / RS_TILE; b * RS_TILE < hi; ++b) first_splat[b] = (uint32_t)r;
 }
 
+template <class KT>
 __global__ __launch_bounds__(GP_THREADS) void k_gen_pairs(const uint32_t* __restrict__ sorted_vals, size_t n_sorted,
                                                           const uint32_t* __restrict__ offsets,
                                                           const uint32_t* __restrict__ total,
                                                           const uint32_t* __restrict__ cnt,
                                                           const uint2* __restrict__ rect, int cap,
                                                           const CamDev* __restrict__ cams,
-                                                          uint32_t* __restrict__ pair_keys,
+                                                          KT* __restrict__ pair_keys,
                                                           uint32_t* __restrict__ pair_vals, size_t k_cap,
                                                           uint32_t* __restrict__ k_out, uint32_t* __restrict__ status,
                                                           const uint32_t* __restrict__ first_splat,
@@ -909,7 +918,7 @@ __global__ __launch_bounds__(GP_THREADS) void k_gen_pairs(const uint32_t* __rest
       const uint32_t ty = (org >> 16) + q;
       const uint32_t tx = (org & 0xffffu) + (local - q * w);
       const uint32_t key = s_tb[j] + ty * s_ntx[j] + tx;
-      pair_keys[p] = key;
+      pair_keys[p] = (KT)key;
       pair_vals[p] = s_vi[j];
       atomicAdd(&s_h[key & hmask], 1u);
     }
@@ -929,9 +938,15 @@ void launch_gen_pairs(const uint32_t* sorted_vals, size_t n_sorted, const uint32
                        total, k_cap, first_splat);
   size_t nb = (size_t)radix_nb(k_cap);
   if (nb == 0) nb = 1;
-  hipLaunchKernelGGL(k_gen_pairs, dim3(xcd_grid(nb)), dim3(GP_THREADS), 0, st, sorted_vals, n_sorted, offsets, total,
-                     S.cnt, S.rect, cap, cams, pair_keys, pair_vals, k_cap, k_out, status, first_splat, rs.hist,
-                     rs.nb_max, n_dev, (1u << radix_digit_bits(key_bits)) - 1u);
+  const uint32_t hmask = (1u << radix_digit_bits(key_bits)) - 1u;
+  if (tile_keys_16(key_bits))
+    hipLaunchKernelGGL(k_gen_pairs<uint16_t>, dim3(xcd_grid(nb)), dim3(GP_THREADS), 0, st, sorted_vals, n_sorted,
+                       offsets, total, S.cnt, S.rect, cap, cams, reinterpret_cast<uint16_t*>(pair_keys), pair_vals,
+                       k_cap, k_out, status, first_splat, rs.hist, rs.nb_max, n_dev, hmask);
+  else
+    hipLaunchKernelGGL(k_gen_pairs<uint32_t>, dim3(xcd_grid(nb)), dim3(GP_THREADS), 0, st, sorted_vals, n_sorted,
+                       offsets, total, S.cnt, S.rect, cap, cams, pair_keys, pair_vals, k_cap, k_out, status,
+                       first_splat, rs.hist, rs.nb_max, n_dev, hmask);
 }
 
 // ------------------------------------------------------- depth-cut binning

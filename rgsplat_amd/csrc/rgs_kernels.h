@@ -87,6 +87,9 @@ inline int radix_digit_bits(int bits) {
   const int passes = (bits + 7) / 8;
   return passes ? (bits + passes - 1) / passes : 8;
 }
+// Whether the pair generation and tile sort keep the bin keys as 16 bits
+// (at most 2^16 bins).
+inline bool tile_keys_16(int bits) { return bits <= 16; }
 // Stable LSD radix sort of (key, value) pairs over bits [0, bits); count read
 // from device memory (*n_dev <= n_max). Ping-pongs between the two buffer
 // pairs; returns the index (0/1) of the buffer pair holding the result.
@@ -124,7 +127,8 @@ void launch_gen_pairs(const uint32_t* sorted_vals, size_t n_sorted, const uint32
                       size_t k_cap, uint32_t* k_out, uint32_t* status, uint32_t* first_splat, RadixScratch rs,
                       int key_bits, hipStream_t st, const uint32_t* n_dev = nullptr);
 // Tile sort of the pairs from launch_gen_pairs (whose first-pass histogram is
-// already written). Only the values of the result are written (the keys are
+// already written; the keys are 16-bit when tile_keys_16(bits), in the
+// same buffers). Only the values of the result are written (the keys are
 // implied by the ranges); the ranges come out of the last pass.
 int radix_sort_tiles(uint32_t* keys[2], uint32_t* vals[2], const uint32_t* n_dev, size_t n_max, int bits,
                      RadixScratch rs, TileRanges tr, hipStream_t st);
