@@ -82,8 +82,9 @@ def test_device_expf_matches_glibc(ctx):
 
 
 # --------------------------------------------------------------- forward ---
+# (4128, 4096): 66048 tiles, beyond the 16-bit bin keys of the tile sort
 @pytest.mark.parametrize("name,n,size", [("c0_10k_256", None, None), ("c1_100k_800", 30_000, (480, 360)),
-                                         ("c0_10k_256", 3000, (100, 75))])
+                                         ("c0_10k_256", 3000, (100, 75)), ("c0_10k_256", 4000, (4128, 4096))])
 def test_binning_and_image_bitexact(ctx, name, n, size):
     g, cams, _ = workload(name, n, [size] if size else None)
     cam = cams[0]
