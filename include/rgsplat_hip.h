@@ -215,7 +215,10 @@ int rgs_step_stats(rgs_ctx* ctx, uint64_t* n_pairs, uint64_t* fwd_evals, uint64_
  * and sorts the pairs of the splats in front of a cut depth first, and only
  * the tiles those lists leave unsaturated get the pairs behind the cut.
  * mode 0 (default): the cut is the previous step's saturation depth + 5%
- * (none on the first step); 1: no cut (whole lists); 2: cut at the fixed
+ * (none on the first step), paused for 256 steps (whole lists) after a step
+ * that needed back lists while fewer than an eighth as many visible splats
+ * lay behind the cut as in front of it, and resumed with the back-list stage
+ * on its first two steps; 1: no cut (whole lists); 2: cut at the fixed
  * camera-space depth `depth` (> 0). While the previous step needed no back
  * list, a step runs without the back-list stage (a captured graph of its
  * own); if it then leaves a bin unsaturated, the device skips its update and
