@@ -131,10 +131,14 @@ __device__ __forceinline__ void alpha_fast(float qf, float op, const uint64_t* s
                                            float& alpha_raw) {
   g_exp = __builtin_amdgcn_exp2f(qf * NEG_HALF_LOG2E);
   alpha_raw = op * g_exp;
-  if (__builtin_fabsf(alpha_raw - ALPHA_THRESHOLD) <= 1e-5f * ALPHA_THRESHOLD ||
-      __builtin_fabsf(alpha_raw - MAX_ALPHA) <= 1e-5f * MAX_ALPHA) {
-    g_exp = expf_tab(-0.5f * qf, s_tab);
-    alpha_raw = op * g_exp;
+  const bool near = __builtin_fabsf(alpha_raw - ALPHA_THRESHOLD) <= 1e-5f * ALPHA_THRESHOLD ||
+                    __builtin_fabsf(alpha_raw - MAX_ALPHA) <= 1e-5f * MAX_ALPHA;
+  // a wave-uniform test first: the common case (no lane near a threshold)
+  // leaves the execution mask untouched
+  if (__builtin_amdgcn_ballot_w64(near) != 0) {
+    const float ge = expf_tab(-0.5f * qf, s_tab);
+    g_exp = near ? ge : g_exp;
+    alpha_raw = op * g_exp;  // unchanged where not near
   }
 }
 // f32::min(alpha_raw, 0.99) (NaN -> 0.99); *clamped is set when the clamp
@@ -313,12 +317,10 @@ __device__ __forceinline__ void fwd_body(const RasterIn& in, const int lb, float
       for (int q = 0; q < 4; ++q) {
         if (!(m >> q & 1u)) continue;  // wave-uniform
         if (COUNT) n_visit += 1;
-        if (T[q] < TRANSMITTANCE_STOP) continue;
-        if (COUNT) n_eval += 1;
+        if (COUNT && !(T[q] < TRANSMITTANCE_STOP)) n_eval += 1;
         float dx = fpx - A.x;
         float dy = (fpy0 + (float)(4 * q)) - A.y;
         const float qf = A.z * dx * dx + 2.0f * A.w * dx * dy + B.x * dy * dy;
-        if (qf < POWER_POSITIVE_QF) continue;  // power > 0
         float alpha;
         if (FAST) {
           float g_exp, alpha_raw;
@@ -327,7 +329,11 @@ __device__ __forceinline__ void fwd_body(const RasterIn& in, const int lb, float
         } else {
           alpha = fmin_(B.y * expf_tab(-0.5f * qf, s_tab), MAX_ALPHA);
         }
-        if (alpha < ALPHA_THRESHOLD) continue;
+        // a pixel that takes no more splats, power > 0 or alpha below
+        // 1/255: no composite. One test after alpha rather than a branch
+        // each: every branch is an execution-mask region, and those, not the
+        // arithmetic, bound this loop (alpha of a dropped lane is wasted)
+        if (T[q] < TRANSMITTANCE_STOP || qf < POWER_POSITIVE_QF || alpha < ALPHA_THRESHOLD) continue;
         if (COUNT) n_comp += 1;
         // the reference's operations (no fused multiply-adds): the training
         // forward's images then differ from the reference's only where the
@@ -736,16 +742,16 @@ __device__ __forceinline__ void bwd_body(const RasterIn& in, const int bin, cons
       for (int q = 0; q < 4; ++q) {
         if (!(m >> q & 1u)) continue;  // wave-uniform
         if (COUNT) n_visit += 1;
-        if (li > lc[q]) continue;
-        if (COUNT) n_eval += 1;
+        if (COUNT && !(li > lc[q])) n_eval += 1;
         float dy = (fpy0 + (float)(4 * q)) - A.y;
         const float qf = A.z * dx * dx + 2.0f * A.w * dx * dy + B.x * dy * dy;
-        if (qf < POWER_POSITIVE_QF) continue;  // power > 0; NaN proceeds, as in the reference
         float g_exp, alpha_raw;
         alpha_fast(qf, B.y, s_tab, g_exp, alpha_raw);
         bool clamped;
         const float alpha = clamp_alpha(alpha_raw, &clamped);
-        if (!(alpha >= ALPHA_THRESHOLD)) continue;
+        // past the pixel's last contributor, power > 0 (NaN proceeds, as in
+        // the reference) or alpha below 1/255: one test, as in the forward
+        if (li > lc[q] || qf < POWER_POSITIVE_QF || !(alpha >= ALPHA_THRESHOLD)) continue;
         if (COUNT) n_comp += 1;
         contrib = true;
         {
