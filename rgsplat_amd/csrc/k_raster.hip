@@ -590,6 +590,17 @@ __device__ __forceinline__ float row_sum8_scatter(const float g[8], int lane) {
   float t = kp + swz_xor<4>(snd);
   return t + dpp_f<0x128>(t);  // row_ror:8 = lane ^ 8 inside a row of 16
 }
+// Sum over the four rows of 16 lanes, complete in row 3 (lanes 48-63). The
+// first result of v_permlane16_swap(v, v) holds rows (0, 0, 2, 2) and that
+// of v_permlane32_swap(v, v) rows (0, 1, 0, 1); adding each to v leaves row
+// 1 + 0 and 3 + 2, then row 3 + 2 + 1 + 0. (Only the first result of each
+// swap is used.)
+__device__ __forceinline__ float cross_row_sum3(float v) {
+  const uint32_t u = __builtin_bit_cast(uint32_t, v);
+  v += __builtin_bit_cast(float, __builtin_amdgcn_permlane16_swap(u, u, false, false)[0]);
+  const uint32_t u2 = __builtin_bit_cast(uint32_t, v);
+  return v + __builtin_bit_cast(float, __builtin_amdgcn_permlane32_swap(u2, u2, false, false)[0]);
+}
 // Sum over each row of 16 lanes, in every lane of the row.
 __device__ __forceinline__ float row_sum(float v) {
   v += dpp_f<0xB1>(v);   // quad_perm [1,0,3,2]
@@ -628,6 +639,7 @@ __device__ __forceinline__ void bwd_body(const RasterIn& in, const int bin, cons
   // splat's reduced sums (values 0-7 of one column) land in eight different
   // banks
   __shared__ float s_g[NW][9][GS];
+  __shared__ float s_sink[NW][64 + NH * BT];  // written, never read (see g_dst)
   __shared__ int s_top[NW];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // wave-uniform
@@ -690,9 +702,13 @@ __device__ __forceinline__ void bwd_body(const RasterIn& in, const int bin, cons
   float g[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) g[k] = 0.0f;
-  // LDS slot offset of the value this lane adds after a row reduction
+  // where this lane stores after a wave reduction: lanes 48-56 (row 3,
+  // which holds the wave's sums) to the splat's column of value rev3(l & 15)
+  // (lane 56: value 8), the others to a sink of their own, so that every
+  // lane stores and the execution mask stays whole
   const int lr = lane & 15;
-  const int g_slot = (lr < 8 ? rev3(lr) : 8) * GS;
+  float* const g_dst =
+      lane >= 48 && lr <= 8 ? &s_g[w][lr < 8 ? rev3(lr) : 8][0] : &s_sink[w][lane];
   uint32_t n_eval = 0, n_visit = 0, n_comp = 0;
   for (int hi = top; hi >= 0; hi -= FB) {
     const int lo = max(0, hi - (FB - 1));
@@ -795,15 +811,12 @@ __device__ __forceinline__ void bwd_body(const RasterIn& in, const int bin, cons
           for (int k = 0; k < 9; ++k) atomicAdd(&s_g[w][k][j], g[k]);
         }
       } else {
-        // row sums; lanes 0-7 of each row add value rev3(l) and lane 8 adds
-        // value 8, four rows into each slot
+        // row sums (lanes 0-7 of each row: value rev3(l), lane 8: value 8),
+        // then the four rows summed in registers; each slot of the splat is
+        // this wave's alone, so a plain store (no LDS atomic) sets it
         const float t8 = row_sum8_scatter(g, lane);
         const float s8 = row_sum(g[8]);
-        // formed here with all lanes active (the last DPP exchanges then fold
-        // into their adds) rather than sunk into the branch below
-        float add = lr < 8 ? t8 : s8;
-        asm volatile("" : "+v"(add));
-        if (lr <= 8) atomicAdd(&s_g[w][0][j] + g_slot, add);
+        g_dst[j] = cross_row_sum3(lr < 8 ? t8 : s8);
       }
       // the accumulators live across splats and restart here, which spares
       // re-initialising them for every splat (4, 6 and 7 are set above)
