@@ -135,7 +135,7 @@ __device__ __forceinline__ void alpha_fast(float qf, float op, const uint64_t* s
                     __builtin_fabsf(alpha_raw - MAX_ALPHA) <= 1e-5f * MAX_ALPHA;
   // a wave-uniform test first: the common case (no lane near a threshold)
   // leaves the execution mask untouched
-  if (__builtin_amdgcn_ballot_w64(near) != 0) {
+  if (__builtin_expect(__builtin_amdgcn_ballot_w64(near) != 0, 0)) {
     const float ge = expf_tab(-0.5f * qf, s_tab);
     g_exp = near ? ge : g_exp;
     alpha_raw = op * g_exp;  // unchanged where not near
@@ -315,7 +315,7 @@ __device__ __forceinline__ void fwd_body(const RasterIn& in, const int lb, float
       const int32_t li = li0 + (int32_t)(b0 - start) + j;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        if (!(m >> q & 1u)) continue;  // wave-uniform
+        if (__builtin_expect(!(m >> q & 1u), 0)) continue;  // wave-uniform; most visits set it
         if (COUNT) n_visit += 1;
         if (COUNT && !(T[q] < TRANSMITTANCE_STOP)) n_eval += 1;
         float dx = fpx - A.x;
@@ -756,7 +756,7 @@ __device__ __forceinline__ void bwd_body(const RasterIn& in, const int bin, cons
       bool contrib = false;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        if (!(m >> q & 1u)) continue;  // wave-uniform
+        if (__builtin_expect(!(m >> q & 1u), 0)) continue;  // wave-uniform; most visits set it
         if (COUNT) n_visit += 1;
         if (COUNT && !(li > lc[q])) n_eval += 1;
         float dy = (fpy0 + (float)(4 * q)) - A.y;
