@@ -243,8 +243,10 @@ __device__ __forceinline__ void fwd_body(const RasterIn& in, const int lb, float
   int32_t last[4];
   // pixel-centre row of group 0; group q's is fpy0 + 4 q (exact)
   const float fpy0 = (float)(ty * TILE + (lane >> 4)) + 0.5f;
+  // (T stays in [+0, 1], where the bit patterns order as the values do)
   auto pixels_left = [&]() {
-    return fmaxf(fmaxf(T[0], T[1]), fmaxf(T[2], T[3])) >= TRANSMITTANCE_STOP;
+    const uint32_t tm = max(max(f2u(T[0]), f2u(T[1])), max(f2u(T[2]), f2u(T[3])));
+    return tm >= f2u(TRANSMITTANCE_STOP);
   };
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -289,7 +291,7 @@ __device__ __forceinline__ void fwd_body(const RasterIn& in, const int lb, float
     __syncthreads();
     // all lanes take part in the ballot: the lanes past the list's end may be
     // the ones with pixels left
-    const bool wave_live = __ballot(pixels_left()) != 0;
+    const bool wave_live = __builtin_amdgcn_ballot_w64(pixels_left()) != 0;
     bool done = false;
 #pragma unroll 1
     for (int h = 0; h < NH && !done; ++h) {
@@ -303,7 +305,7 @@ __device__ __forceinline__ void fwd_body(const RasterIn& in, const int lb, float
     // splats that reach some pixel group, visited front to back as set bits
     // of a scalar mask (their LDS reads are not prefetched: holding a second
     // register set costs more than the other waves need to cover the wait)
-    uint64_t live = __ballot(my_m != 0);
+    uint64_t live = __builtin_amdgcn_ballot_w64(my_m != 0);
     while (live) {
       const int jl = __builtin_ctzll(live);
       live &= live - 1;
@@ -346,7 +348,7 @@ __device__ __forceinline__ void fwd_body(const RasterIn& in, const int lb, float
         T[q] *= 1.0f - alpha;
         last[q] = li;
       }
-      if (__ballot(pixels_left()) == 0) {
+      if (__builtin_expect(__builtin_amdgcn_ballot_w64(pixels_left()) == 0, 0)) {
         fin = li;
         done = true;
         break;
@@ -797,7 +799,7 @@ __device__ __forceinline__ void bwd_body(const RasterIn& in, const int bin, cons
         H[q] += weight * dcol;
         }
       }
-      const uint64_t bm = __ballot(contrib);
+      const uint64_t bm = __builtin_amdgcn_ballot_w64(contrib);
       if (bm == 0) return;
       {
 #pragma clang fp contract(fast)
@@ -822,7 +824,7 @@ __device__ __forceinline__ void bwd_body(const RasterIn& in, const int bin, cons
       // re-initialising them for every splat (4, 6 and 7 are set above)
       g[0] = g[1] = g[2] = g[3] = g[5] = g[8] = 0.0f;
     };
-    uint64_t live = __ballot(my_m != 0);
+    uint64_t live = __builtin_amdgcn_ballot_w64(my_m != 0);
     while (live) {
       const int jl = 63 - __builtin_clzll(live);
       live &= ~(1ull << jl);
