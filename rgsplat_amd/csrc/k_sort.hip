@@ -287,7 +287,7 @@ static int radix_sort_impl(K* keys[2], uint32_t* vals[2], const uint32_t* n_dev,
   int nb = radix_nb(n_max);
   if (nb == 0) return 0;
   const unsigned grid = xcd_grid((size_t)nb);
-  if (tr) {
+  if (tr && !tr->prefilled) {
     launch_fill_u32(tr->rstart, tr->n_keys, 0xffffffffu, st);
     launch_fill_u32(tr->rend, tr->n_keys, 0u, st);
   }
@@ -1054,6 +1054,17 @@ __global__ void k_cut_end(CutState* cs) {
 
 void launch_cut_end(CutState* cs, hipStream_t st) { hipLaunchKernelGGL(k_cut_end, dim3(1), dim3(1), 0, st, cs); }
 
+// k_need_back then k_cut_end, in one launch.
+__global__ void k_need_back_cut_end(const uint32_t* n_flagged, uint32_t* status, CutState* cs) {
+  if (*n_flagged) atomicOr(status, ST_NEED_BACK);
+  if (cs->mode == 0) cs->key = cs->zneed ? f2u(__uint_as_float(cs->zneed) * 1.05f) : 0xffffffffu;
+  cs->zneed = 0;
+}
+
+void launch_need_back_cut_end(const uint32_t* n_flagged, uint32_t* status, CutState* cs, hipStream_t st) {
+  hipLaunchKernelGGL(k_need_back_cut_end, dim3(1), dim3(1), 0, st, n_flagged, status, cs);
+}
+
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -1078,6 +1089,23 @@ void launch_copy16(const void* a, void* b, size_t n16, hipStream_t st) {
 void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t st) {
   if (!n) return;
   hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+}
+
+// Several fills in one launch (segments in turn; block-strided in each).
+__global__ __launch_bounds__(256) void k_fill_multi(FillSegs f) {
+  for (int s = 0; s < f.count; ++s) {
+    uint32_t* const p = f.p[s];
+    const uint32_t n = f.n[s], v = f.v[s];
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) p[i] = v;
+  }
+}
+
+void launch_fill_multi(const FillSegs& f, hipStream_t st) {
+  uint32_t most = 0;
+  for (int s = 0; s < f.count; ++s) most = std::max(most, f.n[s]);
+  if (!most) return;
+  const unsigned grid = std::min(1024u, (most + 255u) / 256u);
+  hipLaunchKernelGGL(k_fill_multi, dim3(grid), dim3(256), 0, st, f);
 }
 void launch_fill_f32(float* p, size_t n, float v, hipStream_t st) {
   if (!n) return;

@@ -534,7 +534,7 @@ void enqueue_project_cut(rgs_ctx* c, const float* P, size_t cap, int n, const Vi
   int nv = (int)vs.cams.size();
   size_t dn = (size_t)nv * n;
   const CutState* cs = P_<CutState>(c->cutst);
-  if (cover) launch_fill_u32(reinterpret_cast<uint32_t*>(cover), ((size_t)vs.total_tiles + 3) / 4, 0u, st);
+  // (cover was cleared by the step's first fill)
   stage_begin(c, S_PROJECT);
   launch_project(P, (int)cap, n, P_<CamDev>(c->cams_d), nv, splats(c), P_<uint32_t>(c->dk[0]),
                  P_<uint32_t>(c->dv[0]), sc + SC_STATUS, 1, cover, st);
@@ -636,7 +636,9 @@ void enqueue_bin_and_raster_cut(rgs_ctx* c, size_t cap, int n, const ViewSet& vs
   uint32_t* keys[2] = {P_<uint32_t>(c->pk[0]), P_<uint32_t>(c->pk[1])};
   uint32_t* vals[2] = {P_<uint32_t>(c->pv[0]), P_<uint32_t>(c->pv[1])};
   stage_begin(c, S_TSORT);
-  TileRanges tr{P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend), (uint32_t)vs.total_bins};
+  // (the ranges, the flags and the queue counters were set by the step's
+  // first fill)
+  TileRanges tr{P_<uint32_t>(c->rstart), P_<uint32_t>(c->rend), (uint32_t)vs.total_bins, 1u};
   int cur = radix_sort_tiles(keys, vals, sc + SC_K, c->k_cap, vs.bits(), rs, tr, st);
   stage_end(c, S_TSORT);
   c->pair_vals = vals[cur];
@@ -649,9 +651,6 @@ void enqueue_bin_and_raster_cut(rgs_ctx* c, size_t cap, int n, const ViewSet& vs
   in.zneed = &cs->zneed;
   lpt_forward(c, in);
   in.queue = fwd_queue(c);
-  if (c->raster_queue) launch_fill_u32(P_<uint32_t>(c->rqueue), RQ_WORDS, 0u, st);
-  launch_fill_u32(P_<uint32_t>(c->flag), ((size_t)vs.total_bins + 3) / 4, 0u, st);
-  launch_fill_u32(P_<uint32_t>(c->tile_flag), ((size_t)vs.total_tiles + 3) / 4, 0u, st);
   stage_begin(c, S_FWD);
   launch_raster_fwd(in, 3, rgb_out, P_<float>(c->final_t), P_<int32_t>(c->last), st);
   stage_end(c, S_FWD);
@@ -659,8 +658,7 @@ void enqueue_bin_and_raster_cut(rgs_ctx* c, size_t cap, int n, const ViewSet& vs
     // no back-list stage: a flagged bin voids the step (Adam is skipped and
     // the host re-runs it with the stage)
     stage_begin(c, S_BACK);
-    launch_need_back(sc + SC_NFLAG, sc + SC_STATUS, st);
-    launch_cut_end(cs, st);
+    launch_need_back_cut_end(sc + SC_NFLAG, sc + SC_STATUS, cs, st);
     stage_end(c, S_BACK);
     c->back_vals = nullptr;
     return;
@@ -2147,7 +2145,19 @@ static int adapt_cut(rgs_ctx* c) {
 
 // Enqueues one whole training step on the context's stream (no host sync).
 static void enqueue_step(rgs_ctx* c, const ViewSet& vs, const float bg[3], float lambda, bool with_back) {
-  launch_fill_u32(P_<uint32_t>(c->scalars), step_scalar_words(c), 0u, c->st);
+  // one launch clears what the step accumulates into or flags: the scalars
+  // and look-back states, the bin and tile flags of the depth cut, the tile
+  // ranges (the tile sort's atomic minima and maxima), the persistent-raster
+  // queue counters and the coverage of a background step
+  FillSegs f{};
+  f.add(P_<uint32_t>(c->scalars), step_scalar_words(c), 0u);
+  f.add(P_<uint32_t>(c->flag), ((size_t)vs.total_bins + 3) / 4, 0u);
+  f.add(P_<uint32_t>(c->tile_flag), ((size_t)vs.total_tiles + 3) / 4, 0u);
+  f.add(P_<uint32_t>(c->rstart), (size_t)vs.total_bins, 0xffffffffu);
+  f.add(P_<uint32_t>(c->rend), (size_t)vs.total_bins, 0u);
+  if (c->raster_queue) f.add(P_<uint32_t>(c->rqueue), RQ_WORDS, 0u);
+  if (uint8_t* cover = step_cover(c, bg)) f.add(reinterpret_cast<uint32_t*>(cover), ((size_t)vs.total_tiles + 3) / 4, 0u);
+  launch_fill_multi(f, c->st);
   enqueue_project_cut(c, P_<float>(c->P), c->cap, (int)c->n, vs, step_cover(c, bg), with_back);
   enqueue_step_rest(c, vs, bg, lambda, with_back);
 }

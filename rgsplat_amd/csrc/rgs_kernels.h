@@ -78,6 +78,7 @@ struct TileRanges {
   uint32_t* rstart;
   uint32_t* rend;
   uint32_t n_keys;
+  uint32_t prefilled = 0;  // nonzero: rstart/rend already hold ~0u / 0 (the caller filled them)
 };
 
 // Digit width of the LSD radix sorts over `bits` key bits: the fewest passes
@@ -389,6 +390,24 @@ void launch_densify(const float* P, float* P_out, int cap, int n, const float* g
                     float grad_thresh, float prune_thresh, uint64_t max_gaussians, float extent, DensifyScratch ds,
                     uint32_t* n_out, hipStream_t st, float split_div = 0.0f);
 void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t st);
+// Up to 8 fills (p[s][0 .. n[s]) = v[s]) in one launch.
+struct FillSegs {
+  uint32_t* p[8];
+  uint32_t n[8];
+  uint32_t v[8];
+  int count;
+  void add(uint32_t* ptr, size_t len, uint32_t val) {
+    p[count] = ptr;
+    n[count] = (uint32_t)len;
+    v[count] = val;
+    ++count;
+  }
+};
+void launch_fill_multi(const FillSegs& f, hipStream_t st);
+// The step's end of the front raster without the back-list stage: flags the
+// step as needing that stage when a bin was flagged, then sets the next
+// step's cut (launch_need_back and launch_cut_end in one launch).
+void launch_need_back_cut_end(const uint32_t* n_flagged, uint32_t* status, CutState* cs, hipStream_t st);
 // b[i] = a[i] for n16 16-byte elements (the HBM copy probe).
 void launch_copy16(const void* a, void* b, size_t n16, hipStream_t st);
 void launch_fill_f32(float* p, size_t n, float v, hipStream_t st);
