@@ -280,12 +280,265 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
   }
 }
 
+// ------------------------------------------------- single-pass digit offsets
+// The digit totals of every pass in one read of the keys (gtot[p * 256 + d],
+// zero at launch): a block histogram in LDS per pass, then one global add per
+// non-empty digit and block.
+template <class K>
+__global__ __launch_bounds__(256) void k_radix_gtot(const K* __restrict__ keys, const uint32_t* __restrict__ n_dev,
+                                                    int bits, int db, uint32_t* __restrict__ gtot) {
+  __shared__ uint32_t h[OS_PASSES][256];
+#pragma unroll
+  for (int p = 0; p < OS_PASSES; ++p) h[p][threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t n = *n_dev;
+  const int passes = (bits + db - 1) / db;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+    const uint32_t k = keys[i];
+    for (int p = 0; p < passes; ++p) {
+      const int w = min(db, bits - p * db);
+      atomicAdd(&h[p][(k >> (p * db)) & ((1u << w) - 1u)], 1u);
+    }
+  }
+  __syncthreads();
+  for (int p = 0; p < passes; ++p) {
+    const uint32_t v = h[p][threadIdx.x];
+    if (v) atomicAdd(&gtot[p * 256 + threadIdx.x], v);
+  }
+}
+
+// One radix pass with the block's digit offsets from a decoupled look-back
+// (instead of a histogram pass and a row scan): blocks take tiles in ticket
+// order, rank their keys (as k_radix_scatter), publish each digit's count,
+// then each thread (one digit) walks back over the earlier tiles until one
+// that has published its inclusive prefix. A tile waits only for tiles
+// whose workgroups already run and have published their counts before
+// waiting, so the look-back ends; its wait is bounded all the same
+// (OS_SPIN_LIMIT polls), past which it flags ST_SCAN_STALL and goes on with
+// a wrong offset. The rest is k_radix_scatter's staging and writes, which
+// keep the sort stable.
+constexpr uint32_t OS_AGG = 1u << 30, OS_INC = 2u << 30, OS_VAL = OS_AGG - 1u;
+constexpr uint32_t OS_SPIN_LIMIT = 1u << 16;
+constexpr int OS_WIN = 8;
+template <bool LAST, bool D8, class K>
+__global__ __launch_bounds__(RS_THREADS) void k_radix_onesweep(
+    const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, K* __restrict__ keys_out,
+    uint32_t* __restrict__ vals_out, const uint32_t* __restrict__ n_dev, int shift, int dbits,
+    const uint32_t* __restrict__ gtot, uint32_t* __restrict__ tw, uint32_t* __restrict__ status, TileRanges tr) {
+  if (D8) dbits = 8;
+  const uint32_t mask = (1u << dbits) - 1u;
+  __shared__ uint32_t s_cnt[RS_WAVES][256];
+  __shared__ uint32_t s_blk_start[256];
+  __shared__ uint32_t s_glob[256];
+  __shared__ uint32_t s_stage[RS_TILE];
+  __shared__ uint8_t s_dig[RS_TILE];
+  __shared__ uint32_t s_tmp[8];
+  __shared__ uint32_t s_b;
+
+  const uint32_t n = *n_dev;
+  const uint32_t nb = (n + RS_TILE - 1) / RS_TILE;
+  if (blockIdx.x >= nb) return;  // exactly nb workgroups take tickets 0..nb-1
+  const uint32_t t = threadIdx.x;
+  if (t == 0) s_b = atomicAdd(&tw[0], 1u);
+#pragma unroll
+  for (int k = 0; k < RS_WAVES; ++k) s_cnt[k][t] = 0;
+  __syncthreads();
+  const uint32_t b = s_b;
+  const uint32_t lane = lane_id();
+  const uint32_t w = t >> 6;
+  const size_t base = (size_t)b * RS_TILE;
+  const uint32_t blk_n = (uint32_t)min((size_t)RS_TILE, (size_t)n - base);
+
+  uint32_t key[RS_ITEMS], val[RS_ITEMS], pos[RS_ITEMS];
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const uint32_t w_lo = w * RS_WAVE_ITEMS;
+  const uint32_t w_n = blk_n > w_lo ? min(blk_n - w_lo, (uint32_t)RS_WAVE_ITEMS) : 0u;
+  const K* kin = keys_in + base + w_lo + lane;
+  const uint32_t* vin = vals_in + base + w_lo + lane;
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    bool valid = (uint32_t)r * 64 + lane < w_n;
+    key[r] = valid ? (uint32_t)kin[r * 64] : 0xffffffffu;
+  }
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    bool valid = (uint32_t)r * 64 + lane < w_n;
+    uint32_t d = (key[r] >> shift) & mask;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 8; ++bit) {
+      if (!D8 && bit >= dbits) break;  // uniform
+      uint64_t bb = __ballot((d >> bit) & 1u);
+      peers &= ((d >> bit) & 1u) ? bb : ~bb;
+    }
+    uint32_t lrank = (uint32_t)__popcll(peers & lt_mask);
+    uint32_t before = 0;
+    if (valid) before = s_cnt[w][d];
+    __builtin_amdgcn_wave_barrier();
+    if (valid && lrank == 0) s_cnt[w][d] = before + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    pos[r] = before + lrank;
+  }
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    bool valid = (uint32_t)r * 64 + lane < w_n;
+    val[r] = valid ? vin[r * 64] : 0u;
+  }
+  __syncthreads();
+  uint32_t c0 = s_cnt[0][t], c1 = s_cnt[1][t], c2 = s_cnt[2][t], c3 = s_cnt[3][t];
+  uint32_t tot = c0 + c1 + c2 + c3;
+  // digit t of this tile: publish the count, look back for the tiles before
+  uint32_t* words = tw + OS_HEAD;
+  __hip_atomic_store(&words[(size_t)b * 256 + t], (b == 0 ? OS_INC : OS_AGG) | tot, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t prefix = 0;
+  if (b > 0) {
+    // OS_WIN tiles' words in flight per step (tiles j, j - 1, ...): their
+    // counts are added up to the nearest inclusive prefix, or up to the
+    // first tile that has published nothing yet, from where the walk resumes
+    int j = (int)b - 1;
+    uint32_t polls = 0;
+    while (j >= 0) {
+      uint32_t wv[OS_WIN];
+#pragma unroll
+      for (int k = 0; k < OS_WIN; ++k)
+        wv[k] = j - k >= 0 ? __hip_atomic_load(&words[(size_t)(j - k) * 256 + t], __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT)
+                           : OS_INC;
+      int used = 0;
+      bool done = false;
+#pragma unroll
+      for (int k = 0; k < OS_WIN; ++k) {
+        const uint32_t fl = wv[k] & ~OS_VAL;
+        if (done || fl == 0) break;
+        prefix += wv[k] & OS_VAL;
+        used = k + 1;
+        done = fl == OS_INC;
+      }
+      if (done) break;
+      j -= used;
+      if (used == 0) {
+        if (++polls > OS_SPIN_LIMIT) {
+          atomicOr(status, ST_SCAN_STALL);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __hip_atomic_store(&words[(size_t)b * 256 + t], OS_INC | ((prefix + tot) & OS_VAL), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+  // global start of digit t: the totals of the digits below it, then the
+  // tiles before this one
+  const uint32_t dig_off = block_excl_scan_256(t <= mask ? gtot[t] : 0u, s_tmp, nullptr);
+  s_glob[t] = dig_off + prefix;
+  uint32_t start = block_excl_scan_256(tot, s_tmp, nullptr);
+  s_blk_start[t] = start;
+  __syncthreads();
+  s_cnt[0][t] = start;
+  s_cnt[1][t] = start + c0;
+  s_cnt[2][t] = start + c0 + c1;
+  s_cnt[3][t] = start + c0 + c1 + c2;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    asm volatile("" : "+v"(key[r]));
+    if ((uint32_t)r * 64 + lane < w_n) {
+      uint32_t d = (key[r] >> shift) & mask;
+      pos[r] += s_cnt[w][d];
+      s_stage[pos[r]] = key[r];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    uint32_t p = (uint32_t)r * RS_THREADS + t;
+    if (p < blk_n) {
+      uint32_t k = s_stage[p];
+      uint32_t d = (k >> shift) & mask;
+      s_dig[p] = (uint8_t)d;
+      uint32_t o = s_glob[d] + (p - s_blk_start[d]);
+      if (!LAST) {
+        keys_out[o] = (K)k;
+      } else if (k < tr.n_keys) {
+        if (p == 0 || s_stage[p - 1] != k) atomicMin(&tr.rstart[k], o);
+        if (p + 1 == blk_n || s_stage[p + 1] != k) atomicMax(&tr.rend[k], o + 1);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r)
+    if ((uint32_t)r * 64 + lane < w_n) s_stage[pos[r]] = val[r];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    uint32_t p = (uint32_t)r * RS_THREADS + t;
+    if (p < blk_n) {
+      uint32_t d = s_dig[p];
+      vals_out[s_glob[d] + (p - s_blk_start[d])] = s_stage[p];
+    }
+  }
+}
+
+// The sort's passes with single-pass digit offsets (rs.os set).
+template <class K>
+static int radix_sort_onesweep(K* keys[2], uint32_t* vals[2], const uint32_t* n_dev, size_t n_max, int bits,
+                               RadixScratch rs, hipStream_t st, const TileRanges* tr) {
+  const int nb = radix_nb(n_max);
+  const int db = radix_digit_bits(bits);
+  const int passes = (bits + db - 1) / db;
+  // the totals and the tile words of the passes, zeroed in one launch
+  // (unless the caller cleared them)
+  if (!rs.os_zeroed) launch_fill_u32(rs.os, radix_os_used_words(nb, passes), 0u, st);
+  if (tr && !tr->prefilled) {
+    launch_fill_u32(tr->rstart, tr->n_keys, 0xffffffffu, st);
+    launch_fill_u32(tr->rend, tr->n_keys, 0u, st);
+  }
+  // a modest grid: every workgroup adds its non-empty digit counts to the
+  // totals with global atomics
+  hipLaunchKernelGGL(k_radix_gtot<K>, dim3((unsigned)std::min(nb, 128)), dim3(256), 0, st, keys[0], n_dev, bits, db,
+                     rs.os);
+  int cur = 0;
+  for (int p = 0; p < passes; ++p) {
+    const int shift = p * db;
+    const int w = std::min(db, bits - shift);
+    const bool last = p + 1 == passes;
+    const uint32_t* gt = rs.os + (size_t)256 * p;
+    uint32_t* tw = rs.os + (size_t)256 * OS_PASSES + (size_t)p * radix_os_pass_words(nb);
+    const TileRanges none{nullptr, nullptr, 0u};
+    const dim3 grid((unsigned)nb);
+    if (last && tr && w == 8)
+      hipLaunchKernelGGL((k_radix_onesweep<true, true, K>), grid, dim3(RS_THREADS), 0, st, keys[cur], vals[cur],
+                         keys[cur ^ 1], vals[cur ^ 1], n_dev, shift, w, gt, tw, rs.status, *tr);
+    else if (last && tr)
+      hipLaunchKernelGGL((k_radix_onesweep<true, false, K>), grid, dim3(RS_THREADS), 0, st, keys[cur], vals[cur],
+                         keys[cur ^ 1], vals[cur ^ 1], n_dev, shift, w, gt, tw, rs.status, *tr);
+    else if (w == 8)
+      hipLaunchKernelGGL((k_radix_onesweep<false, true, K>), grid, dim3(RS_THREADS), 0, st, keys[cur], vals[cur],
+                         keys[cur ^ 1], vals[cur ^ 1], n_dev, shift, w, gt, tw, rs.status, none);
+    else
+      hipLaunchKernelGGL((k_radix_onesweep<false, false, K>), grid, dim3(RS_THREADS), 0, st, keys[cur], vals[cur],
+                         keys[cur ^ 1], vals[cur ^ 1], n_dev, shift, w, gt, tw, rs.status, none);
+    cur ^= 1;
+  }
+  return cur;
+}
+
 template <class K>
 static int radix_sort_impl(K* keys[2], uint32_t* vals[2], const uint32_t* n_dev, size_t n_max, int bits,
                            RadixScratch rs, hipStream_t st, bool first_hist_done, const TileRanges* tr) {
   int cur = 0;
   int nb = radix_nb(n_max);
   if (nb == 0) return 0;
+  // single-pass offsets for the depth sorts only: over the tile sort's
+  // thousand-odd tiles the look-back's round trips to device-coherent memory
+  // cost more than the histogram and row-scan launches they replace
+  if (rs.os && rs.status && !tr && sizeof(K) == 4 && n_max < ((size_t)1 << 30) &&
+      radix_digit_bits(bits) * OS_PASSES >= bits)
+    return radix_sort_onesweep(keys, vals, n_dev, n_max, bits, rs, st, tr);
   const unsigned grid = xcd_grid((size_t)nb);
   if (tr && !tr->prefilled) {
     launch_fill_u32(tr->rstart, tr->n_keys, 0xffffffffu, st);

@@ -231,7 +231,7 @@ struct rgs_ctx {
   // render workspace (shared by every entry point)
   DBuf cams_d, xy_ab, c_op_rg, col_b, cnt, rect, reach, depth;
   DBuf dk[2], dv[2], offsets, bsums, scalars;
-  DBuf pk[2], pv[2], hist, rowtot;
+  DBuf pk[2], pv[2], hist, rowtot, os;
   DBuf rstart, rend, rgb, final_t, last, dl, partial, loss;
   DBuf first_splat;
   DBuf accum;
@@ -455,6 +455,7 @@ int ensure_render(rgs_ctx* c, size_t cap, int n, const ViewSet& vs, size_t k_cap
   CHK(ensure(c, c->hist, (size_t)256 * nb * 4));
   c->hist_nb = nb;
   CHK(ensure(c, c->rowtot, 256 * 4));
+  CHK(ensure(c, c->os, radix_os_words(nb) * 4));
   CHK(ensure(c, c->rstart, (size_t)vs.total_bins * 4));
   CHK(ensure(c, c->rend, (size_t)vs.total_bins * 4));
   CHK(ensure(c, c->rbstart, (size_t)vs.total_bins * 4));
@@ -524,6 +525,15 @@ uint64_t* lb_state(rgs_ctx* c, int k) {
   return reinterpret_cast<uint64_t*>(P_<uint32_t>(c->scalars) + SC_COUNT) + (size_t)k * lookback_words(c->depth_cap);
 }
 
+// The training step's radix sorts: single-pass digit offsets, a look-back
+// that gives up flagging the step's status.
+RadixScratch step_radix_scratch(rgs_ctx* c) {
+  RadixScratch rs{P_<uint32_t>(c->hist), P_<uint32_t>(c->rowtot), c->hist_nb};
+  rs.os = P_<uint32_t>(c->os);
+  rs.status = P_<uint32_t>(c->scalars) + SC_STATUS;
+  return rs;
+}
+
 // The training step's projection, partition at the cut (back entries only
 // when the step has the back-list stage), depth sort of the front entries
 // and pair offsets.
@@ -539,7 +549,8 @@ void enqueue_project_cut(rgs_ctx* c, const float* P, size_t cap, int n, const Vi
   launch_project(P, (int)cap, n, P_<CamDev>(c->cams_d), nv, splats(c), P_<uint32_t>(c->dk[0]),
                  P_<uint32_t>(c->dv[0]), sc + SC_STATUS, 1, cover, st);
   stage_end(c, S_PROJECT);
-  RadixScratch rs{P_<uint32_t>(c->hist), P_<uint32_t>(c->rowtot), c->hist_nb};
+  RadixScratch rs = step_radix_scratch(c);
+  rs.os_zeroed = true;  // by the step's first fill
   stage_begin(c, S_DSORT);
   partition_at_cut(P_<uint32_t>(c->dk[0]), P_<uint32_t>(c->dv[0]), dn, cs, P_<uint32_t>(c->dk[1]),
                    P_<uint32_t>(c->dv[1]), with_back ? P_<uint32_t>(c->kB) : nullptr,
@@ -626,7 +637,7 @@ void enqueue_bin_and_raster_cut(rgs_ctx* c, size_t cap, int n, const ViewSet& vs
   hipStream_t st = c->st;
   uint32_t* sc = P_<uint32_t>(c->scalars);
   size_t dn = vs.cams.size() * (size_t)n;
-  RadixScratch rs{P_<uint32_t>(c->hist), P_<uint32_t>(c->rowtot), c->hist_nb};
+  RadixScratch rs = step_radix_scratch(c);
   CutState* cs = P_<CutState>(c->cutst);
   stage_begin(c, S_GEN);
   launch_gen_pairs(c->sorted_vals, dn, P_<uint32_t>(c->offsets), sc + SC_TOTAL, splats(c), (int)cap,
@@ -882,7 +893,7 @@ void rgs_destroy(rgs_ctx* c) {
   drop_graph(c);
   DBuf* all[] = {&c->cams_d, &c->reach, &c->xy_ab, &c->c_op_rg, &c->col_b, &c->cnt, &c->rect, &c->dk[0], &c->dk[1],
                  &c->dv[0], &c->dv[1], &c->offsets, &c->bsums, &c->scalars, &c->pk[0], &c->pk[1], &c->pv[0],
-                 &c->pv[1], &c->hist, &c->rowtot, &c->rstart, &c->rend, &c->rgb, &c->final_t, &c->last, &c->dl,
+                 &c->pv[1], &c->hist, &c->rowtot, &c->os, &c->rstart, &c->rend, &c->rgb, &c->final_t, &c->last, &c->dl,
                  &c->partial, &c->loss, &c->accum, &c->api_aos, &c->api_P, &c->api_grads,
                  &c->api_target, &c->api_m1, &c->api_m2, &c->api_acc, &c->api_den, &c->api_P2, &c->dens_want,
                  &c->dens_cnt, &c->dens_W, &c->dens_packed, &c->dens_prefix, &c->dens_b32, &c->dens_b64,
@@ -2157,6 +2168,8 @@ static void enqueue_step(rgs_ctx* c, const ViewSet& vs, const float bg[3], float
   f.add(P_<uint32_t>(c->rend), (size_t)vs.total_bins, 0u);
   if (c->raster_queue) f.add(P_<uint32_t>(c->rqueue), RQ_WORDS, 0u);
   if (uint8_t* cover = step_cover(c, bg)) f.add(reinterpret_cast<uint32_t*>(cover), ((size_t)vs.total_tiles + 3) / 4, 0u);
+  // the state of the front depth sort's single-pass digit offsets
+  f.add(P_<uint32_t>(c->os), radix_os_used_words(radix_nb(vs.cams.size() * c->n), 4), 0u);
   launch_fill_multi(f, c->st);
   enqueue_project_cut(c, P_<float>(c->P), c->cap, (int)c->n, vs, step_cover(c, bg), with_back);
   enqueue_step_rest(c, vs, bg, lambda, with_back);

@@ -51,10 +51,28 @@ struct RadixScratch {
   uint32_t* hist;       // 256 * nb_max, hist[d * nb_max + b]
   uint32_t* row_total;  // 256
   int nb_max;           // >= radix_nb(n_max), a multiple of 256
+  // optional: single-pass digit offsets (radix_os_words(nb_max) words) and
+  // the status word that flags a look-back that gave up (ST_SCAN_STALL);
+  // without them each pass is a histogram, a row scan and a scatter
+  uint32_t* os = nullptr;
+  uint32_t* status = nullptr;
+  bool os_zeroed = false;  // the caller zeroed os's first radix_os_used_words words
 };
 constexpr int RS_TILE = 4096;
 inline int radix_nb(size_t n_max) { return (int)((n_max + RS_TILE - 1) / RS_TILE); }
 inline int radix_nb_max(size_t n_max) { return (radix_nb(n_max) + 255) / 256 * 256; }
+// Single-pass digit offsets: the digit totals of up to OS_PASSES passes,
+// then per pass OS_HEAD words (the first a block ticket) and 256 words per
+// block (bits 30-31 a flag, bits 0-29 a count: sorts of fewer than 2^30
+// keys).
+constexpr int OS_PASSES = 4;
+constexpr int OS_HEAD = 64;
+inline size_t radix_os_pass_words(int nb) { return (size_t)OS_HEAD + (size_t)256 * (size_t)nb; }
+inline size_t radix_os_words(int nb_max) { return (size_t)256 * OS_PASSES + OS_PASSES * radix_os_pass_words(nb_max); }
+// The words a sort of at most nb tiles in `passes` passes uses.
+inline size_t radix_os_used_words(int nb, int passes) {
+  return (size_t)256 * OS_PASSES + (size_t)passes * radix_os_pass_words(nb);
+}
 
 // Logical block ids for per-radix-tile kernels. The hardware deals blocks to
 // the 8 XCDs round-robin; this mapping gives each XCD runs of XG consecutive
