@@ -992,19 +992,29 @@ struct PartitionAtCut {
   }
 };
 // Pair offsets: the exclusive scan of cnt[vals[r]] over the first *n_dev
-// sorted entries.
+// sorted entries; with first_splat set, also the entry holding the first
+// pair of every RS_TILE-pair block below k_cap (entry r's pairs are
+// [o, o + c)), which pair generation starts from.
 struct PairOffsets {
   static constexpr bool kTally = false;
   const uint32_t* vals;
   const uint32_t* cnt;
   uint32_t* out;
+  uint32_t* first_splat;
+  size_t k_cap;
   struct Item {
     uint32_t c;
   };
   __device__ Item item(size_t r) const { return {cnt[vals[r]]}; }
   __device__ static Item empty() { return {0u}; }
   __device__ uint32_t count(const Item& it) const { return it.c; }
-  __device__ void put(size_t r, const Item&, uint64_t o) const { out[r] = (uint32_t)o; }
+  __device__ void put(size_t r, const Item& it, uint64_t o) const {
+    out[r] = (uint32_t)o;
+    if (first_splat) {
+      const uint64_t hi = min(o + it.c, (uint64_t)k_cap);
+      for (uint64_t b = (o + RS_TILE - 1) / RS_TILE; b * RS_TILE < hi; ++b) first_splat[b] = (uint32_t)r;
+    }
+  }
   __device__ void last(uint64_t) const {}
   __device__ uint32_t tally(const Item&) const { return 0u; }
   uint32_t* tally_out = nullptr;
@@ -1023,9 +1033,10 @@ void partition_at_cut(const uint32_t* keys, const uint32_t* vals, size_t n, cons
 }
 
 void scan_pair_offsets(const uint32_t* vals, const uint32_t* cnt, size_t n_max, const uint32_t* n_dev,
-                       uint32_t* out, uint32_t* total, uint64_t* state, uint32_t* status, hipStream_t st) {
+                       uint32_t* out, uint32_t* total, uint64_t* state, uint32_t* status, hipStream_t st,
+                       uint32_t* first_splat, size_t k_cap) {
   const int nb = std::max(lookback_tiles(n_max), 1);
-  PairOffsets f{vals, cnt, out, nullptr};
+  PairOffsets f{vals, cnt, out, first_splat, k_cap, nullptr};
   hipLaunchKernelGGL((k_scan_lookback<PairOffsets, LB_TILE>), dim3(nb), dim3(SCAN_THREADS), 0, st, f, n_max, n_dev, state,
                      total, status);
 }
@@ -1186,7 +1197,9 @@ void launch_gen_pairs(const uint32_t* sorted_vals, size_t n_sorted, const uint32
                       const SplatArrays S, int cap, const CamDev* cams, uint32_t* pair_keys, uint32_t* pair_vals,
                       size_t k_cap, uint32_t* k_out, uint32_t* status, uint32_t* first_splat, RadixScratch rs,
                       int key_bits, hipStream_t st, const uint32_t* n_dev) {
-  if (n_sorted)
+  // with n_dev the pair-offset scan has recorded the blocks' first entries
+  // (it sees only the first *n_dev entries; the offsets past them are stale)
+  if (n_sorted && !n_dev)
     hipLaunchKernelGGL(k_block_first, dim3((unsigned)((n_sorted + 255) / 256)), dim3(256), 0, st, offsets, n_sorted,
                        total, k_cap, first_splat);
   size_t nb = (size_t)radix_nb(k_cap);

@@ -564,7 +564,7 @@ void enqueue_project_cut(rgs_ctx* c, const float* P, size_t cap, int n, const Vi
   c->sorted_keys = keys[cur];
   stage_begin(c, S_SCAN);
   scan_pair_offsets(vals[cur], P_<uint32_t>(c->cnt), dn, sc + SC_NF, P_<uint32_t>(c->offsets), sc + SC_TOTAL,
-                    lb_state(c, 1), sc + SC_STATUS, st);
+                    lb_state(c, 1), sc + SC_STATUS, st, P_<uint32_t>(c->first_splat), c->k_cap);
   stage_end(c, S_SCAN);
 }
 

@@ -140,7 +140,8 @@ __device__ __forceinline__ uint2 bin_rect(uint2 r, int sh) {
 // (hist[d * nb_max + b] for the low 8 key bits). K = min(total, k_cap) goes to
 // *k_out; ST_PAIR_OVERFLOW is raised when total > k_cap. first_splat needs
 // radix_nb(k_cap) entries. n_dev (optional): only the first *n_dev sorted
-// entries are valid.
+// entries are valid, and first_splat was written by scan_pair_offsets
+// (without n_dev it is made here from the offsets of all n_sorted entries).
 void launch_gen_pairs(const uint32_t* sorted_vals, size_t n_sorted, const uint32_t* offsets, const uint32_t* total,
                       const SplatArrays S, int cap, const CamDev* cams, uint32_t* pair_keys, uint32_t* pair_vals,
                       size_t k_cap, uint32_t* k_out, uint32_t* status, uint32_t* first_splat, RadixScratch rs,
@@ -183,9 +184,12 @@ void partition_at_cut(const uint32_t* keys, const uint32_t* vals, size_t n, cons
                       uint32_t* vf, uint32_t* kb, uint32_t* vb, uint32_t* n_front, uint32_t* n_back, uint32_t* cut_on,
                       uint32_t* n_back_visible, uint64_t* state, uint32_t* status, hipStream_t st);
 // Exclusive scan of cnt[vals[r]] over the first *n_dev of n_max entries
-// (pair offsets in depth order); the total goes to *total.
+// (pair offsets in depth order); the total goes to *total. first_splat[b]
+// gets the entry holding pair b * RS_TILE, for every such pair below the
+// total and k_cap.
 void scan_pair_offsets(const uint32_t* vals, const uint32_t* cnt, size_t n_max, const uint32_t* n_dev,
-                       uint32_t* out, uint32_t* total, uint64_t* state, uint32_t* status, hipStream_t st);
+                       uint32_t* out, uint32_t* total, uint64_t* state, uint32_t* status, hipStream_t st,
+                       uint32_t* first_splat, size_t k_cap);
 // Exclusive count of the back entries (first *n_dev) whose rect meets a
 // flagged bin (all 0 when no bin is flagged).
 void scan_touch_flagged(const uint32_t* vals, size_t n, const uint32_t* n_dev, const uint32_t* n_flagged,
