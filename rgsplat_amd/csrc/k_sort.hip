@@ -60,13 +60,21 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const K* __restrict__
   uint32_t nb = (n + RS_TILE - 1) / RS_TILE;
   if (b >= nb) return;
   h[threadIdx.x] = 0;
-  __syncthreads();
   size_t base = (size_t)b * RS_TILE;
-#pragma unroll 4
-  for (int k = 0; k < RS_ITEMS; ++k) {
-    size_t e = base + (size_t)k * RS_THREADS + threadIdx.x;
-    if (e < n) atomicAdd(&h[(keys[e] >> shift) & mask], 1u);
+  // the tile's keys first (all loads in flight together), then the counts
+  uint32_t kv[RS_ITEMS];
+  const K* kp = keys + base + threadIdx.x;
+  if (base + RS_TILE <= n) {
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; ++k) kv[k] = kp[k * RS_THREADS];
+  } else {
+#pragma unroll
+    for (int k = 0; k < RS_ITEMS; ++k) kv[k] = base + (size_t)k * RS_THREADS + threadIdx.x < n ? kp[k * RS_THREADS] : 0u;
   }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < RS_ITEMS; ++k)
+    if (base + (size_t)k * RS_THREADS + threadIdx.x < n) atomicAdd(&h[(kv[k] >> shift) & mask], 1u);
   __syncthreads();
   hist[(size_t)threadIdx.x * nb_max + b] = h[threadIdx.x];
 }
@@ -866,11 +874,17 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_lookback(F f, size_t n_ma
   const size_t base = (size_t)tile * TILE;
   typename F::Item it[LB_ITEMS];
   // thread t holds entries base + k * SCAN_THREADS + t (row k): coalesced
-  // loads and stores; entries past n hold the item of count 0
+  // loads and stores; entries past n hold the item of count 0. A full tile
+  // loads unguarded, so that all its loads are in flight together.
+  if (base + TILE <= n) {
 #pragma unroll
-  for (int k = 0; k < LB_ITEMS; ++k) {
-    const size_t e = base + (size_t)k * SCAN_THREADS + threadIdx.x;
-    it[k] = e < n ? f.item(e) : F::empty();
+    for (int k = 0; k < LB_ITEMS; ++k) it[k] = f.item(base + (size_t)k * SCAN_THREADS + threadIdx.x);
+  } else {
+#pragma unroll
+    for (int k = 0; k < LB_ITEMS; ++k) {
+      const size_t e = base + (size_t)k * SCAN_THREADS + threadIdx.x;
+      it[k] = e < n ? f.item(e) : F::empty();
+    }
   }
   // exclusive prefix within the tile: within the row's wave, then the row's
   // earlier waves, then the earlier rows
@@ -1020,16 +1034,143 @@ struct PairOffsets {
   uint32_t* tally_out = nullptr;
 };
 
+// The partition in two launches, with no look-back: each tile's count of
+// front entries (and the visible back tally), then every workgroup sums the
+// counts of the tiles before its own (a few KB, from the cache), ranks its
+// entries by ballot and writes them. Over the thousand-odd tiles of a step a
+// look-back's round trips to device-coherent memory cost more than the
+// second read of the keys.
+constexpr int PT_ITEMS = 16;
+constexpr int PT_TILE = PT_ITEMS * SCAN_THREADS;  // 4096 entries per tile
+
+__device__ __forceinline__ uint32_t block_sum_u32(uint32_t x, uint32_t* s_tmp /* >= SCAN_THREADS / 64 */) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
+  if ((threadIdx.x & 63) == 0) s_tmp[threadIdx.x >> 6] = x;
+  __syncthreads();
+  uint32_t r = 0;
+#pragma unroll
+  for (int w = 0; w < SCAN_THREADS / 64; ++w) r += s_tmp[w];
+  __syncthreads();
+  return r;
+}
+
+// The tile's keys (row k: base + k * SCAN_THREADS + t; ~0u past n), loaded
+// all at once: a full tile unguarded, the last one guarded.
+__device__ __forceinline__ void pt_load_keys(const uint32_t* __restrict__ keys, size_t n, size_t base,
+                                             uint32_t (&key)[PT_ITEMS]) {
+  const uint32_t* kp = keys + base + threadIdx.x;
+  if (base + PT_TILE <= n) {
+#pragma unroll
+    for (int k = 0; k < PT_ITEMS; ++k) key[k] = kp[k * SCAN_THREADS];
+  } else {
+#pragma unroll
+    for (int k = 0; k < PT_ITEMS; ++k)
+      key[k] = base + (size_t)k * SCAN_THREADS + threadIdx.x < n ? kp[k * SCAN_THREADS] : 0xffffffffu;
+  }
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_part_count(const uint32_t* __restrict__ keys, size_t n,
+                                                            const CutState* __restrict__ cs,
+                                                            uint32_t* __restrict__ tcount,
+                                                            uint32_t* __restrict__ tally_out) {
+  __shared__ uint32_t s_tmp[SCAN_THREADS / 64];
+  const uint32_t cut = cs->key;
+  const size_t base = (size_t)blockIdx.x * PT_TILE;
+  uint32_t key[PT_ITEMS];
+  pt_load_keys(keys, n, base, key);
+  uint32_t f = 0, vis = 0;
+#pragma unroll
+  for (int k = 0; k < PT_ITEMS; ++k) {
+    f += key[k] < cut ? 1u : 0u;
+    vis += (key[k] >= cut && key[k] != 0xffffffffu) ? 1u : 0u;
+  }
+  f = block_sum_u32(f, s_tmp);
+  vis = block_sum_u32(vis, s_tmp);
+  if (threadIdx.x == 0) {
+    tcount[blockIdx.x] = f;
+    if (vis) atomicAdd(tally_out, vis);
+  }
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_part_scatter(
+    const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, size_t n, const CutState* __restrict__ cs,
+    const uint32_t* __restrict__ tcount, uint32_t* __restrict__ kf, uint32_t* __restrict__ vf,
+    uint32_t* __restrict__ kb, uint32_t* __restrict__ vb, uint32_t* __restrict__ n_front,
+    uint32_t* __restrict__ n_back, uint32_t* __restrict__ cut_on) {
+  __shared__ uint32_t s_tmp[SCAN_THREADS / 64];
+  __shared__ uint32_t s_wt[PT_ITEMS][SCAN_THREADS / 64];
+  const uint32_t cut = cs->key;
+  const uint32_t b = blockIdx.x;
+  const size_t base = (size_t)b * PT_TILE;
+  const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+  // the front entries of the tiles before this one
+  uint32_t pre = 0;
+  for (uint32_t i = threadIdx.x; i < b; i += SCAN_THREADS) pre += tcount[i];
+  pre = block_sum_u32(pre, s_tmp);
+  // row k: entries base + k * SCAN_THREADS + t; rank of a front entry within
+  // its wave by ballot, then the waves and rows before it
+  uint32_t key[PT_ITEMS], rank[PT_ITEMS];
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  pt_load_keys(keys, n, base, key);
+#pragma unroll
+  for (int k = 0; k < PT_ITEMS; ++k) {
+    const uint64_t bal = __builtin_amdgcn_ballot_w64(key[k] < cut);
+    rank[k] = (uint32_t)__popcll(bal & lt);
+    if (lane == 0) s_wt[k][w] = (uint32_t)__popcll(bal);
+  }
+  __syncthreads();
+  uint32_t run = 0;
+#pragma unroll
+  for (int k = 0; k < PT_ITEMS; ++k) {
+    uint32_t before = 0, row = 0;
+#pragma unroll
+    for (int u = 0; u < SCAN_THREADS / 64; ++u) {
+      const uint32_t c = s_wt[k][u];
+      if (u < w) before += c;
+      row += c;
+    }
+    rank[k] += run + before;
+    run += row;
+  }
+  // the values of the entries written, loaded together before the writes
+  uint32_t val[PT_ITEMS];
+#pragma unroll
+  for (int k = 0; k < PT_ITEMS; ++k) {
+    const size_t e = base + (size_t)k * SCAN_THREADS + threadIdx.x;
+    val[k] = (e < n && (key[k] < cut || kb)) ? vals[e] : 0u;
+  }
+#pragma unroll
+  for (int k = 0; k < PT_ITEMS; ++k) {
+    const size_t e = base + (size_t)k * SCAN_THREADS + threadIdx.x;
+    if (e >= n) continue;
+    const uint32_t o = pre + rank[k];  // front entries before e
+    if (key[k] < cut) {
+      kf[o] = key[k];
+      vf[o] = val[k];
+    } else if (kb) {
+      kb[e - o] = key[k];
+      vb[e - o] = val[k];
+    }
+  }
+  if (threadIdx.x == 0 && b + 1 == gridDim.x) {
+    *n_front = pre + run;
+    *n_back = (uint32_t)(n - (pre + run));
+    *cut_on = cut != 0xffffffffu ? 1u : 0u;
+  }
+}
+
 void partition_at_cut(const uint32_t* keys, const uint32_t* vals, size_t n, const CutState* cs, uint32_t* kf,
                       uint32_t* vf, uint32_t* kb, uint32_t* vb, uint32_t* n_front, uint32_t* n_back, uint32_t* cut_on,
                       uint32_t* n_back_visible, uint64_t* state, uint32_t* status, hipStream_t st) {
-  // 4096-entry tiles: every tile is live here, and fewer of them shorten
-  // the look-back
-  constexpr int TILE = 2 * LB_TILE;
-  const int nb = std::max((int)((n + TILE - 1) / TILE), 1);
-  PartitionAtCut f{keys, vals, cs, kf, vf, kb, vb, n, n_back, cut_on, n_back_visible};
-  hipLaunchKernelGGL((k_scan_lookback<PartitionAtCut, TILE>), dim3(nb), dim3(SCAN_THREADS), 0, st, f, n, nullptr, state,
-                     n_front, status);
+  (void)status;
+  const int nb = std::max((int)((n + PT_TILE - 1) / PT_TILE), 1);
+  // the tile counts in the look-back state's words (lookback_words(n) u64
+  // words hold more than nb counts)
+  uint32_t* tcount = reinterpret_cast<uint32_t*>(state);
+  hipLaunchKernelGGL(k_part_count, dim3(nb), dim3(SCAN_THREADS), 0, st, keys, n, cs, tcount, n_back_visible);
+  hipLaunchKernelGGL(k_part_scatter, dim3(nb), dim3(SCAN_THREADS), 0, st, keys, vals, n, cs, tcount, kf, vf, kb, vb,
+                     n_front, n_back, cut_on);
 }
 
 void scan_pair_offsets(const uint32_t* vals, const uint32_t* cnt, size_t n_max, const uint32_t* n_dev,
