@@ -560,14 +560,21 @@ __device__ __forceinline__ void train_geometry(float* __restrict__ P, float* __r
     const int nv = min(TU_VCH, n_views - v0);
     // the chunk's counts and accumulator values, all in flight together
     // while the colour waves work, then staged in LDS
+    // (loaded into registers first and stored after: a store right after
+    // its load waits for it, which would serialise the loads)
+    float ld[TU_VCH][7];
 #pragma unroll
     for (int j = 0; j < TU_VCH; ++j) {
       const bool in = live && j < nv;
       const size_t vi = (size_t)(v0 + j) * cap + i;
-      s_ga[j][0][lane] = __builtin_bit_cast(float, in ? cnt[vi] : 0u);
+      ld[j][0] = __builtin_bit_cast(float, in ? cnt[vi] : 0u);
 #pragma unroll
-      for (int q = 0; q < 6; ++q) s_ga[j][1 + q][lane] = in ? accum[(size_t)(q + 3) * acc_stride + vi] : 0.0f;
+      for (int q = 0; q < 6; ++q) ld[j][1 + q] = in ? accum[(size_t)(q + 3) * acc_stride + vi] : 0.0f;
     }
+#pragma unroll
+    for (int j = 0; j < TU_VCH; ++j)
+#pragma unroll
+      for (int q = 0; q < 7; ++q) s_ga[j][q][lane] = ld[j][q];
     __syncthreads();  // direction gradients of the chunk are in LDS
 #pragma unroll 1
     for (int j = 0; j < nv; ++j) {
